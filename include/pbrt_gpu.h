@@ -1,0 +1,149 @@
+/*
+ * pbrt_gpu.h — C ABI of the MI355X ray hot path (gfx950, HIP).
+ *
+ * The drop-in boundary for nmichel/pbrt's ray path. Every entry point returns an int status
+ * (PBRT_OK = 0, negative = error; pbrt_last_error() gives a thread-local message); no C++ exception
+ * crosses the boundary. Input arrays are copied at call time, so the caller keeps ownership.
+ * Parameters ending in `_dev` are device pointers (HBM); `stream` is a hipStream_t (NULL = the
+ * context's own stream). Committed meshes and scenes are immutable and may be shared by threads.
+ *
+ * Reference interfaces replaced are cited as file:line into the reference source tree.
+ */
+#ifndef PBRT_GPU_H
+#define PBRT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference panics instead: triangle_mesh.rs:28, config.rs:155-166) ---- */
+#define PBRT_OK 0
+#define PBRT_ERR_INVALID_ARG (-1)
+#define PBRT_ERR_EMPTY_MESH (-2)
+#define PBRT_ERR_DEVICE (-3)
+#define PBRT_ERR_OUT_OF_MEMORY (-4)
+#define PBRT_ERR_LAYOUT (-5)
+#define PBRT_ERR_INTERNAL (-6)
+
+typedef struct pbrt_ctx pbrt_ctx;
+typedef struct pbrt_mesh pbrt_mesh;
+
+/* A ray as stored in HBM: 32 bytes, two 16-byte halves. The direction is used as given (the
+ * reference normalises in Ray::new, src/geom/ray.rs:10-18; callers pass normalised directions).
+ * tmin/tmax are the reference's `near`/`far` arguments (triangle_mesh/bvh.rs:350). A tmax above
+ * FLT_MAX (i.e. +inf) is treated as FLT_MAX. */
+typedef struct pbrt_ray {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+} pbrt_ray;
+
+/* Closest hit: 16 bytes. prim = index of the triangle in the caller's index array (the reference's
+ * `current_tri_idx`, triangle_mesh/bvh.rs:489), -1 on a miss; t, u, v as in
+ * TriangleIntersection (triangle_mesh.rs:176-181). */
+typedef struct pbrt_hit {
+  int32_t prim;
+  float t, u, v;
+} pbrt_hit;
+
+/* Mesh BVH build parameters. The reference is bins = 8, t_trav = 0, max_leaf = 0 (no bound)
+ * (triangle_mesh/bvh.rs:129, 725-735). A node is split when t_trav*A + (A_L*N_L + A_R*N_R) < A*N,
+ * and always when N > max_leaf (max_leaf > 0). Device leaves hold at most 32 triangles. */
+typedef struct pbrt_bvh_params {
+  uint32_t bins;
+  uint32_t max_leaf;
+  double t_trav;
+} pbrt_bvh_params;
+
+/* triangle_mesh/bvh.rs:65-80 (BuildStats). */
+typedef struct pbrt_build_stats {
+  uint64_t node_count, leaf_count, max_depth, max_leaf_tri_count, total_leaf_tri_count;
+} pbrt_build_stats;
+
+/* Launch shape of the traversal kernels. Zero fields take the tuned defaults. */
+typedef struct pbrt_launch {
+  uint32_t block;        /* threads per workgroup (64, 128, 256) */
+  uint32_t waves_per_cu; /* persistent grid: resident waves per CU */
+  uint32_t lds_stack;    /* LDS short-stack entries per lane (8, 16, 24, 32); deeper entries spill to HBM */
+  uint32_t reserved;
+} pbrt_launch;
+
+/* ---- context, errors, device plumbing ---- */
+const char* pbrt_last_error(void);
+int pbrt_version(void);
+int pbrt_ctx_create(int device, pbrt_ctx** out);
+int pbrt_ctx_destroy(pbrt_ctx* ctx);
+int pbrt_ctx_stream(pbrt_ctx* ctx, void** stream_out);
+int pbrt_ctx_device_info(pbrt_ctx* ctx, int* cu_count, char* name, size_t name_len);
+int pbrt_dev_alloc(pbrt_ctx* ctx, size_t bytes, void** dptr_out);
+int pbrt_dev_free(pbrt_ctx* ctx, void* dptr);
+int pbrt_memcpy_h2d(pbrt_ctx* ctx, void* dst_dev, const void* src, size_t bytes);
+int pbrt_memcpy_d2h(pbrt_ctx* ctx, void* dst, const void* src_dev, size_t bytes);
+int pbrt_memset_dev(pbrt_ctx* ctx, void* dst_dev, int value, size_t bytes);
+int pbrt_stream_sync(pbrt_ctx* ctx, void* stream);
+int pbrt_event_create(pbrt_ctx* ctx, void** ev_out);
+int pbrt_event_record(void* ev, void* stream);
+int pbrt_event_elapsed_ms(void* ev_start, void* ev_end, float* ms_out);
+int pbrt_event_destroy(void* ev);
+
+/* ---- mesh: TriangleMesh::new (triangle_mesh.rs:27-41) = BVHTree::new + build (bvh.rs:179-230) ----
+ * xyz: nv*3 FP32 vertex coordinates; idx: nt*3 vertex indices. An empty mesh is
+ * PBRT_ERR_EMPTY_MESH (the reference asserts, triangle_mesh.rs:28). The BVH is built on the host
+ * and flattened into 64-byte device nodes (two child boxes each) and 48-byte triangles. With
+ * ctx == NULL the mesh is built and flattened on the host only (for inspection; tracing it fails). */
+int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                     const pbrt_bvh_params* params, pbrt_mesh** out);
+int pbrt_mesh_destroy(pbrt_mesh* mesh);
+/* BVHTree::build_stats (triangle_mesh/bvh.rs:539-546); triangle_count (bvh.rs:529-531). */
+int pbrt_mesh_build_stats(const pbrt_mesh* mesh, pbrt_build_stats* out);
+int pbrt_mesh_triangle_count(const pbrt_mesh* mesh, uint32_t* out);
+/* Bytes of the device BVH: nodes and triangles. */
+int pbrt_mesh_device_bytes(const pbrt_mesh* mesh, uint64_t* node_bytes, uint64_t* tri_bytes);
+/* Host copy of the built tree, in preorder: per node its box (min xyz, max xyz, f64), whether it is
+ * a leaf, child node indices (interior), and for leaves [first, first+count) into leaf_tris, which
+ * lists original triangle indices. Arrays sized by build_stats.node_count and the triangle count. */
+int pbrt_mesh_export_tree(const pbrt_mesh* mesh, double* boxes, uint8_t* is_leaf, uint32_t* left, uint32_t* right,
+                          uint32_t* first, uint32_t* count, uint32_t* leaf_tris);
+
+/* Batched closest hit: BVHTree::query (triangle_mesh/bvh.rs:350-352, ordered traversal 450-526),
+ * with the order-independent tie rule (smallest t, then smallest triangle index). */
+int pbrt_mesh_trace_closest(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, pbrt_hit* hits_dev, void* stream);
+/* Same, counting the work per ray like BVHTree::query_instrumented (bvh.rs:359-361): stats_dev
+ * receives n x {nodes_visited, box_tests, triangle_tests} (uint32). */
+int pbrt_mesh_trace_closest_stats(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, pbrt_hit* hits_dev,
+                                  uint32_t* stats_dev, void* stream);
+/* Batched any hit: BVHTree::intersect_p (triangle_mesh/bvh.rs:381-420); occluded_dev[i] = 0 or 1. */
+int pbrt_mesh_trace_any(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, uint8_t* occluded_dev, void* stream);
+/* Launch shape used by the mesh's traversal kernels (all fields 0 = defaults). */
+int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
+
+/* Sticky error word of the mesh's kernels (waits for the device): 0 = fine, bit 0 = a traversal
+ * stack exceeded its bound (cannot happen for a tree of the reported depth). */
+int pbrt_mesh_status(pbrt_mesh* mesh, int* status_out);
+
+/* ---- synthetic inputs and ray generation (host, f64 as the reference, rays rounded to FP32) ----
+ * pinhole camera of src/cameras/pin_hole.rs:17-50 looking from `eye` at `look` (up = +y, fov in
+ * radians); writes width*height rays through pixel centres (px + 0.5, py + 0.5), row-major. */
+int pbrt_pinhole_rays(uint32_t width, uint32_t height, double fov, const double eye[3], const double look[3],
+                      float tmin, float tmax, pbrt_ray* out);
+/* Camera `view` (0..5) of the bvh_stats orbit framing a vertex set (src/bin/bvh_stats.rs:345-401). */
+int pbrt_orbit_rays(const float* xyz, uint32_t nv, uint32_t view, uint32_t side, double fov, float tmin, float tmax,
+                    pbrt_ray* out);
+/* n seeded rays, origins uniform in a ball, directions uniform on the sphere (src/pdfs/sphere.rs:14-23);
+ * ray k uses counter first + k. */
+int pbrt_random_rays(const double center[3], double radius, uint64_t seed, uint64_t first, uint32_t n, float tmin,
+                     float tmax, pbrt_ray* out);
+/* Seeded icosphere of `subdiv` subdivisions (10*4^s+2 vertices, 20*4^s triangles) wound so the reference's
+ * mesh normal (triangle_mesh.rs:119) points outward; radial displacement: noise_kind 0 none,
+ * 1 per-vertex uniform +-amp*radius, 2 fBm value noise (octaves, base frequency freq) * amp*radius.
+ * With null outputs only the sizes are written. */
+int pbrt_gen_icosphere(uint32_t subdiv, double radius, const double center[3], int noise_kind, double amp, int octaves,
+                       double freq, uint64_t seed, float* xyz_out, uint32_t* idx_out, uint32_t* nv_out,
+                       uint32_t* nt_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBRT_GPU_H */

@@ -1,0 +1,230 @@
+// ORACLE — test infrastructure, not product code.
+//
+// C entry points over the CPU restatements in mesh_oracle.hpp / camera_oracle.hpp, loaded by the
+// tests and by the CPU-baseline leg of bench.py through ctypes. Precision 64 is "Oracle A" (the
+// reference's f64 arithmetic), precision 32 is "Oracle B" (the same statements in FP32).
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <thread>
+#include <vector>
+
+#include "camera_oracle.hpp"
+#include "mesh_oracle.hpp"
+
+using namespace orc;
+
+namespace {
+
+struct MeshHandle {
+  int precision;
+  MeshBVH<double>* a = nullptr;
+  MeshBVH<float>* b = nullptr;
+  ~MeshHandle() {
+    delete a;
+    delete b;
+  }
+};
+
+template <class R>
+const typename MeshBVH<R>::Node* node_ptr(const MeshBVH<R>* m) {
+  return m->nodes.data();
+}
+
+// Runs fn(i) for i in [0, n) over `threads` threads in static contiguous chunks.
+template <class F>
+void parallel_for(uint64_t n, int threads, F fn) {
+  if (threads <= 1 || n < 1024) {
+    for (uint64_t i = 0; i < n; ++i) fn(i, 0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  uint64_t chunk = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    uint64_t lo = t * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    pool.emplace_back([=]() {
+      for (uint64_t i = lo; i < hi; ++i) fn(i, t);
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// mode: 0 reference-ordered closest hit, 1 canonical closest hit, 2 brute force, 3 any-hit.
+template <class R, class RayT>
+void trace_impl(const MeshBVH<R>* m, const RayT* rays, uint64_t n, int mode, int64_t* prim, double* t, double* u,
+                double* v, uint32_t* stats_per_ray, uint64_t* stats_sum, int threads) {
+  std::vector<Stats> per_thread(std::max(threads, 1));
+  parallel_for(n, threads, [&](uint64_t i, int tid) {
+    const RayT* r = rays + 8 * i;
+    Ray<R> ray{{(R)r[0], (R)r[1], (R)r[2]}, {(R)r[4], (R)r[5], (R)r[6]}};
+    R near = (R)r[3], far = (R)r[7];
+    Stats st;
+    TriHit<R> h{};
+    uint64_t id = 0;
+    bool found;
+    if (mode == 3) {
+      found = m->intersect_p(ray, near, far);
+    } else if (mode == 2) {
+      found = m->brute_force(ray, near, far, &h, &id);
+    } else {
+      found = m->traverse(ray, near, far, mode == 1, &st, &h, &id);
+    }
+    if (prim) prim[i] = found ? (mode == 3 ? 1 : (int64_t)id) : -1;
+    if (found && mode != 3) {
+      if (t) t[i] = (double)h.t;
+      if (u) u[i] = (double)h.u;
+      if (v) v[i] = (double)h.v;
+    } else {
+      if (t) t[i] = 0.0;
+      if (u) u[i] = 0.0;
+      if (v) v[i] = 0.0;
+    }
+    if (stats_per_ray) {
+      stats_per_ray[3 * i] = (uint32_t)st.nodes_visited;
+      stats_per_ray[3 * i + 1] = (uint32_t)st.box_tests;
+      stats_per_ray[3 * i + 2] = (uint32_t)st.triangle_tests;
+    }
+    Stats& acc = per_thread[tid];
+    acc.nodes_visited += st.nodes_visited;
+    acc.box_tests += st.box_tests;
+    acc.triangle_tests += st.triangle_tests;
+  });
+  if (stats_sum) {
+    stats_sum[0] = stats_sum[1] = stats_sum[2] = 0;
+    for (auto& s : per_thread) {
+      stats_sum[0] += s.nodes_visited;
+      stats_sum[1] += s.box_tests;
+      stats_sum[2] += s.triangle_tests;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_mesh_create(const float* xyz, uint64_t nv, const uint32_t* idx, uint64_t nt, int precision, int bins,
+                      double t_trav, int max_leaf) {
+  if (nt == 0 || bins < 2) return nullptr;  // triangle_mesh.rs:28 refuses an empty mesh
+  BuildParams p;
+  p.bins = bins;
+  p.t_trav = t_trav;
+  p.max_leaf = max_leaf;
+  auto* h = new MeshHandle();
+  h->precision = precision;
+  if (precision == 64)
+    h->a = new MeshBVH<double>(xyz, nv, idx, nt, p);
+  else
+    h->b = new MeshBVH<float>(xyz, nv, idx, nt, p);
+  return h;
+}
+
+void orc_mesh_destroy(void* h) { delete (MeshHandle*)h; }
+
+void orc_mesh_build_stats(void* hv, uint64_t* out) {
+  auto* h = (MeshHandle*)hv;
+  BuildStats s = h->a ? h->a->build_stats() : h->b->build_stats();
+  out[0] = s.node_count;
+  out[1] = s.leaf_count;
+  out[2] = s.max_depth;
+  out[3] = s.max_leaf_tri_count;
+  out[4] = s.total_leaf_tri_count;
+}
+
+uint64_t orc_mesh_node_count(void* hv) {
+  auto* h = (MeshHandle*)hv;
+  return h->a ? h->a->nodes.size() : h->b->nodes.size();
+}
+
+// Tree in the reference's own layout: node boxes (f64, min xyz then max xyz), left_first,
+// tri_count (0 = interior), and the triangle permutation tri_idx.
+void orc_mesh_export(void* hv, double* boxes, uint64_t* left_first, uint64_t* tri_count, uint64_t* tri_idx) {
+  auto* h = (MeshHandle*)hv;
+  auto dump = [&](auto* m) {
+    for (size_t i = 0; i < m->nodes.size(); ++i) {
+      const auto& n = m->nodes[i];
+      double* b = boxes + 6 * i;
+      b[0] = n.bbox.bmin.x; b[1] = n.bbox.bmin.y; b[2] = n.bbox.bmin.z;
+      b[3] = n.bbox.bmax.x; b[4] = n.bbox.bmax.y; b[5] = n.bbox.bmax.z;
+      left_first[i] = n.left_first;
+      tri_count[i] = n.tri_count;
+    }
+    for (size_t i = 0; i < m->tri_idx.size(); ++i) tri_idx[i] = m->tri_idx[i];
+  };
+  if (h->a) dump(h->a); else dump(h->b);
+}
+
+// Rays as 8 floats (ox, oy, oz, tmin, dx, dy, dz, tmax); Oracle A widens them exactly to f64.
+int orc_mesh_trace_f32(void* hv, const float* rays, uint64_t n, int mode, int64_t* prim, double* t, double* u,
+                       double* v, uint32_t* stats_per_ray, uint64_t* stats_sum, int threads) {
+  auto* h = (MeshHandle*)hv;
+  if (h->a)
+    trace_impl(h->a, rays, n, mode, prim, t, u, v, stats_per_ray, stats_sum, threads);
+  else
+    trace_impl(h->b, rays, n, mode, prim, t, u, v, stats_per_ray, stats_sum, threads);
+  return 0;
+}
+
+// Rays as 8 doubles; Oracle B rounds them to FP32.
+int orc_mesh_trace_f64(void* hv, const double* rays, uint64_t n, int mode, int64_t* prim, double* t, double* u,
+                       double* v, uint32_t* stats_per_ray, uint64_t* stats_sum, int threads) {
+  auto* h = (MeshHandle*)hv;
+  if (h->a)
+    trace_impl(h->a, rays, n, mode, prim, t, u, v, stats_per_ray, stats_sum, threads);
+  else
+    trace_impl(h->b, rays, n, mode, prim, t, u, v, stats_per_ray, stats_sum, threads);
+  return 0;
+}
+
+// Split cost of the best binned plane for the node spanning tri_idx[left_first, +count) of an
+// unbuilt tree, and the exhaustive recount of the same plane (triangle_mesh/bvh.rs:867-894).
+// Returns 0 when no plane separates the range.
+int orc_split_costs(const float* xyz, uint64_t nv, const uint32_t* idx, uint64_t nt, int bins, uint64_t left_first,
+                    uint64_t count, double* binned, double* exhaustive) {
+  BuildParams p;
+  p.bins = bins;
+  p.max_leaf = 0;
+  // Build with a huge t_trav so the root stays a leaf: tri_idx keeps its identity order.
+  p.t_trav = 1e300;
+  MeshBVH<double> m(xyz, nv, idx, nt, p);
+  MeshBVH<double>::Node node{AABB<double>::empty(), left_first, count};
+  m.update_bounds(node);
+  auto s = m.find_best_split_plane(node);
+  if (!s.valid) return 0;
+  *binned = s.cost;
+  *exhaustive = m.exhaustive_split_cost(node, s);
+  return 1;
+}
+
+// The bvh_stats ray protocol (bvh_stats.rs:345-401) as 8 doubles per ray.
+void orc_orbit_rays(const float* xyz, uint64_t nv, unsigned side, double fov, int views, double* out) {
+  std::vector<double> v(nv * 3);
+  for (uint64_t i = 0; i < nv * 3; ++i) v[i] = (double)xyz[i];
+  std::vector<Ray<double>> rays((uint64_t)views * side * side);
+  orbit_rays(v.data(), nv, side, fov, views, rays.data());
+  for (size_t i = 0; i < rays.size(); ++i) {
+    double* o = out + 8 * i;
+    o[0] = rays[i].o.x; o[1] = rays[i].o.y; o[2] = rays[i].o.z; o[3] = 1e-4;
+    o[4] = rays[i].d.x; o[5] = rays[i].d.y; o[6] = rays[i].d.z; o[7] = 1000.0;
+  }
+}
+
+// One slab test (aabound.rs:323-355) at either precision, for the reference's box cases.
+int orc_box_hit(int precision, const double* box6, const double* ray6, double tmin, double tmax, double* entry) {
+  if (precision == 64) {
+    AABB<double> b{{box6[0], box6[1], box6[2]}, {box6[3], box6[4], box6[5]}};
+    Ray<double> r{{ray6[0], ray6[1], ray6[2]}, {ray6[3], ray6[4], ray6[5]}};
+    return b.hit(r, tmin, tmax, entry) ? 1 : 0;
+  }
+  AABB<float> b{{(float)box6[0], (float)box6[1], (float)box6[2]}, {(float)box6[3], (float)box6[4], (float)box6[5]}};
+  Ray<float> r{{(float)ray6[0], (float)ray6[1], (float)ray6[2]}, {(float)ray6[3], (float)ray6[4], (float)ray6[5]}};
+  float e = 0;
+  int ok = b.hit(r, (float)tmin, (float)tmax, &e) ? 1 : 0;
+  *entry = e;
+  return ok;
+}
+
+double orc_slab_widening(int precision) { return precision == 64 ? slab_widening<double>() : (double)slab_widening<float>(); }
+
+}  // extern "C"

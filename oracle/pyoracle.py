@@ -1,0 +1,133 @@
+"""ORACLE — test infrastructure, not product code.
+
+ctypes binding of liboracle.so (the CPU restatement of the reference's ray hot path). Only the
+tests, the smoke check and the CPU-baseline leg of bench.py import this module, as a checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+P = ctypes.c_void_p
+U64 = ctypes.c_uint64
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        L.orc_mesh_create.restype = P
+        L.orc_mesh_create.argtypes = [P, U64, P, U64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.orc_mesh_destroy.argtypes = [P]
+        L.orc_mesh_build_stats.argtypes = [P, P]
+        L.orc_mesh_node_count.restype = U64
+        L.orc_mesh_node_count.argtypes = [P]
+        L.orc_mesh_export.argtypes = [P, P, P, P, P]
+        for fn in (L.orc_mesh_trace_f32, L.orc_mesh_trace_f64):
+            fn.restype = ctypes.c_int
+            fn.argtypes = [P, P, U64, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
+        L.orc_split_costs.restype = ctypes.c_int
+        L.orc_split_costs.argtypes = [P, U64, P, U64, ctypes.c_int, U64, U64, P, P]
+        L.orc_orbit_rays.argtypes = [P, U64, ctypes.c_uint, ctypes.c_double, ctypes.c_int, P]
+        L.orc_box_hit.restype = ctypes.c_int
+        L.orc_box_hit.argtypes = [ctypes.c_int, P, P, ctypes.c_double, ctypes.c_double, P]
+        L.orc_slab_widening.restype = ctypes.c_double
+        L.orc_slab_widening.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(P) if a is not None else None
+
+
+MODE_REFERENCE, MODE_CANONICAL, MODE_BRUTE, MODE_ANY = 0, 1, 2, 3
+
+
+class Mesh:
+    """A mesh BVH built by the oracle. precision=64: Oracle A; precision=32: Oracle B."""
+
+    def __init__(self, vertices, faces, precision=64, bins=8, t_trav=0.0, max_leaf=0):
+        self.v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        self.h = lib().orc_mesh_create(_p(self.v), len(self.v), _p(self.f), len(self.f), precision, bins, t_trav,
+                                       max_leaf)
+        if not self.h:
+            raise ValueError("a mesh needs at least one triangle")
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_destroy(self.h)
+            self.h = None
+
+    def build_stats(self):
+        out = np.zeros(5, np.uint64)
+        lib().orc_mesh_build_stats(self.h, _p(out))
+        keys = ("node_count", "leaf_count", "max_depth", "max_leaf_tri_count", "total_leaf_tri_count")
+        return {k: int(x) for k, x in zip(keys, out)}
+
+    def export(self):
+        n = int(lib().orc_mesh_node_count(self.h))
+        boxes = np.zeros((n, 6), np.float64)
+        lf = np.zeros(n, np.uint64)
+        tc = np.zeros(n, np.uint64)
+        ti = np.zeros(len(self.f), np.uint64)
+        lib().orc_mesh_export(self.h, _p(boxes), _p(lf), _p(tc), _p(ti))
+        return boxes, lf, tc, ti
+
+    def trace(self, rays, mode=MODE_CANONICAL, threads=1, per_ray_stats=False):
+        """rays: (n, 8) float32 or float64 rows (ox, oy, oz, tmin, dx, dy, dz, tmax).
+
+        Returns dict(prim (int64, -1 = miss), t, u, v (float64), stats_sum (3,), stats (n,3)?)."""
+        rays = np.ascontiguousarray(rays)
+        n = len(rays)
+        prim = np.empty(n, np.int64)
+        t = np.empty(n, np.float64)
+        u = np.empty(n, np.float64)
+        v = np.empty(n, np.float64)
+        ssum = np.zeros(3, np.uint64)
+        spr = np.zeros((n, 3), np.uint32) if per_ray_stats else None
+        fn = lib().orc_mesh_trace_f32 if rays.dtype == np.float32 else lib().orc_mesh_trace_f64
+        assert rays.dtype in (np.float32, np.float64) and rays.shape[1] == 8
+        fn(self.h, _p(rays), n, mode, _p(prim), _p(t), _p(u), _p(v), _p(spr), _p(ssum), threads)
+        out = dict(prim=prim, t=t, u=u, v=v, stats_sum=ssum)
+        if per_ray_stats:
+            out["stats"] = spr
+        return out
+
+
+def orbit_rays(vertices, side=200, fov=40.0 * np.pi / 180.0, views=6):
+    """The bvh_stats protocol's rays (bvh_stats.rs:345-401) as (n, 8) float64 rows."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    out = np.empty((views * side * side, 8), np.float64)
+    lib().orc_orbit_rays(_p(v), len(v), side, fov, views, _p(out))
+    return out
+
+
+def split_costs(vertices, faces, bins, left_first, count):
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(faces, np.uint32)
+    a = ctypes.c_double()
+    b = ctypes.c_double()
+    ok = lib().orc_split_costs(_p(v), len(v), _p(f), len(f), bins, left_first, count, ctypes.byref(a),
+                               ctypes.byref(b))
+    return (a.value, b.value) if ok else None
+
+
+def box_hit(precision, box, ray, tmin, tmax):
+    b = np.ascontiguousarray(box, np.float64)
+    r = np.ascontiguousarray(ray, np.float64)
+    e = ctypes.c_double()
+    ok = lib().orc_box_hit(precision, _p(b), _p(r), tmin, tmax, ctypes.byref(e))
+    return e.value if ok else None
+
+
+def slab_widening(precision):
+    return lib().orc_slab_widening(precision)

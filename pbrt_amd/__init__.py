@@ -1,0 +1,259 @@
+"""MI355X-native ray hot path of nmichel/pbrt: host-side mirror of the reference's mesh interface.
+
+Names follow the reference (src/shapes/triangle_mesh/{triangle_mesh.rs, bvh.rs}): a `TriangleMesh`
+is built from vertices and indices, reports `build_stats()` and `triangle_count()`, and answers
+`query` (closest hit), `query_instrumented` (closest hit with work counters) and `intersect_p`
+(any hit) — here over batches of rays resident in device memory, through the C ABI of
+include/pbrt_gpu.h. There is no CPU fallback: without the native library every call raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import BuildStats, BvhParams, Launch, PbrtError, call, lib
+
+__all__ = ["Context", "DeviceArray", "TriangleMesh", "PbrtError", "icosphere", "pinhole_rays", "orbit_rays",
+           "random_rays", "RAY_DTYPE", "HIT_DTYPE", "REFERENCE_BVH", "DEVICE_BVH"]
+
+# pbrt_ray / pbrt_hit as numpy record layouts (32 and 16 bytes)
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"), ("dx", "<f4"), ("dy", "<f4"),
+                      ("dz", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+
+# The reference's mesh BVH parameters (triangle_mesh/bvh.rs:129, 733-735) and the device default.
+REFERENCE_BVH = dict(bins=8, max_leaf=0, t_trav=0.0)
+DEVICE_BVH = dict(bins=12, max_leaf=4, t_trav=1.0)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device, one stream (pbrt_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        call("pbrt_ctx_create", device, ctypes.byref(h))
+        self.h = h
+        s = ctypes.c_void_p()
+        call("pbrt_ctx_stream", self.h, ctypes.byref(s))
+        self.stream = s
+        cu = ctypes.c_int()
+        name = ctypes.create_string_buffer(64)
+        call("pbrt_ctx_device_info", self.h, ctypes.byref(cu), name, 64)
+        self.cu_count = cu.value
+        self.arch = name.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().pbrt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        call("pbrt_stream_sync", self.h, self.stream)
+
+    def alloc(self, nbytes):
+        return DeviceArray(self, nbytes)
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        d = DeviceArray(self, a.nbytes)
+        d.write(a)
+        return d
+
+    def event(self):
+        return Event(self)
+
+
+class Event:
+    def __init__(self, ctx):
+        h = ctypes.c_void_p()
+        call("pbrt_event_create", ctx.h, ctypes.byref(h))
+        self.h = h
+
+    def record(self, stream):
+        call("pbrt_event_record", self.h, stream)
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        call("pbrt_event_elapsed_ms", self.h, end.h, ctypes.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().pbrt_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation (HBM) owned by a context."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        call("pbrt_dev_alloc", ctx.h, self.nbytes, ctypes.byref(p))
+        self.ptr = p
+
+    def write(self, array):
+        a = np.ascontiguousarray(array)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device allocation")
+        call("pbrt_memcpy_h2d", self.ctx.h, self.ptr, _ptr(a), a.nbytes)
+
+    def read(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        call("pbrt_memcpy_d2h", self.ctx.h, _ptr(out), self.ptr, out.nbytes)
+        return out
+
+    def fill(self, byte):
+        call("pbrt_memset_dev", self.ctx.h, self.ptr, byte, self.nbytes)
+
+    def free(self):
+        if self.ptr:
+            lib().pbrt_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class TriangleMesh:
+    """TriangleMesh::new (src/shapes/triangle_mesh/triangle_mesh.rs:27-41) with its BVH on the device.
+
+    vertices: (nv, 3) float32; indices: (nt, 3) uint32. An empty mesh raises PbrtError
+    (PBRT_ERR_EMPTY_MESH), as the reference's constructor panics (triangle_mesh.rs:28)."""
+
+    def __init__(self, ctx, vertices, indices, bins=12, max_leaf=4, t_trav=1.0):
+        self.ctx = ctx
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        f = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        params = BvhParams(bins, max_leaf, t_trav)
+        h = ctypes.c_void_p()
+        call("pbrt_mesh_create", ctx.h if ctx is not None else None, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(params), ctypes.byref(h))
+        self.h = h
+        self.params = dict(bins=bins, max_leaf=max_leaf, t_trav=t_trav)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pbrt_mesh_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def build_stats(self):
+        s = BuildStats()
+        call("pbrt_mesh_build_stats", self.h, ctypes.byref(s))
+        return {k: int(getattr(s, k)) for k, _ in BuildStats._fields_}
+
+    def triangle_count(self):
+        n = ctypes.c_uint32()
+        call("pbrt_mesh_triangle_count", self.h, ctypes.byref(n))
+        return n.value
+
+    def device_bytes(self):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        call("pbrt_mesh_device_bytes", self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def set_launch(self, block=0, waves_per_cu=0, lds_stack=0):
+        call("pbrt_mesh_set_launch", self.h, ctypes.byref(Launch(block, waves_per_cu, lds_stack, 0)))
+
+    def export_tree(self):
+        s = self.build_stats()
+        n, nt = s["node_count"], self.triangle_count()
+        boxes = np.zeros((n, 6), np.float64)
+        leaf = np.zeros(n, np.uint8)
+        left, right, first, count = (np.zeros(n, np.uint32) for _ in range(4))
+        tris = np.zeros(nt, np.uint32)
+        call("pbrt_mesh_export_tree", self.h, _ptr(boxes), _ptr(leaf), _ptr(left), _ptr(right), _ptr(first),
+             _ptr(count), _ptr(tris))
+        return dict(boxes=boxes, leaf=leaf, left=left, right=right, first=first, count=count, tris=tris)
+
+    # -- batched queries over device-resident rays ------------------------------------------------
+    def query(self, rays, n, hits, stream=None):
+        """BVHTree::query (triangle_mesh/bvh.rs:350-352): closest hit per ray into `hits` (pbrt_hit)."""
+        call("pbrt_mesh_trace_closest", self.h, rays.ptr, n, hits.ptr, stream)
+
+    def query_instrumented(self, rays, n, hits, stats, stream=None):
+        """BVHTree::query_instrumented (bvh.rs:359-361): also n x {nodes, boxes, triangles} uint32."""
+        call("pbrt_mesh_trace_closest_stats", self.h, rays.ptr, n, hits.ptr, stats.ptr, stream)
+
+    def intersect_p(self, rays, n, occluded, stream=None):
+        """BVHTree::intersect_p (bvh.rs:381-420): occluded[i] in {0, 1}."""
+        call("pbrt_mesh_trace_any", self.h, rays.ptr, n, occluded.ptr, stream)
+
+    def status(self):
+        s = ctypes.c_int()
+        call("pbrt_mesh_status", self.h, ctypes.byref(s))
+        return s.value
+
+    # -- host convenience: upload, trace, download ------------------------------------------------
+    def trace_host(self, rays, mode="closest", stats=False):
+        rays = np.ascontiguousarray(rays)
+        if rays.dtype != RAY_DTYPE:
+            rays = np.ascontiguousarray(rays, np.float32).view(RAY_DTYPE).reshape(-1)
+        n = len(rays)
+        dr = self.ctx.upload(rays)
+        if mode == "any":
+            occ = self.ctx.alloc(max(n, 1))
+            self.intersect_p(dr, n, occ)
+            return occ.read(np.uint8, n)
+        dh = self.ctx.alloc(max(n, 1) * 16)
+        if stats:
+            ds = self.ctx.alloc(max(n, 1) * 12)
+            self.query_instrumented(dr, n, dh, ds)
+            return dh.read(HIT_DTYPE, n), ds.read(np.uint32, 3 * n).reshape(n, 3)
+        self.query(dr, n, dh)
+        return dh.read(HIT_DTYPE, n)
+
+
+# ---- synthetic inputs ----------------------------------------------------------------------------
+def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0), noise="none", amp=0.0, octaves=0, freq=1.5, seed=0):
+    kinds = {"none": 0, "uniform": 1, "fbm": 2}
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    c = np.ascontiguousarray(center, np.float64)
+    call("pbrt_gen_icosphere", subdiv, radius, _ptr(c), kinds[noise], amp, octaves, freq, seed, None, None,
+         ctypes.byref(nv), ctypes.byref(nt))
+    v = np.empty((nv.value, 3), np.float32)
+    f = np.empty((nt.value, 3), np.uint32)
+    call("pbrt_gen_icosphere", subdiv, radius, _ptr(c), kinds[noise], amp, octaves, freq, seed, _ptr(v), _ptr(f),
+         ctypes.byref(nv), ctypes.byref(nt))
+    return v, f
+
+
+def pinhole_rays(width, height, fov, eye, look, tmin=1e-4, tmax=1000.0):
+    out = np.empty(width * height, RAY_DTYPE)
+    e = np.ascontiguousarray(eye, np.float64)
+    l = np.ascontiguousarray(look, np.float64)
+    call("pbrt_pinhole_rays", width, height, fov, _ptr(e), _ptr(l), tmin, tmax, _ptr(out))
+    return out
+
+
+def orbit_rays(vertices, view, side, fov=40.0 * np.pi / 180.0, tmin=1e-4, tmax=1000.0):
+    v = np.ascontiguousarray(vertices, np.float32)
+    out = np.empty(side * side, RAY_DTYPE)
+    call("pbrt_orbit_rays", _ptr(v), len(v), view, side, fov, tmin, tmax, _ptr(out))
+    return out
+
+
+def random_rays(center, radius, n, seed=1, first=0, tmin=1e-4, tmax=1000.0):
+    c = np.ascontiguousarray(center, np.float64)
+    out = np.empty(n, RAY_DTYPE)
+    call("pbrt_random_rays", _ptr(c), radius, seed, first, n, tmin, tmax, _ptr(out))
+    return out

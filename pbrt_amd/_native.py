@@ -1,0 +1,100 @@
+"""ctypes binding of libpbrt_gpu.so (the C ABI declared in include/pbrt_gpu.h).
+
+The library is loaded from this package directory only. If it is missing, every call raises: there
+is no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpbrt_gpu.so")
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int
+F32 = ctypes.c_float
+F64 = ctypes.c_double
+PP = ctypes.POINTER(ctypes.c_void_p)
+
+
+class BvhParams(ctypes.Structure):
+    _fields_ = [("bins", U32), ("max_leaf", U32), ("t_trav", F64)]
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [("node_count", U64), ("leaf_count", U64), ("max_depth", U64), ("max_leaf_tri_count", U64),
+                ("total_leaf_tri_count", U64)]
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [("block", U32), ("waves_per_cu", U32), ("lds_stack", U32), ("reserved", U32)]
+
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "pbrt_last_error": [],
+    "pbrt_version": [],
+    "pbrt_ctx_create": [I32, PP],
+    "pbrt_ctx_destroy": [P],
+    "pbrt_ctx_stream": [P, PP],
+    "pbrt_ctx_device_info": [P, ctypes.POINTER(I32), ctypes.c_char_p, ctypes.c_size_t],
+    "pbrt_dev_alloc": [P, ctypes.c_size_t, PP],
+    "pbrt_dev_free": [P, P],
+    "pbrt_memcpy_h2d": [P, P, P, ctypes.c_size_t],
+    "pbrt_memcpy_d2h": [P, P, P, ctypes.c_size_t],
+    "pbrt_memset_dev": [P, P, I32, ctypes.c_size_t],
+    "pbrt_stream_sync": [P, P],
+    "pbrt_event_create": [P, PP],
+    "pbrt_event_record": [P, P],
+    "pbrt_event_elapsed_ms": [P, P, ctypes.POINTER(F32)],
+    "pbrt_event_destroy": [P],
+    "pbrt_mesh_create": [P, P, U32, P, U32, ctypes.POINTER(BvhParams), PP],
+    "pbrt_mesh_destroy": [P],
+    "pbrt_mesh_build_stats": [P, ctypes.POINTER(BuildStats)],
+    "pbrt_mesh_triangle_count": [P, ctypes.POINTER(U32)],
+    "pbrt_mesh_device_bytes": [P, ctypes.POINTER(U64), ctypes.POINTER(U64)],
+    "pbrt_mesh_export_tree": [P, P, P, P, P, P, P, P],
+    "pbrt_mesh_trace_closest": [P, P, U32, P, P],
+    "pbrt_mesh_trace_closest_stats": [P, P, U32, P, P, P],
+    "pbrt_mesh_trace_any": [P, P, U32, P, P],
+    "pbrt_mesh_set_launch": [P, ctypes.POINTER(Launch)],
+    "pbrt_mesh_status": [P, ctypes.POINTER(I32)],
+    "pbrt_pinhole_rays": [U32, U32, F64, P, P, F32, F32, P],
+    "pbrt_orbit_rays": [P, U32, U32, U32, F64, F32, F32, P],
+    "pbrt_random_rays": [P, F64, U64, U64, U32, F32, F32, P],
+    "pbrt_gen_icosphere": [U32, F64, P, I32, F64, I32, F64, U64, P, P, ctypes.POINTER(U32), ctypes.POINTER(U32)],
+}
+_RESTYPES = {"pbrt_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+class PbrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pbrt error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library not built: {LIB_PATH} (run `python -m pbrt_amd.build`)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        raise PbrtError(code, lib().pbrt_last_error().decode(errors="replace"))
+    return code
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))

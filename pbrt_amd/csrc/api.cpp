@@ -1,0 +1,362 @@
+// C ABI of the ray path: context and device plumbing, mesh create/trace. See include/pbrt_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+#include "traverse.hpp"
+
+namespace pbrt {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace pbrt
+
+using namespace pbrt;
+
+struct pbrt_ctx {
+  int device = 0;
+  int cu_count = 0;
+  std::string name;
+  hipStream_t stream = nullptr;
+};
+
+struct pbrt_mesh {
+  pbrt_ctx* ctx = nullptr;
+  uint32_t nt = 0;
+  pbrt_build_stats stats{};
+  BuiltBVH host;            // kept for export_tree
+  std::vector<uint32_t> idx;
+  DevNode* nodes = nullptr;
+  DevTri* tris = nullptr;
+  uint64_t n_nodes = 0;
+  RootDesc root{};
+  uint2* spill = nullptr;
+  uint64_t spill_entries = 0;
+  int* status = nullptr;
+  pbrt_launch launch{128, 16, 16, 0};
+  std::mutex mu;
+};
+
+namespace {
+
+void check_ctx(pbrt_ctx* c) {
+  if (!c) throw Error(PBRT_ERR_INVALID_ARG, "null context");
+  PBRT_HIP(hipSetDevice(c->device));
+}
+
+hipStream_t stream_of(pbrt_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
+  LaunchArgs a{};
+  a.nodes = m->nodes;
+  a.tris = m->tris;
+  a.root = m->root;
+  a.n = n;
+  a.status = m->status;
+  a.stream = stream_of(m->ctx, stream);
+  a.block = m->launch.block;
+  a.lds_stack = m->launch.lds_stack;
+  a.waves_per_cu = m->launch.waves_per_cu;
+  a.cu_count = (uint32_t)m->ctx->cu_count;
+  return a;
+}
+
+// Sizes the HBM spill area for the launch shape and checks the kernels' error word.
+void prepare_spill(pbrt_mesh* m, LaunchArgs* a) {
+  const uint32_t threads = traversal_grid_threads(*a);
+  if (threads == 0) throw Error(PBRT_ERR_INVALID_ARG, "unsupported block size");
+  const uint64_t depth = m->stats.max_depth;  // the stack never holds more than depth - 1 entries
+  const uint32_t per_thread = depth + 1 > a->lds_stack ? (uint32_t)(depth + 1 - a->lds_stack) : 1;
+  const uint64_t need = (uint64_t)per_thread * threads;
+  if (need > m->spill_entries) {
+    if (m->spill) PBRT_HIP(hipFree(m->spill));
+    m->spill = nullptr;
+    m->spill_entries = 0;
+    PBRT_HIP(hipMalloc(&m->spill, need * sizeof(uint2)));
+    m->spill_entries = need;
+  }
+  a->spill = m->spill;
+  a->spill_entries_per_thread = per_thread;
+  a->spill_capacity_entries = m->spill_entries;
+}
+
+void run(pbrt_mesh* m, LaunchArgs& a, int kind) {
+  if (a.n == 0) return;
+  std::lock_guard<std::mutex> lock(m->mu);
+  prepare_spill(m, &a);
+  PBRT_HIP(launch_traversal(a, kind));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pbrt_last_error(void) { return g_last_error.c_str(); }
+int pbrt_version(void) { return 1; }
+
+int pbrt_ctx_create(int device, pbrt_ctx** out) {
+  return guarded([&] {
+    if (!out) throw Error(PBRT_ERR_INVALID_ARG, "null output");
+    int n = 0;
+    PBRT_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error(PBRT_ERR_INVALID_ARG, "no such device");
+    auto c = std::make_unique<pbrt_ctx>();
+    c->device = device;
+    PBRT_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    PBRT_HIP(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    c->name = prop.gcnArchName;
+    PBRT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+  });
+}
+
+int pbrt_ctx_destroy(pbrt_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int pbrt_ctx_stream(pbrt_ctx* ctx, void** s) {
+  return guarded([&] {
+    if (!ctx || !s) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    *s = ctx->stream;
+  });
+}
+
+int pbrt_ctx_device_info(pbrt_ctx* ctx, int* cu, char* name, size_t len) {
+  return guarded([&] {
+    if (!ctx) throw Error(PBRT_ERR_INVALID_ARG, "null context");
+    if (cu) *cu = ctx->cu_count;
+    if (name && len) {
+      std::strncpy(name, ctx->name.c_str(), len - 1);
+      name[len - 1] = 0;
+    }
+  });
+}
+
+int pbrt_dev_alloc(pbrt_ctx* ctx, size_t bytes, void** p) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!p) throw Error(PBRT_ERR_INVALID_ARG, "null output");
+    PBRT_HIP(hipMalloc(p, bytes ? bytes : 1));
+  });
+}
+
+int pbrt_dev_free(pbrt_ctx* ctx, void* p) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (p) PBRT_HIP(hipFree(p));
+  });
+}
+
+int pbrt_memcpy_h2d(pbrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (bytes) PBRT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int pbrt_memcpy_d2h(pbrt_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    PBRT_HIP(hipStreamSynchronize(ctx->stream));
+    if (bytes) PBRT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int pbrt_memset_dev(pbrt_ctx* ctx, void* dst, int value, size_t bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (bytes) PBRT_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  });
+}
+
+int pbrt_stream_sync(pbrt_ctx* ctx, void* s) {
+  return guarded([&] {
+    check_ctx(ctx);
+    PBRT_HIP(hipStreamSynchronize(stream_of(ctx, s)));
+  });
+}
+
+int pbrt_event_create(pbrt_ctx* ctx, void** ev) {
+  return guarded([&] {
+    check_ctx(ctx);
+    hipEvent_t e;
+    PBRT_HIP(hipEventCreate(&e));
+    *ev = e;
+  });
+}
+
+int pbrt_event_record(void* ev, void* s) {
+  return guarded([&] { PBRT_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)s)); });
+}
+
+int pbrt_event_elapsed_ms(void* a, void* b, float* ms) {
+  return guarded([&] {
+    PBRT_HIP(hipEventSynchronize((hipEvent_t)b));
+    PBRT_HIP(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  });
+}
+
+int pbrt_event_destroy(void* ev) {
+  return guarded([&] { PBRT_HIP(hipEventDestroy((hipEvent_t)ev)); });
+}
+
+int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                     const pbrt_bvh_params* params, pbrt_mesh** out) {
+  return guarded([&] {
+    if (ctx) check_ctx(ctx);
+    if (!out) throw Error(PBRT_ERR_INVALID_ARG, "null output");
+    if (nt == 0) throw Error(PBRT_ERR_EMPTY_MESH, "a TriangleMesh needs at least one triangle");
+    if (!xyz || !idx) throw Error(PBRT_ERR_INVALID_ARG, "null vertex or index array");
+    pbrt_bvh_params p = params ? *params : pbrt_bvh_params{12, 4, 1.0};
+    if (p.max_leaf > kMaxLeafTris) throw Error(PBRT_ERR_INVALID_ARG, "max_leaf above the device bound of 32");
+    auto m = std::make_unique<pbrt_mesh>();
+    m->ctx = ctx;
+    m->nt = nt;
+    m->host = build_mesh_bvh(xyz, nv, idx, nt, p);
+    m->stats = m->host.stats;
+    m->idx.assign(idx, idx + 3 * (size_t)nt);
+    std::vector<DevNode> nodes;
+    std::vector<DevTri> tris;
+    flatten_mesh_bvh(m->host, xyz, idx, &nodes, &tris, &m->root);
+    m->n_nodes = nodes.size();
+    if (!ctx) {  // host-only mesh: built and flattened for inspection, never traced
+      *out = m.release();
+      return;
+    }
+    PBRT_HIP(hipMalloc(&m->nodes, std::max<size_t>(1, nodes.size()) * sizeof(DevNode)));
+    PBRT_HIP(hipMalloc(&m->tris, tris.size() * sizeof(DevTri)));
+    PBRT_HIP(hipMalloc(&m->status, sizeof(int)));
+    if (!nodes.empty()) PBRT_HIP(hipMemcpy(m->nodes, nodes.data(), nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice));
+    PBRT_HIP(hipMemcpy(m->tris, tris.data(), tris.size() * sizeof(DevTri), hipMemcpyHostToDevice));
+    PBRT_HIP(hipMemset(m->status, 0, sizeof(int)));
+    *out = m.release();
+  });
+}
+
+int pbrt_mesh_destroy(pbrt_mesh* m) {
+  return guarded([&] {
+    if (!m) return;
+    if (m->ctx) hipSetDevice(m->ctx->device);
+    hipFree(m->nodes);
+    hipFree(m->tris);
+    hipFree(m->spill);
+    hipFree(m->status);
+    delete m;
+  });
+}
+
+int pbrt_mesh_build_stats(const pbrt_mesh* m, pbrt_build_stats* out) {
+  return guarded([&] {
+    if (!m || !out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    *out = m->stats;
+  });
+}
+
+int pbrt_mesh_triangle_count(const pbrt_mesh* m, uint32_t* out) {
+  return guarded([&] {
+    if (!m || !out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    *out = m->nt;
+  });
+}
+
+int pbrt_mesh_device_bytes(const pbrt_mesh* m, uint64_t* nb, uint64_t* tb) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    if (nb) *nb = m->n_nodes * sizeof(DevNode);
+    if (tb) *tb = (uint64_t)m->nt * sizeof(DevTri);
+  });
+}
+
+int pbrt_mesh_export_tree(const pbrt_mesh* m, double* boxes, uint8_t* is_leaf, uint32_t* left, uint32_t* right,
+                          uint32_t* first, uint32_t* count, uint32_t* leaf_tris) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    const auto& b = m->host;
+    for (size_t i = 0; i < b.nodes.size(); ++i) {
+      const BuildNode& n = b.nodes[i];
+      for (int k = 0; k < 3; ++k) {
+        boxes[6 * i + k] = n.lo[k];
+        boxes[6 * i + 3 + k] = n.hi[k];
+      }
+      is_leaf[i] = n.leaf ? 1 : 0;
+      left[i] = n.leaf ? 0 : n.left;
+      right[i] = n.leaf ? 0 : n.right;
+      first[i] = n.leaf ? n.first : 0;
+      count[i] = n.leaf ? n.count : 0;
+    }
+    for (size_t i = 0; i < b.order.size(); ++i) leaf_tris[i] = b.order[i];
+  });
+}
+
+int pbrt_mesh_set_launch(pbrt_mesh* m, const pbrt_launch* l) {
+  return guarded([&] {
+    if (!m || !l) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    pbrt_launch v = *l;
+    if (!v.block) v.block = 128;
+    if (!v.waves_per_cu) v.waves_per_cu = 16;
+    if (!v.lds_stack) v.lds_stack = 16;
+    if (v.block != 64 && v.block != 128 && v.block != 256) throw Error(PBRT_ERR_INVALID_ARG, "block must be 64, 128 or 256");
+    if (v.lds_stack != 8 && v.lds_stack != 16 && v.lds_stack != 24 && v.lds_stack != 32)
+      throw Error(PBRT_ERR_INVALID_ARG, "lds_stack must be 8, 16, 24 or 32");
+    if (v.waves_per_cu > 32) throw Error(PBRT_ERR_INVALID_ARG, "waves_per_cu above 32");
+    m->launch = v;
+  });
+}
+
+static int trace_common(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, pbrt_hit* hits, uint8_t* occ, uint32_t* stats,
+                        void* stream, int kind) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
+    check_ctx(m->ctx);
+    if (n && !rays) throw Error(PBRT_ERR_INVALID_ARG, "null rays");
+    if (n && kind != 2 && !hits) throw Error(PBRT_ERR_INVALID_ARG, "null hits");
+    if (n && kind == 2 && !occ) throw Error(PBRT_ERR_INVALID_ARG, "null occlusion output");
+    if (n && kind == 1 && !stats) throw Error(PBRT_ERR_INVALID_ARG, "null stats output");
+    LaunchArgs a = base_args(m, n, stream);
+    a.rays = rays;
+    a.hits = hits;
+    a.occluded = occ;
+    a.stats = stats;
+    run(m, a, kind);
+  });
+}
+
+int pbrt_mesh_trace_closest(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, pbrt_hit* hits, void* stream) {
+  return trace_common(m, rays, n, hits, nullptr, nullptr, stream, 0);
+}
+
+int pbrt_mesh_trace_closest_stats(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, pbrt_hit* hits, uint32_t* stats,
+                                  void* stream) {
+  return trace_common(m, rays, n, hits, nullptr, stats, stream, 1);
+}
+
+int pbrt_mesh_trace_any(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, uint8_t* occ, void* stream) {
+  return trace_common(m, rays, n, nullptr, occ, nullptr, stream, 2);
+}
+
+int pbrt_mesh_status(pbrt_mesh* m, int* status_out) {
+  return guarded([&] {
+    if (!m || !status_out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
+    check_ctx(m->ctx);
+    PBRT_HIP(hipDeviceSynchronize());
+    PBRT_HIP(hipMemcpy(status_out, m->status, sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

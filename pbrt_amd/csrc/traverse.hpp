@@ -1,0 +1,29 @@
+// Launch interface of the traversal kernels (traverse.hip).
+#pragma once
+#include "common.hpp"
+
+namespace pbrt {
+
+struct LaunchArgs {
+  const DevNode* nodes;
+  const DevTri* tris;
+  RootDesc root;
+  const pbrt_ray* rays;
+  pbrt_hit* hits;
+  uint8_t* occluded;
+  uint32_t* stats;
+  uint32_t n;
+  uint2* spill;
+  uint32_t spill_entries_per_thread;
+  uint64_t spill_capacity_entries;
+  int* status;
+  void* stream;
+  uint32_t block, lds_stack, waves_per_cu, cu_count;
+};
+
+// kind: 0 closest hit, 1 closest hit with per-ray work counters, 2 any hit.
+hipError_t launch_traversal(const LaunchArgs& a, int kind);
+// Threads of the persistent grid the launch would use (the spill area is sized from it).
+uint32_t traversal_grid_threads(const LaunchArgs& a);
+
+}  // namespace pbrt

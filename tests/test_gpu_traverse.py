@@ -1,0 +1,153 @@
+"""Parity of the device traversal kernels with the FP32 oracle (Oracle B), through the C ABI.
+
+Bar: triangle indices bit-exact, t/u/v bit-exact (same FP32 operation sequence), per-ray work
+counters equal to the oracle's on the same tree, any-hit equal; hit t within 1e-5 relative of an
+f64 re-evaluation of the hit triangle. Published hit counts (docs/mesures_bvh.md:706-713) are
+reproduced on the device.
+"""
+import numpy as np
+import pytest
+
+import pbrt_amd
+from conftest import PUBLISHED, grid_mesh, grid_ray_set, load_mesh
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def as_rows(rays):
+    return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+
+
+def check_closest(ctx, v, f, rays_rows, params, stats=True, launch=None):
+    rays_rows = np.ascontiguousarray(rays_rows, np.float32)
+    mesh = pbrt_amd.TriangleMesh(ctx, v, f, **params)
+    if launch:
+        mesh.set_launch(**launch)
+    ref = po.Mesh(v, f, precision=32, **params).trace(rays_rows, mode=po.MODE_CANONICAL, threads=8,
+                                                      per_ray_stats=stats)
+    if stats:
+        hits, st = mesh.trace_host(rays_rows, stats=True)
+    else:
+        hits = mesh.trace_host(rays_rows)
+    assert mesh.status() == 0
+    np.testing.assert_array_equal(hits["prim"].astype(np.int64), ref["prim"])
+    hit = ref["prim"] >= 0
+    np.testing.assert_array_equal(hits["t"][hit].astype(np.float64), ref["t"][hit])
+    np.testing.assert_array_equal(hits["u"][hit].astype(np.float64), ref["u"][hit])
+    np.testing.assert_array_equal(hits["v"][hit].astype(np.float64), ref["v"][hit])
+    if stats:
+        np.testing.assert_array_equal(st, ref["stats"])
+    return mesh, hits, ref
+
+
+@pytest.mark.parametrize("name", list(PUBLISHED))
+@pytest.mark.parametrize("which", ["reference", "device"])
+def test_published_meshes_bit_exact(gpu_ctx, name, which):
+    v, f = load_mesh(name)
+    params = pbrt_amd.REFERENCE_BVH if which == "reference" else pbrt_amd.DEVICE_BVH
+    rays = po.orbit_rays(v).astype(np.float32)
+    _, hits, _ = check_closest(gpu_ctx, v, f, rays, params)
+    assert int((hits["prim"] >= 0).sum()) == PUBLISHED[name][0]
+
+
+def test_hit_t_against_f64_reevaluation(gpu_ctx):
+    """FP32 hit t versus the same triangle re-intersected in f64 by the f64 oracle's brute force on
+    a one-triangle mesh, for a sample of hits on the bunny."""
+    v, f = load_mesh("bunny")
+    rays = po.orbit_rays(v).astype(np.float32)
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
+    hits = mesh.trace_host(rays)
+    idx = np.nonzero(hits["prim"] >= 0)[0][::97]
+    worst = 0.0
+    for i in idx:
+        tri = f[hits["prim"][i]]
+        m = po.Mesh(v[tri], np.array([[0, 1, 2]], np.uint32), precision=64)
+        r = m.trace(rays[i:i + 1].astype(np.float64), mode=po.MODE_BRUTE)
+        if r["prim"][0] == 0:
+            worst = max(worst, abs(float(hits["t"][i]) - r["t"][0]) / r["t"][0])
+    assert worst < 1e-5, worst
+
+
+@pytest.mark.parametrize("precision_params", [pbrt_amd.REFERENCE_BVH, pbrt_amd.DEVICE_BVH])
+def test_reference_grid_fixture_and_brute_force(gpu_ctx, precision_params):
+    """The reference's own fixture (triangle_mesh/bvh.rs:746-854): grid mesh, centroid rays and
+    axis-aligned pencils (exactly-zero direction components take the literal slab path)."""
+    v, f = grid_mesh(3)
+    rays = grid_ray_set(v, f).astype(np.float32)
+    _, hits, _ = check_closest(gpu_ctx, v, f, rays, precision_params)
+    brute = po.Mesh(v, f, precision=32).trace(rays, mode=po.MODE_BRUTE)
+    np.testing.assert_array_equal(hits["prim"].astype(np.int64), brute["prim"])
+    assert (hits["prim"] >= 0).sum() > 20
+
+
+def test_any_hit_and_far_sweep(gpu_ctx):
+    """triangle_mesh/bvh.rs:937-975 on the device: any-hit equals closest-hit existence, also with
+    `far` cut at factors of the nearest distance; and equals the oracle's any-hit."""
+    v, f = load_mesh("dragon_vrip_res4")
+    rays = po.orbit_rays(v).astype(np.float32)[::3]
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
+    orc = po.Mesh(v, f, precision=32, **pbrt_amd.DEVICE_BVH)
+    hits = mesh.trace_host(rays)
+    occ = mesh.trace_host(rays, mode="any")
+    np.testing.assert_array_equal(occ.astype(bool), hits["prim"] >= 0)
+    hit = np.nonzero(hits["prim"] >= 0)[0]
+    for factor in [0.5, 0.99, 0.999999, 1.0, 1.000001]:
+        rr = rays[hit].copy()
+        rr[:, 7] = (hits["t"][hit].astype(np.float64) * factor).astype(np.float32)
+        occ = mesh.trace_host(rr, mode="any")
+        ref = orc.trace(rr, mode=po.MODE_ANY)
+        np.testing.assert_array_equal(occ.astype(np.int64), (ref["prim"] >= 0).astype(np.int64))
+        closest = mesh.trace_host(rr)
+        np.testing.assert_array_equal(occ.astype(bool), closest["prim"] >= 0)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("lds_stack", [8, 16, 32])
+def test_launch_shapes_agree(gpu_ctx, block, lds_stack):
+    """Every launch shape gives the oracle's answer, including an 8-entry LDS stack that spills to
+    HBM on the deep reference-mode tree."""
+    v, f = load_mesh("dragon_vrip_res3")
+    rays = np.concatenate([as_rows(pbrt_amd.orbit_rays(v, 1, 96)),
+                           as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 20000, seed=9))])
+    check_closest(gpu_ctx, v, f, rays, pbrt_amd.REFERENCE_BVH, stats=True,
+                  launch=dict(block=block, lds_stack=lds_stack, waves_per_cu=8))
+
+
+def test_incoherent_rays_on_displaced_icosphere(gpu_ctx):
+    v, f = pbrt_amd.icosphere(6, noise="fbm", amp=0.08, octaves=3, seed=2)
+    rays = as_rows(pbrt_amd.random_rays((0, 0, 0), 1.1, 100000, seed=4))
+    _, hits, _ = check_closest(gpu_ctx, v, f, rays, pbrt_amd.DEVICE_BVH)
+    # rays start inside the bounding ball, most of them inside the closed surface: nearly all hit
+    assert (hits["prim"] >= 0).mean() > 0.9
+
+
+def test_edge_cases(gpu_ctx):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)  # the root is a leaf
+    nan = np.float32("nan")
+    rays = np.array([
+        [0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, -1.0, 1000.0],   # straight down: t = 1
+        [0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, 1.0, 1000.0],    # away
+        [0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, -1.0, 0.5],      # far before the hit
+        [0.25, 0.25, 1.0, 2.0, 0.0, 0.0, -1.0, 1000.0],    # near past the hit
+        [0.25, 0.25, 1.0, 1e-4, nan, nan, nan, 1000.0],    # NaN direction: no hit, terminates
+        [0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, -1.0, np.inf],   # infinite far
+    ], np.float32)
+    hits = mesh.trace_host(rays)
+    assert hits["prim"].tolist() == [0, -1, -1, -1, -1, 0]
+    assert hits["t"][0] == 1.0
+    # an empty batch is a no-op
+    d = gpu_ctx.alloc(64)
+    mesh.query(d, 0, d)
+    assert mesh.status() == 0
+
+
+def test_repeated_runs_are_bitwise_identical(gpu_ctx):
+    v, f = pbrt_amd.icosphere(5, noise="fbm", amp=0.1, octaves=5, seed=1)
+    rays = as_rows(pbrt_amd.random_rays((0, 0, 0), 1.0, 50000, seed=8))
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
+    a = mesh.trace_host(rays)
+    b = mesh.trace_host(rays)
+    assert a.tobytes() == b.tobytes()
