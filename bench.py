@@ -46,27 +46,30 @@ def make_workload(pa, side=2048, n_incoherent=1 << 22):
 
 
 def cpu_baseline(v, f, coherent, incoherent, side, threads):
-    """f64 restatement of the reference CPU traversal on a bounded sample: every 16th image row of
-    batch (a) and the first 262,144 rays of batch (b)."""
+    """f64 restatement of the reference CPU traversal over the whole step (both 4M-ray batches, all
+    host threads up to 16), plus a one-thread pass over every 16th image row of batch (a) and the
+    first 131,072 rays of batch (b) for comparison with the reference's single-thread figures."""
     from oracle import pyoracle as po
     t0 = time.time()
     m = po.Mesh(v, f, precision=64, bins=8, t_trav=0.0)
     build_s = time.time() - t0
-    a = rows(coherent).reshape(side, side, 8)[::16].reshape(-1, 8)
-    b = rows(incoherent)[:262144]
-    sample = np.concatenate([a, b])
-    m.trace(sample[:4096], mode=po.MODE_REFERENCE, threads=threads)  # warm-up
+    sample = np.concatenate([rows(coherent), rows(incoherent)])
+    m.trace(sample[:65536], mode=po.MODE_REFERENCE, threads=threads)  # warm-up
     best = None
     for _ in range(2):
         t0 = time.time()
         m.trace(sample, mode=po.MODE_REFERENCE, threads=threads)
         dt = time.time() - t0
         best = dt if best is None else min(best, dt)
+    small = np.concatenate([rows(coherent).reshape(side, side, 8)[::16].reshape(-1, 8), rows(incoherent)[:131072]])
+    t0 = time.time()
+    m.trace(small, mode=po.MODE_REFERENCE, threads=1)
+    one = len(small) / (time.time() - t0) / 1e6
     return dict(value=len(sample) / best / 1e6, unit="Mrays/s", cores=threads, kind="port",
-                sample=f"{len(sample)} rays: every 16th row of the 2048^2 coherent batch + 262144 incoherent rays; "
-                       f"f64 C++ restatement of the reference traversal, reference BVH (8 bins, t_trav=0), "
-                       f"fastest of 2 passes after a warm-up; build {build_s:.1f}s",
-                seconds=best)
+                sample=f"the full step ({len(sample)} rays, both batches) on {threads} threads; f64 C++ restatement "
+                       f"of the reference traversal over the reference BVH (8 bins, t_trav=0), fastest of 2 passes "
+                       f"after a warm-up; BVH build {build_s:.1f}s not timed",
+                seconds=best, one_thread_mrays_s=round(one, 3), one_thread_sample_rays=len(small))
 
 
 def main():

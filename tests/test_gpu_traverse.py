@@ -118,8 +118,9 @@ def test_incoherent_rays_on_displaced_icosphere(gpu_ctx):
     v, f = pbrt_amd.icosphere(6, noise="fbm", amp=0.08, octaves=3, seed=2)
     rays = as_rows(pbrt_amd.random_rays((0, 0, 0), 1.1, 100000, seed=4))
     _, hits, _ = check_closest(gpu_ctx, v, f, rays, pbrt_amd.DEVICE_BVH)
-    # rays start inside the bounding ball, most of them inside the closed surface: nearly all hit
-    assert (hits["prim"] >= 0).mean() > 0.9
+    # origins fill a ball of radius 1.1 around a closed surface of radius about 1: at least the
+    # (1/1.1)^3 = 75% that start inside it must hit
+    assert (hits["prim"] >= 0).mean() > 0.75
 
 
 def test_edge_cases(gpu_ctx):
