@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--side", type=int, default=2048)
+    ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard trivially; see DESIGN.md)",
@@ -129,29 +130,24 @@ def main():
         raise RuntimeError("traversal kernel reported a stack overflow")
 
     stream = ctx.stream
-    evs = [(ctx.event(), ctx.event()) for _ in range(2)]
+    # one event pair per launch, so the timed loop never waits on the host
+    evs = [[(ctx.event(), ctx.event()) for _ in dev] for _ in range(args.steps)]
 
-    def step(record):
-        times = []
-        for j, (k, (dr, dh, n)) in enumerate(dev.items()):
-            e0, e1 = evs[j]
+    for _ in range(args.warmup):
+        for dr, dh, n in dev.values():
+            mesh.query(dr, n, dh)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        for j, (dr, dh, n) in enumerate(dev.values()):
+            e0, e1 = evs[s][j]
             e0.record(stream)
             mesh.query(dr, n, dh)
             e1.record(stream)
-            if record:
-                times.append((e0, e1))
-        return times
-
-    for _ in range(args.warmup):
-        step(False)
-    ctx.synchronize()
-    per_batch_ms = {k: [] for k in dev}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for (e0, e1), k in zip(step(True), dev):
-            per_batch_ms[k].append(e0.elapsed_ms(e1))  # waits on e1
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    per_batch_ms = {k: [evs[s][j][0].elapsed_ms(evs[s][j][1]) for s in range(args.steps)]
+                    for j, k in enumerate(dev)}
 
     total_rays = sum(n for (_, _, n) in dev.values())
     kern_ms = {k: float(np.mean(x)) for k, x in per_batch_ms.items()}
@@ -169,13 +165,27 @@ def main():
     for k, (dr, dh, n) in dev.items():
         occ = ctx.alloc(n)
         mesh.intersect_p(dr, n, occ)
-        e0, e1 = evs[0]
+        e0, e1 = evs[0][0]
         e0.record(stream)
         for _ in range(5):
             mesh.intersect_p(dr, n, occ)
         e1.record(stream)
         anyhit[k] = n * 5 / (e0.elapsed_ms(e1) * 1e-3) / 1e6
         occ.free()
+
+    copy_gbs = None
+    if args.calibrate:
+        # a plain copy of 1 GiB: the known-size launch that hardware byte counters are scaled by
+        nbytes = 1 << 30
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        src.fill(1)
+        e0, e1 = evs[0][0]
+        e0.record(stream)
+        pa._native.call("pbrt_dev_copy", ctx.h, dst.ptr, src.ptr, nbytes, stream)
+        e1.record(stream)
+        copy_gbs = 2 * nbytes / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+        src.free()
+        dst.free()
 
     cpu = None
     if not args.no_cpu:
@@ -217,6 +227,7 @@ def main():
         },
         "per_batch": per_batch,
         "anyhit_mrays_s": anyhit,
+        "copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

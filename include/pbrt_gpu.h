@@ -82,6 +82,9 @@ int pbrt_dev_free(pbrt_ctx* ctx, void* dptr);
 int pbrt_memcpy_h2d(pbrt_ctx* ctx, void* dst_dev, const void* src, size_t bytes);
 int pbrt_memcpy_d2h(pbrt_ctx* ctx, void* dst, const void* src_dev, size_t bytes);
 int pbrt_memset_dev(pbrt_ctx* ctx, void* dst_dev, int value, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (bytes a multiple of 16); also the known-size
+ * reference for calibrating hardware byte counters. */
+int pbrt_dev_copy(pbrt_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes, void* stream);
 int pbrt_stream_sync(pbrt_ctx* ctx, void* stream);
 int pbrt_event_create(pbrt_ctx* ctx, void** ev_out);
 int pbrt_event_record(void* ev, void* stream);

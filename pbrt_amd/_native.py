@@ -44,6 +44,7 @@ SIGNATURES = {
     "pbrt_memcpy_h2d": [P, P, P, ctypes.c_size_t],
     "pbrt_memcpy_d2h": [P, P, P, ctypes.c_size_t],
     "pbrt_memset_dev": [P, P, I32, ctypes.c_size_t],
+    "pbrt_dev_copy": [P, P, P, ctypes.c_size_t, P],
     "pbrt_stream_sync": [P, P],
     "pbrt_event_create": [P, PP],
     "pbrt_event_record": [P, P],

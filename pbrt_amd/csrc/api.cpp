@@ -183,6 +183,14 @@ int pbrt_memset_dev(pbrt_ctx* ctx, void* dst, int value, size_t bytes) {
   });
 }
 
+int pbrt_dev_copy(pbrt_ctx* ctx, void* dst, const void* src, size_t bytes, void* s) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (bytes % 16) throw Error(PBRT_ERR_INVALID_ARG, "copy size must be a multiple of 16 bytes");
+    PBRT_HIP(launch_copy(src, dst, bytes, stream_of(ctx, s)));
+  });
+}
+
 int pbrt_stream_sync(pbrt_ctx* ctx, void* s) {
   return guarded([&] {
     check_ctx(ctx);

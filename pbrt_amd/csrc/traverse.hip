@@ -347,7 +347,23 @@ hipError_t launch_b(const LaunchArgs& a, int kind) {
   }
 }
 
+// Plain streaming copy (16-byte words, grid-stride): the known-size reference against which
+// hardware byte counters are calibrated.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n16) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256ull) dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
+  const uint64_t n16 = bytes / 16;
+  if (n16 == 0) return hipSuccess;
+  uint64_t blocks = (n16 + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, (const float4*)src,
+                     (float4*)dst, n16);
+  return hipGetLastError();
+}
 
 uint32_t traversal_grid_threads(const LaunchArgs& a) {
   uint32_t grid = 0;

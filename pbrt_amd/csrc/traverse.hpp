@@ -25,5 +25,7 @@ struct LaunchArgs {
 hipError_t launch_traversal(const LaunchArgs& a, int kind);
 // Threads of the persistent grid the launch would use (the spill area is sized from it).
 uint32_t traversal_grid_threads(const LaunchArgs& a);
+// Device-to-device copy of bytes (multiple of 16) by a plain streaming kernel.
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, void* stream);
 
 }  // namespace pbrt
