@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard trivially; see DESIGN.md)",
@@ -108,10 +109,19 @@ def main():
     print(f"[bench] BVH built in {build_s:.1f}s: {bs}", file=sys.stderr, flush=True)
 
     batches = {"coherent": coherent, "incoherent": incoherent}
+    all_rays = ctx.upload(np.concatenate([coherent, incoherent]))
+    all_hits = ctx.alloc(16 * (len(coherent) + len(incoherent)))
     dev = {}
+    off = 0
     for k, r in batches.items():
         n = len(r)
-        dev[k] = (ctx.upload(r), ctx.alloc(16 * n), n)
+        dev[k] = (all_rays.view(32 * off, 32 * n), all_hits.view(16 * off, 16 * n), n)
+        off += n
+
+    tuned = None
+    if not (args.block or args.waves or args.lds_stack or args.no_autotune):
+        tuned = mesh.autotune(all_rays, off, all_hits, any_hit=False, reps=2)
+        print(f"[bench] autotuned launch: {tuned}", file=sys.stderr, flush=True)
 
     # exact per-ray work on the same rays (instrumented kernel, untimed)
     work = {}
@@ -214,6 +224,8 @@ def main():
             "rays_per_step": total_rays,
             "bvh": dict(bs, node_bytes=node_bytes, tri_bytes=tri_bytes),
             "parallelism": "single GPU",
+            "launch": mesh.get_launch(),
+            "autotuned": tuned is not None,
         },
         "roofline": {
             "bound": "hbm",

@@ -121,6 +121,13 @@ int pbrt_mesh_trace_closest_stats(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uin
 int pbrt_mesh_trace_any(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, uint8_t* occluded_dev, void* stream);
 /* Launch shape used by the mesh's traversal kernels (all fields 0 = defaults). */
 int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
+int pbrt_mesh_get_launch(const pbrt_mesh* mesh, pbrt_launch* out);
+/* Autotuner (no counterpart in the reference, which has one CPU code path): times every supported
+ * launch shape — block size x LDS stack depth x resident waves per CU — of the closest-hit
+ * (any_hit = 0) or any-hit kernel on the caller's rays, `reps` timed runs each, and keeps the
+ * fastest for this mesh. out_dev receives hits (pbrt_hit) or occlusion bytes and is overwritten. */
+int pbrt_mesh_autotune(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, void* out_dev, int any_hit,
+                       uint32_t reps, pbrt_launch* chosen, float* best_ms);
 
 /* Sticky error word of the mesh's kernels (waits for the device): 0 = fine, bit 0 = a traversal
  * stack exceeded its bound (cannot happen for a tree of the reported depth). */

@@ -117,10 +117,23 @@ class DeviceArray:
         call("pbrt_memcpy_d2h", self.ctx.h, _ptr(out), self.ptr, out.nbytes)
         return out
 
+    def view(self, offset, nbytes):
+        """A non-owning window [offset, offset + nbytes) of this allocation."""
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("view outside the allocation")
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.nbytes, v.owner = self.ctx, int(nbytes), self
+        v.ptr = ctypes.c_void_p(self.ptr.value + offset)
+        v._view = True
+        return v
+
     def fill(self, byte):
         call("pbrt_memset_dev", self.ctx.h, self.ptr, byte, self.nbytes)
 
     def free(self):
+        if getattr(self, "_view", False):
+            self.ptr = None
+            return
         if self.ptr:
             lib().pbrt_dev_free(self.ctx.h, self.ptr)
             self.ptr = None
@@ -173,6 +186,18 @@ class TriangleMesh:
 
     def set_launch(self, block=0, waves_per_cu=0, lds_stack=0):
         call("pbrt_mesh_set_launch", self.h, ctypes.byref(Launch(block, waves_per_cu, lds_stack, 0)))
+
+    def get_launch(self):
+        l = Launch()
+        call("pbrt_mesh_get_launch", self.h, ctypes.byref(l))
+        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack)
+
+    def autotune(self, rays, n, out, any_hit=False, reps=2):
+        """Times every launch shape on these device rays and keeps the fastest (pbrt_mesh_autotune)."""
+        l = Launch()
+        ms = ctypes.c_float()
+        call("pbrt_mesh_autotune", self.h, rays.ptr, n, out.ptr, int(any_hit), reps, ctypes.byref(l), ctypes.byref(ms))
+        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, ms=ms.value)
 
     def export_tree(self):
         s = self.build_stats()

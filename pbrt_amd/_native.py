@@ -60,6 +60,8 @@ SIGNATURES = {
     "pbrt_mesh_trace_closest_stats": [P, P, U32, P, P, P],
     "pbrt_mesh_trace_any": [P, P, U32, P, P],
     "pbrt_mesh_set_launch": [P, ctypes.POINTER(Launch)],
+    "pbrt_mesh_get_launch": [P, ctypes.POINTER(Launch)],
+    "pbrt_mesh_autotune": [P, P, U32, P, I32, U32, ctypes.POINTER(Launch), ctypes.POINTER(F32)],
     "pbrt_mesh_status": [P, ctypes.POINTER(I32)],
     "pbrt_pinhole_rays": [U32, U32, F64, P, P, F32, F32, P],
     "pbrt_orbit_rays": [P, U32, U32, U32, F64, F32, F32, P],

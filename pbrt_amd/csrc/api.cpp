@@ -122,8 +122,8 @@ int pbrt_ctx_create(int device, pbrt_ctx** out) {
 int pbrt_ctx_destroy(pbrt_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
 }
@@ -258,11 +258,11 @@ int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_
 int pbrt_mesh_destroy(pbrt_mesh* m) {
   return guarded([&] {
     if (!m) return;
-    if (m->ctx) hipSetDevice(m->ctx->device);
-    hipFree(m->nodes);
-    hipFree(m->tris);
-    hipFree(m->spill);
-    hipFree(m->status);
+    if (m->ctx) (void)hipSetDevice(m->ctx->device);
+    (void)hipFree(m->nodes);
+    (void)hipFree(m->tris);
+    (void)hipFree(m->spill);
+    (void)hipFree(m->status);
     delete m;
   });
 }
@@ -355,6 +355,72 @@ int pbrt_mesh_trace_closest_stats(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n
 
 int pbrt_mesh_trace_any(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, uint8_t* occ, void* stream) {
   return trace_common(m, rays, n, nullptr, occ, nullptr, stream, 2);
+}
+
+// Times every supported launch shape of one traversal kernel on the caller's rays and keeps the
+// fastest: block {64, 128, 256} x LDS stack {8, 16, 24, 32} x resident waves per CU
+// {8, 12, 16, 24, 32}; each candidate is run `reps` times after one warm-up launch, scored by its
+// fastest run. The hits/occlusion output buffer is overwritten.
+int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out_dev, int any_hit, uint32_t reps,
+                       pbrt_launch* chosen, float* best_ms) {
+  return guarded([&] {
+    if (!m || !rays || !out_dev || !n) throw Error(PBRT_ERR_INVALID_ARG, "null argument or empty batch");
+    if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
+    check_ctx(m->ctx);
+    if (reps == 0) reps = 2;
+    hipEvent_t e0, e1;
+    PBRT_HIP(hipEventCreate(&e0));
+    PBRT_HIP(hipEventCreate(&e1));
+    const uint32_t blocks[] = {64, 128, 256};
+    const uint32_t depths[] = {8, 16, 24, 32};
+    const uint32_t waves[] = {8, 12, 16, 24, 32};
+    pbrt_launch best{};
+    float best_t = 1e30f;
+    const int kind = any_hit ? 2 : 0;
+    try {
+      for (uint32_t b : blocks)
+        for (uint32_t d : depths)
+          for (uint32_t w : waves) {
+            if (w < b / 64) continue;
+            m->launch = pbrt_launch{b, w, d, 0};
+            LaunchArgs a = base_args(m, n, nullptr);
+            a.rays = rays;
+            a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
+            a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
+            run(m, a, kind);  // warm-up
+            float cand = 1e30f;
+            for (uint32_t r = 0; r < reps; ++r) {
+              PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
+              run(m, a, kind);
+              PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
+              PBRT_HIP(hipEventSynchronize(e1));
+              float ms = 0.f;
+              PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+              cand = std::min(cand, ms);
+            }
+            if (cand < best_t) {
+              best_t = cand;
+              best = m->launch;
+            }
+          }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    m->launch = best;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+  });
+}
+
+int pbrt_mesh_get_launch(const pbrt_mesh* m, pbrt_launch* out) {
+  return guarded([&] {
+    if (!m || !out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    *out = m->launch;
+  });
 }
 
 int pbrt_mesh_status(pbrt_mesh* m, int* status_out) {

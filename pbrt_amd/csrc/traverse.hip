@@ -78,31 +78,31 @@ __device__ __forceinline__ bool box_fast(const RayState& r, float lx, float hx, 
 }
 
 // The reference's statement order, including its branch for a direction component of exactly
-// zero (inside the slab iff the origin is). Used only by rays that have such a component.
-__device__ __noinline__ bool box_literal(const RayState& r, float lx, float hx, float ly, float hy, float lz, float hz,
-                                         float tmax, float* entry) {
+// zero (inside the slab iff the origin is). Taken only by rays that have such a component.
+__device__ __forceinline__ bool slab_literal(float o, float d, float lo, float hi, float W, float* tmin, float* tmax) {
+  if (d == 0.0f) return !(o < lo || o > hi);
+  const float inv = 1.0f / d;
+  float t0 = (lo - o) * inv;
+  float t1 = (hi - o) * inv;
+  if (inv < 0.0f) {
+    const float s = t0;
+    t0 = t1;
+    t1 = s;
+  }
+  t0 = t0 - fabsf(t0) * W;
+  t1 = t1 + fabsf(t1) * W;
+  *tmin = fmaxf(t0, *tmin);
+  *tmax = fminf(t1, *tmax);
+  return !(*tmax < *tmin);
+}
+
+__device__ __forceinline__ bool box_literal(const RayState& r, float lx, float hx, float ly, float hy, float lz,
+                                            float hz, float tmax, float* entry) {
   const float W = slab_widening();
   float tmin = r.near;
-  const float lo[3] = {lx, ly, lz}, hi[3] = {hx, hy, hz};
-  const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if (d[i] == 0.0f) {
-      if (o[i] < lo[i] || o[i] > hi[i]) return false;
-      continue;
-    }
-    const float inv = 1.0f / d[i];
-    float t0 = (lo[i] - o[i]) * inv;
-    float t1 = (hi[i] - o[i]) * inv;
-    if (inv < 0.0f) {
-      float s = t0; t0 = t1; t1 = s;
-    }
-    t0 = t0 - fabsf(t0) * W;
-    t1 = t1 + fabsf(t1) * W;
-    tmin = fmaxf(t0, tmin);
-    tmax = fminf(t1, tmax);
-    if (tmax < tmin) return false;
-  }
+  if (!slab_literal(r.ox, r.dx, lx, hx, W, &tmin, &tmax)) return false;
+  if (!slab_literal(r.oy, r.dy, ly, hy, W, &tmin, &tmax)) return false;
+  if (!slab_literal(r.oz, r.dz, lz, hz, W, &tmin, &tmax)) return false;
   *entry = tmin;
   return true;
 }
@@ -168,6 +168,16 @@ struct Stack {
   }
 };
 
+// Per-ray state of the closest-hit traversal.
+struct Closest {
+  float min_t, t, u, v, cur_far;
+  uint32_t best, n_vis, n_box, n_tri;
+};
+
+// Flat traversal loop: each iteration advances every live lane by one interior node and/or one
+// leaf, and a lane whose ray is finished writes its result and takes its next ray at once (rays
+// i, i + stride, i + 2*stride, ... per lane) instead of waiting for the rest of its wave. The order
+// of node visits within a ray is the reference's, so results and work counters are unchanged.
 template <int BLOCK, int DEPTH, bool STATS>
 __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                    RootDesc root, const float4* __restrict__ rays,
@@ -177,79 +187,103 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
-  for (uint32_t i = st.gtid; i < n; i += stride) {
-    const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
-    uint32_t n_vis = 0, n_box = 1, n_tri = 0;
-    float min_t = FLT_MAX, bt = 0.f, bu = 0.f, bv = 0.f;
-    uint32_t best = 0xffffffffu;
-    float cur_far = r.far;  // min(far, min_t): the box tests' narrowed interval
-    float entry;
+  uint32_t i = st.gtid;
+  if (i >= n) return;
+  RayState r;
+  Closest s;
+  int32_t cur = 0;
+  bool live = false;
+  auto start = [&](uint32_t k) {
+    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    s.min_t = FLT_MAX;
+    s.t = s.u = s.v = 0.f;
+    s.cur_far = r.far;  // min(far, min_t): the box tests' narrowed interval
+    s.best = 0xffffffffu;
+    s.n_vis = 0;
+    s.n_box = 1;
+    s.n_tri = 0;
     st.sp = 0;
-    int32_t cur;
-    bool live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
+    float entry;
+    live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
     cur = root.ref;
-    while (live) {
-      if (cur >= 0) {
-        // interior node: test both children over [near, min(far, min_t)]
-        if (STATS) { ++n_vis; n_box += 2; }
-        const DevNode* nd = nodes + cur;
-        const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
-        const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
-        const float4 z = *reinterpret_cast<const float4*>(nd->z01);
-        const int4 c = *reinterpret_cast<const int4*>(nd->ref);
-        float dl, dr;
-        const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, cur_far, &dl);
-        const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, cur_far, &dr);
-        if (hl && hr) {
-          const bool right_first = dr < dl;
-          const int32_t nref = right_first ? c.y : c.x;
-          const int32_t fref = right_first ? c.x : c.y;
-          const float fd = right_first ? dl : dr;
-          if (!st.push(fref, fd)) { atomicOr(status, 1); live = false; break; }
-          cur = nref;
-          continue;
+  };
+  start(i);
+  for (;;) {
+    bool pop_now = false;
+    if (live && cur >= 0) {
+      // interior node: test both children over [near, min(far, min_t)]
+      if (STATS) { ++s.n_vis; s.n_box += 2; }
+      const DevNode* nd = nodes + cur;
+      const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
+      const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
+      const float4 z = *reinterpret_cast<const float4*>(nd->z01);
+      const int2 c = *reinterpret_cast<const int2*>(nd->ref);
+      float dl, dr;
+      const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, s.cur_far, &dl);
+      const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, s.cur_far, &dr);
+      if (hl && hr) {
+        const bool right_first = dr < dl;
+        if (!st.push(right_first ? c.x : c.y, right_first ? dl : dr)) {
+          atomicOr(status, 1);
+          live = false;
         }
-        if (hl) { cur = c.x; continue; }
-        if (hr) { cur = c.y; continue; }
+        cur = right_first ? c.y : c.x;
+      } else if (hl) {
+        cur = c.x;
+      } else if (hr) {
+        cur = c.y;
       } else {
-        // leaf: ~cur = (first << 5) | (count - 1)
-        if (STATS) ++n_vis;
-        const uint32_t code = ~(uint32_t)cur;
-        const uint32_t first = code >> kLeafCountBits;
-        const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-        for (uint32_t k = first; k < first + cnt; ++k) {
-          if (STATS) ++n_tri;
-          float t, u, v;
-          uint32_t id;
-          if (tri_test(r, tris, k, &t, &u, &v, &id)) {
-            if (t >= r.near && t <= r.far && (t < min_t || (t == min_t && id < best))) {
-              min_t = t; bt = t; bu = u; bv = v; best = id;
-              cur_far = fminf(r.far, min_t);
-            }
+        pop_now = true;
+      }
+    }
+    if (live && !pop_now && cur < 0) {
+      // leaf: ~cur = (first << 5) | (count - 1)
+      if (STATS) ++s.n_vis;
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        if (STATS) ++s.n_tri;
+        float t, u, v;
+        uint32_t id;
+        if (tri_test(r, tris, k, &t, &u, &v, &id)) {
+          if (t >= r.near && t <= r.far && (t < s.min_t || (t == s.min_t && id < s.best))) {
+            s.min_t = t; s.t = t; s.u = u; s.v = v; s.best = id;
+            s.cur_far = fminf(r.far, s.min_t);
           }
         }
       }
-      // pop the next pending node whose entry is not beyond the nearest hit
+      pop_now = true;
+    }
+    if (live && pop_now) {
+      // the next pending node whose entry is not beyond the nearest hit
       live = false;
       while (st.sp > 0) {
         const uint2 e = st.pop();
-        if (__uint_as_float(e.y) > min_t) continue;
+        if (__uint_as_float(e.y) > s.min_t) continue;
         cur = (int32_t)e.x;
         live = true;
         break;
       }
     }
-    const bool found = best != 0xffffffffu;
-    hits[i] = make_float4(__int_as_float(found ? (int32_t)best : -1), found ? bt : 0.f, found ? bu : 0.f,
-                          found ? bv : 0.f);
-    if (STATS) {
-      stats[3 * (size_t)i] = n_vis;
-      stats[3 * (size_t)i + 1] = n_box;
-      stats[3 * (size_t)i + 2] = n_tri;
+    if (!live) {
+      const bool found = s.best != 0xffffffffu;
+      hits[i] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
+                            found ? s.v : 0.f);
+      if (STATS) {
+        stats[3 * (size_t)i] = s.n_vis;
+        stats[3 * (size_t)i + 1] = s.n_box;
+        stats[3 * (size_t)i + 2] = s.n_tri;
+      }
+      i += stride;
+      if (i >= n) break;
+      start(i);
     }
   }
 }
 
+// Any hit, unordered (bvh.rs:381-420): the reference pushes left then right, so the right child is
+// examined first; the first triangle hit within [near, far] ends the ray. Same flat loop.
 template <int BLOCK, int DEPTH>
 __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                RootDesc root, const float4* __restrict__ rays,
@@ -258,59 +292,77 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
-  for (uint32_t i = st.gtid; i < n; i += stride) {
-    const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
-    float entry;
+  uint32_t i = st.gtid;
+  if (i >= n) return;
+  RayState r;
+  int32_t cur = 0;
+  bool live = false, hit = false;
+  auto start = [&](uint32_t k) {
+    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
     st.sp = 0;
-    bool hit = false;
-    int32_t cur = root.ref;
-    bool live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
-    while (live) {
-      if (cur >= 0) {
-        const DevNode* nd = nodes + cur;
-        const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
-        const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
-        const float4 z = *reinterpret_cast<const float4*>(nd->z01);
-        const int4 c = *reinterpret_cast<const int4*>(nd->ref);
-        float dl, dr;
-        const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, r.far, &dl);
-        const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, r.far, &dr);
-        // the reference pushes left then right, so the right child is examined first
-        if (hl && hr) {
-          if (!st.push(c.x, 0.f)) { atomicOr(status, 1); break; }
-          cur = c.y;
-          continue;
+    hit = false;
+    float entry;
+    live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
+    cur = root.ref;
+  };
+  start(i);
+  for (;;) {
+    bool pop_now = false;
+    if (live && cur >= 0) {
+      const DevNode* nd = nodes + cur;
+      const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
+      const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
+      const float4 z = *reinterpret_cast<const float4*>(nd->z01);
+      const int2 c = *reinterpret_cast<const int2*>(nd->ref);
+      float dl, dr;
+      const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, r.far, &dl);
+      const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, r.far, &dr);
+      if (hl && hr) {
+        if (!st.push(c.x, 0.f)) {
+          atomicOr(status, 1);
+          live = false;
         }
-        if (hr) { cur = c.y; continue; }
-        if (hl) { cur = c.x; continue; }
+        cur = c.y;
+      } else if (hr) {
+        cur = c.y;
+      } else if (hl) {
+        cur = c.x;
       } else {
-        const uint32_t code = ~(uint32_t)cur;
-        const uint32_t first = code >> kLeafCountBits;
-        const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-        for (uint32_t k = first; k < first + cnt; ++k) {
-          float t, u, v;
-          uint32_t id;
-          if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
-            hit = true;
-            break;
-          }
-        }
-        if (hit) break;
-      }
-      live = false;
-      if (st.sp > 0) {
-        cur = (int32_t)st.pop().x;
-        live = true;
+        pop_now = true;
       }
     }
-    occluded[i] = hit ? 1 : 0;
+    if (live && !pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        float t, u, v;
+        uint32_t id;
+        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+          hit = true;
+          break;
+        }
+      }
+      pop_now = true;
+      if (hit) live = false;
+    }
+    if (live && pop_now) {
+      live = st.sp > 0;
+      if (live) cur = (int32_t)st.pop().x;
+    }
+    if (!live) {
+      occluded[i] = hit ? 1 : 0;
+      i += stride;
+      if (i >= n) break;
+      start(i);
+    }
   }
 }
 
 template <int BLOCK, int DEPTH>
 void launch_shape(const LaunchArgs& a, uint32_t* grid) {
   const uint32_t per_block_waves = BLOCK / 64;
-  uint32_t g = a.cu_count * (a.waves_per_cu / per_block_waves > 0 ? a.waves_per_cu / per_block_waves : 1);
+  const uint32_t g = a.cu_count * (a.waves_per_cu / per_block_waves > 0 ? a.waves_per_cu / per_block_waves : 1);
   const uint32_t need = (a.n + BLOCK - 1) / BLOCK;
   *grid = g < need ? g : need;
   if (*grid == 0) *grid = 1;
