@@ -228,3 +228,188 @@ int orc_box_hit(int precision, const double* box6, const double* ray6, double tm
 double orc_slab_widening(int precision) { return precision == 64 ? slab_widening<double>() : (double)slab_widening<float>(); }
 
 }  // extern "C"
+
+// ---- scene level -------------------------------------------------------------------------------
+#include "scene_oracle.hpp"
+
+namespace {
+struct SceneHandle {
+  SceneOracle<double>* a = nullptr;
+  SceneOracle<float>* b = nullptr;
+  ~SceneHandle() {
+    delete a;
+    delete b;
+  }
+};
+
+template <class R, class RayT>
+void scene_trace_impl(const SceneOracle<R>* s, const RayT* rays, uint64_t n, int any, int64_t* obj, int64_t* prim,
+                      double* t, double* u, double* v, double* p, uint64_t* stats_sum, int threads) {
+  std::vector<typename SceneOracle<R>::SStats> per(std::max(threads, 1));
+  parallel_for(n, threads, [&](uint64_t i, int tid) {
+    const RayT* r = rays + 8 * i;
+    Ray<R> ray{{(R)r[0], (R)r[1], (R)r[2]}, {(R)r[4], (R)r[5], (R)r[6]}};
+    R near = (R)r[3], far = (R)r[7];
+    auto& st = per[tid];
+    if (any) {
+      obj[i] = s->intersect_p(ray, near, far, &st) ? 1 : -1;
+      return;
+    }
+    SceneHit<R> h;
+    bool f = s->intersect(ray, near, far, &h, &st);
+    obj[i] = f ? h.object : -1;
+    if (prim) prim[i] = f ? h.prim : -1;
+    if (t) t[i] = f ? (double)h.is.d : 0.0;
+    if (u) u[i] = f ? (double)h.is.u : 0.0;
+    if (v) v[i] = f ? (double)h.is.v : 0.0;
+    if (p) {
+      p[3 * i] = f ? (double)h.is.p.x : 0.0;
+      p[3 * i + 1] = f ? (double)h.is.p.y : 0.0;
+      p[3 * i + 2] = f ? (double)h.is.p.z : 0.0;
+    }
+  });
+  if (stats_sum) {
+    stats_sum[0] = stats_sum[1] = stats_sum[2] = 0;
+    for (auto& q : per) {
+      stats_sum[0] += q.nodes_visited;
+      stats_sum[1] += q.box_tests;
+      stats_sum[2] += q.object_tests;
+    }
+  }
+}
+
+template <class R>
+void render_impl(const SceneOracle<R>* s, uint32_t w, uint32_t h, double fov, const double* eye, const double* look,
+                 uint32_t spp, uint32_t max_depth, uint64_t seed, double near, double far, uint32_t y0, uint32_t y1,
+                 double* film, uint64_t* rays_cast, int threads) {
+  PinHole cam(w, h, fov, near, far, m4_look_at({eye[0], eye[1], eye[2]}, {look[0], look[1], look[2]}, {0, 1, 0}));
+  std::vector<uint64_t> per(std::max(threads, 1), 0);
+  const uint64_t npix = (uint64_t)w * (y1 - y0);
+  parallel_for(npix, threads, [&](uint64_t k, int tid) {
+    const uint64_t px = k % w, py = y0 + k / w;
+    const uint64_t pixel = py * w + px;
+    Spec<R> sum{{0, 0, 0}};
+    double acc[3] = {0, 0, 0};
+    for (uint32_t sidx = 0; sidx < spp; ++sidx) {
+      const uint64_t path = pixel * spp + sidx;
+      // renderers/mt.rs:123-127: pixel + jitter, camera ray, li
+      double jx = rand_u(seed, path, 0, kJitterX), jy = rand_u(seed, path, 0, kJitterY);
+      Ray<double> cr = cam.get_ray((double)px + jx, (double)py + jy);
+      Ray<R> ray{{(R)cr.o.x, (R)cr.o.y, (R)cr.o.z}, {(R)cr.d.x, (R)cr.d.y, (R)cr.d.z}};
+      Spec<R> L = s->li(ray, (R)near, (R)far, max_depth, seed, path, &per[tid]);
+      for (int c = 0; c < 3; ++c) acc[c] += (double)L.c[c];
+      (void)sum;
+    }
+    for (int c = 0; c < 3; ++c) film[3 * k + c] = acc[c] / (double)spp;
+  });
+  if (rays_cast) {
+    *rays_cast = 0;
+    for (auto x : per) *rays_cast += x;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+// precision 64: reference arithmetic and tie order; 32: FP32 with the order-independent tie rule.
+void* orc_scene_create(int precision, const float* verts, const uint64_t* vert_off, const uint32_t* idx,
+                       const uint64_t* idx_off, uint32_t n_mesh, const void* objects, uint32_t n_obj,
+                       const void* materials, uint32_t n_mat, const void* lights, uint32_t n_light, int bins,
+                       double t_trav, int max_leaf) {
+  BuildParams bp;
+  bp.bins = bins;
+  bp.t_trav = t_trav;
+  bp.max_leaf = max_leaf;
+  auto* h = new SceneHandle();
+  if (precision == 64) {
+    h->a = new SceneOracle<double>();
+    h->a->build(verts, vert_off, idx, idx_off, n_mesh, (const ObjectDesc*)objects, n_obj, (const MaterialDesc*)materials,
+                n_mat, (const LightDesc*)lights, n_light, bp, false);
+  } else {
+    h->b = new SceneOracle<float>();
+    h->b->build(verts, vert_off, idx, idx_off, n_mesh, (const ObjectDesc*)objects, n_obj, (const MaterialDesc*)materials,
+                n_mat, (const LightDesc*)lights, n_light, bp, true);
+  }
+  return h;
+}
+
+void orc_scene_destroy(void* h) { delete (SceneHandle*)h; }
+
+uint32_t orc_desc_sizes(uint32_t* obj, uint32_t* mat, uint32_t* light) {
+  *obj = sizeof(ObjectDesc);
+  *mat = sizeof(MaterialDesc);
+  *light = sizeof(LightDesc);
+  return 0;
+}
+
+int orc_scene_trace_f32(void* hv, const float* rays, uint64_t n, int any, int64_t* obj, int64_t* prim, double* t,
+                        double* u, double* v, double* p, uint64_t* stats, int threads) {
+  auto* h = (SceneHandle*)hv;
+  if (h->a) scene_trace_impl(h->a, rays, n, any, obj, prim, t, u, v, p, stats, threads);
+  else scene_trace_impl(h->b, rays, n, any, obj, prim, t, u, v, p, stats, threads);
+  return 0;
+}
+
+int orc_scene_trace_f64(void* hv, const double* rays, uint64_t n, int any, int64_t* obj, int64_t* prim, double* t,
+                        double* u, double* v, double* p, uint64_t* stats, int threads) {
+  auto* h = (SceneHandle*)hv;
+  if (h->a) scene_trace_impl(h->a, rays, n, any, obj, prim, t, u, v, p, stats, threads);
+  else scene_trace_impl(h->b, rays, n, any, obj, prim, t, u, v, p, stats, threads);
+  return 0;
+}
+
+// Pinhole rays through pixel centres (pin_hole.rs:17-50), 8 doubles each, row-major.
+void orc_pinhole_rays(uint32_t w, uint32_t hgt, double fov, const double* eye, const double* look, const double* up,
+                      double near, double far, double* out) {
+  PinHole cam(w, hgt, fov, near, far, m4_look_at({eye[0], eye[1], eye[2]}, {look[0], look[1], look[2]}, {up[0], up[1], up[2]}));
+  for (uint32_t y = 0; y < hgt; ++y)
+    for (uint32_t x = 0; x < w; ++x) {
+      Ray<double> r = cam.get_ray((double)x + 0.5, (double)y + 0.5);
+      double* o = out + 8 * ((uint64_t)y * w + x);
+      o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = near;
+      o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = far;
+    }
+}
+
+// The scene-mode measurement of bvh_stats.rs:254-286 in f64: primary rays through every pixel
+// centre, and from each hit a shadow ray towards `target` over [shadow_near, distance]. out:
+// primary rays, primary hits, shadow hits; stats: primary {nodes, boxes, objects}, shadow {...}.
+int orc_scene_bvh_stats(void* hv, uint32_t w, uint32_t hgt, double fov, const double* eye, const double* look,
+                        const double* up, double near, double far, const double* target, double shadow_near,
+                        uint64_t* out, uint64_t* stats) {
+  auto* h = (SceneHandle*)hv;
+  if (!h->a) return -1;
+  const SceneOracle<double>& s = *h->a;
+  PinHole cam(w, hgt, fov, near, far, m4_look_at({eye[0], eye[1], eye[2]}, {look[0], look[1], look[2]}, {up[0], up[1], up[2]}));
+  SceneOracle<double>::SStats ps, ss;
+  uint64_t ph = 0, sh = 0;
+  const V3<double> tgt{target[0], target[1], target[2]};
+  for (uint32_t y = 0; y < hgt; ++y)
+    for (uint32_t x = 0; x < w; ++x) {
+      Ray<double> r = cam.get_ray((double)x + 0.5, (double)y + 0.5);
+      SceneHit<double> hit;
+      if (!s.intersect(r, near, far, &hit, &ps)) continue;
+      ph++;
+      Ray<double> sr{hit.is.p, normalized(normalized(tgt - hit.is.p))};  // Ray::spawn_from_through
+      double dist = length(tgt - hit.is.p);
+      if (s.intersect_p(sr, shadow_near, dist, &ss)) sh++;
+    }
+  out[0] = (uint64_t)w * hgt;
+  out[1] = ph;
+  out[2] = sh;
+  stats[0] = ps.nodes_visited; stats[1] = ps.box_tests; stats[2] = ps.object_tests;
+  stats[3] = ss.nodes_visited; stats[4] = ss.box_tests; stats[5] = ss.object_tests;
+  return 0;
+}
+
+// Rows [y0, y1) of a width x height render: film receives the per-pixel mean radiance (f64 RGB).
+int orc_scene_render(void* hv, uint32_t w, uint32_t hgt, double fov, const double* eye, const double* look, uint32_t spp,
+                     uint32_t max_depth, uint64_t seed, double near, double far, uint32_t y0, uint32_t y1, double* film,
+                     uint64_t* rays_cast, int threads) {
+  auto* h = (SceneHandle*)hv;
+  if (h->a) render_impl(h->a, w, hgt, fov, eye, look, spp, max_depth, seed, near, far, y0, y1, film, rays_cast, threads);
+  else render_impl(h->b, w, hgt, fov, eye, look, spp, max_depth, seed, near, far, y0, y1, film, rays_cast, threads);
+  return 0;
+}
+
+}  // extern "C"

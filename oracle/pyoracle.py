@@ -131,3 +131,94 @@ def box_hit(precision, box, ray, tmin, tmax):
 
 def slab_widening(precision):
     return lib().orc_slab_widening(precision)
+
+
+# ---- scene level ----------------------------------------------------------------------------------
+def _scene_sigs(L):
+    if getattr(L, "_scene_sigs", False):
+        return
+    L.orc_scene_create.restype = P
+    L.orc_scene_create.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_uint32, P, ctypes.c_uint32, P, ctypes.c_uint32, P,
+                                   ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    L.orc_scene_destroy.argtypes = [P]
+    for fn in (L.orc_scene_trace_f32, L.orc_scene_trace_f64):
+        fn.restype = ctypes.c_int
+        fn.argtypes = [P, P, U64, ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int]
+    L.orc_scene_render.restype = ctypes.c_int
+    L.orc_scene_render.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, ctypes.c_uint32,
+                                   ctypes.c_uint32, U64, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
+                                   ctypes.c_uint32, P, P, ctypes.c_int]
+    L.orc_pinhole_rays.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, P, ctypes.c_double,
+                                   ctypes.c_double, P]
+    L.orc_scene_bvh_stats.restype = ctypes.c_int
+    L.orc_scene_bvh_stats.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, P, ctypes.c_double,
+                                      ctypes.c_double, P, ctypes.c_double, P, P]
+    L.orc_desc_sizes.argtypes = [P, P, P]
+    L._scene_sigs = True
+
+
+def pinhole_rays(w, h, fov, eye, look, up=(0.0, 1.0, 0.0), near=1e-4, far=1000.0):
+    L = lib()
+    _scene_sigs(L)
+    out = np.empty((w * h, 8), np.float64)
+    a = [np.ascontiguousarray(x, np.float64) for x in (eye, look, up)]
+    L.orc_pinhole_rays(w, h, fov, _p(a[0]), _p(a[1]), _p(a[2]), near, far, _p(out))
+    return out
+
+
+class Scene:
+    """The oracle's scene (objects, median-split scene BVH, materials, lights) from a SceneDesc-like
+    object exposing object_array / material_array / light_array / mesh_arrays.
+    precision=64: the reference's arithmetic and ties; 32: FP32, order-independent ties."""
+
+    def __init__(self, desc, precision=64, bins=8, t_trav=0.0, max_leaf=0):
+        L = lib()
+        _scene_sigs(L)
+        so, sm, sl = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        L.orc_desc_sizes(ctypes.byref(so), ctypes.byref(sm), ctypes.byref(sl))
+        self.objs = desc.object_array()
+        self.mats = desc.material_array()
+        self.lights = desc.light_array()
+        assert (self.objs.dtype.itemsize, self.mats.dtype.itemsize, self.lights.dtype.itemsize) == \
+            (so.value, sm.value, sl.value), "descriptor layout mismatch"
+        self.v, self.voff, self.f, self.foff = desc.mesh_arrays()
+        self.h = L.orc_scene_create(precision, _p(self.v), _p(self.voff), _p(self.f), _p(self.foff), len(self.voff) - 1,
+                                    _p(self.objs), len(self.objs), _p(self.mats), len(self.mats), _p(self.lights),
+                                    len(self.lights), bins, t_trav, max_leaf)
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def trace(self, rays, any_hit=False, threads=8):
+        rays = np.ascontiguousarray(rays)
+        n = len(rays)
+        obj = np.empty(n, np.int64)
+        prim = np.empty(n, np.int64)
+        t, u, v = (np.empty(n, np.float64) for _ in range(3))
+        p = np.empty((n, 3), np.float64)
+        st = np.zeros(3, np.uint64)
+        fn = lib().orc_scene_trace_f32 if rays.dtype == np.float32 else lib().orc_scene_trace_f64
+        fn(self.h, _p(rays), n, int(any_hit), _p(obj), _p(prim), _p(t), _p(u), _p(v), _p(p), _p(st), threads)
+        return dict(obj=obj, prim=prim, t=t, u=u, v=v, p=p, stats_sum=st)
+
+    def bvh_stats(self, w, h, fov, eye, look, up, near=1e-4, far=1000.0, target=(0.0, 2.0, 1.0), shadow_near=1e-5):
+        a = [np.ascontiguousarray(x, np.float64) for x in (eye, look, up, target)]
+        out = np.zeros(3, np.uint64)
+        st = np.zeros(6, np.uint64)
+        lib().orc_scene_bvh_stats(self.h, w, h, fov, _p(a[0]), _p(a[1]), _p(a[2]), near, far, _p(a[3]), shadow_near,
+                                  _p(out), _p(st))
+        return dict(rays=int(out[0]), primary_hits=int(out[1]), shadow_hits=int(out[2]),
+                    primary=st[:3] / out[0], shadow=st[3:] / max(int(out[1]), 1))
+
+    def render(self, w, h, fov, eye, look, spp, max_depth, seed, near=1e-4, far=1000.0, rows=None, threads=8):
+        y0, y1 = rows if rows is not None else (0, h)
+        film = np.zeros(((y1 - y0) * w, 3), np.float64)
+        rays = ctypes.c_uint64()
+        e = np.ascontiguousarray(eye, np.float64)
+        l = np.ascontiguousarray(look, np.float64)
+        lib().orc_scene_render(self.h, w, h, fov, _p(e), _p(l), spp, max_depth, seed, near, far, y0, y1, _p(film),
+                               ctypes.byref(rays), threads)
+        return film.reshape(y1 - y0, w, 3), rays.value
