@@ -133,6 +133,82 @@ int pbrt_mesh_autotune(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, vo
  * stack exceeded its bound (cannot happen for a tree of the reported depth). */
 int pbrt_mesh_status(pbrt_mesh* mesh, int* status_out);
 
+/* ---- scene: Scene::{new, add_object, add_light, commit} (src/scene.rs:80-115) ----------------
+ * Construction mirrors the reference as data: meshes, materials, objects (a shape + material,
+ * optionally under a rigid affine transform given with its inverse, as Transformed does,
+ * src/objects/transformed.rs) and lights. commit() builds one BVH per mesh and the reference's
+ * median-split object BVH (src/bvh.rs:291-335) and uploads the scene. */
+typedef struct pbrt_scene pbrt_scene;
+
+#define PBRT_SHAPE_MESH 0      /* TriangleMesh (triangle_mesh.rs) */
+#define PBRT_SHAPE_SPHERE 1    /* Sphere(r) at the origin (sphere.rs), param[0] = r */
+#define PBRT_SHAPE_BOX 2       /* AABox(extent) centred at the origin (aabox.rs), param = extent */
+#define PBRT_SHAPE_RECT 3      /* Rectangle(w, h) in y = 0, normal +y (rectangle.rs), param = w, h */
+#define PBRT_MAT_LAMBERTIAN 0  /* materials/lambertian.rs */
+#define PBRT_MAT_DIELECTRIC 1  /* materials/dielectric.rs */
+#define PBRT_MAT_DIFFUSE_LIGHT 2 /* materials/diffuse_light.rs (rgb = emitted radiance) */
+#define PBRT_MAT_GGX 3         /* glossy microfacet reflector (no counterpart in the reference) */
+
+/* Scene-level closest hit: 32 bytes. object = the caller's object index (-1 on a miss); prim = the
+ * mesh triangle (-1 for other shapes); t = distance along the ray (local distance for a transformed
+ * object, as the reference); u, v = triangle barycentrics (0 for other shapes). */
+typedef struct pbrt_scene_hit {
+  float t, u, v;
+  int32_t leaf_slot;
+  int32_t prim, object;
+  int32_t reserved0, reserved1;
+} pbrt_scene_hit;
+
+int pbrt_scene_create(pbrt_ctx* ctx, const pbrt_bvh_params* mesh_params, pbrt_scene** out);
+int pbrt_scene_destroy(pbrt_scene* scene);
+int pbrt_scene_add_mesh(pbrt_scene* scene, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                        uint32_t* mesh_id);
+int pbrt_scene_add_material(pbrt_scene* scene, int kind, const double rgb[3], double ior, double roughness,
+                            uint32_t* material_id);
+/* m16 / minv16: row-major 4x4 local->world and world->local (both NULL for no transform). */
+int pbrt_scene_add_object(pbrt_scene* scene, int shape, int mesh, const double param[3], uint32_t material,
+                          const double* m16, const double* minv16, uint32_t* object_id);
+/* PointLight (lights/point_light.rs): radiant intensity rgb at pos. */
+int pbrt_scene_add_point_light(pbrt_scene* scene, const double pos[3], const double rgb[3]);
+/* Area light on a rectangle object (no counterpart in the reference; follows ideas/area_light.md):
+ * uniform area sampling, one-sided emission along the rectangle's normal, radiance rgb. */
+int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double rgb[3]);
+int pbrt_scene_commit(pbrt_scene* scene);
+int pbrt_scene_info(pbrt_scene* scene, uint32_t* objects, uint32_t* tlas_nodes, uint32_t* tlas_depth,
+                    uint32_t* mesh_depth);
+/* Scene::intersect (scene.rs:118-120) batched: closest hit per ray. Ties at equal distance go to the
+ * smaller (object, triangle) index. */
+int pbrt_scene_trace_closest(pbrt_scene* scene, const pbrt_ray* rays_dev, uint32_t n, pbrt_scene_hit* hits_dev,
+                             void* stream);
+/* Scene::intersect_p (scene.rs:130-132) batched. */
+int pbrt_scene_trace_any(pbrt_scene* scene, const pbrt_ray* rays_dev, uint32_t n, uint8_t* occluded_dev, void* stream);
+int pbrt_scene_status(pbrt_scene* scene, int* status_out);
+
+/* ---- renderer: renderers::mt::render (src/renderers/mt.rs:23) with PathIntegrator::li
+ * (src/integrators/path.rs:48-115) as a device wavefront. Pinhole camera looking from eye at look
+ * (pin_hole.rs), fov in radians; each sample jitters its pixel by two uniforms; every random number
+ * is drawn from a counter keyed by (seed, pixel * spp + sample, bounce, slot). */
+typedef struct pbrt_render_config {
+  uint32_t width, height, spp, max_depth;
+  uint64_t seed;
+  double fov;
+  double eye[3], look[3], up[3];
+  double near, far;
+  uint32_t samples_per_pass; /* samples of every pixel in flight at once (0 = about 4M paths) */
+  uint32_t reserved;
+} pbrt_render_config;
+
+typedef struct pbrt_render_stats {
+  uint64_t rays;   /* every ray traced: camera, extension and shadow */
+  uint64_t paths;  /* width * height * spp */
+  uint64_t passes;
+  double seconds;  /* host wall time of the call */
+} pbrt_render_stats;
+
+/* film_out (host, may be NULL): width*height*3 doubles, the per-pixel mean radiance, accumulated
+ * in FP64 in a fixed sample order. */
+int pbrt_render(pbrt_scene* scene, const pbrt_render_config* config, double* film_out, pbrt_render_stats* stats);
+
 /* ---- synthetic inputs and ray generation (host, f64 as the reference, rays rounded to FP32) ----
  * pinhole camera of src/cameras/pin_hole.rs:17-50 looking from `eye` at `look` (up = +y, fov in
  * radians); writes width*height rays through pixel centres (px + 0.5, py + 0.5), row-major. */

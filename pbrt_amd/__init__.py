@@ -282,3 +282,97 @@ def random_rays(center, radius, n, seed=1, first=0, tmin=1e-4, tmax=1000.0):
     out = np.empty(n, RAY_DTYPE)
     call("pbrt_random_rays", _ptr(c), radius, seed, first, n, tmin, tmax, _ptr(out))
     return out
+
+
+# ---- scene and renderer ----------------------------------------------------------------------------
+SCENE_HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("leaf_slot", "<i4"), ("prim", "<i4"),
+                            ("object", "<i4"), ("r0", "<i4"), ("r1", "<i4")])
+
+
+class Scene:
+    """Scene::new / add_object / add_light / commit (src/scene.rs:80-115) on the device, from a
+    `pbrt_amd.scene.SceneDesc`; `intersect` / `intersect_p` are batched (scene.rs:118-132) and
+    `render` is renderers::mt::render (src/renderers/mt.rs:23) with the path integrator."""
+
+    def __init__(self, ctx, desc, bins=12, max_leaf=4, t_trav=1.0):
+        self.ctx = ctx
+        self.desc = desc
+        h = ctypes.c_void_p()
+        call("pbrt_scene_create", ctx.h, ctypes.byref(BvhParams(bins, max_leaf, t_trav)), ctypes.byref(h))
+        self.h = h
+        out = ctypes.c_uint32()
+        for v, f in desc.meshes:
+            call("pbrt_scene_add_mesh", h, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(out))
+        for kind, rgb, ior, rough in desc.materials:
+            c = np.array(rgb, np.float64)
+            call("pbrt_scene_add_material", h, kind, _ptr(c), ior, rough, ctypes.byref(out))
+        for shape, mesh, mat, xf, param in desc.objects:
+            p = np.array(param, np.float64)
+            if xf is not None:
+                m, mi = (np.ascontiguousarray(a, np.float64) for a in xf.flat())
+                call("pbrt_scene_add_object", h, shape, mesh, _ptr(p), mat, _ptr(m), _ptr(mi), ctypes.byref(out))
+            else:
+                call("pbrt_scene_add_object", h, shape, mesh, _ptr(p), mat, None, None, ctypes.byref(out))
+        for kind, obj, pos, rgb in desc.lights:
+            c = np.array(rgb, np.float64)
+            if kind == 0:
+                call("pbrt_scene_add_point_light", h, _ptr(np.array(pos, np.float64)), _ptr(c))
+            else:
+                call("pbrt_scene_add_area_light", h, obj, _ptr(c))
+        call("pbrt_scene_commit", h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pbrt_scene_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        a, b, c, d = (ctypes.c_uint32() for _ in range(4))
+        call("pbrt_scene_info", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+        return dict(objects=a.value, tlas_nodes=b.value, tlas_depth=c.value, mesh_depth=d.value)
+
+    def intersect(self, rays, n, hits, stream=None):
+        call("pbrt_scene_trace_closest", self.h, rays.ptr, n, hits.ptr, stream)
+
+    def intersect_p(self, rays, n, occluded, stream=None):
+        call("pbrt_scene_trace_any", self.h, rays.ptr, n, occluded.ptr, stream)
+
+    def status(self):
+        s = ctypes.c_int()
+        call("pbrt_scene_status", self.h, ctypes.byref(s))
+        return s.value
+
+    def trace_host(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays)
+        if rays.dtype != RAY_DTYPE:
+            rays = np.ascontiguousarray(rays, np.float32).view(RAY_DTYPE).reshape(-1)
+        n = len(rays)
+        dr = self.ctx.upload(rays)
+        if any_hit:
+            occ = self.ctx.alloc(max(n, 1))
+            self.intersect_p(dr, n, occ)
+            return occ.read(np.uint8, n)
+        dh = self.ctx.alloc(max(n, 1) * 32)
+        self.intersect(dr, n, dh)
+        return dh.read(SCENE_HIT_DTYPE, n)
+
+    def render(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
+               near=1e-4, far=1000.0, samples_per_pass=0, film=True):
+        """Per-pixel mean radiance (height, width, 3) float64 and the render statistics."""
+        c = _native.RenderConfig()
+        c.width, c.height, c.spp, c.max_depth, c.seed, c.fov = width, height, spp, max_depth, seed, fov
+        c.eye[:], c.look[:], c.up[:] = list(eye), list(look), list(up)
+        c.near, c.far, c.samples_per_pass = near, far, samples_per_pass
+        out = np.zeros((height, width, 3), np.float64) if film else None
+        st = _native.RenderStats()
+        call("pbrt_render", self.h, ctypes.byref(c), _ptr(out) if film else None, ctypes.byref(st))
+        return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds)
+
+
+def to_rgb8(film):
+    """Spectrum::gamma_correct + to_rgb (src/spectrum.rs:13-30): sqrt, clamp to [0, 1], x255, truncate."""
+    g = np.sqrt(np.asarray(film, np.float64))
+    return (np.clip(g, 0.0, 1.0) * 255.0).astype(np.uint8)

@@ -31,6 +31,16 @@ class Launch(ctypes.Structure):
     _fields_ = [("block", U32), ("waves_per_cu", U32), ("lds_stack", U32), ("reserved", U32)]
 
 
+class RenderConfig(ctypes.Structure):
+    _fields_ = [("width", U32), ("height", U32), ("spp", U32), ("max_depth", U32), ("seed", U64), ("fov", F64),
+                ("eye", F64 * 3), ("look", F64 * 3), ("up", F64 * 3), ("near", F64), ("far", F64),
+                ("samples_per_pass", U32), ("reserved", U32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("rays", U64), ("paths", U64), ("passes", U64), ("seconds", F64)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "pbrt_last_error": [],
@@ -63,6 +73,19 @@ SIGNATURES = {
     "pbrt_mesh_get_launch": [P, ctypes.POINTER(Launch)],
     "pbrt_mesh_autotune": [P, P, U32, P, I32, U32, ctypes.POINTER(Launch), ctypes.POINTER(F32)],
     "pbrt_mesh_status": [P, ctypes.POINTER(I32)],
+    "pbrt_scene_create": [P, ctypes.POINTER(BvhParams), PP],
+    "pbrt_scene_destroy": [P],
+    "pbrt_scene_add_mesh": [P, P, U32, P, U32, ctypes.POINTER(U32)],
+    "pbrt_scene_add_material": [P, I32, P, F64, F64, ctypes.POINTER(U32)],
+    "pbrt_scene_add_object": [P, I32, I32, P, U32, P, P, ctypes.POINTER(U32)],
+    "pbrt_scene_add_point_light": [P, P, P],
+    "pbrt_scene_add_area_light": [P, U32, P],
+    "pbrt_scene_commit": [P],
+    "pbrt_scene_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)],
+    "pbrt_scene_trace_closest": [P, P, U32, P, P],
+    "pbrt_scene_trace_any": [P, P, U32, P, P],
+    "pbrt_scene_status": [P, ctypes.POINTER(I32)],
+    "pbrt_render": [P, P, P, P],
     "pbrt_pinhole_rays": [U32, U32, F64, P, P, F32, F32, P],
     "pbrt_orbit_rays": [P, U32, U32, U32, F64, F32, F32, P],
     "pbrt_random_rays": [P, F64, U64, U64, U32, F32, F32, P],

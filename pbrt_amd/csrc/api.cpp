@@ -9,6 +9,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "context.hpp"
 #include "traverse.hpp"
 
 namespace pbrt {
@@ -20,12 +21,6 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace pbrt;
 
-struct pbrt_ctx {
-  int device = 0;
-  int cu_count = 0;
-  std::string name;
-  hipStream_t stream = nullptr;
-};
 
 struct pbrt_mesh {
   pbrt_ctx* ctx = nullptr;

@@ -1,0 +1,12 @@
+// The device context shared by every entry point of the C ABI: one device, one stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+struct pbrt_ctx {
+  int device = 0;
+  int cu_count = 0;
+  std::string name;
+  hipStream_t stream = nullptr;
+};

@@ -1,0 +1,983 @@
+// Scene-level traversal and the wavefront path tracer on CDNA4 (gfx950).
+//
+// Scene traversal restates the reference's two levels in FP32:
+//   top level    the median-split object BVH, ordered closest hit (src/bvh.rs:153-230) with the
+//                visitor of Scene::find_nearest (src/scene.rs:186-220) and unordered any hit
+//                (bvh.rs:239-275); box tests narrowed to [near, nearest hit]
+//   objects      Transformed (src/objects/transformed.rs:26-77, src/geom/transform.rs:98-116),
+//                Simple taking the shape's first intersection (src/objects/simple.rs:27-42)
+//   shapes       Sphere (src/shapes/sphere.rs:108-187), AABox (src/shapes/aabox.rs:66-193),
+//                Rectangle (src/shapes/rectangle.rs:26-58), TriangleMesh's BVH (nested, sharing the
+//                lane's stack) and shading quantities (triangle_mesh.rs:73-132, 219-258)
+// Ties between objects at equal distance go to the smaller (object, triangle) index.
+//
+// The path tracer is PathIntegrator::li (src/integrators/path.rs:48-115) turned into a wavefront:
+// generate (pinhole camera, src/cameras/pin_hole.rs, pixel jitter of renderers/mt.rs:123-127) ->
+// per bounce: extend (closest hit) -> shade (emission after specular bounces, next-event estimation
+// with one uniformly chosen light, material scatter) -> shadow (any hit) -> accumulate (per-pixel
+// FP64 sums in a fixed sample order). Every random number is drawn from a counter keyed by (seed,
+// pixel * spp + sample, bounce, slot), so the result does not depend on scheduling.
+#pragma clang fp contract(off)
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "device_common.hpp"
+#include "rng.hpp"
+#include "scene.hpp"
+
+namespace pbrt {
+namespace {
+using namespace dev;
+
+constexpr float kPi = (float)3.14159265358979323846;
+
+// ---- small FP32 vector algebra in the reference's operation order (src/geom/vector3.rs) ---------
+struct V {
+  float x, y, z;
+};
+__device__ __forceinline__ V vadd(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V vmul(V a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V vcross(V u, V v) {
+  return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+__device__ __forceinline__ V vnorm(V v) {
+  const float n = vdot(v, v);
+  const float inv = 1.0f / sqrtf(n);
+  return vmul(v, inv);
+}
+__device__ __forceinline__ float vlen(V v) { return sqrtf(vdot(v, v)); }
+
+// matrix4.rs:187-213 on the first three rows (affine: w = 1)
+__device__ __forceinline__ V xpoint(const float* M, V v) {
+  return {M[0] * v.x + M[1] * v.y + M[2] * v.z + M[3], M[4] * v.x + M[5] * v.y + M[6] * v.z + M[7],
+          M[8] * v.x + M[9] * v.y + M[10] * v.z + M[11]};
+}
+__device__ __forceinline__ V xdir(const float* M, V v) {
+  return {M[0] * v.x + M[1] * v.y + M[2] * v.z, M[4] * v.x + M[5] * v.y + M[6] * v.z,
+          M[8] * v.x + M[9] * v.y + M[10] * v.z};
+}
+__device__ __forceinline__ V xnormal(const float* M, V v) {
+  return {M[0] * v.x + M[4] * v.y + M[8] * v.z, M[1] * v.x + M[5] * v.y + M[9] * v.z,
+          M[2] * v.x + M[6] * v.y + M[10] * v.z};
+}
+
+__device__ __forceinline__ RayState ray_from(V o, V d, float near, float far) {
+  return make_ray(make_float4(o.x, o.y, o.z, near), make_float4(d.x, d.y, d.z, far));
+}
+
+// ---- analytic shapes: distance of the first intersection the shape records -------------------------
+// sphere.rs:108-144
+__device__ __forceinline__ bool sphere_t(V o, V d, float r, float near, float far, float* t_out) {
+  const V l = vmul(o, -1.0f);
+  const float tca = vdot(l, d);
+  if (tca < 0.0f) return false;
+  const float r2 = r * r;
+  const float d2 = vdot(l, l) - tca * tca;
+  if (d2 > r2) return false;
+  const float thc = sqrtf(r2 - d2);
+  const float t0 = tca - thc, t1 = tca + thc;
+  const float tmin = fminf(t0, t1), tmax = fmaxf(t0, t1);
+  if (tmax < near || tmin > far) return false;
+  if (tmin > near) {
+    *t_out = tmin;
+    return true;
+  }
+  if (tmax > near && tmax < far) {
+    *t_out = tmax;
+    return true;
+  }
+  return false;
+}
+// aabox.rs:66-193; *face: index into the normal table (x-, x+, y-, y+, z-, z+)
+__device__ __forceinline__ bool box_t(V o, V d, const float* ext, float near, float far, float* t_out, int* face) {
+  const V mn{0.0f - ext[0] * 0.5f, 0.0f - ext[1] * 0.5f, 0.0f - ext[2] * 0.5f};
+  const V mx{ext[0] * 0.5f, ext[1] * 0.5f, ext[2] * 0.5f};
+  const V inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  float tmin, tmax;
+  int nmin, nmax;
+  if (inv.x >= 0.0f) { tmin = (mn.x - o.x) * inv.x; tmax = (mx.x - o.x) * inv.x; nmin = 0; nmax = 1; }
+  else { tmin = (mx.x - o.x) * inv.x; tmax = (mn.x - o.x) * inv.x; nmin = 1; nmax = 0; }
+  float tymin, tymax;
+  int nymin, nymax;
+  if (inv.y >= 0.0f) { tymin = (mn.y - o.y) * inv.y; tymax = (mx.y - o.y) * inv.y; nymin = 2; nymax = 3; }
+  else { tymin = (mx.y - o.y) * inv.y; tymax = (mn.y - o.y) * inv.y; nymin = 3; nymax = 2; }
+  if (tmin > tymax || tymin > tmax) return false;
+  if (tymin > tmin) { tmin = tymin; nmin = nymin; }
+  if (tymax < tmax) { tmax = tymax; nmax = nymax; }
+  float tzmin, tzmax;
+  int nzmin, nzmax;
+  if (inv.z >= 0.0f) { tzmin = (mn.z - o.z) * inv.z; tzmax = (mx.z - o.z) * inv.z; nzmin = 4; nzmax = 5; }
+  else { tzmin = (mx.z - o.z) * inv.z; tzmax = (mn.z - o.z) * inv.z; nzmin = 5; nzmax = 4; }
+  if (tmin > tzmax || tzmin > tmax) return false;
+  if (tzmin > tmin) { tmin = tzmin; nmin = nzmin; }
+  if (tzmax < tmax) { tmax = tzmax; nmax = nzmax; }
+  if (tmax < near || tmin > far) return false;
+  if (tmin > near) {
+    *t_out = tmin;
+    *face = nmin;
+    return true;
+  }
+  if (tmax < far) {
+    *t_out = tmax;
+    *face = nmax;
+    return true;
+  }
+  return false;
+}
+// rectangle.rs:26-58
+__device__ __forceinline__ bool rect_t(V o, V d, float w, float h, float near, float far, float* t_out) {
+  if (d.y == 0.0f) return false;
+  const float t = (o.y * -1.0f) / d.y;
+  if (t < near || t > far) return false;
+  V p = vadd(o, vmul(d, t));
+  if (fabsf(p.x) > w / 2.0f || fabsf(p.z) > h / 2.0f) return false;
+  *t_out = t;
+  return true;
+}
+
+// A pushed traversal entry: node reference and entry distance.
+template <int BLOCK, int DEPTH>
+__device__ __forceinline__ bool push_or_flag(Stack<BLOCK, DEPTH>& st, int32_t ref, float d, int* status) {
+  if (st.push(ref, d)) return true;
+  atomicOr(status, 1);
+  return false;
+}
+
+// BVHTree::query on an object's mesh (nested on the lane's stack above `base`): closest (t, prim)
+// over [near, far] with the order-independent tie rule.
+template <int BLOCK, int DEPTH>
+__device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayState& r, float far,
+                             Stack<BLOCK, DEPTH>& st, int* status, float* t_out, float* u_out, float* v_out,
+                             uint32_t* prim_out) {
+  float entry;
+  if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], far,
+                &entry))
+    return false;
+  const int base = st.sp;
+  const DevNode* nodes = S.blas + ob.node_base;
+  const DevTri* tris = S.tris + ob.tri_base;
+  float min_t = FLT_MAX, cur_far = far;
+  uint32_t best = 0xffffffffu;
+  int32_t cur = ob.root_ref;
+  bool live = true;
+  while (live) {
+    bool pop_now = false;
+    if (cur >= 0) {
+      const DevNode* nd = nodes + cur;
+      const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
+      const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
+      const float4 z = *reinterpret_cast<const float4*>(nd->z01);
+      const int2 c = *reinterpret_cast<const int2*>(nd->ref);
+      float dl, dr;
+      const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, cur_far, &dl);
+      const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, cur_far, &dr);
+      if (hl && hr) {
+        const bool right_first = dr < dl;
+        if (!push_or_flag(st, right_first ? c.x : c.y, right_first ? dl : dr, status)) break;
+        cur = right_first ? c.y : c.x;
+      } else if (hl) {
+        cur = c.x;
+      } else if (hr) {
+        cur = c.y;
+      } else {
+        pop_now = true;
+      }
+    }
+    if (!pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        float t, u, v;
+        uint32_t id;
+        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= far &&
+            (t < min_t || (t == min_t && id < best))) {
+          min_t = t;
+          best = id;
+          *t_out = t;
+          *u_out = u;
+          *v_out = v;
+          cur_far = fminf(far, min_t);
+        }
+      }
+      pop_now = true;
+    }
+    if (pop_now) {
+      live = false;
+      while (st.sp > base) {
+        const uint2 e = st.pop();
+        if (__uint_as_float(e.y) > min_t) continue;
+        cur = (int32_t)e.x;
+        live = true;
+        break;
+      }
+    }
+  }
+  st.sp = base;
+  *prim_out = best;
+  return best != 0xffffffffu;
+}
+
+template <int BLOCK, int DEPTH>
+__device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState& r, Stack<BLOCK, DEPTH>& st,
+                         int* status) {
+  float entry;
+  if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], r.far,
+                &entry))
+    return false;
+  const int base = st.sp;
+  const DevNode* nodes = S.blas + ob.node_base;
+  const DevTri* tris = S.tris + ob.tri_base;
+  int32_t cur = ob.root_ref;
+  bool hit = false, live = true;
+  while (live && !hit) {
+    bool pop_now = false;
+    if (cur >= 0) {
+      const DevNode* nd = nodes + cur;
+      const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
+      const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
+      const float4 z = *reinterpret_cast<const float4*>(nd->z01);
+      const int2 c = *reinterpret_cast<const int2*>(nd->ref);
+      float dl, dr;
+      const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, r.far, &dl);
+      const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, r.far, &dr);
+      if (hl && hr) {
+        if (!push_or_flag(st, c.x, 0.f, status)) break;
+        cur = c.y;
+      } else if (hr) {
+        cur = c.y;
+      } else if (hl) {
+        cur = c.x;
+      } else {
+        pop_now = true;
+      }
+    }
+    if (!pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        float t, u, v;
+        uint32_t id;
+        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+          hit = true;
+          break;
+        }
+      }
+      pop_now = true;
+    }
+    if (pop_now) {
+      live = st.sp > base;
+      if (live) cur = (int32_t)st.pop().x;
+    }
+  }
+  st.sp = base;
+  return hit;
+}
+
+// Object::intersect for leaf-order object k on the world ray; far = the nearest hit so far.
+template <int BLOCK, int DEPTH>
+__device__ bool object_closest(const SceneView& S, uint32_t k, V wo, V wd, float near, float far,
+                               Stack<BLOCK, DEPTH>& st, int* status, float* t, float* u, float* v, int32_t* prim,
+                               int32_t* face) {
+  const DevObject& ob = S.objs[k];
+  V o = wo, d = wd;
+  if (ob.has_xform) {  // transform.rs:98-103: the local direction is renormalised
+    o = xpoint(ob.minv, wo);
+    d = vnorm(xdir(ob.minv, wd));
+  }
+  *prim = -1;
+  *u = 0.f;
+  *v = 0.f;
+  switch (ob.shape) {
+    case kShapeSphere: return sphere_t(o, d, ob.param[0], near, far, t);
+    case kShapeBox: return box_t(o, d, ob.param, near, far, t, face);
+    case kShapeRect: return rect_t(o, d, ob.param[0], ob.param[1], near, far, t);
+    default: {
+      const RayState r = ray_from(o, d, near, far);
+      uint32_t id;
+      if (!mesh_closest(S, ob, r, far, st, status, t, u, v, &id)) return false;
+      *prim = (int32_t)id;
+      return true;
+    }
+  }
+}
+
+template <int BLOCK, int DEPTH>
+__device__ bool object_any(const SceneView& S, uint32_t k, V wo, V wd, float near, float far, Stack<BLOCK, DEPTH>& st,
+                           int* status) {
+  const DevObject& ob = S.objs[k];
+  V o = wo, d = wd;
+  if (ob.has_xform) {
+    o = xpoint(ob.minv, wo);
+    d = vnorm(xdir(ob.minv, wd));
+  }
+  float t;
+  switch (ob.shape) {
+    case kShapeSphere: return sphere_t(o, d, ob.param[0], near, far, &t);
+    case kShapeBox: {
+      int face;
+      return box_t(o, d, ob.param, near, far, &t, &face);
+    }
+    case kShapeRect: return rect_t(o, d, ob.param[0], ob.param[1], near, far, &t);
+    default: return mesh_any(S, ob, ray_from(o, d, near, far), st, status);
+  }
+}
+
+// Closest hit over the scene (bvh.rs:153-230 + scene.rs:186-220).
+template <int BLOCK, int DEPTH>
+__device__ bool scene_closest(const SceneView& S, V o, V d, float near, float far_in, Stack<BLOCK, DEPTH>& st,
+                              int* status, SceneHitRec* out) {
+  out->obj = -1;
+  out->prim = -1;
+  out->orig = -1;
+  if (!S.has_tlas) return false;
+  const float far = fminf(far_in, FLT_MAX);
+  const RayState r = ray_from(o, d, near, far);
+  float min_t = far;
+  int32_t best_orig = INT_MAX, best_prim = INT_MAX;
+  bool found = false;
+  float entry;
+  st.sp = 0;
+  if (!box_test(r, S.root.lo[0], S.root.hi[0], S.root.lo[1], S.root.hi[1], S.root.lo[2], S.root.hi[2], min_t, &entry))
+    return false;
+  int32_t cur = S.root.ref;
+  bool live = true;
+  while (live) {
+    bool pop_now = false;
+    if (cur >= 0) {
+      const DevNode* nd = S.tlas + cur;
+      const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
+      const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
+      const float4 z = *reinterpret_cast<const float4*>(nd->z01);
+      const int2 c = *reinterpret_cast<const int2*>(nd->ref);
+      float dl, dr;
+      const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, min_t, &dl);
+      const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, min_t, &dr);
+      if (hl && hr) {
+        const bool right_first = dr < dl;
+        if (!push_or_flag(st, right_first ? c.x : c.y, right_first ? dl : dr, status)) break;
+        cur = right_first ? c.y : c.x;
+      } else if (hl) {
+        cur = c.x;
+      } else if (hr) {
+        cur = c.y;
+      } else {
+        pop_now = true;
+      }
+    }
+    if (!pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        float t, u, v;
+        int32_t prim, face = 0;
+        if (!object_closest(S, k, o, d, near, min_t, st, status, &t, &u, &v, &prim, &face)) continue;
+        const int32_t orig = S.objs[k].orig;
+        const bool take = t < min_t || (t == min_t && (orig < best_orig || (orig == best_orig && prim < best_prim)));
+        if (!take) continue;
+        min_t = t;
+        best_orig = orig;
+        best_prim = prim;
+        found = true;
+        out->t = t;
+        out->u = u;
+        out->v = v;
+        out->obj = (int32_t)k;
+        out->prim = prim;
+        out->orig = orig;
+        out->face = face;
+      }
+      pop_now = true;
+    }
+    if (pop_now) {
+      live = false;
+      while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (__uint_as_float(e.y) > min_t) continue;
+        cur = (int32_t)e.x;
+        live = true;
+        break;
+      }
+    }
+  }
+  return found;
+}
+
+// Any hit over the scene (bvh.rs:239-275 + scene.rs:157-168).
+template <int BLOCK, int DEPTH>
+__device__ bool scene_any(const SceneView& S, V o, V d, float near, float far_in, Stack<BLOCK, DEPTH>& st,
+                          int* status) {
+  if (!S.has_tlas) return false;
+  const float far = fminf(far_in, FLT_MAX);
+  const RayState r = ray_from(o, d, near, far);
+  float entry;
+  st.sp = 0;
+  if (!box_test(r, S.root.lo[0], S.root.hi[0], S.root.lo[1], S.root.hi[1], S.root.lo[2], S.root.hi[2], far, &entry))
+    return false;
+  int32_t cur = S.root.ref;
+  for (;;) {
+    bool pop_now = false;
+    if (cur >= 0) {
+      const DevNode* nd = S.tlas + cur;
+      const float4 a = *reinterpret_cast<const float4*>(nd->xy0);
+      const float4 b = *reinterpret_cast<const float4*>(nd->xy1);
+      const float4 z = *reinterpret_cast<const float4*>(nd->z01);
+      const int2 c = *reinterpret_cast<const int2*>(nd->ref);
+      float dl, dr;
+      const bool hl = box_test(r, a.x, a.y, a.z, a.w, z.x, z.y, far, &dl);
+      const bool hr = box_test(r, b.x, b.y, b.z, b.w, z.z, z.w, far, &dr);
+      if (hl && hr) {
+        if (!push_or_flag(st, c.y, 0.f, status)) return false;
+        cur = c.x;  // the reference pushes right then left: the left child is examined first
+      } else if (hl) {
+        cur = c.x;
+      } else if (hr) {
+        cur = c.y;
+      } else {
+        pop_now = true;
+      }
+    }
+    if (!pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k)
+        if (object_any(S, k, o, d, near, far, st, status)) return true;
+      pop_now = true;
+    }
+    if (pop_now) {
+      if (st.sp == 0) return false;
+      cur = (int32_t)st.pop().x;
+    }
+  }
+}
+
+template <int DEPTH>
+__global__ __launch_bounds__(kSceneBlock) void k_scene_trace(SceneView S, const float4* __restrict__ rays,
+                                                             SceneHitRec* __restrict__ hits,
+                                                             uint8_t* __restrict__ occluded, uint32_t n,
+                                                             uint2* __restrict__ spill, uint32_t spill_cap,
+                                                             int* __restrict__ status, int any_hit) {
+  __shared__ uint2 lds[DEPTH][kSceneBlock];
+  Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
+                               blockIdx.x * kSceneBlock + threadIdx.x, 0};
+  for (uint32_t i = st.gtid; i < n; i += gridDim.x * kSceneBlock) {
+    const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+    const V o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+    if (any_hit) {
+      occluded[i] = scene_any(S, o, d, a.w, b.w, st, status) ? 1 : 0;
+    } else {
+      SceneHitRec h;
+      scene_closest(S, o, d, a.w, b.w, st, status, &h);
+      hits[i] = h;
+    }
+  }
+}
+
+// ---- shading ---------------------------------------------------------------------------------------
+struct Isect {
+  V p, n, wo, dpdu;
+  float d;
+};
+
+// The world-space intersection record of a hit (triangle_mesh.rs:73-132, sphere.rs:152-187,
+// aabox.rs:31-60 + 155-190, rectangle.rs:45-54, transform.rs:105-116).
+__device__ Isect interaction(const SceneView& S, const SceneHitRec& h, V wo_ray, V wd) {
+  const DevObject& ob = S.objs[h.obj];
+  V o = wo_ray, d = wd;
+  if (ob.has_xform) {
+    o = xpoint(ob.minv, wo_ray);
+    d = vnorm(xdir(ob.minv, wd));
+  }
+  Isect is;
+  is.d = h.t;
+  is.wo = vmul(d, -1.0f);
+  const V hit = vadd(o, vmul(d, h.t));
+  switch (ob.shape) {
+    case kShapeSphere: {
+      is.p = hit;
+      is.n = vnorm(hit);
+      is.dpdu = vmul(V{-hit.y, hit.x, 0.0f}, 2.0f * kPi);
+      break;
+    }
+    case kShapeBox: {
+      const int face = h.face;
+      const float NT[6][6] = {{-1, 0, 0, 0, 0, -1}, {1, 0, 0, 0, 0, 1}, {0, -1, 0, -1, 0, 0},
+                              {0, 1, 0, 1, 0, 0},   {0, 0, -1, 0, -1, 0}, {0, 0, 1, 0, 1, 0}};
+      is.p = hit;
+      is.n = V{NT[face][0], NT[face][1], NT[face][2]};
+      is.dpdu = V{NT[face][3], NT[face][4], NT[face][5]};
+      break;
+    }
+    case kShapeRect: {
+      V p = hit;
+      p.y = 0.0f;
+      is.p = p;
+      is.n = V{0.0f, 1.0f, 0.0f};
+      is.dpdu = V{1.0f, 0.0f, 0.0f};
+      break;
+    }
+    default: {
+      const float* vb = S.verts + 3 * (size_t)ob.vert_base;
+      const uint32_t* ib = S.idx + 3 * (size_t)(ob.idx_base + (uint32_t)h.prim);
+      const V p0{vb[3 * ib[0]], vb[3 * ib[0] + 1], vb[3 * ib[0] + 2]};
+      const V p1{vb[3 * ib[1]], vb[3 * ib[1] + 1], vb[3 * ib[1] + 2]};
+      const V p2{vb[3 * ib[2]], vb[3 * ib[2] + 1], vb[3 * ib[2] + 2]};
+      // compute_texture_derivatives with the default UVs (0,0), (1,0), (1,1): regular case, det = -1
+      const float duv02x = -1.0f, duv02y = -1.0f, duv12x = 0.0f, duv12y = -1.0f;
+      const V dp02 = vsub(p0, p2), dp12 = vsub(p1, p2);
+      const float det = duv12x * duv02y - duv12y * duv02x;
+      const float inv_det = 1.0f / det;
+      const V dpdu = vmul(vsub(vmul(dp12, duv02y), vmul(dp02, duv12y)), inv_det);
+      const V dpdv = vmul(vsub(vmul(dp02, duv12x), vmul(dp12, duv02x)), inv_det);
+      is.p = hit;
+      is.n = vnorm(vcross(dpdv, dpdu));
+      is.dpdu = dpdu;
+      break;
+    }
+  }
+  if (ob.has_xform) {
+    is.n = xnormal(ob.minv, is.n);
+    is.p = xpoint(ob.m, is.p);
+    is.wo = xdir(ob.m, is.wo);
+    is.dpdu = xdir(ob.m, is.dpdu);
+  }
+  return is;
+}
+
+// intersectable.rs:92-144
+__device__ __forceinline__ V to_local(const Isect& i, V v) {
+  const V ss = vnorm(i.dpdu);
+  const V ts = vcross(i.n, ss);
+  return {vdot(ss, v), vdot(ts, v), vdot(i.n, v)};
+}
+__device__ __forceinline__ V to_world(const Isect& i, V v) {
+  const V ns = i.n, ss = vnorm(i.dpdu);
+  const V ts = vcross(ns, ss);
+  return {ss.x * v.x + ts.x * v.y + ns.x * v.z, ss.y * v.x + ts.y * v.y + ns.y * v.z,
+          ss.z * v.x + ts.z * v.y + ns.z * v.z};
+}
+
+__device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
+  return (float)to_unit(hash4(seed, path, bounce, slot));
+}
+enum : uint32_t { kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7 };
+
+// GGX microfacet reflection with Smith masking and Schlick Fresnel (albedo as F0)
+__device__ __forceinline__ float ggx_d(float cos_h, float a) {
+  const float a2 = a * a;
+  const float c2 = cos_h * cos_h;
+  const float den = c2 * (a2 - 1.0f) + 1.0f;
+  return a2 / (kPi * den * den);
+}
+__device__ __forceinline__ float ggx_g1(float cos_v, float a) {
+  const float c = fabsf(cos_v);
+  const float a2 = a * a;
+  return 2.0f * c / (c + sqrtf(a2 + (1.0f - a2) * c * c));
+}
+__device__ __forceinline__ V ggx_f(const DevMaterial& m, const Isect& is, V wo_w, V wi_w) {
+  const V wo = to_local(is, wo_w), wi = to_local(is, wi_w);
+  if (wo.z <= 0.0f || wi.z <= 0.0f) return {0.f, 0.f, 0.f};
+  const V h = vnorm(vadd(wo, wi));
+  const float a = m.roughness;
+  const float D = ggx_d(h.z, a);
+  const float G = ggx_g1(wo.z, a) * ggx_g1(wi.z, a);
+  const float c = 1.0f - vdot(wo, h);
+  const float c5 = c * c * c * c * c;
+  const float s = D * G / (4.0f * wo.z * wi.z);
+  return {(m.rgb[0] + (1.0f - m.rgb[0]) * c5) * s, (m.rgb[1] + (1.0f - m.rgb[1]) * c5) * s,
+          (m.rgb[2] + (1.0f - m.rgb[2]) * c5) * s};
+}
+
+// Material::f (lambertian.rs:41-52)
+__device__ __forceinline__ V mat_f(const DevMaterial& m, const Isect& is, V wo_w, V wi_w) {
+  if (m.kind == kMatLambertian) {
+    const V wi = to_local(is, wi_w), wo = to_local(is, wo_w);
+    if (wo.z * wi.z > 0.0f) return {m.rgb[0] / kPi, m.rgb[1] / kPi, m.rgb[2] / kPi};
+    return {0.f, 0.f, 0.f};
+  }
+  if (m.kind == kMatGGX) return ggx_f(m, is, wo_w, wi_w);
+  return {0.f, 0.f, 0.f};
+}
+
+struct ScatterOut {
+  V att, o, d;
+  float pdf;
+  bool specular;
+};
+
+// Material::scatter: lambertian.rs:24-38 (+ pdfs/cosine.rs:14-24), dielectric.rs:31-182.
+__device__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, uint64_t path, uint32_t bounce,
+                        ScatterOut* s) {
+  if (m.kind == kMatLambertian) {
+    const float two_pi = 2.0f * kPi;
+    const float phi = two_pi * urand(seed, path, bounce, kSlotS0);
+    const float r2 = urand(seed, path, bounce, kSlotS1);
+    const float r2s = sqrtf(r2);
+    const V lwi{cosf(phi) * r2s, sinf(phi) * r2s, sqrtf(1.0f - r2)};
+    const V wi = to_world(is, lwi);
+    s->o = vadd(is.p, vmul(is.n, 0.001f));
+    s->d = vnorm(wi);
+    s->att = {m.rgb[0] / kPi, m.rgb[1] / kPi, m.rgb[2] / kPi};
+    s->pdf = fabsf(lwi.z / kPi);
+    s->specular = false;
+    return true;
+  }
+  if (m.kind == kMatDielectric) {
+    const V lwo = vnorm(to_local(is, is.wo));
+    const V lrefl{-lwo.x, -lwo.y, lwo.z};
+    V lnorm, wnorm;
+    float ni, nt;
+    if (lwo.z <= 0.0f) {
+      lnorm = {0.f, 0.f, -1.f};
+      wnorm = vmul(is.n, -1.0f);
+      ni = m.ior;
+      nt = 1.0f;
+    } else {
+      lnorm = {0.f, 0.f, 1.f};
+      wnorm = is.n;
+      ni = 1.0f;
+      nt = m.ior;
+    }
+    const float eta = ni / nt;
+    // refract (dielectric.rs:129-157)
+    const float ci = vdot(lwo, lnorm);
+    const float s2i = 1.0f - ci * ci;
+    const float s2t = eta * eta * s2i;
+    const float disc = 1.0f - s2t;
+    V ldir, org;
+    if (disc > 0.0f) {
+      const float ct = sqrtf(disc);
+      const V lrefr = vnorm(vadd(vmul(vmul(lwo, -1.0f), eta), vmul(lnorm, eta * ci - ct)));
+      // fresnel (dielectric.rs:168-182)
+      const float e = ni / nt;
+      const float fci = lwo.z * lnorm.z;
+      const float fct = sqrtf(1.0f - e * e * (1.0f - fci * fci));
+      const float ro = (ni * fci - nt * fct) / (ni * fci + nt * fct);
+      const float rp = (nt * fci - ni * fct) / (nt * fci + ni * fct);
+      const float refl = (ro * ro + rp * rp) * 0.5f;
+      if (urand(seed, path, bounce, kSlotCoin) < refl) {
+        ldir = lrefl;
+        org = vadd(is.p, vmul(wnorm, 0.001f));
+      } else {
+        ldir = lrefr;
+        org = vadd(is.p, vmul(wnorm, -0.001f));
+      }
+    } else {
+      ldir = lrefl;
+      org = vadd(is.p, vmul(wnorm, 0.001f));
+    }
+    s->o = org;
+    s->d = vnorm(to_world(is, ldir));
+    const float c = fabsf(ldir.z);
+    s->att = {m.rgb[0] / c, m.rgb[1] / c, m.rgb[2] / c};
+    s->pdf = 1.0f;
+    s->specular = true;
+    return true;
+  }
+  if (m.kind == kMatGGX) {
+    const V wo = to_local(is, is.wo);
+    if (wo.z <= 0.0f) return false;
+    const float a = m.roughness;
+    const float u1 = urand(seed, path, bounce, kSlotS0), u2 = urand(seed, path, bounce, kSlotS1);
+    const float phi = 2.0f * kPi * u1;
+    const float cos_t = sqrtf((1.0f - u2) / (1.0f + (a * a - 1.0f) * u2));
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    const V h{sin_t * cosf(phi), sin_t * sinf(phi), cos_t};
+    const float woh = vdot(wo, h);
+    const V wi = vsub(vmul(h, 2.0f * woh), wo);
+    if (wi.z <= 0.0f) return false;
+    s->pdf = ggx_d(h.z, a) * h.z / (4.0f * woh);
+    s->att = ggx_f(m, is, is.wo, to_world(is, wi));
+    s->o = vadd(is.p, vmul(is.n, 0.001f));
+    s->d = vnorm(to_world(is, wi));
+    s->specular = false;
+    return true;
+  }
+  return false;
+}
+
+struct LiOut {
+  V li, wi, so, sd;
+  float pdf, dist;
+};
+
+// Light::sample_li: point_light.rs:23-35 + lights.rs:38-46; area light: uniform point on the
+// rectangle, one-sided emission, solid-angle pdf d^2 / (A cos) (ideas/area_light.md §3).
+__device__ bool sample_li(const SceneView& S, const DevLight& L, const Isect& is, uint64_t seed, uint64_t path,
+                          uint32_t bounce, LiOut* s) {
+  if (L.kind == kLightPoint) {
+    const V pos{L.pos[0], L.pos[1], L.pos[2]};
+    const V wi = vsub(pos, is.p);
+    const float d2 = vdot(wi, wi);
+    s->li = {L.rgb[0] / d2, L.rgb[1] / d2, L.rgb[2] / d2};
+    s->wi = vnorm(wi);
+    s->pdf = 1.0f;
+    s->so = is.p;
+    s->sd = vnorm(vnorm(vsub(pos, is.p)));
+    s->dist = vlen(vsub(pos, is.p));
+    return true;
+  }
+  const DevObject& o = S.objs[L.object];
+  const float w = o.param[0], h = o.param[1];
+  const float u1 = urand(seed, path, bounce, kSlotLu), u2 = urand(seed, path, bounce, kSlotLv);
+  const V pl_local{(u1 - 0.5f) * w, 0.0f, (u2 - 0.5f) * h};
+  const V pl = o.has_xform ? xpoint(o.m, pl_local) : pl_local;
+  const V nl = o.has_xform ? vnorm(xnormal(o.minv, V{0.f, 1.f, 0.f})) : V{0.f, 1.f, 0.f};
+  const V dv = vsub(pl, is.p);
+  const float dist2 = vdot(dv, dv);
+  const float dist = sqrtf(dist2);
+  const V wi = vmul(dv, 1.0f / dist);
+  const float cos_l = -vdot(nl, wi);
+  if (!(cos_l > 0.0f)) return false;
+  const float sx = o.has_xform ? vlen(xdir(o.m, V{1.f, 0.f, 0.f})) : 1.0f;
+  const float sz = o.has_xform ? vlen(xdir(o.m, V{0.f, 0.f, 1.f})) : 1.0f;
+  const float area = w * sx * h * sz;
+  s->li = {L.rgb[0], L.rgb[1], L.rgb[2]};
+  s->wi = wi;
+  s->pdf = dist2 / (area * cos_l);
+  s->so = is.p;
+  s->sd = wi;
+  s->dist = dist - 1e-4f;
+  return true;
+}
+
+// ---- wavefront kernels -------------------------------------------------------------------------------
+__device__ __forceinline__ void cam_point(const double* M, double x, double y, double z, double* r) {
+  double rx = M[0] * x + M[1] * y + M[2] * z + M[3];
+  double ry = M[4] * x + M[5] * y + M[6] * z + M[7];
+  double rz = M[8] * x + M[9] * y + M[10] * z + M[11];
+  const double w = M[12] * x + M[13] * y + M[14] * z + M[15];
+  if (w != 1.0) {
+    const double iw = 1.0 / w;
+    rx = rx * iw;
+    ry = ry * iw;
+    rz = rz * iw;
+  }
+  r[0] = rx;
+  r[1] = ry;
+  r[2] = rz;
+}
+__device__ __forceinline__ void dnorm(double* v) {
+  const double n = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const double inv = 1.0 / sqrt(n);
+  v[0] = v[0] * inv;
+  v[1] = v[1] * inv;
+  v[2] = v[2] * inv;
+}
+
+// One camera path per (pixel, sample of this pass): PinHoleCamera::get_ray (pin_hole.rs:42-50) in
+// f64, rounded to FP32, through the pixel jittered by two uniforms (renderers/mt.rs:123-127).
+__global__ void k_generate(RenderParams p, PathBuffers b) {
+  const uint32_t npix = p.width * p.height;
+  const uint32_t n = npix * p.pass_samples;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t pixel = j % npix, sample = p.pass_first_sample + j / npix;
+    const uint64_t path = (uint64_t)pixel * p.spp + sample;
+    const double jx = to_unit(hash4(p.seed, path, 0, kSlotJx)), jy = to_unit(hash4(p.seed, path, 0, kSlotJy));
+    double cv[3], o[3], d[3];
+    cam_point(p.r2s, (double)(pixel % p.width) + jx, (double)(pixel / p.width) + jy, 0.0, cv);
+    dnorm(cv);
+    cam_point(p.c2w, 0.0, 0.0, 0.0, o);
+    d[0] = p.c2w[0] * cv[0] + p.c2w[1] * cv[1] + p.c2w[2] * cv[2];
+    d[1] = p.c2w[4] * cv[0] + p.c2w[5] * cv[1] + p.c2w[6] * cv[2];
+    d[2] = p.c2w[8] * cv[0] + p.c2w[9] * cv[1] + p.c2w[10] * cv[2];
+    dnorm(d);
+    b.ray_o[j] = make_float4((float)o[0], (float)o[1], (float)o[2], p.near);
+    b.ray_d[j] = make_float4((float)d[0], (float)d[1], (float)d[2], p.far);
+    b.beta[j] = make_float4(1.f, 1.f, 1.f, 1.f);
+    b.L[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    b.info[j] = make_uint4(pixel, sample, 0u, 1u);
+    b.queue[0][j] = j;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.counts[0] = n;
+    b.counts[1] = 0;
+    b.counts[2] = 0;
+  }
+}
+
+__global__ void k_reset(PathBuffers b, int q) {
+  if (threadIdx.x == 0) {
+    b.counts[q ^ 1] = 0;
+    b.counts[2] = 0;
+  }
+}
+
+// counts[4..5] hold a 64-bit count of every ray traced (extension and shadow)
+__device__ __forceinline__ void count_rays(PathBuffers& b, uint32_t n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(reinterpret_cast<unsigned long long*>(b.counts + 4), (unsigned long long)n);
+}
+
+template <int DEPTH>
+__global__ __launch_bounds__(kSceneBlock) void k_extend(SceneView S, PathBuffers b, int q, uint2* __restrict__ spill,
+                                                        uint32_t spill_cap, int* __restrict__ status) {
+  __shared__ uint2 lds[DEPTH][kSceneBlock];
+  Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
+                               blockIdx.x * kSceneBlock + threadIdx.x, 0};
+  const uint32_t n = b.counts[q];
+  count_rays(b, n);
+  for (uint32_t i = st.gtid; i < n; i += gridDim.x * kSceneBlock) {
+    const uint32_t j = b.queue[q][i];
+    const float4 o = b.ray_o[j], d = b.ray_d[j];
+    SceneHitRec h;
+    scene_closest(S, V{o.x, o.y, o.z}, V{d.x, d.y, d.z}, o.w, d.w, st, status, &h);
+    b.hit[j] = h;
+  }
+}
+
+// One path vertex (path.rs:55-112): emission after a specular bounce, the bounce cut, next-event
+// estimation (its shadow ray queued with its contribution), then scattering into the next queue.
+__global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
+  const uint32_t n = b.counts[q];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t j = b.queue[q][i];
+    const SceneHitRec h = b.hit[j];
+    if (h.obj < 0) continue;  // no infinite light in these scenes: the path ends
+    uint4 info = b.info[j];
+    const uint64_t path = (uint64_t)info.x * p.spp + info.y;
+    float4 beta4 = b.beta[j], L4 = b.L[j];
+    const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
+    const V rd{d4.x, d4.y, d4.z};
+    const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd);
+    const DevMaterial m = S.mats[S.objs[h.obj].material];
+    const bool specular = beta4.w != 0.0f;
+    if (specular && m.kind == kMatDiffuseLight) {
+      L4.x = L4.x + m.rgb[0] * beta4.x;
+      L4.y = L4.y + m.rgb[1] * beta4.y;
+      L4.z = L4.z + m.rgb[2] * beta4.z;
+    }
+    info.z += 1;
+    const uint32_t bounce = info.z;
+    b.L[j] = L4;
+    b.info[j] = info;
+    if (bounce == p.max_depth) continue;
+    if (S.n_lights > 0) {
+      const float u = urand(p.seed, path, bounce, kSlotLight);
+      const float fi = fminf(u * (float)S.n_lights, (float)S.n_lights - 1.0f);
+      const DevLight Ld = S.lights[(uint32_t)fi];
+      const float pl = 1.0f / (float)S.n_lights;
+      LiOut ls;
+      if (sample_li(S, Ld, is, p.seed, path, bounce, &ls)) {
+        const V f0 = mat_f(m, is, vmul(rd, -1.0f), ls.wi);
+        const float c = fabsf(vdot(ls.wi, is.n));
+        const V f{f0.x * c, f0.y * c, f0.z * c};
+        const float den = pl * ls.pdf;
+        const V contrib{((ls.li.x * f.x) * beta4.x) / den, ((ls.li.y * f.y) * beta4.y) / den,
+                        ((ls.li.z * f.z) * beta4.z) / den};
+        const uint32_t k = atomicAdd(b.counts + 2, 1u);
+        b.sh_o[k] = make_float4(ls.so.x, ls.so.y, ls.so.z, 1e-5f);
+        b.sh_d[k] = make_float4(ls.sd.x, ls.sd.y, ls.sd.z, ls.dist);
+        b.sh_c[k] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(j));
+      }
+    }
+    ScatterOut s;
+    if (!scatter(m, is, p.seed, path, bounce, &s)) continue;
+    const float c = fabsf(vdot(s.d, is.n));
+    beta4.x = beta4.x * ((s.att.x * c) / s.pdf);
+    beta4.y = beta4.y * ((s.att.y * c) / s.pdf);
+    beta4.z = beta4.z * ((s.att.z * c) / s.pdf);
+    beta4.w = s.specular ? 1.0f : 0.0f;
+    b.beta[j] = beta4;
+    b.ray_o[j] = make_float4(s.o.x, s.o.y, s.o.z, p.near);
+    b.ray_d[j] = make_float4(s.d.x, s.d.y, s.d.z, p.far);
+    const uint32_t k = atomicAdd(b.counts + (q ^ 1), 1u);
+    b.queue[q ^ 1][k] = j;
+  }
+}
+
+// VisibilityTester::unoccluded (lights.rs:67-69): add the contribution when nothing blocks it.
+template <int DEPTH>
+__global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, PathBuffers b, uint2* __restrict__ spill,
+                                                        uint32_t spill_cap, int* __restrict__ status) {
+  __shared__ uint2 lds[DEPTH][kSceneBlock];
+  Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
+                               blockIdx.x * kSceneBlock + threadIdx.x, 0};
+  const uint32_t n = b.counts[2];
+  count_rays(b, n);
+  for (uint32_t i = st.gtid; i < n; i += gridDim.x * kSceneBlock) {
+    const float4 o = b.sh_o[i], d = b.sh_d[i], c = b.sh_c[i];
+    if (scene_any(S, V{o.x, o.y, o.z}, V{d.x, d.y, d.z}, o.w, d.w, st, status)) continue;
+    const uint32_t j = __float_as_uint(c.w);
+    float4 L = b.L[j];
+    L.x = L.x + c.x;
+    L.y = L.y + c.y;
+    L.z = L.z + c.z;
+    b.L[j] = L;
+  }
+}
+
+// Per-pixel FP64 sums over this pass's samples, in sample order.
+__global__ void k_accumulate(RenderParams p, PathBuffers b) {
+  const uint32_t npix = p.width * p.height;
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    double acc[3] = {b.film[3 * (size_t)px], b.film[3 * (size_t)px + 1], b.film[3 * (size_t)px + 2]};
+    for (uint32_t s = 0; s < p.pass_samples; ++s) {
+      const float4 L = b.L[(size_t)s * npix + px];
+      acc[0] += (double)L.x;
+      acc[1] += (double)L.y;
+      acc[2] += (double)L.z;
+    }
+    b.film[3 * (size_t)px] = acc[0];
+    b.film[3 * (size_t)px + 1] = acc[1];
+    b.film[3 * (size_t)px + 2] = acc[2];
+  }
+}
+
+uint32_t blocks_for(uint32_t n, uint32_t block) {
+  uint32_t g = (n + block - 1) / block;
+  return g < 1 ? 1 : (g > 65535 ? 65535 : g);
+}
+
+}  // namespace
+
+hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_scene_trace<kSceneStack>), dim3(a.grid), dim3(kSceneBlock), 0, (hipStream_t)a.stream, a.view,
+                     (const float4*)a.rays, a.hits, a.occluded, a.n, a.spill, a.spill_per_thread, a.status, any_hit);
+  return hipGetLastError();
+}
+
+hipError_t launch_generate(const SceneView&, const RenderParams& p, const PathBuffers& b, void* stream) {
+  const uint32_t n = p.width * p.height * p.pass_samples;
+  hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, (hipStream_t)stream, b, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
+                         uint32_t grid, int* status, void* stream) {
+  hipLaunchKernelGGL((k_extend<kSceneStack>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, b, q, spill,
+                     spill_per_thread, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream) {
+  hipLaunchKernelGGL(k_shade, dim3(blocks_for(b.capacity, 256)), dim3(256), 0, (hipStream_t)stream, v, p, b, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill, uint32_t spill_per_thread,
+                         uint32_t grid, int* status, void* stream) {
+  hipLaunchKernelGGL((k_shadow<kSceneStack>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, b, spill,
+                     spill_per_thread, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const RenderParams& p, const PathBuffers& b, void* stream) {
+  hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(p.width * p.height, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
+  return hipGetLastError();
+}
+
+}  // namespace pbrt

@@ -1,0 +1,112 @@
+// Device scene layout and the launch interface of the scene and renderer kernels (scene.hip).
+#pragma once
+#include "common.hpp"
+
+namespace pbrt {
+
+enum ShapeKind : int32_t { kShapeMesh = 0, kShapeSphere = 1, kShapeBox = 2, kShapeRect = 3 };
+enum MatKind : int32_t { kMatLambertian = 0, kMatDielectric = 1, kMatDiffuseLight = 2, kMatGGX = 3 };
+enum LightKind : int32_t { kLightPoint = 0, kLightArea = 1 };
+
+// One scene object (a shape, a material and an optional rigid transform), in the leaf order of
+// the scene BVH. Matrices are the first three rows of the 4x4 local->world and world->local
+// transforms (the fourth row of an affine transform is 0 0 0 1 and contributes w = 1).
+struct alignas(16) DevObject {
+  int32_t shape, material, has_xform, orig;   // orig: the object's index in the caller's scene
+  float param[4];                             // sphere r; box extents; rectangle width, height
+  float m[12], minv[12];
+  int32_t node_base, tri_base, root_ref, mesh;
+  float root_lo[4], root_hi[4];               // mesh BVH root box
+  uint32_t vert_base, idx_base, pad0, pad1;   // mesh vertices / indices for shading
+};
+
+struct alignas(16) DevMaterial {
+  int32_t kind, pad;
+  float rgb[3];
+  float ior, roughness, pad2;
+};
+
+struct alignas(16) DevLight {
+  int32_t kind, object;  // object: leaf-order index of an area light's rectangle
+  float pos[3], rgb[3];
+};
+
+// Device view of a committed scene.
+struct SceneView {
+  const DevNode* tlas;
+  RootDesc root;
+  const DevObject* objs;
+  const DevNode* blas;
+  const DevTri* tris;
+  const float* verts;     // xyz per vertex, all meshes
+  const uint32_t* idx;    // 3 per triangle, relative to the mesh's vert_base
+  const DevMaterial* mats;
+  const DevLight* lights;
+  uint32_t n_objects, n_lights;
+  int has_tlas;
+};
+
+// Closest hit at scene level: object (caller's index), primitive (mesh triangle or -1), distance
+// and the shape's (u, v) — barycentrics for a triangle.
+struct alignas(16) SceneHitRec {
+  float t, u, v;
+  int32_t obj;   // leaf-order object index, -1 on a miss
+  int32_t prim;  // original triangle index for meshes, -1 otherwise
+  int32_t orig;  // caller's object index
+  int32_t face;  // box face (x-, x+, y-, y+, z-, z+) of a box hit
+  int32_t pad;
+};
+
+struct SceneLaunch {
+  SceneView view;
+  const pbrt_ray* rays;
+  SceneHitRec* hits;
+  uint8_t* occluded;
+  uint32_t n;
+  uint2* spill;
+  uint32_t spill_per_thread;
+  int* status;
+  void* stream;
+  uint32_t grid, block;
+};
+
+hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit);
+
+// ---- wavefront renderer ----
+struct RenderParams {
+  uint32_t width, height, spp, max_depth;
+  uint32_t pass_first_sample, pass_samples;  // samples [first, first + count) of every pixel
+  uint64_t seed;
+  double r2s[16], c2w[16];  // pinhole raster->camera and camera->world (f64, row-major)
+  float near, far;
+};
+
+struct PathBuffers {
+  float4* ray_o;   // origin xyz, near
+  float4* ray_d;   // direction xyz, far
+  float4* beta;    // throughput rgb, specular flag (1/0)
+  float4* L;       // radiance rgb
+  uint4* info;     // pixel, sample, bounce, alive
+  uint32_t* queue[2];
+  uint32_t* counts;  // [0], [1]: path queues; [2]: shadow queue; [4..5]: rays traced (uint64)
+  SceneHitRec* hit;
+  float4* sh_o;    // shadow ray origin, near
+  float4* sh_d;    // shadow ray direction, far
+  float4* sh_c;    // contribution rgb, path index bits
+  double* film;    // width * height * 3, running sum
+  uint32_t capacity;
+};
+
+hipError_t launch_generate(const SceneView& v, const RenderParams& p, const PathBuffers& b, void* stream);
+hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
+                         uint32_t grid, int* status, void* stream);
+hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream);
+hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill, uint32_t spill_per_thread,
+                         uint32_t grid, int* status, void* stream);
+hipError_t launch_accumulate(const RenderParams& p, const PathBuffers& b, void* stream);
+hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream);
+
+constexpr uint32_t kSceneBlock = 128;
+constexpr int kSceneStack = 16;
+
+}  // namespace pbrt

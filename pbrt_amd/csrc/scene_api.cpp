@@ -1,0 +1,623 @@
+// C ABI of the scene seam and the renderer: Scene::{new, add_object, add_light, commit, intersect,
+// intersect_p} (src/scene.rs:80-150) and renderers::render (src/renderers/mt.rs:23) over the device
+// scene of scene.hip. See include/pbrt_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "camera.hpp"
+#include "common.hpp"
+#include "context.hpp"
+#include "scene.hpp"
+
+namespace pbrt {
+namespace {
+
+struct HostObject {
+  int32_t shape, mesh, material, has_xform;
+  double param[3];
+  double m[16], minv[16];
+};
+
+struct Box64 {
+  double lo[3], hi[3];
+  static Box64 empty() { return {{DBL_MAX, DBL_MAX, DBL_MAX}, {-DBL_MAX, -DBL_MAX, -DBL_MAX}}; }
+  void grow(const Box64& b) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::fmin(lo[a], b.lo[a]);
+      hi[a] = std::fmax(hi[a], b.hi[a]);
+    }
+  }
+  double centroid(int a) const { return (hi[a] + lo[a]) * 0.5; }
+};
+
+template <class T>
+void dev_upload(T** dst, const std::vector<T>& src) {
+  const size_t n = std::max<size_t>(1, src.size());
+  PBRT_HIP(hipMalloc(dst, n * sizeof(T)));
+  if (!src.empty()) PBRT_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+}  // namespace pbrt
+
+using namespace pbrt;
+
+struct pbrt_scene {
+  pbrt_ctx* ctx = nullptr;
+  pbrt_bvh_params mesh_params{12, 4, 1.0};
+  std::vector<std::vector<float>> mesh_xyz;
+  std::vector<std::vector<uint32_t>> mesh_idx;
+  std::vector<HostObject> objects;
+  std::vector<DevMaterial> materials;
+  struct HL {
+    int32_t kind, object;
+    double pos[3], rgb[3];
+  };
+  std::vector<HL> lights;
+  bool committed = false;
+  DevNode *tlas = nullptr, *blas = nullptr;
+  DevTri* tris = nullptr;
+  DevObject* objs = nullptr;
+  float* verts = nullptr;
+  uint32_t* idx = nullptr;
+  DevMaterial* mats = nullptr;
+  DevLight* dlights = nullptr;
+  RootDesc root{};
+  bool has_tlas = false;
+  uint32_t tlas_depth = 0, blas_depth = 0;
+  pbrt_build_stats tlas_stats{};
+  int* status = nullptr;
+  uint2* spill = nullptr;
+  uint64_t spill_entries = 0;
+  PathBuffers pb{};
+  std::mutex mu;
+};
+
+namespace {
+
+void check_scene(pbrt_scene* s, bool need_commit) {
+  if (!s) throw Error(PBRT_ERR_INVALID_ARG, "null scene");
+  if (need_commit && !s->committed) throw Error(PBRT_ERR_INVALID_ARG, "scene not committed");
+  if (!need_commit && s->committed) throw Error(PBRT_ERR_INVALID_ARG, "scene already committed");
+  PBRT_HIP(hipSetDevice(s->ctx->device));
+}
+
+SceneView view_of(const pbrt_scene* s) {
+  SceneView v{};
+  v.tlas = s->tlas;
+  v.root = s->root;
+  v.objs = s->objs;
+  v.blas = s->blas;
+  v.tris = s->tris;
+  v.verts = s->verts;
+  v.idx = s->idx;
+  v.mats = s->mats;
+  v.lights = s->dlights;
+  v.n_objects = (uint32_t)s->objects.size();
+  v.n_lights = (uint32_t)s->lights.size();
+  v.has_tlas = s->has_tlas ? 1 : 0;
+  return v;
+}
+
+uint32_t persistent_grid(const pbrt_scene* s, uint32_t n) {
+  const uint32_t cap = (uint32_t)std::max(1, s->ctx->cu_count) * 8;  // 8 blocks of 2 waves per CU
+  const uint32_t need = (n + kSceneBlock - 1) / kSceneBlock;
+  return std::max(1u, std::min(cap, need));
+}
+
+// The HBM spill area for a grid: a lane's stack holds at most (top-level + mesh depth) entries.
+uint32_t ensure_spill(pbrt_scene* s, uint32_t grid) {
+  const uint32_t depth = s->tlas_depth + s->blas_depth + 2;
+  const uint32_t per = depth > (uint32_t)kSceneStack ? depth - (uint32_t)kSceneStack : 1u;
+  const uint64_t need = (uint64_t)per * grid * kSceneBlock;
+  if (need > s->spill_entries) {
+    if (s->spill) PBRT_HIP(hipFree(s->spill));
+    s->spill = nullptr;
+    s->spill_entries = 0;
+    PBRT_HIP(hipMalloc(&s->spill, need * sizeof(uint2)));
+    s->spill_entries = need;
+  }
+  return per;
+}
+
+// AABoundingBox::transform (src/geom/aabound.rs): the eight corners through the transform.
+Box64 world_box(const Box64& b, const HostObject& o) {
+  if (!o.has_xform) return b;
+  const double* M = o.m;
+  Box64 r = Box64::empty();
+  const double xs[2] = {b.lo[0], b.hi[0]}, ys[2] = {b.lo[1], b.hi[1]}, zs[2] = {b.lo[2], b.hi[2]};
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) {
+        const double x = xs[i], y = ys[j], z = zs[k];
+        double px = M[0] * x + M[1] * y + M[2] * z + M[3];
+        double py = M[4] * x + M[5] * y + M[6] * z + M[7];
+        double pz = M[8] * x + M[9] * y + M[10] * z + M[11];
+        const double w = M[12] * x + M[13] * y + M[14] * z + M[15];
+        if (w != 1.0) {
+          px = px * (1.0 / w);
+          py = py * (1.0 / w);
+          pz = pz * (1.0 / w);
+        }
+        r.lo[0] = std::fmin(r.lo[0], px); r.hi[0] = std::fmax(r.hi[0], px);
+        r.lo[1] = std::fmin(r.lo[1], py); r.hi[1] = std::fmax(r.hi[1], py);
+        r.lo[2] = std::fmin(r.lo[2], pz); r.hi[2] = std::fmax(r.hi[2], pz);
+      }
+  return r;
+}
+
+// BVH::subdivide (src/bvh.rs:291-335): median split on the widest centroid axis, stable sort,
+// one object per leaf; returns the node's box. Nodes are emitted in preorder.
+struct TlasBuilder {
+  const std::vector<Box64>& boxes;
+  std::vector<uint32_t>& order;
+  std::vector<BuildNode>& nodes;
+  uint32_t max_depth = 0;
+  Box64 build(uint32_t first, uint32_t count, uint32_t depth, uint32_t* me_out) {
+    const uint32_t me = (uint32_t)nodes.size();
+    nodes.push_back(BuildNode{});
+    *me_out = me;
+    max_depth = std::max(max_depth, depth);
+    if (count == 1) {
+      const Box64 b = boxes[order[first]];
+      BuildNode& n = nodes[me];
+      for (int a = 0; a < 3; ++a) {
+        n.lo[a] = b.lo[a];
+        n.hi[a] = b.hi[a];
+      }
+      n.leaf = true;
+      n.first = first;
+      n.count = 1;
+      return b;
+    }
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (uint32_t i = first; i < first + count; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const double c = boxes[order[i]].centroid(a);
+        mn[a] = std::fmin(mn[a], c);
+        mx[a] = std::fmax(mx[a], c);
+      }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+      if (mx[a] - mn[a] > mx[axis] - mn[axis]) axis = a;
+    std::stable_sort(order.begin() + first, order.begin() + first + count,
+                     [&](uint32_t p, uint32_t q) { return boxes[p].centroid(axis) < boxes[q].centroid(axis); });
+    const uint32_t lc = count / 2;
+    uint32_t l, r;
+    const Box64 lb = build(first, lc, depth + 1, &l);
+    const Box64 rb = build(first + lc, count - lc, depth + 1, &r);
+    Box64 b = Box64::empty();
+    b.grow(lb);
+    b.grow(rb);
+    BuildNode& n = nodes[me];
+    for (int a = 0; a < 3; ++a) {
+      n.lo[a] = b.lo[a];
+      n.hi[a] = b.hi[a];
+    }
+    n.leaf = false;
+    n.left = l;
+    n.right = r;
+    return b;
+  }
+};
+
+void flatten_tlas(const std::vector<BuildNode>& b, std::vector<DevNode>* out, RootDesc* root) {
+  std::vector<int32_t> dev_index(b.size(), -1);
+  int32_t n_int = 0;
+  for (size_t i = 0; i < b.size(); ++i)
+    if (!b[i].leaf) dev_index[i] = n_int++;
+  auto ref_of = [&](uint32_t i) { return b[i].leaf ? leaf_ref(b[i].first, 1) : dev_index[i]; };
+  out->assign(n_int, DevNode{});
+  for (size_t i = 0; i < b.size(); ++i) {
+    if (b[i].leaf) continue;
+    DevNode& d = (*out)[dev_index[i]];
+    const BuildNode &L = b[b[i].left], &R = b[b[i].right];
+    d.xy0[0] = (float)L.lo[0]; d.xy0[1] = (float)L.hi[0]; d.xy0[2] = (float)L.lo[1]; d.xy0[3] = (float)L.hi[1];
+    d.xy1[0] = (float)R.lo[0]; d.xy1[1] = (float)R.hi[0]; d.xy1[2] = (float)R.lo[1]; d.xy1[3] = (float)R.hi[1];
+    d.z01[0] = (float)L.lo[2]; d.z01[1] = (float)L.hi[2]; d.z01[2] = (float)R.lo[2]; d.z01[3] = (float)R.hi[2];
+    d.ref[0] = ref_of(b[i].left);
+    d.ref[1] = ref_of(b[i].right);
+  }
+  for (int a = 0; a < 3; ++a) {
+    root->lo[a] = (float)b[0].lo[a];
+    root->hi[a] = (float)b[0].hi[a];
+  }
+  root->ref = ref_of(0);
+}
+
+void free_paths(PathBuffers& p) {
+  for (void* q : {(void*)p.ray_o, (void*)p.ray_d, (void*)p.beta, (void*)p.L, (void*)p.info, (void*)p.queue[0],
+                  (void*)p.queue[1], (void*)p.counts, (void*)p.hit, (void*)p.sh_o, (void*)p.sh_d, (void*)p.sh_c})
+    if (q) (void)hipFree(q);
+  p = PathBuffers{};
+}
+
+void ensure_paths(pbrt_scene* s, uint32_t capacity) {
+  PathBuffers& p = s->pb;
+  if (p.capacity >= capacity) return;
+  free_paths(p);
+  const size_t n = capacity;
+  PBRT_HIP(hipMalloc(&p.ray_o, n * sizeof(float4)));
+  PBRT_HIP(hipMalloc(&p.ray_d, n * sizeof(float4)));
+  PBRT_HIP(hipMalloc(&p.beta, n * sizeof(float4)));
+  PBRT_HIP(hipMalloc(&p.L, n * sizeof(float4)));
+  PBRT_HIP(hipMalloc(&p.info, n * sizeof(uint4)));
+  PBRT_HIP(hipMalloc(&p.queue[0], n * sizeof(uint32_t)));
+  PBRT_HIP(hipMalloc(&p.queue[1], n * sizeof(uint32_t)));
+  PBRT_HIP(hipMalloc(&p.counts, 16 * sizeof(uint32_t)));
+  PBRT_HIP(hipMalloc(&p.hit, n * sizeof(SceneHitRec)));
+  PBRT_HIP(hipMalloc(&p.sh_o, n * sizeof(float4)));
+  PBRT_HIP(hipMalloc(&p.sh_d, n * sizeof(float4)));
+  PBRT_HIP(hipMalloc(&p.sh_c, n * sizeof(float4)));
+  p.capacity = capacity;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbrt_scene_create(pbrt_ctx* ctx, const pbrt_bvh_params* mesh_params, pbrt_scene** out) {
+  return guarded([&] {
+    if (!ctx || !out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    auto s = std::make_unique<pbrt_scene>();
+    s->ctx = ctx;
+    if (mesh_params) s->mesh_params = *mesh_params;
+    if (s->mesh_params.max_leaf > kMaxLeafTris) throw Error(PBRT_ERR_INVALID_ARG, "max_leaf above 32");
+    *out = s.release();
+  });
+}
+
+int pbrt_scene_destroy(pbrt_scene* s) {
+  return guarded([&] {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    for (void* p : {(void*)s->tlas, (void*)s->blas, (void*)s->tris, (void*)s->objs, (void*)s->verts, (void*)s->idx,
+                    (void*)s->mats, (void*)s->dlights, (void*)s->status, (void*)s->spill})
+      if (p) (void)hipFree(p);
+    free_paths(s->pb);
+    delete s;
+  });
+}
+
+int pbrt_scene_add_mesh(pbrt_scene* s, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt, uint32_t* id) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (nt == 0) throw Error(PBRT_ERR_EMPTY_MESH, "a TriangleMesh needs at least one triangle");
+    if (!xyz || !idx) throw Error(PBRT_ERR_INVALID_ARG, "null vertex or index array");
+    for (uint64_t i = 0; i < 3ull * nt; ++i)
+      if (idx[i] >= nv) throw Error(PBRT_ERR_INVALID_ARG, "vertex index out of range");
+    s->mesh_xyz.emplace_back(xyz, xyz + 3 * (size_t)nv);
+    s->mesh_idx.emplace_back(idx, idx + 3 * (size_t)nt);
+    if (id) *id = (uint32_t)s->mesh_xyz.size() - 1;
+  });
+}
+
+int pbrt_scene_add_material(pbrt_scene* s, int kind, const double rgb[3], double ior, double roughness, uint32_t* id) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (kind < kMatLambertian || kind > kMatGGX) throw Error(PBRT_ERR_INVALID_ARG, "unknown material kind");
+    DevMaterial m{};
+    m.kind = kind;
+    for (int c = 0; c < 3; ++c) m.rgb[c] = (float)rgb[c];
+    m.ior = (float)ior;
+    m.roughness = (float)roughness;
+    s->materials.push_back(m);
+    if (id) *id = (uint32_t)s->materials.size() - 1;
+  });
+}
+
+int pbrt_scene_add_object(pbrt_scene* s, int shape, int mesh, const double param[3], uint32_t material,
+                          const double* m16, const double* minv16, uint32_t* id) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (shape < kShapeMesh || shape > kShapeRect) throw Error(PBRT_ERR_INVALID_ARG, "unknown shape kind");
+    if (shape == kShapeMesh && (mesh < 0 || (size_t)mesh >= s->mesh_xyz.size()))
+      throw Error(PBRT_ERR_INVALID_ARG, "no such mesh");
+    if (material >= s->materials.size()) throw Error(PBRT_ERR_INVALID_ARG, "no such material");
+    if ((m16 == nullptr) != (minv16 == nullptr)) throw Error(PBRT_ERR_INVALID_ARG, "a transform needs its inverse");
+    HostObject o{};
+    o.shape = shape;
+    o.mesh = mesh;
+    o.material = (int32_t)material;
+    for (int k = 0; k < 3; ++k) o.param[k] = param ? param[k] : 0.0;
+    if (m16) {
+      for (int k = 12; k < 16; ++k)
+        if (m16[k] != (k == 15 ? 1.0 : 0.0) || minv16[k] != (k == 15 ? 1.0 : 0.0))
+          throw Error(PBRT_ERR_INVALID_ARG, "transforms must be affine (last row 0 0 0 1)");
+      o.has_xform = 1;
+      std::memcpy(o.m, m16, sizeof(o.m));
+      std::memcpy(o.minv, minv16, sizeof(o.minv));
+    }
+    s->objects.push_back(o);
+    if (id) *id = (uint32_t)s->objects.size() - 1;
+  });
+}
+
+int pbrt_scene_add_point_light(pbrt_scene* s, const double pos[3], const double rgb[3]) {
+  return guarded([&] {
+    check_scene(s, false);
+    pbrt_scene::HL l{kLightPoint, -1, {pos[0], pos[1], pos[2]}, {rgb[0], rgb[1], rgb[2]}};
+    s->lights.push_back(l);
+  });
+}
+
+int pbrt_scene_add_area_light(pbrt_scene* s, uint32_t object, const double rgb[3]) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (object >= s->objects.size() || s->objects[object].shape != kShapeRect)
+      throw Error(PBRT_ERR_INVALID_ARG, "an area light needs a rectangle object");
+    pbrt_scene::HL l{kLightArea, (int32_t)object, {0, 0, 0}, {rgb[0], rgb[1], rgb[2]}};
+    s->lights.push_back(l);
+  });
+}
+
+// Scene::commit (scene.rs:101-115): per-mesh BVHs, the object BVH, upload.
+int pbrt_scene_commit(pbrt_scene* s) {
+  return guarded([&] {
+    check_scene(s, false);
+    // mesh BVHs, flattened into shared node / triangle arrays
+    std::vector<DevNode> blas;
+    std::vector<DevTri> tris;
+    std::vector<float> verts;
+    std::vector<uint32_t> idx;
+    struct MeshInfo {
+      int32_t node_base, tri_base, root_ref;
+      float lo[3], hi[3];
+      uint32_t vert_base, idx_base;
+      Box64 bounds;
+    };
+    std::vector<MeshInfo> info;
+    for (size_t k = 0; k < s->mesh_xyz.size(); ++k) {
+      const auto& x = s->mesh_xyz[k];
+      const auto& ix = s->mesh_idx[k];
+      const uint32_t nv = (uint32_t)(x.size() / 3), nt = (uint32_t)(ix.size() / 3);
+      BuiltBVH b = build_mesh_bvh(x.data(), nv, ix.data(), nt, s->mesh_params);
+      s->blas_depth = std::max<uint32_t>(s->blas_depth, (uint32_t)b.stats.max_depth);
+      std::vector<DevNode> n;
+      std::vector<DevTri> t;
+      RootDesc r;
+      flatten_mesh_bvh(b, x.data(), ix.data(), &n, &t, &r);
+      MeshInfo mi{};
+      mi.node_base = (int32_t)blas.size();
+      mi.tri_base = (int32_t)tris.size();
+      mi.root_ref = r.ref;
+      for (int a = 0; a < 3; ++a) {
+        mi.lo[a] = r.lo[a];
+        mi.hi[a] = r.hi[a];
+      }
+      mi.vert_base = (uint32_t)(verts.size() / 3);
+      mi.idx_base = (uint32_t)(idx.size() / 3);
+      // TriangleMesh::get_bounding_box (triangle_mesh.rs:150-174): every vertex
+      mi.bounds = Box64::empty();
+      for (uint32_t v = 0; v < nv; ++v)
+        for (int a = 0; a < 3; ++a) {
+          mi.bounds.lo[a] = std::fmin((double)x[3 * v + a], mi.bounds.lo[a]);
+          mi.bounds.hi[a] = std::fmax((double)x[3 * v + a], mi.bounds.hi[a]);
+        }
+      blas.insert(blas.end(), n.begin(), n.end());
+      tris.insert(tris.end(), t.begin(), t.end());
+      verts.insert(verts.end(), x.begin(), x.end());
+      idx.insert(idx.end(), ix.begin(), ix.end());
+      info.push_back(mi);
+    }
+    // object boxes (local bounds through the transform) and the median-split object BVH
+    const uint32_t no = (uint32_t)s->objects.size();
+    std::vector<Box64> boxes(no);
+    for (uint32_t k = 0; k < no; ++k) {
+      const HostObject& o = s->objects[k];
+      Box64 b;
+      switch (o.shape) {
+        case kShapeSphere: b = {{-o.param[0], -o.param[0], -o.param[0]}, {o.param[0], o.param[0], o.param[0]}}; break;
+        case kShapeBox:
+          b = {{0.0 - o.param[0] * 0.5, 0.0 - o.param[1] * 0.5, 0.0 - o.param[2] * 0.5},
+               {o.param[0] * 0.5, o.param[1] * 0.5, o.param[2] * 0.5}};
+          break;
+        case kShapeRect:
+          b = {{-o.param[0] / 2.0, 0.0, -o.param[1] / 2.0}, {o.param[0] / 2.0, 0.0, o.param[1] / 2.0}};
+          break;
+        default: b = info[o.mesh].bounds; break;
+      }
+      boxes[k] = world_box(b, o);
+    }
+    std::vector<uint32_t> order(no);
+    for (uint32_t k = 0; k < no; ++k) order[k] = k;
+    std::vector<DevNode> tlas;
+    if (no > 0) {
+      std::vector<BuildNode> nodes;
+      TlasBuilder tb{boxes, order, nodes};
+      uint32_t root;
+      tb.build(0, no, 1, &root);
+      s->tlas_depth = tb.max_depth;
+      flatten_tlas(nodes, &tlas, &s->root);
+      s->has_tlas = true;
+      s->tlas_stats.node_count = nodes.size();
+    }
+    std::vector<uint32_t> pos_of(no);
+    std::vector<DevObject> dobjs(no);
+    for (uint32_t p = 0; p < no; ++p) {
+      const uint32_t k = order[p];
+      pos_of[k] = p;
+      const HostObject& o = s->objects[k];
+      DevObject& d = dobjs[p];
+      d.shape = o.shape;
+      d.material = o.material;
+      d.has_xform = o.has_xform;
+      d.orig = (int32_t)k;
+      for (int a = 0; a < 3; ++a) d.param[a] = (float)o.param[a];
+      for (int a = 0; a < 12; ++a) {
+        d.m[a] = (float)o.m[a];
+        d.minv[a] = (float)o.minv[a];
+      }
+      d.mesh = o.mesh;
+      if (o.shape == kShapeMesh) {
+        const MeshInfo& mi = info[o.mesh];
+        d.node_base = mi.node_base;
+        d.tri_base = mi.tri_base;
+        d.root_ref = mi.root_ref;
+        for (int a = 0; a < 3; ++a) {
+          d.root_lo[a] = mi.lo[a];
+          d.root_hi[a] = mi.hi[a];
+        }
+        d.vert_base = mi.vert_base;
+        d.idx_base = mi.idx_base;
+      }
+    }
+    std::vector<DevLight> dl;
+    for (const auto& l : s->lights) {
+      DevLight d{};
+      d.kind = l.kind;
+      d.object = l.kind == kLightArea ? (int32_t)pos_of[l.object] : -1;
+      for (int a = 0; a < 3; ++a) {
+        d.pos[a] = (float)l.pos[a];
+        d.rgb[a] = (float)l.rgb[a];
+      }
+      dl.push_back(d);
+    }
+    dev_upload(&s->tlas, tlas);
+    dev_upload(&s->blas, blas);
+    dev_upload(&s->tris, tris);
+    dev_upload(&s->objs, dobjs);
+    dev_upload(&s->verts, verts);
+    dev_upload(&s->idx, idx);
+    dev_upload(&s->mats, s->materials);
+    dev_upload(&s->dlights, dl);
+    PBRT_HIP(hipMalloc(&s->status, sizeof(int)));
+    PBRT_HIP(hipMemset(s->status, 0, sizeof(int)));
+    s->committed = true;
+  });
+}
+
+int pbrt_scene_info(pbrt_scene* s, uint32_t* objects, uint32_t* tlas_nodes, uint32_t* tlas_depth, uint32_t* blas_depth) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (objects) *objects = (uint32_t)s->objects.size();
+    if (tlas_nodes) *tlas_nodes = (uint32_t)s->tlas_stats.node_count;
+    if (tlas_depth) *tlas_depth = s->tlas_depth;
+    if (blas_depth) *blas_depth = s->blas_depth;
+  });
+}
+
+static int scene_trace(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, pbrt_scene_hit* hits, uint8_t* occ,
+                       void* stream, int any) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (n == 0) return;
+    if (!rays || (any ? !occ : !hits)) throw Error(PBRT_ERR_INVALID_ARG, "null buffer");
+    std::lock_guard<std::mutex> lock(s->mu);
+    SceneLaunch a{};
+    a.view = view_of(s);
+    a.rays = rays;
+    a.hits = reinterpret_cast<SceneHitRec*>(hits);
+    a.occluded = occ;
+    a.n = n;
+    a.grid = persistent_grid(s, n);
+    a.spill_per_thread = ensure_spill(s, a.grid);
+    a.spill = s->spill;
+    a.status = s->status;
+    a.stream = stream ? stream : s->ctx->stream;
+    PBRT_HIP(launch_scene_trace(a, any));
+  });
+}
+
+int pbrt_scene_trace_closest(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, pbrt_scene_hit* hits, void* stream) {
+  return scene_trace(s, rays, n, hits, nullptr, stream, 0);
+}
+
+int pbrt_scene_trace_any(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, uint8_t* occ, void* stream) {
+  return scene_trace(s, rays, n, nullptr, occ, stream, 1);
+}
+
+int pbrt_scene_status(pbrt_scene* s, int* out) {
+  return guarded([&] {
+    check_scene(s, true);
+    PBRT_HIP(hipDeviceSynchronize());
+    PBRT_HIP(hipMemcpy(out, s->status, sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+// renderers::mt::render (src/renderers/mt.rs:23, 112-130) on the device: every pixel's `spp`
+// samples traced by the wavefront of scene.hip; film_out receives the per-pixel mean radiance
+// (f64 RGB, row-major), before the reference's gamma and 8-bit quantisation (spectrum.rs:13-30).
+int pbrt_render(pbrt_scene* s, const pbrt_render_config* c, double* film_out, pbrt_render_stats* stats) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (!c || !c->width || !c->height || !c->spp || !c->max_depth) throw Error(PBRT_ERR_INVALID_ARG, "bad render config");
+    std::lock_guard<std::mutex> lock(s->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t npix = c->width * c->height;
+    uint32_t per_pass = c->samples_per_pass ? c->samples_per_pass : std::max(1u, (1u << 22) / npix);
+    per_pass = std::min(per_pass, c->spp);
+    ensure_paths(s, npix * per_pass);
+    hipStream_t st = s->ctx->stream;
+    double* film = nullptr;
+    PBRT_HIP(hipMalloc(&film, (size_t)npix * 3 * sizeof(double)));
+    PBRT_HIP(hipMemsetAsync(film, 0, (size_t)npix * 3 * sizeof(double), st));
+    PBRT_HIP(hipMemsetAsync(s->pb.counts, 0, 16 * sizeof(uint32_t), st));
+    PathBuffers pb = s->pb;
+    pb.film = film;
+    const Pinhole cam(c->width, c->height, c->fov, c->near, c->far,
+                      look_at({c->eye[0], c->eye[1], c->eye[2]}, {c->look[0], c->look[1], c->look[2]},
+                              {c->up[0], c->up[1], c->up[2]}));
+    RenderParams p{};
+    p.width = c->width;
+    p.height = c->height;
+    p.spp = c->spp;
+    p.max_depth = c->max_depth;
+    p.seed = c->seed;
+    for (int i = 0; i < 16; ++i) {
+      p.r2s[i] = cam.r2s.m[i / 4][i % 4];
+      p.c2w[i] = cam.c2w.m[i / 4][i % 4];
+    }
+    p.near = (float)c->near;
+    p.far = (float)c->far;
+    const SceneView v = view_of(s);
+    const uint32_t grid = persistent_grid(s, npix * per_pass);
+    const uint32_t spill_per = ensure_spill(s, grid);
+    uint64_t passes = 0;
+    for (uint32_t first = 0; first < c->spp; first += per_pass) {
+      p.pass_first_sample = first;
+      p.pass_samples = std::min(per_pass, c->spp - first);
+      PBRT_HIP(launch_generate(v, p, pb, st));
+      int q = 0;
+      for (uint32_t b = 0; b < c->max_depth; ++b) {
+        PBRT_HIP(launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st));
+        PBRT_HIP(launch_reset_queue(pb, q, st));
+        PBRT_HIP(launch_shade(v, p, pb, q, st));
+        PBRT_HIP(launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st));
+        q ^= 1;
+      }
+      PBRT_HIP(launch_accumulate(p, pb, st));
+      passes++;
+    }
+    uint32_t counts[16];
+    PBRT_HIP(hipStreamSynchronize(st));
+    PBRT_HIP(hipMemcpy(counts, pb.counts, sizeof(counts), hipMemcpyDeviceToHost));
+    if (film_out) {
+      std::vector<double> sum((size_t)npix * 3);
+      PBRT_HIP(hipMemcpy(sum.data(), film, sum.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < sum.size(); ++i) film_out[i] = sum[i] / (double)c->spp;
+    }
+    PBRT_HIP(hipFree(film));
+    int status = 0;
+    PBRT_HIP(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (status) throw Error(PBRT_ERR_INTERNAL, "a traversal stack overflowed");
+    if (stats) {
+      uint64_t rays;
+      std::memcpy(&rays, counts + 4, sizeof(rays));
+      stats->rays = rays;
+      stats->paths = (uint64_t)npix * c->spp;
+      stats->passes = passes;
+      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  });
+}
+
+}  // extern "C"

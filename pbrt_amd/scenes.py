@@ -1,0 +1,88 @@
+"""Seeded synthetic scenes of the path-tracing workloads, as `SceneDesc` + camera.
+
+  small(): the CPU-runnable scene: an icosphere of 4 subdivisions (5,120 triangles) with +-5%
+      seeded radial noise on a 2-triangle floor, 64 seeded spheres (radius U[0.2, 1], centres uniform
+      in a 20^3 box), Lambertian albedo U[0.2, 0.8]^3, one point light; 256x256, 16 spp, depth 8.
+  room(): a closed room of five diffuse wall rectangles (albedo 0.73, one red and one green side
+      wall), two boxes turned +-17 degrees about y, an emissive ceiling rectangle (radiance 15)
+      covering 1/16 of the ceiling and lighting the room as an area light, and an icosphere of
+      `subdiv` subdivisions with 3-octave value-noise displacement of 8%; Lambertian and dielectric
+      (IOR 1.5) materials; 512x512, 64 spp, depth 8.
+"""
+import math
+
+import numpy as np
+
+from . import icosphere
+from .scene import SceneDesc, Transform
+
+
+def _u(seed, *key):
+    rng = np.random.default_rng([seed] + list(key))
+    return rng
+
+
+def small(seed=7):
+    d = SceneDesc()
+    rng = np.random.default_rng(seed)
+    v, f = icosphere(4, radius=3.0, center=(0.0, 0.0, 0.0), noise="uniform", amp=0.05, seed=seed)
+    m = d.add_material("lambertian", rng.uniform(0.2, 0.8, 3))
+    d.add_object("mesh", m, mesh=d.add_mesh(v, f))
+    floor_v = np.array([[-15, -10, -15], [15, -10, -15], [15, -10, 15], [-15, -10, 15]], np.float32)
+    # wound so that the mesh normal -normalize(cross(e1, e2)) points up (triangle_mesh.rs:119)
+    floor_f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    d.add_object("mesh", d.add_material("lambertian", rng.uniform(0.2, 0.8, 3)), mesh=d.add_mesh(floor_v, floor_f))
+    for _ in range(64):
+        r = rng.uniform(0.2, 1.0)
+        c = rng.uniform(-10.0, 10.0, 3)
+        mat = d.add_material("lambertian", rng.uniform(0.2, 0.8, 3))
+        d.add_object("sphere", mat, param=(r,), transform=Transform.translation(*c))
+    d.add_point_light((0.0, 15.0, 10.0), (900.0, 900.0, 900.0))
+    camera = dict(eye=(0.0, 6.0, 34.0), look=(0.0, -1.0, 0.0), fov=50.0 * math.pi / 180.0, width=256, height=256,
+                  spp=16, max_depth=8)
+    return d, camera
+
+
+def room(subdiv=6, seed=11, ggx=False, big_mesh=None):
+    """The room. `big_mesh` replaces the icosphere (vertices, faces); `ggx` gives the mesh and the
+    boxes glossy GGX materials of roughness U[0.05, 0.5]."""
+    d = SceneDesc()
+    rng = np.random.default_rng(seed)
+    H = 2.0  # half size of the room: walls at +-2
+    white = d.add_material("lambertian", (0.73, 0.73, 0.73))
+    red = d.add_material("lambertian", (0.65, 0.05, 0.05))
+    green = d.add_material("lambertian", (0.12, 0.45, 0.15))
+    # Rectangle: y = 0 plane, normal +y (rectangle.rs); each wall is turned so its normal faces inwards
+    wall = (2 * H, 2 * H)
+    d.add_object("rectangle", white, wall, transform=Transform.translation(0.0, -H, 0.0))  # floor
+    d.add_object("rectangle", white, wall,
+                 transform=Transform.translation(0.0, H, 0.0) * Transform.rotation("x", math.pi))  # ceiling
+    d.add_object("rectangle", white, wall,
+                 transform=Transform.translation(0.0, 0.0, -H) * Transform.rotation("x", math.pi / 2))  # back
+    d.add_object("rectangle", red, wall,
+                 transform=Transform.translation(-H, 0.0, 0.0) * Transform.rotation("z", -math.pi / 2))  # left
+    d.add_object("rectangle", green, wall,
+                 transform=Transform.translation(H, 0.0, 0.0) * Transform.rotation("z", math.pi / 2))  # right
+    # emissive ceiling panel, 1/16 of the ceiling area, just below it, facing down
+    light = d.add_material("diffuse_light", (15.0, 15.0, 15.0))
+    panel = d.add_object("rectangle", light, (H / 2, H / 2),
+                         transform=Transform.translation(0.0, H - 1e-3, 0.0) * Transform.rotation("x", math.pi))
+    d.add_area_light(panel, (15.0, 15.0, 15.0))
+
+    def glossy():
+        return d.add_material("ggx", (0.9, 0.85, 0.8), roughness=float(rng.uniform(0.05, 0.5)))
+
+    box_mat = glossy() if ggx else white
+    d.add_object("aabox", box_mat, (0.9, 1.8, 0.9),
+                 transform=Transform.translation(-0.8, -H + 0.9, -0.7) * Transform.rotation("y", math.radians(17)))
+    d.add_object("aabox", glossy() if ggx else white, (0.8, 0.8, 0.8),
+                 transform=Transform.translation(0.9, -H + 0.4, 0.6) * Transform.rotation("y", math.radians(-17)))
+    if big_mesh is None:
+        v, f = icosphere(subdiv, radius=0.55, center=(0.25, -0.4, 0.9), noise="fbm", amp=0.08, octaves=3, seed=seed)
+    else:
+        v, f = big_mesh
+    mesh_mat = glossy() if ggx else d.add_material("dielectric", (1.0, 1.0, 1.0), ior=1.5)
+    d.add_object("mesh", mesh_mat, mesh=d.add_mesh(v, f))
+    camera = dict(eye=(0.0, 0.0, 5.8), look=(0.0, 0.0, 0.0), fov=40.0 * math.pi / 180.0, width=512, height=512,
+                  spp=64, max_depth=8)
+    return d, camera
