@@ -413,3 +413,36 @@ int orc_scene_render(void* hv, uint32_t w, uint32_t hgt, double fov, const doubl
 }
 
 }  // extern "C"
+
+// ---- neural radiance cache -----------------------------------------------------------------------
+#include "nrc_oracle.hpp"
+
+extern "C" {
+
+int orc_nrc_nparams(void) { return nrc::n_params(); }
+
+void orc_nrc_encode(const float* raw, uint64_t n, const double* lo, const double* hi, float* out) {
+  for (uint64_t i = 0; i < n; ++i) nrc::encode(raw + 12 * i, lo, hi, out + (size_t)nrc::kIn * i);
+}
+
+void orc_nrc_forward(const float* params, const float* x, uint64_t n, int bf16, double* out, int threads) {
+  parallel_for(n, threads, [&](uint64_t i, int) {
+    nrc::Forward f;
+    nrc::forward(params, x + (size_t)nrc::kIn * i, bf16 != 0, &f);
+    for (int c = 0; c < nrc::kOut; ++c) out[3 * i + c] = f.act[nrc::kLayers][c];
+  });
+}
+
+double orc_nrc_loss_grad(const float* params, const float* x, const float* targets, uint64_t n, int bf16, double* grad) {
+  std::vector<double> g;
+  double loss = nrc::loss_grad(params, x, targets, (int)n, bf16 != 0, &g);
+  std::memcpy(grad, g.data(), g.size() * sizeof(double));
+  return loss;
+}
+
+void orc_nrc_adam(float* params, float* m, float* v, const float* grad, int n, int t, float lr, float b1, float b2,
+                  float eps) {
+  nrc::adam(params, m, v, grad, n, t, lr, b1, b2, eps);
+}
+
+}  // extern "C"
