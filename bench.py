@@ -30,6 +30,15 @@ HBM_PEAK_GBS = 8000.0
 SEED_MESH = 2024
 SEED_RAYS = 7
 
+# Memory-side bytes per k_closest launch on this workload (mean of the coherent and incoherent
+# launches), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2,
+# WRITE x1); profiles/k_closest_icosphere1p3m_counters.md. Keyed by (block, waves_per_cu,
+# lds_stack): reported only for the launch shape that was measured.
+MEASURED_TRAFFIC = {
+    (64, 32, 8): {"bytes_per_launch": 1282.23e6, "coherent": 255.70e6, "incoherent": 2308.76e6,
+                  "source": "profiles/k_closest_icosphere1p3m_counters.md"},
+}
+
 
 def rows(r):
     return np.ascontiguousarray(r).view(np.float32).reshape(-1, 8)
@@ -169,6 +178,10 @@ def main():
     # roofline over both launches of the dominant kernel: algorithmic bytes / mean kernel time
     alg_bytes = sum(work[k]["bytes_per_launch"] for k in dev)
     achieved = alg_bytes / (step_kernel_ms * 1e-3) / 1e9
+    launch = mesh.get_launch()
+    measured = MEASURED_TRAFFIC.get((launch["block"], launch["waves_per_cu"], launch["lds_stack"]))
+    if args.side != 2048:
+        measured = None
 
     # any-hit throughput on the same batches (informational)
     anyhit = {}
@@ -224,7 +237,7 @@ def main():
             "rays_per_step": total_rays,
             "bvh": dict(bs, node_bytes=node_bytes, tri_bytes=tri_bytes),
             "parallelism": "single GPU",
-            "launch": mesh.get_launch(),
+            "launch": launch,
             "autotuned": tuned is not None,
         },
         "roofline": {
@@ -233,9 +246,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured["bytes_per_launch"] if measured else None,
             "kernel": "k_closest",
-            "algorithmic_bytes_per_step": alg_bytes,
+            "algorithmic_bytes_per_launch": alg_bytes / len(dev),
+            "traffic_source": measured["source"] if measured else None,
         },
         "per_batch": per_batch,
         "anyhit_mrays_s": anyhit,

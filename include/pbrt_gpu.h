@@ -209,6 +209,51 @@ typedef struct pbrt_render_stats {
  * in FP64 in a fixed sample order. */
 int pbrt_render(pbrt_scene* scene, const pbrt_render_config* config, double* film_out, pbrt_render_stats* stats);
 
+/* ---- neural radiance cache ---------------------------------------------------------------------
+ * No counterpart in the reference, whose only integrator is plain path tracing
+ * (src/integrators/path.rs:13-97). An MLP 81 -> 64 -> 64 -> 64 -> 64 -> 3 (ReLU hidden layers,
+ * linear output) maps an encoded path vertex to outgoing radiance: inputs are the position under a
+ * 12-frequency sin/cos encoding relative to the scene box, the outgoing direction, the normal and
+ * the albedo (12 raw floats per vertex: p, wo, n, albedo). Weights and activations are BF16 on the
+ * matrix cores with FP32 accumulation; FP32 master parameters and Adam moments. Flat parameter
+ * order: per layer W [out][in] row-major, then b [out]; PBRT_NRC_PARAMS floats.
+ * Every `_dev` pointer is device memory, every other pointer host memory. */
+#define PBRT_NRC_PARAMS 17923
+#define PBRT_NRC_FEATURES 96 /* BF16 features per encoded vertex (81 used, zero padded) */
+typedef struct pbrt_nrc pbrt_nrc;
+
+typedef struct pbrt_nrc_optim {
+  float lr, beta1, beta2, eps;
+} pbrt_nrc_optim;
+
+/* Seeded He-uniform weights, zero biases, Adam lr 1e-3, betas 0.9 / 0.99, eps 1e-8 (override with
+ * pbrt_nrc_set_optim); scene box [0, 1]^3 until pbrt_nrc_set_bounds. */
+int pbrt_nrc_create(pbrt_ctx* ctx, uint64_t seed, pbrt_nrc** out);
+int pbrt_nrc_destroy(pbrt_nrc* nrc);
+int pbrt_nrc_set_bounds(pbrt_nrc* nrc, const float lo[3], const float hi[3]);
+int pbrt_nrc_set_optim(pbrt_nrc* nrc, const pbrt_nrc_optim* optim);
+/* FP32 master parameters in flat order (n must be PBRT_NRC_PARAMS); setting them also resets the
+ * Adam moments and the step count. */
+int pbrt_nrc_set_params(pbrt_nrc* nrc, const float* params, uint32_t n);
+int pbrt_nrc_get_params(pbrt_nrc* nrc, float* params, uint32_t n);
+/* raw_dev: n x 12 floats -> x_dev: n x PBRT_NRC_FEATURES BF16 (uint16 bit patterns). */
+int pbrt_nrc_encode(pbrt_nrc* nrc, const float* raw_dev, uint32_t n, uint16_t* x_dev, void* stream);
+/* x_dev (encoded) -> out_dev: n x 3 FP32 radiance. */
+int pbrt_nrc_infer(pbrt_nrc* nrc, const uint16_t* x_dev, uint32_t n, float* out_dev, void* stream);
+/* Loss (relative L2: mean over samples and channels of (y - t)^2 / (y^2 + 0.01), y held constant
+ * in the denominator) and its gradient in flat order, without an update. grads may be NULL. */
+int pbrt_nrc_gradients(pbrt_nrc* nrc, const uint16_t* x_dev, const float* target_dev, uint32_t n, float* grads,
+                       float* loss);
+/* One optimisation step on a batch: gradient, Adam update, refreshed BF16 weights; all on the
+ * stream, no host synchronisation. The batch loss lands in loss_dev (one float) if not NULL. */
+int pbrt_nrc_train_step(pbrt_nrc* nrc, const uint16_t* x_dev, const float* target_dev, uint32_t n, float* loss_dev,
+                        void* stream);
+/* Adam steps taken since creation or the last set_params. */
+int pbrt_nrc_step_count(pbrt_nrc* nrc, uint64_t* steps);
+/* One 16 x 16 x 32 BF16 matrix-core product D = A B (A 16 x 32 row-major, B 32 x 16 row-major,
+ * host BF16 bit patterns in, FP32 out): the operand layout the network's kernels assume. */
+int pbrt_nrc_mfma_check(pbrt_ctx* ctx, const uint16_t* a, const uint16_t* b, float* d);
+
 /* ---- synthetic inputs and ray generation (host, f64 as the reference, rays rounded to FP32) ----
  * pinhole camera of src/cameras/pin_hole.rs:17-50 looking from `eye` at `look` (up = +y, fov in
  * radians); writes width*height rays through pixel centres (px + 0.5, py + 0.5), row-major. */

@@ -52,14 +52,17 @@ inline float bf16_from_bits(uint16_t b) {
   return r;
 }
 
-// The 81 input features of one sample from position (normalised by the scene box), outgoing
-// direction, normal and albedo: for each axis a, for each frequency k: sin(2^k pi p_a), cos(...).
-inline void encode(const float* raw12, const double* lo, const double* hi, float* out81) {
+// The 81 input features of one sample from position, outgoing direction, normal and albedo. The
+// position is normalised by the scene box in FP32, p_a = (x_a - lo_a) * (1 / (hi_a - lo_a)), and
+// that FP32 value is encoded exactly: for each axis a, for each frequency k, sin(2^k pi p_a) and
+// cos(2^k pi p_a), evaluated here in FP64 and rounded once.
+inline void encode(const float* raw12, const float* lo, const float* hi, float* out81) {
   int j = 0;
   for (int a = 0; a < 3; ++a) {
-    const double p = ((double)raw12[a] - lo[a]) / (hi[a] - lo[a]);
+    const float inv = 1.0f / (hi[a] - lo[a]);
+    const float p = (raw12[a] - lo[a]) * inv;
     for (int k = 0; k < kFreq; ++k) {
-      const double arg = std::ldexp(M_PI, k) * p;
+      const double arg = std::ldexp(M_PI, k) * (double)p;
       out81[j++] = (float)std::sin(arg);
       out81[j++] = (float)std::cos(arg);
     }

@@ -421,7 +421,7 @@ extern "C" {
 
 int orc_nrc_nparams(void) { return nrc::n_params(); }
 
-void orc_nrc_encode(const float* raw, uint64_t n, const double* lo, const double* hi, float* out) {
+void orc_nrc_encode(const float* raw, uint64_t n, const float* lo, const float* hi, float* out) {
   for (uint64_t i = 0; i < n; ++i) nrc::encode(raw + 12 * i, lo, hi, out + (size_t)nrc::kIn * i);
 }
 

@@ -222,3 +222,69 @@ class Scene:
         lib().orc_scene_render(self.h, w, h, fov, _p(e), _p(l), spp, max_depth, seed, near, far, y0, y1, _p(film),
                                ctypes.byref(rays), threads)
         return film.reshape(y1 - y0, w, 3), rays.value
+
+
+# ---- neural radiance cache (CPU restatement of the network, FP64 accumulation) ------------------
+def _nrc_sigs(L):
+    if getattr(L, "_nrc_sigs", False):
+        return
+    L.orc_nrc_nparams.restype = ctypes.c_int
+    L.orc_nrc_encode.argtypes = [P, U64, P, P, P]
+    L.orc_nrc_forward.argtypes = [P, P, U64, ctypes.c_int, P, ctypes.c_int]
+    L.orc_nrc_loss_grad.restype = ctypes.c_double
+    L.orc_nrc_loss_grad.argtypes = [P, P, P, U64, ctypes.c_int, P]
+    L.orc_nrc_adam.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float]
+    L._nrc_sigs = True
+
+
+NRC_IN = 81
+
+
+def nrc_nparams():
+    L = lib()
+    _nrc_sigs(L)
+    return L.orc_nrc_nparams()
+
+
+def nrc_encode(raw12, lo, hi):
+    """n x 12 floats (p, wo, n, albedo) -> n x 81 FP32 features (before BF16 rounding)."""
+    L = lib()
+    _nrc_sigs(L)
+    raw = np.ascontiguousarray(raw12, np.float32).reshape(-1, 12)
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    out = np.empty((len(raw), NRC_IN), np.float32)
+    L.orc_nrc_encode(_p(raw), len(raw), _p(lo), _p(hi), _p(out))
+    return out
+
+
+def nrc_forward(params, x81, bf16=True, threads=8):
+    L = lib()
+    _nrc_sigs(L)
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x81, np.float32).reshape(-1, NRC_IN)
+    out = np.empty((len(x), 3), np.float64)
+    L.orc_nrc_forward(_p(params), _p(x), len(x), int(bf16), _p(out), threads)
+    return out
+
+
+def nrc_loss_grad(params, x81, targets, bf16=True):
+    L = lib()
+    _nrc_sigs(L)
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x81, np.float32).reshape(-1, NRC_IN)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    g = np.empty(len(params), np.float64)
+    loss = L.orc_nrc_loss_grad(_p(params), _p(x), _p(t), len(x), int(bf16), _p(g))
+    return loss, g
+
+
+def nrc_adam(params, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
+    """One Adam step in place on FP32 arrays."""
+    L = lib()
+    _nrc_sigs(L)
+    grad = np.ascontiguousarray(grad, np.float32)
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    L.orc_nrc_adam(_p(params), _p(m), _p(v), _p(grad), len(params), step, lr, b1, b2, eps)

@@ -14,7 +14,8 @@ from . import _native
 from ._native import BuildStats, BvhParams, Launch, PbrtError, call, lib
 
 __all__ = ["Context", "DeviceArray", "TriangleMesh", "PbrtError", "icosphere", "pinhole_rays", "orbit_rays",
-           "random_rays", "RAY_DTYPE", "HIT_DTYPE", "REFERENCE_BVH", "DEVICE_BVH"]
+           "random_rays", "RAY_DTYPE", "HIT_DTYPE", "REFERENCE_BVH", "DEVICE_BVH", "RadianceCache", "bf16_bits",
+           "bf16_float", "mfma_check", "NRC_PARAMS", "NRC_FEATURES"]
 
 # pbrt_ray / pbrt_hit as numpy record layouts (32 and 16 bytes)
 RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"), ("dx", "<f4"), ("dy", "<f4"),
@@ -370,6 +371,123 @@ class Scene:
         st = _native.RenderStats()
         call("pbrt_render", self.h, ctypes.byref(c), _ptr(out) if film else None, ctypes.byref(st))
         return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds)
+
+
+NRC_PARAMS = 17923
+NRC_FEATURES = 96
+
+
+class RadianceCache:
+    """Neural radiance cache (pbrt_nrc): an 81 -> 64 x 4 -> 3 ReLU MLP on the matrix cores, trained
+    online with a relative L2 loss and Adam. No counterpart in the reference (its integrator is plain
+    path tracing, src/integrators/path.rs). Device-pointer entry points (`encode`, `infer`,
+    `train_step`) take DeviceArrays; the `*_host` helpers copy numpy arrays in and out."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        call("pbrt_nrc_create", ctx.h, seed, ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pbrt_nrc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_bounds(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        call("pbrt_nrc_set_bounds", self.h, _ptr(lo), _ptr(hi))
+
+    def set_optim(self, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
+        call("pbrt_nrc_set_optim", self.h, ctypes.byref(_native.NrcOptim(lr, beta1, beta2, eps)))
+
+    @property
+    def params(self):
+        out = np.empty(NRC_PARAMS, np.float32)
+        call("pbrt_nrc_get_params", self.h, _ptr(out), NRC_PARAMS)
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.shape != (NRC_PARAMS,):
+            raise ValueError(f"expected {NRC_PARAMS} parameters")
+        call("pbrt_nrc_set_params", self.h, _ptr(p), NRC_PARAMS)
+
+    @property
+    def steps(self):
+        s = ctypes.c_uint64()
+        call("pbrt_nrc_step_count", self.h, ctypes.byref(s))
+        return s.value
+
+    def encode(self, raw, n, x, stream=None):
+        call("pbrt_nrc_encode", self.h, raw.ptr, n, x.ptr, stream)
+
+    def infer(self, x, n, out, stream=None):
+        call("pbrt_nrc_infer", self.h, x.ptr, n, out.ptr, stream)
+
+    def train_step(self, x, targets, n, loss=None, stream=None):
+        call("pbrt_nrc_train_step", self.h, x.ptr, targets.ptr, n, loss.ptr if loss is not None else None, stream)
+
+    def gradients(self, x, targets, n):
+        g = np.empty(NRC_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        call("pbrt_nrc_gradients", self.h, x.ptr, targets.ptr, n, _ptr(g), ctypes.byref(loss))
+        return loss.value, g
+
+    # host-array conveniences
+    def encode_host(self, raw12):
+        raw = np.ascontiguousarray(raw12, np.float32).reshape(-1, 12)
+        n = len(raw)
+        dr = self.ctx.upload(raw)
+        dx = self.ctx.alloc(max(n, 1) * NRC_FEATURES * 2)
+        self.encode(dr, n, dx)
+        return dx.read(np.uint16, n * NRC_FEATURES).reshape(n, NRC_FEATURES)
+
+    def infer_host(self, x_bits):
+        x = np.ascontiguousarray(x_bits, np.uint16).reshape(-1, NRC_FEATURES)
+        n = len(x)
+        dx = self.ctx.upload(x)
+        do = self.ctx.alloc(max(n, 1) * 12)
+        self.infer(dx, n, do)
+        return do.read(np.float32, n * 3).reshape(n, 3)
+
+    def gradients_host(self, x_bits, targets):
+        x = np.ascontiguousarray(x_bits, np.uint16).reshape(-1, NRC_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        return self.gradients(self.ctx.upload(x), self.ctx.upload(t), len(x))
+
+    def train_step_host(self, x_bits, targets):
+        x = np.ascontiguousarray(x_bits, np.uint16).reshape(-1, NRC_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        dl = self.ctx.alloc(4)
+        self.train_step(self.ctx.upload(x), self.ctx.upload(t), len(x), dl)
+        return float(dl.read(np.float32, 1)[0])
+
+
+def bf16_bits(x):
+    """FP32 -> BF16 bit patterns, round to nearest even (NaN and infinities kept)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    special = (u & 0x7F800000) == 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return np.where(special, (u >> 16).astype(np.uint16), r)
+
+
+def bf16_float(bits):
+    return (np.asarray(bits, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def mfma_check(ctx, a_bits, b_bits):
+    """D = A B on one 16x16x32 BF16 matrix-core product (A 16x32, B 32x16 bit patterns)."""
+    a = np.ascontiguousarray(a_bits, np.uint16).reshape(16, 32)
+    b = np.ascontiguousarray(b_bits, np.uint16).reshape(32, 16)
+    d = np.empty((16, 16), np.float32)
+    call("pbrt_nrc_mfma_check", ctx.h, _ptr(a), _ptr(b), _ptr(d))
+    return d
 
 
 def to_rgb8(film):

@@ -41,6 +41,10 @@ class RenderStats(ctypes.Structure):
     _fields_ = [("rays", U64), ("paths", U64), ("passes", U64), ("seconds", F64)]
 
 
+class NrcOptim(ctypes.Structure):
+    _fields_ = [("lr", F32), ("beta1", F32), ("beta2", F32), ("eps", F32)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "pbrt_last_error": [],
@@ -86,6 +90,18 @@ SIGNATURES = {
     "pbrt_scene_trace_any": [P, P, U32, P, P],
     "pbrt_scene_status": [P, ctypes.POINTER(I32)],
     "pbrt_render": [P, P, P, P],
+    "pbrt_nrc_create": [P, U64, PP],
+    "pbrt_nrc_destroy": [P],
+    "pbrt_nrc_set_bounds": [P, P, P],
+    "pbrt_nrc_set_optim": [P, ctypes.POINTER(NrcOptim)],
+    "pbrt_nrc_set_params": [P, P, U32],
+    "pbrt_nrc_get_params": [P, P, U32],
+    "pbrt_nrc_encode": [P, P, U32, P, P],
+    "pbrt_nrc_infer": [P, P, U32, P, P],
+    "pbrt_nrc_gradients": [P, P, P, U32, P, ctypes.POINTER(F32)],
+    "pbrt_nrc_train_step": [P, P, P, U32, P, P],
+    "pbrt_nrc_step_count": [P, ctypes.POINTER(U64)],
+    "pbrt_nrc_mfma_check": [P, P, P, P],
     "pbrt_pinhole_rays": [U32, U32, F64, P, P, F32, F32, P],
     "pbrt_orbit_rays": [P, U32, U32, U32, F64, F32, F32, P],
     "pbrt_random_rays": [P, F64, U64, U64, U32, F32, F32, P],

@@ -1,0 +1,476 @@
+// Neural radiance cache on CDNA4 (gfx950): a 81 -> 64 -> 64 -> 64 -> 64 -> 3 ReLU MLP whose
+// inference, backpropagation and Adam update are hand-written here, with BF16 operands on the
+// v_mfma_f32_16x16x32_bf16 matrix cores and FP32 accumulation. The reference renderer has no
+// radiance cache; this follows Müller et al., "Real-time Neural Radiance Caching for Path
+// Tracing" (2021): relative L2 loss, online training, paths terminated into the cache.
+//
+// Data layout (padded so every matrix product is a whole number of 16 x 16 x 32 MFMA tiles):
+//   inputs   N x 96 BF16 (81 features, zero padded), row-major
+//   forward  Wf0 [64][96], Wf1..3 [64][64], Wf4 [16][64] (rows 3.. zero), [out][in] row-major
+//   backward Wt1..3 [64][64], Wt4 [64][32] ([in][out], out padded with zeros)
+//   biases   b0..b3 [64], b4 [16] (FP32)
+// MFMA operand layout (16x16x32): lane l feeds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16] for j = 0..7, and receives D[4 (l / 16) + i][l % 16] for i = 0..3.
+#pragma clang fp contract(off)
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nrc.hpp"
+
+namespace pbrt {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+union Pack8 {
+  uint16_t u[8];
+  uint4 q;
+  bf16x8 v;
+};
+
+// 8 consecutive BF16 values (16-byte aligned)
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+  Pack8 x;
+  x.q = *reinterpret_cast<const uint4*>(p);
+  return x.v;
+}
+// 8 values `stride` elements apart
+__device__ __forceinline__ bf16x8 gather8(const uint16_t* p, int stride) {
+  Pack8 x;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x.u[j] = p[j * stride];
+  return x.v;
+}
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// LDS written by some lanes of a wave and then read by others of the same wave: order the accesses
+// (no code motion across, stores visible before the loads).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// sum over the four lanes l, l ^ 16, l ^ 32, l ^ 48 (same MFMA output column), fixed order
+__device__ __forceinline__ float col_sum(float s) {
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s;
+}
+
+constexpr int IN = kNrcIn, INP = kNrcInPad, H = kNrcHidden, OUTP = 16, OUTK = 32;
+
+// One layer of the forward pass for the 16 rows of a wave: out[r][c] = act(sum_k in[r][k] W[c][k]
+// + b[c]), `in` in LDS (row stride in_stride), W the forward weights in HBM.
+template <int K, int NOUT, bool RELU>
+__device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, const uint16_t* __restrict__ W,
+                                          const float* __restrict__ b, uint16_t* out, int out_stride, int lane,
+                                          f32x4* keep) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int nb = 0; nb < NOUT / 16; ++nb) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < K / 32; ++kk) {
+      const bf16x8 a = load8(in + r * in_stride + kk * 32 + g * 8);
+      const bf16x8 bb = load8(W + (nb * 16 + r) * K + kk * 32 + g * 8);
+      acc = mfma(a, bb, acc);
+    }
+    const int col = nb * 16 + r;
+    const float bias = b[col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float v = acc[i] + bias;
+      if (RELU) v = v > 0.f ? v : 0.f;
+      acc[i] = v;
+      if (out) out[(g * 4 + i) * out_stride + col] = bf16_bits(v);
+    }
+    if (keep) keep[nb] = acc;
+  }
+}
+
+// Inference: rows of `x` (N x 96 BF16) -> out (N x 3 FP32). One wave per 16-row tile, grid-stride.
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcDevice P, const uint16_t* __restrict__ x, float* __restrict__ out,
+                                                   uint32_t n) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t tiles = (n + 15) / 16;
+  for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+    const uint32_t row0 = t * 16;
+    // layer 0 reads its A operand (8 consecutive BF16 of one input row) straight from HBM
+    const int r = lane & 15, g = lane >> 4;
+    const uint32_t row = row0 + r;
+    uint16_t* h0 = act[wave][0];
+    uint16_t* h1 = act[wave][1];
+    {
+#pragma unroll
+      for (int nb = 0; nb < H / 16; ++nb) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < INP / 32; ++kk) {
+          bf16x8 a;
+          if (row < n) {
+            a = load8(x + (size_t)row * INP + kk * 32 + g * 8);
+          } else {
+            Pack8 z;
+            z.q = make_uint4(0, 0, 0, 0);
+            a = z.v;
+          }
+          const bf16x8 bb = load8(P.wf0 + (nb * 16 + r) * INP + kk * 32 + g * 8);
+          acc = mfma(a, bb, acc);
+        }
+        const int col = nb * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float v = acc[i] + P.b0[col];
+          h0[(g * 4 + i) * H + col] = bf16_bits(v > 0.f ? v : 0.f);
+        }
+      }
+    }
+    wave_sync();
+    layer_fwd<H, H, true>(h0, H, P.wf1, P.b1, h1, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(h1, H, P.wf2, P.b2, h0, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(h0, H, P.wf3, P.b3, h1, H, lane, nullptr);
+    wave_sync();
+    f32x4 y;
+    layer_fwd<H, OUTP, false>(h1, H, P.wf4, P.b4, nullptr, 0, lane, &y);
+    if (r < 3) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t orow = row0 + g * 4 + i;
+        if (orow < n) out[(size_t)orow * 3 + r] = y[i];
+      }
+    }
+    wave_sync();
+  }
+}
+
+// Training: forward, relative-L2 loss gradient and backward over 64-row tiles (4 waves x 16 rows),
+// weight gradients accumulated in MFMA registers across every tile the block processes, then one
+// partial (padded layout, FP32) per block. Deterministic: fixed tile order, fixed reduction order.
+struct TrainLds {
+  uint16_t x[64 * INP];
+  uint16_t h[4][64 * H];   // post-ReLU activations of layers 0..3
+  uint16_t dz[2][64 * H];  // BF16 gradients at a layer's pre-activation (ping-pong)
+  uint16_t dout[64 * OUTK];
+  float db[4][5][H];       // per-wave bias gradient partials
+  float loss[4];
+};
+
+// dW tile (rows m0.. of the output dim, cols n0.. of the input dim) += dz^T act over the tile's 64 rows
+__device__ __forceinline__ f32x4 dw_tile(const uint16_t* dz, int dz_stride, const uint16_t* act, int act_stride, int m0,
+                                         int n0, int lane, f32x4 acc) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const bf16x8 a = gather8(dz + (kk * 32 + g * 8) * dz_stride + m0 + r, dz_stride);
+    const bf16x8 b = gather8(act + (kk * 32 + g * 8) * act_stride + n0 + r, act_stride);
+    acc = mfma(a, b, acc);
+  }
+  return acc;
+}
+
+// Backward through W_l for the wave's 16 rows: da = dz W (K = the layer's output dim), times
+// ReLU'(a) from the stored activation; writes BF16 dz of the previous layer and sums its FP32
+// values into the bias partial.
+template <int KOUT>
+__device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, const uint16_t* __restrict__ Wt,
+                                          const uint16_t* act, uint16_t* dz_out, float* dbp, int row0, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int nb = 0; nb < H / 16; ++nb) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < KOUT / 32; ++kk) {
+      const bf16x8 a = load8(dz_in + (row0 + r) * dz_stride + kk * 32 + g * 8);
+      const bf16x8 b = load8(Wt + (nb * 16 + r) * KOUT + kk * 32 + g * 8);
+      acc = mfma(a, b, acc);
+    }
+    const int col = nb * 16 + r;
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = row0 + g * 4 + i;
+      const float v = bf16_float(act[row * H + col]) > 0.f ? acc[i] : 0.f;
+      s += v;
+      dz_out[row * H + col] = bf16_bits(v);
+    }
+    s = col_sum(s);
+    if (g == 0) dbp[col] += s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* __restrict__ x,
+                                                   const float* __restrict__ target, uint32_t n, float inv_count,
+                                                   float* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) TrainLds L;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+  for (int i = threadIdx.x; i < 4 * 5 * H; i += 256) (&L.db[0][0][0])[i] = 0.f;
+  for (int i = threadIdx.x; i < 64 * OUTK; i += 256) L.dout[i] = 0;
+  if (threadIdx.x < 4) L.loss[threadIdx.x] = 0.f;
+  // weight-gradient accumulators, tile t of a layer owned by wave t % 4
+  f32x4 gw0[6], gw1[4], gw2[4], gw3[4], gw4;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gw0[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) gw1[i] = gw2[i] = gw3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  gw4 = f32x4{0.f, 0.f, 0.f, 0.f};
+  float loss_acc = 0.f;
+  __syncthreads();
+  const uint32_t tiles = (n + 63) / 64;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t base = t * 64;
+    // stage the 64 input rows (16-byte chunks), zeros past the end
+    for (int c = threadIdx.x; c < 64 * INP / 8; c += 256) {
+      const int row = c / (INP / 8), k8 = c % (INP / 8);
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (base + row < n) v = *reinterpret_cast<const uint4*>(x + (size_t)(base + row) * INP + k8 * 8);
+      *reinterpret_cast<uint4*>(L.x + row * INP + k8 * 8) = v;
+    }
+    __syncthreads();
+    const int row0 = wave * 16;
+    // forward (the wave's own rows)
+    layer_fwd<INP, H, true>(L.x + row0 * INP, INP, P.wf0, P.b0, L.h[0] + row0 * H, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(L.h[0] + row0 * H, H, P.wf1, P.b1, L.h[1] + row0 * H, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(L.h[1] + row0 * H, H, P.wf2, P.b2, L.h[2] + row0 * H, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(L.h[2] + row0 * H, H, P.wf3, P.b3, L.h[3] + row0 * H, H, lane, nullptr);
+    wave_sync();
+    f32x4 y;
+    layer_fwd<H, OUTP, false>(L.h[3] + row0 * H, H, P.wf4, P.b4, nullptr, 0, lane, &y);
+    // relative L2 loss and its gradient at the outputs
+    {
+      float s = 0.f;
+      if (r < 3) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = row0 + g * 4 + i;
+          float dz = 0.f;
+          if (base + row < n) {
+            const float tv = target[(size_t)(base + row) * 3 + r];
+            const float yv = y[i];
+            const float den = yv * yv + 0.01f;
+            const float diff = yv - tv;
+            loss_acc += diff * diff / den * inv_count;
+            dz = 2.0f * diff / den * inv_count;
+          }
+          s += dz;
+          L.dout[row * OUTK + r] = bf16_bits(dz);
+        }
+      }
+      s = col_sum(s);
+      if (g == 0 && r < 3) L.db[wave][4][r] += s;
+    }
+    __syncthreads();
+    // layer 4: dW4 (one 16-row tile x four 16-col tiles: wave w owns col tile w), then dz3 = dout W4 * relu'
+    gw4 = dw_tile(L.dout, OUTK, L.h[3], H, 0, wave * 16, lane, gw4);
+    back_rows<OUTK>(L.dout, OUTK, P.wt4, L.h[3], L.dz[0], L.db[wave][3], row0, lane);
+    __syncthreads();
+    // layer 3
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int tile = wave + 4 * i;
+      gw3[i] = dw_tile(L.dz[0], H, L.h[2], H, (tile / 4) * 16, (tile % 4) * 16, lane, gw3[i]);
+    }
+    back_rows<H>(L.dz[0], H, P.wt3, L.h[2], L.dz[1], L.db[wave][2], row0, lane);
+    __syncthreads();
+    // layer 2
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int tile = wave + 4 * i;
+      gw2[i] = dw_tile(L.dz[1], H, L.h[1], H, (tile / 4) * 16, (tile % 4) * 16, lane, gw2[i]);
+    }
+    back_rows<H>(L.dz[1], H, P.wt2, L.h[1], L.dz[0], L.db[wave][1], row0, lane);
+    __syncthreads();
+    // layer 1
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int tile = wave + 4 * i;
+      gw1[i] = dw_tile(L.dz[0], H, L.h[0], H, (tile / 4) * 16, (tile % 4) * 16, lane, gw1[i]);
+    }
+    back_rows<H>(L.dz[0], H, P.wt1, L.h[0], L.dz[1], L.db[wave][0], row0, lane);
+    __syncthreads();
+    // layer 0: dW0 = dz0^T x (4 row tiles x 6 col tiles)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int tile = wave + 4 * i;
+      gw0[i] = dw_tile(L.dz[1], H, L.x, INP, (tile / 6) * 16, (tile % 6) * 16, lane, gw0[i]);
+    }
+    __syncthreads();
+  }
+  // write this block's partial in the padded layout
+  float* out = partial + (size_t)blockIdx.x * kNrcPartialStride;
+  auto put = [&](int off, int ncols, int m0, int n0, f32x4 v) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[off + (m0 + g * 4 + i) * ncols + n0 + r] = v[i];
+  };
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const int tile = wave + 4 * i;
+    put(kNrcOffW0, INP, (tile / 6) * 16, (tile % 6) * 16, gw0[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int tile = wave + 4 * i;
+    put(kNrcOffW1, H, (tile / 4) * 16, (tile % 4) * 16, gw1[i]);
+    put(kNrcOffW2, H, (tile / 4) * 16, (tile % 4) * 16, gw2[i]);
+    put(kNrcOffW3, H, (tile / 4) * 16, (tile % 4) * 16, gw3[i]);
+  }
+  put(kNrcOffW4, H, 0, wave * 16, gw4);
+  // loss: reduce over the wave's lanes in a fixed order
+  for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
+  if (lane == 0) L.loss[wave] = loss_acc;
+  __syncthreads();
+  for (int i = threadIdx.x; i < 5 * H; i += 256) {
+    const int l = i / H, c = i % H;  // bias of layer l (0..3 hidden, 4 output), summed over the waves
+    const float s = ((L.db[0][l][c] + L.db[1][l][c]) + L.db[2][l][c]) + L.db[3][l][c];
+    out[kNrcOffB + l * H + c] = s;
+  }
+  if (threadIdx.x == 0) out[kNrcOffLoss] = ((L.loss[0] + L.loss[1]) + L.loss[2]) + L.loss[3];
+}
+
+// grads (padded layout) = sum over blocks in block order; loss likewise.
+__global__ void k_nrc_reduce(const float* __restrict__ partial, uint32_t blocks, float* __restrict__ grads) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= (uint32_t)kNrcOffLoss; i += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (uint32_t b = 0; b < blocks; ++b) s += partial[(size_t)b * kNrcPartialStride + i];
+    grads[i] = s;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the flat FP32 parameters, then the BF16 forward / backward copies.
+__global__ void k_nrc_adam(NrcDevice P, const float* __restrict__ grads, float lr, float b1, float b2, float eps,
+                           float c1, float c2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNrcParams; i += gridDim.x * blockDim.x) {
+    const float gr = grads[nrc_padded_index(i)];
+    const float m = b1 * P.m[i] + (1.0f - b1) * gr;
+    const float v = b2 * P.v[i] + (1.0f - b2) * gr * gr;
+    P.m[i] = m;
+    P.v[i] = v;
+    const float mh = m / c1, vh = v / c2;
+    P.master[i] = P.master[i] - lr * mh / (sqrtf(vh) + eps);
+  }
+}
+
+__global__ void k_nrc_pack(NrcDevice P) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNrcParams; i += gridDim.x * blockDim.x) {
+    const float w = P.master[i];
+    int layer, o, k;
+    const bool is_bias = nrc_locate(i, &layer, &o, &k);
+    if (is_bias) {
+      float* b = layer == 0 ? P.b0 : layer == 1 ? P.b1 : layer == 2 ? P.b2 : layer == 3 ? P.b3 : P.b4;
+      b[o] = w;
+      continue;
+    }
+    const uint16_t q = bf16_bits(w);
+    switch (layer) {
+      case 0: P.wf0[o * INP + k] = q; break;
+      case 1: P.wf1[o * H + k] = q; P.wt1[k * H + o] = q; break;
+      case 2: P.wf2[o * H + k] = q; P.wt2[k * H + o] = q; break;
+      case 3: P.wf3[o * H + k] = q; P.wt3[k * H + o] = q; break;
+      default: P.wf4[o * H + k] = q; P.wt4[k * OUTK + o] = q; break;
+    }
+  }
+}
+
+// 12 raw floats per sample (position, outgoing direction, normal, albedo) -> 96 BF16 features
+// (81 used, zero padded); p = (x - lo) * (1 / (hi - lo)) in FP32.
+__global__ void k_nrc_encode(const float* __restrict__ raw, uint32_t n, float3 lo, float3 inv_ext,
+                             uint16_t* __restrict__ x) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const float* q = raw + (size_t)s * 12;
+    uint16_t* o = x + (size_t)s * INP;
+    const float p[3] = {(q[0] - lo.x) * inv_ext.x, (q[1] - lo.y) * inv_ext.y, (q[2] - lo.z) * inv_ext.z};
+    int j = 0;
+    for (int a = 0; a < 3; ++a) {
+      // 2^k p is exact in FP32; reduce it modulo 2 (exact) and evaluate sin / cos of pi times it
+      float f = p[a];
+      for (int k = 0; k < kNrcFreq; ++k) {
+        const float red = f - 2.0f * floorf(f * 0.5f);
+        o[j++] = bf16_bits(sinpif(red));
+        o[j++] = bf16_bits(cospif(red));
+        f = f * 2.0f;
+      }
+    }
+    for (int i = 3; i < 12; ++i) o[j++] = bf16_bits(q[i]);
+    for (; j < INP; ++j) o[j] = 0;
+  }
+}
+
+// The MFMA operand layout this file assumes, exercised on one 16x16x32 product.
+__global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, float* __restrict__ D) {
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  const bf16x8 a = load8(A + r * 32 + g * 8);
+  const bf16x8 b = gather8(B + (g * 8) * 16 + r, 16);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = mfma(a, b, acc);
+  for (int i = 0; i < 4; ++i) D[(g * 4 + i) * 16 + r] = acc[i];
+}
+
+}  // namespace
+
+hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, uint32_t cu, void* s) {
+  if (!n) return hipSuccess;
+  const uint32_t tiles = (n + 15) / 16;
+  uint32_t blocks = (tiles + 3) / 4;
+  const uint32_t cap = cu * 4;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, out, n);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n, uint32_t blocks,
+                            float* partial, float* grads, void* s) {
+  hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, target, n,
+                     1.0f / (3.0f * (float)n), partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcPartialStride + 255) / 256), dim3(256), 0, (hipStream_t)s, partial,
+                     blocks, grads);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,
+                           void* s) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, (hipStream_t)s, P, grads, lr, b1, b2,
+                     eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_pack(const NrcDevice& P, void* s) {
+  hipLaunchKernelGGL(k_nrc_pack, dim3((kNrcParams + 255) / 256), dim3(256), 0, (hipStream_t)s, P);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_encode(const float* raw, uint32_t n, const float lo[3], const float hi[3], uint16_t* x, void* s) {
+  if (!n) return hipSuccess;
+  const float3 l = make_float3(lo[0], lo[1], lo[2]);
+  const float3 ie = make_float3(1.0f / (hi[0] - lo[0]), 1.0f / (hi[1] - lo[1]), 1.0f / (hi[2] - lo[2]));
+  uint32_t blocks = (n + 255) / 256;
+  if (blocks > 65535) blocks = 65535;
+  hipLaunchKernelGGL(k_nrc_encode, dim3(blocks), dim3(256), 0, (hipStream_t)s, raw, n, l, ie, x);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_probe(const uint16_t* A, const uint16_t* B, float* D, void* s) {
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)s, A, B, D);
+  return hipGetLastError();
+}
+
+}  // namespace pbrt

@@ -1,0 +1,88 @@
+// Neural radiance cache: sizes, padded device layouts and the launch helpers of nrc.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pbrt {
+
+constexpr int kNrcIn = 81;      // 3 axes x 12 frequencies x (sin, cos) + direction + normal + albedo
+constexpr int kNrcInPad = 96;   // three K = 32 MFMA steps
+constexpr int kNrcHidden = 64;
+constexpr int kNrcOut = 3;
+constexpr int kNrcFreq = 12;
+constexpr int kNrcLayers = 5;
+constexpr int kNrcParams = (kNrcIn * 64 + 64) + 3 * (64 * 64 + 64) + (64 * 3 + 3);  // 17923
+
+// Padded FP32 gradient layout (also one block's partial): W0 [64][96], W1..W3 [64][64], W4 [16][64],
+// biases [5][64], the loss, padded to a multiple of 64 floats.
+constexpr int kNrcOffW0 = 0;
+constexpr int kNrcOffW1 = kNrcOffW0 + 64 * kNrcInPad;
+constexpr int kNrcOffW2 = kNrcOffW1 + 64 * 64;
+constexpr int kNrcOffW3 = kNrcOffW2 + 64 * 64;
+constexpr int kNrcOffW4 = kNrcOffW3 + 64 * 64;
+constexpr int kNrcOffB = kNrcOffW4 + 16 * 64;
+constexpr int kNrcOffLoss = kNrcOffB + 5 * 64;
+constexpr int kNrcPartialStride = (kNrcOffLoss + 1 + 63) / 64 * 64;
+
+// Position of flat parameter i (per layer: W [out][in] row-major, then b [out]). Returns true for
+// a bias (then *k is 0).
+__host__ __device__ inline bool nrc_locate(int i, int* layer, int* o, int* k) {
+  int off = 0;
+  for (int l = 0; l < kNrcLayers; ++l) {
+    const int ni = l == 0 ? kNrcIn : kNrcHidden, no = l == kNrcLayers - 1 ? kNrcOut : kNrcHidden;
+    if (i < off + ni * no) {
+      *layer = l;
+      *o = (i - off) / ni;
+      *k = (i - off) % ni;
+      return false;
+    }
+    off += ni * no;
+    if (i < off + no) {
+      *layer = l;
+      *o = i - off;
+      *k = 0;
+      return true;
+    }
+    off += no;
+  }
+  *layer = -1;
+  *o = *k = 0;
+  return false;
+}
+
+__host__ __device__ inline int nrc_padded_index(int i) {
+  int l, o, k;
+  if (nrc_locate(i, &l, &o, &k)) return kNrcOffB + l * 64 + o;
+  switch (l) {
+    case 0: return kNrcOffW0 + o * kNrcInPad + k;
+    case 1: return kNrcOffW1 + o * 64 + k;
+    case 2: return kNrcOffW2 + o * 64 + k;
+    case 3: return kNrcOffW3 + o * 64 + k;
+    default: return kNrcOffW4 + o * 64 + k;
+  }
+}
+
+// Device views of the network (one allocation). BF16 values are stored as their 16-bit patterns.
+struct NrcDevice {
+  uint16_t *wf0, *wf1, *wf2, *wf3, *wf4;  // forward, [out][in]: [64][96], [64][64] x 3, [16][64]
+  uint16_t *wt1, *wt2, *wt3, *wt4;        // backward, [in][out]: [64][64] x 3, [64][32]
+  float *b0, *b1, *b2, *b3, *b4;          // [64] x 4, [16]
+  float *master, *m, *v;                  // FP32 parameters and Adam moments, flat order
+};
+
+// Bytes of one NrcDevice allocation and the carving of it.
+size_t nrc_device_bytes();
+NrcDevice nrc_carve(void* base);
+
+hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, uint32_t cu, void* stream);
+hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n, uint32_t blocks,
+                            float* partial, float* grads, void* stream);
+hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,
+                           void* stream);
+hipError_t nrc_launch_pack(const NrcDevice& P, void* stream);
+hipError_t nrc_launch_encode(const float* raw, uint32_t n, const float lo[3], const float hi[3], uint16_t* x,
+                             void* stream);
+hipError_t nrc_launch_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream);
+
+}  // namespace pbrt
