@@ -52,6 +52,22 @@ inline float bf16_from_bits(uint16_t b) {
   return r;
 }
 
+// sin(pi x) for x = 2^k p with p an FP32 value: x is exact in FP64, the period and the symmetries
+// sin(pi (x - 1)) = -sin(pi x), sin(pi (1 - x)) = sin(pi x) reduce it exactly to [0, 1/2], where
+// one FP64 sin is taken (so the exact zeros and ones of the function come out exact).
+inline double sinpi(double x) {
+  x = std::fmod(x, 2.0);
+  if (x < 0.0) x += 2.0;
+  double sign = 1.0;
+  if (x >= 1.0) {
+    x -= 1.0;
+    sign = -1.0;
+  }
+  if (x > 0.5) x = 1.0 - x;
+  return sign * std::sin(M_PI * x);
+}
+inline double cospi(double x) { return sinpi(x + 0.5); }
+
 // The 81 input features of one sample from position, outgoing direction, normal and albedo. The
 // position is normalised by the scene box in FP32, p_a = (x_a - lo_a) * (1 / (hi_a - lo_a)), and
 // that FP32 value is encoded exactly: for each axis a, for each frequency k, sin(2^k pi p_a) and
@@ -62,9 +78,9 @@ inline void encode(const float* raw12, const float* lo, const float* hi, float* 
     const float inv = 1.0f / (hi[a] - lo[a]);
     const float p = (raw12[a] - lo[a]) * inv;
     for (int k = 0; k < kFreq; ++k) {
-      const double arg = std::ldexp(M_PI, k) * (double)p;
-      out81[j++] = (float)std::sin(arg);
-      out81[j++] = (float)std::cos(arg);
+      const double x = std::ldexp((double)p, k);
+      out81[j++] = (float)sinpi(x);
+      out81[j++] = (float)cospi(x);
     }
   }
   for (int i = 3; i < 12; ++i) out81[j++] = raw12[i];

@@ -54,8 +54,9 @@ def test_encoding_matches_oracle(gpu_ctx):
     ref = pad_features(po.nrc_encode(raw, lo, hi))
     assert (dev[:, 81:] == 0).all()
     diff = np.abs(pbrt_amd.bf16_float(dev).astype(np.float64) - pbrt_amd.bf16_float(ref))
-    step = np.abs(pbrt_amd.bf16_float(ref).astype(np.float64)) * 2.0 ** -7 + 1e-30
-    assert (diff <= step + 2.0 ** -133).all()
+    # one BF16 step of the oracle's value, with a floor of 2^-24 for values at or near an exact zero
+    step = np.maximum(np.abs(pbrt_amd.bf16_float(ref).astype(np.float64)) * 2.0 ** -7, 2.0 ** -24)
+    assert (diff <= step).all(), diff.max()
     assert (dev == ref).mean() >= 0.99
     np.testing.assert_array_equal(dev[:, 72:81], pbrt_amd.bf16_bits(raw[:, 3:]))
 
