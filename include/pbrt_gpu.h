@@ -254,6 +254,44 @@ int pbrt_nrc_step_count(pbrt_nrc* nrc, uint64_t* steps);
  * host BF16 bit patterns in, FP32 out): the operand layout the network's kernels assume. */
 int pbrt_nrc_mfma_check(pbrt_ctx* ctx, const uint16_t* a, const uint16_t* b, float* d);
 
+/* ---- rendering with the radiance cache, and progressive convergence --------------------------- */
+typedef struct pbrt_nrc_render {
+  /* A path that is not a training path ends into the cache at its first non-specular,
+   * non-emissive vertex at or past query_vertex (3: after two bounces); the cache's answer times
+   * the path throughput replaces everything the path would have gathered from there on. */
+  uint32_t query_vertex;
+  /* One path in train_stride of every pass is a training path (33: 3 %), traced to max_depth
+   * without the cache; 0 = no training. Its vertices 1 .. query_vertex give training samples whose
+   * target is the radiance gathered over the next max_depth - query_vertex vertices divided by the
+   * path throughput at the vertex (the same window the cache stands for at query_vertex). */
+  uint32_t train_stride;
+  uint32_t train_batch;  /* samples per optimisation step, at most (65536); spread over the pass */
+  uint32_t train_steps;  /* optimisation steps per pass (1) */
+  /* Optional host output of the last pass's training slots, for inspection: slot s = rank *
+   * query_vertex + (v - 1); the first debug_slots * 12 floats are the raw cache inputs, the next
+   * debug_slots * 4 the target rgb and a validity flag. debug_slots = 0 disables it. */
+  float* debug;
+  uint32_t debug_slots;
+  uint32_t reserved;
+} pbrt_nrc_render;
+
+typedef struct pbrt_progress {
+  const double* reference; /* host width*height*3 film to measure against (NULL: no measure) */
+  double* rmse;            /* host, per pass: RMSE over pixels and channels of the running mean
+                            * against the reference, reduced in FP64 on the device */
+  double* ms;              /* host, per pass: milliseconds from the first launch to the pass's end
+                            * (HIP events on the render stream) */
+  uint32_t capacity;       /* entries available in rmse / ms */
+  uint32_t count;          /* out: passes recorded */
+} pbrt_progress;
+
+/* pbrt_render with the radiance cache (nrc != NULL; one pass per sample of every pixel when
+ * config->samples_per_pass is 0, and the cache trains after every pass) and/or a per-pass
+ * convergence record (progress != NULL). The cache's scene box is set to the scene's bounds.
+ * With nrc == NULL this is pbrt_render. */
+int pbrt_render_ex(pbrt_scene* scene, const pbrt_render_config* config, pbrt_nrc* nrc, const pbrt_nrc_render* nrc_cfg,
+                   pbrt_progress* progress, double* film_out, pbrt_render_stats* stats);
+
 /* ---- synthetic inputs and ray generation (host, f64 as the reference, rays rounded to FP32) ----
  * pinhole camera of src/cameras/pin_hole.rs:17-50 looking from `eye` at `look` (up = +y, fov in
  * radians); writes width*height rays through pixel centres (px + 0.5, py + 0.5), row-major. */

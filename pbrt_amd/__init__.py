@@ -360,17 +360,66 @@ class Scene:
         self.intersect(dr, n, dh)
         return dh.read(SCENE_HIT_DTYPE, n)
 
-    def render(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
-               near=1e-4, far=1000.0, samples_per_pass=0, film=True):
-        """Per-pixel mean radiance (height, width, 3) float64 and the render statistics."""
+    @staticmethod
+    def _config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass):
         c = _native.RenderConfig()
         c.width, c.height, c.spp, c.max_depth, c.seed, c.fov = width, height, spp, max_depth, seed, fov
         c.eye[:], c.look[:], c.up[:] = list(eye), list(look), list(up)
         c.near, c.far, c.samples_per_pass = near, far, samples_per_pass
+        return c
+
+    def render(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
+               near=1e-4, far=1000.0, samples_per_pass=0, film=True):
+        """Per-pixel mean radiance (height, width, 3) float64 and the render statistics."""
+        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass)
         out = np.zeros((height, width, 3), np.float64) if film else None
         st = _native.RenderStats()
         call("pbrt_render", self.h, ctypes.byref(c), _ptr(out) if film else None, ctypes.byref(st))
         return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds)
+
+    def render_ex(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
+                  near=1e-4, far=1000.0, samples_per_pass=0, cache=None, query_vertex=3, train_stride=33,
+                  train_batch=65536, train_steps=1, reference=None, record=True, debug_slots=0):
+        """Render with the radiance cache `cache` (a RadianceCache, or None for plain path tracing)
+        and, with `record`, the per-pass time (ms) and RMSE against `reference` (height, width, 3).
+        Returns (film, stats, progress); progress has `ms`, `rmse` (or None) and `spp` per pass, and
+        `debug` (raw inputs, targets) of the last pass's training slots when debug_slots > 0."""
+        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass)
+        out = np.zeros((height, width, 3), np.float64)
+        st = _native.RenderStats()
+        nc = None
+        dbg = None
+        if cache is not None:
+            nc = _native.NrcRender(query_vertex, train_stride, train_batch, train_steps, None, debug_slots, 0)
+            if debug_slots:
+                dbg = np.zeros(debug_slots * 16, np.float32)
+                nc.debug = dbg.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        prog = None
+        per = samples_per_pass or (1 if cache is not None else max(1, (1 << 22) // (width * height)))
+        per = min(per, spp)
+        passes = (spp + per - 1) // per
+        if record:
+            ms = np.zeros(passes, np.float64)
+            rmse = np.zeros(passes, np.float64)
+            ref = None if reference is None else np.ascontiguousarray(reference, np.float64)
+            if ref is not None and ref.shape != (height, width, 3):
+                raise ValueError("reference must be (height, width, 3)")
+            dp = ctypes.POINTER(ctypes.c_double)
+            prog = _native.Progress(ref.ctypes.data_as(dp) if ref is not None else None, rmse.ctypes.data_as(dp),
+                                    ms.ctypes.data_as(dp), passes, 0)
+        call("pbrt_render_ex", self.h, ctypes.byref(c), cache.h if cache is not None else None,
+             ctypes.byref(nc) if nc is not None else None, ctypes.byref(prog) if prog is not None else None,
+             _ptr(out), ctypes.byref(st))
+        progress = None
+        if record:
+            spp_done = np.minimum(np.arange(1, passes + 1) * per, spp)
+            progress = dict(ms=ms[:prog.count], rmse=rmse[:prog.count] if reference is not None else None,
+                            spp=spp_done[:prog.count])
+        if dbg is not None:
+            progress = progress or {}
+            progress["debug"] = (dbg[:debug_slots * 12].reshape(debug_slots, 12),
+                                 dbg[debug_slots * 12:].reshape(debug_slots, 4))
+        return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds), progress
 
 
 NRC_PARAMS = 17923

@@ -45,6 +45,16 @@ class NrcOptim(ctypes.Structure):
     _fields_ = [("lr", F32), ("beta1", F32), ("beta2", F32), ("eps", F32)]
 
 
+class NrcRender(ctypes.Structure):
+    _fields_ = [("query_vertex", U32), ("train_stride", U32), ("train_batch", U32), ("train_steps", U32),
+                ("debug", ctypes.POINTER(F32)), ("debug_slots", U32), ("reserved", U32)]
+
+
+class Progress(ctypes.Structure):
+    _fields_ = [("reference", ctypes.POINTER(F64)), ("rmse", ctypes.POINTER(F64)), ("ms", ctypes.POINTER(F64)),
+                ("capacity", U32), ("count", U32)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "pbrt_last_error": [],
@@ -90,6 +100,7 @@ SIGNATURES = {
     "pbrt_scene_trace_any": [P, P, U32, P, P],
     "pbrt_scene_status": [P, ctypes.POINTER(I32)],
     "pbrt_render": [P, P, P, P],
+    "pbrt_render_ex": [P, P, P, ctypes.POINTER(NrcRender), ctypes.POINTER(Progress), P, P],
     "pbrt_nrc_create": [P, U64, PP],
     "pbrt_nrc_destroy": [P],
     "pbrt_nrc_set_bounds": [P, P, P],

@@ -16,7 +16,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpbrt_gpu.so")
 
 SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "scene_api.cpp", "nrc_api.cpp", "traverse.hip", "scene.hip",
-           "nrc.hip"]
+           "nrc.hip", "nrc_render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

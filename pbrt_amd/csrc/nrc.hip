@@ -101,9 +101,10 @@ __device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, con
 
 // Inference: rows of `x` (N x 96 BF16) -> out (N x 3 FP32). One wave per 16-row tile, grid-stride.
 __global__ __launch_bounds__(256) void k_nrc_infer(NrcDevice P, const uint16_t* __restrict__ x, float* __restrict__ out,
-                                                   uint32_t n) {
+                                                   uint32_t n, const uint32_t* __restrict__ n_dev) {
   __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (n_dev) n = min(n, *n_dev);  // a batch counted on the device, bounded by the buffers' capacity
   const uint32_t tiles = (n + 15) / 16;
   for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
     const uint32_t row0 = t * 16;
@@ -213,10 +214,12 @@ __device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, 
 }
 
 __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* __restrict__ x,
-                                                   const float* __restrict__ target, uint32_t n, float inv_count,
-                                                   float* __restrict__ partial) {
+                                                   const float* __restrict__ target, uint32_t n,
+                                                   const uint32_t* __restrict__ n_dev, float* __restrict__ partial) {
   __shared__ __attribute__((aligned(16))) TrainLds L;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (n_dev) n = min(n, *n_dev);
+  const float inv_count = n ? 1.0f / (3.0f * (float)n) : 0.0f;  // mean over samples and channels
   const int r = lane & 15, g = lane >> 4;
   for (int i = threadIdx.x; i < 4 * 5 * H; i += 256) (&L.db[0][0][0])[i] = 0.f;
   for (int i = threadIdx.x; i < 64 * OUTK; i += 256) L.dout[i] = 0;
@@ -390,8 +393,9 @@ __global__ void k_nrc_pack(NrcDevice P) {
 
 // 12 raw floats per sample (position, outgoing direction, normal, albedo) -> 96 BF16 features
 // (81 used, zero padded); p = (x - lo) * (1 / (hi - lo)) in FP32.
-__global__ void k_nrc_encode(const float* __restrict__ raw, uint32_t n, float3 lo, float3 inv_ext,
-                             uint16_t* __restrict__ x) {
+__global__ void k_nrc_encode(const float* __restrict__ raw, uint32_t n, const uint32_t* __restrict__ n_dev, float3 lo,
+                             float3 inv_ext, uint16_t* __restrict__ x) {
+  if (n_dev) n = min(n, *n_dev);
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const float* q = raw + (size_t)s * 12;
     uint16_t* o = x + (size_t)s * INP;
@@ -424,20 +428,20 @@ __global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __r
 
 }  // namespace
 
-hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, uint32_t cu, void* s) {
+hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
+                            uint32_t cu, void* s) {
   if (!n) return hipSuccess;
   const uint32_t tiles = (n + 15) / 16;
   uint32_t blocks = (tiles + 3) / 4;
   const uint32_t cap = cu * 4;
   if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, out, n);
+  hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, out, n, n_dev);
   return hipGetLastError();
 }
 
-hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n, uint32_t blocks,
-                            float* partial, float* grads, void* s) {
-  hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, target, n,
-                     1.0f / (3.0f * (float)n), partial);
+hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
+                            const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, void* s) {
+  hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, target, n, n_dev, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcPartialStride + 255) / 256), dim3(256), 0, (hipStream_t)s, partial,
@@ -458,13 +462,14 @@ hipError_t nrc_launch_pack(const NrcDevice& P, void* s) {
   return hipGetLastError();
 }
 
-hipError_t nrc_launch_encode(const float* raw, uint32_t n, const float lo[3], const float hi[3], uint16_t* x, void* s) {
+hipError_t nrc_launch_encode(const float* raw, uint32_t n, const uint32_t* n_dev, const float lo[3], const float hi[3],
+                             uint16_t* x, void* s) {
   if (!n) return hipSuccess;
   const float3 l = make_float3(lo[0], lo[1], lo[2]);
   const float3 ie = make_float3(1.0f / (hi[0] - lo[0]), 1.0f / (hi[1] - lo[1]), 1.0f / (hi[2] - lo[2]));
   uint32_t blocks = (n + 255) / 256;
   if (blocks > 65535) blocks = 65535;
-  hipLaunchKernelGGL(k_nrc_encode, dim3(blocks), dim3(256), 0, (hipStream_t)s, raw, n, l, ie, x);
+  hipLaunchKernelGGL(k_nrc_encode, dim3(blocks), dim3(256), 0, (hipStream_t)s, raw, n, n_dev, l, ie, x);
   return hipGetLastError();
 }
 

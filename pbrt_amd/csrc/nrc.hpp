@@ -4,6 +4,9 @@
 
 #include <cstdint>
 
+#include "../../include/pbrt_gpu.h"
+#include "context.hpp"
+
 namespace pbrt {
 
 constexpr int kNrcIn = 81;      // 3 axes x 12 frequencies x (sin, cos) + direction + normal + albedo
@@ -75,14 +78,36 @@ struct NrcDevice {
 size_t nrc_device_bytes();
 NrcDevice nrc_carve(void* base);
 
-hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, uint32_t cu, void* stream);
-hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n, uint32_t blocks,
-                            float* partial, float* grads, void* stream);
+// `n` is the batch size, or with `n_dev` set the capacity bound of a batch counted on the device.
+hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
+                            uint32_t cu, void* stream);
+hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
+                            const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, void* stream);
 hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,
                            void* stream);
 hipError_t nrc_launch_pack(const NrcDevice& P, void* stream);
-hipError_t nrc_launch_encode(const float* raw, uint32_t n, const float lo[3], const float hi[3], uint16_t* x,
-                             void* stream);
+hipError_t nrc_launch_encode(const float* raw, uint32_t n, const uint32_t* n_dev, const float lo[3], const float hi[3],
+                             uint16_t* x, void* stream);
 hipError_t nrc_launch_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream);
 
+}  // namespace pbrt
+
+// The cache behind the C ABI's opaque handle.
+struct pbrt_nrc {
+  pbrt_ctx* ctx = nullptr;
+  void* base = nullptr;       // NrcDevice allocation
+  pbrt::NrcDevice dev{};
+  float* partial = nullptr;   // per-block gradient partials
+  float* grads = nullptr;     // reduced gradient, padded layout, loss at kNrcOffLoss
+  uint32_t max_blocks = 0;
+  float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
+  pbrt_nrc_optim optim{1e-3f, 0.9f, 0.99f, 1e-8f};
+  uint64_t steps = 0;
+};
+
+namespace pbrt {
+// Gradient over a batch of at most `cap` samples (`n_dev`: the count on the device, or NULL for
+// exactly cap), one Adam step and the BF16 refresh, all on `stream`.
+hipError_t nrc_train_device(pbrt_nrc* n, const uint16_t* x, const float* target, uint32_t cap, const uint32_t* n_dev,
+                            void* stream);
 }  // namespace pbrt

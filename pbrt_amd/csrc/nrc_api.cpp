@@ -13,18 +13,6 @@
 
 using namespace pbrt;
 
-struct pbrt_nrc {
-  pbrt_ctx* ctx = nullptr;
-  void* base = nullptr;       // NrcDevice allocation
-  NrcDevice dev{};
-  float* partial = nullptr;   // per-block gradient partials
-  float* grads = nullptr;     // reduced gradient, padded layout, loss at kNrcOffLoss
-  uint32_t max_blocks = 0;
-  float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
-  pbrt_nrc_optim optim{1e-3f, 0.9f, 0.99f, 1e-8f};
-  uint64_t steps = 0;
-};
-
 namespace pbrt {
 
 namespace {
@@ -69,6 +57,20 @@ NrcDevice nrc_carve(void* base) {
   return d;
 }
 
+hipError_t nrc_train_device(pbrt_nrc* n, const uint16_t* x, const float* target, uint32_t cap, const uint32_t* n_dev,
+                            void* stream) {
+  const uint32_t tiles = (cap + 63) / 64;
+  const uint32_t blocks = tiles < n->max_blocks ? tiles : n->max_blocks;
+  hipError_t e = nrc_launch_train(n->dev, x, target, cap, n_dev, blocks ? blocks : 1, n->partial, n->grads, stream);
+  if (e != hipSuccess) return e;
+  const pbrt_nrc_optim& o = n->optim;
+  e = nrc_launch_adam(n->dev, n->grads, o.lr, o.beta1, o.beta2, o.eps, (int)(n->steps + 1), stream);
+  if (e != hipSuccess) return e;
+  e = nrc_launch_pack(n->dev, stream);
+  if (e == hipSuccess) ++n->steps;
+  return e;
+}
+
 }  // namespace pbrt
 
 namespace {
@@ -103,7 +105,7 @@ void run_gradients(pbrt_nrc* n, const uint16_t* x, const float* target, uint32_t
   check_batch(x, target, count);
   const uint32_t tiles = (count + 63) / 64;
   const uint32_t blocks = tiles < n->max_blocks ? tiles : n->max_blocks;
-  PBRT_HIP(nrc_launch_train(n->dev, x, target, count, blocks, n->partial, n->grads, s));
+  PBRT_HIP(nrc_launch_train(n->dev, x, target, count, nullptr, blocks, n->partial, n->grads, s));
 }
 
 }  // namespace
@@ -200,7 +202,7 @@ int pbrt_nrc_encode(pbrt_nrc* n, const float* raw, uint32_t count, uint16_t* x, 
     check(n);
     if (count && (!raw || !x)) throw Error(PBRT_ERR_INVALID_ARG, "null buffer");
     check_batch(x, raw, count);
-    PBRT_HIP(nrc_launch_encode(raw, count, n->lo, n->hi, x, stream_of(n, stream)));
+    PBRT_HIP(nrc_launch_encode(raw, count, nullptr, n->lo, n->hi, x, stream_of(n, stream)));
   });
 }
 
@@ -209,7 +211,7 @@ int pbrt_nrc_infer(pbrt_nrc* n, const uint16_t* x, uint32_t count, float* out, v
     check(n);
     if (count && (!x || !out)) throw Error(PBRT_ERR_INVALID_ARG, "null buffer");
     check_batch(x, out, count);
-    PBRT_HIP(nrc_launch_infer(n->dev, x, out, count, (uint32_t)n->ctx->cu_count, stream_of(n, stream)));
+    PBRT_HIP(nrc_launch_infer(n->dev, x, out, count, nullptr, (uint32_t)n->ctx->cu_count, stream_of(n, stream)));
   });
 }
 
@@ -233,12 +235,11 @@ int pbrt_nrc_train_step(pbrt_nrc* n, const uint16_t* x, const float* target, uin
   return guarded([&] {
     check(n);
     hipStream_t s = stream_of(n, stream);
-    run_gradients(n, x, target, count, s);
-    const pbrt_nrc_optim& o = n->optim;
-    PBRT_HIP(nrc_launch_adam(n->dev, n->grads, o.lr, o.beta1, o.beta2, o.eps, (int)(n->steps + 1), s));
-    PBRT_HIP(nrc_launch_pack(n->dev, s));
+    if (count == 0) throw Error(PBRT_ERR_INVALID_ARG, "empty training batch");
+    if (!x || !target) throw Error(PBRT_ERR_INVALID_ARG, "null batch");
+    check_batch(x, target, count);
+    PBRT_HIP(nrc_train_device(n, x, target, count, nullptr, s));
     if (loss_dev) PBRT_HIP(hipMemcpyAsync(loss_dev, n->grads + kNrcOffLoss, 4, hipMemcpyDeviceToDevice, s));
-    ++n->steps;
   });
 }
 

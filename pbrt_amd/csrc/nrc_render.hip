@@ -1,0 +1,174 @@
+// The radiance cache inside the wavefront renderer: applying cache answers to the paths that ended
+// into it, turning training-path records into (input, target) samples, choosing a frame's training
+// batch, and the FP64 convergence measure against a reference film.
+#pragma clang fp contract(off)
+#include <hip/hip_runtime.h>
+
+#include "nrc_render.hpp"
+
+namespace pbrt {
+namespace {
+
+// L[j] += beta * max(cache, 0) for every query of the pass (radiance is non-negative)
+__global__ void k_nrc_apply(PathBuffers b, const float* __restrict__ pred, uint32_t cap) {
+  const uint32_t n = min(cap, b.counts[6]);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const float4 bt = b.q_beta[k];
+    const uint32_t j = __float_as_uint(bt.w);
+    const float r = fmaxf(pred[3 * (size_t)k], 0.f), g = fmaxf(pred[3 * (size_t)k + 1], 0.f),
+                bl = fmaxf(pred[3 * (size_t)k + 2], 0.f);
+    float4 L = b.L[j];
+    L.x = L.x + bt.x * r;
+    L.y = L.y + bt.y * g;
+    L.z = L.z + bt.z * bl;
+    b.L[j] = L;
+  }
+}
+
+// Slot (rank, v) for vertices v = 1..V of every training path: the radiance the path gathered over
+// the window the cache stands for at a query vertex — next-event terms at vertices v .. v+W-1 and
+// emission at v+1 .. v+W, summed in the order the path added them — divided by the throughput on
+// arrival at v. Valid when v was recorded as eligible and the throughput is positive.
+__global__ void k_nrc_targets(RenderParams p, PathBuffers b, NrcTrainBuffers t, uint32_t ranks) {
+  const uint32_t V = p.query_vertex, W = p.max_depth - p.query_vertex, D = p.max_depth;
+  const size_t cap = b.train_cap, fs = (size_t)D * cap;
+  const uint32_t slots = ranks * V;
+  const uint64_t npass = (uint64_t)p.width * p.height * p.pass_samples;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += gridDim.x * blockDim.x) {
+    const uint32_t r = s / V, v = s % V + 1;
+    // rank r is path r * stride - offset of this pass; ranks outside the pass hold no record
+    const uint64_t jr = (uint64_t)r * p.train_stride;
+    const bool in_pass = jr >= p.train_offset && jr - p.train_offset < npass;
+    const float4 beta = in_pass ? b.rec[kRecBeta * fs + (size_t)(v - 1) * cap + r] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (beta.w == 3.0f && beta.x > 0.f && beta.y > 0.f && beta.z > 0.f) {
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+      for (uint32_t m = v; m < v + W && m <= D; ++m) {
+        const float4 ne = b.rec[kRecNee * fs + (size_t)(m - 1) * cap + r];
+        sx = sx + ne.x;
+        sy = sy + ne.y;
+        sz = sz + ne.z;
+        if (m + 1 <= D) {
+          const float4 em = b.rec[kRecEmit * fs + (size_t)m * cap + r];
+          sx = sx + em.x;
+          sy = sy + em.y;
+          sz = sz + em.z;
+        }
+      }
+      const float tx = sx / beta.x, ty = sy / beta.y, tz = sz / beta.z;
+      if (isfinite(tx) && isfinite(ty) && isfinite(tz)) out = make_float4(tx, ty, tz, 1.f);
+    }
+    t.slot_t[s] = out;
+    for (int f = 0; f < 3; ++f) t.slot_raw[3 * (size_t)s + f] = b.rec[(kRecRaw + f) * fs + (size_t)(v - 1) * cap + r];
+  }
+}
+
+// Valid slots in slot order (one workgroup, deterministic): idx[0 .. count) and counts[7] = count.
+constexpr int kCompactThreads = 1024;
+__global__ __launch_bounds__(kCompactThreads) void k_nrc_compact(PathBuffers b, NrcTrainBuffers t, uint32_t slots) {
+  __shared__ uint32_t part[kCompactThreads];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = (slots + kCompactThreads - 1) / kCompactThreads;
+  const uint32_t lo = min(slots, tid * chunk), hi = min(slots, lo + chunk);
+  uint32_t c = 0;
+  for (uint32_t s = lo; s < hi; ++s) c += t.slot_t[s].w != 0.f ? 1u : 0u;
+  part[tid] = c;
+  __syncthreads();
+  // inclusive Hillis-Steele scan over the 1024 counts
+  for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
+    const uint32_t add = tid >= off ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += add;
+    __syncthreads();
+  }
+  uint32_t w = part[tid] - c;
+  for (uint32_t s = lo; s < hi; ++s)
+    if (t.slot_t[s].w != 0.f) t.idx[w++] = s;
+  if (tid == kCompactThreads - 1) b.counts[7] = part[tid];
+}
+
+// The frame's training batch: all valid slots when there are at most `batch`, else `batch` of them
+// spread evenly over the slot order. counts[8] = batch size.
+__global__ void k_nrc_gather(PathBuffers b, NrcTrainBuffers t, uint32_t batch) {
+  const uint32_t total = b.counts[7];
+  const uint32_t n = min(total, batch);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t src = total <= batch ? i : (uint32_t)(((uint64_t)i * total) / batch);
+    const uint32_t s = t.idx[src];
+    for (int f = 0; f < 3; ++f) t.batch_raw[3 * (size_t)i + f] = t.slot_raw[3 * (size_t)s + f];
+    const float4 tt = t.slot_t[s];
+    t.batch_t[3 * (size_t)i] = tt.x;
+    t.batch_t[3 * (size_t)i + 1] = tt.y;
+    t.batch_t[3 * (size_t)i + 2] = tt.z;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.counts[8] = n;
+}
+
+// Sum over pixels and channels of (film / k - ref)^2 in FP64: per-workgroup partials in a fixed
+// tree order, then one workgroup adds the partials in order and writes the RMSE.
+constexpr int kRmseThreads = 256;
+__global__ __launch_bounds__(kRmseThreads) void k_rmse_partial(const double* __restrict__ film, double inv_k,
+                                                               const double* __restrict__ ref, uint32_t n,
+                                                               double* __restrict__ partial) {
+  __shared__ double sh[kRmseThreads];
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * kRmseThreads + threadIdx.x; i < n; i += gridDim.x * kRmseThreads) {
+    const double d = film[i] * inv_k - ref[i];
+    acc += d * d;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kRmseThreads / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_rmse_final(const double* __restrict__ partial, uint32_t blocks, uint32_t n, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (uint32_t i = 0; i < blocks; ++i) s += partial[i];
+    *out = sqrt(s / (double)n);
+  }
+}
+
+uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
+  uint32_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return g > cap ? cap : g;
+}
+
+}  // namespace
+
+hipError_t launch_nrc_apply(const PathBuffers& b, const float* pred, uint32_t cap, void* s) {
+  hipLaunchKernelGGL(k_nrc_apply, dim3(grid_for(cap, 256, 4096)), dim3(256), 0, (hipStream_t)s, b, pred, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_nrc_targets(const RenderParams& p, const PathBuffers& b, const NrcTrainBuffers& t, uint32_t ranks,
+                              void* s) {
+  const uint32_t slots = ranks * p.query_vertex;
+  hipLaunchKernelGGL(k_nrc_targets, dim3(grid_for(slots, 256, 4096)), dim3(256), 0, (hipStream_t)s, p, b, t, ranks);
+  return hipGetLastError();
+}
+
+hipError_t launch_nrc_select(const PathBuffers& b, const NrcTrainBuffers& t, uint32_t slots, uint32_t batch, void* s) {
+  hipLaunchKernelGGL(k_nrc_compact, dim3(1), dim3(kCompactThreads), 0, (hipStream_t)s, b, t, slots);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nrc_gather, dim3(grid_for(batch, 256, 1024)), dim3(256), 0, (hipStream_t)s, b, t, batch);
+  return hipGetLastError();
+}
+
+hipError_t launch_rmse(const double* film, double inv_k, const double* ref, uint32_t n, double* partial,
+                       uint32_t partial_cap, double* out, void* s) {
+  const uint32_t blocks = grid_for(n, kRmseThreads, partial_cap);
+  hipLaunchKernelGGL(k_rmse_partial, dim3(blocks), dim3(kRmseThreads), 0, (hipStream_t)s, film, inv_k, ref, n, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_rmse_final, dim3(1), dim3(64), 0, (hipStream_t)s, partial, blocks, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace pbrt

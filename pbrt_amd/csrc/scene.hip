@@ -774,10 +774,17 @@ __device__ __forceinline__ void dnorm(double* v) {
 
 // One camera path per (pixel, sample of this pass): PinHoleCamera::get_ray (pin_hole.rs:42-50) in
 // f64, rounded to FP32, through the pixel jittered by two uniforms (renderers/mt.rs:123-127).
+template <bool NRC>
 __global__ void k_generate(RenderParams p, PathBuffers b) {
   const uint32_t npix = p.width * p.height;
   const uint32_t n = npix * p.pass_samples;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t rank;
+    if (NRC && is_training_path(p, j, &rank)) {
+      for (uint32_t f = 0; f < 3; ++f)
+        for (uint32_t m = 0; m < p.max_depth; ++m)
+          b.rec[((size_t)f * p.max_depth + m) * b.train_cap + rank] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     const uint32_t pixel = j % npix, sample = p.pass_first_sample + j / npix;
     const uint64_t path = (uint64_t)pixel * p.spp + sample;
     const double jx = to_unit(hash4(p.seed, path, 0, kSlotJx)), jy = to_unit(hash4(p.seed, path, 0, kSlotJy));
@@ -800,6 +807,7 @@ __global__ void k_generate(RenderParams p, PathBuffers b) {
     b.counts[0] = n;
     b.counts[1] = 0;
     b.counts[2] = 0;
+    if (NRC) b.counts[6] = 0;
   }
 }
 
@@ -835,6 +843,10 @@ __global__ __launch_bounds__(kSceneBlock) void k_extend(SceneView S, PathBuffers
 
 // One path vertex (path.rs:55-112): emission after a specular bounce, the bounce cut, next-event
 // estimation (its shadow ray queued with its contribution), then scattering into the next queue.
+// With the radiance cache (NRC): a training path records the vertex; any other path ends into the
+// cache at its first non-specular, non-emissive vertex at or past p.query_vertex, queueing the
+// query in place of next-event estimation and scattering.
+template <bool NRC>
 __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
   const uint32_t n = b.counts[q];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -849,15 +861,43 @@ __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
     const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd);
     const DevMaterial m = S.mats[S.objs[h.obj].material];
     const bool specular = beta4.w != 0.0f;
+    V emitted{0.f, 0.f, 0.f};
     if (specular && m.kind == kMatDiffuseLight) {
-      L4.x = L4.x + m.rgb[0] * beta4.x;
-      L4.y = L4.y + m.rgb[1] * beta4.y;
-      L4.z = L4.z + m.rgb[2] * beta4.z;
+      emitted = {m.rgb[0] * beta4.x, m.rgb[1] * beta4.y, m.rgb[2] * beta4.z};
+      L4.x = L4.x + emitted.x;
+      L4.y = L4.y + emitted.y;
+      L4.z = L4.z + emitted.z;
     }
     info.z += 1;
     const uint32_t bounce = info.z;
     b.L[j] = L4;
     b.info[j] = info;
+    if (NRC) {
+      const bool eligible = m.kind == kMatLambertian || m.kind == kMatGGX;
+      uint32_t rank;
+      const bool training = is_training_path(p, j, &rank);
+      if (training || (eligible && bounce >= p.query_vertex && bounce < p.max_depth)) {
+        const V wo = vmul(rd, -1.0f);
+        const float4 raw0 = make_float4(is.p.x, is.p.y, is.p.z, wo.x);
+        const float4 raw1 = make_float4(wo.y, wo.z, is.n.x, is.n.y);
+        const float4 raw2 = make_float4(is.n.z, m.rgb[0], m.rgb[1], m.rgb[2]);
+        if (training) {
+          const size_t cap = b.train_cap, base = (size_t)(bounce - 1) * cap + rank, fs = (size_t)p.max_depth * cap;
+          b.rec[kRecBeta * fs + base] = make_float4(beta4.x, beta4.y, beta4.z, eligible ? 3.0f : 1.0f);
+          b.rec[kRecEmit * fs + base] = make_float4(emitted.x, emitted.y, emitted.z, 0.f);
+          b.rec[(kRecRaw + 0) * fs + base] = raw0;
+          b.rec[(kRecRaw + 1) * fs + base] = raw1;
+          b.rec[(kRecRaw + 2) * fs + base] = raw2;
+        } else {
+          const uint32_t k = atomicAdd(b.counts + 6, 1u);
+          b.q_raw[3 * (size_t)k + 0] = raw0;
+          b.q_raw[3 * (size_t)k + 1] = raw1;
+          b.q_raw[3 * (size_t)k + 2] = raw2;
+          b.q_beta[k] = make_float4(beta4.x, beta4.y, beta4.z, __uint_as_float(j));
+          continue;  // the cache stands for everything this path would gather from here on
+        }
+      }
+    }
     if (bounce == p.max_depth) continue;
     if (S.n_lights > 0) {
       const float u = urand(p.seed, path, bounce, kSlotLight);
@@ -894,9 +934,10 @@ __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
 }
 
 // VisibilityTester::unoccluded (lights.rs:67-69): add the contribution when nothing blocks it.
-template <int DEPTH>
-__global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, PathBuffers b, uint2* __restrict__ spill,
-                                                        uint32_t spill_cap, int* __restrict__ status) {
+template <int DEPTH, bool NRC>
+__global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, RenderParams p, PathBuffers b,
+                                                        uint2* __restrict__ spill, uint32_t spill_cap,
+                                                        int* __restrict__ status) {
   __shared__ uint2 lds[DEPTH][kSceneBlock];
   Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
                                blockIdx.x * kSceneBlock + threadIdx.x, 0};
@@ -911,6 +952,11 @@ __global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, PathBuffers
     L.y = L.y + c.y;
     L.z = L.z + c.z;
     b.L[j] = L;
+    uint32_t rank;
+    if (NRC && is_training_path(p, j, &rank)) {  // one shadow ray per vertex: the vertex's own record
+      const uint32_t m = b.info[j].z;
+      b.rec[((size_t)kRecNee * p.max_depth + (m - 1)) * b.train_cap + rank] = make_float4(c.x, c.y, c.z, 0.f);
+    }
   }
 }
 
@@ -947,7 +993,13 @@ hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit) {
 
 hipError_t launch_generate(const SceneView&, const RenderParams& p, const PathBuffers& b, void* stream) {
   const uint32_t n = p.width * p.height * p.pass_samples;
-  hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
+  hipLaunchKernelGGL(k_generate<false>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_generate_nrc(const RenderParams& p, const PathBuffers& b, void* stream) {
+  const uint32_t n = p.width * p.height * p.pass_samples;
+  hipLaunchKernelGGL(k_generate<true>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
   return hipGetLastError();
 }
 
@@ -964,14 +1016,29 @@ hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2*
 }
 
 hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream) {
-  hipLaunchKernelGGL(k_shade, dim3(blocks_for(b.capacity, 256)), dim3(256), 0, (hipStream_t)stream, v, p, b, q);
+  hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(b.capacity, 256)), dim3(256), 0, (hipStream_t)stream, v, p, b,
+                     q);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream) {
+  hipLaunchKernelGGL(k_shade<true>, dim3(blocks_for(b.capacity, 256)), dim3(256), 0, (hipStream_t)stream, v, p, b,
+                     q);
   return hipGetLastError();
 }
 
 hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream) {
-  hipLaunchKernelGGL((k_shadow<kSceneStack>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, b, spill,
-                     spill_per_thread, status);
+  RenderParams none{};
+  hipLaunchKernelGGL((k_shadow<kSceneStack, false>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, none,
+                     b, spill, spill_per_thread, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, uint2* spill,
+                             uint32_t spill_per_thread, uint32_t grid, int* status, void* stream) {
+  hipLaunchKernelGGL((k_shadow<kSceneStack, true>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, p, b,
+                     spill, spill_per_thread, status);
   return hipGetLastError();
 }
 

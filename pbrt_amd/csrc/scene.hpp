@@ -79,7 +79,18 @@ struct RenderParams {
   uint64_t seed;
   double r2s[16], c2w[16];  // pinhole raster->camera and camera->world (f64, row-major)
   float near, far;
+  // radiance cache (nrc != 0): a path that is not a training path ends into the cache at its first
+  // non-specular, non-emissive vertex at or past `query_vertex`; path j of the pass is a training
+  // path when (j + train_offset) % train_stride == 0 (never if train_stride is 0), traced to
+  // max_depth with its vertices recorded.
+  uint32_t nrc, query_vertex, train_stride, train_offset;
 };
+
+// Per-vertex record of a training path, field-major: rec[(f * max_depth + m - 1) * train_cap + rank]
+// for vertex m (1-based). Fields: throughput on arrival (w: 1 = recorded, 3 = eligible as a cache
+// query point), emission added at m, next-event contribution added at m, and the 12 raw cache
+// inputs (position, outgoing direction, normal, albedo) in three float4.
+enum RecField : uint32_t { kRecBeta = 0, kRecEmit = 1, kRecNee = 2, kRecRaw = 3, kRecFields = 6 };
 
 struct PathBuffers {
   float4* ray_o;   // origin xyz, near
@@ -95,7 +106,20 @@ struct PathBuffers {
   float4* sh_c;    // contribution rgb, path index bits
   double* film;    // width * height * 3, running sum
   uint32_t capacity;
+  // radiance cache: counts[6] = cache queries of the pass
+  float4* q_raw;   // [capacity][3] raw cache inputs of each query
+  float4* q_beta;  // throughput rgb, path index bits in w
+  float4* rec;     // training-path records (see RecField)
+  uint32_t train_cap;
 };
+
+__host__ __device__ inline bool is_training_path(const RenderParams& p, uint32_t j, uint32_t* rank) {
+  if (!p.nrc || p.train_stride == 0) return false;
+  const uint32_t k = j + p.train_offset;
+  if (k % p.train_stride != 0) return false;
+  *rank = k / p.train_stride;
+  return true;
+}
 
 hipError_t launch_generate(const SceneView& v, const RenderParams& p, const PathBuffers& b, void* stream);
 hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
@@ -105,6 +129,10 @@ hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill,
                          uint32_t grid, int* status, void* stream);
 hipError_t launch_accumulate(const RenderParams& p, const PathBuffers& b, void* stream);
 hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream);
+hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream);
+hipError_t launch_shadow_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, uint2* spill,
+                             uint32_t spill_per_thread, uint32_t grid, int* status, void* stream);
+hipError_t launch_generate_nrc(const RenderParams& p, const PathBuffers& b, void* stream);
 
 constexpr uint32_t kSceneBlock = 128;
 constexpr int kSceneStack = 16;

@@ -15,6 +15,9 @@
 #include "camera.hpp"
 #include "common.hpp"
 #include "context.hpp"
+#include "nrc.hpp"
+#include "nrc_render.hpp"
+#include "rng.hpp"
 #include "scene.hpp"
 
 namespace pbrt {
@@ -547,18 +550,58 @@ int pbrt_scene_status(pbrt_scene* s, int* out) {
 // samples traced by the wavefront of scene.hip; film_out receives the per-pixel mean radiance
 // (f64 RGB, row-major), before the reference's gamma and 8-bit quantisation (spectrum.rs:13-30).
 int pbrt_render(pbrt_scene* s, const pbrt_render_config* c, double* film_out, pbrt_render_stats* stats) {
+  return pbrt_render_ex(s, c, nullptr, nullptr, nullptr, film_out, stats);
+}
+
+int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, const pbrt_nrc_render* nc,
+                   pbrt_progress* progress, double* film_out, pbrt_render_stats* stats) {
   return guarded([&] {
     check_scene(s, true);
     if (!c || !c->width || !c->height || !c->spp || !c->max_depth) throw Error(PBRT_ERR_INVALID_ARG, "bad render config");
+    if ((uint64_t)c->width * c->height > (1u << 26)) throw Error(PBRT_ERR_INVALID_ARG, "image larger than 2^26 pixels");
+    pbrt_nrc_render ncfg{3, 33, 65536, 1, nullptr, 0, 0};
+    if (nrc) {
+      if (nrc->ctx != s->ctx) throw Error(PBRT_ERR_INVALID_ARG, "cache and scene on different contexts");
+      if (nc) ncfg = *nc;
+      if (ncfg.query_vertex < 1 || ncfg.query_vertex > c->max_depth)
+        throw Error(PBRT_ERR_INVALID_ARG, "query_vertex must be in [1, max_depth]");
+      if (ncfg.train_batch == 0 || ncfg.train_batch > 65536) throw Error(PBRT_ERR_INVALID_ARG, "train_batch in [1, 65536]");
+    }
     std::lock_guard<std::mutex> lock(s->mu);
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t npix = c->width * c->height;
-    uint32_t per_pass = c->samples_per_pass ? c->samples_per_pass : std::max(1u, (1u << 22) / npix);
+    uint32_t per_pass = c->samples_per_pass ? c->samples_per_pass
+                                            : (nrc ? 1u : std::max(1u, (1u << 22) / npix));
     per_pass = std::min(per_pass, c->spp);
+    if ((uint64_t)npix * per_pass > (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "pass larger than 2^28 paths");
     ensure_paths(s, npix * per_pass);
     hipStream_t st = s->ctx->stream;
-    double* film = nullptr;
-    PBRT_HIP(hipMalloc(&film, (size_t)npix * 3 * sizeof(double)));
+    const uint32_t npass_paths = npix * per_pass;
+    const uint32_t passes_total = (c->spp + per_pass - 1) / per_pass;
+
+    // every device allocation of the call, released on every exit path
+    std::vector<void*> owned;
+    struct Release {
+      std::vector<void*>& v;
+      ~Release() {
+        for (void* q : v) (void)hipFree(q);
+      }
+    } release{owned};
+    auto alloc = [&](size_t bytes) {
+      void* q = nullptr;
+      PBRT_HIP(hipMalloc(&q, bytes ? bytes : 16));
+      owned.push_back(q);
+      return q;
+    };
+    std::vector<hipEvent_t> events;
+    struct ReleaseEvents {
+      std::vector<hipEvent_t>& v;
+      ~ReleaseEvents() {
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+      }
+    } release_events{events};
+
+    double* film = (double*)alloc((size_t)npix * 3 * sizeof(double));
     PBRT_HIP(hipMemsetAsync(film, 0, (size_t)npix * 3 * sizeof(double), st));
     PBRT_HIP(hipMemsetAsync(s->pb.counts, 0, 16 * sizeof(uint32_t), st));
     PathBuffers pb = s->pb;
@@ -578,23 +621,118 @@ int pbrt_render(pbrt_scene* s, const pbrt_render_config* c, double* film_out, pb
     }
     p.near = (float)c->near;
     p.far = (float)c->far;
+
+    // radiance cache buffers
+    NrcTrainBuffers tb{};
+    uint16_t* q_x = nullptr;
+    float* q_out = nullptr;
+    uint16_t* batch_x = nullptr;
+    uint32_t ranks_cap = 0;
+    if (nrc) {
+      p.nrc = 1;
+      p.query_vertex = ncfg.query_vertex;
+      p.train_stride = ncfg.train_stride;
+      pb.q_raw = (float4*)alloc((size_t)npass_paths * 3 * sizeof(float4));
+      pb.q_beta = (float4*)alloc((size_t)npass_paths * sizeof(float4));
+      q_x = (uint16_t*)alloc((size_t)npass_paths * kNrcInPad * 2);
+      q_out = (float*)alloc((size_t)npass_paths * 3 * sizeof(float));
+      if (ncfg.train_stride) {
+        ranks_cap = npass_paths / ncfg.train_stride + 2;
+        pb.train_cap = ranks_cap;
+        pb.rec = (float4*)alloc((size_t)kRecFields * c->max_depth * ranks_cap * sizeof(float4));
+        const size_t slots = (size_t)ranks_cap * ncfg.query_vertex;
+        tb.slot_raw = (float4*)alloc(slots * 3 * sizeof(float4));
+        tb.slot_t = (float4*)alloc(slots * sizeof(float4));
+        tb.idx = (uint32_t*)alloc(slots * sizeof(uint32_t));
+        tb.batch_raw = (float4*)alloc((size_t)ncfg.train_batch * 3 * sizeof(float4));
+        tb.batch_t = (float*)alloc((size_t)ncfg.train_batch * 3 * sizeof(float));
+        batch_x = (uint16_t*)alloc((size_t)ncfg.train_batch * kNrcInPad * 2);
+      }
+      // the cache's encoding box: the scene's bounds, widened by 1 % (+1e-3) on every side
+      float lo[3] = {-1.f, -1.f, -1.f}, hi[3] = {1.f, 1.f, 1.f};
+      if (s->has_tlas)
+        for (int a = 0; a < 3; ++a) {
+          const float e = s->root.hi[a] - s->root.lo[a];
+          lo[a] = s->root.lo[a] - 0.01f * e - 1e-3f;
+          hi[a] = s->root.hi[a] + 0.01f * e + 1e-3f;
+        }
+      std::memcpy(nrc->lo, lo, sizeof lo);
+      std::memcpy(nrc->hi, hi, sizeof hi);
+    }
+
+    // convergence record
+    double* ref = nullptr;
+    double* rmse_dev = nullptr;
+    double* rmse_partial = nullptr;
+    constexpr uint32_t kRmsePartials = 1024;
+    const uint32_t n_record = progress ? std::min(progress->capacity, passes_total) : 0;
+    if (progress) {
+      if (n_record && (!progress->ms || (progress->reference && !progress->rmse)))
+        throw Error(PBRT_ERR_INVALID_ARG, "progress needs ms (and rmse with a reference)");
+      events.resize(n_record + 1);
+      for (auto& e : events) PBRT_HIP(hipEventCreate(&e));
+      if (progress->reference) {
+        ref = (double*)alloc((size_t)npix * 3 * sizeof(double));
+        PBRT_HIP(hipMemcpyAsync(ref, progress->reference, (size_t)npix * 3 * sizeof(double), hipMemcpyHostToDevice,
+                                st));
+        rmse_dev = (double*)alloc((size_t)std::max(1u, n_record) * sizeof(double));
+        rmse_partial = (double*)alloc(kRmsePartials * sizeof(double));
+      }
+      PBRT_HIP(hipStreamSynchronize(st));
+    }
+
     const SceneView v = view_of(s);
-    const uint32_t grid = persistent_grid(s, npix * per_pass);
+    const uint32_t grid = persistent_grid(s, npass_paths);
     const uint32_t spill_per = ensure_spill(s, grid);
+    if (!events.empty()) PBRT_HIP(hipEventRecord(events[0], st));
     uint64_t passes = 0;
     for (uint32_t first = 0; first < c->spp; first += per_pass) {
       p.pass_first_sample = first;
       p.pass_samples = std::min(per_pass, c->spp - first);
-      PBRT_HIP(launch_generate(v, p, pb, st));
+      const uint32_t npass = npix * p.pass_samples;
+      if (nrc && p.train_stride) p.train_offset = (uint32_t)(hash4(c->seed, passes, 0x747261696eull, 0) % p.train_stride);
+      if (nrc) PBRT_HIP(launch_generate_nrc(p, pb, st));
+      else PBRT_HIP(launch_generate(v, p, pb, st));
       int q = 0;
       for (uint32_t b = 0; b < c->max_depth; ++b) {
         PBRT_HIP(launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st));
         PBRT_HIP(launch_reset_queue(pb, q, st));
-        PBRT_HIP(launch_shade(v, p, pb, q, st));
-        PBRT_HIP(launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st));
+        if (nrc) {
+          PBRT_HIP(launch_shade_nrc(v, p, pb, q, st));
+          PBRT_HIP(launch_shadow_nrc(v, p, pb, s->spill, spill_per, grid, s->status, st));
+        } else {
+          PBRT_HIP(launch_shade(v, p, pb, q, st));
+          PBRT_HIP(launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st));
+        }
         q ^= 1;
       }
+      if (nrc) {  // answer the pass's cache queries
+        PBRT_HIP(nrc_launch_encode((const float*)pb.q_raw, npass, pb.counts + 6, nrc->lo, nrc->hi, q_x, st));
+        PBRT_HIP(nrc_launch_infer(nrc->dev, q_x, q_out, npass, pb.counts + 6, (uint32_t)s->ctx->cu_count, st));
+        PBRT_HIP(launch_nrc_apply(pb, q_out, npass, st));
+      }
       PBRT_HIP(launch_accumulate(p, pb, st));
+      if (nrc && p.train_stride) {  // train on the pass's training paths
+        const uint32_t ranks = (npass - 1 + p.train_offset) / p.train_stride + 1;
+        PBRT_HIP(launch_nrc_targets(p, pb, tb, ranks, st));
+        PBRT_HIP(launch_nrc_select(pb, tb, ranks * p.query_vertex, ncfg.train_batch, st));
+        PBRT_HIP(nrc_launch_encode((const float*)tb.batch_raw, ncfg.train_batch, pb.counts + 8, nrc->lo, nrc->hi,
+                                   batch_x, st));
+        for (uint32_t k = 0; k < std::max(1u, ncfg.train_steps); ++k)
+          PBRT_HIP(nrc_train_device(nrc, batch_x, tb.batch_t, ncfg.train_batch, pb.counts + 8, st));
+        if (ncfg.debug && ncfg.debug_slots && passes + 1 == passes_total) {
+          const size_t ns = std::min<size_t>(ncfg.debug_slots, (size_t)ranks * p.query_vertex);
+          PBRT_HIP(hipMemcpyAsync(ncfg.debug, tb.slot_raw, ns * 3 * sizeof(float4), hipMemcpyDeviceToHost, st));
+          PBRT_HIP(hipMemcpyAsync(ncfg.debug + (size_t)ncfg.debug_slots * 12, tb.slot_t, ns * sizeof(float4),
+                                  hipMemcpyDeviceToHost, st));
+        }
+      }
+      if (passes < n_record) {
+        if (ref)
+          PBRT_HIP(launch_rmse(film, 1.0 / (double)(first + p.pass_samples), ref, npix * 3, rmse_partial, kRmsePartials,
+                               rmse_dev + passes, st));
+        PBRT_HIP(hipEventRecord(events[passes + 1], st));
+      }
       passes++;
     }
     uint32_t counts[16];
@@ -605,7 +743,15 @@ int pbrt_render(pbrt_scene* s, const pbrt_render_config* c, double* film_out, pb
       PBRT_HIP(hipMemcpy(sum.data(), film, sum.size() * sizeof(double), hipMemcpyDeviceToHost));
       for (size_t i = 0; i < sum.size(); ++i) film_out[i] = sum[i] / (double)c->spp;
     }
-    PBRT_HIP(hipFree(film));
+    if (progress) {
+      for (uint32_t k = 0; k < n_record; ++k) {
+        float ms = 0.f;
+        PBRT_HIP(hipEventElapsedTime(&ms, events[0], events[k + 1]));
+        progress->ms[k] = ms;
+      }
+      if (ref && n_record) PBRT_HIP(hipMemcpy(progress->rmse, rmse_dev, n_record * sizeof(double), hipMemcpyDeviceToHost));
+      progress->count = n_record;
+    }
     int status = 0;
     PBRT_HIP(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost));
     if (status) throw Error(PBRT_ERR_INTERNAL, "a traversal stack overflowed");

@@ -1,0 +1,101 @@
+"""The radiance cache inside the wavefront renderer.
+
+Pins of the mechanism, bit-exact against the plain renderer under the same counter-based sampler:
+- every path a training path (stride 1) and queries at vertex 1: no path ends into the cache, so
+  the film is the plain film, and the target of vertex 1 — the radiance gathered over the window
+  of max_depth - 1 vertices — is each path's own radiance (the small scene has no emitter the
+  camera sees, so nothing is added before the window starts);
+- no training paths and a cache that answers 0: every path ends at its third vertex with nothing
+  added, which is the plain path tracer cut at max_depth 3 (the small scene is all Lambertian, so
+  no path passes a specular vertex there).
+And what the trained cache buys: on the room, a cache-terminated render after the cache has
+trained is closer to the converged plain render than the same render whose paths end at the
+third vertex with nothing (the method without the learned part), at equal samples per pixel.
+"""
+import numpy as np
+import pytest
+
+import pbrt_amd
+from pbrt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def small_scene(ctx):
+    desc, cam = scenes.small()
+    return pbrt_amd.Scene(ctx, desc), cam
+
+
+def test_all_training_paths_equal_plain_and_targets_are_path_radiance(gpu_ctx):
+    scene, cam = small_scene(gpu_ctx)
+    w = h = 32
+    plain, _ = scene.render(w, h, 1, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=5)
+    cache = pbrt_amd.RadianceCache(gpu_ctx, seed=2)
+    film, stats, prog = scene.render_ex(w, h, 1, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=5,
+                                        cache=cache, query_vertex=1, train_stride=1, record=False,
+                                        debug_slots=w * h)
+    assert film.tobytes() == plain.tobytes()
+    raw, tgt = prog["debug"]
+    valid = tgt[:, 3] == 1.0
+    assert valid.mean() > 0.2  # primary hits on Lambertian surfaces
+    # stride 1: the pass offset is 0 and slot r is path r, i.e. pixel r
+    flat = plain.reshape(-1, 3).astype(np.float32)
+    np.testing.assert_array_equal(tgt[valid, :3], flat[valid])
+    # the cache inputs of vertex 1: outgoing direction unit length, albedo in the scene's range
+    wo = raw[valid, 3:6]
+    np.testing.assert_allclose((wo * wo).sum(1), 1.0, rtol=1e-5)
+    assert (raw[valid, 9:] >= 0.2).all() and (raw[valid, 9:] <= 0.8).all()
+    assert cache.steps == 1
+
+
+def test_zero_cache_without_training_is_plain_cut_at_the_query_vertex(gpu_ctx):
+    scene, cam = small_scene(gpu_ctx)
+    w = h = 48
+    cut, _ = scene.render(w, h, 4, 3, cam["eye"], cam["look"], fov=cam["fov"], seed=9)
+    cache = pbrt_amd.RadianceCache(gpu_ctx, seed=2)
+    cache.params = np.zeros(pbrt_amd.NRC_PARAMS, np.float32)
+    film, stats, _ = scene.render_ex(w, h, 4, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=9,
+                                     cache=cache, query_vertex=3, train_stride=0, record=False)
+    assert film.tobytes() == cut.tobytes()
+    assert cache.steps == 0
+
+
+def test_progress_record(gpu_ctx):
+    scene, cam = small_scene(gpu_ctx)
+    w = h = 32
+    ref, _ = scene.render(w, h, 64, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=1)
+    film, _, prog = scene.render_ex(w, h, 16, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=3,
+                                    samples_per_pass=2, reference=ref)
+    assert len(prog["ms"]) == 8 and np.all(np.diff(prog["ms"]) > 0)
+    final = np.sqrt(np.mean((film - ref) ** 2))
+    assert prog["rmse"][-1] == pytest.approx(final, rel=1e-9)
+    assert prog["rmse"][-1] < prog["rmse"][0]
+
+
+def test_trained_cache_beats_termination_without_it(gpu_ctx):
+    desc, cam = scenes.room(subdiv=4)
+    scene = pbrt_amd.Scene(gpu_ctx, desc)
+    w = h = 64
+    depth = cam["max_depth"]
+    args = dict(fov=cam["fov"], seed=21)
+    ref, _ = scene.render(w, h, 2048, depth, cam["eye"], cam["look"], **args)
+    spp = 96
+    # the method without the learned part: paths end at the third vertex with nothing added
+    zero = pbrt_amd.RadianceCache(gpu_ctx, seed=4)
+    zero.params = np.zeros(pbrt_amd.NRC_PARAMS, np.float32)
+    cut, _, _ = scene.render_ex(w, h, spp, depth, cam["eye"], cam["look"], cache=zero, train_stride=0,
+                                record=False, **args)
+    # the radiance cache: one path in 8 trains (a 64x64 image has few paths), 4 steps after every pass
+    cache = pbrt_amd.RadianceCache(gpu_ctx, seed=4)
+    cache.set_optim(lr=1e-2)
+    nrc, stats, prog = scene.render_ex(w, h, spp, depth, cam["eye"], cam["look"], cache=cache, train_stride=8,
+                                       train_batch=4096, train_steps=4, reference=ref, **args)
+    rm_cut = np.sqrt(np.mean((cut - ref) ** 2))
+    rm_nrc = np.sqrt(np.mean((nrc - ref) ** 2))
+    bias_cut = abs(cut.mean() - ref.mean()) / ref.mean()
+    bias_nrc = abs(nrc.mean() - ref.mean()) / ref.mean()
+    print(f"rmse cut {rm_cut:.4f} nrc {rm_nrc:.4f}; mean bias cut {bias_cut:.3f} nrc {bias_nrc:.3f}; "
+          f"cache steps {cache.steps}")
+    assert cache.steps == spp * 4
+    assert rm_nrc < rm_cut
+    assert bias_nrc < bias_cut / 2
