@@ -45,6 +45,18 @@ inline double to_unit(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254
 inline double rand_u(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
   return to_unit(hash4(seed, path, bounce, slot));
 }
+// The same draw at the working precision, in [0, 1): 53 bits in f64; in FP32 the top 24 bits,
+// exactly representable (rounding 53 bits to FP32 could give 1.0, which the samplers exclude).
+template <class R>
+inline R uni(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot);
+template <>
+inline double uni<double>(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
+  return rand_u(seed, path, bounce, slot);
+}
+template <>
+inline float uni<float>(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
+  return (float)(hash4(seed, path, bounce, slot) >> 40) * 0x1p-24f;
+}
 enum Slot : uint32_t { kJitterX = 0, kJitterY = 1, kLightPick = 2, kLightU = 3, kLightV = 4, kScatter0 = 5, kScatter1 = 6, kCoin = 7 };
 
 // ---- scene description (shared layout with the ctypes side) -------------------------------------
@@ -557,8 +569,8 @@ class SceneOracle {
     if (m.kind == kLambertian) {
       // lambertian.rs:24-38, pdfs/cosine.rs:14-24
       R two_pi = R(2) * R(M_PI);
-      R phi = two_pi * (R)rand_u(seed, path, bounce, kScatter0);
-      R r2 = (R)rand_u(seed, path, bounce, kScatter1);
+      R phi = two_pi * uni<R>(seed, path, bounce, kScatter0);
+      R r2 = uni<R>(seed, path, bounce, kScatter1);
       R r2s = std::sqrt(r2);
       V3<R> lwi{std::cos(phi) * r2s, std::sin(phi) * r2s, std::sqrt(R(1) - r2)};
       V3<R> wi = to_world(is, lwi);
@@ -573,7 +585,7 @@ class SceneOracle {
       V3<R> wo = to_local(is, is.wo);
       if (wo.z <= R(0)) return false;
       R a = (R)m.roughness;
-      R u1 = (R)rand_u(seed, path, bounce, kScatter0), u2 = (R)rand_u(seed, path, bounce, kScatter1);
+      R u1 = uni<R>(seed, path, bounce, kScatter0), u2 = uni<R>(seed, path, bounce, kScatter1);
       // half vector from D(h) cos(theta_h): tan^2 = a^2 u / (1 - u)
       R phi = R(2) * R(M_PI) * u1;
       R cos_t = std::sqrt((R(1) - u2) / (R(1) + (a * a - R(1)) * u2));
@@ -634,7 +646,7 @@ class SceneOracle {
     V3<R> ldir, org;
     if (refract(lwo, lnorm, ni / nt, &lrefr)) {
       R refl = fresnel(lwo, lnorm, ni, nt);
-      if ((R)rand_u(seed, path, bounce, kCoin) < refl) {
+      if (uni<R>(seed, path, bounce, kCoin) < refl) {
         ldir = lrefl;
         org = is.p + wnorm * R(0.001);
       } else {
@@ -674,7 +686,7 @@ class SceneOracle {
     }
     const Obj& o = objs[L.object];
     R w = (R)o.d.param[0], h = (R)o.d.param[1];
-    R u1 = (R)rand_u(seed, path, bounce, kLightU), u2 = (R)rand_u(seed, path, bounce, kLightV);
+    R u1 = uni<R>(seed, path, bounce, kLightU), u2 = uni<R>(seed, path, bounce, kLightV);
     V3<R> pl_local{(u1 - R(0.5)) * w, R(0), (u2 - R(0.5)) * h};
     V3<R> pl = o.d.has_xform ? o.x.point(o.x.m, pl_local) : pl_local;
     V3<R> nl = o.d.has_xform ? normalized(o.x.normal(o.x.inv, V3<R>{0, 1, 0})) : V3<R>{0, 1, 0};
@@ -712,7 +724,7 @@ class SceneOracle {
       if (bounce == max_depth) break;
       if (!lights.empty()) {
         uint32_t n = (uint32_t)lights.size();
-        R u = (R)rand_u(seed, path, bounce, kLightPick);
+        R u = uni<R>(seed, path, bounce, kLightPick);
         R fi = std::fmin(u * (R)n, (R)n - R(1));
         const LightDesc& Ld = lights[(uint32_t)fi];
         R pl = R(1) / (R)n;
@@ -727,6 +739,9 @@ class SceneOracle {
       if (!scatter(m, hit.is, seed, path, bounce, &s)) break;
       R c = std::fabs(dot(s.ray.d, hit.is.n));
       beta = smul(beta, sdiv(sscale(s.att, c), s.pdf));
+      // a non-finite throughput (grazing specular event, 1/|cos| = inf) ends the path, as on the
+      // device; the reference would carry a NaN
+      if (!(std::isfinite(beta.c[0]) && std::isfinite(beta.c[1]) && std::isfinite(beta.c[2]))) break;
       ray = s.ray;
       specular = s.specular;
     }

@@ -31,5 +31,7 @@ PBRT_HD uint64_t hash4(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
 
 // 53 high bits -> [0, 1)
 PBRT_HD double to_unit(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+// 24 high bits -> [0, 1) in FP32, exactly (rounding the 53-bit value to FP32 could give 1.0)
+PBRT_HD float to_unit_f32(uint64_t h) { return (float)(h >> 40) * 0x1p-24f; }
 
 }  // namespace pbrt

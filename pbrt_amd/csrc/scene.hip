@@ -565,7 +565,7 @@ __device__ __forceinline__ V to_world(const Isect& i, V v) {
 }
 
 __device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
-  return (float)to_unit(hash4(seed, path, bounce, slot));
+  return to_unit_f32(hash4(seed, path, bounce, slot));
 }
 enum : uint32_t { kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7 };
 
@@ -924,6 +924,9 @@ __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
     beta4.x = beta4.x * ((s.att.x * c) / s.pdf);
     beta4.y = beta4.y * ((s.att.y * c) / s.pdf);
     beta4.z = beta4.z * ((s.att.z * c) / s.pdf);
+    // a grazing specular event can give 1/|cos| = inf against cos = 0 in FP32: such a path carries
+    // nothing further (the reference, in f64, would carry a NaN); end it
+    if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) continue;
     beta4.w = s.specular ? 1.0f : 0.0f;
     b.beta[j] = beta4;
     b.ray_o[j] = make_float4(s.o.x, s.o.y, s.o.z, p.near);
