@@ -81,6 +81,101 @@ def cpu_baseline(v, f, coherent, incoherent, side, threads):
                 seconds=best, one_thread_mrays_s=round(one, 3), one_thread_sample_rays=len(small))
 
 
+def first_below(prog, bar):
+    """Milliseconds and samples per pixel at the first pass whose RMSE is at or below `bar`."""
+    hit = np.nonzero(prog["rmse"] <= bar)[0]
+    if len(hit) == 0:
+        return None, None
+    k = int(hit[0])
+    return float(prog["ms"][k]), int(prog["spp"][k])
+
+
+def nrc_measure(pa, ctx, bar=1e-2, frames=256, frames_per_pass=4, ref_spp=4096, plain_spp=1024):
+    """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
+    cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
+    max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device
+    after every pass; times are HIP events on the render stream from the first launch of the render.
+    Plain path tracing runs passes of 4 samples per pixel; the cache renders 1 sample per pixel per
+    frame, `frames_per_pass` frames per pass, one training step of 2^16 samples per frame from the
+    3 % training paths, and ends the other paths into the cache at their third vertex."""
+    from pbrt_amd import scenes
+    desc, cam = scenes.nrc_room()
+    t0 = time.time()
+    scene = pa.Scene(ctx, desc)
+    build_s = time.time() - t0
+    W, H, D = cam["width"], cam["height"], cam["max_depth"]
+    view = dict(fov=cam["fov"])
+    E, L = cam["eye"], cam["look"]
+    ref, ref_st, ref_pr = scene.render_ex(W, H, ref_spp, D, E, L, seed=7, **view)
+    print(f"[bench] nrc: reference {ref_spp} spp in {ref_pr['ms'][-1] / 1e3:.1f}s, mean radiance {ref.mean():.4f}",
+          file=sys.stderr, flush=True)
+    plain, plain_st, plain_pr = scene.render_ex(W, H, plain_spp, D, E, L, seed=8, reference=ref, samples_per_pass=4,
+                                                **view)
+    t_plain, spp_plain = first_below(plain_pr, bar)
+    cache = pa.RadianceCache(ctx, seed=4)
+    film, nrc_st, nrc_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=cache, reference=ref,
+                                           samples_per_pass=frames_per_pass, query_vertex=3, train_stride=33,
+                                           train_batch=65536, train_steps=1, **view)
+    t_nrc, f_nrc = first_below(nrc_pr, bar)
+    # the same cache-terminated render without the learned part (paths end at vertex 3 with nothing)
+    zero = pa.RadianceCache(ctx, seed=4)
+    zero.params = np.zeros(pa.NRC_PARAMS, np.float32)
+    cut, _, cut_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=zero, reference=ref,
+                                     samples_per_pass=frames_per_pass, train_stride=0, **view)
+    k = min(frames, len(plain_pr["rmse"]) * 4) // 4 - 1
+    print(f"[bench] nrc: plain reaches RMSE {bar} at {spp_plain} spp / {t_plain} ms; cache at {f_nrc} frames / "
+          f"{t_nrc} ms; after {frames} frames cache RMSE {nrc_pr['rmse'][-1]:.4f} (cut {cut_pr['rmse'][-1]:.4f}, "
+          f"plain {plain_pr['rmse'][k]:.4f} at {plain_pr['spp'][k]} spp)", file=sys.stderr, flush=True)
+
+    # MLP throughput: inference over one frame's worth of queries, a training step of 2^16
+    n_inf, n_tr = W * H, 65536
+    rng = np.random.default_rng(1)
+    raw = rng.uniform(0.0, 1.0, (n_inf, 12)).astype(np.float32)
+    d_raw = ctx.upload(raw)
+    d_x = ctx.alloc(n_inf * pa.NRC_FEATURES * 2)
+    d_y = ctx.alloc(n_inf * 12)
+    d_t = ctx.upload(rng.uniform(0.0, 1.0, (n_inf, 3)).astype(np.float32))
+    cache.encode(d_raw, n_inf, d_x)
+    e0, e1 = ctx.event(), ctx.event()
+    reps = 20
+    cache.infer(d_x, n_inf, d_y)
+    e0.record(ctx.stream)
+    for _ in range(reps):
+        cache.infer(d_x, n_inf, d_y)
+    e1.record(ctx.stream)
+    infer_ms = e0.elapsed_ms(e1) / reps
+    cache.train_step(d_x, d_t, n_tr)
+    e0.record(ctx.stream)
+    for _ in range(reps):
+        cache.train_step(d_x, d_t, n_tr)
+    e1.record(ctx.stream)
+    train_ms = e0.elapsed_ms(e1) / reps
+    flop_q = 2 * (81 * 64 + 3 * 64 * 64 + 64 * 3)
+    infer_tf = n_inf * flop_q / (infer_ms * 1e-3) / 1e12
+    train_tf = n_tr * 3 * flop_q / (train_ms * 1e-3) / 1e12
+    return {
+        "metric": f"time to RMSE {bar}, radiance cache vs plain path tracing",
+        "workload": "room with the 1,310,720-triangle displaced icosphere and GGX materials (roughness "
+                    "U[0.05,0.5]), 1280x720, max depth 8; RMSE of linear radiance against a "
+                    f"{ref_spp}-spp plain render (mean radiance {ref.mean():.4f})",
+        "plain_ms": t_plain, "plain_spp": spp_plain,
+        "cache_ms": t_nrc, "cache_frames": f_nrc,
+        "time_ratio": (t_nrc / t_plain) if (t_nrc and t_plain) else None,
+        "cache_rmse_final": float(nrc_pr["rmse"][-1]), "cut_rmse_final": float(cut_pr["rmse"][-1]),
+        "frames": frames, "frames_per_pass": frames_per_pass,
+        "cache_ms_per_frame": float(nrc_pr["ms"][-1] / frames),
+        "plain_ms_per_spp": float(plain_pr["ms"][-1] / plain_pr["spp"][-1]),
+        "plain_mrays_s": plain_st["rays"] / (plain_pr["ms"][-1] * 1e-3) / 1e6,
+        "plain_samples_s": W * H * plain_pr["spp"][-1] / (plain_pr["ms"][-1] * 1e-3),
+        "cache_samples_s": W * H * frames / (nrc_pr["ms"][-1] * 1e-3),
+        "reference_s": float(ref_pr["ms"][-1] / 1e3), "scene_build_s": build_s,
+        "mlp": {"infer_ms": infer_ms, "infer_queries": n_inf, "infer_tflops": infer_tf,
+                "train_step_ms": train_ms, "train_samples": n_tr, "train_tflops": train_tf,
+                "peak_bf16_tflops": 2500.0, "infer_frac": infer_tf / 2500.0, "train_frac": train_tf / 2500.0,
+                "flop_per_query": flop_q},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +188,8 @@ def main():
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
+    ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard trivially; see DESIGN.md)",
@@ -100,6 +197,10 @@ def main():
         sys.exit(2)
 
     import pbrt_amd as pa
+
+    if args.nrc_only:
+        print(json.dumps(nrc_measure(pa, pa.Context(0))))
+        return
 
     t0 = time.time()
     v, f, coherent, incoherent = make_workload(pa, side=args.side)
@@ -210,6 +311,10 @@ def main():
         src.free()
         dst.free()
 
+    nrc = None
+    if not args.no_nrc:
+        nrc = nrc_measure(pa, ctx)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -217,7 +322,7 @@ def main():
         print(f"[bench] cpu baseline: {cpu['value']:.2f} Mrays/s on {threads} threads", file=sys.stderr, flush=True)
 
     out = {
-        "metric": "closest-hit Mrays/s (1.3M tris)",
+        "metric": "closest-hit Mrays/s (1.3M tris); time-to-RMSE 1e-2, radiance cache vs plain PT",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -254,6 +359,7 @@ def main():
         "per_batch": per_batch,
         "anyhit_mrays_s": anyhit,
         "copy_gbs": copy_gbs,
+        "radiance_cache": nrc,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

@@ -226,8 +226,9 @@ typedef struct pbrt_nrc_optim {
   float lr, beta1, beta2, eps;
 } pbrt_nrc_optim;
 
-/* Seeded He-uniform weights, zero biases, Adam lr 1e-3, betas 0.9 / 0.99, eps 1e-8 (override with
- * pbrt_nrc_set_optim); scene box [0, 1]^3 until pbrt_nrc_set_bounds. */
+/* Seeded He-uniform hidden weights, zero biases and output layer (an untrained cache answers 0),
+ * Adam lr 1e-3, betas 0.9 / 0.99, eps 1e-8 (override with pbrt_nrc_set_optim); scene box [0, 1]^3
+ * until pbrt_nrc_set_bounds. */
 int pbrt_nrc_create(pbrt_ctx* ctx, uint64_t seed, pbrt_nrc** out);
 int pbrt_nrc_destroy(pbrt_nrc* nrc);
 int pbrt_nrc_set_bounds(pbrt_nrc* nrc, const float lo[3], const float hi[3]);
@@ -265,8 +266,9 @@ typedef struct pbrt_nrc_render {
    * target is the radiance gathered over the next max_depth - query_vertex vertices divided by the
    * path throughput at the vertex (the same window the cache stands for at query_vertex). */
   uint32_t train_stride;
-  uint32_t train_batch;  /* samples per optimisation step, at most (65536); spread over the pass */
-  uint32_t train_steps;  /* optimisation steps per pass (1) */
+  uint32_t train_batch;  /* samples per optimisation step, at most (65536) */
+  uint32_t train_steps;  /* optimisation steps per frame (1); a pass of F samples per pixel holds F
+                          * frames and trains F batches, each drawn evenly from its frame's slots */
   /* Optional host output of the last pass's training slots, for inspection: slot s = rank *
    * query_vertex + (v - 1); the first debug_slots * 12 floats are the raw cache inputs, the next
    * debug_slots * 4 the target rgb and a validity flag. debug_slots = 0 disables it. */
@@ -285,10 +287,11 @@ typedef struct pbrt_progress {
   uint32_t count;          /* out: passes recorded */
 } pbrt_progress;
 
-/* pbrt_render with the radiance cache (nrc != NULL; one pass per sample of every pixel when
- * config->samples_per_pass is 0, and the cache trains after every pass) and/or a per-pass
- * convergence record (progress != NULL). The cache's scene box is set to the scene's bounds.
- * With nrc == NULL this is pbrt_render. */
+/* pbrt_render with the radiance cache (nrc != NULL) and/or a per-pass convergence record
+ * (progress != NULL). With the cache, every sample of every pixel is a frame; a pass holds
+ * config->samples_per_pass frames (0: 1) traced together, then trains the cache once per frame
+ * (frames later in a pass query a cache up to samples_per_pass - 1 steps older). The cache's scene
+ * box is set to the scene's bounds. With nrc == NULL this is pbrt_render. */
 int pbrt_render_ex(pbrt_scene* scene, const pbrt_render_config* config, pbrt_nrc* nrc, const pbrt_nrc_render* nrc_cfg,
                    pbrt_progress* progress, double* film_out, pbrt_render_stats* stats);
 

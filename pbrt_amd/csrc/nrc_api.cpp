@@ -125,11 +125,12 @@ int pbrt_nrc_create(pbrt_ctx* ctx, uint64_t seed, pbrt_nrc** out) {
       n->max_blocks = (uint32_t)ctx->cu_count;
       PBRT_HIP(hipMalloc(&n->partial, (size_t)n->max_blocks * kNrcPartialStride * 4));
       PBRT_HIP(hipMalloc(&n->grads, (size_t)kNrcPartialStride * 4));
-      // He-uniform weights U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)), zero biases
+      // He-uniform hidden weights U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)); zero biases and a zero
+      // output layer, so an untrained cache answers 0 rather than noise
       std::vector<float> p(kNrcParams, 0.f);
       for (int i = 0; i < kNrcParams; ++i) {
         int l, o, k;
-        if (nrc_locate(i, &l, &o, &k)) continue;
+        if (nrc_locate(i, &l, &o, &k) || l == kNrcLayers - 1) continue;
         const double fan_in = l == 0 ? kNrcIn : kNrcHidden;
         const double u = to_unit(hash4(seed, (uint64_t)i, 0x6e7263ull, 0));
         p[i] = (float)((2.0 * u - 1.0) * std::sqrt(6.0 / fan_in));

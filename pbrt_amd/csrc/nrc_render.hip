@@ -87,21 +87,23 @@ __global__ __launch_bounds__(kCompactThreads) void k_nrc_compact(PathBuffers b, 
   if (tid == kCompactThreads - 1) b.counts[7] = part[tid];
 }
 
-// The frame's training batch: all valid slots when there are at most `batch`, else `batch` of them
-// spread evenly over the slot order. counts[8] = batch size.
-__global__ void k_nrc_gather(PathBuffers b, NrcTrainBuffers t, uint32_t batch) {
+// The pass's training batches, one per frame of the pass: b = min(batch, count / frames) samples
+// for each of the `frames` optimisation steps, spread evenly over the slot order (which runs frame
+// by frame), step k's samples at rows [k * batch, k * batch + b). counts[8] = b.
+__global__ void k_nrc_gather(PathBuffers b, NrcTrainBuffers t, uint32_t batch, uint32_t frames) {
   const uint32_t total = b.counts[7];
-  const uint32_t n = min(total, batch);
+  const uint32_t bsz = min(batch, total / frames);
+  const uint32_t n = bsz * frames;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t src = total <= batch ? i : (uint32_t)(((uint64_t)i * total) / batch);
-    const uint32_t s = t.idx[src];
-    for (int f = 0; f < 3; ++f) t.batch_raw[3 * (size_t)i + f] = t.slot_raw[3 * (size_t)s + f];
+    const uint32_t s = t.idx[(uint32_t)(((uint64_t)i * total) / n)];
+    const size_t row = (size_t)(i / bsz) * batch + i % bsz;
+    for (int f = 0; f < 3; ++f) t.batch_raw[3 * row + f] = t.slot_raw[3 * (size_t)s + f];
     const float4 tt = t.slot_t[s];
-    t.batch_t[3 * (size_t)i] = tt.x;
-    t.batch_t[3 * (size_t)i + 1] = tt.y;
-    t.batch_t[3 * (size_t)i + 2] = tt.z;
+    t.batch_t[3 * row] = tt.x;
+    t.batch_t[3 * row + 1] = tt.y;
+    t.batch_t[3 * row + 2] = tt.z;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.counts[8] = n;
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.counts[8] = bsz;
 }
 
 // Sum over pixels and channels of (film / k - ref)^2 in FP64: per-workgroup partials in a fixed
@@ -153,11 +155,13 @@ hipError_t launch_nrc_targets(const RenderParams& p, const PathBuffers& b, const
   return hipGetLastError();
 }
 
-hipError_t launch_nrc_select(const PathBuffers& b, const NrcTrainBuffers& t, uint32_t slots, uint32_t batch, void* s) {
+hipError_t launch_nrc_select(const PathBuffers& b, const NrcTrainBuffers& t, uint32_t slots, uint32_t batch,
+                             uint32_t frames, void* s) {
   hipLaunchKernelGGL(k_nrc_compact, dim3(1), dim3(kCompactThreads), 0, (hipStream_t)s, b, t, slots);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nrc_gather, dim3(grid_for(batch, 256, 1024)), dim3(256), 0, (hipStream_t)s, b, t, batch);
+  hipLaunchKernelGGL(k_nrc_gather, dim3(grid_for(batch * frames, 256, 2048)), dim3(256), 0, (hipStream_t)s, b, t,
+                     batch, frames);
   return hipGetLastError();
 }
 

@@ -644,9 +644,10 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         tb.slot_raw = (float4*)alloc(slots * 3 * sizeof(float4));
         tb.slot_t = (float4*)alloc(slots * sizeof(float4));
         tb.idx = (uint32_t*)alloc(slots * sizeof(uint32_t));
-        tb.batch_raw = (float4*)alloc((size_t)ncfg.train_batch * 3 * sizeof(float4));
-        tb.batch_t = (float*)alloc((size_t)ncfg.train_batch * 3 * sizeof(float));
-        batch_x = (uint16_t*)alloc((size_t)ncfg.train_batch * kNrcInPad * 2);
+        const size_t rows = (size_t)ncfg.train_batch * per_pass;  // one batch per frame of a pass
+        tb.batch_raw = (float4*)alloc(rows * 3 * sizeof(float4));
+        tb.batch_t = (float*)alloc(rows * 3 * sizeof(float));
+        batch_x = (uint16_t*)alloc(rows * kNrcInPad * 2);
       }
       // the cache's encoding box: the scene's bounds, widened by 1 % (+1e-3) on every side
       float lo[3] = {-1.f, -1.f, -1.f}, hi[3] = {1.f, 1.f, 1.f};
@@ -715,11 +716,14 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       if (nrc && p.train_stride) {  // train on the pass's training paths
         const uint32_t ranks = (npass - 1 + p.train_offset) / p.train_stride + 1;
         PBRT_HIP(launch_nrc_targets(p, pb, tb, ranks, st));
-        PBRT_HIP(launch_nrc_select(pb, tb, ranks * p.query_vertex, ncfg.train_batch, st));
-        PBRT_HIP(nrc_launch_encode((const float*)tb.batch_raw, ncfg.train_batch, pb.counts + 8, nrc->lo, nrc->hi,
-                                   batch_x, st));
-        for (uint32_t k = 0; k < std::max(1u, ncfg.train_steps); ++k)
-          PBRT_HIP(nrc_train_device(nrc, batch_x, tb.batch_t, ncfg.train_batch, pb.counts + 8, st));
+        // one batch per frame (sample) of the pass, each trained train_steps times, in frame order
+        const uint32_t T = ncfg.train_batch, frames = p.pass_samples;
+        PBRT_HIP(launch_nrc_select(pb, tb, ranks * p.query_vertex, T, frames, st));
+        PBRT_HIP(nrc_launch_encode((const float*)tb.batch_raw, T * frames, nullptr, nrc->lo, nrc->hi, batch_x, st));
+        for (uint32_t f = 0; f < frames; ++f)
+          for (uint32_t k = 0; k < std::max(1u, ncfg.train_steps); ++k)
+            PBRT_HIP(nrc_train_device(nrc, batch_x + (size_t)f * T * kNrcInPad, tb.batch_t + (size_t)f * T * 3, T,
+                                      pb.counts + 8, st));
         if (ncfg.debug && ncfg.debug_slots && passes + 1 == passes_total) {
           const size_t ns = std::min<size_t>(ncfg.debug_slots, (size_t)ranks * p.query_vertex);
           PBRT_HIP(hipMemcpyAsync(ncfg.debug, tb.slot_raw, ns * 3 * sizeof(float4), hipMemcpyDeviceToHost, st));

@@ -86,3 +86,15 @@ def room(subdiv=6, seed=11, ggx=False, big_mesh=None):
     camera = dict(eye=(0.0, 0.0, 5.8), look=(0.0, 0.0, 0.0), fov=40.0 * math.pi / 180.0, width=512, height=512,
                   spp=64, max_depth=8)
     return d, camera
+
+
+def nrc_room(seed=11, mesh_seed=2024):
+    """The radiance-cache workload: the room with its icosphere replaced by the 1,310,720-triangle
+    displaced icosphere (8 subdivisions, 5-octave value noise of 10 % amplitude) scaled to the
+    icosphere's place, and glossy GGX materials (roughness U[0.05, 0.5]) on the mesh and the boxes.
+    1280x720, 1 sample per pixel per frame for 256 frames, max depth 8."""
+    v, f = icosphere(8, radius=0.55, center=(0.25, -0.4, 0.9), noise="fbm", amp=0.10, octaves=5, seed=mesh_seed)
+    d, cam = room(seed=seed, ggx=True, big_mesh=(v, f))
+    cam = dict(cam, width=1280, height=720, spp=1, frames=256, reference_spp=4096)
+    return d, cam
+
