@@ -347,13 +347,20 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
   if (threadIdx.x == 0) out[kNrcOffLoss] = ((L.loss[0] + L.loss[1]) + L.loss[2]) + L.loss[3];
 }
 
-// grads (padded layout) = sum over blocks in block order; loss likewise.
-__global__ void k_nrc_reduce(const float* __restrict__ partial, uint32_t blocks, float* __restrict__ grads) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= (uint32_t)kNrcOffLoss; i += gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (uint32_t b = 0; b < blocks; ++b) s += partial[(size_t)b * kNrcPartialStride + i];
-    grads[i] = s;
+// grads (padded layout) = sum over the workgroups' partials, loss likewise: eight running sums
+// over blocks b = 8k + r (independent loads in flight), added in a fixed order at the end.
+__global__ __launch_bounds__(64) void k_nrc_reduce(const float* __restrict__ partial, uint32_t blocks,
+                                                   float* __restrict__ grads) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  if (i > (uint32_t)kNrcOffLoss) return;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  uint32_t b = 0;
+  for (; b + 8 <= blocks; b += 8) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[r] += partial[(size_t)(b + r) * kNrcPartialStride + i];
   }
+  for (int r = 0; b < blocks; ++b, ++r) acc[r] += partial[(size_t)b * kNrcPartialStride + i];
+  grads[i] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // Adam (Kingma & Ba 2015) on the flat FP32 parameters, then the BF16 forward / backward copies.
@@ -444,8 +451,8 @@ hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* 
   hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, target, n, n_dev, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcPartialStride + 255) / 256), dim3(256), 0, (hipStream_t)s, partial,
-                     blocks, grads);
+  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcOffLoss + 1 + 63) / 64), dim3(64), 0, (hipStream_t)s, partial, blocks,
+                     grads);
   return hipGetLastError();
 }
 

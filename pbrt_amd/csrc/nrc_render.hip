@@ -63,28 +63,63 @@ __global__ void k_nrc_targets(RenderParams p, PathBuffers b, NrcTrainBuffers t, 
   }
 }
 
-// Valid slots in slot order (one workgroup, deterministic): idx[0 .. count) and counts[7] = count.
-constexpr int kCompactThreads = 1024;
-__global__ __launch_bounds__(kCompactThreads) void k_nrc_compact(PathBuffers b, NrcTrainBuffers t, uint32_t slots) {
-  __shared__ uint32_t part[kCompactThreads];
+// Valid slots in slot order, deterministic: per-workgroup counts, one workgroup scanning them,
+// then each workgroup writing its valid slots from its offset in lane order (wave ballots).
+constexpr int kCompactBlock = 256;
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kCompactBlock) void k_compact_count(NrcTrainBuffers t, uint32_t slots,
+                                                                 uint32_t* __restrict__ block_count) {
+  __shared__ uint32_t wsum[kCompactBlock / 64];
+  const uint32_t s = blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool v = s < slots && t.slot_t[s].w != 0.f;
+  const uint64_t m = __ballot(v);
+  if (__lane_id() == 0) wsum[threadIdx.x / 64] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t c = 0;
+    for (int w = 0; w < kCompactBlock / 64; ++w) c += wsum[w];
+    block_count[blockIdx.x] = c;
+  }
+}
+
+// exclusive scan of the workgroup counts in place; counts[7] = total
+__global__ __launch_bounds__(kScanThreads) void k_compact_scan(PathBuffers b, uint32_t* __restrict__ block_count,
+                                                               uint32_t nblocks) {
+  __shared__ uint32_t part[kScanThreads];
   const uint32_t tid = threadIdx.x;
-  const uint32_t chunk = (slots + kCompactThreads - 1) / kCompactThreads;
-  const uint32_t lo = min(slots, tid * chunk), hi = min(slots, lo + chunk);
+  const uint32_t chunk = (nblocks + kScanThreads - 1) / kScanThreads;
+  const uint32_t lo = min(nblocks, tid * chunk), hi = min(nblocks, lo + chunk);
   uint32_t c = 0;
-  for (uint32_t s = lo; s < hi; ++s) c += t.slot_t[s].w != 0.f ? 1u : 0u;
+  for (uint32_t k = lo; k < hi; ++k) c += block_count[k];
   part[tid] = c;
   __syncthreads();
-  // inclusive Hillis-Steele scan over the 1024 counts
-  for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
+  for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
     const uint32_t add = tid >= off ? part[tid - off] : 0u;
     __syncthreads();
     part[tid] += add;
     __syncthreads();
   }
-  uint32_t w = part[tid] - c;
-  for (uint32_t s = lo; s < hi; ++s)
-    if (t.slot_t[s].w != 0.f) t.idx[w++] = s;
-  if (tid == kCompactThreads - 1) b.counts[7] = part[tid];
+  uint32_t run = part[tid] - c;
+  for (uint32_t k = lo; k < hi; ++k) {
+    const uint32_t x = block_count[k];
+    block_count[k] = run;
+    run += x;
+  }
+  if (tid == kScanThreads - 1) b.counts[7] = part[tid];
+}
+
+__global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(NrcTrainBuffers t, uint32_t slots,
+                                                                   const uint32_t* __restrict__ block_off) {
+  __shared__ uint32_t wsum[kCompactBlock / 64];
+  const uint32_t s = blockIdx.x * kCompactBlock + threadIdx.x;
+  const bool v = s < slots && t.slot_t[s].w != 0.f;
+  const uint64_t m = __ballot(v);
+  const uint32_t lane = __lane_id(), w = threadIdx.x / 64;
+  if (lane == 0) wsum[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t off = block_off[blockIdx.x];
+  for (uint32_t k = 0; k < w; ++k) off += wsum[k];
+  if (v) t.idx[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = s;
 }
 
 // The pass's training batches, one per frame of the pass: b = min(batch, count / frames) samples
@@ -157,8 +192,16 @@ hipError_t launch_nrc_targets(const RenderParams& p, const PathBuffers& b, const
 
 hipError_t launch_nrc_select(const PathBuffers& b, const NrcTrainBuffers& t, uint32_t slots, uint32_t batch,
                              uint32_t frames, void* s) {
-  hipLaunchKernelGGL(k_nrc_compact, dim3(1), dim3(kCompactThreads), 0, (hipStream_t)s, b, t, slots);
+  const uint32_t nb = (slots + kCompactBlock - 1) / kCompactBlock;
+  if (nb == 0 || nb > t.block_cap) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(kCompactBlock), 0, (hipStream_t)s, t, slots, t.block_count);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kScanThreads), 0, (hipStream_t)s, b, t.block_count, nb);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(kCompactBlock), 0, (hipStream_t)s, t, slots, t.block_count);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_nrc_gather, dim3(grid_for(batch * frames, 256, 2048)), dim3(256), 0, (hipStream_t)s, b, t,
                      batch, frames);

@@ -11,6 +11,8 @@ struct NrcTrainBuffers {
   uint32_t* idx;      // [slots] valid slots in slot order
   float4* batch_raw;  // [frames * batch][3]
   float* batch_t;     // [frames * batch][3]
+  uint32_t* block_count;  // [block_cap] compaction counts / offsets (256 slots per workgroup)
+  uint32_t block_cap;
 };
 
 hipError_t launch_nrc_apply(const PathBuffers& b, const float* pred, uint32_t cap, void* stream);

@@ -841,98 +841,148 @@ __global__ __launch_bounds__(kSceneBlock) void k_extend(SceneView S, PathBuffers
   }
 }
 
+// Slots in a queue for the lanes of a wave that want one: one atomic per wave (the lowest
+// requesting lane adds the wave's count), each lane's slot at its rank among the requesting lanes.
+// Every lane of the wave must call it (a convergent point).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+  const uint64_t mask = __ballot(want);
+  if (mask == 0) return 0;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, (int)leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// What one shaded vertex appends to the queues.
+struct ShadeOut {
+  bool shadow = false, next = false, query = false;
+  float4 sh_o, sh_d, sh_c;     // shadow ray and its contribution (path index bits in w)
+  float4 q0, q1, q2, qb;       // cache query: raw inputs, throughput (path index bits in w)
+};
+
 // One path vertex (path.rs:55-112): emission after a specular bounce, the bounce cut, next-event
 // estimation (its shadow ray queued with its contribution), then scattering into the next queue.
 // With the radiance cache (NRC): a training path records the vertex; any other path ends into the
 // cache at its first non-specular, non-emissive vertex at or past p.query_vertex, queueing the
 // query in place of next-event estimation and scattering.
 template <bool NRC>
+__device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderParams& p, const PathBuffers& b,
+                                             uint32_t j, ShadeOut* out) {
+  const SceneHitRec h = b.hit[j];
+  if (h.obj < 0) return;  // no infinite light in these scenes: the path ends
+  uint4 info = b.info[j];
+  const uint64_t path = (uint64_t)info.x * p.spp + info.y;
+  float4 beta4 = b.beta[j], L4 = b.L[j];
+  const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
+  const V rd{d4.x, d4.y, d4.z};
+  const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd);
+  const DevMaterial m = S.mats[S.objs[h.obj].material];
+  const bool specular = beta4.w != 0.0f;
+  V emitted{0.f, 0.f, 0.f};
+  if (specular && m.kind == kMatDiffuseLight) {
+    emitted = {m.rgb[0] * beta4.x, m.rgb[1] * beta4.y, m.rgb[2] * beta4.z};
+    L4.x = L4.x + emitted.x;
+    L4.y = L4.y + emitted.y;
+    L4.z = L4.z + emitted.z;
+  }
+  info.z += 1;
+  const uint32_t bounce = info.z;
+  b.L[j] = L4;
+  b.info[j] = info;
+  if (NRC) {
+    const bool eligible = m.kind == kMatLambertian || m.kind == kMatGGX;
+    uint32_t rank;
+    const bool training = is_training_path(p, j, &rank);
+    if (training || (eligible && bounce >= p.query_vertex && bounce < p.max_depth)) {
+      const V wo = vmul(rd, -1.0f);
+      const float4 raw0 = make_float4(is.p.x, is.p.y, is.p.z, wo.x);
+      const float4 raw1 = make_float4(wo.y, wo.z, is.n.x, is.n.y);
+      const float4 raw2 = make_float4(is.n.z, m.rgb[0], m.rgb[1], m.rgb[2]);
+      if (training) {
+        const size_t cap = b.train_cap, base = (size_t)(bounce - 1) * cap + rank, fs = (size_t)p.max_depth * cap;
+        b.rec[kRecBeta * fs + base] = make_float4(beta4.x, beta4.y, beta4.z, eligible ? 3.0f : 1.0f);
+        b.rec[kRecEmit * fs + base] = make_float4(emitted.x, emitted.y, emitted.z, 0.f);
+        b.rec[(kRecRaw + 0) * fs + base] = raw0;
+        b.rec[(kRecRaw + 1) * fs + base] = raw1;
+        b.rec[(kRecRaw + 2) * fs + base] = raw2;
+      } else {
+        out->query = true;  // the cache stands for everything this path would gather from here on
+        out->q0 = raw0;
+        out->q1 = raw1;
+        out->q2 = raw2;
+        out->qb = make_float4(beta4.x, beta4.y, beta4.z, __uint_as_float(j));
+        return;
+      }
+    }
+  }
+  if (bounce == p.max_depth) return;
+  if (S.n_lights > 0) {
+    const float u = urand(p.seed, path, bounce, kSlotLight);
+    const float fi = fminf(u * (float)S.n_lights, (float)S.n_lights - 1.0f);
+    const DevLight Ld = S.lights[(uint32_t)fi];
+    const float pl = 1.0f / (float)S.n_lights;
+    LiOut ls;
+    if (sample_li(S, Ld, is, p.seed, path, bounce, &ls)) {
+      const V f0 = mat_f(m, is, vmul(rd, -1.0f), ls.wi);
+      const float c = fabsf(vdot(ls.wi, is.n));
+      const V f{f0.x * c, f0.y * c, f0.z * c};
+      const float den = pl * ls.pdf;
+      const V contrib{((ls.li.x * f.x) * beta4.x) / den, ((ls.li.y * f.y) * beta4.y) / den,
+                      ((ls.li.z * f.z) * beta4.z) / den};
+      out->shadow = true;
+      out->sh_o = make_float4(ls.so.x, ls.so.y, ls.so.z, 1e-5f);
+      out->sh_d = make_float4(ls.sd.x, ls.sd.y, ls.sd.z, ls.dist);
+      out->sh_c = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(j));
+    }
+  }
+  ScatterOut sc;
+  if (!scatter(m, is, p.seed, path, bounce, &sc)) return;
+  const float c = fabsf(vdot(sc.d, is.n));
+  beta4.x = beta4.x * ((sc.att.x * c) / sc.pdf);
+  beta4.y = beta4.y * ((sc.att.y * c) / sc.pdf);
+  beta4.z = beta4.z * ((sc.att.z * c) / sc.pdf);
+  // a grazing specular event can give 1/|cos| = inf against cos = 0 in FP32: such a path carries
+  // nothing further (the reference, in f64, would carry a NaN); end it
+  if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) return;
+  beta4.w = sc.specular ? 1.0f : 0.0f;
+  b.beta[j] = beta4;
+  b.ray_o[j] = make_float4(sc.o.x, sc.o.y, sc.o.z, p.near);
+  b.ray_d[j] = make_float4(sc.d.x, sc.d.y, sc.d.z, p.far);
+  out->next = true;
+}
+
+// The vertices of queue q: the grid strides over the queue a workgroup at a time, so every lane of
+// a wave reaches the queue appends together.
+template <bool NRC>
 __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
   const uint32_t n = b.counts[q];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t j = b.queue[q][i];
-    const SceneHitRec h = b.hit[j];
-    if (h.obj < 0) continue;  // no infinite light in these scenes: the path ends
-    uint4 info = b.info[j];
-    const uint64_t path = (uint64_t)info.x * p.spp + info.y;
-    float4 beta4 = b.beta[j], L4 = b.L[j];
-    const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
-    const V rd{d4.x, d4.y, d4.z};
-    const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd);
-    const DevMaterial m = S.mats[S.objs[h.obj].material];
-    const bool specular = beta4.w != 0.0f;
-    V emitted{0.f, 0.f, 0.f};
-    if (specular && m.kind == kMatDiffuseLight) {
-      emitted = {m.rgb[0] * beta4.x, m.rgb[1] * beta4.y, m.rgb[2] * beta4.z};
-      L4.x = L4.x + emitted.x;
-      L4.y = L4.y + emitted.y;
-      L4.z = L4.z + emitted.z;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    ShadeOut o;
+    uint32_t j = 0;
+    if (i < n) {
+      j = b.queue[q][i];
+      shade_vertex<NRC>(S, p, b, j, &o);
     }
-    info.z += 1;
-    const uint32_t bounce = info.z;
-    b.L[j] = L4;
-    b.info[j] = info;
+    const uint32_t ks = wave_append(b.counts + 2, o.shadow);
+    if (o.shadow) {
+      b.sh_o[ks] = o.sh_o;
+      b.sh_d[ks] = o.sh_d;
+      b.sh_c[ks] = o.sh_c;
+    }
+    const uint32_t kn = wave_append(b.counts + (q ^ 1), o.next);
+    if (o.next) b.queue[q ^ 1][kn] = j;
     if (NRC) {
-      const bool eligible = m.kind == kMatLambertian || m.kind == kMatGGX;
-      uint32_t rank;
-      const bool training = is_training_path(p, j, &rank);
-      if (training || (eligible && bounce >= p.query_vertex && bounce < p.max_depth)) {
-        const V wo = vmul(rd, -1.0f);
-        const float4 raw0 = make_float4(is.p.x, is.p.y, is.p.z, wo.x);
-        const float4 raw1 = make_float4(wo.y, wo.z, is.n.x, is.n.y);
-        const float4 raw2 = make_float4(is.n.z, m.rgb[0], m.rgb[1], m.rgb[2]);
-        if (training) {
-          const size_t cap = b.train_cap, base = (size_t)(bounce - 1) * cap + rank, fs = (size_t)p.max_depth * cap;
-          b.rec[kRecBeta * fs + base] = make_float4(beta4.x, beta4.y, beta4.z, eligible ? 3.0f : 1.0f);
-          b.rec[kRecEmit * fs + base] = make_float4(emitted.x, emitted.y, emitted.z, 0.f);
-          b.rec[(kRecRaw + 0) * fs + base] = raw0;
-          b.rec[(kRecRaw + 1) * fs + base] = raw1;
-          b.rec[(kRecRaw + 2) * fs + base] = raw2;
-        } else {
-          const uint32_t k = atomicAdd(b.counts + 6, 1u);
-          b.q_raw[3 * (size_t)k + 0] = raw0;
-          b.q_raw[3 * (size_t)k + 1] = raw1;
-          b.q_raw[3 * (size_t)k + 2] = raw2;
-          b.q_beta[k] = make_float4(beta4.x, beta4.y, beta4.z, __uint_as_float(j));
-          continue;  // the cache stands for everything this path would gather from here on
-        }
+      const uint32_t kq = wave_append(b.counts + 6, o.query);
+      if (o.query) {
+        b.q_raw[3 * (size_t)kq + 0] = o.q0;
+        b.q_raw[3 * (size_t)kq + 1] = o.q1;
+        b.q_raw[3 * (size_t)kq + 2] = o.q2;
+        b.q_beta[kq] = o.qb;
       }
     }
-    if (bounce == p.max_depth) continue;
-    if (S.n_lights > 0) {
-      const float u = urand(p.seed, path, bounce, kSlotLight);
-      const float fi = fminf(u * (float)S.n_lights, (float)S.n_lights - 1.0f);
-      const DevLight Ld = S.lights[(uint32_t)fi];
-      const float pl = 1.0f / (float)S.n_lights;
-      LiOut ls;
-      if (sample_li(S, Ld, is, p.seed, path, bounce, &ls)) {
-        const V f0 = mat_f(m, is, vmul(rd, -1.0f), ls.wi);
-        const float c = fabsf(vdot(ls.wi, is.n));
-        const V f{f0.x * c, f0.y * c, f0.z * c};
-        const float den = pl * ls.pdf;
-        const V contrib{((ls.li.x * f.x) * beta4.x) / den, ((ls.li.y * f.y) * beta4.y) / den,
-                        ((ls.li.z * f.z) * beta4.z) / den};
-        const uint32_t k = atomicAdd(b.counts + 2, 1u);
-        b.sh_o[k] = make_float4(ls.so.x, ls.so.y, ls.so.z, 1e-5f);
-        b.sh_d[k] = make_float4(ls.sd.x, ls.sd.y, ls.sd.z, ls.dist);
-        b.sh_c[k] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(j));
-      }
-    }
-    ScatterOut s;
-    if (!scatter(m, is, p.seed, path, bounce, &s)) continue;
-    const float c = fabsf(vdot(s.d, is.n));
-    beta4.x = beta4.x * ((s.att.x * c) / s.pdf);
-    beta4.y = beta4.y * ((s.att.y * c) / s.pdf);
-    beta4.z = beta4.z * ((s.att.z * c) / s.pdf);
-    // a grazing specular event can give 1/|cos| = inf against cos = 0 in FP32: such a path carries
-    // nothing further (the reference, in f64, would carry a NaN); end it
-    if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) continue;
-    beta4.w = s.specular ? 1.0f : 0.0f;
-    b.beta[j] = beta4;
-    b.ray_o[j] = make_float4(s.o.x, s.o.y, s.o.z, p.near);
-    b.ray_d[j] = make_float4(s.d.x, s.d.y, s.d.z, p.far);
-    const uint32_t k = atomicAdd(b.counts + (q ^ 1), 1u);
-    b.queue[q ^ 1][k] = j;
   }
 }
 

@@ -644,6 +644,8 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         tb.slot_raw = (float4*)alloc(slots * 3 * sizeof(float4));
         tb.slot_t = (float4*)alloc(slots * sizeof(float4));
         tb.idx = (uint32_t*)alloc(slots * sizeof(uint32_t));
+        tb.block_cap = (uint32_t)((slots + 255) / 256);
+        tb.block_count = (uint32_t*)alloc((size_t)tb.block_cap * sizeof(uint32_t));
         const size_t rows = (size_t)ncfg.train_batch * per_pass;  // one batch per frame of a pass
         tb.batch_raw = (float4*)alloc(rows * 3 * sizeof(float4));
         tb.batch_t = (float*)alloc(rows * 3 * sizeof(float));
