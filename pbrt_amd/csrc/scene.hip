@@ -488,7 +488,7 @@ struct Isect {
 
 // The world-space intersection record of a hit (triangle_mesh.rs:73-132, sphere.rs:152-187,
 // aabox.rs:31-60 + 155-190, rectangle.rs:45-54, transform.rs:105-116).
-__device__ Isect interaction(const SceneView& S, const SceneHitRec& h, V wo_ray, V wd) {
+__device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitRec& h, V wo_ray, V wd) {
   const DevObject& ob = S.objs[h.obj];
   V o = wo_ray, d = wd;
   if (ob.has_xform) {
@@ -507,12 +507,14 @@ __device__ Isect interaction(const SceneView& S, const SceneHitRec& h, V wo_ray,
       break;
     }
     case kShapeBox: {
-      const int face = h.face;
-      const float NT[6][6] = {{-1, 0, 0, 0, 0, -1}, {1, 0, 0, 0, 0, 1}, {0, -1, 0, -1, 0, 0},
-                              {0, 1, 0, 1, 0, 0},   {0, 0, -1, 0, -1, 0}, {0, 0, 1, 0, 1, 0}};
+      // the face table of aabox.rs:31-60: face 2a + s has normal (2s - 1) e_a and dpdu
+      // (2s - 1) e_(a+2 mod 3); selected per component (a table indexed by the face would be
+      // placed in private memory)
+      const int face = h.face, axis = face >> 1;
+      const float sg = (face & 1) ? 1.0f : -1.0f;
       is.p = hit;
-      is.n = V{NT[face][0], NT[face][1], NT[face][2]};
-      is.dpdu = V{NT[face][3], NT[face][4], NT[face][5]};
+      is.n = V{axis == 0 ? sg : 0.0f, axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f};
+      is.dpdu = V{axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f, axis == 0 ? sg : 0.0f};
       break;
     }
     case kShapeRect: {
@@ -612,9 +614,14 @@ struct ScatterOut {
   bool specular;
 };
 
-// Material::scatter: lambertian.rs:24-38 (+ pdfs/cosine.rs:14-24), dielectric.rs:31-182.
-__device__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, uint64_t path, uint32_t bounce,
-                        ScatterOut* s) {
+// Material::scatter: lambertian.rs:24-38 (+ pdfs/cosine.rs:14-24), dielectric.rs:31-182. Each
+// branch computes into locals and the output is written once (stores to different fields from
+// different branches would otherwise keep the output in private memory).
+__device__ __forceinline__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, uint64_t path,
+                                        uint32_t bounce, ScatterOut* s) {
+  V att, org, dir;
+  float pdf;
+  bool spec;
   if (m.kind == kMatLambertian) {
     const float two_pi = 2.0f * kPi;
     const float phi = two_pi * urand(seed, path, bounce, kSlotS0);
@@ -622,39 +629,29 @@ __device__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, ui
     const float r2s = sqrtf(r2);
     const V lwi{cosf(phi) * r2s, sinf(phi) * r2s, sqrtf(1.0f - r2)};
     const V wi = to_world(is, lwi);
-    s->o = vadd(is.p, vmul(is.n, 0.001f));
-    s->d = vnorm(wi);
-    s->att = {m.rgb[0] / kPi, m.rgb[1] / kPi, m.rgb[2] / kPi};
-    s->pdf = fabsf(lwi.z / kPi);
-    s->specular = false;
-    return true;
-  }
-  if (m.kind == kMatDielectric) {
+    org = vadd(is.p, vmul(is.n, 0.001f));
+    dir = vnorm(wi);
+    att = {m.rgb[0] / kPi, m.rgb[1] / kPi, m.rgb[2] / kPi};
+    pdf = fabsf(lwi.z / kPi);
+    spec = false;
+  } else if (m.kind == kMatDielectric) {
     const V lwo = vnorm(to_local(is, is.wo));
     const V lrefl{-lwo.x, -lwo.y, lwo.z};
-    V lnorm, wnorm;
-    float ni, nt;
-    if (lwo.z <= 0.0f) {
-      lnorm = {0.f, 0.f, -1.f};
-      wnorm = vmul(is.n, -1.0f);
-      ni = m.ior;
-      nt = 1.0f;
-    } else {
-      lnorm = {0.f, 0.f, 1.f};
-      wnorm = is.n;
-      ni = 1.0f;
-      nt = m.ior;
-    }
+    const bool leaving = lwo.z <= 0.0f;
+    const V lnorm{0.f, 0.f, leaving ? -1.f : 1.f};
+    const V wnorm = leaving ? vmul(is.n, -1.0f) : is.n;
+    const float ni = leaving ? m.ior : 1.0f, nt = leaving ? 1.0f : m.ior;
     const float eta = ni / nt;
     // refract (dielectric.rs:129-157)
     const float ci = vdot(lwo, lnorm);
     const float s2i = 1.0f - ci * ci;
     const float s2t = eta * eta * s2i;
     const float disc = 1.0f - s2t;
-    V ldir, org;
+    bool reflect = true;
+    V lrefr{0.f, 0.f, 0.f};
     if (disc > 0.0f) {
       const float ct = sqrtf(disc);
-      const V lrefr = vnorm(vadd(vmul(vmul(lwo, -1.0f), eta), vmul(lnorm, eta * ci - ct)));
+      lrefr = vnorm(vadd(vmul(vmul(lwo, -1.0f), eta), vmul(lnorm, eta * ci - ct)));
       // fresnel (dielectric.rs:168-182)
       const float e = ni / nt;
       const float fci = lwo.z * lnorm.z;
@@ -662,26 +659,16 @@ __device__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, ui
       const float ro = (ni * fci - nt * fct) / (ni * fci + nt * fct);
       const float rp = (nt * fci - ni * fct) / (nt * fci + ni * fct);
       const float refl = (ro * ro + rp * rp) * 0.5f;
-      if (urand(seed, path, bounce, kSlotCoin) < refl) {
-        ldir = lrefl;
-        org = vadd(is.p, vmul(wnorm, 0.001f));
-      } else {
-        ldir = lrefr;
-        org = vadd(is.p, vmul(wnorm, -0.001f));
-      }
-    } else {
-      ldir = lrefl;
-      org = vadd(is.p, vmul(wnorm, 0.001f));
+      reflect = urand(seed, path, bounce, kSlotCoin) < refl;
     }
-    s->o = org;
-    s->d = vnorm(to_world(is, ldir));
+    const V ldir = reflect ? lrefl : lrefr;
+    org = vadd(is.p, vmul(wnorm, reflect ? 0.001f : -0.001f));
+    dir = vnorm(to_world(is, ldir));
     const float c = fabsf(ldir.z);
-    s->att = {m.rgb[0] / c, m.rgb[1] / c, m.rgb[2] / c};
-    s->pdf = 1.0f;
-    s->specular = true;
-    return true;
-  }
-  if (m.kind == kMatGGX) {
+    att = {m.rgb[0] / c, m.rgb[1] / c, m.rgb[2] / c};
+    pdf = 1.0f;
+    spec = true;
+  } else if (m.kind == kMatGGX) {
     const V wo = to_local(is, is.wo);
     if (wo.z <= 0.0f) return false;
     const float a = m.roughness;
@@ -693,14 +680,20 @@ __device__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, ui
     const float woh = vdot(wo, h);
     const V wi = vsub(vmul(h, 2.0f * woh), wo);
     if (wi.z <= 0.0f) return false;
-    s->pdf = ggx_d(h.z, a) * h.z / (4.0f * woh);
-    s->att = ggx_f(m, is, is.wo, to_world(is, wi));
-    s->o = vadd(is.p, vmul(is.n, 0.001f));
-    s->d = vnorm(to_world(is, wi));
-    s->specular = false;
-    return true;
+    pdf = ggx_d(h.z, a) * h.z / (4.0f * woh);
+    att = ggx_f(m, is, is.wo, to_world(is, wi));
+    org = vadd(is.p, vmul(is.n, 0.001f));
+    dir = vnorm(to_world(is, wi));
+    spec = false;
+  } else {
+    return false;
   }
-  return false;
+  s->att = att;
+  s->o = org;
+  s->d = dir;
+  s->pdf = pdf;
+  s->specular = spec;
+  return true;
 }
 
 struct LiOut {
@@ -710,7 +703,7 @@ struct LiOut {
 
 // Light::sample_li: point_light.rs:23-35 + lights.rs:38-46; area light: uniform point on the
 // rectangle, one-sided emission, solid-angle pdf d^2 / (A cos) (ideas/area_light.md §3).
-__device__ bool sample_li(const SceneView& S, const DevLight& L, const Isect& is, uint64_t seed, uint64_t path,
+__device__ __forceinline__ bool sample_li(const SceneView& S, const DevLight& L, const Isect& is, uint64_t seed, uint64_t path,
                           uint32_t bounce, LiOut* s) {
   if (L.kind == kLightPoint) {
     const V pos{L.pos[0], L.pos[1], L.pos[2]};
