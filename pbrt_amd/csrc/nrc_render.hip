@@ -11,7 +11,7 @@ namespace {
 
 // L[j] += beta * max(cache, 0) for every query of the pass (radiance is non-negative)
 __global__ void k_nrc_apply(PathBuffers b, const float* __restrict__ pred, uint32_t cap) {
-  const uint32_t n = min(cap, b.counts[6]);
+  const uint32_t n = min(cap, *counter(b.counts, kCntQuery));
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const float4 bt = b.q_beta[k];
     const uint32_t j = __float_as_uint(bt.w);
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact_count(NrcTrainBuffers
   }
 }
 
-// exclusive scan of the workgroup counts in place; counts[7] = total
+// exclusive scan of the workgroup counts in place; the kCntValid counter = total
 __global__ __launch_bounds__(kScanThreads) void k_compact_scan(PathBuffers b, uint32_t* __restrict__ block_count,
                                                                uint32_t nblocks) {
   __shared__ uint32_t part[kScanThreads];
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scan(PathBuffers b, ui
     block_count[k] = run;
     run += x;
   }
-  if (tid == kScanThreads - 1) b.counts[7] = part[tid];
+  if (tid == kScanThreads - 1) *counter(b.counts, kCntValid) = part[tid];
 }
 
 __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(NrcTrainBuffers t, uint32_t slots,
@@ -124,9 +124,9 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(NrcTrainBuffe
 
 // The pass's training batches, one per frame of the pass: b = min(batch, count / frames) samples
 // for each of the `frames` optimisation steps, spread evenly over the slot order (which runs frame
-// by frame), step k's samples at rows [k * batch, k * batch + b). counts[8] = b.
+// by frame), step k's samples at rows [k * batch, k * batch + b). the kCntBatch counter = b.
 __global__ void k_nrc_gather(PathBuffers b, NrcTrainBuffers t, uint32_t batch, uint32_t frames) {
-  const uint32_t total = b.counts[7];
+  const uint32_t total = *counter(b.counts, kCntValid);
   const uint32_t bsz = min(batch, total / frames);
   const uint32_t n = bsz * frames;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -138,7 +138,7 @@ __global__ void k_nrc_gather(PathBuffers b, NrcTrainBuffers t, uint32_t batch, u
     t.batch_t[3 * row + 1] = tt.y;
     t.batch_t[3 * row + 2] = tt.z;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.counts[8] = bsz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *counter(b.counts, kCntBatch) = bsz;
 }
 
 // Sum over pixels and channels of (film / k - ref)^2 in FP64: per-workgroup partials in a fixed

@@ -797,24 +797,24 @@ __global__ void k_generate(RenderParams p, PathBuffers b) {
     b.queue[0][j] = j;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    b.counts[0] = n;
-    b.counts[1] = 0;
-    b.counts[2] = 0;
-    if (NRC) b.counts[6] = 0;
+    *counter(b.counts, kCntQueue0) = n;
+    *counter(b.counts, kCntQueue1) = 0;
+    *counter(b.counts, kCntShadow) = 0;
+    if (NRC) *counter(b.counts, kCntQuery) = 0;
   }
 }
 
 __global__ void k_reset(PathBuffers b, int q) {
   if (threadIdx.x == 0) {
-    b.counts[q ^ 1] = 0;
-    b.counts[2] = 0;
+    *counter(b.counts, q ^ 1) = 0;
+    *counter(b.counts, kCntShadow) = 0;
   }
 }
 
-// counts[4..5] hold a 64-bit count of every ray traced (extension and shadow)
+// counter kCntRays holds a 64-bit count of every ray traced (extension and shadow)
 __device__ __forceinline__ void count_rays(PathBuffers& b, uint32_t n) {
   if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(reinterpret_cast<unsigned long long*>(b.counts + 4), (unsigned long long)n);
+    atomicAdd(reinterpret_cast<unsigned long long*>(counter(b.counts, kCntRays)), (unsigned long long)n);
 }
 
 template <int DEPTH>
@@ -823,7 +823,7 @@ __global__ __launch_bounds__(kSceneBlock) void k_extend(SceneView S, PathBuffers
   __shared__ uint2 lds[DEPTH][kSceneBlock];
   Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
                                blockIdx.x * kSceneBlock + threadIdx.x, 0};
-  const uint32_t n = b.counts[q];
+  const uint32_t n = *counter(b.counts, q);
   count_rays(b, n);
   for (uint32_t i = st.gtid; i < n; i += gridDim.x * kSceneBlock) {
     const uint32_t j = b.queue[q][i];
@@ -950,7 +950,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
 // a wave reaches the queue appends together.
 template <bool NRC>
 __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
-  const uint32_t n = b.counts[q];
+  const uint32_t n = *counter(b.counts, q);
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + threadIdx.x;
     ShadeOut o;
@@ -959,16 +959,16 @@ __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
       j = b.queue[q][i];
       shade_vertex<NRC>(S, p, b, j, &o);
     }
-    const uint32_t ks = wave_append(b.counts + 2, o.shadow);
+    const uint32_t ks = wave_append(counter(b.counts, kCntShadow), o.shadow);
     if (o.shadow) {
       b.sh_o[ks] = o.sh_o;
       b.sh_d[ks] = o.sh_d;
       b.sh_c[ks] = o.sh_c;
     }
-    const uint32_t kn = wave_append(b.counts + (q ^ 1), o.next);
+    const uint32_t kn = wave_append(counter(b.counts, q ^ 1), o.next);
     if (o.next) b.queue[q ^ 1][kn] = j;
     if (NRC) {
-      const uint32_t kq = wave_append(b.counts + 6, o.query);
+      const uint32_t kq = wave_append(counter(b.counts, kCntQuery), o.query);
       if (o.query) {
         b.q_raw[3 * (size_t)kq + 0] = o.q0;
         b.q_raw[3 * (size_t)kq + 1] = o.q1;
@@ -987,7 +987,7 @@ __global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, RenderParam
   __shared__ uint2 lds[DEPTH][kSceneBlock];
   Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
                                blockIdx.x * kSceneBlock + threadIdx.x, 0};
-  const uint32_t n = b.counts[2];
+  const uint32_t n = *counter(b.counts, kCntShadow);
   count_rays(b, n);
   for (uint32_t i = st.gtid; i < n; i += gridDim.x * kSceneBlock) {
     const float4 o = b.sh_o[i], d = b.sh_d[i], c = b.sh_c[i];

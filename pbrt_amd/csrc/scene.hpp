@@ -92,6 +92,16 @@ struct RenderParams {
 // inputs (position, outgoing direction, normal, albedo) in three float4.
 enum RecField : uint32_t { kRecBeta = 0, kRecEmit = 1, kRecNee = 2, kRecRaw = 3, kRecFields = 6 };
 
+// Queue and batch counters of the renderer, each on its own 256-byte line (atomics to one L2
+// line serialise): counter(base, k) for k in Counter.
+constexpr uint32_t kCounterStride = 64;
+enum Counter : uint32_t {
+  kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntRays = 4 /* uint64 */, kCntQuery = 6, kCntValid = 7,
+  kCntBatch = 8, kCounters = 16
+};
+constexpr uint32_t kCounterWords = kCounters * kCounterStride;
+__host__ __device__ inline uint32_t* counter(uint32_t* base, uint32_t k) { return base + (size_t)k * kCounterStride; }
+
 struct PathBuffers {
   float4* ray_o;   // origin xyz, near
   float4* ray_d;   // direction xyz, far
@@ -99,14 +109,14 @@ struct PathBuffers {
   float4* L;       // radiance rgb
   uint4* info;     // pixel, sample, bounce, alive
   uint32_t* queue[2];
-  uint32_t* counts;  // [0], [1]: path queues; [2]: shadow queue; [4..5]: rays traced (uint64)
+  uint32_t* counts;  // kCounterWords words: see Counter
   SceneHitRec* hit;
   float4* sh_o;    // shadow ray origin, near
   float4* sh_d;    // shadow ray direction, far
   float4* sh_c;    // contribution rgb, path index bits
   double* film;    // width * height * 3, running sum
   uint32_t capacity;
-  // radiance cache: counts[6] = cache queries of the pass
+  // radiance cache (the pass's query count in counter kCntQuery)
   float4* q_raw;   // [capacity][3] raw cache inputs of each query
   float4* q_beta;  // throughput rgb, path index bits in w
   float4* rec;     // training-path records (see RecField)

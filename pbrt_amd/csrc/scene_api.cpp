@@ -255,7 +255,7 @@ void ensure_paths(pbrt_scene* s, uint32_t capacity) {
   PBRT_HIP(hipMalloc(&p.info, n * sizeof(uint4)));
   PBRT_HIP(hipMalloc(&p.queue[0], n * sizeof(uint32_t)));
   PBRT_HIP(hipMalloc(&p.queue[1], n * sizeof(uint32_t)));
-  PBRT_HIP(hipMalloc(&p.counts, 16 * sizeof(uint32_t)));
+  PBRT_HIP(hipMalloc(&p.counts, kCounterWords * sizeof(uint32_t)));
   PBRT_HIP(hipMalloc(&p.hit, n * sizeof(SceneHitRec)));
   PBRT_HIP(hipMalloc(&p.sh_o, n * sizeof(float4)));
   PBRT_HIP(hipMalloc(&p.sh_d, n * sizeof(float4)));
@@ -603,7 +603,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
 
     double* film = (double*)alloc((size_t)npix * 3 * sizeof(double));
     PBRT_HIP(hipMemsetAsync(film, 0, (size_t)npix * 3 * sizeof(double), st));
-    PBRT_HIP(hipMemsetAsync(s->pb.counts, 0, 16 * sizeof(uint32_t), st));
+    PBRT_HIP(hipMemsetAsync(s->pb.counts, 0, kCounterWords * sizeof(uint32_t), st));
     PathBuffers pb = s->pb;
     pb.film = film;
     const Pinhole cam(c->width, c->height, c->fov, c->near, c->far,
@@ -710,8 +710,9 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         q ^= 1;
       }
       if (nrc) {  // answer the pass's cache queries
-        PBRT_HIP(nrc_launch_encode((const float*)pb.q_raw, npass, pb.counts + 6, nrc->lo, nrc->hi, q_x, st));
-        PBRT_HIP(nrc_launch_infer(nrc->dev, q_x, q_out, npass, pb.counts + 6, (uint32_t)s->ctx->cu_count, st));
+        const uint32_t* nq = counter(pb.counts, kCntQuery);
+        PBRT_HIP(nrc_launch_encode((const float*)pb.q_raw, npass, nq, nrc->lo, nrc->hi, q_x, st));
+        PBRT_HIP(nrc_launch_infer(nrc->dev, q_x, q_out, npass, nq, (uint32_t)s->ctx->cu_count, st));
         PBRT_HIP(launch_nrc_apply(pb, q_out, npass, st));
       }
       PBRT_HIP(launch_accumulate(p, pb, st));
@@ -725,7 +726,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         for (uint32_t f = 0; f < frames; ++f)
           for (uint32_t k = 0; k < std::max(1u, ncfg.train_steps); ++k)
             PBRT_HIP(nrc_train_device(nrc, batch_x + (size_t)f * T * kNrcInPad, tb.batch_t + (size_t)f * T * 3, T,
-                                      pb.counts + 8, st));
+                                      counter(pb.counts, kCntBatch), st));
         if (ncfg.debug && ncfg.debug_slots && passes + 1 == passes_total) {
           const size_t ns = std::min<size_t>(ncfg.debug_slots, (size_t)ranks * p.query_vertex);
           PBRT_HIP(hipMemcpyAsync(ncfg.debug, tb.slot_raw, ns * 3 * sizeof(float4), hipMemcpyDeviceToHost, st));
@@ -741,9 +742,9 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       }
       passes++;
     }
-    uint32_t counts[16];
+    uint64_t rays = 0;
     PBRT_HIP(hipStreamSynchronize(st));
-    PBRT_HIP(hipMemcpy(counts, pb.counts, sizeof(counts), hipMemcpyDeviceToHost));
+    PBRT_HIP(hipMemcpy(&rays, counter(pb.counts, kCntRays), sizeof(rays), hipMemcpyDeviceToHost));
     if (film_out) {
       std::vector<double> sum((size_t)npix * 3);
       PBRT_HIP(hipMemcpy(sum.data(), film, sum.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -762,8 +763,6 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     PBRT_HIP(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost));
     if (status) throw Error(PBRT_ERR_INTERNAL, "a traversal stack overflowed");
     if (stats) {
-      uint64_t rays;
-      std::memcpy(&rays, counts + 4, sizeof(rays));
       stats->rays = rays;
       stats->paths = (uint64_t)npix * c->spp;
       stats->passes = passes;
