@@ -90,7 +90,7 @@ def first_below(prog, bar):
     return float(prog["ms"][k]), int(prog["spp"][k])
 
 
-def nrc_measure(pa, ctx, bar=1e-2, frames=256, frames_per_pass=4, ref_spp=4096, plain_spp=1024):
+def nrc_measure(pa, ctx, bar=1e-2, frames=256, frames_per_pass=8, ref_spp=4096, plain_spp=1024):
     """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
     cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
     max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device

@@ -241,6 +241,9 @@ int pbrt_nrc_get_params(pbrt_nrc* nrc, float* params, uint32_t n);
 int pbrt_nrc_encode(pbrt_nrc* nrc, const float* raw_dev, uint32_t n, uint16_t* x_dev, void* stream);
 /* x_dev (encoded) -> out_dev: n x 3 FP32 radiance. */
 int pbrt_nrc_infer(pbrt_nrc* nrc, const uint16_t* x_dev, uint32_t n, float* out_dev, void* stream);
+/* raw_dev (n x 12 floats, rows 16-byte aligned) -> out_dev (n x 3 FP32): encode and infer in one
+ * kernel, the features computed in registers (equal to pbrt_nrc_encode followed by pbrt_nrc_infer). */
+int pbrt_nrc_query(pbrt_nrc* nrc, const float* raw_dev, uint32_t n, float* out_dev, void* stream);
 /* Loss (relative L2: mean over samples and channels of (y - t)^2 / (y^2 + 0.01), y held constant
  * in the denominator) and its gradient in flat order, without an update. grads may be NULL. */
 int pbrt_nrc_gradients(pbrt_nrc* nrc, const uint16_t* x_dev, const float* target_dev, uint32_t n, float* grads,

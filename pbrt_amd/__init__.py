@@ -479,6 +479,18 @@ class RadianceCache:
     def infer(self, x, n, out, stream=None):
         call("pbrt_nrc_infer", self.h, x.ptr, n, out.ptr, stream)
 
+    def query(self, raw, n, out, stream=None):
+        """Raw inputs (n x 12 floats) to radiance (n x 3), features computed in the kernel."""
+        call("pbrt_nrc_query", self.h, raw.ptr, n, out.ptr, stream)
+
+    def query_host(self, raw12):
+        raw = np.ascontiguousarray(raw12, np.float32).reshape(-1, 12)
+        n = len(raw)
+        dr = self.ctx.upload(raw)
+        do = self.ctx.alloc(max(n, 1) * 12)
+        self.query(dr, n, do)
+        return do.read(np.float32, n * 3).reshape(n, 3)
+
     def train_step(self, x, targets, n, loss=None, stream=None):
         call("pbrt_nrc_train_step", self.h, x.ptr, targets.ptr, n, loss.ptr if loss is not None else None, stream)
 

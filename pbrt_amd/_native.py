@@ -109,6 +109,7 @@ SIGNATURES = {
     "pbrt_nrc_get_params": [P, P, U32],
     "pbrt_nrc_encode": [P, P, U32, P, P],
     "pbrt_nrc_infer": [P, P, U32, P, P],
+    "pbrt_nrc_query": [P, P, U32, P, P],
     "pbrt_nrc_gradients": [P, P, P, U32, P, ctypes.POINTER(F32)],
     "pbrt_nrc_train_step": [P, P, P, U32, P, P],
     "pbrt_nrc_step_count": [P, ctypes.POINTER(U64)],

@@ -158,6 +158,92 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDevice P, const uint16_t* 
   }
 }
 
+// Eight consecutive input features j0 .. j0 + 7 (j0 a multiple of 8) of the sample with raw
+// inputs q (position, outgoing direction, normal, albedo): the same values k_nrc_encode stores.
+__device__ __forceinline__ bf16x8 features8(const float* q, float3 lo, float3 ie, int j0) {
+  Pack8 x;
+  if (j0 < 72) {  // 24 features per axis: sin, cos of 2^k pi p for k = 0..11; 8 never straddle an axis
+    const int a = j0 / 24, k0 = (j0 % 24) / 2;
+    const float p = a == 0 ? (q[0] - lo.x) * ie.x : a == 1 ? (q[1] - lo.y) * ie.y : (q[2] - lo.z) * ie.z;
+    float f = ldexpf(p, k0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float red = f - 2.0f * floorf(f * 0.5f);
+      x.u[2 * i] = bf16_bits(sinpif(red));
+      x.u[2 * i + 1] = bf16_bits(cospif(red));
+      f = f * 2.0f;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int j = j0 + i;
+      x.u[i] = j < IN ? bf16_bits(q[3 + j - 72]) : (uint16_t)0;
+    }
+  }
+  return x.v;
+}
+
+// Inference straight from raw inputs (n x 12 floats): the features of each lane's A operand are
+// computed in registers (no encoded copy in HBM), then the same layers as k_nrc_infer.
+__global__ __launch_bounds__(256) void k_nrc_query(NrcDevice P, const float* __restrict__ raw, float3 lo, float3 ie,
+                                                   float* __restrict__ out, uint32_t n,
+                                                   const uint32_t* __restrict__ n_dev) {
+  __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (n_dev) n = min(n, *n_dev);
+  const uint32_t tiles = (n + 15) / 16;
+  for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+    const uint32_t row0 = t * 16;
+    const int r = lane & 15, g = lane >> 4;
+    const uint32_t row = row0 + r;
+    float q[12];
+    if (row < n) {
+      const float4* src = reinterpret_cast<const float4*>(raw + (size_t)row * 12);
+      const float4 a = src[0], b = src[1], c = src[2];
+      q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
+      q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
+      q[8] = c.x; q[9] = c.y; q[10] = c.z; q[11] = c.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) q[i] = lo.x;  // any finite value: the row is never stored
+    }
+    bf16x8 A[INP / 32];
+#pragma unroll
+    for (int kk = 0; kk < INP / 32; ++kk) A[kk] = features8(q, lo, ie, kk * 32 + g * 8);
+    uint16_t* h0 = act[wave][0];
+    uint16_t* h1 = act[wave][1];
+#pragma unroll
+    for (int nb = 0; nb < H / 16; ++nb) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < INP / 32; ++kk) acc = mfma(A[kk], load8(P.wf0 + (nb * 16 + r) * INP + kk * 32 + g * 8), acc);
+      const int col = nb * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float v = acc[i] + P.b0[col];
+        h0[(g * 4 + i) * H + col] = bf16_bits(v > 0.f ? v : 0.f);
+      }
+    }
+    wave_sync();
+    layer_fwd<H, H, true>(h0, H, P.wf1, P.b1, h1, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(h1, H, P.wf2, P.b2, h0, H, lane, nullptr);
+    wave_sync();
+    layer_fwd<H, H, true>(h0, H, P.wf3, P.b3, h1, H, lane, nullptr);
+    wave_sync();
+    f32x4 y;
+    layer_fwd<H, OUTP, false>(h1, H, P.wf4, P.b4, nullptr, 0, lane, &y);
+    if (r < 3) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t orow = row0 + g * 4 + i;
+        if (orow < n) out[(size_t)orow * 3 + r] = y[i];
+      }
+    }
+    wave_sync();
+  }
+}
+
 // Training: forward, relative-L2 loss gradient and backward over 64-row tiles (4 waves x 16 rows),
 // weight gradients accumulated in MFMA registers across every tile the block processes, then one
 // partial (padded layout, FP32) per block. Deterministic: fixed tile order, fixed reduction order.
@@ -443,6 +529,19 @@ hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, u
   const uint32_t cap = cu * 4;
   if (blocks > cap) blocks = cap;
   hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, out, n, n_dev);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_query(const NrcDevice& P, const float* raw, const float lo[3], const float hi[3], float* out,
+                            uint32_t n, const uint32_t* n_dev, uint32_t cu, void* s) {
+  if (!n) return hipSuccess;
+  const float3 l = make_float3(lo[0], lo[1], lo[2]);
+  const float3 ie = make_float3(1.0f / (hi[0] - lo[0]), 1.0f / (hi[1] - lo[1]), 1.0f / (hi[2] - lo[2]));
+  const uint32_t tiles = (n + 15) / 16;
+  uint32_t blocks = (tiles + 3) / 4;
+  const uint32_t cap = cu * 4;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(k_nrc_query, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, raw, l, ie, out, n, n_dev);
   return hipGetLastError();
 }
 

@@ -81,6 +81,9 @@ NrcDevice nrc_carve(void* base);
 // `n` is the batch size, or with `n_dev` set the capacity bound of a batch counted on the device.
 hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
                             uint32_t cu, void* stream);
+// Inference from raw inputs (n x 12 floats, 16-byte aligned rows), features computed in registers.
+hipError_t nrc_launch_query(const NrcDevice& P, const float* raw, const float lo[3], const float hi[3], float* out,
+                            uint32_t n, const uint32_t* n_dev, uint32_t cu, void* stream);
 hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
                             const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, void* stream);
 hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,

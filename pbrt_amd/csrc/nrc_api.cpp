@@ -216,6 +216,16 @@ int pbrt_nrc_infer(pbrt_nrc* n, const uint16_t* x, uint32_t count, float* out, v
   });
 }
 
+int pbrt_nrc_query(pbrt_nrc* n, const float* raw, uint32_t count, float* out, void* stream) {
+  return guarded([&] {
+    check(n);
+    if (count && (!raw || !out)) throw Error(PBRT_ERR_INVALID_ARG, "null buffer");
+    check_batch(raw, out, count);
+    PBRT_HIP(nrc_launch_query(n->dev, raw, n->lo, n->hi, out, count, nullptr, (uint32_t)n->ctx->cu_count,
+                              stream_of(n, stream)));
+  });
+}
+
 int pbrt_nrc_gradients(pbrt_nrc* n, const uint16_t* x, const float* target, uint32_t count, float* grads,
                        float* loss) {
   return guarded([&] {

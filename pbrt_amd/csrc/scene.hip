@@ -946,12 +946,44 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   out->next = true;
 }
 
-// The vertices of queue q: the grid strides over the queue a workgroup at a time, so every lane of
-// a wave reaches the queue appends together.
+// Queue slots for a whole workgroup of kShadeBlock threads: per-wave counts by ballot, one atomic
+// per queue per workgroup, each lane's slot at its rank in (wave, lane) order. Every thread of the
+// workgroup must call it.
+constexpr int kShadeBlock = 256;
+template <int K>
+__device__ __forceinline__ void block_append(uint32_t* const* ctr, const bool* want, uint32_t* slot) {
+  __shared__ uint32_t wcount[K][kShadeBlock / 64];
+  __shared__ uint32_t base[K];
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+  uint64_t mask[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    mask[k] = __ballot(want[k]);
+    if (lane == 0) wcount[k][wave] = (uint32_t)__popcll(mask[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    uint32_t total = 0;
+    for (int w = 0; w < kShadeBlock / 64; ++w) total += wcount[k][w];
+    base[k] = total ? atomicAdd(ctr[k], total) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t off = base[k];
+    for (uint32_t w = 0; w < wave; ++w) off += wcount[k][w];
+    slot[k] = off + (uint32_t)__popcll(mask[k] & ((1ull << lane) - 1ull));
+  }
+  __syncthreads();  // the shared counts are reused by the next call
+}
+
+// The vertices of queue q: the grid strides over the queue a workgroup at a time, so every thread
+// of a workgroup reaches the queue appends together.
 template <bool NRC>
-__global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
   const uint32_t n = *counter(b.counts, q);
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
     ShadeOut o;
     uint32_t j = 0;
@@ -959,22 +991,22 @@ __global__ void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
       j = b.queue[q][i];
       shade_vertex<NRC>(S, p, b, j, &o);
     }
-    const uint32_t ks = wave_append(counter(b.counts, kCntShadow), o.shadow);
+    constexpr int K = NRC ? 3 : 2;
+    uint32_t* const ctr[3] = {counter(b.counts, kCntShadow), counter(b.counts, q ^ 1), counter(b.counts, kCntQuery)};
+    const bool want[3] = {o.shadow, o.next, o.query};
+    uint32_t slot[3] = {0, 0, 0};
+    block_append<K>(ctr, want, slot);
     if (o.shadow) {
-      b.sh_o[ks] = o.sh_o;
-      b.sh_d[ks] = o.sh_d;
-      b.sh_c[ks] = o.sh_c;
+      b.sh_o[slot[0]] = o.sh_o;
+      b.sh_d[slot[0]] = o.sh_d;
+      b.sh_c[slot[0]] = o.sh_c;
     }
-    const uint32_t kn = wave_append(counter(b.counts, q ^ 1), o.next);
-    if (o.next) b.queue[q ^ 1][kn] = j;
-    if (NRC) {
-      const uint32_t kq = wave_append(counter(b.counts, kCntQuery), o.query);
-      if (o.query) {
-        b.q_raw[3 * (size_t)kq + 0] = o.q0;
-        b.q_raw[3 * (size_t)kq + 1] = o.q1;
-        b.q_raw[3 * (size_t)kq + 2] = o.q2;
-        b.q_beta[kq] = o.qb;
-      }
+    if (o.next) b.queue[q ^ 1][slot[1]] = j;
+    if (NRC && o.query) {
+      b.q_raw[3 * (size_t)slot[2] + 0] = o.q0;
+      b.q_raw[3 * (size_t)slot[2] + 1] = o.q1;
+      b.q_raw[3 * (size_t)slot[2] + 2] = o.q2;
+      b.q_beta[slot[2]] = o.qb;
     }
   }
 }
@@ -1062,14 +1094,14 @@ hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2*
 }
 
 hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream) {
-  hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(b.capacity, 256)), dim3(256), 0, (hipStream_t)stream, v, p, b,
-                     q);
+  hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(b.capacity, kShadeBlock)), dim3(kShadeBlock), 0,
+                     (hipStream_t)stream, v, p, b, q);
   return hipGetLastError();
 }
 
 hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream) {
-  hipLaunchKernelGGL(k_shade<true>, dim3(blocks_for(b.capacity, 256)), dim3(256), 0, (hipStream_t)stream, v, p, b,
-                     q);
+  hipLaunchKernelGGL(k_shade<true>, dim3(blocks_for(b.capacity, kShadeBlock)), dim3(kShadeBlock), 0,
+                     (hipStream_t)stream, v, p, b, q);
   return hipGetLastError();
 }
 

@@ -624,7 +624,6 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
 
     // radiance cache buffers
     NrcTrainBuffers tb{};
-    uint16_t* q_x = nullptr;
     float* q_out = nullptr;
     uint16_t* batch_x = nullptr;
     uint32_t ranks_cap = 0;
@@ -634,7 +633,6 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       p.train_stride = ncfg.train_stride;
       pb.q_raw = (float4*)alloc((size_t)npass_paths * 3 * sizeof(float4));
       pb.q_beta = (float4*)alloc((size_t)npass_paths * sizeof(float4));
-      q_x = (uint16_t*)alloc((size_t)npass_paths * kNrcInPad * 2);
       q_out = (float*)alloc((size_t)npass_paths * 3 * sizeof(float));
       if (ncfg.train_stride) {
         ranks_cap = npass_paths / ncfg.train_stride + 2;
@@ -711,8 +709,8 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       }
       if (nrc) {  // answer the pass's cache queries
         const uint32_t* nq = counter(pb.counts, kCntQuery);
-        PBRT_HIP(nrc_launch_encode((const float*)pb.q_raw, npass, nq, nrc->lo, nrc->hi, q_x, st));
-        PBRT_HIP(nrc_launch_infer(nrc->dev, q_x, q_out, npass, nq, (uint32_t)s->ctx->cu_count, st));
+        PBRT_HIP(nrc_launch_query(nrc->dev, (const float*)pb.q_raw, nrc->lo, nrc->hi, q_out, npass, nq,
+                                  (uint32_t)s->ctx->cu_count, st));
         PBRT_HIP(launch_nrc_apply(pb, q_out, npass, st));
       }
       PBRT_HIP(launch_accumulate(p, pb, st));

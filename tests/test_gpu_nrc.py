@@ -175,3 +175,18 @@ def test_training_lowers_loss(gpu_ctx):
         nrc.train_step_host(nrc.encode_host(raw), t)
     last = nrc.gradients_host(x_eval, t_eval)[0]
     assert last < first / 10, (first, last)
+
+
+@pytest.mark.parametrize("n", [1, 100, 20000])
+def test_query_equals_encode_then_infer(gpu_ctx, n):
+    rng = np.random.default_rng(50 + n)
+    lo, hi = np.array([-2.0, -1.0, 0.5], np.float32), np.array([2.0, 3.0, 1.5], np.float32)
+    raw = np.empty((n, 12), np.float32)
+    raw[:, :3] = rng.uniform(lo, hi, (n, 3))
+    raw[:, 3:] = rng.uniform(-1.0, 1.0, (n, 9))
+    nrc = pbrt_amd.RadianceCache(gpu_ctx, seed=8)
+    p = nrc.params
+    p[-(64 * 3 + 3):] = rng.uniform(-0.2, 0.2, 64 * 3 + 3)  # a non-zero output layer
+    nrc.params = p
+    nrc.set_bounds(lo, hi)
+    np.testing.assert_array_equal(nrc.query_host(raw), nrc.infer_host(nrc.encode_host(raw)))
