@@ -685,64 +685,147 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     const SceneView v = view_of(s);
     const uint32_t grid = persistent_grid(s, npass_paths);
     const uint32_t spill_per = ensure_spill(s, grid);
-    if (!events.empty()) PBRT_HIP(hipEventRecord(events[0], st));
     uint64_t passes = 0;
+    // with the cache: two path sets and a second stream. Pass k (set k % 2) traces bounces
+    // 1..query_vertex and answers its cache queries on the render stream; its training paths'
+    // remaining bounces, the film accumulation and the cache training run on the second stream,
+    // under the next pass's first bounces. The next pass's queries wait for that training, and a
+    // set is reused only once the pass before last has finished with it, so the results are those
+    // of running every step in order on one stream.
+    PathBuffers sets[2] = {pb, pb};
+    hipStream_t aux = nullptr;
+    uint2* spill_aux = nullptr;
+    hipEvent_t ev_front[2] = {nullptr, nullptr}, ev_back[2] = {nullptr, nullptr};
+    struct ReleaseAux {
+      hipStream_t& s;
+      hipEvent_t* a;
+      hipEvent_t* b;
+      ~ReleaseAux() {
+        for (int i = 0; i < 2; ++i) {
+          if (a[i]) (void)hipEventDestroy(a[i]);
+          if (b[i]) (void)hipEventDestroy(b[i]);
+        }
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } release_aux{aux, ev_front, ev_back};
+    if (nrc) {
+      PathBuffers& o = sets[1];
+      const size_t n = npass_paths;
+      o.ray_o = (float4*)alloc(n * sizeof(float4));
+      o.ray_d = (float4*)alloc(n * sizeof(float4));
+      o.beta = (float4*)alloc(n * sizeof(float4));
+      o.L = (float4*)alloc(n * sizeof(float4));
+      o.info = (uint4*)alloc(n * sizeof(uint4));
+      o.queue[0] = (uint32_t*)alloc(n * sizeof(uint32_t));
+      o.queue[1] = (uint32_t*)alloc(n * sizeof(uint32_t));
+      o.counts = (uint32_t*)alloc(kCounterWords * sizeof(uint32_t));
+      o.hit = (SceneHitRec*)alloc(n * sizeof(SceneHitRec));
+      o.sh_o = (float4*)alloc(n * sizeof(float4));
+      o.sh_d = (float4*)alloc(n * sizeof(float4));
+      o.sh_c = (float4*)alloc(n * sizeof(float4));
+      o.capacity = (uint32_t)n;
+      if (ncfg.train_stride) o.rec = (float4*)alloc((size_t)kRecFields * c->max_depth * ranks_cap * sizeof(float4));
+      PBRT_HIP(hipMemsetAsync(o.counts, 0, kCounterWords * sizeof(uint32_t), st));
+      spill_aux = (uint2*)alloc((size_t)spill_per * grid * kSceneBlock * sizeof(uint2));
+      PBRT_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+      for (int i = 0; i < 2; ++i) {
+        PBRT_HIP(hipEventCreateWithFlags(&ev_front[i], hipEventDisableTiming));
+        PBRT_HIP(hipEventCreateWithFlags(&ev_back[i], hipEventDisableTiming));
+      }
+      PBRT_HIP(hipStreamSynchronize(st));
+    }
+    if (!events.empty()) PBRT_HIP(hipEventRecord(events[0], st));
     for (uint32_t first = 0; first < c->spp; first += per_pass) {
       p.pass_first_sample = first;
       p.pass_samples = std::min(per_pass, c->spp - first);
       const uint32_t npass = npix * p.pass_samples;
-      if (nrc && p.train_stride) p.train_offset = (uint32_t)(hash4(c->seed, passes, 0x747261696eull, 0) % p.train_stride);
-      if (nrc) PBRT_HIP(launch_generate_nrc(p, pb, st));
-      else PBRT_HIP(launch_generate(v, p, pb, st));
-      int q = 0;
-      for (uint32_t b = 0; b < c->max_depth; ++b) {
-        PBRT_HIP(launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st));
-        PBRT_HIP(launch_reset_queue(pb, q, st));
-        if (nrc) {
-          PBRT_HIP(launch_shade_nrc(v, p, pb, q, st));
-          PBRT_HIP(launch_shadow_nrc(v, p, pb, s->spill, spill_per, grid, s->status, st));
-        } else {
+      if (!nrc) {
+        PBRT_HIP(launch_generate(v, p, pb, st));
+        int q = 0;
+        for (uint32_t b = 0; b < c->max_depth; ++b) {
+          PBRT_HIP(launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st));
+          PBRT_HIP(launch_reset_queue(pb, q, st));
           PBRT_HIP(launch_shade(v, p, pb, q, st));
           PBRT_HIP(launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st));
+          q ^= 1;
         }
+        PBRT_HIP(launch_accumulate(p, pb, st));
+        if (passes < n_record) {
+          if (ref)
+            PBRT_HIP(launch_rmse(film, 1.0 / (double)(first + p.pass_samples), ref, npix * 3, rmse_partial,
+                                 kRmsePartials, rmse_dev + passes, st));
+          PBRT_HIP(hipEventRecord(events[passes + 1], st));
+        }
+        passes++;
+        continue;
+      }
+      const int S = (int)(passes & 1);
+      PathBuffers& B = sets[S];
+      if (p.train_stride) p.train_offset = (uint32_t)(hash4(c->seed, passes, 0x747261696eull, 0) % p.train_stride);
+      // front: the set is free once the pass before last has finished with it
+      if (passes >= 2) PBRT_HIP(hipStreamWaitEvent(st, ev_back[S], 0));
+      PBRT_HIP(launch_generate_nrc(p, B, st));
+      int q = 0;
+      const uint32_t front = std::min(p.query_vertex, c->max_depth);
+      for (uint32_t b = 0; b < front; ++b) {
+        PBRT_HIP(launch_extend(v, B, q, s->spill, spill_per, grid, s->status, st));
+        PBRT_HIP(launch_reset_queue(B, q, st));
+        PBRT_HIP(launch_shade_nrc(v, p, B, q, st));
+        PBRT_HIP(launch_shadow_nrc(v, p, B, s->spill, spill_per, grid, s->status, st));
         q ^= 1;
       }
-      if (nrc) {  // answer the pass's cache queries
-        const uint32_t* nq = counter(pb.counts, kCntQuery);
-        PBRT_HIP(nrc_launch_query(nrc->dev, (const float*)pb.q_raw, nrc->lo, nrc->hi, q_out, npass, nq,
-                                  (uint32_t)s->ctx->cu_count, st));
-        PBRT_HIP(launch_nrc_apply(pb, q_out, npass, st));
+      // answer the pass's cache queries with the cache trained on every earlier pass
+      if (passes >= 1) PBRT_HIP(hipStreamWaitEvent(st, ev_back[S ^ 1], 0));
+      const uint32_t* nq = counter(B.counts, kCntQuery);
+      PBRT_HIP(nrc_launch_query(nrc->dev, (const float*)B.q_raw, nrc->lo, nrc->hi, q_out, npass, nq,
+                                (uint32_t)s->ctx->cu_count, st));
+      PBRT_HIP(launch_nrc_apply(B, q_out, npass, st));
+      PBRT_HIP(hipEventRecord(ev_front[S], st));
+      // back: the training paths' remaining bounces, the film, the training
+      PBRT_HIP(hipStreamWaitEvent(aux, ev_front[S], 0));
+      for (uint32_t b = front; b < c->max_depth; ++b) {
+        PBRT_HIP(launch_extend(v, B, q, spill_aux, spill_per, grid, s->status, aux));
+        PBRT_HIP(launch_reset_queue(B, q, aux));
+        PBRT_HIP(launch_shade_nrc(v, p, B, q, aux));
+        PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
+        q ^= 1;
       }
-      PBRT_HIP(launch_accumulate(p, pb, st));
-      if (nrc && p.train_stride) {  // train on the pass's training paths
+      PBRT_HIP(launch_accumulate(p, B, aux));
+      if (p.train_stride) {
         const uint32_t ranks = (npass - 1 + p.train_offset) / p.train_stride + 1;
-        PBRT_HIP(launch_nrc_targets(p, pb, tb, ranks, st));
+        PBRT_HIP(launch_nrc_targets(p, B, tb, ranks, aux));
         // one batch per frame (sample) of the pass, each trained train_steps times, in frame order
         const uint32_t T = ncfg.train_batch, frames = p.pass_samples;
-        PBRT_HIP(launch_nrc_select(pb, tb, ranks * p.query_vertex, T, frames, st));
-        PBRT_HIP(nrc_launch_encode((const float*)tb.batch_raw, T * frames, nullptr, nrc->lo, nrc->hi, batch_x, st));
+        PBRT_HIP(launch_nrc_select(B, tb, ranks * p.query_vertex, T, frames, aux));
+        PBRT_HIP(nrc_launch_encode((const float*)tb.batch_raw, T * frames, nullptr, nrc->lo, nrc->hi, batch_x, aux));
         for (uint32_t f = 0; f < frames; ++f)
           for (uint32_t k = 0; k < std::max(1u, ncfg.train_steps); ++k)
             PBRT_HIP(nrc_train_device(nrc, batch_x + (size_t)f * T * kNrcInPad, tb.batch_t + (size_t)f * T * 3, T,
-                                      counter(pb.counts, kCntBatch), st));
+                                      counter(B.counts, kCntBatch), aux));
         if (ncfg.debug && ncfg.debug_slots && passes + 1 == passes_total) {
           const size_t ns = std::min<size_t>(ncfg.debug_slots, (size_t)ranks * p.query_vertex);
-          PBRT_HIP(hipMemcpyAsync(ncfg.debug, tb.slot_raw, ns * 3 * sizeof(float4), hipMemcpyDeviceToHost, st));
+          PBRT_HIP(hipMemcpyAsync(ncfg.debug, tb.slot_raw, ns * 3 * sizeof(float4), hipMemcpyDeviceToHost, aux));
           PBRT_HIP(hipMemcpyAsync(ncfg.debug + (size_t)ncfg.debug_slots * 12, tb.slot_t, ns * sizeof(float4),
-                                  hipMemcpyDeviceToHost, st));
+                                  hipMemcpyDeviceToHost, aux));
         }
       }
       if (passes < n_record) {
         if (ref)
           PBRT_HIP(launch_rmse(film, 1.0 / (double)(first + p.pass_samples), ref, npix * 3, rmse_partial, kRmsePartials,
-                               rmse_dev + passes, st));
-        PBRT_HIP(hipEventRecord(events[passes + 1], st));
+                               rmse_dev + passes, aux));
+        PBRT_HIP(hipEventRecord(events[passes + 1], aux));
       }
+      PBRT_HIP(hipEventRecord(ev_back[S], aux));
       passes++;
     }
+    if (aux) PBRT_HIP(hipStreamSynchronize(aux));
     uint64_t rays = 0;
     PBRT_HIP(hipStreamSynchronize(st));
-    PBRT_HIP(hipMemcpy(&rays, counter(pb.counts, kCntRays), sizeof(rays), hipMemcpyDeviceToHost));
+    for (int k = 0; k < (nrc ? 2 : 1); ++k) {
+      uint64_t r = 0;
+      PBRT_HIP(hipMemcpy(&r, counter(sets[k].counts, kCntRays), sizeof(r), hipMemcpyDeviceToHost));
+      rays += r;
+    }
     if (film_out) {
       std::vector<double> sum((size_t)npix * 3);
       PBRT_HIP(hipMemcpy(sum.data(), film, sum.size() * sizeof(double), hipMemcpyDeviceToHost));
