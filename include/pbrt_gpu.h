@@ -224,6 +224,9 @@ typedef struct pbrt_nrc pbrt_nrc;
 
 typedef struct pbrt_nrc_optim {
   float lr, beta1, beta2, eps;
+  /* Inference reads an exponential moving average of the parameters, avg = ema * avg + (1 - ema) *
+   * params after every step, starting from the parameters when set; 0 = the parameters. */
+  float ema;
 } pbrt_nrc_optim;
 
 /* Seeded He-uniform hidden weights, zero biases and output layer (an untrained cache answers 0),

@@ -451,8 +451,8 @@ class RadianceCache:
         hi = np.ascontiguousarray(hi, np.float32)
         call("pbrt_nrc_set_bounds", self.h, _ptr(lo), _ptr(hi))
 
-    def set_optim(self, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
-        call("pbrt_nrc_set_optim", self.h, ctypes.byref(_native.NrcOptim(lr, beta1, beta2, eps)))
+    def set_optim(self, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8, ema=0.0):
+        call("pbrt_nrc_set_optim", self.h, ctypes.byref(_native.NrcOptim(lr, beta1, beta2, eps, ema)))
 
     @property
     def params(self):

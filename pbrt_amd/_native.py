@@ -42,7 +42,7 @@ class RenderStats(ctypes.Structure):
 
 
 class NrcOptim(ctypes.Structure):
-    _fields_ = [("lr", F32), ("beta1", F32), ("beta2", F32), ("eps", F32)]
+    _fields_ = [("lr", F32), ("beta1", F32), ("beta2", F32), ("eps", F32), ("ema", F32)]
 
 
 class NrcRender(ctypes.Structure):

@@ -100,7 +100,7 @@ __device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, con
 }
 
 // Inference: rows of `x` (N x 96 BF16) -> out (N x 3 FP32). One wave per 16-row tile, grid-stride.
-__global__ __launch_bounds__(256) void k_nrc_infer(NrcDevice P, const uint16_t* __restrict__ x, float* __restrict__ out,
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcForward P, const uint16_t* __restrict__ x, float* __restrict__ out,
                                                    uint32_t n, const uint32_t* __restrict__ n_dev) {
   __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDevice P, const uint16_t* 
             z.q = make_uint4(0, 0, 0, 0);
             a = z.v;
           }
-          const bf16x8 bb = load8(P.wf0 + (nb * 16 + r) * INP + kk * 32 + g * 8);
+          const bf16x8 bb = load8(P.w0 + (nb * 16 + r) * INP + kk * 32 + g * 8);
           acc = mfma(a, bb, acc);
         }
         const int col = nb * 16 + r;
@@ -139,14 +139,14 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcDevice P, const uint16_t* 
       }
     }
     wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.wf1, P.b1, h1, H, lane, nullptr);
+    layer_fwd<H, H, true>(h0, H, P.w1, P.b1, h1, H, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(h1, H, P.wf2, P.b2, h0, H, lane, nullptr);
+    layer_fwd<H, H, true>(h1, H, P.w2, P.b2, h0, H, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.wf3, P.b3, h1, H, lane, nullptr);
+    layer_fwd<H, H, true>(h0, H, P.w3, P.b3, h1, H, lane, nullptr);
     wave_sync();
     f32x4 y;
-    layer_fwd<H, OUTP, false>(h1, H, P.wf4, P.b4, nullptr, 0, lane, &y);
+    layer_fwd<H, OUTP, false>(h1, H, P.w4, P.b4, nullptr, 0, lane, &y);
     if (r < 3) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -185,7 +185,7 @@ __device__ __forceinline__ bf16x8 features8(const float* q, float3 lo, float3 ie
 
 // Inference straight from raw inputs (n x 12 floats): the features of each lane's A operand are
 // computed in registers (no encoded copy in HBM), then the same layers as k_nrc_infer.
-__global__ __launch_bounds__(256) void k_nrc_query(NrcDevice P, const float* __restrict__ raw, float3 lo, float3 ie,
+__global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __restrict__ raw, float3 lo, float3 ie,
                                                    float* __restrict__ out, uint32_t n,
                                                    const uint32_t* __restrict__ n_dev) {
   __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void k_nrc_query(NrcDevice P, const float* __r
     for (int nb = 0; nb < H / 16; ++nb) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kk = 0; kk < INP / 32; ++kk) acc = mfma(A[kk], load8(P.wf0 + (nb * 16 + r) * INP + kk * 32 + g * 8), acc);
+      for (int kk = 0; kk < INP / 32; ++kk) acc = mfma(A[kk], load8(P.w0 + (nb * 16 + r) * INP + kk * 32 + g * 8), acc);
       const int col = nb * 16 + r;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -225,14 +225,14 @@ __global__ __launch_bounds__(256) void k_nrc_query(NrcDevice P, const float* __r
       }
     }
     wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.wf1, P.b1, h1, H, lane, nullptr);
+    layer_fwd<H, H, true>(h0, H, P.w1, P.b1, h1, H, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(h1, H, P.wf2, P.b2, h0, H, lane, nullptr);
+    layer_fwd<H, H, true>(h1, H, P.w2, P.b2, h0, H, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.wf3, P.b3, h1, H, lane, nullptr);
+    layer_fwd<H, H, true>(h0, H, P.w3, P.b3, h1, H, lane, nullptr);
     wave_sync();
     f32x4 y;
-    layer_fwd<H, OUTP, false>(h1, H, P.wf4, P.b4, nullptr, 0, lane, &y);
+    layer_fwd<H, OUTP, false>(h1, H, P.w4, P.b4, nullptr, 0, lane, &y);
     if (r < 3) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -459,7 +459,25 @@ __global__ void k_nrc_adam(NrcDevice P, const float* __restrict__ grads, float l
     P.m[i] = m;
     P.v[i] = v;
     const float mh = m / c1, vh = v / c2;
-    P.master[i] = P.master[i] - lr * mh / (sqrtf(vh) + eps);
+    const float w = P.master[i] - lr * mh / (sqrtf(vh) + eps);
+    P.master[i] = w;
+    if (P.ema_decay > 0.f) P.ema[i] = P.ema_decay * P.ema[i] + (1.0f - P.ema_decay) * w;
+  }
+}
+
+__device__ __forceinline__ void pack_forward(const NrcForward& F, int layer, int o, int k, bool is_bias, float w) {
+  if (is_bias) {
+    float* b = layer == 0 ? F.b0 : layer == 1 ? F.b1 : layer == 2 ? F.b2 : layer == 3 ? F.b3 : F.b4;
+    b[o] = w;
+    return;
+  }
+  const uint16_t q = bf16_bits(w);
+  switch (layer) {
+    case 0: F.w0[o * INP + k] = q; break;
+    case 1: F.w1[o * H + k] = q; break;
+    case 2: F.w2[o * H + k] = q; break;
+    case 3: F.w3[o * H + k] = q; break;
+    default: F.w4[o * H + k] = q; break;
   }
 }
 
@@ -468,18 +486,16 @@ __global__ void k_nrc_pack(NrcDevice P) {
     const float w = P.master[i];
     int layer, o, k;
     const bool is_bias = nrc_locate(i, &layer, &o, &k);
-    if (is_bias) {
-      float* b = layer == 0 ? P.b0 : layer == 1 ? P.b1 : layer == 2 ? P.b2 : layer == 3 ? P.b3 : P.b4;
-      b[o] = w;
-      continue;
-    }
+    pack_forward(P.train(), layer, o, k, is_bias, w);
+    if (P.ema_decay > 0.f) pack_forward(P.avg, layer, o, k, is_bias, P.ema[i]);
+    if (is_bias) continue;
     const uint16_t q = bf16_bits(w);
     switch (layer) {
-      case 0: P.wf0[o * INP + k] = q; break;
-      case 1: P.wf1[o * H + k] = q; P.wt1[k * H + o] = q; break;
-      case 2: P.wf2[o * H + k] = q; P.wt2[k * H + o] = q; break;
-      case 3: P.wf3[o * H + k] = q; P.wt3[k * H + o] = q; break;
-      default: P.wf4[o * H + k] = q; P.wt4[k * OUTK + o] = q; break;
+      case 0: break;
+      case 1: P.wt1[k * H + o] = q; break;
+      case 2: P.wt2[k * H + o] = q; break;
+      case 3: P.wt3[k * H + o] = q; break;
+      default: P.wt4[k * OUTK + o] = q; break;
     }
   }
 }
@@ -521,7 +537,7 @@ __global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __r
 
 }  // namespace
 
-hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
+hipError_t nrc_launch_infer(const NrcForward& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
                             uint32_t cu, void* s) {
   if (!n) return hipSuccess;
   const uint32_t tiles = (n + 15) / 16;
@@ -532,7 +548,7 @@ hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, u
   return hipGetLastError();
 }
 
-hipError_t nrc_launch_query(const NrcDevice& P, const float* raw, const float lo[3], const float hi[3], float* out,
+hipError_t nrc_launch_query(const NrcForward& P, const float* raw, const float lo[3], const float hi[3], float* out,
                             uint32_t n, const uint32_t* n_dev, uint32_t cu, void* s) {
   if (!n) return hipSuccess;
   const float3 l = make_float3(lo[0], lo[1], lo[2]);

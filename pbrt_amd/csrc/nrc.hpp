@@ -66,12 +66,24 @@ __host__ __device__ inline int nrc_padded_index(int i) {
   }
 }
 
+// The weights a forward pass reads: BF16 [out][in] matrices ([64][96], [64][64] x 3, [16][64]) and
+// FP32 biases ([64] x 4, [16]).
+struct NrcForward {
+  uint16_t *w0, *w1, *w2, *w3, *w4;
+  float *b0, *b1, *b2, *b3, *b4;
+};
+
 // Device views of the network (one allocation). BF16 values are stored as their 16-bit patterns.
 struct NrcDevice {
   uint16_t *wf0, *wf1, *wf2, *wf3, *wf4;  // forward, [out][in]: [64][96], [64][64] x 3, [16][64]
   uint16_t *wt1, *wt2, *wt3, *wt4;        // backward, [in][out]: [64][64] x 3, [64][32]
   float *b0, *b1, *b2, *b3, *b4;          // [64] x 4, [16]
   float *master, *m, *v;                  // FP32 parameters and Adam moments, flat order
+  float* ema;                             // exponential moving average of the parameters, flat order
+  NrcForward avg;                         // BF16 forward weights of the average (inference)
+  float ema_decay;                        // 0: inference reads the parameters themselves
+  __host__ __device__ NrcForward train() const { return {wf0, wf1, wf2, wf3, wf4, b0, b1, b2, b3, b4}; }
+  __host__ __device__ NrcForward infer() const { return ema_decay > 0.f ? avg : train(); }
 };
 
 // Bytes of one NrcDevice allocation and the carving of it.
@@ -79,15 +91,16 @@ size_t nrc_device_bytes();
 NrcDevice nrc_carve(void* base);
 
 // `n` is the batch size, or with `n_dev` set the capacity bound of a batch counted on the device.
-hipError_t nrc_launch_infer(const NrcDevice& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
+hipError_t nrc_launch_infer(const NrcForward& F, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
                             uint32_t cu, void* stream);
 // Inference from raw inputs (n x 12 floats, 16-byte aligned rows), features computed in registers.
-hipError_t nrc_launch_query(const NrcDevice& P, const float* raw, const float lo[3], const float hi[3], float* out,
+hipError_t nrc_launch_query(const NrcForward& F, const float* raw, const float lo[3], const float hi[3], float* out,
                             uint32_t n, const uint32_t* n_dev, uint32_t cu, void* stream);
 hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
                             const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, void* stream);
 hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,
                            void* stream);
+// BF16 copies of the parameters (training) and, with ema_decay > 0, of their average (inference).
 hipError_t nrc_launch_pack(const NrcDevice& P, void* stream);
 hipError_t nrc_launch_encode(const float* raw, uint32_t n, const uint32_t* n_dev, const float lo[3], const float hi[3],
                              uint16_t* x, void* stream);
@@ -104,7 +117,7 @@ struct pbrt_nrc {
   float* grads = nullptr;     // reduced gradient, padded layout, loss at kNrcOffLoss
   uint32_t max_blocks = 0;
   float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
-  pbrt_nrc_optim optim{1e-3f, 0.9f, 0.99f, 1e-8f};
+  pbrt_nrc_optim optim{1e-3f, 0.9f, 0.99f, 1e-8f, 0.f};
   uint64_t steps = 0;
 };
 

@@ -25,7 +25,9 @@ size_t nrc_device_bytes() {
   b += up(64 * kNrcInPad * 2) + 3 * up(64 * 64 * 2) + up(16 * 64 * 2);  // forward weights
   b += 3 * up(64 * 64 * 2) + up(64 * 32 * 2);                          // backward weights
   b += 4 * up(64 * 4) + up(16 * 4);                                     // biases
-  b += 3 * up((size_t)kNrcParams * 4);                                  // master, m, v
+  b += 4 * up((size_t)kNrcParams * 4);                                  // master, m, v, ema
+  b += up(64 * kNrcInPad * 2) + 3 * up(64 * 64 * 2) + up(16 * 64 * 2);  // forward weights of the average
+  b += 4 * up(64 * 4) + up(16 * 4);                                     // and its biases
   return b;
 }
 
@@ -54,6 +56,18 @@ NrcDevice nrc_carve(void* base) {
   d.master = (float*)take((size_t)kNrcParams * 4);
   d.m = (float*)take((size_t)kNrcParams * 4);
   d.v = (float*)take((size_t)kNrcParams * 4);
+  d.ema = (float*)take((size_t)kNrcParams * 4);
+  d.avg.w0 = (uint16_t*)take(64 * kNrcInPad * 2);
+  d.avg.w1 = (uint16_t*)take(64 * 64 * 2);
+  d.avg.w2 = (uint16_t*)take(64 * 64 * 2);
+  d.avg.w3 = (uint16_t*)take(64 * 64 * 2);
+  d.avg.w4 = (uint16_t*)take(16 * 64 * 2);
+  d.avg.b0 = (float*)take(64 * 4);
+  d.avg.b1 = (float*)take(64 * 4);
+  d.avg.b2 = (float*)take(64 * 4);
+  d.avg.b3 = (float*)take(64 * 4);
+  d.avg.b4 = (float*)take(16 * 4);
+  d.ema_decay = 0.f;
   return d;
 }
 
@@ -94,6 +108,7 @@ void upload_params(pbrt_nrc* n, const float* params) {
   PBRT_HIP(hipMemcpyAsync(n->dev.master, params, (size_t)kNrcParams * 4, hipMemcpyHostToDevice, s));
   PBRT_HIP(hipMemsetAsync(n->dev.m, 0, (size_t)kNrcParams * 4, s));
   PBRT_HIP(hipMemsetAsync(n->dev.v, 0, (size_t)kNrcParams * 4, s));
+  PBRT_HIP(hipMemcpyAsync(n->dev.ema, params, (size_t)kNrcParams * 4, hipMemcpyHostToDevice, s));
   PBRT_HIP(nrc_launch_pack(n->dev, s));
   PBRT_HIP(hipStreamSynchronize(s));
   n->steps = 0;
@@ -175,9 +190,18 @@ int pbrt_nrc_set_optim(pbrt_nrc* n, const pbrt_nrc_optim* o) {
   return guarded([&] {
     check(n);
     if (!o || !(o->lr > 0.f) || !(o->beta1 >= 0.f && o->beta1 < 1.f) || !(o->beta2 >= 0.f && o->beta2 < 1.f) ||
-        !(o->eps > 0.f))
-      throw Error(PBRT_ERR_INVALID_ARG, "optimiser needs lr > 0, betas in [0, 1), eps > 0");
+        !(o->eps > 0.f) || !(o->ema >= 0.f && o->ema < 1.f))
+      throw Error(PBRT_ERR_INVALID_ARG, "optimiser needs lr > 0, betas and ema in [0, 1), eps > 0");
+    PBRT_HIP(hipStreamSynchronize(n->ctx->stream));
+    const bool was = n->dev.ema_decay > 0.f;
     n->optim = *o;
+    n->dev.ema_decay = o->ema;
+    if (o->ema > 0.f && !was) {  // the average starts from the current parameters
+      PBRT_HIP(hipMemcpyAsync(n->dev.ema, n->dev.master, (size_t)kNrcParams * 4, hipMemcpyDeviceToDevice,
+                              n->ctx->stream));
+      PBRT_HIP(nrc_launch_pack(n->dev, n->ctx->stream));
+      PBRT_HIP(hipStreamSynchronize(n->ctx->stream));
+    }
   });
 }
 
@@ -212,7 +236,7 @@ int pbrt_nrc_infer(pbrt_nrc* n, const uint16_t* x, uint32_t count, float* out, v
     check(n);
     if (count && (!x || !out)) throw Error(PBRT_ERR_INVALID_ARG, "null buffer");
     check_batch(x, out, count);
-    PBRT_HIP(nrc_launch_infer(n->dev, x, out, count, nullptr, (uint32_t)n->ctx->cu_count, stream_of(n, stream)));
+    PBRT_HIP(nrc_launch_infer(n->dev.infer(), x, out, count, nullptr, (uint32_t)n->ctx->cu_count, stream_of(n, stream)));
   });
 }
 
@@ -221,7 +245,7 @@ int pbrt_nrc_query(pbrt_nrc* n, const float* raw, uint32_t count, float* out, vo
     check(n);
     if (count && (!raw || !out)) throw Error(PBRT_ERR_INVALID_ARG, "null buffer");
     check_batch(raw, out, count);
-    PBRT_HIP(nrc_launch_query(n->dev, raw, n->lo, n->hi, out, count, nullptr, (uint32_t)n->ctx->cu_count,
+    PBRT_HIP(nrc_launch_query(n->dev.infer(), raw, n->lo, n->hi, out, count, nullptr, (uint32_t)n->ctx->cu_count,
                               stream_of(n, stream)));
   });
 }

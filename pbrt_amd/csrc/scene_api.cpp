@@ -777,7 +777,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       // answer the pass's cache queries with the cache trained on every earlier pass
       if (passes >= 1) PBRT_HIP(hipStreamWaitEvent(st, ev_back[S ^ 1], 0));
       const uint32_t* nq = counter(B.counts, kCntQuery);
-      PBRT_HIP(nrc_launch_query(nrc->dev, (const float*)B.q_raw, nrc->lo, nrc->hi, q_out, npass, nq,
+      PBRT_HIP(nrc_launch_query(nrc->dev.infer(), (const float*)B.q_raw, nrc->lo, nrc->hi, q_out, npass, nq,
                                 (uint32_t)s->ctx->cu_count, st));
       PBRT_HIP(launch_nrc_apply(B, q_out, npass, st));
       PBRT_HIP(hipEventRecord(ev_front[S], st));

@@ -190,3 +190,26 @@ def test_query_equals_encode_then_infer(gpu_ctx, n):
     nrc.params = p
     nrc.set_bounds(lo, hi)
     np.testing.assert_array_equal(nrc.query_host(raw), nrc.infer_host(nrc.encode_host(raw)))
+
+
+def test_inference_reads_the_parameter_average(gpu_ctx):
+    """With ema = 0.5 the inference weights after one step are the mean of the parameters before
+    and after it (0.5 a + 0.5 b, one rounding, as the device computes it)."""
+    rng = np.random.default_rng(77)
+    nrc = pbrt_amd.RadianceCache(gpu_ctx, seed=9)
+    p = nrc.params
+    p[-(64 * 3 + 3):] = rng.uniform(-0.2, 0.2, 64 * 3 + 3)
+    nrc.params = p
+    nrc.set_optim(lr=1e-2, ema=0.5)
+    before = nrc.params
+    x, t = random_batch(rng, 1024)
+    xb = pad_features(x)
+    nrc.train_step_host(xb, t)
+    after = nrc.params
+    avg = (np.float32(0.5) * before + np.float32(0.5) * after).astype(np.float32)
+    y = nrc.infer_host(xb).astype(np.float64)
+    ref = po.nrc_forward(avg, pbrt_amd.bf16_float(xb)[:, :81], bf16=True)
+    other = po.nrc_forward(after, pbrt_amd.bf16_float(xb)[:, :81], bf16=True)
+    scale = float(np.sqrt(np.mean(ref ** 2)))
+    assert np.sqrt(np.mean((y - ref) ** 2)) / scale < 2e-2
+    assert np.sqrt(np.mean((y - other) ** 2)) > 5 * np.sqrt(np.mean((y - ref) ** 2))
