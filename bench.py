@@ -179,6 +179,22 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
     }
 
 
+def render_profile(pa, ctx, spp=16, frames_per_pass=16):
+    """A short run for kernel traces of the renderer on the radiance-cache workload: `spp` samples
+    per pixel of plain path tracing (4 per pass) and `spp` frames of the cache render."""
+    from pbrt_amd import scenes
+    desc, cam = scenes.nrc_room()
+    scene = pa.Scene(ctx, desc)
+    W, H, D = cam["width"], cam["height"], cam["max_depth"]
+    view = dict(fov=cam["fov"])
+    _, st, pr = scene.render_ex(W, H, spp, D, cam["eye"], cam["look"], seed=1, samples_per_pass=4, **view)
+    plain = dict(ms=float(pr["ms"][-1]), rays=st["rays"])
+    cache = pa.RadianceCache(ctx, seed=4)
+    _, st, pr = scene.render_ex(W, H, spp, D, cam["eye"], cam["look"], seed=1, cache=cache,
+                                samples_per_pass=frames_per_pass, **view)
+    return {"plain": plain, "cache": dict(ms=float(pr["ms"][-1]), rays=st["rays"], frames=spp)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +209,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
+    ap.add_argument("--render-profile", action="store_true",
+                    help="only a short plain + cache render of the cache workload, for kernel traces")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays shard trivially; see DESIGN.md)",
@@ -203,6 +221,9 @@ def main():
 
     if args.nrc_only:
         print(json.dumps(nrc_measure(pa, pa.Context(0))))
+        return
+    if args.render_profile:
+        print(json.dumps(render_profile(pa, pa.Context(0))))
         return
 
     t0 = time.time()
