@@ -144,6 +144,10 @@ typedef struct pbrt_scene pbrt_scene;
 #define PBRT_SHAPE_SPHERE 1    /* Sphere(r) at the origin (sphere.rs), param[0] = r */
 #define PBRT_SHAPE_BOX 2       /* AABox(extent) centred at the origin (aabox.rs), param = extent */
 #define PBRT_SHAPE_RECT 3      /* Rectangle(w, h) in y = 0, normal +y (rectangle.rs), param = w, h */
+#define PBRT_SHAPE_CSG 4       /* CSG of spheres and boxes (shapes/csg/*.rs), see pbrt_scene_add_csg */
+#define PBRT_CSG_UNION 0        /* csg/union.rs */
+#define PBRT_CSG_INTERSECTION 1 /* csg/intersection.rs */
+#define PBRT_CSG_SUBTRACTION 2  /* csg/substraction.rs: elements 1.. subtracted from element 0 */
 #define PBRT_MAT_LAMBERTIAN 0  /* materials/lambertian.rs */
 #define PBRT_MAT_DIELECTRIC 1  /* materials/dielectric.rs */
 #define PBRT_MAT_DIFFUSE_LIGHT 2 /* materials/diffuse_light.rs (rgb = emitted radiance) */
@@ -169,6 +173,14 @@ int pbrt_scene_add_material(pbrt_scene* scene, int kind, const double rgb[3], do
 int pbrt_scene_add_object(pbrt_scene* scene, int shape, int mesh, const double param[3], uint32_t material,
                           const double* m16, const double* minv16, uint32_t* object_id);
 /* PointLight (lights/point_light.rs): radiant intensity rgb at pos. */
+/* A CSG object: csg::{Union, Intersection, Substraction}::new(elements) (shapes/csg/*.rs) wrapped in
+ * Simple (+ Transformed when m16 != NULL). Element k is shapes[k] (PBRT_SHAPE_SPHERE with params[3k]
+ * = r, or PBRT_SHAPE_BOX with params[3k..3k+2] = extents) under the affine transform m[16k..] (row-
+ * major 4x4) with inverse minv[16k..], in the order the constructor receives them. 1 to 64
+ * elements. A hit reports the element it lies on as its primitive. */
+int pbrt_scene_add_csg(pbrt_scene* scene, int op, uint32_t n, const int* shapes, const double* params,
+                       const double* m, const double* minv, uint32_t material, const double* m16,
+                       const double* minv16, uint32_t* id);
 int pbrt_scene_add_point_light(pbrt_scene* scene, const double pos[3], const double rgb[3]);
 /* Area light on a rectangle object (no counterpart in the reference; follows ideas/area_light.md):
  * uniform area sampling, one-sided emission along the rectangle's normal, radiance rgb. */

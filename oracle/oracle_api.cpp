@@ -314,8 +314,8 @@ extern "C" {
 // precision 64: reference arithmetic and tie order; 32: FP32 with the order-independent tie rule.
 void* orc_scene_create(int precision, const float* verts, const uint64_t* vert_off, const uint32_t* idx,
                        const uint64_t* idx_off, uint32_t n_mesh, const void* objects, uint32_t n_obj,
-                       const void* materials, uint32_t n_mat, const void* lights, uint32_t n_light, int bins,
-                       double t_trav, int max_leaf) {
+                       const void* materials, uint32_t n_mat, const void* lights, uint32_t n_light, const void* csg,
+                       uint32_t n_csg, int bins, double t_trav, int max_leaf) {
   BuildParams bp;
   bp.bins = bins;
   bp.t_trav = t_trav;
@@ -324,11 +324,11 @@ void* orc_scene_create(int precision, const float* verts, const uint64_t* vert_o
   if (precision == 64) {
     h->a = new SceneOracle<double>();
     h->a->build(verts, vert_off, idx, idx_off, n_mesh, (const ObjectDesc*)objects, n_obj, (const MaterialDesc*)materials,
-                n_mat, (const LightDesc*)lights, n_light, bp, false);
+                n_mat, (const LightDesc*)lights, n_light, (const CsgElemDesc*)csg, n_csg, bp, false);
   } else {
     h->b = new SceneOracle<float>();
     h->b->build(verts, vert_off, idx, idx_off, n_mesh, (const ObjectDesc*)objects, n_obj, (const MaterialDesc*)materials,
-                n_mat, (const LightDesc*)lights, n_light, bp, true);
+                n_mat, (const LightDesc*)lights, n_light, (const CsgElemDesc*)csg, n_csg, bp, true);
   }
   return h;
 }
@@ -339,7 +339,7 @@ uint32_t orc_desc_sizes(uint32_t* obj, uint32_t* mat, uint32_t* light) {
   *obj = sizeof(ObjectDesc);
   *mat = sizeof(MaterialDesc);
   *light = sizeof(LightDesc);
-  return 0;
+  return sizeof(CsgElemDesc);
 }
 
 int orc_scene_trace_f32(void* hv, const float* rays, uint64_t n, int any, int64_t* obj, int64_t* prim, double* t,

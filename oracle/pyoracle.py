@@ -139,7 +139,8 @@ def _scene_sigs(L):
         return
     L.orc_scene_create.restype = P
     L.orc_scene_create.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_uint32, P, ctypes.c_uint32, P, ctypes.c_uint32, P,
-                                   ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+                                   ctypes.c_uint32, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    L.orc_desc_sizes.restype = ctypes.c_uint32
     L.orc_scene_destroy.argtypes = [P]
     for fn in (L.orc_scene_trace_f32, L.orc_scene_trace_f64):
         fn.restype = ctypes.c_int
@@ -175,16 +176,17 @@ class Scene:
         L = lib()
         _scene_sigs(L)
         so, sm, sl = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-        L.orc_desc_sizes(ctypes.byref(so), ctypes.byref(sm), ctypes.byref(sl))
+        sc = L.orc_desc_sizes(ctypes.byref(so), ctypes.byref(sm), ctypes.byref(sl))
         self.objs = desc.object_array()
         self.mats = desc.material_array()
         self.lights = desc.light_array()
-        assert (self.objs.dtype.itemsize, self.mats.dtype.itemsize, self.lights.dtype.itemsize) == \
-            (so.value, sm.value, sl.value), "descriptor layout mismatch"
+        self.csg = desc.csg_array()
+        assert (self.objs.dtype.itemsize, self.mats.dtype.itemsize, self.lights.dtype.itemsize,
+                self.csg.dtype.itemsize) == (so.value, sm.value, sl.value, sc), "descriptor layout mismatch"
         self.v, self.voff, self.f, self.foff = desc.mesh_arrays()
         self.h = L.orc_scene_create(precision, _p(self.v), _p(self.voff), _p(self.f), _p(self.foff), len(self.voff) - 1,
                                     _p(self.objs), len(self.objs), _p(self.mats), len(self.mats), _p(self.lights),
-                                    len(self.lights), bins, t_trav, max_leaf)
+                                    len(self.lights), _p(self.csg), len(desc.csg), bins, t_trav, max_leaf)
         self.precision = precision
 
     def __del__(self):

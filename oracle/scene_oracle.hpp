@@ -60,7 +60,8 @@ inline float uni<float>(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t 
 enum Slot : uint32_t { kJitterX = 0, kJitterY = 1, kLightPick = 2, kLightU = 3, kLightV = 4, kScatter0 = 5, kScatter1 = 6, kCoin = 7 };
 
 // ---- scene description (shared layout with the ctypes side) -------------------------------------
-enum ShapeKind : int32_t { kMesh = 0, kSphere = 1, kBox = 2, kRect = 3 };
+enum ShapeKind : int32_t { kMesh = 0, kSphere = 1, kBox = 2, kRect = 3, kCsg = 4 };
+enum CsgOp : int32_t { kCsgUnion = 0, kCsgIntersection = 1, kCsgSubtraction = 2 };
 enum MatKind : int32_t { kLambertian = 0, kDielectric = 1, kDiffuseLight = 2, kGGX = 3 };
 enum LightKind : int32_t { kPointLight = 0, kAreaLight = 1 };
 
@@ -74,6 +75,15 @@ struct ObjectDesc {
   int32_t material, has_xform;
   double param[3];          // sphere: r; box: extent xyz; rect: width, height
   double m[16], minv[16];   // local -> world and its inverse, row-major
+};
+// One element of a CSG shape (shapes/csg.rs: Elem { shape, transform }): a sphere or a box and its
+// transform. A CSG object is an ObjectDesc with shape kCsg, mesh = its first element, param[0] =
+// the operation and param[1] = the element count, elements in the order the operation's
+// constructor receives them (Union::new(elements), csg/union.rs:13-17).
+struct CsgElemDesc {
+  int32_t shape, pad;
+  double param[3];
+  double m[16], minv[16];
 };
 struct LightDesc {
   int32_t kind, object;     // object: the emitting rectangle of an area light
@@ -144,8 +154,13 @@ class SceneOracle {
     Xform<R> x;
     AABB<double> box;  // world bounds (f64, as the reference)
   };
+  struct CsgElem {
+    CsgElemDesc d;
+    Xform<R> x;
+  };
   std::vector<MeshBVH<R>*> meshes;
   std::vector<Obj> objs;
+  std::vector<CsgElem> csg;
   std::vector<MaterialDesc> mats;
   std::vector<LightDesc> lights;
   bool canonical;
@@ -165,25 +180,27 @@ class SceneOracle {
   // ---- shapes ------------------------------------------------------------------------------------
   // sphere.rs:108-144 (+ 152-187). Returns the first recorded intersection (Simple takes [0]).
   bool sphere(const Ray<R>& ray, R r, R near, R far, Isect<R>* out) const {
+    Isect<R> all[2];
+    if (sphere_all(ray, r, near, far, all) == 0) return false;
+    *out = all[0];
+    return true;
+  }
+  // Every intersection the sphere records, in order (at most 2).
+  int sphere_all(const Ray<R>& ray, R r, R near, R far, Isect<R>* out) const {
     V3<R> l = ray.o * R(-1);
     R tca = dot(l, ray.d);
-    if (tca < R(0)) return false;
+    if (tca < R(0)) return 0;
     R r2 = r * r;
     R d2 = dot(l, l) - tca * tca;
-    if (d2 > r2) return false;
+    if (d2 > r2) return 0;
     R thc = std::sqrt(r2 - d2);
     R t0 = tca - thc, t1 = tca + thc;
     R tmin = rmin(t0, t1), tmax = rmax(t0, t1);
-    if (tmax < near || tmin > far) return false;
-    if (tmin > near) {
-      *out = sphere_details(ray, r, tmin);
-      return true;
-    }
-    if (tmax > near && tmax < far) {
-      *out = sphere_details(ray, r, tmax);
-      return true;
-    }
-    return false;
+    if (tmax < near || tmin > far) return 0;
+    int n = 0;
+    if (tmin > near) out[n++] = sphere_details(ray, r, tmin);
+    if (tmax > near && tmax < far) out[n++] = sphere_details(ray, r, tmax);
+    return n;
   }
   Isect<R> sphere_details(const Ray<R>& ray, R r, R t) const {
     const R PI = R(M_PI);
@@ -205,6 +222,14 @@ class SceneOracle {
   }
   // aabox.rs:66-193 with the constant normal/derivative table of aabox.rs:31-60.
   bool box(const Ray<R>& ray, const R ext[3], R near, R far, Isect<R>* out) const {
+    Isect<R> all[2];
+    int face[2];
+    if (box_all(ray, ext, near, far, all, face) == 0) return false;
+    *out = all[0];
+    return true;
+  }
+  // Every intersection the box records, in order (at most 2), with the face of each.
+  int box_all(const Ray<R>& ray, const R ext[3], R near, R far, Isect<R>* out, int* face) const {
     static const R NT[6][9] = {
         {-1, 0, 0, 0, 0, -1, 0, 1, 0}, {1, 0, 0, 0, 0, 1, 0, 1, 0},  {0, -1, 0, -1, 0, 0, 0, 0, 1},
         {0, 1, 0, 1, 0, 0, 0, 0, 1},   {0, 0, -1, 0, -1, 0, 1, 0, 0}, {0, 0, 1, 0, 1, 0, 1, 0, 0}};
@@ -219,32 +244,33 @@ class SceneOracle {
     int nymin, nymax;
     if (inv.y >= R(0)) { tymin = (mn.y - ray.o.y) * inv.y; tymax = (mx.y - ray.o.y) * inv.y; nymin = 2; nymax = 3; }
     else { tymin = (mx.y - ray.o.y) * inv.y; tymax = (mn.y - ray.o.y) * inv.y; nymin = 3; nymax = 2; }
-    if (tmin > tymax || tymin > tmax) return false;
+    if (tmin > tymax || tymin > tmax) return 0;
     if (tymin > tmin) { tmin = tymin; nmin = nymin; }
     if (tymax < tmax) { tmax = tymax; nmax = nymax; }
     R tzmin, tzmax;
     int nzmin, nzmax;
     if (inv.z >= R(0)) { tzmin = (mn.z - ray.o.z) * inv.z; tzmax = (mx.z - ray.o.z) * inv.z; nzmin = 4; nzmax = 5; }
     else { tzmin = (mx.z - ray.o.z) * inv.z; tzmax = (mn.z - ray.o.z) * inv.z; nzmin = 5; nzmax = 4; }
-    if (tmin > tzmax || tzmin > tmax) return false;
+    if (tmin > tzmax || tzmin > tmax) return 0;
     if (tzmin > tmin) { tmin = tzmin; nmin = nzmin; }
     if (tzmax < tmax) { tmax = tzmax; nmax = nzmax; }
-    if (tmax < near || tmin > far) return false;
+    if (tmax < near || tmin > far) return 0;
     auto rec = [&](R t, int k) {
       V3<R> p = ray.o + ray.d * t;
       const R* e = NT[k];
       V3<R> du{e[3], e[4], e[5]}, dv{e[6], e[7], e[8]};
       return Isect<R>{p, {e[0], e[1], e[2]}, ray.d * R(-1), du, dv, t, dot(p, du), dot(p, dv)};
     };
+    int n = 0;
     if (tmin > near) {
-      *out = rec(tmin, nmin);
-      return true;
+      face[n] = nmin;
+      out[n++] = rec(tmin, nmin);
     }
     if (tmax < far) {
-      *out = rec(tmax, nmax);
-      return true;
+      face[n] = nmax;
+      out[n++] = rec(tmax, nmax);
     }
-    return false;
+    return n;
   }
   // rectangle.rs:26-58 (the constructor halves width and height, rectangle.rs:15-20).
   bool rect(const Ray<R>& ray, R w, R h, R near, R far, Isect<R>* out) const {
@@ -277,10 +303,93 @@ class SceneOracle {
     return {hit, n, ray.d * R(-1), dpdu, dpdv, h.t, uu, vv};
   }
 
+  // ---- CSG (shapes/csg/*.rs) -------------------------------------------------------------------
+  // Shape::contain_point of an element at a point of the CSG frame: sphere.rs:146-148 (closed),
+  // aabox.rs:195-202 (closed), through transform_point_to_local.
+  bool elem_contains(const CsgElem& e, V3<R> p) const {
+    const V3<R> lp = e.x.point(e.x.inv, p);
+    if (e.d.shape == kSphere) {
+      const R r = (R)e.d.param[0];
+      return dot(lp, lp) <= r * r;
+    }
+    const R hx = (R)e.d.param[0] * R(0.5), hy = (R)e.d.param[1] * R(0.5), hz = (R)e.d.param[2] * R(0.5);
+    return lp.x >= R(0) - hx && lp.x <= hx && lp.y >= R(0) - hy && lp.y <= hy && lp.z >= R(0) - hz && lp.z <= hz;
+  }
+  // The element's intersections in the CSG frame (transform_ray_to_local, shape.intersect,
+  // transform_interaction_to_world), with the box face of each.
+  int elem_hits(const CsgElem& e, const Ray<R>& ray, R near, R far, Isect<R>* out, int* face) const {
+    const Ray<R> lr = e.x.to_local(ray);
+    Isect<R> li[2];
+    int n;
+    if (e.d.shape == kSphere) {
+      n = sphere_all(lr, (R)e.d.param[0], near, far, li);
+      face[0] = face[1] = 0;
+    } else {
+      const R ext[3] = {(R)e.d.param[0], (R)e.d.param[1], (R)e.d.param[2]};
+      n = box_all(lr, ext, near, far, li, face);
+    }
+    for (int i = 0; i < n; ++i) out[i] = e.x.to_world(li[i]);
+    return n;
+  }
+  // The first intersection of a CSG shape: union.rs:22-42 (kept when inside no other element),
+  // intersection.rs:23-43 (kept when inside every other element), substraction.rs:22-70 (the
+  // base's hits outside every subtracted element; a subtracted element's hits inside the base and
+  // outside every other subtracted element, normal flipped). The candidates are sorted by distance
+  // with a stable sort, so the first is the nearest, the earliest recorded among equals. *prim is
+  // the element the hit belongs to.
+  bool csg_hit(const Obj& o, const Ray<R>& ray, R near, R far, Isect<R>* out, int64_t* prim) const {
+    const int op = (int)o.d.param[0];
+    const uint32_t base = (uint32_t)o.d.mesh, cnt = (uint32_t)o.d.param[1];
+    bool found = false;
+    R best = R(0);
+    auto consider = [&](const Isect<R>& h, uint32_t k) {
+      if (!found || h.d < best) {
+        found = true;
+        best = h.d;
+        *out = h;
+        *prim = (int64_t)k;
+      }
+    };
+    auto in_subtracted = [&](V3<R> p, uint32_t exclude) {
+      for (uint32_t k = 1; k < cnt; ++k)
+        if (k != exclude && elem_contains(csg[base + k], p)) return true;
+      return false;
+    };
+    Isect<R> h[2];
+    int face[2];
+    if (op == kCsgSubtraction) {
+      if (cnt == 0) return false;
+      const int n0 = elem_hits(csg[base], ray, near, far, h, face);
+      for (int i = 0; i < n0; ++i)
+        if (!in_subtracted(h[i].p, 0xffffffffu)) consider(h[i], 0);
+      for (uint32_t k = 1; k < cnt; ++k) {
+        const int n = elem_hits(csg[base + k], ray, near, far, h, face);
+        for (int i = 0; i < n; ++i)
+          if (elem_contains(csg[base], h[i].p) && !in_subtracted(h[i].p, k)) {
+            h[i].n = h[i].n * R(-1);
+            consider(h[i], k);
+          }
+      }
+      return found;
+    }
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const int n = elem_hits(csg[base + k], ray, near, far, h, face);
+      for (int i = 0; i < n; ++i) {
+        // union: inside no other element; intersection: inside every other element
+        bool keep = true;
+        for (uint32_t m = 0; m < cnt && keep; ++m)
+          if (m != k && elem_contains(csg[base + m], h[i].p) == (op == kCsgUnion)) keep = false;
+        if (keep) consider(h[i], k);
+      }
+    }
+    return found;
+  }
+
   // Object::intersect on a local (or world) ray: the first intersection of the shape.
   bool shape_hit(const Obj& o, const Ray<R>& ray, R near, R far, Isect<R>* out, int64_t* prim) const {
     *prim = -1;
     switch (o.d.shape) {
+      case kCsg: return csg_hit(o, ray, near, far, out, prim);
       case kSphere: return sphere(ray, (R)o.d.param[0], near, far, out);
       case kBox: {
         R e[3] = {(R)o.d.param[0], (R)o.d.param[1], (R)o.d.param[2]};
@@ -332,6 +441,34 @@ class SceneOracle {
       case kRect: return {{-d.param[0] / 2.0, 0.0, -d.param[1] / 2.0}, {d.param[0] / 2.0, 0.0, d.param[1] / 2.0}};
       default: return vertex_bounds(m->verts64.data(), m->verts64.size() / 3);
     }
+  }
+  // CSG bounds in its own frame: union.rs:58-72 (union of the elements' boxes), intersection.rs:
+  // 72-85 (their intersection), substraction.rs:94-104 (the base element's box).
+  AABB<double> csg_bounds(const ObjectDesc& d) const {
+    const int op = (int)d.param[0];
+    const uint32_t base = (uint32_t)d.mesh, cnt = (uint32_t)d.param[1];
+    if (cnt == 0) return {{0, 0, 0}, {0, 0, 0}};
+    auto elem_box = [&](uint32_t k) {
+      const CsgElemDesc& e = csg[base + k].d;
+      ObjectDesc tmp{};
+      tmp.shape = e.shape;
+      tmp.has_xform = 1;
+      for (int a = 0; a < 3; ++a) tmp.param[a] = e.param[a];
+      for (int a = 0; a < 16; ++a) tmp.m[a] = e.m[a];
+      return world_bounds(local_bounds(tmp, nullptr), tmp);
+    };
+    AABB<double> b = elem_box(0);
+    if (op == kCsgSubtraction) return b;
+    for (uint32_t k = 1; k < cnt; ++k) {
+      const AABB<double> e = elem_box(k);
+      if (op == kCsgUnion) {
+        b.combine_with(e);
+      } else {
+        b.bmin = {rmax(b.bmin.x, e.bmin.x), rmax(b.bmin.y, e.bmin.y), rmax(b.bmin.z, e.bmin.z)};
+        b.bmax = {rmin(b.bmax.x, e.bmax.x), rmin(b.bmax.y, e.bmax.y), rmin(b.bmax.z, e.bmax.z)};
+      }
+    }
+    return b;
   }
   // aabound.rs 'transform': the eight corners through the transform.
   static AABB<double> world_bounds(const AABB<double>& b, const ObjectDesc& d) {
@@ -393,8 +530,18 @@ class SceneOracle {
 
   void build(const float* verts, const uint64_t* vert_off, const uint32_t* idx, const uint64_t* idx_off, uint32_t n_mesh,
              const ObjectDesc* o, uint32_t n_obj, const MaterialDesc* m, uint32_t n_mat, const LightDesc* l,
-             uint32_t n_light, BuildParams bp, bool canon) {
+             uint32_t n_light, const CsgElemDesc* ce, uint32_t n_csg, BuildParams bp, bool canon) {
     canonical = canon;
+    for (uint32_t i = 0; i < n_csg; ++i) {
+      CsgElem e;
+      e.d = ce[i];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+          e.x.m[r][c] = (R)ce[i].m[4 * r + c];
+          e.x.inv[r][c] = (R)ce[i].minv[4 * r + c];
+        }
+      csg.push_back(e);
+    }
     for (uint32_t i = 0; i < n_mesh; ++i) {
       uint64_t nv = (vert_off[i + 1] - vert_off[i]), nt = (idx_off[i + 1] - idx_off[i]);
       meshes.push_back(new MeshBVH<R>(verts + 3 * vert_off[i], nv, idx + 3 * idx_off[i], nt, bp));
@@ -409,7 +556,9 @@ class SceneOracle {
           ob.x.m[r][c] = (R)o[i].m[4 * r + c];
           ob.x.inv[r][c] = (R)o[i].minv[4 * r + c];
         }
-      ob.box = world_bounds(local_bounds(o[i], o[i].shape == kMesh ? meshes[o[i].mesh] : nullptr), o[i]);
+      ob.box = world_bounds(o[i].shape == kCsg ? csg_bounds(o[i])
+                                               : local_bounds(o[i], o[i].shape == kMesh ? meshes[o[i].mesh] : nullptr),
+                            o[i]);
       objs.push_back(ob);
     }
     if (objs.empty()) return;  // scene.rs:278-284: an empty scene has no tree

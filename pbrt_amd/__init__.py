@@ -309,6 +309,17 @@ class Scene:
             call("pbrt_scene_add_material", h, kind, _ptr(c), ior, rough, ctypes.byref(out))
         for shape, mesh, mat, xf, param in desc.objects:
             p = np.array(param, np.float64)
+            if shape == 4:  # CSG: elements [mesh, mesh + count) of desc.csg
+                els = desc.csg[mesh:mesh + int(param[1])]
+                shapes = np.array([e[0] for e in els], np.int32)
+                prm = np.array([e[1] for e in els], np.float64)
+                mats = [e[2].flat() if e[2] is not None else (np.eye(4).ravel(), np.eye(4).ravel()) for e in els]
+                em = np.ascontiguousarray([m for m, _ in mats], np.float64)
+                emi = np.ascontiguousarray([mi for _, mi in mats], np.float64)
+                om, omi = (np.ascontiguousarray(a, np.float64) for a in xf.flat()) if xf is not None else (None, None)
+                call("pbrt_scene_add_csg", h, int(param[0]), len(els), _ptr(shapes), _ptr(prm), _ptr(em), _ptr(emi), mat,
+                     _ptr(om) if om is not None else None, _ptr(omi) if omi is not None else None, ctypes.byref(out))
+                continue
             if xf is not None:
                 m, mi = (np.ascontiguousarray(a, np.float64) for a in xf.flat())
                 call("pbrt_scene_add_object", h, shape, mesh, _ptr(p), mat, _ptr(m), _ptr(mi), ctypes.byref(out))

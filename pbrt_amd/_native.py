@@ -92,6 +92,7 @@ SIGNATURES = {
     "pbrt_scene_add_mesh": [P, P, U32, P, U32, ctypes.POINTER(U32)],
     "pbrt_scene_add_material": [P, I32, P, F64, F64, ctypes.POINTER(U32)],
     "pbrt_scene_add_object": [P, I32, I32, P, U32, P, P, ctypes.POINTER(U32)],
+    "pbrt_scene_add_csg": [P, I32, U32, P, P, P, P, U32, P, P, ctypes.POINTER(U32)],
     "pbrt_scene_add_point_light": [P, P, P],
     "pbrt_scene_add_area_light": [P, U32, P],
     "pbrt_scene_commit": [P],

@@ -139,6 +139,140 @@ __device__ __forceinline__ bool rect_t(V o, V d, float w, float h, float near, f
   return true;
 }
 
+// ---- CSG (shapes/csg/*.rs) ----------------------------------------------------------------------
+// Every intersection a sphere records, in order (sphere.rs:133-143): at most two distances.
+__device__ __forceinline__ int sphere_all(V o, V d, float r, float near, float far, float* t) {
+  const V l = vmul(o, -1.0f);
+  const float tca = vdot(l, d);
+  if (tca < 0.0f) return 0;
+  const float r2 = r * r;
+  const float d2 = vdot(l, l) - tca * tca;
+  if (d2 > r2) return 0;
+  const float thc = sqrtf(r2 - d2);
+  const float t0 = tca - thc, t1 = tca + thc;
+  const float tmin = fminf(t0, t1), tmax = fmaxf(t0, t1);
+  if (tmax < near || tmin > far) return 0;
+  int n = 0;
+  if (tmin > near) t[n++] = tmin;
+  if (tmax > near && tmax < far) t[n++] = tmax;
+  return n;
+}
+// Every intersection a box records, in order (aabox.rs:155-190), with its face.
+__device__ __forceinline__ int box_all(V o, V d, const float* ext, float near, float far, float* t, int* face) {
+  const V mn{0.0f - ext[0] * 0.5f, 0.0f - ext[1] * 0.5f, 0.0f - ext[2] * 0.5f};
+  const V mx{ext[0] * 0.5f, ext[1] * 0.5f, ext[2] * 0.5f};
+  const V inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  float tmin, tmax;
+  int nmin, nmax;
+  if (inv.x >= 0.0f) { tmin = (mn.x - o.x) * inv.x; tmax = (mx.x - o.x) * inv.x; nmin = 0; nmax = 1; }
+  else { tmin = (mx.x - o.x) * inv.x; tmax = (mn.x - o.x) * inv.x; nmin = 1; nmax = 0; }
+  float tymin, tymax;
+  int nymin, nymax;
+  if (inv.y >= 0.0f) { tymin = (mn.y - o.y) * inv.y; tymax = (mx.y - o.y) * inv.y; nymin = 2; nymax = 3; }
+  else { tymin = (mx.y - o.y) * inv.y; tymax = (mn.y - o.y) * inv.y; nymin = 3; nymax = 2; }
+  if (tmin > tymax || tymin > tmax) return 0;
+  if (tymin > tmin) { tmin = tymin; nmin = nymin; }
+  if (tymax < tmax) { tmax = tymax; nmax = nymax; }
+  float tzmin, tzmax;
+  int nzmin, nzmax;
+  if (inv.z >= 0.0f) { tzmin = (mn.z - o.z) * inv.z; tzmax = (mx.z - o.z) * inv.z; nzmin = 4; nzmax = 5; }
+  else { tzmin = (mx.z - o.z) * inv.z; tzmax = (mn.z - o.z) * inv.z; nzmin = 5; nzmax = 4; }
+  if (tmin > tzmax || tzmin > tmax) return 0;
+  if (tzmin > tmin) { tmin = tzmin; nmin = nzmin; }
+  if (tzmax < tmax) { tmax = tzmax; nmax = nzmax; }
+  if (tmax < near || tmin > far) return 0;
+  int n = 0;
+  if (tmin > near) {
+    face[n] = nmin;
+    t[n++] = tmin;
+  }
+  if (tmax < far) {
+    face[n] = nmax;
+    t[n++] = tmax;
+  }
+  return n;
+}
+// Shape::contain_point of an element at a point of the CSG frame (sphere.rs:146-148,
+// aabox.rs:195-202; both closed), through transform_point_to_local.
+__device__ __forceinline__ bool csg_contains(const DevCsgElem& e, V p) {
+  const V lp = xpoint(e.minv, p);
+  if (e.shape == kShapeSphere) {
+    const float r = e.param[0];
+    return vdot(lp, lp) <= r * r;
+  }
+  const float hx = e.param[0] * 0.5f, hy = e.param[1] * 0.5f, hz = e.param[2] * 0.5f;
+  return lp.x >= 0.0f - hx && lp.x <= hx && lp.y >= 0.0f - hy && lp.y <= hy && lp.z >= 0.0f - hz && lp.z <= hz;
+}
+// An element's intersections: distances, faces and the points in the CSG frame.
+__device__ __forceinline__ int csg_elem_hits(const DevCsgElem& e, V o, V d, float near, float far, float* t, int* face,
+                                             V* p) {
+  const V lo = xpoint(e.minv, o), ld = vnorm(xdir(e.minv, d));
+  int n;
+  if (e.shape == kShapeSphere) {
+    n = sphere_all(lo, ld, e.param[0], near, far, t);
+    face[0] = face[1] = 0;
+  } else {
+    n = box_all(lo, ld, e.param, near, far, t, face);
+  }
+  for (int i = 0; i < n; ++i) p[i] = xpoint(e.m, vadd(lo, vmul(ld, t[i])));
+  return n;
+}
+// The first intersection of a CSG shape on a ray of its frame (union.rs:22-42, intersection.rs:
+// 23-43, substraction.rs:22-70): the nearest kept candidate, the earliest recorded among equal
+// distances (the reference sorts them with a stable sort). *elem: the element it belongs to;
+// *face: its box face, | 8 when the normal is flipped.
+__device__ __forceinline__ bool csg_first(const SceneView& S, const DevObject& ob, V o, V d, float near, float far,
+                                         float* t_out, int32_t* elem, int32_t* face_out) {
+  const DevCsgElem* E = S.csg + ob.mesh;
+  const int op = (int)ob.param[0];
+  const uint32_t cnt = (uint32_t)ob.param[1];
+  bool found = false;
+  float best = 0.0f;
+  float t[2];
+  int face[2];
+  V p[2];
+  auto consider = [&](float tt, uint32_t k, int f) {
+    if (!found || tt < best) {
+      found = true;
+      best = tt;
+      *t_out = tt;
+      *elem = (int32_t)k;
+      *face_out = f;
+    }
+  };
+  if (op == kCsgSubtraction) {
+    if (cnt == 0) return false;
+    int n = csg_elem_hits(E[0], o, d, near, far, t, face, p);
+    for (int i = 0; i < n; ++i) {
+      bool covered = false;
+      for (uint32_t m = 1; m < cnt && !covered; ++m) covered = csg_contains(E[m], p[i]);
+      if (!covered) consider(t[i], 0, face[i]);
+    }
+    for (uint32_t k = 1; k < cnt; ++k) {
+      n = csg_elem_hits(E[k], o, d, near, far, t, face, p);
+      for (int i = 0; i < n; ++i) {
+        if (!csg_contains(E[0], p[i])) continue;
+        bool covered = false;
+        for (uint32_t m = 1; m < cnt && !covered; ++m) covered = m != k && csg_contains(E[m], p[i]);
+        if (!covered) consider(t[i], k, face[i] | 8);
+      }
+    }
+    return found;
+  }
+  const bool uni = op == kCsgUnion;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const int n = csg_elem_hits(E[k], o, d, near, far, t, face, p);
+    for (int i = 0; i < n; ++i) {
+      // union: inside no other element; intersection: inside every other element
+      bool keep = true;
+      for (uint32_t m = 0; m < cnt && keep; ++m)
+        if (m != k && csg_contains(E[m], p[i]) == uni) keep = false;
+      if (keep) consider(t[i], k, face[i]);
+    }
+  }
+  return found;
+}
+
 // A pushed traversal entry: node reference and entry distance.
 template <int BLOCK, int DEPTH>
 __device__ __forceinline__ bool push_or_flag(Stack<BLOCK, DEPTH>& st, int32_t ref, float d, int* status) {
@@ -294,6 +428,7 @@ __device__ bool object_closest(const SceneView& S, uint32_t k, V wo, V wd, float
   *u = 0.f;
   *v = 0.f;
   switch (ob.shape) {
+    case kShapeCsg: return csg_first(S, ob, o, d, near, far, t, prim, face);
     case kShapeSphere: return sphere_t(o, d, ob.param[0], near, far, t);
     case kShapeBox: return box_t(o, d, ob.param, near, far, t, face);
     case kShapeRect: return rect_t(o, d, ob.param[0], ob.param[1], near, far, t);
@@ -318,6 +453,10 @@ __device__ bool object_any(const SceneView& S, uint32_t k, V wo, V wd, float nea
   }
   float t;
   switch (ob.shape) {
+    case kShapeCsg: {
+      int32_t elem, face;
+      return csg_first(S, ob, o, d, near, far, &t, &elem, &face);
+    }
     case kShapeSphere: return sphere_t(o, d, ob.param[0], near, far, &t);
     case kShapeBox: {
       int face;
@@ -500,6 +639,29 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
   is.wo = vmul(d, -1.0f);
   const V hit = vadd(o, vmul(d, h.t));
   switch (ob.shape) {
+    case kShapeCsg: {
+      // the element's record (sphere.rs:152-187 or the box face table) through its transform to
+      // the CSG frame (transform.rs:105-116), the normal flipped for a subtracted element
+      const DevCsgElem& e = S.csg[ob.mesh + h.prim];
+      const V lo = xpoint(e.minv, o), ld = vnorm(xdir(e.minv, d));
+      const V lh = vadd(lo, vmul(ld, h.t));
+      V n, du;
+      if (e.shape == kShapeSphere) {
+        n = vnorm(lh);
+        du = vmul(V{-lh.y, lh.x, 0.0f}, 2.0f * kPi);
+      } else {
+        const int face = h.face & 7, axis = face >> 1;
+        const float sg = (face & 1) ? 1.0f : -1.0f;
+        n = V{axis == 0 ? sg : 0.0f, axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f};
+        du = V{axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f, axis == 0 ? sg : 0.0f};
+      }
+      is.p = xpoint(e.m, lh);
+      is.n = xnormal(e.minv, n);
+      if (h.face & 8) is.n = vmul(is.n, -1.0f);
+      is.wo = xdir(e.m, vmul(ld, -1.0f));
+      is.dpdu = xdir(e.m, du);
+      break;
+    }
     case kShapeSphere: {
       is.p = hit;
       is.n = vnorm(hit);

@@ -4,7 +4,19 @@
 
 namespace pbrt {
 
-enum ShapeKind : int32_t { kShapeMesh = 0, kShapeSphere = 1, kShapeBox = 2, kShapeRect = 3 };
+enum ShapeKind : int32_t { kShapeMesh = 0, kShapeSphere = 1, kShapeBox = 2, kShapeRect = 3, kShapeCsg = 4 };
+enum CsgOp : int32_t { kCsgUnion = 0, kCsgIntersection = 1, kCsgSubtraction = 2 };
+
+// One element of a CSG object (shapes/csg.rs: Elem): a sphere (param[0] = r) or a box (param =
+// extents) and its transform to the CSG object's frame (rows 0-2 of the 4x4 and of its inverse).
+// A CSG DevObject has shape kShapeCsg, param[0] = the operation, param[1] = the element count and
+// mesh = its first element in SceneView::csg.
+struct alignas(16) DevCsgElem {
+  int32_t shape, pad0, pad1, pad2;
+  float param[4];
+  float m[12], minv[12];
+};
+static_assert(sizeof(DevCsgElem) == 128, "CSG element is 128 bytes");
 enum MatKind : int32_t { kMatLambertian = 0, kMatDielectric = 1, kMatDiffuseLight = 2, kMatGGX = 3 };
 enum LightKind : int32_t { kLightPoint = 0, kLightArea = 1 };
 
@@ -42,6 +54,7 @@ struct SceneView {
   const uint32_t* idx;    // 3 per triangle, relative to the mesh's vert_base
   const DevMaterial* mats;
   const DevLight* lights;
+  const DevCsgElem* csg;
   uint32_t n_objects, n_lights;
   int has_tlas;
 };
@@ -51,9 +64,10 @@ struct SceneView {
 struct alignas(16) SceneHitRec {
   float t, u, v;
   int32_t obj;   // leaf-order object index, -1 on a miss
-  int32_t prim;  // original triangle index for meshes, -1 otherwise
+  int32_t prim;  // original triangle index for meshes, element index for CSG, -1 otherwise
   int32_t orig;  // caller's object index
-  int32_t face;  // box face (x-, x+, y-, y+, z-, z+) of a box hit
+  int32_t face;  // box face (x-, x+, y-, y+, z-, z+) of a box hit; for a CSG hit | 8 when the normal
+                 // is flipped (a subtracted element's surface)
   int32_t pad;
 };
 

@@ -24,7 +24,13 @@ namespace pbrt {
 namespace {
 
 struct HostObject {
-  int32_t shape, mesh, material, has_xform;
+  int32_t shape, mesh, material, has_xform;  // for a CSG object, mesh = its first element
+  double param[3];                           // CSG: operation, element count
+  double m[16], minv[16];
+};
+
+struct HostCsgElem {
+  int32_t shape;
   double param[3];
   double m[16], minv[16];
 };
@@ -59,6 +65,7 @@ struct pbrt_scene {
   std::vector<std::vector<float>> mesh_xyz;
   std::vector<std::vector<uint32_t>> mesh_idx;
   std::vector<HostObject> objects;
+  std::vector<HostCsgElem> csg;
   std::vector<DevMaterial> materials;
   struct HL {
     int32_t kind, object;
@@ -73,6 +80,7 @@ struct pbrt_scene {
   uint32_t* idx = nullptr;
   DevMaterial* mats = nullptr;
   DevLight* dlights = nullptr;
+  DevCsgElem* dcsg = nullptr;
   RootDesc root{};
   bool has_tlas = false;
   uint32_t tlas_depth = 0, blas_depth = 0;
@@ -104,6 +112,7 @@ SceneView view_of(const pbrt_scene* s) {
   v.idx = s->idx;
   v.mats = s->mats;
   v.lights = s->dlights;
+  v.csg = s->dcsg;
   v.n_objects = (uint32_t)s->objects.size();
   v.n_lights = (uint32_t)s->lights.size();
   v.has_tlas = s->has_tlas ? 1 : 0;
@@ -155,6 +164,39 @@ Box64 world_box(const Box64& b, const HostObject& o) {
         r.lo[2] = std::fmin(r.lo[2], pz); r.hi[2] = std::fmax(r.hi[2], pz);
       }
   return r;
+}
+
+// CSG bounds in the object's frame (csg/union.rs:58-72: union of the elements' boxes,
+// csg/intersection.rs:72-85: their intersection, csg/substraction.rs:94-104: the base's box).
+Box64 csg_bounds(const pbrt_scene* s, const HostObject& o) {
+  const int op = (int)o.param[0];
+  const uint32_t cnt = (uint32_t)o.param[1];
+  auto elem_box = [&](uint32_t k) {
+    const HostCsgElem& e = s->csg[(size_t)o.mesh + k];
+    HostObject t{};
+    t.has_xform = 1;
+    std::memcpy(t.m, e.m, sizeof(t.m));
+    const Box64 lb = e.shape == kShapeSphere
+                         ? Box64{{-e.param[0], -e.param[0], -e.param[0]}, {e.param[0], e.param[0], e.param[0]}}
+                         : Box64{{0.0 - e.param[0] * 0.5, 0.0 - e.param[1] * 0.5, 0.0 - e.param[2] * 0.5},
+                                 {e.param[0] * 0.5, e.param[1] * 0.5, e.param[2] * 0.5}};
+    return world_box(lb, t);
+  };
+  Box64 b = elem_box(0);
+  if (op == kCsgSubtraction) return b;
+  for (uint32_t k = 1; k < cnt; ++k) {
+    const Box64 e = elem_box(k);
+    for (int a = 0; a < 3; ++a) {
+      if (op == kCsgUnion) {
+        b.lo[a] = std::fmin(b.lo[a], e.lo[a]);
+        b.hi[a] = std::fmax(b.hi[a], e.hi[a]);
+      } else {
+        b.lo[a] = std::fmax(b.lo[a], e.lo[a]);
+        b.hi[a] = std::fmin(b.hi[a], e.hi[a]);
+      }
+    }
+  }
+  return b;
 }
 
 // BVH::subdivide (src/bvh.rs:291-335): median split on the widest centroid axis, stable sort,
@@ -283,7 +325,7 @@ int pbrt_scene_destroy(pbrt_scene* s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     for (void* p : {(void*)s->tlas, (void*)s->blas, (void*)s->tris, (void*)s->objs, (void*)s->verts, (void*)s->idx,
-                    (void*)s->mats, (void*)s->dlights, (void*)s->status, (void*)s->spill})
+                    (void*)s->mats, (void*)s->dlights, (void*)s->dcsg, (void*)s->status, (void*)s->spill})
       if (p) (void)hipFree(p);
     free_paths(s->pb);
     delete s;
@@ -339,6 +381,48 @@ int pbrt_scene_add_object(pbrt_scene* s, int shape, int mesh, const double param
       std::memcpy(o.m, m16, sizeof(o.m));
       std::memcpy(o.minv, minv16, sizeof(o.minv));
     }
+    s->objects.push_back(o);
+    if (id) *id = (uint32_t)s->objects.size() - 1;
+  });
+}
+
+int pbrt_scene_add_csg(pbrt_scene* s, int op, uint32_t n, const int* shapes, const double* params, const double* m,
+                       const double* minv, uint32_t material, const double* m16, const double* minv16, uint32_t* id) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (op < kCsgUnion || op > kCsgSubtraction) throw Error(PBRT_ERR_INVALID_ARG, "unknown CSG operation");
+    if (n == 0 || n > 64 || !shapes || !params || !m || !minv)
+      throw Error(PBRT_ERR_INVALID_ARG, "a CSG object needs 1 to 64 elements with transforms");
+    if (material >= s->materials.size()) throw Error(PBRT_ERR_INVALID_ARG, "no such material");
+    if ((m16 == nullptr) != (minv16 == nullptr)) throw Error(PBRT_ERR_INVALID_ARG, "a transform needs its inverse");
+    auto affine = [](const double* a) {
+      for (int k = 12; k < 16; ++k)
+        if (a[k] != (k == 15 ? 1.0 : 0.0)) return false;
+      return true;
+    };
+    HostObject o{};
+    o.shape = kShapeCsg;
+    o.mesh = (int32_t)s->csg.size();
+    o.material = (int32_t)material;
+    o.param[0] = op;
+    o.param[1] = n;
+    if (m16) {
+      if (!affine(m16) || !affine(minv16)) throw Error(PBRT_ERR_INVALID_ARG, "transforms must be affine");
+      o.has_xform = 1;
+      std::memcpy(o.m, m16, sizeof(o.m));
+      std::memcpy(o.minv, minv16, sizeof(o.minv));
+    }
+    std::vector<HostCsgElem> els(n);
+    for (uint32_t k = 0; k < n; ++k) {
+      if (shapes[k] != kShapeSphere && shapes[k] != kShapeBox)
+        throw Error(PBRT_ERR_INVALID_ARG, "CSG elements are spheres and boxes");
+      if (!affine(m + 16 * k) || !affine(minv + 16 * k)) throw Error(PBRT_ERR_INVALID_ARG, "transforms must be affine");
+      els[k].shape = shapes[k];
+      for (int a = 0; a < 3; ++a) els[k].param[a] = params[3 * k + a];
+      std::memcpy(els[k].m, m + 16 * k, sizeof(els[k].m));
+      std::memcpy(els[k].minv, minv + 16 * k, sizeof(els[k].minv));
+    }
+    s->csg.insert(s->csg.end(), els.begin(), els.end());
     s->objects.push_back(o);
     if (id) *id = (uint32_t)s->objects.size() - 1;
   });
@@ -426,6 +510,7 @@ int pbrt_scene_commit(pbrt_scene* s) {
         case kShapeRect:
           b = {{-o.param[0] / 2.0, 0.0, -o.param[1] / 2.0}, {o.param[0] / 2.0, 0.0, o.param[1] / 2.0}};
           break;
+        case kShapeCsg: b = csg_bounds(s, o); break;
         default: b = info[o.mesh].bounds; break;
       }
       boxes[k] = world_box(b, o);
@@ -484,6 +569,19 @@ int pbrt_scene_commit(pbrt_scene* s) {
       }
       dl.push_back(d);
     }
+    std::vector<DevCsgElem> dc(std::max<size_t>(1, s->csg.size()));
+    for (size_t k = 0; k < s->csg.size(); ++k) {
+      const HostCsgElem& e = s->csg[k];
+      DevCsgElem& d = dc[k];
+      d = DevCsgElem{};
+      d.shape = e.shape;
+      for (int a = 0; a < 3; ++a) d.param[a] = (float)e.param[a];
+      for (int a = 0; a < 12; ++a) {
+        d.m[a] = (float)e.m[a];
+        d.minv[a] = (float)e.minv[a];
+      }
+    }
+    dev_upload(&s->dcsg, dc);
     dev_upload(&s->tlas, tlas);
     dev_upload(&s->blas, blas);
     dev_upload(&s->tris, tris);

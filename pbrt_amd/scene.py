@@ -14,7 +14,8 @@ import math
 
 import numpy as np
 
-SHAPE = {"mesh": 0, "sphere": 1, "aabox": 2, "box": 2, "rectangle": 3, "rect": 3}
+SHAPE = {"mesh": 0, "sphere": 1, "aabox": 2, "box": 2, "rectangle": 3, "rect": 3, "csg": 4}
+CSG_OP = {"union": 0, "intersection": 1, "subtraction": 2, "substraction": 2}
 MATERIAL = {"lambertian": 0, "dielectric": 1, "diffuse_light": 2, "ggx": 3}
 LIGHT = {"point": 0, "area": 1}
 
@@ -22,6 +23,8 @@ OBJECT_DTYPE = np.dtype([("shape", "<i4"), ("mesh", "<i4"), ("material", "<i4"),
                          ("param", "<f8", 3), ("m", "<f8", 16), ("minv", "<f8", 16)])
 MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("pad", "<i4"), ("rgb", "<f8", 3), ("ior", "<f8"), ("roughness", "<f8")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("object", "<i4"), ("pos", "<f8", 3), ("rgb", "<f8", 3)])
+CSG_ELEM_DTYPE = np.dtype([("shape", "<i4"), ("pad", "<i4"), ("param", "<f8", 3), ("m", "<f8", 16),
+                           ("minv", "<f8", 16)])
 
 
 def _mat_mul(a, b):
@@ -109,6 +112,7 @@ class SceneDesc:
         self.materials = []
         self.objects = []
         self.lights = []
+        self.csg = []         # (shape, param, transform) of every CSG element, objects' elements contiguous
 
     def add_mesh(self, vertices, faces):
         self.meshes.append((np.ascontiguousarray(vertices, np.float32).reshape(-1, 3),
@@ -122,6 +126,20 @@ class SceneDesc:
     def add_object(self, shape, material, param=(0.0, 0.0, 0.0), mesh=-1, transform=None):
         p = list(param) + [0.0] * (3 - len(param))
         self.objects.append((SHAPE[shape], int(mesh), int(material), transform, tuple(float(x) for x in p)))
+        return len(self.objects) - 1
+
+    def add_csg(self, op, elements, material, transform=None):
+        """A CSG object (shapes/csg/*.rs): `op` is "union", "intersection" or "subtraction";
+        `elements` lists (shape, param, transform) with shape "sphere" or "aabox", in the order the
+        operation's constructor receives them (for a subtraction the first is the base)."""
+        first = len(self.csg)
+        for shape, param, xf in elements:
+            if SHAPE[shape] not in (SHAPE["sphere"], SHAPE["aabox"]):
+                raise ValueError("CSG elements are spheres and boxes")
+            p = list(param) + [0.0] * (3 - len(param))
+            self.csg.append((SHAPE[shape], tuple(float(x) for x in p), xf))
+        self.objects.append((SHAPE["csg"], first, int(material), transform,
+                             (float(CSG_OP[op]), float(len(elements)), 0.0)))
         return len(self.objects) - 1
 
     def add_point_light(self, pos, rgb):
@@ -145,6 +163,14 @@ class SceneDesc:
             else:
                 a[i]["m"] = np.eye(4).ravel()
                 a[i]["minv"] = np.eye(4).ravel()
+        return a
+
+    def csg_array(self):
+        a = np.zeros(max(len(self.csg), 1), CSG_ELEM_DTYPE)
+        for i, (shape, param, xf) in enumerate(self.csg):
+            a[i]["shape"], a[i]["param"] = shape, param
+            m, mi = xf.flat() if xf is not None else (np.eye(4).ravel(), np.eye(4).ravel())
+            a[i]["m"], a[i]["minv"] = m, mi
         return a
 
     def material_array(self):
@@ -182,7 +208,13 @@ def from_stage_json(scene, meshes):
         m = o["material"]
         mat = d.add_material(m["kind"] if m["kind"] in MATERIAL else "lambertian", m["color"], ior=m.get("ior", 1.5))
         xf = Transform.from_steps(o["transform"]) if o.get("transform") else None
-        if o["shape"] == "mesh":
+        if o["shape"] == "csg":
+            # the builder pops the elements off a stack: the constructor receives them reversed
+            # (scene_builder.rs:117-145)
+            els = [(e["shape"], e["param"], Transform.from_steps(e["transform"]) if e.get("transform") else None)
+                   for e in reversed(o["elems"])]
+            d.add_csg(o["op"], els, mat, transform=xf)
+        elif o["shape"] == "mesh":
             key = (o["file"], o.get("reverse", False))
             if key not in mesh_ids:
                 v, f = meshes[o["file"]]

@@ -1,7 +1,8 @@
 """Writes the reference's measured `.stage` scenes as JSON fixtures (tests/golden/stages/*.json).
 
 The four scenes of the reference's scene-mode known answers (docs/mesures_bvh.md:729-738):
-cornell_box_canonical, bunny_mesh, many_spheres, many_meshes. Only the data the geometry needs is
+cornell_box_canonical, bunny_mesh, many_spheres, many_meshes; and the reference's CSG scene,
+csg_union_spheres (a union of nine spheres over a floor rectangle). Only the data the geometry needs is
 kept: the camera, and per object its shape, its parameters, its material kind and colour, and its
 transform steps in the order listed. Objects are stored in file order; the reference's scene builder
 adds them to the scene in reverse (src/loader/visitors/scene_builder.rs:53-57), which the test
@@ -14,7 +15,7 @@ import json
 import os
 import sys
 
-SCENES = ["cornell_box_canonical", "bunny_mesh", "many_spheres", "many_meshes"]
+SCENES = ["cornell_box_canonical", "bunny_mesh", "many_spheres", "many_meshes", "csg_union_spheres"]
 
 
 def tokens(text):
@@ -79,8 +80,29 @@ class Parser:
         shape["material"] = self.material()
         return shape
 
+    def steps(self):
+        self.expect("transform")
+        self.expect("{")
+        out = []
+        while self.t[self.i] != "}":
+            op = self.next()
+            out.append([op] + (self.vec() if op == "translate" else [self.num()]))
+        self.expect("}")
+        return out
+
     def shape(self):
         kind = self.next()
+        if kind == "csg":
+            op = self.next()
+            self.expect("{")
+            elems = []
+            while self.t[self.i] != "}":
+                self.expect("elem")
+                e = self.shape()
+                e["transform"] = self.steps()
+                elems.append(e)
+            self.expect("}")
+            return {"shape": "csg", "op": op, "elems": elems}
         if kind == "rectangle":
             return {"shape": "rectangle", "param": self.vec(2)}
         if kind == "aabox":
