@@ -221,6 +221,17 @@ typedef struct pbrt_render_stats {
  * in FP64 in a fixed sample order. */
 int pbrt_render(pbrt_scene* scene, const pbrt_render_config* config, double* film_out, pbrt_render_stats* stats);
 
+/* A batch of n render jobs (1 to 16) on one scene traced together (no counterpart in the
+ * reference, which renders one image per run): every pass gives each unfinished job a share of
+ * about 4M paths (at least one sample per pixel), and the jobs' paths share the wavefront's
+ * device-resident queues and its persistent traversal kernels. Each job's film is bit for bit what
+ * pbrt_render returns for that job alone. films_out: n host pointers (NULL entries skipped), each
+ * width*height*3 doubles. use_graphs != 0: each bounce's kernels (closest hit, queue reset, shade,
+ * shadow rays) are captured once per queue parity as a hipGraph and replayed for every bounce of
+ * every pass. stats: totals over the batch. */
+int pbrt_render_jobs(pbrt_scene* scene, const pbrt_render_config* configs, uint32_t n, double* const* films_out,
+                     int use_graphs, pbrt_render_stats* stats);
+
 /* ---- neural radiance cache ---------------------------------------------------------------------
  * No counterpart in the reference, whose only integrator is plain path tracing
  * (src/integrators/path.rs:13-97). An MLP 81 -> 64 -> 64 -> 64 -> 64 -> 3 (ReLU hidden layers,

@@ -388,6 +388,23 @@ class Scene:
         call("pbrt_render", self.h, ctypes.byref(c), _ptr(out) if film else None, ctypes.byref(st))
         return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds)
 
+    def render_jobs(self, jobs, use_graphs=True):
+        """Render a batch of jobs together (pbrt_render_jobs). `jobs`: dicts with width, height,
+        spp, max_depth, eye, look and optionally up, fov, seed, near, far. Returns the films (each
+        (height, width, 3) float64) and the batch statistics."""
+        n = len(jobs)
+        cfgs = (_native.RenderConfig * n)()
+        films = []
+        for k, j in enumerate(jobs):
+            cfgs[k] = self._config(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"], j["look"],
+                                   j.get("up", (0.0, 1.0, 0.0)), j.get("fov", np.pi / 2), j.get("seed", 1),
+                                   j.get("near", 1e-4), j.get("far", 1000.0), 0)
+            films.append(np.zeros((j["height"], j["width"], 3), np.float64))
+        ptrs = (ctypes.c_void_p * n)(*[f.ctypes.data for f in films])
+        st = _native.RenderStats()
+        call("pbrt_render_jobs", self.h, cfgs, n, ptrs, int(bool(use_graphs)), ctypes.byref(st))
+        return films, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds)
+
     def render_ex(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
                   near=1e-4, far=1000.0, samples_per_pass=0, cache=None, query_vertex=3, train_stride=33,
                   train_batch=65536, train_steps=1, reference=None, record=True, debug_slots=0):

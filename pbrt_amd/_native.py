@@ -102,6 +102,7 @@ SIGNATURES = {
     "pbrt_scene_status": [P, ctypes.POINTER(I32)],
     "pbrt_render": [P, P, P, P],
     "pbrt_render_ex": [P, P, P, ctypes.POINTER(NrcRender), ctypes.POINTER(Progress), P, P],
+    "pbrt_render_jobs": [P, P, U32, P, I32, P],
     "pbrt_nrc_create": [P, U64, PP],
     "pbrt_nrc_destroy": [P],
     "pbrt_nrc_set_bounds": [P, P, P],

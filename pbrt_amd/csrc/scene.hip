@@ -1028,7 +1028,19 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   const SceneHitRec h = b.hit[j];
   if (h.obj < 0) return;  // no infinite light in these scenes: the path ends
   uint4 info = b.info[j];
-  const uint64_t path = (uint64_t)info.x * p.spp + info.y;
+  // the path's job (a batch of jobs) or the render's own parameters
+  uint32_t spp = p.spp, max_depth = p.max_depth;
+  uint64_t seed = p.seed;
+  float near = p.near, far = p.far;
+  if (p.n_jobs) {
+    const JobDesc& J = p.jobs[info.w];
+    spp = J.spp;
+    max_depth = J.max_depth;
+    seed = J.seed;
+    near = J.near;
+    far = J.far;
+  }
+  const uint64_t path = (uint64_t)info.x * spp + info.y;
   float4 beta4 = b.beta[j], L4 = b.L[j];
   const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
   const V rd{d4.x, d4.y, d4.z};
@@ -1072,14 +1084,14 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
       }
     }
   }
-  if (bounce == p.max_depth) return;
+  if (bounce == max_depth) return;
   if (S.n_lights > 0) {
-    const float u = urand(p.seed, path, bounce, kSlotLight);
+    const float u = urand(seed, path, bounce, kSlotLight);
     const float fi = fminf(u * (float)S.n_lights, (float)S.n_lights - 1.0f);
     const DevLight Ld = S.lights[(uint32_t)fi];
     const float pl = 1.0f / (float)S.n_lights;
     LiOut ls;
-    if (sample_li(S, Ld, is, p.seed, path, bounce, &ls)) {
+    if (sample_li(S, Ld, is, seed, path, bounce, &ls)) {
       const V f0 = mat_f(m, is, vmul(rd, -1.0f), ls.wi);
       const float c = fabsf(vdot(ls.wi, is.n));
       const V f{f0.x * c, f0.y * c, f0.z * c};
@@ -1093,7 +1105,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
     }
   }
   ScatterOut sc;
-  if (!scatter(m, is, p.seed, path, bounce, &sc)) return;
+  if (!scatter(m, is, seed, path, bounce, &sc)) return;
   const float c = fabsf(vdot(sc.d, is.n));
   beta4.x = beta4.x * ((sc.att.x * c) / sc.pdf);
   beta4.y = beta4.y * ((sc.att.y * c) / sc.pdf);
@@ -1103,8 +1115,8 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) return;
   beta4.w = sc.specular ? 1.0f : 0.0f;
   b.beta[j] = beta4;
-  b.ray_o[j] = make_float4(sc.o.x, sc.o.y, sc.o.z, p.near);
-  b.ray_d[j] = make_float4(sc.d.x, sc.d.y, sc.d.z, p.far);
+  b.ray_o[j] = make_float4(sc.o.x, sc.o.y, sc.o.z, near);
+  b.ray_d[j] = make_float4(sc.d.x, sc.d.y, sc.d.z, far);
   out->next = true;
 }
 
@@ -1200,6 +1212,66 @@ __global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, RenderParam
   }
 }
 
+// The job of path j (or pixel j) of a batch pass: the last job whose base is at most j.
+__device__ __forceinline__ uint32_t job_of(const JobDesc* jobs, uint32_t n, uint32_t j, bool pixel) {
+  uint32_t k = 0;
+  for (uint32_t i = 1; i < n; ++i)
+    if ((pixel ? jobs[i].pix_base : jobs[i].path_base) <= j) k = i;
+  return k;
+}
+
+// k_generate for a batch of jobs: path j belongs to the job whose paths start at or before it;
+// within the job, the same pixel / sample numbering and camera ray as a render of that job alone.
+__global__ void k_generate_jobs(RenderParams p, PathBuffers b) {
+  const uint32_t n = p.pass_paths;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t k = job_of(p.jobs, p.n_jobs, j, false);
+    const JobDesc& J = p.jobs[k];
+    const uint32_t npix = J.width * J.height, local = j - J.path_base;
+    const uint32_t pixel = local % npix, sample = J.pass_first + local / npix;
+    const uint64_t path = (uint64_t)pixel * J.spp + sample;
+    const double jx = to_unit(hash4(J.seed, path, 0, kSlotJx)), jy = to_unit(hash4(J.seed, path, 0, kSlotJy));
+    double cv[3], o[3], d[3];
+    cam_point(J.r2s, (double)(pixel % J.width) + jx, (double)(pixel / J.width) + jy, 0.0, cv);
+    dnorm(cv);
+    cam_point(J.c2w, 0.0, 0.0, 0.0, o);
+    d[0] = J.c2w[0] * cv[0] + J.c2w[1] * cv[1] + J.c2w[2] * cv[2];
+    d[1] = J.c2w[4] * cv[0] + J.c2w[5] * cv[1] + J.c2w[6] * cv[2];
+    d[2] = J.c2w[8] * cv[0] + J.c2w[9] * cv[1] + J.c2w[10] * cv[2];
+    dnorm(d);
+    b.ray_o[j] = make_float4((float)o[0], (float)o[1], (float)o[2], J.near);
+    b.ray_d[j] = make_float4((float)d[0], (float)d[1], (float)d[2], J.far);
+    b.beta[j] = make_float4(1.f, 1.f, 1.f, 1.f);
+    b.L[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    b.info[j] = make_uint4(pixel, sample, 0u, k);
+    b.queue[0][j] = j;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *counter(b.counts, kCntQueue0) = n;
+    *counter(b.counts, kCntQueue1) = 0;
+    *counter(b.counts, kCntShadow) = 0;
+  }
+}
+
+// k_accumulate for a batch: every job's pixels, each job's samples of the pass in sample order.
+__global__ void k_accumulate_jobs(RenderParams p, PathBuffers b) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < p.batch_pixels; g += gridDim.x * blockDim.x) {
+    const JobDesc& J = p.jobs[job_of(p.jobs, p.n_jobs, g, true)];
+    const uint32_t npix = J.width * J.height, px = g - J.pix_base;
+    double* f = b.film + J.film_base + 3 * (size_t)px;
+    double acc[3] = {f[0], f[1], f[2]};
+    for (uint32_t s = 0; s < J.pass_samples; ++s) {
+      const float4 L = b.L[(size_t)J.path_base + (size_t)s * npix + px];
+      acc[0] += (double)L.x;
+      acc[1] += (double)L.y;
+      acc[2] += (double)L.z;
+    }
+    f[0] = acc[0];
+    f[1] = acc[1];
+    f[2] = acc[2];
+  }
+}
+
 // Per-pixel FP64 sums over this pass's samples, in sample order.
 __global__ void k_accumulate(RenderParams p, PathBuffers b) {
   const uint32_t npix = p.width * p.height;
@@ -1240,6 +1312,17 @@ hipError_t launch_generate(const SceneView&, const RenderParams& p, const PathBu
 hipError_t launch_generate_nrc(const RenderParams& p, const PathBuffers& b, void* stream) {
   const uint32_t n = p.width * p.height * p.pass_samples;
   hipLaunchKernelGGL(k_generate<true>, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_generate_jobs(const RenderParams& p, const PathBuffers& b, void* stream) {
+  hipLaunchKernelGGL(k_generate_jobs, dim3(blocks_for(p.pass_paths, 256)), dim3(256), 0, (hipStream_t)stream, p, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate_jobs(const RenderParams& p, const PathBuffers& b, void* stream) {
+  hipLaunchKernelGGL(k_accumulate_jobs, dim3(blocks_for(p.batch_pixels, 256)), dim3(256), 0, (hipStream_t)stream, p,
+                     b);
   return hipGetLastError();
 }
 

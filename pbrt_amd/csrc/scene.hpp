@@ -87,6 +87,18 @@ struct SceneLaunch {
 hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit);
 
 // ---- wavefront renderer ----
+// One render job of a batch rendered together (pbrt_render_jobs): its camera, sampler seed and
+// sizes, the samples of the current pass, and where its paths, pixels and film start.
+struct JobDesc {
+  double r2s[16], c2w[16];
+  uint64_t seed, film_base;  // film_base: first double of the job's film
+  uint32_t width, height, spp, max_depth;
+  uint32_t pass_first, pass_samples;  // samples [pass_first, pass_first + pass_samples) this pass
+  uint32_t path_base, pix_base;       // first path of the job in the pass, first pixel in the batch
+  float near, far;
+  uint32_t pad[2];
+};
+
 struct RenderParams {
   uint32_t width, height, spp, max_depth;
   uint32_t pass_first_sample, pass_samples;  // samples [first, first + count) of every pixel
@@ -98,6 +110,9 @@ struct RenderParams {
   // path when (j + train_offset) % train_stride == 0 (never if train_stride is 0), traced to
   // max_depth with its vertices recorded.
   uint32_t nrc, query_vertex, train_stride, train_offset;
+  // a batch of jobs (n_jobs > 0): camera, seed, spp and depth come from jobs[info.w] of each path
+  const JobDesc* jobs;
+  uint32_t n_jobs, pass_paths, batch_pixels, pad;
 };
 
 // Per-vertex record of a training path, field-major: rec[(f * max_depth + m - 1) * train_cap + rank]
@@ -157,6 +172,9 @@ hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const Pat
 hipError_t launch_shadow_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, uint2* spill,
                              uint32_t spill_per_thread, uint32_t grid, int* status, void* stream);
 hipError_t launch_generate_nrc(const RenderParams& p, const PathBuffers& b, void* stream);
+// a pass of a job batch: p.n_jobs jobs, p.pass_paths paths, p.batch_pixels pixels in all
+hipError_t launch_generate_jobs(const RenderParams& p, const PathBuffers& b, void* stream);
+hipError_t launch_accumulate_jobs(const RenderParams& p, const PathBuffers& b, void* stream);
 
 constexpr uint32_t kSceneBlock = 128;
 constexpr int kSceneStack = 16;

@@ -950,4 +950,174 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
   });
 }
 
+int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, double* const* films_out,
+                     int use_graphs, pbrt_render_stats* stats) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (!cfg || n == 0 || n > 16) throw Error(PBRT_ERR_INVALID_ARG, "1 to 16 jobs");
+    uint64_t total_pix = 0;
+    uint32_t max_depth = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+      const pbrt_render_config& c = cfg[k];
+      if (!c.width || !c.height || !c.spp || !c.max_depth) throw Error(PBRT_ERR_INVALID_ARG, "bad render config");
+      total_pix += (uint64_t)c.width * c.height;
+      max_depth = std::max(max_depth, c.max_depth);
+    }
+    if (total_pix > (1u << 26)) throw Error(PBRT_ERR_INVALID_ARG, "more than 2^26 pixels in the batch");
+    std::lock_guard<std::mutex> lock(s->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = s->ctx->stream;
+    // the pass schedule, fixed in advance: each pass, every unfinished job takes its share of the
+    // path budget, at least one sample per pixel
+    const uint64_t budget = 1u << 22;
+    std::vector<JobDesc> base(n);
+    uint64_t film_words = 0, pix = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+      const pbrt_render_config& c = cfg[k];
+      const Pinhole cam(c.width, c.height, c.fov, c.near, c.far,
+                        look_at({c.eye[0], c.eye[1], c.eye[2]}, {c.look[0], c.look[1], c.look[2]},
+                                {c.up[0], c.up[1], c.up[2]}));
+      JobDesc& J = base[k];
+      J = JobDesc{};
+      for (int i = 0; i < 16; ++i) {
+        J.r2s[i] = cam.r2s.m[i / 4][i % 4];
+        J.c2w[i] = cam.c2w.m[i / 4][i % 4];
+      }
+      J.seed = c.seed;
+      J.film_base = film_words;
+      J.width = c.width;
+      J.height = c.height;
+      J.spp = c.spp;
+      J.max_depth = c.max_depth;
+      J.pix_base = (uint32_t)pix;
+      J.near = (float)c.near;
+      J.far = (float)c.far;
+      film_words += (uint64_t)c.width * c.height * 3;
+      pix += (uint64_t)c.width * c.height;
+    }
+    std::vector<JobDesc> sched;
+    std::vector<uint32_t> pass_paths;
+    std::vector<uint32_t> done(n, 0);
+    uint64_t capacity = 0;
+    for (;;) {
+      uint32_t active = 0;
+      for (uint32_t k = 0; k < n; ++k) active += done[k] < cfg[k].spp ? 1 : 0;
+      if (!active) break;
+      uint64_t paths = 0;
+      for (uint32_t k = 0; k < n; ++k) {
+        JobDesc J = base[k];
+        const uint64_t npix = (uint64_t)J.width * J.height;
+        const uint32_t left = J.spp - done[k];
+        const uint64_t share = std::max<uint64_t>(1, budget / ((uint64_t)active * npix));
+        J.pass_first = done[k];
+        J.pass_samples = left ? (uint32_t)std::min<uint64_t>(left, share) : 0;
+        J.path_base = (uint32_t)paths;
+        paths += npix * J.pass_samples;
+        done[k] += J.pass_samples;
+        sched.push_back(J);
+      }
+      pass_paths.push_back((uint32_t)paths);
+      capacity = std::max(capacity, paths);
+    }
+    if (capacity > (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "a pass larger than 2^28 paths");
+    const uint32_t passes = (uint32_t)pass_paths.size();
+    ensure_paths(s, (uint32_t)capacity);
+
+    std::vector<void*> owned;
+    struct Release {
+      std::vector<void*>& v;
+      ~Release() {
+        for (void* q : v) (void)hipFree(q);
+      }
+    } release{owned};
+    auto alloc = [&](size_t bytes) {
+      void* q = nullptr;
+      PBRT_HIP(hipMalloc(&q, bytes ? bytes : 16));
+      owned.push_back(q);
+      return q;
+    };
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    struct ReleaseGraphs {
+      hipGraphExec_t* e;
+      hipGraph_t* g;
+      ~ReleaseGraphs() {
+        for (int i = 0; i < 2; ++i) {
+          if (e[i]) (void)hipGraphExecDestroy(e[i]);
+          if (g[i]) (void)hipGraphDestroy(g[i]);
+        }
+      }
+    } release_graphs{exec, graph};
+
+    double* film = (double*)alloc(film_words * sizeof(double));
+    JobDesc* all = (JobDesc*)alloc(sched.size() * sizeof(JobDesc));
+    JobDesc* cur = (JobDesc*)alloc(n * sizeof(JobDesc));
+    PBRT_HIP(hipMemsetAsync(film, 0, film_words * sizeof(double), st));
+    PBRT_HIP(hipMemsetAsync(s->pb.counts, 0, kCounterWords * sizeof(uint32_t), st));
+    PBRT_HIP(hipMemcpyAsync(all, sched.data(), sched.size() * sizeof(JobDesc), hipMemcpyHostToDevice, st));
+    PBRT_HIP(hipStreamSynchronize(st));
+    PathBuffers pb = s->pb;
+    pb.film = film;
+    RenderParams p{};
+    p.jobs = cur;
+    p.n_jobs = n;
+    p.batch_pixels = (uint32_t)pix;
+    const SceneView v = view_of(s);
+    const uint32_t grid = persistent_grid(s, (uint32_t)capacity);
+    const uint32_t spill_per = ensure_spill(s, grid);
+    auto bounce = [&](int q) -> hipError_t {
+      hipError_t e = launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st);
+      if (e == hipSuccess) e = launch_reset_queue(pb, q, st);
+      if (e == hipSuccess) e = launch_shade(v, p, pb, q, st);
+      if (e == hipSuccess) e = launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st);
+      return e;
+    };
+    if (use_graphs) {
+      for (int q = 0; q < 2; ++q) {
+        PBRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = bounce(q);
+        const hipError_t e2 = hipStreamEndCapture(st, &graph[q]);
+        PBRT_HIP(e);
+        PBRT_HIP(e2);
+        PBRT_HIP(hipGraphInstantiate(&exec[q], graph[q], nullptr, nullptr, 0));
+      }
+    }
+    for (uint32_t k = 0; k < passes; ++k) {
+      PBRT_HIP(hipMemcpyAsync(cur, all + (size_t)k * n, n * sizeof(JobDesc), hipMemcpyDeviceToDevice, st));
+      p.pass_paths = pass_paths[k];
+      PBRT_HIP(launch_generate_jobs(p, pb, st));
+      int q = 0;
+      for (uint32_t b = 0; b < max_depth; ++b) {
+        if (use_graphs) PBRT_HIP(hipGraphLaunch(exec[q], st));
+        else PBRT_HIP(bounce(q));
+        q ^= 1;
+      }
+      PBRT_HIP(launch_accumulate_jobs(p, pb, st));
+    }
+    PBRT_HIP(hipStreamSynchronize(st));
+    uint64_t rays = 0;
+    PBRT_HIP(hipMemcpy(&rays, counter(pb.counts, kCntRays), sizeof(rays), hipMemcpyDeviceToHost));
+    if (films_out) {
+      std::vector<double> sum(film_words);
+      PBRT_HIP(hipMemcpy(sum.data(), film, film_words * sizeof(double), hipMemcpyDeviceToHost));
+      for (uint32_t k = 0; k < n; ++k) {
+        if (!films_out[k]) continue;
+        const size_t m = (size_t)cfg[k].width * cfg[k].height * 3;
+        for (size_t i = 0; i < m; ++i) films_out[k][i] = sum[base[k].film_base + i] / (double)cfg[k].spp;
+      }
+    }
+    int status = 0;
+    PBRT_HIP(hipMemcpy(&status, s->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (status) throw Error(PBRT_ERR_INTERNAL, "a traversal stack overflowed");
+    if (stats) {
+      uint64_t paths = 0;
+      for (uint32_t k = 0; k < n; ++k) paths += (uint64_t)cfg[k].width * cfg[k].height * cfg[k].spp;
+      stats->rays = rays;
+      stats->paths = paths;
+      stats->passes = passes;
+      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+  });
+}
+
 }  // extern "C"

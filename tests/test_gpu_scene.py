@@ -195,3 +195,23 @@ def test_csg_render_matches_oracle(gpu_ctx):
     scale = float(f32.mean())
     assert scale > 0
     assert rmse(film, f32) / scale < 0.02, rmse(film, f32) / scale
+
+
+@pytest.mark.parametrize("use_graphs", [False, True])
+def test_job_batch_equals_single_renders(gpu_ctx, use_graphs):
+    """Jobs of different sizes, sample counts, depths, seeds and cameras rendered as one batch give,
+    bit for bit, the films of rendering each alone (with per-bounce graphs replayed for every
+    bounce of every pass, or launched eagerly)."""
+    desc, cam = scenes.small()
+    scene = pbrt_amd.Scene(gpu_ctx, desc)
+    jobs = [dict(width=64, height=48, spp=5, max_depth=8, eye=cam["eye"], look=cam["look"], fov=cam["fov"], seed=3),
+            dict(width=32, height=32, spp=3, max_depth=4, eye=(5.0, 4.0, 20.0), look=(0.0, 0.0, 0.0), fov=0.7,
+                 seed=9),
+            dict(width=80, height=20, spp=1, max_depth=6, eye=(-8.0, 2.0, 15.0), look=(0.0, -1.0, 0.0), fov=0.9,
+                 seed=1)]
+    films, stats = scene.render_jobs(jobs, use_graphs=use_graphs)
+    for j, f in zip(jobs, films):
+        alone, _ = scene.render(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"], j["look"], fov=j["fov"],
+                                seed=j["seed"])
+        assert f.tobytes() == alone.tobytes()
+    assert stats["paths"] == sum(j["width"] * j["height"] * j["spp"] for j in jobs)
