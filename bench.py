@@ -179,6 +179,51 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
     }
 
 
+def jobs_measure(pa, ctx):
+    """The batched-jobs workload: 256 mesh instances (1k-200k triangles each, 9.1M in all) and 32
+    CSG objects; eight render jobs (256^2 to 1024^2, 16 or 64 spp, depth 8) rendered as one batch
+    sharing the wavefront's queues with per-bounce graphs, against the same jobs rendered one after
+    the other. GPU time from HIP events on the render stream around each call."""
+    from pbrt_amd import scenes
+    t0 = time.time()
+    desc = scenes.batch_scene()
+    gen_s = time.time() - t0
+    t0 = time.time()
+    scene = pa.Scene(ctx, desc)
+    build_s = time.time() - t0
+    jobs = scenes.batch_jobs()
+    e0, e1 = ctx.event(), ctx.event()
+
+    def timed(fn):
+        e0.record(ctx.stream)
+        out = fn()
+        e1.record(ctx.stream)
+        ctx.synchronize()
+        return out, e0.elapsed_ms(e1)
+
+    scene.render_jobs(jobs[:1], use_graphs=True)  # warm-up
+    (films, st), batch_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=True))
+    (_, st_eager), eager_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=False))
+    seq_ms, seq_rays = 0.0, 0
+    for j in jobs:
+        (_, sj), ms = timed(lambda j=j: scene.render(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"],
+                                                      j["look"], fov=j["fov"], seed=j["seed"], film=False))
+        seq_ms += ms
+        seq_rays += sj["rays"]
+    paths = st["paths"]
+    print(f"[bench] jobs: batch {batch_ms:.1f} ms ({st['rays'] / batch_ms / 1e3:.0f} Mrays/s), eager batch "
+          f"{eager_ms:.1f} ms, one after another {seq_ms:.1f} ms", file=sys.stderr, flush=True)
+    return {
+        "workload": f"{len(desc.meshes)} mesh instances ({sum(len(f) for _, f in desc.meshes)} triangles), "
+                    f"{sum(1 for o in desc.objects if o[0] == 4)} CSG objects; 8 jobs "
+                    + ", ".join(f"{j['width']}x{j['height']}x{j['spp']}" for j in jobs) + ", max depth 8",
+        "scene_gen_s": gen_s, "scene_build_s": build_s,
+        "batch_ms": batch_ms, "batch_mrays_s": st["rays"] / batch_ms / 1e3, "batch_samples_s": paths / batch_ms * 1e3,
+        "batch_no_graphs_ms": eager_ms, "sequential_ms": seq_ms, "sequential_mrays_s": seq_rays / seq_ms / 1e3,
+        "speedup_vs_sequential": seq_ms / batch_ms, "passes": st["passes"], "rays": st["rays"], "paths": paths,
+    }
+
+
 def render_profile(pa, ctx, spp=16, frames_per_pass=16):
     """A short run for kernel traces of the renderer on the radiance-cache workload: `spp` samples
     per pixel of plain path tracing (4 per pass) and `spp` frames of the cache render."""
@@ -209,6 +254,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
+    ap.add_argument("--no-jobs", action="store_true", help="skip the batched-jobs leg")
+    ap.add_argument("--jobs-only", action="store_true", help="run only the batched-jobs leg (prints its JSON)")
     ap.add_argument("--render-profile", action="store_true",
                     help="only a short plain + cache render of the cache workload, for kernel traces")
     args = ap.parse_args()
@@ -221,6 +268,9 @@ def main():
 
     if args.nrc_only:
         print(json.dumps(nrc_measure(pa, pa.Context(0))))
+        return
+    if args.jobs_only:
+        print(json.dumps(jobs_measure(pa, pa.Context(0))))
         return
     if args.render_profile:
         print(json.dumps(render_profile(pa, pa.Context(0))))
@@ -338,6 +388,9 @@ def main():
     nrc = None
     if not args.no_nrc:
         nrc = nrc_measure(pa, ctx)
+    jobs = None
+    if not args.no_jobs:
+        jobs = jobs_measure(pa, ctx)
 
     cpu = None
     if not args.no_cpu:
@@ -384,6 +437,7 @@ def main():
         "anyhit_mrays_s": anyhit,
         "copy_gbs": copy_gbs,
         "radiance_cache": nrc,
+        "render_jobs": jobs,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

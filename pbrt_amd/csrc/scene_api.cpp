@@ -4,12 +4,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "camera.hpp"
@@ -462,11 +465,37 @@ int pbrt_scene_commit(pbrt_scene* s) {
       Box64 bounds;
     };
     std::vector<MeshInfo> info;
-    for (size_t k = 0; k < s->mesh_xyz.size(); ++k) {
+    // the meshes' trees are independent: build them on the host's cores, then flatten in order
+    const size_t nm = s->mesh_xyz.size();
+    std::vector<BuiltBVH> built(nm);
+    {
+      const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      std::atomic<size_t> next{0};
+      std::vector<std::string> errors(hw);
+      auto work = [&](unsigned w) {
+        try {
+          for (size_t k = next++; k < nm; k = next++) {
+            const auto& x = s->mesh_xyz[k];
+            const auto& ix = s->mesh_idx[k];
+            built[k] = build_mesh_bvh(x.data(), (uint32_t)(x.size() / 3), ix.data(), (uint32_t)(ix.size() / 3),
+                                      s->mesh_params);
+          }
+        } catch (const std::exception& e) {
+          errors[w] = e.what();
+        }
+      };
+      std::vector<std::thread> pool;
+      for (unsigned w = 1; w < std::min<size_t>(hw, nm); ++w) pool.emplace_back(work, w);
+      work(0);
+      for (auto& t : pool) t.join();
+      for (const auto& e : errors)
+        if (!e.empty()) throw Error(PBRT_ERR_INTERNAL, "mesh BVH build: " + e);
+    }
+    for (size_t k = 0; k < nm; ++k) {
       const auto& x = s->mesh_xyz[k];
       const auto& ix = s->mesh_idx[k];
-      const uint32_t nv = (uint32_t)(x.size() / 3), nt = (uint32_t)(ix.size() / 3);
-      BuiltBVH b = build_mesh_bvh(x.data(), nv, ix.data(), nt, s->mesh_params);
+      const uint32_t nv = (uint32_t)(x.size() / 3);
+      BuiltBVH b = std::move(built[k]);
       s->blas_depth = std::max<uint32_t>(s->blas_depth, (uint32_t)b.stats.max_depth);
       std::vector<DevNode> n;
       std::vector<DevTri> t;

@@ -98,3 +98,83 @@ def nrc_room(seed=11, mesh_seed=2024):
     cam = dict(cam, width=1280, height=720, spp=1, frames=256, reference_spp=4096)
     return d, cam
 
+
+
+def torus(nu, nv, major=1.0, minor=0.35):
+    """A torus around the y axis of nu x nv quads (2 nu nv triangles), wound so that the reference's
+    mesh normal, -normalize((p1 - p0) x (p2 - p0)) (triangle_mesh.rs:119), points outward."""
+    u = np.arange(nu) * (2.0 * math.pi / nu)
+    v = np.arange(nv) * (2.0 * math.pi / nv)
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    r = major + minor * np.cos(vv)
+    verts = np.stack([r * np.cos(uu), minor * np.sin(vv), r * np.sin(uu)], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    faces = np.concatenate([np.stack([a, b, d], -1).reshape(-1, 3), np.stack([b, c, d], -1).reshape(-1, 3)])
+    return verts.astype(np.float32), faces.astype(np.uint32)
+
+
+def batch_scene(n_meshes=256, n_csg=32, seed=5, tri_range=(1000, 200000)):
+    """The batched-jobs workload: `n_meshes` mesh instances of seeded size log-uniform in
+    `tri_range` triangles (displaced icospheres and tessellated tori), each under a random rigid
+    transform, plus `n_csg` CSG objects (union, intersection, subtraction of unit spheres and cubes),
+    over a floor, lit by a point light and an emissive panel."""
+    rng = np.random.default_rng(seed)
+    d = SceneDesc()
+    floor = d.add_material("lambertian", (0.6, 0.6, 0.6))
+    d.add_object("rectangle", floor, (120.0, 120.0), transform=Transform.translation(0.0, -2.0, 0.0))
+    light = d.add_material("diffuse_light", (8.0, 8.0, 8.0))
+    panel = d.add_object("rectangle", light, (20.0, 20.0),
+                         transform=Transform.translation(0.0, 30.0, 0.0) * Transform.rotation("x", math.pi))
+    d.add_area_light(panel, (8.0, 8.0, 8.0))
+    d.add_point_light((10.0, 25.0, 20.0), (3000.0, 3000.0, 3000.0))
+    side = int(math.ceil(math.sqrt(n_meshes + n_csg)))
+    spots = [(x, z) for x in range(side) for z in range(side)]
+    order = rng.permutation(len(spots))
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    for k in range(n_meshes):
+        target = math.exp(rng.uniform(lo, hi))
+        if rng.uniform() < 0.5:
+            s = int(min(6, max(3, round(math.log(target / 20.0, 4)))))
+            v, f = icosphere(s, radius=1.0, noise="fbm", amp=0.08, octaves=3, seed=int(rng.integers(1 << 30)))
+        else:
+            nu = max(8, int(round(math.sqrt(target))))
+            nv = max(4, int(round(target / (2 * nu))))
+            v, f = torus(nu, nv)
+        mat = d.add_material("ggx" if rng.uniform() < 0.3 else "lambertian", rng.uniform(0.2, 0.8, 3),
+                             roughness=float(rng.uniform(0.05, 0.5)))
+        gx, gz = spots[order[k]]
+        xf = (Transform.translation((gx - side / 2) * 3.0, rng.uniform(-1.0, 1.0), (gz - side / 2) * 3.0)
+              * Transform.rotation("y", float(rng.uniform(0, 2 * math.pi)))
+              * Transform.rotation("x", float(rng.uniform(-0.5, 0.5))))
+        d.add_object("mesh", mat, mesh=d.add_mesh(v, f), transform=xf)
+    ops = ("union", "intersection", "subtraction")
+    for k in range(n_csg):
+        op = ops[k % 3]
+        els = []
+        for _ in range(2 + int(rng.integers(0, 3))):
+            c = rng.uniform(-0.5, 0.5, 3) * (0.5 if op == "intersection" else 1.0)
+            xf = Transform.translation(*c) * Transform.rotation("y", float(rng.uniform(0, 2 * math.pi)))
+            els.append(("sphere", (1.0,), xf) if rng.uniform() < 0.5 else ("aabox", (1.5, 1.5, 1.5), xf))
+        gx, gz = spots[order[n_meshes + k]]
+        d.add_csg(op, els, d.add_material("lambertian", rng.uniform(0.2, 0.8, 3)),
+                  transform=Transform.translation((gx - side / 2) * 3.0, 0.0, (gz - side / 2) * 3.0))
+    return d
+
+
+def batch_jobs(seed=6, n=8):
+    """Eight render jobs of the batched-jobs workload: resolutions drawn from 256^2, 512^2, 768x432
+    and 1024^2, 16 or 64 samples per pixel, max depth 8, cameras around the scene."""
+    rng = np.random.default_rng(seed)
+    sizes = [(256, 256), (512, 512), (768, 432), (1024, 1024)]
+    jobs = []
+    for k in range(n):
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        a = rng.uniform(0, 2 * math.pi)
+        eye = (40.0 * math.cos(a), float(rng.uniform(8.0, 20.0)), 40.0 * math.sin(a))
+        jobs.append(dict(width=w, height=h, spp=int(rng.choice([16, 64])), max_depth=8, eye=eye,
+                         look=(0.0, 0.0, 0.0), fov=math.radians(50.0), seed=100 + k))
+    return jobs

@@ -215,3 +215,15 @@ def test_job_batch_equals_single_renders(gpu_ctx, use_graphs):
                                 seed=j["seed"])
         assert f.tobytes() == alone.tobytes()
     assert stats["paths"] == sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+
+
+def test_batch_scene_random_rays_bit_exact(gpu_ctx):
+    """A reduced batched-jobs scene (instanced displaced icospheres and tori under rigid transforms,
+    CSG objects, an emissive panel): random rays bit-exact against the FP32 oracle."""
+    desc = scenes.batch_scene(n_meshes=24, n_csg=8, seed=7, tri_range=(500, 20000))
+    rays = np.ascontiguousarray(pbrt_amd.random_rays((0.0, 0.0, 0.0), 12.0, 60000, seed=4)).view(np.float32)
+    rays = rays.reshape(-1, 8)
+    scene = pbrt_amd.Scene(gpu_ctx, desc)
+    orc = po.Scene(desc, precision=32, **pbrt_amd.DEVICE_BVH)
+    compare_hits(scene.trace_host(rays), orc.trace(rays))
+    np.testing.assert_array_equal(scene.trace_host(rays, any_hit=True).astype(bool), orc.trace(rays, any_hit=True)["obj"] >= 0)
