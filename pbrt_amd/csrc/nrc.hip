@@ -24,11 +24,30 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t bf16_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / NaN
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// FP32 -> BF16 bit pattern, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ uint16_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+
+// sin(pi x) and cos(pi x) for x in [0, 2], x exact: x = q / 2 + r with q = round(2x) and |r| <= 1/4
+// (both exact), then the Taylor series of sin(pi r) to r^9 and of cos(pi r) to r^10 (truncation
+// below 3e-9 at |r| = 1/4), and the quadrant q mod 4.
+__device__ __forceinline__ void sincospi_red(float x, float* s_out, float* c_out) {
+  const float q = rintf(x * 2.0f);
+  const float r = x - q * 0.5f;
+  const float r2 = r * r;
+  // coefficients (-1)^k pi^(2k+1) / (2k+1)! and (-1)^k pi^(2k) / (2k)!
+  float sp = __builtin_fmaf(r2, 8.21458866111282287e-02f, -5.99264529320792077e-01f);
+  sp = __builtin_fmaf(r2, sp, 2.55016403987734544e+00f);
+  sp = __builtin_fmaf(r2, sp, -5.16771278004997003e+00f);
+  sp = __builtin_fmaf(r2, sp, 3.14159265358979324e+00f);
+  const float sv = r * sp;
+  float cp = __builtin_fmaf(r2, -2.58068913900140508e-02f, 2.35330630358893123e-01f);
+  cp = __builtin_fmaf(r2, cp, -1.33526276885458950e+00f);
+  cp = __builtin_fmaf(r2, cp, 4.05871212641676822e+00f);
+  cp = __builtin_fmaf(r2, cp, -4.93480220054467931e+00f);
+  const float cv = __builtin_fmaf(r2, cp, 1.0f);
+  const int qi = (int)q & 3;
+  *s_out = qi == 0 ? sv : qi == 1 ? cv : qi == 2 ? -sv : -cv;
+  *c_out = qi == 0 ? cv : qi == 1 ? -sv : qi == 2 ? -cv : sv;
 }
 __device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -168,9 +187,10 @@ __device__ __forceinline__ bf16x8 features8(const float* q, float3 lo, float3 ie
     float f = ldexpf(p, k0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const float red = f - 2.0f * floorf(f * 0.5f);
-      x.u[2 * i] = bf16_bits(sinpif(red));
-      x.u[2 * i + 1] = bf16_bits(cospif(red));
+      float sv, cv;
+      sincospi_red(f - 2.0f * floorf(f * 0.5f), &sv, &cv);
+      x.u[2 * i] = bf16_bits(sv);
+      x.u[2 * i + 1] = bf16_bits(cv);
       f = f * 2.0f;
     }
   } else {
@@ -514,9 +534,10 @@ __global__ void k_nrc_encode(const float* __restrict__ raw, uint32_t n, const ui
       // 2^k p is exact in FP32; reduce it modulo 2 (exact) and evaluate sin / cos of pi times it
       float f = p[a];
       for (int k = 0; k < kNrcFreq; ++k) {
-        const float red = f - 2.0f * floorf(f * 0.5f);
-        o[j++] = bf16_bits(sinpif(red));
-        o[j++] = bf16_bits(cospif(red));
+        float sv, cv;
+        sincospi_red(f - 2.0f * floorf(f * 0.5f), &sv, &cv);
+        o[j++] = bf16_bits(sv);
+        o[j++] = bf16_bits(cv);
         f = f * 2.0f;
       }
     }
