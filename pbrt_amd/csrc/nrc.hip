@@ -7,6 +7,7 @@
 // Data layout (padded so every matrix product is a whole number of 16 x 16 x 32 MFMA tiles):
 //   inputs   N x 96 BF16 (81 features, zero padded), row-major
 //   forward  Wf0 [64][96], Wf1..3 [64][64], Wf4 [16][64] (rows 3.. zero), [out][in] row-major
+//   inference the same, with the input columns of Wf1..4 permuted by nrc_chain_column
 //   backward Wt1..3 [64][64], Wt4 [64][32] ([in][out], out padded with zeros)
 //   biases   b0..b3 [64], b4 [16] (FP32)
 // MFMA operand layout (16x16x32): lane l feeds A[l % 16][8 (l / 16) + j] and
@@ -27,12 +28,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // FP32 -> BF16 bit pattern, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ uint16_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 
-// sin(pi x) and cos(pi x) for x in [0, 2], x exact: x = q / 2 + r with q = round(2x) and |r| <= 1/4
-// (both exact), then the Taylor series of sin(pi r) to r^9 and of cos(pi r) to r^10 (truncation
-// below 3e-9 at |r| = 1/4), and the quadrant q mod 4.
-__device__ __forceinline__ void sincospi_red(float x, float* s_out, float* c_out) {
-  const float q = rintf(x * 2.0f);
-  const float r = x - q * 0.5f;
+// sin(pi f) and cos(pi f) for f = 2^k p (exact, |2 f| < 2^24): f = q / 2 + r with q = rint(2 f) and
+// |r| <= 1/4 (both exact), the Taylor series of sin(pi r) to r^9 and of cos(pi r) to r^10
+// (truncation below 3e-9 at |r| = 1/4), then the quadrant q mod 4 as a swap and two sign flips.
+__device__ __forceinline__ void sincospi_red(float f, float* s_out, float* c_out) {
+  const float q = rintf(f * 2.0f);
+  const float r = f - q * 0.5f;
   const float r2 = r * r;
   // coefficients (-1)^k pi^(2k+1) / (2k+1)! and (-1)^k pi^(2k) / (2k)!
   float sp = __builtin_fmaf(r2, 8.21458866111282287e-02f, -5.99264529320792077e-01f);
@@ -45,9 +46,12 @@ __device__ __forceinline__ void sincospi_red(float x, float* s_out, float* c_out
   cp = __builtin_fmaf(r2, cp, 4.05871212641676822e+00f);
   cp = __builtin_fmaf(r2, cp, -4.93480220054467931e+00f);
   const float cv = __builtin_fmaf(r2, cp, 1.0f);
-  const int qi = (int)q & 3;
-  *s_out = qi == 0 ? sv : qi == 1 ? cv : qi == 2 ? -sv : -cv;
-  *c_out = qi == 0 ? cv : qi == 1 ? -sv : qi == 2 ? -cv : sv;
+  // quadrant 0: (s, c); 1: (c, -s); 2: (-s, -c); 3: (-c, s)
+  const uint32_t qi = (uint32_t)(int)q;
+  const bool swap = (qi & 1u) != 0u;
+  const float a = swap ? cv : sv, b = swap ? sv : cv;
+  *s_out = __uint_as_float(__float_as_uint(a) ^ ((qi & 2u) << 30));
+  *c_out = __uint_as_float(__float_as_uint(b) ^ (((qi + 1u) & 2u) << 30));
 }
 __device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -118,149 +122,244 @@ __device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, con
   }
 }
 
-// Inference: rows of `x` (N x 96 BF16) -> out (N x 3 FP32). One wave per 16-row tile, grid-stride.
-__global__ __launch_bounds__(256) void k_nrc_infer(NrcForward P, const uint16_t* __restrict__ x, float* __restrict__ out,
-                                                   uint32_t n, const uint32_t* __restrict__ n_dev) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (n_dev) n = min(n, *n_dev);  // a batch counted on the device, bounded by the buffers' capacity
-  const uint32_t tiles = (n + 15) / 16;
-  for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
-    const uint32_t row0 = t * 16;
-    // layer 0 reads its A operand (8 consecutive BF16 of one input row) straight from HBM
-    const int r = lane & 15, g = lane >> 4;
-    const uint32_t row = row0 + r;
-    uint16_t* h0 = act[wave][0];
-    uint16_t* h1 = act[wave][1];
-    {
+// Inference with the activations in registers. Each product takes the weights as its A operand and
+// 16 batch rows as its B operand, so lane (r, g) receives features 16 nb + 4 g + i of row r; with
+// the hidden layers' weight columns in the chained order (nrc_chain_column) those four-feature
+// groups, rounded to BF16 and rectified, are the lane's B operand of the next layer as they stand.
+// The accumulators start at the biases. A workgroup stages the forward weights in LDS once (rows
+// padded by 16 bytes: the 16 rows a product reads fall in distinct banks); a wave runs RT 16-row
+// tiles through the five layers, every weight operand read from LDS once per RT products.
+constexpr int kS0 = INP + 8, kSH = H + 8;  // LDS row strides (BF16)
+struct ChainLds {
+  uint16_t w0[64 * kS0];
+  uint16_t w1[64 * kSH], w2[64 * kSH], w3[64 * kSH];
+  uint16_t w4[16 * kSH];
+};
+
+__device__ __forceinline__ void chain_stage(const NrcForward& P, ChainLds& L) {
+  for (int c = threadIdx.x; c < 64 * INP / 8; c += blockDim.x) {
+    const int row = c / (INP / 8), k8 = c % (INP / 8);
+    *reinterpret_cast<uint4*>(L.w0 + row * kS0 + k8 * 8) = *reinterpret_cast<const uint4*>(P.w0 + row * INP + k8 * 8);
+  }
+  for (int c = threadIdx.x; c < 64 * H / 8; c += blockDim.x) {
+    const int row = c / (H / 8), k8 = c % (H / 8);
+    *reinterpret_cast<uint4*>(L.w1 + row * kSH + k8 * 8) = *reinterpret_cast<const uint4*>(P.w1 + row * H + k8 * 8);
+    *reinterpret_cast<uint4*>(L.w2 + row * kSH + k8 * 8) = *reinterpret_cast<const uint4*>(P.w2 + row * H + k8 * 8);
+    *reinterpret_cast<uint4*>(L.w3 + row * kSH + k8 * 8) = *reinterpret_cast<const uint4*>(P.w3 + row * H + k8 * 8);
+    if (row < 16)
+      *reinterpret_cast<uint4*>(L.w4 + row * kSH + k8 * 8) = *reinterpret_cast<const uint4*>(P.w4 + row * H + k8 * 8);
+  }
+  __syncthreads();
+}
+
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
+// max(x, 0) of two values as their BF16 patterns: rounding keeps the sign, and a BF16 pattern orders
+// as a 16-bit signed integer the way the value does against zero (one packed integer max)
+__device__ __forceinline__ uint32_t relu_bf16x2(float lo, float hi) {
+  const uint32_t w = (uint32_t)bf16_bits(lo) | ((uint32_t)bf16_bits(hi) << 16);
+  const i16x2 v = __builtin_elementwise_max(__builtin_bit_cast(i16x2, w), i16x2{0, 0});
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+template <int RT>
+__device__ __forceinline__ void chain_act(const f32x4 (&acc)[RT][4], bf16x8 (&h)[RT][2]) {
 #pragma unroll
-      for (int nb = 0; nb < H / 16; ++nb) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < RT; ++t)
 #pragma unroll
-        for (int kk = 0; kk < INP / 32; ++kk) {
-          bf16x8 a;
-          if (row < n) {
-            a = load8(x + (size_t)row * INP + kk * 32 + g * 8);
-          } else {
-            Pack8 z;
-            z.q = make_uint4(0, 0, 0, 0);
-            a = z.v;
-          }
-          const bf16x8 bb = load8(P.w0 + (nb * 16 + r) * INP + kk * 32 + g * 8);
-          acc = mfma(a, bb, acc);
-        }
-        const int col = nb * 16 + r;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float v = acc[i] + P.b0[col];
-          h0[(g * 4 + i) * H + col] = bf16_bits(v > 0.f ? v : 0.f);
-        }
-      }
+    for (int kk = 0; kk < 2; ++kk) {
+      uint4 w;
+      w.x = relu_bf16x2(acc[t][2 * kk][0], acc[t][2 * kk][1]);
+      w.y = relu_bf16x2(acc[t][2 * kk][2], acc[t][2 * kk][3]);
+      w.z = relu_bf16x2(acc[t][2 * kk + 1][0], acc[t][2 * kk + 1][1]);
+      w.w = relu_bf16x2(acc[t][2 * kk + 1][2], acc[t][2 * kk + 1][3]);
+      Pack8 p;
+      p.q = w;
+      h[t][kk] = p.v;
     }
-    wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.w1, P.b1, h1, H, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(h1, H, P.w2, P.b2, h0, H, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.w3, P.b3, h1, H, lane, nullptr);
-    wave_sync();
-    f32x4 y;
-    layer_fwd<H, OUTP, false>(h1, H, P.w4, P.b4, nullptr, 0, lane, &y);
-    if (r < 3) {
+}
+
+// accumulators of the four 16-feature blocks, started at the biases of the lane's features
+template <int RT>
+__device__ __forceinline__ void chain_bias(const float* __restrict__ b, int g, f32x4 (&acc)[RT][4]) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t orow = row0 + g * 4 + i;
-        if (orow < n) out[(size_t)orow * 3 + r] = y[i];
-      }
-    }
-    wave_sync();
+  for (int nb = 0; nb < 4; ++nb) {
+    const float4 v = *reinterpret_cast<const float4*>(b + nb * 16 + 4 * g);
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t][nb] = f32x4{v.x, v.y, v.z, v.w};
   }
 }
 
-// Eight consecutive input features j0 .. j0 + 7 (j0 a multiple of 8) of the sample with raw
-// inputs q (position, outgoing direction, normal, albedo): the same values k_nrc_encode stores.
-__device__ __forceinline__ bf16x8 features8(const float* q, float3 lo, float3 ie, int j0) {
-  Pack8 x;
-  if (j0 < 72) {  // 24 features per axis: sin, cos of 2^k pi p for k = 0..11; 8 never straddle an axis
-    const int a = j0 / 24, k0 = (j0 % 24) / 2;
-    const float p = a == 0 ? (q[0] - lo.x) * ie.x : a == 1 ? (q[1] - lo.y) * ie.y : (q[2] - lo.z) * ie.z;
-    float f = ldexpf(p, k0);
+template <int RT>
+__device__ __forceinline__ void chain_hidden(const bf16x8 (&h)[RT][2], const uint16_t* W, const float* __restrict__ b,
+                                             int r, int g, f32x4 (&acc)[RT][4]) {
+  chain_bias<RT>(b, g, acc);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float sv, cv;
-      sincospi_red(f - 2.0f * floorf(f * 0.5f), &sv, &cv);
-      x.u[2 * i] = bf16_bits(sv);
-      x.u[2 * i + 1] = bf16_bits(cv);
-      f = f * 2.0f;
+  for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const bf16x8 a = load8(W + (nb * 16 + r) * kSH + kk * 32 + g * 8);
+#pragma unroll
+      for (int t = 0; t < RT; ++t) acc[t][nb] = mfma(a, h[t][kk], acc[t][nb]);
     }
-  } else {
+}
+
+// The network on RT tiles whose layer-0 B operands (features 32 kk + 8 g .. + 7 of row r) are x0;
+// y[t][i] is output i of row r of tile t in the lanes with g = 0 (i < 3).
+template <int RT>
+__device__ __forceinline__ void chain_forward(const NrcForward& P, const ChainLds& L, const bf16x8 (&x0)[RT][INP / 32],
+                                              int r, int g, f32x4 (&y)[RT]) {
+  // an opaque zero offset keeps the LDS weight reads inside the caller's loop (hoisted, they would
+  // hold 150 registers for the whole kernel)
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  f32x4 acc[RT][4];
+  bf16x8 h[RT][2];
+  chain_bias<RT>(P.b0, g, acc);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int j = j0 + i;
-      x.u[i] = j < IN ? bf16_bits(q[3 + j - 72]) : (uint16_t)0;
+  for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+    for (int kk = 0; kk < INP / 32; ++kk) {
+      const bf16x8 a = load8(L.w0 + z + (nb * 16 + r) * kS0 + kk * 32 + g * 8);
+#pragma unroll
+      for (int t = 0; t < RT; ++t) acc[t][nb] = mfma(a, x0[t][kk], acc[t][nb]);
+    }
+  chain_act<RT>(acc, h);
+  chain_hidden<RT>(h, L.w1 + z, P.b1, r, g, acc);
+  chain_act<RT>(acc, h);
+  chain_hidden<RT>(h, L.w2 + z, P.b2, r, g, acc);
+  chain_act<RT>(acc, h);
+  chain_hidden<RT>(h, L.w3 + z, P.b3, r, g, acc);
+  chain_act<RT>(acc, h);
+  const float4 b4 = *reinterpret_cast<const float4*>(P.b4 + 4 * g);
+#pragma unroll
+  for (int t = 0; t < RT; ++t) y[t] = f32x4{b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const bf16x8 a = load8(L.w4 + z + r * kSH + kk * 32 + g * 8);
+#pragma unroll
+    for (int t = 0; t < RT; ++t) y[t] = mfma(a, h[t][kk], y[t]);
+  }
+}
+
+constexpr int kChainTiles = 4;
+
+// Outputs of the RT tiles from the lanes with g = 0: 16 lanes x 12 contiguous bytes per tile.
+template <int RT>
+__device__ __forceinline__ void chain_store(const f32x4 (&y)[RT], float* __restrict__ out, uint32_t row0, uint32_t n,
+                                            int r, int g) {
+  if (g != 0) return;
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const uint32_t row = row0 + t * 16 + r;
+    if (row < n) {
+      out[(size_t)row * 3 + 0] = y[t][0];
+      out[(size_t)row * 3 + 1] = y[t][1];
+      out[(size_t)row * 3 + 2] = y[t][2];
+    }
+  }
+}
+
+// Inference: rows of `x` (N x 96 BF16) -> out (N x 3 FP32), kChainTiles 16-row tiles per wave,
+// grid-stride.
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcForward P, const uint16_t* __restrict__ x, float* __restrict__ out,
+                                                   uint32_t n, const uint32_t* __restrict__ n_dev) {
+  constexpr int RT = kChainTiles;
+  __shared__ __attribute__((aligned(16))) ChainLds L;
+  chain_stage(P, L);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  if (n_dev) n = min(n, *n_dev);  // a batch counted on the device, bounded by the buffers' capacity
+  const uint32_t groups = (n + RT * 16 - 1) / (RT * 16);
+  for (uint32_t grp = blockIdx.x * 4 + wave; grp < groups; grp += gridDim.x * 4) {
+    const uint32_t row0 = grp * RT * 16;
+    bf16x8 x0[RT][INP / 32];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+      const uint32_t row = row0 + t * 16 + r;
+#pragma unroll
+      for (int kk = 0; kk < INP / 32; ++kk) {
+        Pack8 z;
+        z.q = make_uint4(0, 0, 0, 0);
+        x0[t][kk] = row < n ? load8(x + (size_t)row * INP + kk * 32 + g * 8) : z.v;
+      }
+    }
+    f32x4 y[RT];
+    chain_forward<RT>(P, L, x0, r, g, y);
+    chain_store<RT>(y, out, row0, n, r, g);
+  }
+}
+
+// sin and cos of 2^k pi p for k = k0 .. k0 + 3, as features (sin, cos) pairs
+__device__ __forceinline__ void sincos4(float p, int k0, Pack8& x) {
+  float f = ldexpf(p, k0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float sv, cv;
+    sincospi_red(f, &sv, &cv);
+    x.u[2 * i] = bf16_bits(sv);
+    x.u[2 * i + 1] = bf16_bits(cv);
+    f = f * 2.0f;
+  }
+}
+
+// Input features KK * 32 + 8 g .. + 7 of the sample with raw inputs q (position, outgoing
+// direction, normal, albedo): the same values k_nrc_encode stores. Branch-free across the lane
+// groups g = 0..3 of a wave.
+template <int KK>
+__device__ __forceinline__ bf16x8 features8(const float* q, float3 lo, float3 ie, int g) {
+  Pack8 x;
+  if (KK < 2) {  // 24 features per axis (sin, cos of 2^k pi p, k = 0..11): 8 never straddle an axis
+    const int j0 = KK * 32 + g * 8, a = j0 / 24;
+    const float p = a == 0 ? (q[0] - lo.x) * ie.x : a == 1 ? (q[1] - lo.y) * ie.y : (q[2] - lo.z) * ie.z;
+    sincos4(p, (j0 % 24) / 2, x);
+  } else {  // features 64 + 8 g: z at k = 8..11 for g = 0; direction, normal, albedo, padding after
+    sincos4((q[2] - lo.z) * ie.z, 8, x);
+    if (g > 0) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const float v = g == 1 ? q[3 + i] : (g == 2 && i == 0) ? q[11] : 0.0f;
+        x.u[i] = bf16_bits(v);
+      }
     }
   }
   return x.v;
 }
 
-// Inference straight from raw inputs (n x 12 floats): the features of each lane's A operand are
-// computed in registers (no encoded copy in HBM), then the same layers as k_nrc_infer.
+// Inference straight from raw inputs (n x 12 floats): each lane's layer-0 B operands are computed
+// in registers (no encoded copy in HBM); otherwise k_nrc_infer, with the same arithmetic per row.
 __global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __restrict__ raw, float3 lo, float3 ie,
                                                    float* __restrict__ out, uint32_t n,
                                                    const uint32_t* __restrict__ n_dev) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * H];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int RT = kChainTiles;
+  __shared__ __attribute__((aligned(16))) ChainLds L;
+  chain_stage(P, L);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   if (n_dev) n = min(n, *n_dev);
-  const uint32_t tiles = (n + 15) / 16;
-  for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
-    const uint32_t row0 = t * 16;
-    const int r = lane & 15, g = lane >> 4;
-    const uint32_t row = row0 + r;
-    float q[12];
-    if (row < n) {
-      const float4* src = reinterpret_cast<const float4*>(raw + (size_t)row * 12);
-      const float4 a = src[0], b = src[1], c = src[2];
-      q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
-      q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
-      q[8] = c.x; q[9] = c.y; q[10] = c.z; q[11] = c.w;
-    } else {
+  const uint32_t groups = (n + RT * 16 - 1) / (RT * 16);
+  for (uint32_t grp = blockIdx.x * 4 + wave; grp < groups; grp += gridDim.x * 4) {
+    const uint32_t row0 = grp * RT * 16;
+    bf16x8 x0[RT][INP / 32];
 #pragma unroll
-      for (int i = 0; i < 12; ++i) q[i] = lo.x;  // any finite value: the row is never stored
-    }
-    bf16x8 A[INP / 32];
+    for (int t = 0; t < RT; ++t) {
+      const uint32_t row = row0 + t * 16 + r;
+      float q[12];
+      if (row < n) {
+        const float4* src = reinterpret_cast<const float4*>(raw + (size_t)row * 12);
+        const float4 a = src[0], b = src[1], c = src[2];
+        q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
+        q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
+        q[8] = c.x; q[9] = c.y; q[10] = c.z; q[11] = c.w;
+      } else {
 #pragma unroll
-    for (int kk = 0; kk < INP / 32; ++kk) A[kk] = features8(q, lo, ie, kk * 32 + g * 8);
-    uint16_t* h0 = act[wave][0];
-    uint16_t* h1 = act[wave][1];
-#pragma unroll
-    for (int nb = 0; nb < H / 16; ++nb) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < INP / 32; ++kk) acc = mfma(A[kk], load8(P.w0 + (nb * 16 + r) * INP + kk * 32 + g * 8), acc);
-      const int col = nb * 16 + r;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float v = acc[i] + P.b0[col];
-        h0[(g * 4 + i) * H + col] = bf16_bits(v > 0.f ? v : 0.f);
+        for (int i = 0; i < 12; ++i) q[i] = lo.x;  // any finite value: the row is never stored
       }
+      x0[t][0] = features8<0>(q, lo, ie, g);
+      x0[t][1] = features8<1>(q, lo, ie, g);
+      x0[t][2] = features8<2>(q, lo, ie, g);
     }
-    wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.w1, P.b1, h1, H, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(h1, H, P.w2, P.b2, h0, H, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(h0, H, P.w3, P.b3, h1, H, lane, nullptr);
-    wave_sync();
-    f32x4 y;
-    layer_fwd<H, OUTP, false>(h1, H, P.w4, P.b4, nullptr, 0, lane, &y);
-    if (r < 3) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t orow = row0 + g * 4 + i;
-        if (orow < n) out[(size_t)orow * 3 + r] = y[i];
-      }
-    }
-    wave_sync();
+    f32x4 y[RT];
+    chain_forward<RT>(P, L, x0, r, g, y);
+    chain_store<RT>(y, out, row0, n, r, g);
   }
 }
 
@@ -485,12 +584,16 @@ __global__ void k_nrc_adam(NrcDevice P, const float* __restrict__ grads, float l
   }
 }
 
-__device__ __forceinline__ void pack_forward(const NrcForward& F, int layer, int o, int k, bool is_bias, float w) {
+// Parameter (layer, o, k) into the forward set F; `chained`: hidden input columns in the order
+// nrc_chain_column gives (the inference sets).
+__device__ __forceinline__ void pack_forward(const NrcForward& F, int layer, int o, int k, bool is_bias, float w,
+                                             bool chained) {
   if (is_bias) {
     float* b = layer == 0 ? F.b0 : layer == 1 ? F.b1 : layer == 2 ? F.b2 : layer == 3 ? F.b3 : F.b4;
     b[o] = w;
     return;
   }
+  if (chained && layer > 0) k = nrc_chain_column(k);
   const uint16_t q = bf16_bits(w);
   switch (layer) {
     case 0: F.w0[o * INP + k] = q; break;
@@ -506,8 +609,9 @@ __global__ void k_nrc_pack(NrcDevice P) {
     const float w = P.master[i];
     int layer, o, k;
     const bool is_bias = nrc_locate(i, &layer, &o, &k);
-    pack_forward(P.train(), layer, o, k, is_bias, w);
-    if (P.ema_decay > 0.f) pack_forward(P.avg, layer, o, k, is_bias, P.ema[i]);
+    pack_forward(P.train(), layer, o, k, is_bias, w, false);
+    pack_forward(P.chain, layer, o, k, is_bias, w, true);  // w0 and biases: the same words again
+    if (P.ema_decay > 0.f) pack_forward(P.avg, layer, o, k, is_bias, P.ema[i], true);
     if (is_bias) continue;
     const uint16_t q = bf16_bits(w);
     switch (layer) {
@@ -531,11 +635,10 @@ __global__ void k_nrc_encode(const float* __restrict__ raw, uint32_t n, const ui
     const float p[3] = {(q[0] - lo.x) * inv_ext.x, (q[1] - lo.y) * inv_ext.y, (q[2] - lo.z) * inv_ext.z};
     int j = 0;
     for (int a = 0; a < 3; ++a) {
-      // 2^k p is exact in FP32; reduce it modulo 2 (exact) and evaluate sin / cos of pi times it
-      float f = p[a];
+      float f = p[a];  // 2^k p is exact in FP32
       for (int k = 0; k < kNrcFreq; ++k) {
         float sv, cv;
-        sincospi_red(f - 2.0f * floorf(f * 0.5f), &sv, &cv);
+        sincospi_red(f, &sv, &cv);
         o[j++] = bf16_bits(sv);
         o[j++] = bf16_bits(cv);
         f = f * 2.0f;
@@ -561,9 +664,9 @@ __global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __r
 hipError_t nrc_launch_infer(const NrcForward& P, const uint16_t* x, float* out, uint32_t n, const uint32_t* n_dev,
                             uint32_t cu, void* s) {
   if (!n) return hipSuccess;
-  const uint32_t tiles = (n + 15) / 16;
-  uint32_t blocks = (tiles + 3) / 4;
-  const uint32_t cap = cu * 4;
+  const uint32_t groups = (n + kChainTiles * 16 - 1) / (kChainTiles * 16);
+  uint32_t blocks = (groups + 3) / 4;
+  const uint32_t cap = cu * 3;  // 39 KB of LDS each: three workgroups per CU
   if (blocks > cap) blocks = cap;
   hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, out, n, n_dev);
   return hipGetLastError();
@@ -574,9 +677,9 @@ hipError_t nrc_launch_query(const NrcForward& P, const float* raw, const float l
   if (!n) return hipSuccess;
   const float3 l = make_float3(lo[0], lo[1], lo[2]);
   const float3 ie = make_float3(1.0f / (hi[0] - lo[0]), 1.0f / (hi[1] - lo[1]), 1.0f / (hi[2] - lo[2]));
-  const uint32_t tiles = (n + 15) / 16;
-  uint32_t blocks = (tiles + 3) / 4;
-  const uint32_t cap = cu * 4;
+  const uint32_t groups = (n + kChainTiles * 16 - 1) / (kChainTiles * 16);
+  uint32_t blocks = (groups + 3) / 4;
+  const uint32_t cap = cu * 3;
   if (blocks > cap) blocks = cap;
   hipLaunchKernelGGL(k_nrc_query, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, raw, l, ie, out, n, n_dev);
   return hipGetLastError();

@@ -66,6 +66,16 @@ __host__ __device__ inline int nrc_padded_index(int i) {
   }
 }
 
+// Column of hidden input feature k (0..63) in the chained layout of the inference weights of
+// layers 1..4. A 16x16x32 MFMA with the weights as A operand and a layer's activations as B
+// operand leaves features 16 nb + 4 g + i (i = 0..3, nb = 0..3) of its batch row in lane group g;
+// the next layer's B operand of lane group g holds columns 32 kk + 8 g + j, so feature
+// 16 (2 kk + j / 4) + 4 g + j % 4 goes to column 32 kk + 8 g + j and the activations never leave
+// the lane.
+__host__ __device__ constexpr int nrc_chain_column(int k) {
+  return (k >> 5) * 32 + ((k >> 2) & 3) * 8 + ((k >> 4) & 1) * 4 + (k & 3);
+}
+
 // The weights a forward pass reads: BF16 [out][in] matrices ([64][96], [64][64] x 3, [16][64]) and
 // FP32 biases ([64] x 4, [16]).
 struct NrcForward {
@@ -80,10 +90,12 @@ struct NrcDevice {
   float *b0, *b1, *b2, *b3, *b4;          // [64] x 4, [16]
   float *master, *m, *v;                  // FP32 parameters and Adam moments, flat order
   float* ema;                             // exponential moving average of the parameters, flat order
-  NrcForward avg;                         // BF16 forward weights of the average (inference)
+  NrcForward chain;                       // inference weights: layers 1..4 in the chained column order
+                                          // (w0 and the biases are those of the training set)
+  NrcForward avg;                         // inference weights of the average, chained likewise
   float ema_decay;                        // 0: inference reads the parameters themselves
   __host__ __device__ NrcForward train() const { return {wf0, wf1, wf2, wf3, wf4, b0, b1, b2, b3, b4}; }
-  __host__ __device__ NrcForward infer() const { return ema_decay > 0.f ? avg : train(); }
+  __host__ __device__ NrcForward infer() const { return ema_decay > 0.f ? avg : chain; }
 };
 
 // Bytes of one NrcDevice allocation and the carving of it.

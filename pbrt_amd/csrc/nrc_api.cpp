@@ -26,6 +26,7 @@ size_t nrc_device_bytes() {
   b += 3 * up(64 * 64 * 2) + up(64 * 32 * 2);                          // backward weights
   b += 4 * up(64 * 4) + up(16 * 4);                                     // biases
   b += 4 * up((size_t)kNrcParams * 4);                                  // master, m, v, ema
+  b += 3 * up(64 * 64 * 2) + up(16 * 64 * 2);                           // chained inference weights
   b += up(64 * kNrcInPad * 2) + 3 * up(64 * 64 * 2) + up(16 * 64 * 2);  // forward weights of the average
   b += 4 * up(64 * 4) + up(16 * 4);                                     // and its biases
   return b;
@@ -57,6 +58,11 @@ NrcDevice nrc_carve(void* base) {
   d.m = (float*)take((size_t)kNrcParams * 4);
   d.v = (float*)take((size_t)kNrcParams * 4);
   d.ema = (float*)take((size_t)kNrcParams * 4);
+  d.chain = d.train();
+  d.chain.w1 = (uint16_t*)take(64 * 64 * 2);
+  d.chain.w2 = (uint16_t*)take(64 * 64 * 2);
+  d.chain.w3 = (uint16_t*)take(64 * 64 * 2);
+  d.chain.w4 = (uint16_t*)take(16 * 64 * 2);
   d.avg.w0 = (uint16_t*)take(64 * kNrcInPad * 2);
   d.avg.w1 = (uint16_t*)take(64 * 64 * 2);
   d.avg.w2 = (uint16_t*)take(64 * 64 * 2);

@@ -891,6 +891,9 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       if (p.train_stride) p.train_offset = (uint32_t)(hash4(c->seed, passes, 0x747261696eull, 0) % p.train_stride);
       // front: the set is free once the pass before last has finished with it
       if (passes >= 2) PBRT_HIP(hipStreamWaitEvent(st, ev_back[S], 0));
+      // front (render stream): the camera rays and every bounce up to the query vertex, where all
+      // but the training paths end; the last of those bounces' shadow rays (training paths and
+      // paths past a specular vertex only) are left to the back
       PBRT_HIP(launch_generate_nrc(p, B, st));
       int q = 0;
       const uint32_t front = std::min(p.query_vertex, c->max_depth);
@@ -898,18 +901,15 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         PBRT_HIP(launch_extend(v, B, q, s->spill, spill_per, grid, s->status, st));
         PBRT_HIP(launch_reset_queue(B, q, st));
         PBRT_HIP(launch_shade_nrc(v, p, B, q, st));
-        PBRT_HIP(launch_shadow_nrc(v, p, B, s->spill, spill_per, grid, s->status, st));
+        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v, p, B, s->spill, spill_per, grid, s->status, st));
         q ^= 1;
       }
-      // answer the pass's cache queries with the cache trained on every earlier pass
-      if (passes >= 1) PBRT_HIP(hipStreamWaitEvent(st, ev_back[S ^ 1], 0));
-      const uint32_t* nq = counter(B.counts, kCntQuery);
-      PBRT_HIP(nrc_launch_query(nrc->dev.infer(), (const float*)B.q_raw, nrc->lo, nrc->hi, q_out, npass, nq,
-                                (uint32_t)s->ctx->cu_count, st));
-      PBRT_HIP(launch_nrc_apply(B, q_out, npass, st));
       PBRT_HIP(hipEventRecord(ev_front[S], st));
-      // back: the training paths' remaining bounces, the film, the training
+      // back (second stream, overlapping the next pass's front): the last front shadow rays, the
+      // remaining bounces, the cache queries of the whole pass answered by the cache trained on
+      // every earlier pass (this stream trained it, in pass order), the film, the training
       PBRT_HIP(hipStreamWaitEvent(aux, ev_front[S], 0));
+      PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
       for (uint32_t b = front; b < c->max_depth; ++b) {
         PBRT_HIP(launch_extend(v, B, q, spill_aux, spill_per, grid, s->status, aux));
         PBRT_HIP(launch_reset_queue(B, q, aux));
@@ -917,6 +917,9 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
         q ^= 1;
       }
+      PBRT_HIP(nrc_launch_query(nrc->dev.infer(), (const float*)B.q_raw, nrc->lo, nrc->hi, q_out, npass,
+                                counter(B.counts, kCntQuery), (uint32_t)s->ctx->cu_count, aux));
+      PBRT_HIP(launch_nrc_apply(B, q_out, npass, aux));
       PBRT_HIP(launch_accumulate(p, B, aux));
       if (p.train_stride) {
         const uint32_t ranks = (npass - 1 + p.train_offset) / p.train_stride + 1;
