@@ -60,6 +60,47 @@ def test_zero_cache_without_training_is_plain_cut_at_the_query_vertex(gpu_ctx):
     assert cache.steps == 0
 
 
+def glass_box_scene():
+    """A closed Lambertian box (six rectangles facing inwards) around a glass sphere; no lights."""
+    import math
+    from pbrt_amd.scene import SceneDesc, Transform
+    d = SceneDesc()
+    wall = d.add_material("lambertian", (0.5, 0.5, 0.5))
+    glass = d.add_material("dielectric", (1.0, 1.0, 1.0), ior=1.5)
+    H = 2.0
+    for xf in [Transform.translation(0.0, -H, 0.0),
+               Transform.translation(0.0, H, 0.0) * Transform.rotation("x", math.pi),
+               Transform.translation(0.0, 0.0, -H) * Transform.rotation("x", math.pi / 2),
+               Transform.translation(0.0, 0.0, H) * Transform.rotation("x", -math.pi / 2),
+               Transform.translation(-H, 0.0, 0.0) * Transform.rotation("z", -math.pi / 2),
+               Transform.translation(H, 0.0, 0.0) * Transform.rotation("z", math.pi / 2)]:
+        d.add_object("rectangle", wall, (2 * H, 2 * H), transform=xf)
+    d.add_object("sphere", glass, param=(0.9,), transform=Transform.translation(0.0, 0.0, -0.5))
+    return d
+
+
+def test_queries_past_a_specular_vertex_are_answered(gpu_ctx):
+    """A cache answering the constant c, queried from vertex 1 on, no training: a path ends at its
+    first Lambertian vertex with throughput 1 (Lambertian vertices before it: none; glass vertices:
+    factors (1 / cos) cos), so every pixel is c, including those seen through the glass sphere,
+    whose paths query at a later bounce than the first."""
+    scene = pbrt_amd.Scene(gpu_ctx, glass_box_scene())
+    w = h = 48
+    c = np.array([0.25, 0.5, 0.75], np.float32)
+    cache = pbrt_amd.RadianceCache(gpu_ctx, seed=2)
+    params = np.zeros(pbrt_amd.NRC_PARAMS, np.float32)
+    params[-3:] = c  # output bias; every weight 0
+    cache.params = params
+    film, _, _ = scene.render_ex(w, h, 4, 8, (0.0, 0.0, 1.8), (0.0, 0.0, -0.5), fov=1.2, seed=3, cache=cache,
+                                 query_vertex=1, train_stride=0, record=False)
+    # the sphere (radius 0.9, 2.3 from the eye) covers about a third of the view; a path caught
+    # inside it past the depth limit gives 0, which is rare; (1 / cos) cos differs from 1 by
+    # rounding, by more at grazing refraction (the cosines come from local and world coordinates)
+    ratio = film / c
+    assert np.mean(np.all(np.abs(ratio - 1.0) < 1e-4, axis=2)) > 0.97
+    assert np.all(ratio <= 1.0 + 1e-2)
+
+
 def test_progress_record(gpu_ctx):
     scene, cam = small_scene(gpu_ctx)
     w = h = 32
