@@ -215,6 +215,7 @@ typedef struct pbrt_render_stats {
   uint64_t paths;  /* width * height * spp */
   uint64_t passes;
   double seconds;  /* host wall time of the call */
+  uint64_t cache_queries; /* paths ended into the radiance cache (pbrt_render_ex with a cache), else 0 */
 } pbrt_render_stats;
 
 /* film_out (host, may be NULL): width*height*3 doubles, the per-pixel mean radiance, accumulated

@@ -447,7 +447,8 @@ class Scene:
             progress = progress or {}
             progress["debug"] = (dbg[:debug_slots * 12].reshape(debug_slots, 12),
                                  dbg[debug_slots * 12:].reshape(debug_slots, 4))
-        return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds), progress
+        return out, dict(rays=st.rays, paths=st.paths, passes=st.passes, seconds=st.seconds,
+                        cache_queries=st.cache_queries), progress
 
 
 NRC_PARAMS = 17923

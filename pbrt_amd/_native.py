@@ -38,7 +38,7 @@ class RenderConfig(ctypes.Structure):
 
 
 class RenderStats(ctypes.Structure):
-    _fields_ = [("rays", U64), ("paths", U64), ("passes", U64), ("seconds", F64)]
+    _fields_ = [("rays", U64), ("paths", U64), ("passes", U64), ("seconds", F64), ("cache_queries", U64)]
 
 
 class NrcOptim(ctypes.Structure):

@@ -260,6 +260,27 @@ __device__ __forceinline__ void chain_store(const f32x4 (&y)[RT], float* __restr
   }
 }
 
+// The radiance of the RT tiles' rows, clamped at 0, times the querying path's throughput, added to
+// the path's radiance: beta[row] holds the throughput (xyz) and the path index (w, as bits).
+template <int RT>
+__device__ __forceinline__ void chain_apply(const f32x4 (&y)[RT], const float4* __restrict__ beta, float4* L,
+                                            uint32_t row0, uint32_t n, int r, int g) {
+  if (g != 0) return;
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const uint32_t row = row0 + t * 16 + r;
+    if (row < n) {
+      const float4 bt = beta[row];
+      const uint32_t j = __float_as_uint(bt.w);
+      float4 l = L[j];
+      l.x = l.x + bt.x * fmaxf(y[t][0], 0.f);
+      l.y = l.y + bt.y * fmaxf(y[t][1], 0.f);
+      l.z = l.z + bt.z * fmaxf(y[t][2], 0.f);
+      L[j] = l;
+    }
+  }
+}
+
 // Inference: rows of `x` (N x 96 BF16) -> out (N x 3 FP32), kChainTiles 16-row tiles per wave,
 // grid-stride.
 __global__ __launch_bounds__(256) void k_nrc_infer(NrcForward P, const uint16_t* __restrict__ x, float* __restrict__ out,
@@ -327,14 +348,19 @@ __device__ __forceinline__ bf16x8 features8(const float* q, float3 lo, float3 ie
 
 // Inference straight from raw inputs (n x 12 floats): each lane's layer-0 B operands are computed
 // in registers (no encoded copy in HBM); otherwise k_nrc_infer, with the same arithmetic per row.
+// APPLY: instead of storing the outputs, add them to the querying paths' radiance (chain_apply) and
+// count the queries into `total`.
+template <bool APPLY>
 __global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __restrict__ raw, float3 lo, float3 ie,
                                                    float* __restrict__ out, uint32_t n,
-                                                   const uint32_t* __restrict__ n_dev) {
+                                                   const uint32_t* __restrict__ n_dev, const float4* __restrict__ beta,
+                                                   float4* radiance, unsigned long long* total) {
   constexpr int RT = kChainTiles;
   __shared__ __attribute__((aligned(16))) ChainLds L;
   chain_stage(P, L);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   if (n_dev) n = min(n, *n_dev);
+  if (APPLY && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(total, (unsigned long long)n);
   const uint32_t groups = (n + RT * 16 - 1) / (RT * 16);
   for (uint32_t grp = blockIdx.x * 4 + wave; grp < groups; grp += gridDim.x * 4) {
     const uint32_t row0 = grp * RT * 16;
@@ -359,7 +385,10 @@ __global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __
     }
     f32x4 y[RT];
     chain_forward<RT>(P, L, x0, r, g, y);
-    chain_store<RT>(y, out, row0, n, r, g);
+    if (APPLY)
+      chain_apply<RT>(y, beta, radiance, row0, n, r, g);
+    else
+      chain_store<RT>(y, out, row0, n, r, g);
   }
 }
 
@@ -681,7 +710,23 @@ hipError_t nrc_launch_query(const NrcForward& P, const float* raw, const float l
   uint32_t blocks = (groups + 3) / 4;
   const uint32_t cap = cu * 3;
   if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(k_nrc_query, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, raw, l, ie, out, n, n_dev);
+  hipLaunchKernelGGL(k_nrc_query<false>, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, raw, l, ie, out, n, n_dev,
+                     nullptr, nullptr, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_query_apply(const NrcForward& P, const float* raw, const float4* beta, float4* radiance,
+                                  unsigned long long* total, const float lo[3], const float hi[3], uint32_t n,
+                                  const uint32_t* n_dev, uint32_t cu, void* s) {
+  if (!n) return hipSuccess;
+  const float3 l = make_float3(lo[0], lo[1], lo[2]);
+  const float3 ie = make_float3(1.0f / (hi[0] - lo[0]), 1.0f / (hi[1] - lo[1]), 1.0f / (hi[2] - lo[2]));
+  const uint32_t groups = (n + kChainTiles * 16 - 1) / (kChainTiles * 16);
+  uint32_t blocks = (groups + 3) / 4;
+  const uint32_t cap = cu * 3;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(k_nrc_query<true>, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, raw, l, ie, nullptr, n, n_dev,
+                     beta, radiance, total);
   return hipGetLastError();
 }
 

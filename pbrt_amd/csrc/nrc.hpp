@@ -108,6 +108,11 @@ hipError_t nrc_launch_infer(const NrcForward& F, const uint16_t* x, float* out, 
 // Inference from raw inputs (n x 12 floats, 16-byte aligned rows), features computed in registers.
 hipError_t nrc_launch_query(const NrcForward& F, const float* raw, const float lo[3], const float hi[3], float* out,
                             uint32_t n, const uint32_t* n_dev, uint32_t cu, void* stream);
+// The same queries (at most n, the count at n_dev) with each output, clamped at 0, times beta[k].xyz
+// added to radiance[bits of beta[k].w]; adds the count to *total.
+hipError_t nrc_launch_query_apply(const NrcForward& F, const float* raw, const float4* beta, float4* radiance,
+                                  unsigned long long* total, const float lo[3], const float hi[3], uint32_t n,
+                                  const uint32_t* n_dev, uint32_t cu, void* stream);
 hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
                             const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, void* stream);
 hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,

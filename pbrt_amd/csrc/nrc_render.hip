@@ -1,6 +1,6 @@
-// The radiance cache inside the wavefront renderer: applying cache answers to the paths that ended
-// into it, turning training-path records into (input, target) samples, choosing a frame's training
-// batch, and the FP64 convergence measure against a reference film.
+// The radiance cache inside the wavefront renderer: turning training-path records into (input,
+// target) samples, choosing a frame's training batch, and the FP64 convergence measure against a
+// reference film. (The cache's answers reach the paths that ended into it in k_nrc_query.)
 #pragma clang fp contract(off)
 #include <hip/hip_runtime.h>
 
@@ -8,22 +8,6 @@
 
 namespace pbrt {
 namespace {
-
-// L[j] += beta * max(cache, 0) for every query of the pass (radiance is non-negative)
-__global__ void k_nrc_apply(PathBuffers b, const float* __restrict__ pred, uint32_t cap) {
-  const uint32_t n = min(cap, *counter(b.counts, kCntQuery));
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const float4 bt = b.q_beta[k];
-    const uint32_t j = __float_as_uint(bt.w);
-    const float r = fmaxf(pred[3 * (size_t)k], 0.f), g = fmaxf(pred[3 * (size_t)k + 1], 0.f),
-                bl = fmaxf(pred[3 * (size_t)k + 2], 0.f);
-    float4 L = b.L[j];
-    L.x = L.x + bt.x * r;
-    L.y = L.y + bt.y * g;
-    L.z = L.z + bt.z * bl;
-    b.L[j] = L;
-  }
-}
 
 // Slot (rank, v) for vertices v = 1..V of every training path: the radiance the path gathered over
 // the window the cache stands for at a query vertex — next-event terms at vertices v .. v+W-1 and
@@ -177,11 +161,6 @@ uint32_t grid_for(uint32_t n, uint32_t block, uint32_t cap) {
 }
 
 }  // namespace
-
-hipError_t launch_nrc_apply(const PathBuffers& b, const float* pred, uint32_t cap, void* s) {
-  hipLaunchKernelGGL(k_nrc_apply, dim3(grid_for(cap, 256, 4096)), dim3(256), 0, (hipStream_t)s, b, pred, cap);
-  return hipGetLastError();
-}
 
 hipError_t launch_nrc_targets(const RenderParams& p, const PathBuffers& b, const NrcTrainBuffers& t, uint32_t ranks,
                               void* s) {

@@ -15,7 +15,6 @@ struct NrcTrainBuffers {
   uint32_t block_cap;
 };
 
-hipError_t launch_nrc_apply(const PathBuffers& b, const float* pred, uint32_t cap, void* stream);
 hipError_t launch_nrc_targets(const RenderParams& p, const PathBuffers& b, const NrcTrainBuffers& t, uint32_t ranks,
                               void* stream);
 // Compacts the valid slots and gathers one batch of at most `batch` samples per frame of the pass.

@@ -126,7 +126,7 @@ enum RecField : uint32_t { kRecBeta = 0, kRecEmit = 1, kRecNee = 2, kRecRaw = 3,
 constexpr uint32_t kCounterStride = 64;
 enum Counter : uint32_t {
   kCntQueue0 = 0, kCntQueue1 = 1, kCntShadow = 2, kCntRays = 4 /* uint64 */, kCntQuery = 6, kCntValid = 7,
-  kCntBatch = 8, kCounters = 16
+  kCntBatch = 8, kCntQueries = 10 /* uint64: cache queries answered, every pass */, kCounters = 16
 };
 constexpr uint32_t kCounterWords = kCounters * kCounterStride;
 __host__ __device__ inline uint32_t* counter(uint32_t* base, uint32_t k) { return base + (size_t)k * kCounterStride; }

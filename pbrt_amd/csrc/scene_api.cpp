@@ -751,7 +751,6 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
 
     // radiance cache buffers
     NrcTrainBuffers tb{};
-    float* q_out = nullptr;
     uint16_t* batch_x = nullptr;
     uint32_t ranks_cap = 0;
     if (nrc) {
@@ -760,7 +759,6 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       p.train_stride = ncfg.train_stride;
       pb.q_raw = (float4*)alloc((size_t)npass_paths * 3 * sizeof(float4));
       pb.q_beta = (float4*)alloc((size_t)npass_paths * sizeof(float4));
-      q_out = (float*)alloc((size_t)npass_paths * 3 * sizeof(float));
       if (ncfg.train_stride) {
         ranks_cap = npass_paths / ncfg.train_stride + 2;
         pb.train_cap = ranks_cap;
@@ -814,11 +812,12 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     const uint32_t spill_per = ensure_spill(s, grid);
     uint64_t passes = 0;
     // with the cache: two path sets and a second stream. Pass k (set k % 2) traces bounces
-    // 1..query_vertex and answers its cache queries on the render stream; its training paths'
-    // remaining bounces, the film accumulation and the cache training run on the second stream,
-    // under the next pass's first bounces. The next pass's queries wait for that training, and a
-    // set is reused only once the pass before last has finished with it, so the results are those
-    // of running every step in order on one stream.
+    // 1..query_vertex on the render stream; the remaining bounces (training paths, paths past a
+    // specular vertex), the cache queries, the film accumulation and the cache training run on the
+    // second stream, under the next pass's first bounces. That stream runs the passes' back halves
+    // in order, so pass k's queries see the training of every earlier pass, and a set is reused only
+    // once the pass before last has finished with it: the results are those of running every step
+    // in order on one stream.
     PathBuffers sets[2] = {pb, pb};
     hipStream_t aux = nullptr;
     uint2* spill_aux = nullptr;
@@ -917,9 +916,10 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
         q ^= 1;
       }
-      PBRT_HIP(nrc_launch_query(nrc->dev.infer(), (const float*)B.q_raw, nrc->lo, nrc->hi, q_out, npass,
-                                counter(B.counts, kCntQuery), (uint32_t)s->ctx->cu_count, aux));
-      PBRT_HIP(launch_nrc_apply(B, q_out, npass, aux));
+      PBRT_HIP(nrc_launch_query_apply(nrc->dev.infer(), (const float*)B.q_raw, B.q_beta, B.L,
+                                      reinterpret_cast<unsigned long long*>(counter(B.counts, kCntQueries)),
+                                      nrc->lo, nrc->hi, npass, counter(B.counts, kCntQuery),
+                                      (uint32_t)s->ctx->cu_count, aux));
       PBRT_HIP(launch_accumulate(p, B, aux));
       if (p.train_stride) {
         const uint32_t ranks = (npass - 1 + p.train_offset) / p.train_stride + 1;
@@ -949,12 +949,16 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       passes++;
     }
     if (aux) PBRT_HIP(hipStreamSynchronize(aux));
-    uint64_t rays = 0;
+    uint64_t rays = 0, queries = 0;
     PBRT_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < (nrc ? 2 : 1); ++k) {
       uint64_t r = 0;
       PBRT_HIP(hipMemcpy(&r, counter(sets[k].counts, kCntRays), sizeof(r), hipMemcpyDeviceToHost));
       rays += r;
+      if (nrc) {
+        PBRT_HIP(hipMemcpy(&r, counter(sets[k].counts, kCntQueries), sizeof(r), hipMemcpyDeviceToHost));
+        queries += r;
+      }
     }
     if (film_out) {
       std::vector<double> sum((size_t)npix * 3);
@@ -977,6 +981,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       stats->rays = rays;
       stats->paths = (uint64_t)npix * c->spp;
       stats->passes = passes;
+      stats->cache_queries = queries;
       stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   });
@@ -1147,6 +1152,7 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
       stats->rays = rays;
       stats->paths = paths;
       stats->passes = passes;
+      stats->cache_queries = 0;
       stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   });
