@@ -30,14 +30,11 @@ HBM_PEAK_GBS = 8000.0
 SEED_MESH = 2024
 SEED_RAYS = 7
 
-# Memory-side bytes per k_closest launch on this workload (mean of the coherent and incoherent
-# launches), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2,
-# WRITE x1); profiles/k_closest_icosphere1p3m_counters.md. Keyed by (block, waves_per_cu,
-# lds_stack): reported only for the launch shape that was measured.
-MEASURED_TRAFFIC = {
-    (64, 32, 8): {"bytes_per_launch": 1282.23e6, "coherent": 255.70e6, "incoherent": 2308.76e6,
-                  "source": "profiles/k_closest_icosphere1p3m_counters.md"},
-}
+# Memory-side bytes of one k_closest launch on each batch of this workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy; profiles/k_closest_icosphere1p3m_counters.md.
+# Keyed by (batch, block, waves_per_cu, lds_stack, refill): reported only for the launch shapes that
+# were measured.
+MEASURED_TRAFFIC = {}
 
 
 def rows(r):
@@ -249,6 +246,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--lds-stack", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
@@ -286,8 +284,8 @@ def main():
     t0 = time.time()
     mesh = pa.TriangleMesh(ctx, v, f, **pa.DEVICE_BVH)
     build_s = time.time() - t0
-    if args.block or args.waves or args.lds_stack:
-        mesh.set_launch(args.block, args.waves, args.lds_stack)
+    if args.block or args.waves or args.lds_stack or args.refill:
+        mesh.set_launch(args.block, args.waves, args.lds_stack, args.refill)
     bs = mesh.build_stats()
     node_bytes, tri_bytes = mesh.device_bytes()
     print(f"[bench] BVH built in {build_s:.1f}s: {bs}", file=sys.stderr, flush=True)
@@ -302,10 +300,15 @@ def main():
         dev[k] = (all_rays.view(32 * off, 32 * n), all_hits.view(16 * off, 16 * n), n)
         off += n
 
+    # launch shape per batch: the autotuner's choice on that batch's own rays, or the fixed one
     tuned = None
-    if not (args.block or args.waves or args.lds_stack or args.no_autotune):
-        tuned = mesh.autotune(all_rays, off, all_hits, any_hit=False, reps=2)
-        print(f"[bench] autotuned launch: {tuned}", file=sys.stderr, flush=True)
+    launches = {k: mesh.get_launch() for k in dev}
+    if not (args.block or args.waves or args.lds_stack or args.refill or args.no_autotune):
+        tuned = {}
+        for k, (dr, dh, n) in dev.items():
+            tuned[k] = mesh.autotune(dr, n, dh, any_hit=False, reps=2)
+            launches[k] = {x: tuned[k][x] for x in ("block", "waves_per_cu", "lds_stack", "refill")}
+        print(f"[bench] autotuned launches: {tuned}", file=sys.stderr, flush=True)
 
     # exact per-ray work on the same rays (instrumented kernel, untimed)
     work = {}
@@ -328,13 +331,15 @@ def main():
     evs = [[(ctx.event(), ctx.event()) for _ in dev] for _ in range(args.steps)]
 
     for _ in range(args.warmup):
-        for dr, dh, n in dev.values():
+        for k, (dr, dh, n) in dev.items():
+            mesh.set_launch(**launches[k])
             mesh.query(dr, n, dh)
     ctx.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        for j, (dr, dh, n) in enumerate(dev.values()):
+        for j, (k, (dr, dh, n)) in enumerate(dev.items()):
             e0, e1 = evs[s][j]
+            mesh.set_launch(**launches[k])
             e0.record(stream)
             mesh.query(dr, n, dh)
             e1.record(stream)
@@ -353,15 +358,15 @@ def main():
     # roofline over both launches of the dominant kernel: algorithmic bytes / mean kernel time
     alg_bytes = sum(work[k]["bytes_per_launch"] for k in dev)
     achieved = alg_bytes / (step_kernel_ms * 1e-3) / 1e9
-    launch = mesh.get_launch()
-    measured = MEASURED_TRAFFIC.get((launch["block"], launch["waves_per_cu"], launch["lds_stack"]))
-    if args.side != 2048:
-        measured = None
+    measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "waves_per_cu", "lds_stack",
+                                                                             "refill"))) for k in dev]
+    measured = None if (args.side != 2048 or None in measured) else measured
 
     # any-hit throughput on the same batches (informational)
     anyhit = {}
     for k, (dr, dh, n) in dev.items():
         occ = ctx.alloc(n)
+        mesh.set_launch(**launches[k])
         mesh.intersect_p(dr, n, occ)
         e0, e1 = evs[0][0]
         e0.record(stream)
@@ -419,7 +424,7 @@ def main():
             "rays_per_step": total_rays,
             "bvh": dict(bs, node_bytes=node_bytes, tri_bytes=tri_bytes),
             "parallelism": "single GPU",
-            "launch": launch,
+            "launch": launches,
             "autotuned": tuned is not None,
         },
         "roofline": {
@@ -428,10 +433,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured["bytes_per_launch"] if measured else None,
+            "traffic": sum(m["bytes"] for m in measured) / len(measured) if measured else None,
             "kernel": "k_closest",
             "algorithmic_bytes_per_launch": alg_bytes / len(dev),
-            "traffic_source": measured["source"] if measured else None,
+            "traffic_source": measured[0]["source"] if measured else None,
         },
         "per_batch": per_batch,
         "anyhit_mrays_s": anyhit,

@@ -185,20 +185,20 @@ class TriangleMesh:
         call("pbrt_mesh_device_bytes", self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
-    def set_launch(self, block=0, waves_per_cu=0, lds_stack=0):
-        call("pbrt_mesh_set_launch", self.h, ctypes.byref(Launch(block, waves_per_cu, lds_stack, 0)))
+    def set_launch(self, block=0, waves_per_cu=0, lds_stack=0, refill=0):
+        call("pbrt_mesh_set_launch", self.h, ctypes.byref(Launch(block, waves_per_cu, lds_stack, refill)))
 
     def get_launch(self):
         l = Launch()
         call("pbrt_mesh_get_launch", self.h, ctypes.byref(l))
-        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack)
+        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, refill=l.refill)
 
     def autotune(self, rays, n, out, any_hit=False, reps=2):
         """Times every launch shape on these device rays and keeps the fastest (pbrt_mesh_autotune)."""
         l = Launch()
         ms = ctypes.c_float()
         call("pbrt_mesh_autotune", self.h, rays.ptr, n, out.ptr, int(any_hit), reps, ctypes.byref(l), ctypes.byref(ms))
-        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, ms=ms.value)
+        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, refill=l.refill, ms=ms.value)
 
     def export_tree(self):
         s = self.build_stats()

@@ -59,6 +59,7 @@ LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
   a.block = m->launch.block;
   a.lds_stack = m->launch.lds_stack;
   a.waves_per_cu = m->launch.waves_per_cu;
+  a.refill = m->launch.refill;
   a.cu_count = (uint32_t)m->ctx->cu_count;
   return a;
 }
@@ -316,6 +317,7 @@ int pbrt_mesh_set_launch(pbrt_mesh* m, const pbrt_launch* l) {
     if (v.lds_stack != 8 && v.lds_stack != 16 && v.lds_stack != 24 && v.lds_stack != 32)
       throw Error(PBRT_ERR_INVALID_ARG, "lds_stack must be 8, 16, 24 or 32");
     if (v.waves_per_cu > 32) throw Error(PBRT_ERR_INVALID_ARG, "waves_per_cu above 32");
+    if (v.refill > 64) throw Error(PBRT_ERR_INVALID_ARG, "refill above 64 lanes");
     m->launch = v;
   });
 }
@@ -354,8 +356,8 @@ int pbrt_mesh_trace_any(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, uint8_t*
 
 // Times every supported launch shape of one traversal kernel on the caller's rays and keeps the
 // fastest: block {64, 128, 256} x LDS stack {8, 16, 24, 32} x resident waves per CU
-// {8, 12, 16, 24, 32}; each candidate is run `reps` times after one warm-up launch, scored by its
-// fastest run. The hits/occlusion output buffer is overwritten.
+// {8, 12, 16, 24, 32} x refill {0, 1, 4}; each candidate is run `reps` times after one warm-up
+// launch, scored by its fastest run. The hits/occlusion output buffer is overwritten.
 int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out_dev, int any_hit, uint32_t reps,
                        pbrt_launch* chosen, float* best_ms) {
   return guarded([&] {
@@ -369,35 +371,37 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
     const uint32_t blocks[] = {64, 128, 256};
     const uint32_t depths[] = {8, 16, 24, 32};
     const uint32_t waves[] = {8, 12, 16, 24, 32};
+    const uint32_t refills[] = {0, 1, 4};
     pbrt_launch best{};
     float best_t = 1e30f;
     const int kind = any_hit ? 2 : 0;
     try {
       for (uint32_t b : blocks)
         for (uint32_t d : depths)
-          for (uint32_t w : waves) {
-            if (w < b / 64) continue;
-            m->launch = pbrt_launch{b, w, d, 0};
-            LaunchArgs a = base_args(m, n, nullptr);
-            a.rays = rays;
-            a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
-            a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
-            run(m, a, kind);  // warm-up
-            float cand = 1e30f;
-            for (uint32_t r = 0; r < reps; ++r) {
-              PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
-              run(m, a, kind);
-              PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
-              PBRT_HIP(hipEventSynchronize(e1));
-              float ms = 0.f;
-              PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
-              cand = std::min(cand, ms);
+          for (uint32_t w : waves)
+            for (uint32_t f : refills) {
+              if (w < b / 64) continue;
+              m->launch = pbrt_launch{b, w, d, f};
+              LaunchArgs a = base_args(m, n, nullptr);
+              a.rays = rays;
+              a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
+              a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
+              run(m, a, kind);  // warm-up
+              float cand = 1e30f;
+              for (uint32_t r = 0; r < reps; ++r) {
+                PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
+                run(m, a, kind);
+                PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
+                PBRT_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                cand = std::min(cand, ms);
+              }
+              if (cand < best_t) {
+                best_t = cand;
+                best = m->launch;
+              }
             }
-            if (cand < best_t) {
-              best_t = cand;
-              best = m->launch;
-            }
-          }
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -33,16 +33,21 @@ struct Closest {
   uint32_t best, n_vis, n_box, n_tri;
 };
 
-// Flat traversal loop: each iteration advances every live lane by one interior node and/or one
-// leaf, and a lane whose ray is finished writes its result and takes its next ray at once (rays
-// i, i + stride, i + 2*stride, ... per lane) instead of waiting for the rest of its wave. The order
-// of node visits within a ray is the reference's, so results and work counters are unchanged.
-template <int BLOCK, int DEPTH, bool STATS>
+// Flat traversal loop: each iteration advances every live lane by one interior node, or by one
+// triangle of the leaf it is in (a leaf's triangles take one iteration each, so the wave never
+// runs a whole leaf loop for a few lanes). A lane whose ray is finished writes its result and
+// takes its next ray (i, i + stride, i + 2*stride, ... per lane) instead of waiting for the rest of
+// its wave: at once (WAVE_REFILL false), or (WAVE_REFILL true) together with the other waiting
+// lanes once `refill` of them wait or no lane of the wave is still tracing. The two forms compile
+// to different schedules; which is faster depends on the rays' coherence, and the autotuner picks.
+// The order of node visits and triangle tests within a ray is the reference's, so results and work
+// counters are the same either way.
+template <int BLOCK, int DEPTH, bool STATS, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                    RootDesc root, const float4* __restrict__ rays,
                                                    float4* __restrict__ hits, uint32_t n, uint32_t* __restrict__ stats,
                                                    uint2* __restrict__ spill, uint32_t spill_cap,
-                                                   int* __restrict__ status) {
+                                                   int* __restrict__ status, uint32_t refill) {
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
@@ -51,7 +56,8 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
   RayState r;
   Closest s;
   int32_t cur = 0;
-  bool live = false;
+  uint32_t tk = 0, tend = 0;  // triangles of the current leaf still to test: [tk, tend)
+  bool live = false, idle = false;
   auto start = [&](uint32_t k) {
     r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
     s.min_t = FLT_MAX;
@@ -62,6 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
     s.n_box = 1;
     s.n_tri = 0;
     st.sp = 0;
+    tk = tend = 0;
     float entry;
     live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
     cur = root.ref;
@@ -96,23 +103,23 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
       }
     }
     if (live && !pop_now && cur < 0) {
-      // leaf: ~cur = (first << 5) | (count - 1)
-      if (STATS) ++s.n_vis;
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code >> kLeafCountBits;
-      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        if (STATS) ++s.n_tri;
-        float t, u, v;
-        uint32_t id;
-        if (tri_test(r, tris, k, &t, &u, &v, &id)) {
-          if (t >= r.near && t <= r.far && (t < s.min_t || (t == s.min_t && id < s.best))) {
-            s.min_t = t; s.t = t; s.u = u; s.v = v; s.best = id;
-            s.cur_far = fminf(r.far, s.min_t);
-          }
+      if (tk == tend) {  // a leaf reached: ~cur = (first << 5) | (count - 1)
+        if (STATS) ++s.n_vis;
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
+      }
+      if (STATS) ++s.n_tri;
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id)) {
+        if (t >= r.near && t <= r.far && (t < s.min_t || (t == s.min_t && id < s.best))) {
+          s.min_t = t; s.t = t; s.u = u; s.v = v; s.best = id;
+          s.cur_far = fminf(r.far, s.min_t);
         }
       }
-      pop_now = true;
+      ++tk;
+      pop_now = tk == tend;
     }
     if (live && pop_now) {
       // the next pending node whose entry is not beyond the nearest hit
@@ -125,7 +132,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
         break;
       }
     }
-    if (!live) {
+    if (!live && !idle) {
       const bool found = s.best != 0xffffffffu;
       hits[i] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
                             found ? s.v : 0.f);
@@ -136,18 +143,28 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
       }
       i += stride;
       if (i >= n) break;
-      start(i);
+      if (WAVE_REFILL)
+        idle = true;
+      else
+        start(i);
+    }
+    if (WAVE_REFILL) {
+      const uint64_t waiting = __ballot(idle), active = __ballot(1);
+      if (idle && ((uint32_t)__popcll(waiting) >= refill || waiting == active)) {
+        start(i);
+        idle = false;
+      }
     }
   }
 }
 
 // Any hit, unordered (bvh.rs:381-420): the reference pushes left then right, so the right child is
 // examined first; the first triangle hit within [near, far] ends the ray. Same flat loop.
-template <int BLOCK, int DEPTH>
+template <int BLOCK, int DEPTH, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                RootDesc root, const float4* __restrict__ rays,
                                                uint8_t* __restrict__ occluded, uint32_t n, uint2* __restrict__ spill,
-                                               uint32_t spill_cap, int* __restrict__ status) {
+                                               uint32_t spill_cap, int* __restrict__ status, uint32_t refill) {
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
@@ -155,11 +172,13 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
   if (i >= n) return;
   RayState r;
   int32_t cur = 0;
-  bool live = false, hit = false;
+  uint32_t tk = 0, tend = 0;
+  bool live = false, hit = false, idle = false;
   auto start = [&](uint32_t k) {
     r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
     st.sp = 0;
     hit = false;
+    tk = tend = 0;
     float entry;
     live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
     cur = root.ref;
@@ -191,29 +210,39 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
       }
     }
     if (live && !pop_now && cur < 0) {
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code >> kLeafCountBits;
-      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        float t, u, v;
-        uint32_t id;
-        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
-          hit = true;
-          break;
-        }
+      if (tk == tend) {
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
       }
-      pop_now = true;
-      if (hit) live = false;
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+        hit = true;
+        live = false;
+      }
+      ++tk;
+      pop_now = tk == tend;
     }
     if (live && pop_now) {
       live = st.sp > 0;
       if (live) cur = (int32_t)st.pop().x;
     }
-    if (!live) {
+    if (!live && !idle) {
       occluded[i] = hit ? 1 : 0;
       i += stride;
       if (i >= n) break;
-      start(i);
+      if (WAVE_REFILL)
+        idle = true;
+      else
+        start(i);
+    }
+    if (WAVE_REFILL) {
+      const uint64_t waiting = __ballot(idle), active = __ballot(1);
+      if (idle && ((uint32_t)__popcll(waiting) >= refill || waiting == active)) {
+        start(i);
+        idle = false;
+      }
     }
   }
 }
@@ -235,15 +264,27 @@ hipError_t launch_t(const LaunchArgs& a, int kind) {
   const uint32_t spill_cap = a.spill_entries_per_thread;
   if ((uint64_t)spill_cap * spill_stride > a.spill_capacity_entries) return hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)a.stream;
-  if (kind == 0)
-    hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
-                       (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status);
+  const bool wave = a.refill > 0;
+  if (kind == 0 && !wave)
+    hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, false, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
+                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status, 0u);
+  else if (kind == 0)
+    hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, false, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
+                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status,
+                       a.refill);
+  else if (kind == 1 && !wave)
+    hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, true, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
+                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, a.stats, a.spill, spill_cap, a.status, 0u);
   else if (kind == 1)
-    hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
-                       (const float4*)a.rays, (float4*)a.hits, a.n, a.stats, a.spill, spill_cap, a.status);
+    hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, true, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
+                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, a.stats, a.spill, spill_cap, a.status,
+                       a.refill);
+  else if (!wave)
+    hipLaunchKernelGGL((k_any<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
+                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, 0u);
   else
-    hipLaunchKernelGGL((k_any<BLOCK, DEPTH>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
-                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status);
+    hipLaunchKernelGGL((k_any<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
+                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, a.refill);
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ struct LaunchArgs {
   int* status;
   void* stream;
   uint32_t block, lds_stack, waves_per_cu, cu_count;
+  uint32_t refill;  // 0: eager per-lane refill; N > 0: wave refill once N lanes wait
 };
 
 // kind: 0 closest hit, 1 closest hit with per-ray work counters, 2 any hit.

@@ -81,12 +81,15 @@ def test_reference_grid_fixture_and_brute_force(gpu_ctx, precision_params):
     assert (hits["prim"] >= 0).sum() > 20
 
 
-def test_any_hit_and_far_sweep(gpu_ctx):
+@pytest.mark.parametrize("refill", [0, 1, 64])
+def test_any_hit_and_far_sweep(gpu_ctx, refill):
     """triangle_mesh/bvh.rs:937-975 on the device: any-hit equals closest-hit existence, also with
-    `far` cut at factors of the nearest distance; and equals the oracle's any-hit."""
+    `far` cut at factors of the nearest distance; and equals the oracle's any-hit. For each refill
+    policy (64: lanes wait until the whole wave is done)."""
     v, f = load_mesh("dragon_vrip_res4")
     rays = po.orbit_rays(v).astype(np.float32)[::3]
     mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
+    mesh.set_launch(refill=refill)
     orc = po.Mesh(v, f, precision=32, **pbrt_amd.DEVICE_BVH)
     hits = mesh.trace_host(rays)
     occ = mesh.trace_host(rays, mode="any")
@@ -104,14 +107,15 @@ def test_any_hit_and_far_sweep(gpu_ctx):
 
 @pytest.mark.parametrize("block", [64, 128, 256])
 @pytest.mark.parametrize("lds_stack", [8, 16, 32])
-def test_launch_shapes_agree(gpu_ctx, block, lds_stack):
-    """Every launch shape gives the oracle's answer, including an 8-entry LDS stack that spills to
-    HBM on the deep reference-mode tree."""
+@pytest.mark.parametrize("refill", [0, 4])
+def test_launch_shapes_agree(gpu_ctx, block, lds_stack, refill):
+    """Every launch shape and refill policy gives the oracle's answer, including an 8-entry LDS
+    stack that spills to HBM on the deep reference-mode tree."""
     v, f = load_mesh("dragon_vrip_res3")
     rays = np.concatenate([as_rows(pbrt_amd.orbit_rays(v, 1, 96)),
                            as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 20000, seed=9))])
     check_closest(gpu_ctx, v, f, rays, pbrt_amd.REFERENCE_BVH, stats=True,
-                  launch=dict(block=block, lds_stack=lds_stack, waves_per_cu=8))
+                  launch=dict(block=block, lds_stack=lds_stack, waves_per_cu=8, refill=refill))
 
 
 def test_incoherent_rays_on_displaced_icosphere(gpu_ctx):
