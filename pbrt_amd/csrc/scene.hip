@@ -1331,6 +1331,17 @@ hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream) {
   return hipGetLastError();
 }
 
+uint32_t scene_blocks_per_cu() {
+  // the fewest workgroups of the traversal kernels that fit on one CU at once: a persistent grid
+  // larger than that would leave workgroups waiting for a slot and finishing last
+  int a = 0, b = 0, c = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<kSceneStack>, kSceneBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<kSceneStack, false>, kSceneBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<kSceneStack, true>, kSceneBlock, 0) != hipSuccess)
+    return 0;
+  return (uint32_t)(a < b ? (a < c ? a : c) : (b < c ? b : c));
+}
+
 hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream) {
   hipLaunchKernelGGL((k_extend<kSceneStack>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, b, q, spill,

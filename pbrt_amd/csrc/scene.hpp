@@ -161,6 +161,8 @@ __host__ __device__ inline bool is_training_path(const RenderParams& p, uint32_t
 }
 
 hipError_t launch_generate(const SceneView& v, const RenderParams& p, const PathBuffers& b, void* stream);
+// Workgroups of the scene traversal kernels resident on one CU at once (0 if the runtime cannot tell).
+uint32_t scene_blocks_per_cu();
 hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream);
 hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream);

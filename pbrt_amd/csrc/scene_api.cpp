@@ -123,7 +123,12 @@ SceneView view_of(const pbrt_scene* s) {
 }
 
 uint32_t persistent_grid(const pbrt_scene* s, uint32_t n) {
-  const uint32_t cap = (uint32_t)std::max(1, s->ctx->cu_count) * 8;  // 8 blocks of 2 waves per CU
+  // as many workgroups of 2 waves as fit on every CU at once (8 at most)
+  static const uint32_t per_cu = [] {
+    const uint32_t b = scene_blocks_per_cu();
+    return b ? std::min(b, 8u) : 8u;
+  }();
+  const uint32_t cap = (uint32_t)std::max(1, s->ctx->cu_count) * per_cu;
   const uint32_t need = (n + kSceneBlock - 1) / kSceneBlock;
   return std::max(1u, std::min(cap, need));
 }
