@@ -224,6 +224,15 @@ typedef struct pbrt_render_stats {
  * in FP64 in a fixed sample order. */
 int pbrt_render(pbrt_scene* scene, const pbrt_render_config* config, double* film_out, pbrt_render_stats* stats);
 
+/* Film output (host). pbrt_film_to_rgba8: width*height RGBA bytes of a film as the reference
+ * writes its image — Spectrum::gamma_correct then to_rgb (spectrum.rs:13-30, renderers/mt.rs:83-90):
+ * sqrt, clamp to [0, 1], x255, truncated; alpha 255. tonemap 1 applies Reinhard x / (1 + x) per
+ * channel first (an option the reference lacks), 0 none. pbrt_film_write_png: the same pixels as an
+ * 8-bit RGBA PNG file (image_write, renderers/mt.rs:152-165), stored deflate blocks: the file is a
+ * pure function of the pixels. */
+int pbrt_film_to_rgba8(const double* film, uint32_t width, uint32_t height, uint32_t tonemap, uint8_t* rgba_out);
+int pbrt_film_write_png(const double* film, uint32_t width, uint32_t height, uint32_t tonemap, const char* path);
+
 /* A batch of n render jobs (1 to 16) on one scene traced together (no counterpart in the
  * reference, which renders one image per run): every pass gives each unfinished job a share of
  * about 4M paths (at least one sample per pixel), and the jobs' paths share the wavefront's

@@ -7,6 +7,7 @@ is built from vertices and indices, reports `build_stats()` and `triangle_count(
 include/pbrt_gpu.h. There is no CPU fallback: without the native library every call raises.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -578,6 +579,22 @@ def mfma_check(ctx, a_bits, b_bits):
     d = np.empty((16, 16), np.float32)
     call("pbrt_nrc_mfma_check", ctx.h, _ptr(a), _ptr(b), _ptr(d))
     return d
+
+
+def film_to_rgba8(film, tonemap=False):
+    """The film (height, width, 3) as the reference's 8-bit RGBA image (pbrt_film_to_rgba8)."""
+    f = np.ascontiguousarray(film, np.float64)
+    h, w = f.shape[:2]
+    out = np.empty((h, w, 4), np.uint8)
+    call("pbrt_film_to_rgba8", _ptr(f), w, h, int(bool(tonemap)), _ptr(out))
+    return out
+
+
+def write_png(film, path, tonemap=False):
+    """The film (height, width, 3) as an 8-bit RGBA PNG file (pbrt_film_write_png)."""
+    f = np.ascontiguousarray(film, np.float64)
+    h, w = f.shape[:2]
+    call("pbrt_film_write_png", _ptr(f), w, h, int(bool(tonemap)), os.fsencode(path))
 
 
 def to_rgb8(film):
