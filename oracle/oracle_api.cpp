@@ -402,6 +402,14 @@ int orc_scene_bvh_stats(void* hv, uint32_t w, uint32_t hgt, double fov, const do
   return 0;
 }
 
+// Multiple importance sampling in the path integrator (0: the reference's estimator).
+int orc_scene_set_mis(void* hv, int mis) {
+  auto* h = (SceneHandle*)hv;
+  if (h->a) h->a->mis = mis != 0;
+  if (h->b) h->b->mis = mis != 0;
+  return 0;
+}
+
 // Rows [y0, y1) of a width x height render: film receives the per-pixel mean radiance (f64 RGB).
 int orc_scene_render(void* hv, uint32_t w, uint32_t hgt, double fov, const double* eye, const double* look, uint32_t spp,
                      uint32_t max_depth, uint64_t seed, double near, double far, uint32_t y0, uint32_t y1, double* film,

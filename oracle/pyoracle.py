@@ -145,6 +145,8 @@ def _scene_sigs(L):
     for fn in (L.orc_scene_trace_f32, L.orc_scene_trace_f64):
         fn.restype = ctypes.c_int
         fn.argtypes = [P, P, U64, ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int]
+    L.orc_scene_set_mis.restype = ctypes.c_int
+    L.orc_scene_set_mis.argtypes = [P, ctypes.c_int]
     L.orc_scene_render.restype = ctypes.c_int
     L.orc_scene_render.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, ctypes.c_uint32,
                                    ctypes.c_uint32, U64, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
@@ -215,7 +217,9 @@ class Scene:
         return dict(rays=int(out[0]), primary_hits=int(out[1]), shadow_hits=int(out[2]),
                     primary=st[:3] / out[0], shadow=st[3:] / max(int(out[1]), 1))
 
-    def render(self, w, h, fov, eye, look, spp, max_depth, seed, near=1e-4, far=1000.0, rows=None, threads=8):
+    def render(self, w, h, fov, eye, look, spp, max_depth, seed, near=1e-4, far=1000.0, rows=None, threads=8,
+               mis=False):
+        lib().orc_scene_set_mis(self.h, int(bool(mis)))
         y0, y1 = rows if rows is not None else (0, h)
         film = np.zeros(((y1 - y0) * w, 3), np.float64)
         rays = ctypes.c_uint64()

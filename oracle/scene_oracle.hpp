@@ -18,7 +18,10 @@
 //   * a counter-based sampler keyed by (seed, pixel, sample, bounce, slot) (ideas/rng_graine.md);
 //   * AreaLight on a rectangle with uniform area sampling and one-sided emission
 //     (ideas/area_light.md:27-165), the specular guard kept;
-//   * the GGX microfacet conductor/dielectric-free glossy reflector (new).
+//   * the GGX microfacet conductor/dielectric-free glossy reflector (new);
+//   * multiple importance sampling of next-event estimation and BSDF sampling with the power
+//     heuristic, as the reference plans it after the area light (ideas/area_light.md:158-165,
+//     IDEAS.md:18-30): an option (mis), off by default.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -164,6 +167,7 @@ class SceneOracle {
   std::vector<MaterialDesc> mats;
   std::vector<LightDesc> lights;
   bool canonical;
+  bool mis = false;  // multiple importance sampling (power heuristic) instead of the specular guard
   // scene BVH in the reference's layout (bvh.rs:15-31)
   struct SNode {
     AABB<double> box;
@@ -856,11 +860,57 @@ class SceneOracle {
     return true;
   }
 
-  // integrators/path.rs:48-115 for one camera ray.
+  // Density, in solid angle, with which scatter() produces direction wi_w at this vertex; 0 for
+  // the specular and non-scattering materials.
+  R bsdf_pdf(const MaterialDesc& m, const Isect<R>& is, V3<R> wo_w, V3<R> wi_w) const {
+    if (m.kind == kLambertian) {
+      R c = dot(is.n, wi_w);
+      return c > R(0) ? c / R(M_PI) : R(0);
+    }
+    if (m.kind == kGGX) {
+      V3<R> wo = to_local(is, wo_w), wi = to_local(is, wi_w);
+      if (wo.z <= R(0) || wi.z <= R(0)) return R(0);
+      V3<R> h = normalized(wo + wi);
+      return ggx_d(h.z, (R)m.roughness) * h.z / (R(4) * dot(wo, h));
+    }
+    return R(0);
+  }
+  // Density, in solid angle and with the choice of light, with which next-event estimation picks
+  // direction d from o towards the area light on object k, met at distance t; 0 if no area light
+  // sits on k or d reaches its back.
+  R light_pdf(uint32_t k, V3<R> d, R t) const {
+    for (const LightDesc& L : lights) {
+      if (L.kind == kPointLight || (uint32_t)L.object != k) continue;
+      const Obj& o = objs[k];
+      V3<R> nl = o.d.has_xform ? normalized(o.x.normal(o.x.inv, V3<R>{0, 1, 0})) : V3<R>{0, 1, 0};
+      R cos_l = -dot(nl, d);
+      if (!(cos_l > R(0))) return R(0);
+      R sx = o.d.has_xform ? length(o.x.dir(o.x.m, V3<R>{1, 0, 0})) : R(1);
+      R sz = o.d.has_xform ? length(o.x.dir(o.x.m, V3<R>{0, 0, 1})) : R(1);
+      R area = (R)o.d.param[0] * sx * (R)o.d.param[1] * sz;
+      return (R(1) / (R)lights.size()) * (t * t / (area * cos_l));
+    }
+    return R(0);
+  }
+  // power heuristic weight of the strategy with density a against b, a^2 / (a^2 + b^2), written as
+  // 1 / (1 + (b / a)^2) so that an infinite density (a light sample at a grazing angle) gives 1 or
+  // 0 rather than inf / inf; 0 when a is 0, 1/2 when both are infinite
+  static R power(R a, R b) {
+    if (!(a > R(0))) return R(0);
+    R r = b / a;
+    if (r != r) return R(0.5);
+    return R(1) / (R(1) + r * r);
+  }
+
+  // integrators/path.rs:48-115 for one camera ray. With mis: next-event estimation weighted by
+  // the power heuristic against the BSDF's density of the light's direction, and the emission a
+  // BSDF-sampled ray meets after a non-specular vertex added too, weighted the other way (an
+  // emitter no light samples gets weight 1).
   Spec<R> li(const Ray<R>& cam, R near, R far, uint32_t max_depth, uint64_t seed, uint64_t path, uint64_t* rays_cast) const {
     Spec<R> beta{{1, 1, 1}}, L{{0, 0, 0}};
     Ray<R> ray = cam;
     bool specular = true;
+    R prev_pdf = R(0);  // BSDF density of the direction that led here (non-specular vertices)
     uint32_t bounce = 0;
     SStats st;
     for (;;) {
@@ -869,6 +919,10 @@ class SceneOracle {
       if (!intersect(ray, near, far, &hit, &st)) break;  // no infinite light in these scenes
       const MaterialDesc& m = mats[objs[hit.object].d.material];
       if (specular && m.kind == kDiffuseLight) L = sadd(L, smul(albedo(m), beta));
+      if (mis && !specular && m.kind == kDiffuseLight) {
+        R w = power(prev_pdf, light_pdf((uint32_t)hit.object, ray.d, hit.is.d));
+        L = sadd(L, sscale(smul(albedo(m), beta), w));
+      }
       bounce += 1;
       if (bounce == max_depth) break;
       if (!lights.empty()) {
@@ -880,8 +934,10 @@ class SceneOracle {
         LiSample ls;
         if (sample_li(Ld, hit.is, seed, path, bounce, &ls)) {
           Spec<R> f = sscale(mat_f(m, hit.is, ray.d * R(-1), ls.wi), std::fabs(dot(ls.wi, hit.is.n)));
+          Spec<R> c = sdiv(smul(smul(ls.li, f), beta), pl * ls.pdf);
+          if (mis && Ld.kind != kPointLight) c = sscale(c, power(pl * ls.pdf, bsdf_pdf(m, hit.is, ray.d * R(-1), ls.wi)));
           (*rays_cast)++;
-          if (!intersect_p(ls.shadow, R(1e-5), ls.dist, &st)) L = sadd(L, sdiv(smul(smul(ls.li, f), beta), pl * ls.pdf));
+          if (!intersect_p(ls.shadow, R(1e-5), ls.dist, &st)) L = sadd(L, c);
         }
       }
       Scatter s;
@@ -893,6 +949,7 @@ class SceneOracle {
       if (!(std::isfinite(beta.c[0]) && std::isfinite(beta.c[1]) && std::isfinite(beta.c[2]))) break;
       ray = s.ray;
       specular = s.specular;
+      prev_pdf = s.pdf;
     }
     return L;
   }

@@ -903,6 +903,49 @@ __device__ __forceinline__ bool sample_li(const SceneView& S, const DevLight& L,
   return true;
 }
 
+// Multiple importance sampling (the reference's plan after the area light, ideas/area_light.md:
+// 158-165): the densities, in solid angle, of the two strategies that can produce a direction.
+// BSDF: the density with which scatter() produces wi_w (0 for specular and non-scattering materials).
+__device__ __forceinline__ float bsdf_pdf(const DevMaterial& m, const Isect& is, V wo_w, V wi_w) {
+  if (m.kind == kMatLambertian) {
+    const float c = vdot(is.n, wi_w);
+    return c > 0.0f ? c / kPi : 0.0f;
+  }
+  if (m.kind == kMatGGX) {
+    const V wo = to_local(is, wo_w), wi = to_local(is, wi_w);
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+    const V h = vnorm(vadd(wo, wi));
+    return ggx_d(h.z, m.roughness) * h.z / (4.0f * vdot(wo, h));
+  }
+  return 0.0f;
+}
+// Light: the density, with the choice of light, with which next-event estimation picks direction d
+// towards the area light on object k met at distance t (sample_li's pdf); 0 if no area light sits
+// on k or d reaches its back.
+__device__ __forceinline__ float light_pdf(const SceneView& S, uint32_t k, V d, float t) {
+  for (uint32_t i = 0; i < S.n_lights; ++i) {
+    const DevLight& L = S.lights[i];
+    if (L.kind == kLightPoint || L.object != k) continue;
+    const DevObject& o = S.objs[k];
+    const V nl = o.has_xform ? vnorm(xnormal(o.minv, V{0.f, 1.f, 0.f})) : V{0.f, 1.f, 0.f};
+    const float cos_l = -vdot(nl, d);
+    if (!(cos_l > 0.0f)) return 0.0f;
+    const float sx = o.has_xform ? vlen(xdir(o.m, V{1.f, 0.f, 0.f})) : 1.0f;
+    const float sz = o.has_xform ? vlen(xdir(o.m, V{0.f, 0.f, 1.f})) : 1.0f;
+    const float area = o.param[0] * sx * o.param[1] * sz;
+    return (1.0f / (float)S.n_lights) * (t * t / (area * cos_l));
+  }
+  return 0.0f;
+}
+// Power heuristic a^2 / (a^2 + b^2) as 1 / (1 + (b / a)^2): an infinite density (a light sample at
+// a grazing angle) gives 1 or 0 rather than inf / inf; 0 when a is 0, 1/2 when both are infinite.
+__device__ __forceinline__ float mis_power(float a, float b) {
+  if (!(a > 0.0f)) return 0.0f;
+  const float r = b / a;
+  if (r != r) return 0.5f;
+  return 1.0f / (1.0f + r * r);
+}
+
 // ---- wavefront kernels -------------------------------------------------------------------------------
 __device__ __forceinline__ void cam_point(const double* M, double x, double y, double z, double* r) {
   double rx = M[0] * x + M[1] * y + M[2] * z + M[3];
@@ -1046,10 +1089,19 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   const V rd{d4.x, d4.y, d4.z};
   const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd);
   const DevMaterial m = S.mats[S.objs[h.obj].material];
-  const bool specular = beta4.w != 0.0f;
+  // beta.w: 1 after a specular bounce (and for the camera ray), else minus the BSDF density of the
+  // direction that led here (0 without multiple importance sampling)
+  const bool specular = beta4.w > 0.0f;
   V emitted{0.f, 0.f, 0.f};
   if (specular && m.kind == kMatDiffuseLight) {
     emitted = {m.rgb[0] * beta4.x, m.rgb[1] * beta4.y, m.rgb[2] * beta4.z};
+    L4.x = L4.x + emitted.x;
+    L4.y = L4.y + emitted.y;
+    L4.z = L4.z + emitted.z;
+  } else if (p.mis && m.kind == kMatDiffuseLight) {
+    // the emission a BSDF sample met, weighted against next-event estimation's density
+    const float w = mis_power(-beta4.w, light_pdf(S, (uint32_t)h.obj, rd, h.t));
+    emitted = {(m.rgb[0] * beta4.x) * w, (m.rgb[1] * beta4.y) * w, (m.rgb[2] * beta4.z) * w};
     L4.x = L4.x + emitted.x;
     L4.y = L4.y + emitted.y;
     L4.z = L4.z + emitted.z;
@@ -1096,8 +1148,12 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
       const float c = fabsf(vdot(ls.wi, is.n));
       const V f{f0.x * c, f0.y * c, f0.z * c};
       const float den = pl * ls.pdf;
-      const V contrib{((ls.li.x * f.x) * beta4.x) / den, ((ls.li.y * f.y) * beta4.y) / den,
-                      ((ls.li.z * f.z) * beta4.z) / den};
+      V contrib{((ls.li.x * f.x) * beta4.x) / den, ((ls.li.y * f.y) * beta4.y) / den,
+                ((ls.li.z * f.z) * beta4.z) / den};
+      if (p.mis && Ld.kind != kLightPoint) {
+        const float w = mis_power(den, bsdf_pdf(m, is, vmul(rd, -1.0f), ls.wi));
+        contrib = {contrib.x * w, contrib.y * w, contrib.z * w};
+      }
       out->shadow = true;
       out->sh_o = make_float4(ls.so.x, ls.so.y, ls.so.z, 1e-5f);
       out->sh_d = make_float4(ls.sd.x, ls.sd.y, ls.sd.z, ls.dist);
@@ -1113,7 +1169,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   // a grazing specular event can give 1/|cos| = inf against cos = 0 in FP32: such a path carries
   // nothing further (the reference, in f64, would carry a NaN); end it
   if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) return;
-  beta4.w = sc.specular ? 1.0f : 0.0f;
+  beta4.w = sc.specular ? 1.0f : (p.mis ? -sc.pdf : 0.0f);
   b.beta[j] = beta4;
   b.ray_o[j] = make_float4(sc.o.x, sc.o.y, sc.o.z, near);
   b.ray_d[j] = make_float4(sc.d.x, sc.d.y, sc.d.z, far);

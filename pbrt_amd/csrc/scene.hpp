@@ -112,7 +112,8 @@ struct RenderParams {
   uint32_t nrc, query_vertex, train_stride, train_offset;
   // a batch of jobs (n_jobs > 0): camera, seed, spp and depth come from jobs[info.w] of each path
   const JobDesc* jobs;
-  uint32_t n_jobs, pass_paths, batch_pixels, pad;
+  uint32_t n_jobs, pass_paths, batch_pixels;
+  uint32_t mis;  // multiple importance sampling (power heuristic) instead of the specular guard
 };
 
 // Per-vertex record of a training path, field-major: rec[(f * max_depth + m - 1) * train_cap + rank]

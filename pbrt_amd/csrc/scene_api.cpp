@@ -753,6 +753,8 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     }
     p.near = (float)c->near;
     p.far = (float)c->far;
+    if (c->mis > 1) throw Error(PBRT_ERR_INVALID_ARG, "mis must be 0 or 1");
+    p.mis = c->mis;
 
     // radiance cache buffers
     NrcTrainBuffers tb{};
@@ -856,6 +858,10 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       o.sh_c = (float4*)alloc(n * sizeof(float4));
       o.capacity = (uint32_t)n;
       if (ncfg.train_stride) o.rec = (float4*)alloc((size_t)kRecFields * c->max_depth * ranks_cap * sizeof(float4));
+      // each set queues its own cache queries: the back half of one pass answers them while the next
+      // pass's front (the other set) queues new ones
+      o.q_raw = (float4*)alloc(n * 3 * sizeof(float4));
+      o.q_beta = (float4*)alloc(n * sizeof(float4));
       PBRT_HIP(hipMemsetAsync(o.counts, 0, kCounterWords * sizeof(uint32_t), st));
       spill_aux = (uint2*)alloc((size_t)spill_per * grid * kSceneBlock * sizeof(uint2));
       PBRT_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
@@ -1103,6 +1109,9 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
     RenderParams p{};
     p.jobs = cur;
     p.n_jobs = n;
+    for (uint32_t k = 0; k < n; ++k)
+      if (cfg[k].mis != cfg[0].mis || cfg[k].mis > 1) throw Error(PBRT_ERR_INVALID_ARG, "mis must be 0 or 1, the same for every job");
+    p.mis = cfg[0].mis;
     p.batch_pixels = (uint32_t)pix;
     const SceneView v = view_of(s);
     const uint32_t grid = persistent_grid(s, (uint32_t)capacity);

@@ -227,3 +227,31 @@ def test_batch_scene_random_rays_bit_exact(gpu_ctx):
     orc = po.Scene(desc, precision=32, **pbrt_amd.DEVICE_BVH)
     compare_hits(scene.trace_host(rays), orc.trace(rays))
     np.testing.assert_array_equal(scene.trace_host(rays, any_hit=True).astype(bool), orc.trace(rays, any_hit=True)["obj"] >= 0)
+
+
+@pytest.mark.parametrize("which", ["room", "glossy_floor"])
+def test_mis_render_matches_oracle(gpu_ctx, which):
+    """Multiple importance sampling on the device against the FP32 oracle's MIS at equal spp under
+    the same sampler (the bars of test_render_matches_oracle), and the default estimator unchanged
+    by the option's code path."""
+    from test_oracle_mis import glossy_floor
+    if which == "room":
+        desc, cam = scenes.room(subdiv=4, ggx=True)
+        depth = cam["max_depth"]
+    else:
+        desc, cam = glossy_floor()
+        depth = 4
+    w = h = 64
+    spp = 8
+    scene = pbrt_amd.Scene(gpu_ctx, desc)
+    film, _ = scene.render(w, h, spp, depth, cam["eye"], cam["look"], fov=cam["fov"], seed=3, mis=True)
+    orc = po.Scene(desc, precision=32, **pbrt_amd.DEVICE_BVH)
+    f32, _ = orc.render(w, h, cam["fov"], cam["eye"], cam["look"], spp, depth, 3, mis=True)
+    scale = float(f32.mean())
+    assert scale > 0 and np.isfinite(film).all()
+    assert rmse(film, f32) / scale < 0.02, rmse(film, f32) / scale
+    assert abs(film.mean() - f32.mean()) / scale < 0.01
+    plain, _ = scene.render(w, h, spp, depth, cam["eye"], cam["look"], fov=cam["fov"], seed=3)
+    p32, _ = orc.render(w, h, cam["fov"], cam["eye"], cam["look"], spp, depth, 3, mis=False)
+    assert rmse(plain, p32) / scale < 0.02
+    assert not np.array_equal(plain, film)
