@@ -225,6 +225,33 @@ def jobs_measure(pa, ctx):
     }
 
 
+def render_measure(pa, ctx):
+    """Path-tracing throughput of the wavefront renderer on the two image configurations: the small
+    scene (5,120-triangle icosphere, floor, 64 spheres, point light; 256^2, 16 spp, depth 8) and the
+    room (walls, emissive ceiling panel as an area light, two boxes, an 81,920-triangle icosphere;
+    512^2, 64 spp, depth 8), each with the reference's estimator and with MIS. GPU time from the
+    render's own progress events (first launch to the last pass)."""
+    from pbrt_amd import scenes
+    out = {}
+    for name, (desc, cam), (w, h, spp) in [("small_256x256_16spp", scenes.small(), (256, 256, 16)),
+                                             ("room_512x512_64spp", scenes.room(), (512, 512, 64))]:
+        scene = pa.Scene(ctx, desc)
+        scene.render(64, 64, 1, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"])  # warm-up
+        row = {}
+        for mis in (False, True):
+            _, st, pr = scene.render_ex(w, h, spp, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=1,
+                                        mis=mis)
+            ms = float(pr["ms"][-1])
+            row["mis" if mis else "reference_estimator"] = {
+                "ms": ms, "mrays_s": st["rays"] / ms / 1e3, "samples_s": w * h * spp / ms * 1e3,
+                "rays_per_sample": st["rays"] / (w * h * spp), "passes": st["passes"]}
+        out[name] = row
+        print(f"[bench] render {name}: {row['reference_estimator']['mrays_s']:.0f} Mrays/s, "
+              f"{row['reference_estimator']['samples_s'] / 1e6:.1f} M samples/s (MIS "
+              f"{row['mis']['mrays_s']:.0f} Mrays/s)", file=sys.stderr, flush=True)
+    return out
+
+
 def render_profile(pa, ctx, spp=16, frames_per_pass=16):
     """A short run for kernel traces of the renderer on the radiance-cache workload: `spp` samples
     per pixel of plain path tracing (4 per pass) and `spp` frames of the cache render."""
@@ -260,13 +287,15 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
     ap.add_argument("--no-jobs", action="store_true", help="skip the batched-jobs leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing throughput leg")
+    ap.add_argument("--render-only", action="store_true", help="run only the path-tracing throughput leg")
     ap.add_argument("--jobs-only", action="store_true", help="run only the batched-jobs leg (prints its JSON)")
     ap.add_argument("--render-profile", action="store_true",
                     help="only a short plain + cache render of the cache workload, for kernel traces")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rays shard trivially; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (rays and render jobs would shard without "
+              "exchange; not built)", file=sys.stderr)
         sys.exit(2)
 
     import pbrt_amd as pa
@@ -276,6 +305,9 @@ def main():
         return
     if args.jobs_only:
         print(json.dumps(jobs_measure(pa, pa.Context(0))))
+        return
+    if args.render_only:
+        print(json.dumps(render_measure(pa, pa.Context(0))))
         return
     if args.render_profile:
         print(json.dumps(render_profile(pa, pa.Context(0))))
@@ -407,6 +439,9 @@ def main():
     jobs = None
     if not args.no_jobs:
         jobs = jobs_measure(pa, ctx)
+    render = None
+    if not args.no_render:
+        render = render_measure(pa, ctx)
 
     cpu = None
     if not args.no_cpu:
@@ -454,6 +489,7 @@ def main():
         "copy_gbs": copy_gbs,
         "radiance_cache": nrc,
         "render_jobs": jobs,
+        "render": render,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }
