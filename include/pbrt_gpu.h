@@ -154,6 +154,9 @@ typedef struct pbrt_scene pbrt_scene;
 #define PBRT_MAT_DIELECTRIC 1  /* materials/dielectric.rs */
 #define PBRT_MAT_DIFFUSE_LIGHT 2 /* materials/diffuse_light.rs (rgb = emitted radiance) */
 #define PBRT_MAT_GGX 3         /* glossy microfacet reflector (no counterpart in the reference) */
+#define PBRT_MAT_METAL 4       /* materials/metal.rs: roughness = fuzz */
+#define PBRT_TEX_COLOR 0       /* textures/plain_color.rs: the material's rgb */
+#define PBRT_TEX_CHECKER 1     /* textures/checker_board.rs: rgb, rgb2, scale at the surface (u, v) */
 
 /* Scene-level closest hit: 32 bytes. object = the caller's object index (-1 on a miss); prim = the
  * mesh triangle (-1 for other shapes); t = distance along the ray (local distance for a transformed
@@ -184,6 +187,14 @@ int pbrt_scene_add_csg(pbrt_scene* scene, int op, uint32_t n, const int* shapes,
                        const double* m, const double* minv, uint32_t material, const double* m16,
                        const double* minv16, uint32_t* id);
 int pbrt_scene_add_point_light(pbrt_scene* scene, const double pos[3], const double rgb[3]);
+/* The material's texture (Lambertian / Metal / Dielectric / DiffuseLight take theirs from a texture
+ * in the reference, textures/*.rs): PBRT_TEX_COLOR (the material's rgb) or PBRT_TEX_CHECKER
+ * (CheckerBoard::new(rgb, rgb2, scale)). */
+int pbrt_scene_set_texture(pbrt_scene* scene, uint32_t material, int kind, const double rgb2[3], double scale);
+/* BackgroundInfiniteLight::new(down, up) (lights/background_infinite_light.rs): radiance blended
+ * from `down` (direction y = -1) to `up` (y = +1); seen by rays that leave the scene after a
+ * specular bounce, and sampled by next-event estimation over the sphere of directions. */
+int pbrt_scene_add_background_light(pbrt_scene* scene, const double down[3], const double up[3]);
 /* Area light on a rectangle object (no counterpart in the reference; follows ideas/area_light.md):
  * uniform area sampling, one-sided emission along the rectangle's normal, radiance rgb. */
 int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double rgb[3]);

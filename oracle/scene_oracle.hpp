@@ -25,6 +25,7 @@
 #pragma once
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <vector>
 
@@ -60,18 +61,25 @@ template <>
 inline float uni<float>(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
   return (float)(hash4(seed, path, bounce, slot) >> 40) * 0x1p-24f;
 }
-enum Slot : uint32_t { kJitterX = 0, kJitterY = 1, kLightPick = 2, kLightU = 3, kLightV = 4, kScatter0 = 5, kScatter1 = 6, kCoin = 7 };
+enum Slot : uint32_t {
+  kJitterX = 0, kJitterY = 1, kLightPick = 2, kLightU = 3, kLightV = 4, kScatter0 = 5, kScatter1 = 6, kCoin = 7,
+  kMetal0 = 8  // Metal's point in the unit ball: attempt k uses slots kMetal0 + 3k .. + 2
+};
+constexpr uint32_t kMetalAttempts = 64;
 
 // ---- scene description (shared layout with the ctypes side) -------------------------------------
 enum ShapeKind : int32_t { kMesh = 0, kSphere = 1, kBox = 2, kRect = 3, kCsg = 4 };
 enum CsgOp : int32_t { kCsgUnion = 0, kCsgIntersection = 1, kCsgSubtraction = 2 };
-enum MatKind : int32_t { kLambertian = 0, kDielectric = 1, kDiffuseLight = 2, kGGX = 3 };
-enum LightKind : int32_t { kPointLight = 0, kAreaLight = 1 };
+enum MatKind : int32_t { kLambertian = 0, kDielectric = 1, kDiffuseLight = 2, kGGX = 3, kMetal = 4 };
+enum LightKind : int32_t { kPointLight = 0, kAreaLight = 1, kBackgroundLight = 2 };
+enum TexKind : int32_t { kPlainColor = 0, kCheckerBoard = 1 };
 
 struct MaterialDesc {
-  int32_t kind, pad;
+  int32_t kind, tex;        // tex: kPlainColor (rgb) or kCheckerBoard (rgb, rgb2, scale)
   double rgb[3];
-  double ior, roughness;
+  double ior, roughness;    // roughness: GGX alpha, or Metal's fuzz
+  double rgb2[3];
+  double scale;
 };
 struct ObjectDesc {
   int32_t shape, mesh;      // shape kind; mesh index for kMesh
@@ -90,7 +98,7 @@ struct CsgElemDesc {
 };
 struct LightDesc {
   int32_t kind, object;     // object: the emitting rectangle of an area light
-  double pos[3], rgb[3];
+  double pos[3], rgb[3];    // background light: pos = the colour looking down, rgb = looking up
 };
 
 template <class R>
@@ -673,6 +681,41 @@ class SceneOracle {
   }
   Spec<R> albedo(const MaterialDesc& m) const { return {{(R)m.rgb[0], (R)m.rgb[1], (R)m.rgb[2]}}; }
 
+  // CheckerBoard::shade (textures/checker_board.rs:17-29) at the surface coordinates (u, v):
+  // Rust's f64 % (fmod) and signum (+-1 by the sign bit).
+  static bool checker_first(R u, R v, R scale) {
+    R su = std::fmod(std::fabs(u) * scale, R(1));
+    R sv = std::fmod(std::fabs(v) * scale, R(1));
+    bool use_color = (su < R(0.5) && sv < R(0.5)) || (su >= R(0.5) && sv >= R(0.5));
+    bool positive = std::copysign(R(1), u) * std::copysign(R(1), v) >= R(0);
+    return use_color != positive;
+  }
+  // The material with its texture evaluated at the hit: rgb is the colour there.
+  static MaterialDesc shaded(const MaterialDesc& m, const Isect<R>& is) {
+    MaterialDesc r = m;
+    if (m.tex == kCheckerBoard && !checker_first(is.u, is.v, (R)m.scale))
+      for (int c = 0; c < 3; ++c) r.rgb[c] = m.rgb2[c];
+    return r;
+  }
+  // BackgroundInfiniteLight::le (lights/background_infinite_light.rs:24-28), summed over the
+  // background lights.
+  Spec<R> background_le(V3<R> d) const {
+    Spec<R> sum{{0, 0, 0}};
+    V3<R> u = normalized(d);
+    R factor = R(0.5) * (u.y + R(1));
+    for (const LightDesc& L : lights) {
+      if (L.kind != kBackgroundLight) continue;
+      for (int c = 0; c < 3; ++c) sum.c[c] = sum.c[c] + ((R)L.pos[c] * (R(1) - factor) + (R)L.rgb[c] * factor);
+    }
+    return sum;
+  }
+  uint32_t n_background() const {
+    uint32_t n = 0;
+    for (const LightDesc& L : lights) n += L.kind == kBackgroundLight ? 1u : 0u;
+    return n;
+  }
+  bool has_background() const { return n_background() > 0; }
+
   // Material::f (lambertian.rs:41-52); 0 for the others (materials.rs:32-34).
   Spec<R> mat_f(const MaterialDesc& m, const Isect<R>& is, V3<R> wo_w, V3<R> wi_w) const {
     if (m.kind == kLambertian) {
@@ -734,6 +777,29 @@ class SceneOracle {
       return true;
     }
     if (m.kind == kDielectric) return dielectric(m, is, seed, path, bounce, s);
+    if (m.kind == kMetal) {
+      // metal.rs:22-54: the mirror direction plus fuzz times a point of the unit ball
+      // (utils.rs:19-27, by rejection: attempt k draws three uniforms of its own)
+      V3<R> lwo = normalized(to_local(is, is.wo));
+      V3<R> lrefl{-lwo.x, -lwo.y, lwo.z};
+      V3<R> q{0, 0, 0};
+      for (uint32_t k = 0; k < kMetalAttempts; ++k) {
+        V3<R> c{uni<R>(seed, path, bounce, kMetal0 + 3 * k) * R(2) - R(1),
+                uni<R>(seed, path, bounce, kMetal0 + 3 * k + 1) * R(2) - R(1),
+                uni<R>(seed, path, bounce, kMetal0 + 3 * k + 2) * R(2) - R(1)};
+        if (dot(c, c) < R(1)) {
+          q = c;
+          break;
+        }
+      }
+      V3<R> lt = normalized(lrefl + q * (R)m.roughness);
+      if (!(lt.z > R(0))) return false;
+      s->ray = {is.p + is.n * R(0.001), normalized(to_world(is, lt))};
+      s->att = sdiv(albedo(m), std::fabs(lt.z));
+      s->pdf = R(1);
+      s->specular = true;
+      return true;
+    }
     if (m.kind == kGGX) {
       V3<R> wo = to_local(is, is.wo);
       if (wo.z <= R(0)) return false;
@@ -837,6 +903,22 @@ class SceneOracle {
       s->dist = length(pos - is.p);
       return true;
     }
+    if (L.kind == kBackgroundLight) {
+      // background_infinite_light.rs:31-44: a direction uniform on the sphere (pdfs/sphere.rs:
+      // 14-23), VisibilityTester::towards_infinity (lights.rs:57-62)
+      R phi = R(2) * R(M_PI) * uni<R>(seed, path, bounce, kLightU);
+      R cos_t = R(1) - R(2) * uni<R>(seed, path, bounce, kLightV);
+      R sin_t = std::sqrt(R(1) - cos_t * cos_t);
+      V3<R> wi{std::cos(phi) * sin_t, std::sin(phi) * sin_t, cos_t};
+      R factor = R(0.5) * (normalized(wi).y + R(1));
+      s->li = {{(R)L.pos[0] * (R(1) - factor) + (R)L.rgb[0] * factor, (R)L.pos[1] * (R(1) - factor) + (R)L.rgb[1] * factor,
+                (R)L.pos[2] * (R(1) - factor) + (R)L.rgb[2] * factor}};
+      s->wi = wi;
+      s->pdf = R(1) / (R(4) * R(M_PI));
+      s->shadow = {is.p, normalized(wi)};
+      s->dist = std::numeric_limits<R>::max();
+      return true;
+    }
     const Obj& o = objs[L.object];
     R w = (R)o.d.param[0], h = (R)o.d.param[1];
     R u1 = uni<R>(seed, path, bounce, kLightU), u2 = uni<R>(seed, path, bounce, kLightV);
@@ -916,8 +998,20 @@ class SceneOracle {
     for (;;) {
       SceneHit<R> hit;
       (*rays_cast)++;
-      if (!intersect(ray, near, far, &hit, &st)) break;  // no infinite light in these scenes
-      const MaterialDesc& m = mats[objs[hit.object].d.material];
+      if (!intersect(ray, near, far, &hit, &st)) {
+        // path.rs:101-108: the background's radiance after a specular bounce; with mis also after a
+        // non-specular one, weighted against next-event estimation's density
+        if (has_background()) {
+          if (specular) {
+            L = sadd(L, smul(background_le(ray.d), beta));
+          } else if (mis) {
+            R w = power(prev_pdf, (R(1) / (R)lights.size()) * (R(1) / (R(4) * R(M_PI))));
+            L = sadd(L, sscale(smul(background_le(ray.d), beta), w));
+          }
+        }
+        break;
+      }
+      const MaterialDesc m = shaded(mats[objs[hit.object].d.material], hit.is);
       if (specular && m.kind == kDiffuseLight) L = sadd(L, smul(albedo(m), beta));
       if (mis && !specular && m.kind == kDiffuseLight) {
         R w = power(prev_pdf, light_pdf((uint32_t)hit.object, ray.d, hit.is.d));

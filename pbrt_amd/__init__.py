@@ -305,9 +305,11 @@ class Scene:
         out = ctypes.c_uint32()
         for v, f in desc.meshes:
             call("pbrt_scene_add_mesh", h, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(out))
-        for kind, rgb, ior, rough in desc.materials:
+        for kind, rgb, ior, rough, tex, rgb2, scale in desc.materials:
             c = np.array(rgb, np.float64)
             call("pbrt_scene_add_material", h, kind, _ptr(c), ior, rough, ctypes.byref(out))
+            if tex:
+                call("pbrt_scene_set_texture", h, out.value, tex, _ptr(np.array(rgb2, np.float64)), scale)
         for shape, mesh, mat, xf, param in desc.objects:
             p = np.array(param, np.float64)
             if shape == 4:  # CSG: elements [mesh, mesh + count) of desc.csg
@@ -330,6 +332,8 @@ class Scene:
             c = np.array(rgb, np.float64)
             if kind == 0:
                 call("pbrt_scene_add_point_light", h, _ptr(np.array(pos, np.float64)), _ptr(c))
+            elif kind == 2:
+                call("pbrt_scene_add_background_light", h, _ptr(np.array(pos, np.float64)), _ptr(c))
             else:
                 call("pbrt_scene_add_area_light", h, obj, _ptr(c))
         call("pbrt_scene_commit", h)

@@ -102,6 +102,8 @@ SIGNATURES = {
     "pbrt_scene_status": [P, ctypes.POINTER(I32)],
     "pbrt_render": [P, P, P, P],
     "pbrt_film_to_rgba8": [P, U32, U32, U32, P],
+    "pbrt_scene_set_texture": [P, U32, I32, P, F64],
+    "pbrt_scene_add_background_light": [P, P, P],
     "pbrt_film_write_png": [P, U32, U32, U32, ctypes.c_char_p],
     "pbrt_render_ex": [P, P, P, ctypes.POINTER(NrcRender), ctypes.POINTER(Progress), P, P],
     "pbrt_render_jobs": [P, P, U32, P, I32, P],

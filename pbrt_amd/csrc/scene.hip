@@ -623,11 +623,24 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_trace(SceneView S, const 
 struct Isect {
   V p, n, wo, dpdu;
   float d;
+  float u, v;  // surface coordinates (textures); 0 unless asked for
 };
 
 // The world-space intersection record of a hit (triangle_mesh.rs:73-132, sphere.rs:152-187,
 // aabox.rs:31-60 + 155-190, rectangle.rs:45-54, transform.rs:105-116).
-__device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitRec& h, V wo_ray, V wd) {
+// Sphere surface coordinates of a local hit (sphere.rs:159-166): u = phi / 2 pi, v = theta / pi.
+__device__ __forceinline__ void sphere_uv(V hit, float r, float* u, float* v) {
+  float phi = atan2f(hit.y, hit.x);
+  if (phi < 0.0f) phi += 2.0f * kPi;
+  *u = phi / (2.0f * kPi);
+  float zr = hit.z / r;
+  zr = zr < -1.0f ? -1.0f : (zr > 1.0f ? 1.0f : zr);
+  *v = acosf(zr) / kPi;
+}
+
+// uv: also the surface coordinates (sphere.rs:159-166, the box face table's du / dv, rectangle.rs:
+// 50-51, the default triangle UVs of triangle_mesh.rs:219-224).
+__device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitRec& h, V wo_ray, V wd, bool uv) {
   const DevObject& ob = S.objs[h.obj];
   V o = wo_ray, d = wd;
   if (ob.has_xform) {
@@ -636,6 +649,7 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
   }
   Isect is;
   is.d = h.t;
+  is.u = is.v = 0.0f;
   is.wo = vmul(d, -1.0f);
   const V hit = vadd(o, vmul(d, h.t));
   switch (ob.shape) {
@@ -649,11 +663,16 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
       if (e.shape == kShapeSphere) {
         n = vnorm(lh);
         du = vmul(V{-lh.y, lh.x, 0.0f}, 2.0f * kPi);
+        if (uv) sphere_uv(lh, e.param[0], &is.u, &is.v);
       } else {
         const int face = h.face & 7, axis = face >> 1;
         const float sg = (face & 1) ? 1.0f : -1.0f;
         n = V{axis == 0 ? sg : 0.0f, axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f};
         du = V{axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f, axis == 0 ? sg : 0.0f};
+        if (uv) {
+          is.u = vdot(lh, du);
+          is.v = vdot(lh, V{axis == 2 ? 1.0f : 0.0f, axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f});
+        }
       }
       is.p = xpoint(e.m, lh);
       is.n = xnormal(e.minv, n);
@@ -666,6 +685,7 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
       is.p = hit;
       is.n = vnorm(hit);
       is.dpdu = vmul(V{-hit.y, hit.x, 0.0f}, 2.0f * kPi);
+      if (uv) sphere_uv(hit, ob.param[0], &is.u, &is.v);
       break;
     }
     case kShapeBox: {
@@ -677,6 +697,10 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
       is.p = hit;
       is.n = V{axis == 0 ? sg : 0.0f, axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f};
       is.dpdu = V{axis == 1 ? sg : 0.0f, axis == 2 ? sg : 0.0f, axis == 0 ? sg : 0.0f};
+      if (uv) {  // dv = e_(a+1 mod 3)
+        is.u = vdot(hit, is.dpdu);
+        is.v = vdot(hit, V{axis == 2 ? 1.0f : 0.0f, axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f});
+      }
       break;
     }
     case kShapeRect: {
@@ -685,6 +709,8 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
       is.p = p;
       is.n = V{0.0f, 1.0f, 0.0f};
       is.dpdu = V{1.0f, 0.0f, 0.0f};
+      is.u = p.x;
+      is.v = p.z;
       break;
     }
     default: {
@@ -703,6 +729,11 @@ __device__ __forceinline__ Isect interaction(const SceneView& S, const SceneHitR
       is.p = hit;
       is.n = vnorm(vcross(dpdv, dpdu));
       is.dpdu = dpdu;
+      if (uv) {  // barycentric interpolation of (0,0), (1,0), (1,1)
+        const float w = 1.0f - h.u - h.v;
+        is.u = w * 0.0f + h.u * 1.0f + h.v * 1.0f;
+        is.v = w * 0.0f + h.u * 0.0f + h.v * 1.0f;
+      }
       break;
     }
   }
@@ -731,7 +762,37 @@ __device__ __forceinline__ V to_world(const Isect& i, V v) {
 __device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
   return to_unit_f32(hash4(seed, path, bounce, slot));
 }
-enum : uint32_t { kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7 };
+enum : uint32_t {
+  kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7,
+  kSlotMetal0 = 8  // Metal's point in the unit ball: attempt k uses slots kSlotMetal0 + 3k .. + 2
+};
+constexpr uint32_t kMetalAttempts = 64;
+
+// CheckerBoard::shade (textures/checker_board.rs:17-29) at (u, v): true for the first colour.
+// Rust's f64 % is fmod; signum is +-1 by the sign bit.
+__device__ __forceinline__ bool checker_first(float u, float v, float scale) {
+  const float su = fmodf(fabsf(u) * scale, 1.0f), sv = fmodf(fabsf(v) * scale, 1.0f);
+  const bool use_color = (su < 0.5f && sv < 0.5f) || (su >= 0.5f && sv >= 0.5f);
+  const bool positive = copysignf(1.0f, u) * copysignf(1.0f, v) >= 0.0f;
+  return use_color != positive;
+}
+
+// BackgroundInfiniteLight::le (lights/background_infinite_light.rs:24-28), summed over the
+// background lights.
+__device__ __forceinline__ V background_le(const SceneView& S, V d) {
+  V sum{0.f, 0.f, 0.f};
+  const V u = vnorm(d);
+  const float factor = 0.5f * (u.y + 1.0f);
+#pragma unroll 1
+  for (uint32_t i = 0; i < S.n_lights; ++i) {
+    const DevLight& L = S.lights[i];
+    if (L.kind != kLightBackground) continue;
+    sum = {sum.x + (L.pos[0] * (1.0f - factor) + L.rgb[0] * factor),
+           sum.y + (L.pos[1] * (1.0f - factor) + L.rgb[1] * factor),
+           sum.z + (L.pos[2] * (1.0f - factor) + L.rgb[2] * factor)};
+  }
+  return sum;
+}
 
 // GGX microfacet reflection with Smith masking and Schlick Fresnel (albedo as F0)
 __device__ __forceinline__ float ggx_d(float cos_h, float a) {
@@ -830,6 +891,30 @@ __device__ __forceinline__ bool scatter(const DevMaterial& m, const Isect& is, u
     att = {m.rgb[0] / c, m.rgb[1] / c, m.rgb[2] / c};
     pdf = 1.0f;
     spec = true;
+  } else if (m.kind == kMatMetal) {
+    // metal.rs:22-54: the mirror direction plus fuzz times a point of the unit ball (utils.rs:19-27,
+    // by rejection; attempt k draws three uniforms of its own)
+    const V lwo = vnorm(to_local(is, is.wo));
+    const V lrefl{-lwo.x, -lwo.y, lwo.z};
+    V q{0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (uint32_t k = 0; k < kMetalAttempts; ++k) {
+      const V c{urand(seed, path, bounce, kSlotMetal0 + 3 * k) * 2.0f - 1.0f,
+                urand(seed, path, bounce, kSlotMetal0 + 3 * k + 1) * 2.0f - 1.0f,
+                urand(seed, path, bounce, kSlotMetal0 + 3 * k + 2) * 2.0f - 1.0f};
+      if (vdot(c, c) < 1.0f) {
+        q = c;
+        break;
+      }
+    }
+    const V lt = vnorm(vadd(lrefl, vmul(q, m.roughness)));
+    if (!(lt.z > 0.0f)) return false;
+    org = vadd(is.p, vmul(is.n, 0.001f));
+    dir = vnorm(to_world(is, lt));
+    const float c = fabsf(lt.z);
+    att = {m.rgb[0] / c, m.rgb[1] / c, m.rgb[2] / c};
+    pdf = 1.0f;
+    spec = true;
   } else if (m.kind == kMatGGX) {
     const V wo = to_local(is, is.wo);
     if (wo.z <= 0.0f) return false;
@@ -877,6 +962,23 @@ __device__ __forceinline__ bool sample_li(const SceneView& S, const DevLight& L,
     s->so = is.p;
     s->sd = vnorm(vnorm(vsub(pos, is.p)));
     s->dist = vlen(vsub(pos, is.p));
+    return true;
+  }
+  if (L.kind == kLightBackground) {
+    // background_infinite_light.rs:31-44: a direction uniform on the sphere (pdfs/sphere.rs:14-23),
+    // VisibilityTester::towards_infinity (lights.rs:57-62)
+    const float phi = 2.0f * kPi * urand(seed, path, bounce, kSlotLu);
+    const float cos_t = 1.0f - 2.0f * urand(seed, path, bounce, kSlotLv);
+    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    const V wi{cosf(phi) * sin_t, sinf(phi) * sin_t, cos_t};
+    const float factor = 0.5f * (vnorm(wi).y + 1.0f);
+    s->li = {L.pos[0] * (1.0f - factor) + L.rgb[0] * factor, L.pos[1] * (1.0f - factor) + L.rgb[1] * factor,
+             L.pos[2] * (1.0f - factor) + L.rgb[2] * factor};
+    s->wi = wi;
+    s->pdf = 1.0f / (4.0f * kPi);
+    s->so = is.p;
+    s->sd = vnorm(wi);
+    s->dist = FLT_MAX;
     return true;
   }
   const DevObject& o = S.objs[L.object];
@@ -1069,7 +1171,7 @@ template <bool NRC>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderParams& p, const PathBuffers& b,
                                              uint32_t j, ShadeOut* out) {
   const SceneHitRec h = b.hit[j];
-  if (h.obj < 0) return;  // no infinite light in these scenes: the path ends
+  if (h.obj < 0 && S.n_background == 0) return;  // the path leaves the scene: it ends
   uint4 info = b.info[j];
   // the path's job (a batch of jobs) or the render's own parameters
   uint32_t spp = p.spp, max_depth = p.max_depth;
@@ -1087,11 +1189,39 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   float4 beta4 = b.beta[j], L4 = b.L[j];
   const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
   const V rd{d4.x, d4.y, d4.z};
-  const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd);
-  const DevMaterial m = S.mats[S.objs[h.obj].material];
   // beta.w: 1 after a specular bounce (and for the camera ray), else minus the BSDF density of the
   // direction that led here (0 without multiple importance sampling)
   const bool specular = beta4.w > 0.0f;
+  if (h.obj < 0) {
+    // path.rs:101-108: the background's radiance after a specular bounce; with MIS also after a
+    // non-specular one, weighted against next-event estimation's density
+    V e{0.f, 0.f, 0.f};
+    if (specular || p.mis) {
+      const V le = background_le(S, rd);
+      const float w = specular ? 1.0f : mis_power(-beta4.w, (1.0f / (float)S.n_lights) * (1.0f / (4.0f * kPi)));
+      e = specular ? V{le.x * beta4.x, le.y * beta4.y, le.z * beta4.z}
+                   : V{(le.x * beta4.x) * w, (le.y * beta4.y) * w, (le.z * beta4.z) * w};
+      L4.x = L4.x + e.x;
+      L4.y = L4.y + e.y;
+      L4.z = L4.z + e.z;
+      b.L[j] = L4;
+    }
+    uint32_t rank;
+    if (NRC && info.z + 1 <= max_depth && is_training_path(p, j, &rank)) {
+      // the emission of the vertex the path would have reached, for the training targets
+      const size_t fs = (size_t)p.max_depth * b.train_cap;
+      b.rec[kRecEmit * fs + (size_t)info.z * b.train_cap + rank] = make_float4(e.x, e.y, e.z, 0.f);
+    }
+    return;
+  }
+  const DevMaterial m0 = S.mats[S.objs[h.obj].material];
+  const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, rd, m0.tex == kTexChecker);
+  DevMaterial m = m0;  // with its texture evaluated at the hit
+  if (m0.tex == kTexChecker && !checker_first(is.u, is.v, m0.scale)) {
+    m.rgb[0] = m0.rgb2[0];
+    m.rgb[1] = m0.rgb2[1];
+    m.rgb[2] = m0.rgb2[2];
+  }
   V emitted{0.f, 0.f, 0.f};
   if (specular && m.kind == kMatDiffuseLight) {
     emitted = {m.rgb[0] * beta4.x, m.rgb[1] * beta4.y, m.rgb[2] * beta4.z};

@@ -17,8 +17,9 @@ struct alignas(16) DevCsgElem {
   float m[12], minv[12];
 };
 static_assert(sizeof(DevCsgElem) == 128, "CSG element is 128 bytes");
-enum MatKind : int32_t { kMatLambertian = 0, kMatDielectric = 1, kMatDiffuseLight = 2, kMatGGX = 3 };
-enum LightKind : int32_t { kLightPoint = 0, kLightArea = 1 };
+enum MatKind : int32_t { kMatLambertian = 0, kMatDielectric = 1, kMatDiffuseLight = 2, kMatGGX = 3, kMatMetal = 4 };
+enum LightKind : int32_t { kLightPoint = 0, kLightArea = 1, kLightBackground = 2 };
+enum TexKind : int32_t { kTexColor = 0, kTexChecker = 1 };
 
 // One scene object (a shape, a material and an optional rigid transform), in the leaf order of
 // the scene BVH. Matrices are the first three rows of the 4x4 local->world and world->local
@@ -33,14 +34,17 @@ struct alignas(16) DevObject {
 };
 
 struct alignas(16) DevMaterial {
-  int32_t kind, pad;
+  int32_t kind, tex;     // tex: kTexColor (rgb) or kTexChecker (rgb, rgb2, scale)
   float rgb[3];
-  float ior, roughness, pad2;
+  float ior, roughness;  // roughness: GGX alpha, or Metal's fuzz
+  float scale;
+  float rgb2[3];
+  float pad;
 };
 
 struct alignas(16) DevLight {
   int32_t kind, object;  // object: leaf-order index of an area light's rectangle
-  float pos[3], rgb[3];
+  float pos[3], rgb[3];  // background light: pos = the colour looking down, rgb = looking up
 };
 
 // Device view of a committed scene.
@@ -57,6 +61,7 @@ struct SceneView {
   const DevCsgElem* csg;
   uint32_t n_objects, n_lights;
   int has_tlas;
+  uint32_t n_background;  // background lights among the lights
 };
 
 // Closest hit at scene level: object (caller's index), primitive (mesh triangle or -1), distance

@@ -119,6 +119,7 @@ SceneView view_of(const pbrt_scene* s) {
   v.n_objects = (uint32_t)s->objects.size();
   v.n_lights = (uint32_t)s->lights.size();
   v.has_tlas = s->has_tlas ? 1 : 0;
+  for (const auto& l : s->lights) v.n_background += l.kind == kLightBackground ? 1u : 0u;
   return v;
 }
 
@@ -356,7 +357,7 @@ int pbrt_scene_add_mesh(pbrt_scene* s, const float* xyz, uint32_t nv, const uint
 int pbrt_scene_add_material(pbrt_scene* s, int kind, const double rgb[3], double ior, double roughness, uint32_t* id) {
   return guarded([&] {
     check_scene(s, false);
-    if (kind < kMatLambertian || kind > kMatGGX) throw Error(PBRT_ERR_INVALID_ARG, "unknown material kind");
+    if (kind < kMatLambertian || kind > kMatMetal) throw Error(PBRT_ERR_INVALID_ARG, "unknown material kind");
     DevMaterial m{};
     m.kind = kind;
     for (int c = 0; c < 3; ++c) m.rgb[c] = (float)rgb[c];
@@ -364,6 +365,19 @@ int pbrt_scene_add_material(pbrt_scene* s, int kind, const double rgb[3], double
     m.roughness = (float)roughness;
     s->materials.push_back(m);
     if (id) *id = (uint32_t)s->materials.size() - 1;
+  });
+}
+
+int pbrt_scene_set_texture(pbrt_scene* s, uint32_t material, int kind, const double rgb2[3], double scale) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (material >= s->materials.size()) throw Error(PBRT_ERR_INVALID_ARG, "no such material");
+    if (kind != kTexColor && kind != kTexChecker) throw Error(PBRT_ERR_INVALID_ARG, "unknown texture kind");
+    if (kind == kTexChecker && !rgb2) throw Error(PBRT_ERR_INVALID_ARG, "a checkerboard needs its second colour");
+    DevMaterial& m = s->materials[material];
+    m.tex = kind;
+    m.scale = (float)scale;
+    for (int c = 0; c < 3; ++c) m.rgb2[c] = kind == kTexChecker ? (float)rgb2[c] : 0.0f;
   });
 }
 
@@ -440,6 +454,15 @@ int pbrt_scene_add_point_light(pbrt_scene* s, const double pos[3], const double 
   return guarded([&] {
     check_scene(s, false);
     pbrt_scene::HL l{kLightPoint, -1, {pos[0], pos[1], pos[2]}, {rgb[0], rgb[1], rgb[2]}};
+    s->lights.push_back(l);
+  });
+}
+
+int pbrt_scene_add_background_light(pbrt_scene* s, const double down[3], const double up[3]) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (!down || !up) throw Error(PBRT_ERR_INVALID_ARG, "null colour");
+    pbrt_scene::HL l{kLightBackground, -1, {down[0], down[1], down[2]}, {up[0], up[1], up[2]}};
     s->lights.push_back(l);
   });
 }

@@ -16,12 +16,14 @@ import numpy as np
 
 SHAPE = {"mesh": 0, "sphere": 1, "aabox": 2, "box": 2, "rectangle": 3, "rect": 3, "csg": 4}
 CSG_OP = {"union": 0, "intersection": 1, "subtraction": 2, "substraction": 2}
-MATERIAL = {"lambertian": 0, "dielectric": 1, "diffuse_light": 2, "ggx": 3}
-LIGHT = {"point": 0, "area": 1}
+MATERIAL = {"lambertian": 0, "dielectric": 1, "diffuse_light": 2, "ggx": 3, "metal": 4}
+LIGHT = {"point": 0, "area": 1, "background": 2}
+TEXTURE = {"color": 0, "checkerboard": 1}
 
 OBJECT_DTYPE = np.dtype([("shape", "<i4"), ("mesh", "<i4"), ("material", "<i4"), ("has_xform", "<i4"),
                          ("param", "<f8", 3), ("m", "<f8", 16), ("minv", "<f8", 16)])
-MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("pad", "<i4"), ("rgb", "<f8", 3), ("ior", "<f8"), ("roughness", "<f8")])
+MATERIAL_DTYPE = np.dtype([("kind", "<i4"), ("tex", "<i4"), ("rgb", "<f8", 3), ("ior", "<f8"), ("roughness", "<f8"),
+                           ("rgb2", "<f8", 3), ("scale", "<f8")])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("object", "<i4"), ("pos", "<f8", 3), ("rgb", "<f8", 3)])
 CSG_ELEM_DTYPE = np.dtype([("shape", "<i4"), ("pad", "<i4"), ("param", "<f8", 3), ("m", "<f8", 16),
                            ("minv", "<f8", 16)])
@@ -119,8 +121,16 @@ class SceneDesc:
                             np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)))
         return len(self.meshes) - 1
 
-    def add_material(self, kind, rgb, ior=1.5, roughness=0.0):
-        self.materials.append((MATERIAL[kind], tuple(float(c) for c in rgb), float(ior), float(roughness)))
+    def add_material(self, kind, rgb, ior=1.5, roughness=0.0, fuzz=None, checker=None):
+        """`rgb`: the colour (texture `color`), or the first colour of a checkerboard given as
+        checker=(rgb2, scale) (textures/checker_board.rs); `fuzz`: Metal's (materials/metal.rs),
+        stored in the roughness slot."""
+        if fuzz is not None:
+            roughness = fuzz
+        tex, rgb2, scale = (0, (0.0, 0.0, 0.0), 0.0) if checker is None else (1, tuple(map(float, checker[0])),
+                                                                                float(checker[1]))
+        self.materials.append((MATERIAL[kind], tuple(float(c) for c in rgb), float(ior), float(roughness), tex, rgb2,
+                               scale))
         return len(self.materials) - 1
 
     def add_object(self, shape, material, param=(0.0, 0.0, 0.0), mesh=-1, transform=None):
@@ -150,6 +160,11 @@ class SceneDesc:
         assert self.objects[obj][0] == SHAPE["rectangle"]
         self.lights.append((LIGHT["area"], int(obj), (0.0, 0.0, 0.0), tuple(map(float, rgb))))
 
+    def add_background_light(self, down=(1.0, 1.0, 1.0), up=(0.5, 0.7, 1.0)):
+        """BackgroundInfiniteLight (lights/background_infinite_light.rs): radiance blended from
+        `down` (direction y = -1) to `up` (y = +1)."""
+        self.lights.append((LIGHT["background"], -1, tuple(map(float, down)), tuple(map(float, up))))
+
     # -- packed arrays (the layout the native code reads) --------------------------------------
     def object_array(self):
         a = np.zeros(len(self.objects), OBJECT_DTYPE)
@@ -175,8 +190,9 @@ class SceneDesc:
 
     def material_array(self):
         a = np.zeros(len(self.materials), MATERIAL_DTYPE)
-        for i, (kind, rgb, ior, rough) in enumerate(self.materials):
+        for i, (kind, rgb, ior, rough, tex, rgb2, scale) in enumerate(self.materials):
             a[i]["kind"], a[i]["rgb"], a[i]["ior"], a[i]["roughness"] = kind, rgb, ior, rough
+            a[i]["tex"], a[i]["rgb2"], a[i]["scale"] = tex, rgb2, scale
         return a
 
     def light_array(self):

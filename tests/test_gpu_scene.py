@@ -255,3 +255,59 @@ def test_mis_render_matches_oracle(gpu_ctx, which):
     p32, _ = orc.render(w, h, cam["fov"], cam["eye"], cam["look"], spp, depth, 3, mis=False)
     assert rmse(plain, p32) / scale < 0.02
     assert not np.array_equal(plain, film)
+
+
+def spheres_on_checkerboard():
+    """default.stage's content built by hand: a fuzzy metal, a glass and a Lambertian sphere on a
+    checkerboard floor, an emissive panel above, the loader's point light and background light."""
+    d = SceneDesc()
+    floor = d.add_material("lambertian", (0.65, 0.0, 0.0), checker=((0.65, 0.65, 0.65), 2.0))
+    metal = d.add_material("metal", (0.0, 0.6470588235294118, 1.0), fuzz=0.5)
+    glass = d.add_material("dielectric", (0.0, 0.6470588235294118, 1.0), ior=1.5)
+    lamb = d.add_material("lambertian", (0.0, 0.6470588235294118, 1.0))
+    light = d.add_material("diffuse_light", (5.0, 5.0, 5.0))
+    d.add_object("sphere", metal, param=(0.4,), transform=Transform.translation(-1.0, 0.0, 0.0))
+    d.add_object("sphere", glass, param=(0.4,), transform=Transform.translation(0.0, 0.0, 0.0))
+    d.add_object("sphere", lamb, param=(0.4,), transform=Transform.translation(1.0, 0.0, 0.0))
+    d.add_object("rectangle", floor, (3.0, 3.0), transform=Transform.translation(0.0, -0.6, 0.0))
+    d.add_object("rectangle", light, (3.0, 3.0),
+                 transform=Transform.translation(0.0, 3.0, 0.0) * Transform.rotation("y", 3.1416))
+    d.add_point_light((0.0, 2.0, 1.0), (15.0, 15.0, 15.0))
+    d.add_background_light((1.0, 1.0, 1.0), (0.5, 0.7, 1.0))
+    return d, dict(eye=(-1.5, 0.0, 1.5), look=(0.0, 0.0, 0.0), fov=math.pi / 2)
+
+
+@pytest.mark.parametrize("mis", [False, True])
+def test_metal_checkerboard_background_match_oracle(gpu_ctx, mis):
+    desc, cam = spheres_on_checkerboard()
+    w = h = 64
+    spp, depth = 8, 8
+    scene = pbrt_amd.Scene(gpu_ctx, desc)
+    film, _ = scene.render(w, h, spp, depth, cam["eye"], cam["look"], fov=cam["fov"], seed=3, mis=mis)
+    f32, _ = po.Scene(desc, precision=32, **pbrt_amd.DEVICE_BVH).render(w, h, cam["fov"], cam["eye"], cam["look"],
+                                                                      spp, depth, 3, mis=mis)
+    scale = float(f32.mean())
+    assert scale > 0 and np.isfinite(film).all()
+    assert rmse(film, f32) / scale < 0.02, rmse(film, f32) / scale
+    assert abs(film.mean() - f32.mean()) / scale < 0.01
+
+
+def test_device_checkerboard_and_background_closed_forms(gpu_ctx):
+    """The oracle's closed forms (tests/test_oracle_materials.py) on the device."""
+    import test_oracle_materials as tm
+    # checkerboard emitter: interior pixels exactly the texel colour
+    c1, c2, sc = (0.9, 0.2, 0.1), (0.1, 0.3, 0.8), 0.8
+    d = SceneDesc()
+    d.add_object("rectangle", d.add_material("diffuse_light", c1, checker=(c2, sc)), (8.0, 8.0))
+    film, _ = pbrt_amd.Scene(gpu_ctx, d).render(48, 48, 4, 2, tm.EYE, tm.LOOK, fov=tm.FOV, seed=1)
+    ref, _ = po.Scene(d, precision=32).render(48, 48, tm.FOV, tm.EYE, tm.LOOK, 4, 2, 1)
+    assert np.mean(np.all(film == ref, axis=2)) > 0.97  # texel edges aside, the same texel
+    # a Lambertian floor under a uniform background: radiance = albedo
+    albedo = np.array([0.2, 0.5, 0.8])
+    d = SceneDesc()
+    d.add_object("rectangle", d.add_material("lambertian", albedo), (50.0, 50.0))
+    d.add_background_light((1.0, 1.0, 1.0), (1.0, 1.0, 1.0))
+    scene = pbrt_amd.Scene(gpu_ctx, d)
+    for mis in (False, True):
+        film, _ = scene.render(16, 16, 1024, 2, tm.EYE, tm.LOOK, fov=tm.FOV, seed=3, mis=mis)
+        np.testing.assert_allclose(film.reshape(-1, 3).mean(0), albedo, rtol=0.02)
