@@ -247,6 +247,26 @@ int pbrt_render(pbrt_scene* scene, const pbrt_render_config* config, double* fil
 int pbrt_film_to_rgba8(const double* film, uint32_t width, uint32_t height, uint32_t tonemap, uint8_t* rgba_out);
 int pbrt_film_write_png(const double* film, uint32_t width, uint32_t height, uint32_t tonemap, const char* path);
 
+/* ---- scene files (host) ----
+ * pbrt_ply_read: an ASCII PLY mesh as the reference loads it (load_ply_mesh, loader/mesh_loader.rs;
+ * loader/ply/ply_loader.rs:251-255 reads ASCII only): the x, y, z vertex properties as FP32
+ * (correctly rounded), triangle faces only, each face's indices reversed when `reverse` != 0.
+ * Call with xyz_out or idx_out NULL to get the counts in *nv / *nt; then with arrays of at least
+ * 3 * *nv floats and 3 * *nt indices (the counts in *nv / *nt are their capacities). */
+int pbrt_ply_read(const char* path, int reverse, float* xyz_out, uint32_t* nv, uint32_t* idx_out, uint32_t* nt);
+/* pbrt_scene_load_stage: Loader::load_scene (src/loader.rs:29-46) — a `.stage` file parsed
+ * (loader/parser.rs, loader/parser/lexer.rs) and built as SceneBuilderVisitor does
+ * (loader/visitors/scene_builder.rs): objects in reverse file order, transform steps applied in
+ * listed order, mesh files read relative to base_dir (NULL: the working directory, as the
+ * reference), then a point light of intensity 15 at (0, 2, 1) and BackgroundInfiniteLight(white,
+ * (0.5, 0.7, 1.0)). Returns a committed scene; view_out (may be NULL) receives the pin-hole camera
+ * (eye, look, up) and the renderer's defaults fov 90 degrees, near 1e-4, far 1000, other fields
+ * untouched. A compound object becomes its children; cylinders, planes and the thin-lens camera
+ * (no device counterpart) are refused with PBRT_ERR_INVALID_ARG, as is any syntax error (the
+ * message names the line). */
+int pbrt_scene_load_stage(pbrt_ctx* ctx, const char* path, const char* base_dir, const pbrt_bvh_params* mesh_params,
+                          pbrt_scene** scene_out, pbrt_render_config* view_out);
+
 /* A batch of n render jobs (1 to 16) on one scene traced together (no counterpart in the
  * reference, which renders one image per run): every pass gives each unfinished job a share of
  * about 4M paths (at least one sample per pixel), and the jobs' paths share the wavefront's

@@ -338,6 +338,21 @@ class Scene:
                 call("pbrt_scene_add_area_light", h, obj, _ptr(c))
         call("pbrt_scene_commit", h)
 
+    @classmethod
+    def load_stage(cls, ctx, path, base_dir=None, bins=12, max_leaf=4, t_trav=1.0):
+        """Loader::load_scene (src/loader.rs:29-46) through pbrt_scene_load_stage: the `.stage`
+        file's scene on the device, mesh files read relative to `base_dir` (None: the working
+        directory). Returns (scene, view), view = dict(eye, look, up, fov, near, far) for render."""
+        self = cls.__new__(cls)
+        self.ctx, self.desc, self.h = ctx, None, None
+        h = ctypes.c_void_p()
+        cfg = _native.RenderConfig()
+        call("pbrt_scene_load_stage", ctx.h, os.fsencode(path), os.fsencode(base_dir) if base_dir else None,
+             ctypes.byref(BvhParams(bins, max_leaf, t_trav)), ctypes.byref(h), ctypes.byref(cfg))
+        self.h = h
+        view = dict(eye=tuple(cfg.eye), look=tuple(cfg.look), up=tuple(cfg.up), fov=cfg.fov, near=cfg.near, far=cfg.far)
+        return self, view
+
     def __del__(self):
         try:
             if self.h:
@@ -601,6 +616,17 @@ def write_png(film, path, tonemap=False):
     f = np.ascontiguousarray(film, np.float64)
     h, w = f.shape[:2]
     call("pbrt_film_write_png", _ptr(f), w, h, int(bool(tonemap)), os.fsencode(path))
+
+
+def read_ply(path, reverse=False):
+    """load_ply_mesh (src/loader/mesh_loader.rs) through pbrt_ply_read: (vertices float32 (nv, 3),
+    faces uint32 (nt, 3)) of an ASCII PLY file, each face reversed when `reverse`."""
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    call("pbrt_ply_read", os.fsencode(path), int(bool(reverse)), None, ctypes.byref(nv), None, ctypes.byref(nt))
+    v = np.empty((nv.value, 3), np.float32)
+    f = np.empty((nt.value, 3), np.uint32)
+    call("pbrt_ply_read", os.fsencode(path), int(bool(reverse)), _ptr(v), ctypes.byref(nv), _ptr(f), ctypes.byref(nt))
+    return v, f
 
 
 def to_rgb8(film):

@@ -105,6 +105,9 @@ SIGNATURES = {
     "pbrt_scene_set_texture": [P, U32, I32, P, F64],
     "pbrt_scene_add_background_light": [P, P, P],
     "pbrt_film_write_png": [P, U32, U32, U32, ctypes.c_char_p],
+    "pbrt_ply_read": [ctypes.c_char_p, I32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32)],
+    "pbrt_scene_load_stage": [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(BvhParams), PP,
+                              ctypes.POINTER(RenderConfig)],
     "pbrt_render_ex": [P, P, P, ctypes.POINTER(NrcRender), ctypes.POINTER(Progress), P, P],
     "pbrt_render_jobs": [P, P, U32, P, I32, P],
     "pbrt_nrc_create": [P, U64, PP],
