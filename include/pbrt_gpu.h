@@ -100,6 +100,16 @@ int pbrt_event_destroy(void* ev);
  * ctx == NULL the mesh is built and flattened on the host only (for inspection; tracing it fails). */
 int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
                      const pbrt_bvh_params* params, pbrt_mesh** out);
+/* The same mesh with its BVH built on the device (BVHTree::build, bvh.rs:240-309, 609-735, as a
+ * level-synchronous GPU build): the tree, build statistics and device arrays are those
+ * pbrt_mesh_create gives, byte for byte. Needs a context. Beyond pbrt_mesh_create's errors, a node
+ * of more than 1024 triangles sharing one centroid (no plane separates them) is refused with
+ * PBRT_ERR_INVALID_ARG; pbrt_mesh_export_tree is not available on such a mesh. */
+int pbrt_mesh_create_device(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                            const pbrt_bvh_params* params, pbrt_mesh** out);
+/* Host copies of the device arrays (node_bytes / tri_bytes of pbrt_mesh_device_bytes; NULL to
+ * skip), the root box (min xyz, max xyz) and the root reference (>= 0 node, < 0 leaf). */
+int pbrt_mesh_copy_layout(const pbrt_mesh* mesh, void* nodes_out, void* tris_out, float* root_box6, int32_t* root_ref);
 int pbrt_mesh_destroy(pbrt_mesh* mesh);
 /* BVHTree::build_stats (triangle_mesh/bvh.rs:539-546); triangle_count (bvh.rs:529-531). */
 int pbrt_mesh_build_stats(const pbrt_mesh* mesh, pbrt_build_stats* out);

@@ -153,13 +153,15 @@ class TriangleMesh:
     vertices: (nv, 3) float32; indices: (nt, 3) uint32. An empty mesh raises PbrtError
     (PBRT_ERR_EMPTY_MESH), as the reference's constructor panics (triangle_mesh.rs:28)."""
 
-    def __init__(self, ctx, vertices, indices, bins=12, max_leaf=4, t_trav=1.0):
+    def __init__(self, ctx, vertices, indices, bins=12, max_leaf=4, t_trav=1.0, device_build=False):
+        """device_build: build the BVH on the GPU (pbrt_mesh_create_device), the same tree."""
         self.ctx = ctx
         v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         f = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         params = BvhParams(bins, max_leaf, t_trav)
         h = ctypes.c_void_p()
-        call("pbrt_mesh_create", ctx.h if ctx is not None else None, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(params), ctypes.byref(h))
+        fn = "pbrt_mesh_create_device" if device_build else "pbrt_mesh_create"
+        call(fn, ctx.h if ctx is not None else None, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(params), ctypes.byref(h))
         self.h = h
         self.params = dict(bins=bins, max_leaf=max_leaf, t_trav=t_trav)
 
@@ -185,6 +187,17 @@ class TriangleMesh:
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         call("pbrt_mesh_device_bytes", self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def layout(self):
+        """Host copies of the device arrays: (nodes (n, 64) uint8, triangles (nt, 48) uint8, root box
+        (6,) float32, root reference)."""
+        nb, tb = self.device_bytes()
+        nodes = np.empty((nb // 64, 64), np.uint8)
+        tris = np.empty((tb // 48, 48), np.uint8)
+        box = np.empty(6, np.float32)
+        ref = ctypes.c_int32()
+        call("pbrt_mesh_copy_layout", self.h, _ptr(nodes), _ptr(tris), _ptr(box), ctypes.byref(ref))
+        return nodes, tris, box, ref.value
 
     def set_launch(self, block=0, waves_per_cu=0, lds_stack=0, refill=0):
         call("pbrt_mesh_set_launch", self.h, ctypes.byref(Launch(block, waves_per_cu, lds_stack, refill)))

@@ -75,6 +75,8 @@ SIGNATURES = {
     "pbrt_event_elapsed_ms": [P, P, ctypes.POINTER(F32)],
     "pbrt_event_destroy": [P],
     "pbrt_mesh_create": [P, P, U32, P, U32, ctypes.POINTER(BvhParams), PP],
+    "pbrt_mesh_create_device": [P, P, U32, P, U32, ctypes.POINTER(BvhParams), PP],
+    "pbrt_mesh_copy_layout": [P, P, P, P, ctypes.POINTER(I32)],
     "pbrt_mesh_destroy": [P],
     "pbrt_mesh_build_stats": [P, ctypes.POINTER(BuildStats)],
     "pbrt_mesh_triangle_count": [P, ctypes.POINTER(U32)],

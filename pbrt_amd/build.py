@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpbrt_gpu.so")
 
-SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "scene_api.cpp", "nrc_api.cpp", "film.cpp", "stage_loader.cpp", "traverse.hip", "scene.hip",
+SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "scene_api.cpp", "nrc_api.cpp", "film.cpp", "stage_loader.cpp", "traverse.hip", "bvh_gpu.hip", "scene.hip",
            "nrc.hip", "nrc_render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

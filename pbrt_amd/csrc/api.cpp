@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 
+#include "bvh_gpu.hpp"
 #include "common.hpp"
 #include "context.hpp"
 #include "traverse.hpp"
@@ -251,6 +252,64 @@ int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_
   });
 }
 
+int pbrt_mesh_create_device(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                            const pbrt_bvh_params* params, pbrt_mesh** out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!out) throw Error(PBRT_ERR_INVALID_ARG, "null output");
+    if (nt == 0) throw Error(PBRT_ERR_EMPTY_MESH, "a TriangleMesh needs at least one triangle");
+    if (!xyz || !idx) throw Error(PBRT_ERR_INVALID_ARG, "null vertex or index array");
+    pbrt_bvh_params p = params ? *params : pbrt_bvh_params{12, 4, 1.0};
+    if (p.max_leaf > kMaxLeafTris) throw Error(PBRT_ERR_INVALID_ARG, "max_leaf above the device bound of 32");
+    auto m = std::make_unique<pbrt_mesh>();
+    m->ctx = ctx;
+    m->nt = nt;
+    m->idx.assign(idx, idx + 3 * (size_t)nt);
+    float* xyz_d = nullptr;
+    uint32_t* idx_d = nullptr;
+    GpuBuilt g;
+    try {
+      PBRT_HIP(hipMalloc(&xyz_d, std::max<size_t>(1, 3 * (size_t)nv) * sizeof(float)));
+      PBRT_HIP(hipMalloc(&idx_d, 3 * (size_t)nt * sizeof(uint32_t)));
+      PBRT_HIP(hipMemcpyAsync(xyz_d, xyz, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      PBRT_HIP(hipMemcpyAsync(idx_d, idx, 3 * (size_t)nt * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+      gpu_build_mesh_bvh(ctx->stream, xyz_d, nv, idx_d, nt, p, &g);
+    } catch (...) {
+      (void)hipFree(xyz_d);
+      (void)hipFree(idx_d);
+      throw;
+    }
+    (void)hipFree(xyz_d);
+    (void)hipFree(idx_d);
+    m->nodes = g.nodes;
+    m->tris = g.tris;
+    m->n_nodes = g.n_nodes;
+    m->root = g.root;
+    m->stats = g.stats;
+    PBRT_HIP(hipMalloc(&m->status, sizeof(int)));
+    PBRT_HIP(hipMemset(m->status, 0, sizeof(int)));
+    *out = m.release();
+  });
+}
+
+int pbrt_mesh_copy_layout(const pbrt_mesh* m, void* nodes_out, void* tris_out, float* root_box6, int32_t* root_ref) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "a host-only mesh has no device layout");
+    PBRT_HIP(hipSetDevice(m->ctx->device));
+    PBRT_HIP(hipStreamSynchronize(m->ctx->stream));
+    if (nodes_out && m->n_nodes)
+      PBRT_HIP(hipMemcpy(nodes_out, m->nodes, m->n_nodes * sizeof(DevNode), hipMemcpyDeviceToHost));
+    if (tris_out) PBRT_HIP(hipMemcpy(tris_out, m->tris, (size_t)m->nt * sizeof(DevTri), hipMemcpyDeviceToHost));
+    if (root_box6)
+      for (int k = 0; k < 3; ++k) {
+        root_box6[k] = m->root.lo[k];
+        root_box6[3 + k] = m->root.hi[k];
+      }
+    if (root_ref) *root_ref = m->root.ref;
+  });
+}
+
 int pbrt_mesh_destroy(pbrt_mesh* m) {
   return guarded([&] {
     if (!m) return;
@@ -290,6 +349,7 @@ int pbrt_mesh_export_tree(const pbrt_mesh* m, double* boxes, uint8_t* is_leaf, u
   return guarded([&] {
     if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
     const auto& b = m->host;
+    if (b.nodes.empty()) throw Error(PBRT_ERR_INVALID_ARG, "the tree was built on the device: no host tree to export");
     for (size_t i = 0; i < b.nodes.size(); ++i) {
       const BuildNode& n = b.nodes[i];
       for (int k = 0; k < 3; ++k) {

@@ -8,6 +8,11 @@
 // planes are scanned axis by axis, boundary by boundary, keeping the first strictly cheaper one
 // (bvh.rs:658-681). So this builder — which keeps precomputed per-triangle boxes and partitions
 // with std::partition — produces the same tree as the reference's in-place swap partition.
+//
+// Two choices make the flattened layout independent of the partition order as well, so that the
+// device builder (bvh_gpu.hip) reproduces it byte for byte: a zero box bound is stored as +0
+// (fmin / fmax of -0 and +0 depend on argument order; the two compare equal, so no decision
+// changes), and each leaf's triangles are sorted by index.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -100,6 +105,7 @@ class Builder {
       mid = first;  // leaf
     }
     if (mid == first || mid == first + count) {
+      std::sort(order.begin() + first, order.begin() + first + count);
       BuildNode& n = out_->nodes[me];
       n.leaf = true;
       n.first = first;
@@ -202,6 +208,8 @@ BuiltBVH build_mesh_bvh(const float* xyz, uint32_t nv, const uint32_t* idx, uint
     for (int k = 0; k < 3; ++k) {
       t.lo[k] = std::fmin(std::fmin(v0[k], v1[k]), v2[k]);
       t.hi[k] = std::fmax(std::fmax(v0[k], v1[k]), v2[k]);
+      if (t.lo[k] == 0.0f) t.lo[k] = 0.0f;
+      if (t.hi[k] == 0.0f) t.hi[k] = 0.0f;
       double s = (double)v0[k];
       s = s + (double)v1[k];
       s = s + (double)v2[k];
