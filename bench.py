@@ -192,6 +192,11 @@ def jobs_measure(pa, ctx):
     t0 = time.time()
     scene = pa.Scene(ctx, desc)
     build_s = time.time() - t0
+    # the same scene with its 256 meshes built in one batched device build
+    pa.TriangleMesh(ctx, *pa.icosphere(2), device_build=True)  # loads the build kernels
+    t0 = time.time()
+    scene_dev = pa.Scene(ctx, desc, device_build=True)
+    dev_build_s = time.time() - t0
     jobs = scenes.batch_jobs()
     e0, e1 = ctx.event(), ctx.event()
 
@@ -212,13 +217,21 @@ def jobs_measure(pa, ctx):
         seq_ms += ms
         seq_rays += sj["rays"]
     paths = st["paths"]
+    j0 = jobs[0]
+    f_host, _ = scene.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
+    f_dev, _ = scene_dev.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
+    builds = {"scene_s": build_s, "mesh_bvh_s": scene.build_seconds(), "device_scene_s": dev_build_s,
+              "device_mesh_bvh_s": scene_dev.build_seconds(), "same_film": bool(np.array_equal(f_host, f_dev))}
+    del scene_dev
+    print(f"[bench] jobs scene mesh BVHs: host (16 threads) {builds['mesh_bvh_s'] * 1e3:.0f} ms, device "
+          f"{builds['device_mesh_bvh_s'] * 1e3:.1f} ms, same film {builds['same_film']}", file=sys.stderr, flush=True)
     print(f"[bench] jobs: batch {batch_ms:.1f} ms ({st['rays'] / batch_ms / 1e3:.0f} Mrays/s), eager batch "
           f"{eager_ms:.1f} ms, one after another {seq_ms:.1f} ms", file=sys.stderr, flush=True)
     return {
         "workload": f"{len(desc.meshes)} mesh instances ({sum(len(f) for _, f in desc.meshes)} triangles), "
                     f"{sum(1 for o in desc.objects if o[0] == 4)} CSG objects; 8 jobs "
                     + ", ".join(f"{j['width']}x{j['height']}x{j['spp']}" for j in jobs) + ", max depth 8",
-        "scene_gen_s": gen_s, "scene_build_s": build_s,
+        "scene_gen_s": gen_s, "scene_build_s": build_s, "scene_builds": builds,
         "batch_ms": batch_ms, "batch_mrays_s": st["rays"] / batch_ms / 1e3, "batch_samples_s": paths / batch_ms * 1e3,
         "batch_no_graphs_ms": eager_ms, "sequential_ms": seq_ms, "sequential_mrays_s": seq_rays / seq_ms / 1e3,
         "speedup_vs_sequential": seq_ms / batch_ms, "passes": st["passes"], "rays": st["rays"], "paths": paths,
@@ -249,6 +262,28 @@ def render_measure(pa, ctx):
         print(f"[bench] render {name}: {row['reference_estimator']['mrays_s']:.0f} Mrays/s, "
               f"{row['reference_estimator']['samples_s'] / 1e6:.1f} M samples/s (MIS "
               f"{row['mis']['mrays_s']:.0f} Mrays/s)", file=sys.stderr, flush=True)
+    return out
+
+
+def build_measure(pa, ctx, v, f, host_mesh, host_s, runs=5):
+    """The mesh BVH built on the device (pbrt_mesh_create_device) against the host build of the same
+    mesh: wall time of each call (vertex and index upload included in both), and whether the device
+    arrays equal the host's byte for byte."""
+    pa.TriangleMesh(ctx, v, f, device_build=True, **pa.DEVICE_BVH)  # warm-up (code objects, allocator)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        dm = pa.TriangleMesh(ctx, v, f, device_build=True, **pa.DEVICE_BVH)
+        times.append(time.perf_counter() - t0)
+    hn, ht, hb, hr = host_mesh.layout()
+    dn, dt, db, dr = dm.layout()
+    same = bool(hr == dr and np.array_equal(hb, db) and np.array_equal(hn, dn) and np.array_equal(ht, dt))
+    dev_ms = float(np.median(times)) * 1e3
+    out = {"triangles": len(f), "host_ms": round(host_s * 1e3, 1), "device_ms": round(dev_ms, 2),
+           "speedup": round(host_s * 1e3 / dev_ms, 1), "device_mtris_s": round(len(f) / dev_ms / 1e3, 1),
+           "identical_arrays": same}
+    print(f"[bench] BVH build: host {out['host_ms']} ms, device {out['device_ms']} ms ({out['speedup']}x), "
+          f"identical arrays: {same}", file=sys.stderr, flush=True)
     return out
 
 
@@ -289,6 +324,8 @@ def main():
     ap.add_argument("--no-jobs", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing throughput leg")
     ap.add_argument("--render-only", action="store_true", help="run only the path-tracing throughput leg")
+    ap.add_argument("--no-build", action="store_true", help="skip the device BVH build leg")
+    ap.add_argument("--build-only", action="store_true", help="run only the device BVH build leg (prints its JSON)")
     ap.add_argument("--jobs-only", action="store_true", help="run only the batched-jobs leg (prints its JSON)")
     ap.add_argument("--render-profile", action="store_true",
                     help="only a short plain + cache render of the cache workload, for kernel traces")
@@ -328,6 +365,9 @@ def main():
     bs = mesh.build_stats()
     node_bytes, tri_bytes = mesh.device_bytes()
     print(f"[bench] BVH built in {build_s:.1f}s: {bs}", file=sys.stderr, flush=True)
+    if args.build_only:
+        print(json.dumps(build_measure(pa, ctx, v, f, mesh, build_s)))
+        return
 
     batches = {"coherent": coherent, "incoherent": incoherent}
     all_rays = ctx.upload(np.concatenate([coherent, incoherent]))
@@ -443,6 +483,10 @@ def main():
     if not args.no_render:
         render = render_measure(pa, ctx)
 
+    build = None
+    if not args.no_build:
+        build = build_measure(pa, ctx, v, f, mesh, build_s)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -490,6 +534,7 @@ def main():
         "radiance_cache": nrc,
         "render_jobs": jobs,
         "render": render,
+        "bvh_build": build,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
     }

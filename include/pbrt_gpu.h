@@ -103,7 +103,7 @@ int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_
 /* The same mesh with its BVH built on the device (BVHTree::build, bvh.rs:240-309, 609-735, as a
  * level-synchronous GPU build): the tree, build statistics and device arrays are those
  * pbrt_mesh_create gives, byte for byte. Needs a context. Beyond pbrt_mesh_create's errors, a node
- * of more than 1024 triangles sharing one centroid (no plane separates them) is refused with
+ * of more than 256 triangles sharing one centroid (no plane separates them) is refused with
  * PBRT_ERR_INVALID_ARG; pbrt_mesh_export_tree is not available on such a mesh. */
 int pbrt_mesh_create_device(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
                             const pbrt_bvh_params* params, pbrt_mesh** out);
@@ -208,6 +208,14 @@ int pbrt_scene_add_background_light(pbrt_scene* scene, const double down[3], con
 /* Area light on a rectangle object (no counterpart in the reference; follows ideas/area_light.md):
  * uniform area sampling, one-sided emission along the rectangle's normal, radiance rgb. */
 int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double rgb[3]);
+/* Where pbrt_scene_commit builds the meshes' BVHs: PBRT_BUILD_HOST (default; the host's cores, one
+ * mesh per thread) or PBRT_BUILD_DEVICE (every mesh in one batched device build, the arrays of
+ * pbrt_mesh_create_device: the same trees, so the same hits). pbrt_scene_build_seconds: wall time
+ * of that step in the last commit. */
+#define PBRT_BUILD_HOST 0
+#define PBRT_BUILD_DEVICE 1
+int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
+int pbrt_scene_build_seconds(pbrt_scene* scene, double* seconds);
 int pbrt_scene_commit(pbrt_scene* scene);
 int pbrt_scene_info(pbrt_scene* scene, uint32_t* objects, uint32_t* tlas_nodes, uint32_t* tlas_depth,
                     uint32_t* mesh_depth);

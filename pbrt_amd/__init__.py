@@ -309,12 +309,15 @@ class Scene:
     `pbrt_amd.scene.SceneDesc`; `intersect` / `intersect_p` are batched (scene.rs:118-132) and
     `render` is renderers::mt::render (src/renderers/mt.rs:23) with the path integrator."""
 
-    def __init__(self, ctx, desc, bins=12, max_leaf=4, t_trav=1.0):
+    def __init__(self, ctx, desc, bins=12, max_leaf=4, t_trav=1.0, device_build=False):
+        """device_build: the meshes' BVHs built on the device at commit (pbrt_scene_set_mesh_build)."""
         self.ctx = ctx
         self.desc = desc
         h = ctypes.c_void_p()
         call("pbrt_scene_create", ctx.h, ctypes.byref(BvhParams(bins, max_leaf, t_trav)), ctypes.byref(h))
         self.h = h
+        if device_build:
+            call("pbrt_scene_set_mesh_build", h, 1)
         out = ctypes.c_uint32()
         for v, f in desc.meshes:
             call("pbrt_scene_add_mesh", h, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(out))
@@ -373,6 +376,12 @@ class Scene:
                 self.h = None
         except Exception:
             pass
+
+    def build_seconds(self):
+        """Wall time of the meshes' BVH builds in the commit."""
+        t = ctypes.c_double()
+        call("pbrt_scene_build_seconds", self.h, ctypes.byref(t))
+        return t.value
 
     def info(self):
         a, b, c, d = (ctypes.c_uint32() for _ in range(4))

@@ -98,6 +98,8 @@ SIGNATURES = {
     "pbrt_scene_add_point_light": [P, P, P],
     "pbrt_scene_add_area_light": [P, U32, P],
     "pbrt_scene_commit": [P],
+    "pbrt_scene_set_mesh_build": [P, I32],
+    "pbrt_scene_build_seconds": [P, ctypes.POINTER(F64)],
     "pbrt_scene_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)],
     "pbrt_scene_trace_closest": [P, P, U32, P, P],
     "pbrt_scene_trace_any": [P, P, U32, P, P],

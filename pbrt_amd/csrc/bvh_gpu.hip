@@ -31,7 +31,7 @@
 namespace pbrt {
 namespace {
 
-constexpr uint32_t kSubtreeMax = 1024;
+constexpr uint32_t kSubtreeMax = 256;
 constexpr uint32_t kChunk = 4096;
 constexpr int kTopBlock = 256;
 constexpr int kStackMax = 24;
@@ -41,7 +41,7 @@ enum : int32_t { kLeaf = 0, kSplit = 1, kHalve = 2 };
 
 struct Task {
   uint32_t f, c, al, depth;  // segment of `order`, ancestors holding it on their left, depth (root 1)
-  int32_t parent;            // slot of the parent (-1: the root)
+  int32_t parent;            // slot of the parent (-1 - m: the root of mesh m)
   uint32_t side;             // 0: left child, 1: right child
 };
 struct Chunk {
@@ -77,6 +77,21 @@ struct Params {
   uint32_t K, max_leaf;
   double t_trav;
 };
+struct MeshDev {  // a mesh of the batch: its triangles at [tri_start, tri_start + nt) of the build order
+  uint32_t tri_start, nt, vert_base, nv;
+};
+
+__device__ inline uint32_t mesh_of(const MeshDev* M, uint32_t nm, uint32_t pos) {
+  uint32_t lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (M[mid].tri_start <= pos)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
 
 // ---- order-preserving keys (no NaN inputs; -0 orders below +0) ------------------------------------
 __device__ inline uint32_t fkey(float f) {
@@ -182,7 +197,7 @@ __device__ inline void union_bins(const uint32_t* B, uint32_t b0, uint32_t b1, f
 
 __device__ inline void link(BuildSlot* slots, int32_t* root_ref, int32_t parent, uint32_t side, int32_t ref) {
   if (parent < 0)
-    *root_ref = ref;
+    root_ref[-1 - parent] = ref;
   else
     slots[parent].ref[side] = ref;
 }
@@ -228,15 +243,19 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
 __device__ inline uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
 
 // ---- kernels ----------------------------------------------------------------------------------------
-__global__ void k_prep(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt, TrisW T, uint32_t* order,
-                       int* err) {
+__global__ void k_prep(const float* xyz, const uint32_t* idx, uint32_t nt, const MeshDev* M, uint32_t nm, TrisW T,
+                       uint32_t* order, int* err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
+  const MeshDev md = M[mesh_of(M, nm, i)];
   uint32_t a = idx[3 * (size_t)i], b = idx[3 * (size_t)i + 1], c = idx[3 * (size_t)i + 2];
-  if (a >= nv || b >= nv || c >= nv) {
+  if (a >= md.nv || b >= md.nv || c >= md.nv) {
     atomicOr(err, kErrIndex);
     a = b = c = 0;
   }
+  a += md.vert_base;
+  b += md.vert_base;
+  c += md.vert_base;
   for (int k = 0; k < 3; ++k) {
     const float v0 = xyz[3 * (size_t)a + k], v1 = xyz[3 * (size_t)b + k], v2 = xyz[3 * (size_t)c + k];
     float lo = fminf(fminf(v0, v1), v2), hi = fmaxf(fmaxf(v0, v1), v2);
@@ -501,17 +520,38 @@ __global__ __launch_bounds__(64) void k_subtree(const Task* tasks, uint32_t* ord
   while (sp > 0) {
     const Task t = s_stack[--sp];
     __syncthreads();
-    // node and centroid bounds
+    // node and centroid bounds; a node of at most 64 triangles keeps one per lane in registers
+    const bool small = t.c <= 64;
+    uint32_t su = 0;
+    float slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double sc[3] = {0.0, 0.0, 0.0};
+    if (small && (uint32_t)lane < t.c) {
+      su = s_ordA[t.f + lane];
+      for (int a = 0; a < 3; ++a) {
+        slo[a] = T.lo[a][su];
+        shi[a] = T.hi[a][su];
+        sc[a] = T.c[a][su];
+      }
+    }
     float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
     unsigned long long clo[3] = {kDKeyPosInf, kDKeyPosInf, kDKeyPosInf}, chi[3] = {kDKeyNegInf, kDKeyNegInf, kDKeyNegInf};
-    for (uint32_t i = lane; i < t.c; i += 64) {
-      const uint32_t u = s_ordA[t.f + i];
-      for (int a = 0; a < 3; ++a) {
-        blo[a] = fminf(blo[a], T.lo[a][u]);
-        bhi[a] = fmaxf(bhi[a], T.hi[a][u]);
-        const unsigned long long k = dkey(T.c[a][u]);
-        clo[a] = min(clo[a], k);
-        chi[a] = max(chi[a], k);
+    if (small) {
+      if ((uint32_t)lane < t.c)
+        for (int a = 0; a < 3; ++a) {
+          blo[a] = slo[a];
+          bhi[a] = shi[a];
+          clo[a] = chi[a] = dkey(sc[a]);
+        }
+    } else {
+      for (uint32_t i = lane; i < t.c; i += 64) {
+        const uint32_t u = s_ordA[t.f + i];
+        for (int a = 0; a < 3; ++a) {
+          blo[a] = fminf(blo[a], T.lo[a][u]);
+          bhi[a] = fmaxf(bhi[a], T.hi[a][u]);
+          const unsigned long long k = dkey(T.c[a][u]);
+          clo[a] = min(clo[a], k);
+          chi[a] = max(chi[a], k);
+        }
       }
     }
     double lo[3], hi[3], cmin[3], w[3];
@@ -526,43 +566,127 @@ __global__ __launch_bounds__(64) void k_subtree(const Task* tasks, uint32_t* ord
       w[a] = (cmax - cmin[a]) / (double)K;
       any = any || act[a];
     }
-    // bins and the best plane
+    // bins and the best plane: its axis, boundary, left count and the two child boxes
     int axis = -1;
-    uint32_t bnd = 0;
+    uint32_t bnd = 0, lc = 0;
     double best = 0.0;
+    float lbox[6], rbox[6];
     if (any) {
       for (uint32_t i = lane; i < 3 * K * 7; i += 64) s_bins[i] = bin_init(i % 7);
       __syncthreads();
-      for (uint32_t i = lane; i < t.c; i += 64) {
-        const uint32_t u = s_ordA[t.f + i];
-        uint32_t klo[3], khi[3];
-        for (int a = 0; a < 3; ++a) {
-          klo[a] = fkey(T.lo[a][u]);
-          khi[a] = fkey(T.hi[a][u]);
-        }
+      auto bin_one = [&](float tl0, float tl1, float tl2, float th0, float th1, float th2, const double* tc) {
+        const uint32_t klo[3] = {fkey(tl0), fkey(tl1), fkey(tl2)}, khi[3] = {fkey(th0), fkey(th1), fkey(th2)};
         for (int a = 0; a < 3; ++a) {
           if (!act[a]) continue;
-          uint32_t* B = s_bins + (a * K + bin_of(T.c[a][u], cmin[a], w[a], K)) * 7;
+          uint32_t* B = s_bins + (a * K + bin_of(tc[a], cmin[a], w[a], K)) * 7;
           for (int k = 0; k < 3; ++k) {
             atomicMin(&B[k], klo[k]);
             atomicMax(&B[3 + k], khi[k]);
           }
           atomicAdd(&B[6], 1u);
         }
+      };
+      if (small) {
+        if ((uint32_t)lane < t.c) bin_one(slo[0], slo[1], slo[2], shi[0], shi[1], shi[2], sc);
+      } else {
+        for (uint32_t i = lane; i < t.c; i += 64) {
+          const uint32_t u = s_ordA[t.f + i];
+          const double tc[3] = {T.c[0][u], T.c[1][u], T.c[2][u]};
+          bin_one(T.lo[0][u], T.lo[1][u], T.lo[2][u], T.hi[0][u], T.hi[1][u], T.hi[2][u], tc);
+        }
       }
       __syncthreads();
-      double cost = 0.0;
-      uint32_t bb = 0;
-      bool found = false;
-      if (lane < 3 && act[lane]) found = axis_best(s_bins + lane * K * 7, K, s_ra + lane * K, s_rc + lane * K, &cost, &bb);
-      for (int a = 0; a < 3; ++a) {  // the axes in order, each replacing only a strictly higher cost
-        const bool fa = __shfl((int)found, a) != 0;
-        const double ca = __shfl(cost, a);
-        const uint32_t ba = (uint32_t)__shfl((int)bb, a);
-        if (fa && (axis < 0 || ca < best)) {
-          axis = a;
-          bnd = ba;
-          best = ca;
+      if (3 * K <= 64) {
+        // one lane per (axis, bin): prefix and suffix unions by shuffles, then the first cheapest
+        // (axis, boundary) in the host's scan order
+        const uint32_t L = (uint32_t)lane, a = L / K, k = L % K;
+        const bool in = L < 3 * K;
+        uint32_t plo[3], phi[3], pc = 0, qlo[3], qhi[3], qc = 0;
+        for (int e = 0; e < 3; ++e) {
+          plo[e] = qlo[e] = in ? s_bins[L * 7 + e] : kKeyPosInf;
+          phi[e] = qhi[e] = in ? s_bins[L * 7 + 3 + e] : kKeyNegInf;
+        }
+        if (in) pc = qc = s_bins[L * 7 + 6];
+        for (uint32_t d = 1; d < K; d <<= 1) {
+          const int from_lo = (int)(L >= d ? L - d : L), from_hi = (int)(L + d < 64 ? L + d : L);
+          const bool take_p = in && k >= d, take_q = in && k + d < K;
+          for (int e = 0; e < 3; ++e) {
+            const uint32_t a0 = (uint32_t)__shfl((int)plo[e], from_lo), a1 = (uint32_t)__shfl((int)phi[e], from_lo);
+            const uint32_t b0 = (uint32_t)__shfl((int)qlo[e], from_hi), b1 = (uint32_t)__shfl((int)qhi[e], from_hi);
+            if (take_p) {
+              plo[e] = min(plo[e], a0);
+              phi[e] = max(phi[e], a1);
+            }
+            if (take_q) {
+              qlo[e] = min(qlo[e], b0);
+              qhi[e] = max(qhi[e], b1);
+            }
+          }
+          const uint32_t ac = (uint32_t)__shfl((int)pc, from_lo), bc = (uint32_t)__shfl((int)qc, from_hi);
+          if (take_p) pc += ac;
+          if (take_q) qc += bc;
+        }
+        // right of boundary k: the suffix from bin k + 1
+        const int nxt = (int)(L + 1 < 64 ? L + 1 : L);
+        uint32_t rlo[3], rhi[3];
+        for (int e = 0; e < 3; ++e) {
+          rlo[e] = (uint32_t)__shfl((int)qlo[e], nxt);
+          rhi[e] = (uint32_t)__shfl((int)qhi[e], nxt);
+        }
+        const uint32_t rcount = (uint32_t)__shfl((int)qc, nxt);
+        const bool axis_on = a == 0 ? act[0] : a == 1 ? act[1] : a == 2 ? act[2] : false;
+        bool valid = in && axis_on && k + 1 < K && pc > 0 && rcount > 0;
+        double cost = 0.0;
+        if (valid) {
+          const double pl[3] = {kfloat(plo[0]), kfloat(plo[1]), kfloat(plo[2])};
+          const double ph[3] = {kfloat(phi[0]), kfloat(phi[1]), kfloat(phi[2])};
+          const double ql[3] = {kfloat(rlo[0]), kfloat(rlo[1]), kfloat(rlo[2])};
+          const double qh[3] = {kfloat(rhi[0]), kfloat(rhi[1]), kfloat(rhi[2])};
+          cost = half_area(pl, ph) * (double)pc + half_area(ql, qh) * (double)rcount;
+        }
+        uint32_t who = valid ? L : 64u;
+        for (int o = 32; o > 0; o >>= 1) {
+          const double oc = __shfl_xor(cost, o);
+          const uint32_t ow = (uint32_t)__shfl_xor((int)who, o);
+          const bool ov = ow < 64u;
+          if (ov && (!valid || oc < cost || (oc == cost && ow < who))) {
+            cost = oc;
+            who = ow;
+            valid = true;
+          }
+        }
+        if (who < 64u) {
+          axis = (int)(who / K);
+          bnd = who % K;
+          best = cost;
+          lc = (uint32_t)__shfl((int)pc, (int)who);
+          for (int e = 0; e < 3; ++e) {
+            lbox[e] = kfloat((uint32_t)__shfl((int)plo[e], (int)who));
+            lbox[3 + e] = kfloat((uint32_t)__shfl((int)phi[e], (int)who));
+            rbox[e] = kfloat((uint32_t)__shfl((int)rlo[e], (int)who));
+            rbox[3 + e] = kfloat((uint32_t)__shfl((int)rhi[e], (int)who));
+          }
+        }
+      } else {
+        double cost = 0.0;
+        uint32_t bb = 0;
+        bool found = false;
+        if (lane < 3 && act[lane]) found = axis_best(s_bins + lane * K * 7, K, s_ra + lane * K, s_rc + lane * K, &cost, &bb);
+        for (int a = 0; a < 3; ++a) {  // the axes in order, each replacing only a strictly higher cost
+          const bool fa = __shfl((int)found, a) != 0;
+          const double ca = __shfl(cost, a);
+          const uint32_t ba = (uint32_t)__shfl((int)bb, a);
+          if (fa && (axis < 0 || ca < best)) {
+            axis = a;
+            bnd = ba;
+            best = ca;
+          }
+        }
+        if (axis >= 0) {
+          const uint32_t* B = s_bins + axis * K * 7;
+          for (uint32_t k = 0; k <= bnd; ++k) lc += B[k * 7 + 6];
+          union_bins(B, 0, bnd + 1, lbox);
+          union_bins(B, bnd + 1, K, rbox);
         }
       }
     }
@@ -590,20 +714,15 @@ __global__ __launch_bounds__(64) void k_subtree(const Task* tasks, uint32_t* ord
       __syncthreads();
       continue;
     }
-    uint32_t lc;
-    float lbox[6], rbox[6];
     if (mode == kSplit) {
-      const uint32_t* B = s_bins + axis * K * 7;
-      lc = 0;
-      for (uint32_t k = 0; k <= bnd; ++k) lc += B[k * 7 + 6];
-      union_bins(B, 0, bnd + 1, lbox);
-      union_bins(B, bnd + 1, K, rbox);
+      const double ca = cmin[axis], wa = w[axis];
+      const double sca = axis == 0 ? sc[0] : axis == 1 ? sc[1] : sc[2];
       uint32_t lpos = 0, rpos = 0;
       for (uint32_t base = 0; base < t.c; base += 64) {
         const uint32_t i = base + lane;
         const bool valid = i < t.c;
-        const uint32_t u = valid ? s_ordA[t.f + i] : 0u;
-        const bool left = valid && bin_of(T.c[axis][u], cmin[axis], w[axis], K) <= bnd;
+        const uint32_t u = !valid ? 0u : small ? su : s_ordA[t.f + i];
+        const bool left = valid && bin_of(small ? sca : T.c[axis][u], ca, wa, K) <= bnd;
         const uint64_t ml = __ballot(left), mr = __ballot(valid) & ~ml;
         if (left)
           s_ordB[t.f + lpos + (uint32_t)__popcll(ml & lanes_below())] = u;
@@ -763,38 +882,57 @@ __global__ void k_number(const uint32_t* used, uint32_t n, const BuildSlot* slot
   pre[s] = (f ? prefix[f - 1] : 0u) + slots[s].al;
 }
 
-__global__ void k_nodes(const uint32_t* used, uint32_t n, const BuildSlot* slots, const uint32_t* pre, DevNode* nodes) {
+// first interior node of each mesh in the global preorder: the used slots before its triangles
+__global__ void k_mesh_base(const MeshDev* M, uint32_t nm, const uint32_t* prefix, uint32_t* node_base) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  node_base[m] = M[m].tri_start ? prefix[M[m].tri_start - 1] : 0u;
+}
+
+// A mesh's nodes keep references local to the mesh (node and triangle indices from its own
+// first), as the host flattening writes them.
+__device__ inline int32_t local_ref(int32_t r, const uint32_t* pre, uint32_t node_base, uint32_t tri_start) {
+  if (r >= 0) return (int32_t)(pre[r] - node_base);
+  const uint32_t v = ~(uint32_t)r;
+  return dev_leaf_ref((v >> kLeafCountBits) - tri_start, (v & (kMaxLeafTris - 1)) + 1);
+}
+
+__global__ void k_nodes(const uint32_t* used, uint32_t n, const BuildSlot* slots, const uint32_t* pre, const MeshDev* M,
+                        uint32_t nm, const uint32_t* node_base, DevNode* nodes) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n || !used[s]) return;
   const BuildSlot S = slots[s];
+  const uint32_t m = mesh_of(M, nm, S.f);
+  const uint32_t nb = node_base[m], ts = M[m].tri_start;
   DevNode d;
   d.xy0[0] = S.lbox[0]; d.xy0[1] = S.lbox[3]; d.xy0[2] = S.lbox[1]; d.xy0[3] = S.lbox[4];
   d.xy1[0] = S.rbox[0]; d.xy1[1] = S.rbox[3]; d.xy1[2] = S.rbox[1]; d.xy1[3] = S.rbox[4];
   d.z01[0] = S.lbox[2]; d.z01[1] = S.lbox[5]; d.z01[2] = S.rbox[2]; d.z01[3] = S.rbox[5];
-  d.ref[0] = S.ref[0] >= 0 ? (int32_t)pre[S.ref[0]] : S.ref[0];
-  d.ref[1] = S.ref[1] >= 0 ? (int32_t)pre[S.ref[1]] : S.ref[1];
+  d.ref[0] = local_ref(S.ref[0], pre, nb, ts);
+  d.ref[1] = local_ref(S.ref[1], pre, nb, ts);
   d.ref[2] = 0;
   d.ref[3] = 0;
   nodes[pre[s]] = d;
 }
 
-__global__ void k_tris(const uint32_t* order, uint32_t nt, const float* xyz, uint32_t nv, const uint32_t* idx,
-                       DevTri* tris) {
+__global__ void k_tris(const uint32_t* order, uint32_t nt, const float* xyz, const uint32_t* idx, const MeshDev* M,
+                       uint32_t nm, DevTri* tris) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
+  const MeshDev md = M[mesh_of(M, nm, i)];
   const uint32_t t = order[i];
   uint32_t a = idx[3 * (size_t)t], b = idx[3 * (size_t)t + 1], c = idx[3 * (size_t)t + 2];
-  if (a >= nv || b >= nv || c >= nv) a = b = c = 0;  // flagged by k_prep; the build fails
-  const float* p0 = xyz + 3 * (size_t)a;
-  const float* p1 = xyz + 3 * (size_t)b;
-  const float* p2 = xyz + 3 * (size_t)c;
+  if (a >= md.nv || b >= md.nv || c >= md.nv) a = b = c = 0;  // flagged by k_prep; the build fails
+  const float* p0 = xyz + 3 * (size_t)(md.vert_base + a);
+  const float* p1 = xyz + 3 * (size_t)(md.vert_base + b);
+  const float* p2 = xyz + 3 * (size_t)(md.vert_base + c);
   DevTri d;
   for (int k = 0; k < 3; ++k) {
     d.p0[k] = p0[k];
     d.e1[k] = p1[k] - p0[k];
     d.e2[k] = p2[k] - p0[k];
   }
-  d.p0[3] = __uint_as_float(t);
+  d.p0[3] = __uint_as_float(t - md.tri_start);  // the triangle's index within its mesh
   d.e1[3] = 0.f;
   d.e2[3] = 0.f;
   tris[i] = d;
@@ -825,20 +963,30 @@ size_t subtree_lds(uint32_t K) {
 
 }  // namespace
 
-void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
-                        const pbrt_bvh_params& bp, GpuBuilt* out) {
-  if (nt == 0) throw Error(PBRT_ERR_EMPTY_MESH, "a TriangleMesh needs at least one triangle");
+void gpu_build_meshes(hipStream_t st, const float* xyz, const uint32_t* idx, const std::vector<GpuMeshSpan>& spans,
+                      const pbrt_bvh_params& bp, GpuBuiltMany* out) {
+  if (spans.empty()) throw Error(PBRT_ERR_INVALID_ARG, "no mesh to build");
   if (bp.bins < 2 || bp.bins > 256) throw Error(PBRT_ERR_INVALID_ARG, "bins must be in [2, 256]");
-  if (nt - 1 > kMaxLeafFirst) throw Error(PBRT_ERR_LAYOUT, "mesh too large for the 27-bit leaf offset");
-  if (nv == 0) throw Error(PBRT_ERR_INVALID_ARG, "vertex index out of range");
+  const uint32_t nm = (uint32_t)spans.size();
+  std::vector<MeshDev> md(nm);
+  uint64_t total = 0;
+  for (uint32_t m = 0; m < nm; ++m) {
+    if (spans[m].nt == 0) throw Error(PBRT_ERR_EMPTY_MESH, "a TriangleMesh needs at least one triangle");
+    if (spans[m].nv == 0) throw Error(PBRT_ERR_INVALID_ARG, "vertex index out of range");
+    md[m] = MeshDev{(uint32_t)total, spans[m].nt, spans[m].vert_base, spans[m].nv};
+    total += spans[m].nt;
+  }
+  if (total - 1 > kMaxLeafFirst) throw Error(PBRT_ERR_LAYOUT, "meshes too large for the 27-bit leaf offset");
+  const uint32_t nt = (uint32_t)total;
   const Params P{bp.bins, bp.max_leaf, bp.t_trav};
   const uint32_t K = P.K;
   const uint32_t ns = nt - 1;  // slots: split positions 1 .. nt - 1
 
+  DevBuf d_mesh(nm * sizeof(MeshDev)), d_root(nm * sizeof(int32_t)), d_base(nm * sizeof(uint32_t));
   DevBuf tri_f(6 * (size_t)nt * sizeof(float)), tri_d(3 * (size_t)nt * sizeof(double));
   DevBuf order(nt * sizeof(uint32_t)), order2(nt * sizeof(uint32_t));
   DevBuf slots((size_t)std::max(ns, 1u) * sizeof(BuildSlot)), used((size_t)std::max(ns, 1u) * sizeof(uint32_t));
-  DevBuf misc(64);  // err (int), root_ref (int32), stats (5 x u64 from offset 16)
+  DevBuf misc(64);  // err (int), stats (5 x u64 from offset 16)
   TrisW TW;
   Tris T;
   for (int a = 0; a < 3; ++a) {
@@ -849,16 +997,18 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
     T.hi[a] = TW.hi[a];
     T.c[a] = TW.c[a];
   }
+  const MeshDev* M = d_mesh.as<MeshDev>();
   int* err = misc.as<int>();
-  int32_t* root_ref = misc.as<int32_t>() + 1;
+  int32_t* root_ref = d_root.as<int32_t>();
   unsigned long long* stats = (unsigned long long*)(misc.as<char>() + 16);
+  PBRT_HIP(hipMemcpyAsync(d_mesh.p, md.data(), nm * sizeof(MeshDev), hipMemcpyHostToDevice, st));
   PBRT_HIP(hipMemsetAsync(misc.p, 0, 64, st));
   PBRT_HIP(hipMemsetAsync(used.p, 0, (size_t)std::max(ns, 1u) * sizeof(uint32_t), st));
-  k_prep<<<blocks_for(nt, 256), 256, 0, st>>>(xyz, nv, idx, nt, TW, order.as<uint32_t>(), err);
+  k_prep<<<blocks_for(nt, 256), 256, 0, st>>>(xyz, idx, nt, M, nm, TW, order.as<uint32_t>(), err);
   PBRT_HIP(hipGetLastError());
 
-  // level buffers, sized for the widest level: nodes above kSubtreeMax triangles are disjoint
-  const size_t max_nodes = nt / (kSubtreeMax + 1) + 1;
+  // level buffers, sized for the widest level: the roots, or nodes above kSubtreeMax triangles
+  const size_t max_nodes = nt / (kSubtreeMax + 1) + nm + 1;
   const size_t max_chunks = nt / kChunk + max_nodes + 1;
   DevBuf d_tasks(max_nodes * sizeof(Task)), d_nb(max_nodes * sizeof(NodeBounds));
   DevBuf d_bins(max_nodes * 3 * K * 7 * sizeof(uint32_t)), d_dec(max_nodes * sizeof(Decision));
@@ -866,39 +1016,43 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
   DevBuf d_chunks(max_chunks * sizeof(Chunk)), d_cl(max_chunks * sizeof(uint32_t));
   DevBuf d_lb(max_chunks * sizeof(uint32_t)), d_rb(max_chunks * sizeof(uint32_t));
 
-  // the root's box (and the first level's bounds when the root is large)
-  NodeBounds root_nb;
-  std::vector<Task> level{Task{0, nt, 0, 1, -1, 0}}, bottom;
-  uint64_t top_interior = 0;
+  std::vector<Task> level(nm), bottom;
+  for (uint32_t m = 0; m < nm; ++m) level[m] = Task{md[m].tri_start, md[m].nt, 0, 1, -1 - (int32_t)m, 0};
   std::vector<Chunk> chunks;
-  std::vector<Decision> dec;
-  std::vector<uint32_t> cl, lb, rb;
-  bool first = true;
-  while (!level.empty()) {
-    std::vector<Task> large;
-    for (const Task& t : level) (t.c > kSubtreeMax || first ? large : bottom).push_back(t);
-    if (large.empty()) break;
-    const uint32_t nl = (uint32_t)large.size();
+  auto make_chunks = [&](const std::vector<Task>& tasks) {
     chunks.clear();
-    for (uint32_t j = 0; j < nl; ++j)
-      for (uint32_t off = 0; off < large[j].c; off += kChunk)
-        chunks.push_back(Chunk{j, large[j].f + off, std::min(kChunk, large[j].c - off)});
-    const uint32_t nc = (uint32_t)chunks.size();
-    PBRT_HIP(hipMemcpyAsync(d_tasks.p, large.data(), nl * sizeof(Task), hipMemcpyHostToDevice, st));
-    PBRT_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), nc * sizeof(Chunk), hipMemcpyHostToDevice, st));
-    k_level_init<<<blocks_for((size_t)nl * 3 * K * 7, 256), 256, 0, st>>>(nl, K, d_nb.as<NodeBounds>(),
+    for (uint32_t j = 0; j < (uint32_t)tasks.size(); ++j)
+      for (uint32_t off = 0; off < tasks[j].c; off += kChunk)
+        chunks.push_back(Chunk{j, tasks[j].f + off, std::min(kChunk, tasks[j].c - off)});
+    PBRT_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, st));
+    PBRT_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, st));
+    return (uint32_t)chunks.size();
+  };
+
+  // the roots' boxes
+  std::vector<NodeBounds> root_nb(nm);
+  {
+    const uint32_t nc = make_chunks(level);
+    k_level_init<<<blocks_for((size_t)nm * 3 * K * 7, 256), 256, 0, st>>>(nm, K, d_nb.as<NodeBounds>(),
                                                                           d_bins.as<uint32_t>());
     k_top_bounds<<<nc, kTopBlock, 0, st>>>(d_chunks.as<Chunk>(), order.as<uint32_t>(), T, d_nb.as<NodeBounds>());
     PBRT_HIP(hipGetLastError());
-    if (first) {
-      PBRT_HIP(hipMemcpyAsync(&root_nb, d_nb.p, sizeof(NodeBounds), hipMemcpyDeviceToHost, st));
-      first = false;
-      if (nt <= kSubtreeMax) {
-        PBRT_HIP(hipStreamSynchronize(st));
-        bottom.push_back(level[0]);
-        break;
-      }
-    }
+    PBRT_HIP(hipMemcpyAsync(root_nb.data(), d_nb.p, nm * sizeof(NodeBounds), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipStreamSynchronize(st));
+  }
+
+  uint64_t top_interior = 0;
+  std::vector<Decision> dec;
+  std::vector<uint32_t> cl, lb, rb;
+  while (!level.empty()) {
+    std::vector<Task> large;
+    for (const Task& t : level) (t.c > kSubtreeMax ? large : bottom).push_back(t);
+    if (large.empty()) break;
+    const uint32_t nl = (uint32_t)large.size();
+    const uint32_t nc = make_chunks(large);
+    k_level_init<<<blocks_for((size_t)nl * 3 * K * 7, 256), 256, 0, st>>>(nl, K, d_nb.as<NodeBounds>(),
+                                                                          d_bins.as<uint32_t>());
+    k_top_bounds<<<nc, kTopBlock, 0, st>>>(d_chunks.as<Chunk>(), order.as<uint32_t>(), T, d_nb.as<NodeBounds>());
     k_top_bins<<<nc, kTopBlock, 3 * K * 7 * sizeof(uint32_t), st>>>(d_chunks.as<Chunk>(), order.as<uint32_t>(), T,
                                                                      d_nb.as<NodeBounds>(), K, d_bins.as<uint32_t>());
     k_top_decide<<<blocks_for(nl, 64), 64, 0, st>>>(nl, d_tasks.as<Task>(), d_nb.as<NodeBounds>(), d_bins.as<uint32_t>(),
@@ -966,8 +1120,9 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
     PBRT_HIP(hipStreamSynchronize(st));  // d_bottom is released at the end of this scope
   }
 
-  // preorder numbering, then the node and triangle arrays
+  // preorder numbering (all meshes in order, each mesh's nodes contiguous), then the arrays
   uint32_t n_interior = 0;
+  std::vector<uint32_t> node_base(nm, 0);
   DevBuf prefix((size_t)std::max(ns, 1u) * sizeof(uint32_t)), pre((size_t)std::max(ns, 1u) * sizeof(uint32_t));
   if (ns > 0) {
     const uint32_t tiles = blocks_for(ns, kScanTile);
@@ -975,11 +1130,15 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
     k_scan_tiles<<<tiles, kScanBlock, 0, st>>>(used.as<uint32_t>(), ns, tile.as<uint32_t>());
     k_scan_sums<<<1, 1024, 0, st>>>(tile.as<uint32_t>(), tiles);
     k_scan_apply<<<tiles, kScanBlock, 0, st>>>(used.as<uint32_t>(), ns, tile.as<uint32_t>(), prefix.as<uint32_t>());
+    k_mesh_base<<<blocks_for(nm, 256), 256, 0, st>>>(M, nm, prefix.as<uint32_t>(), d_base.as<uint32_t>());
     PBRT_HIP(hipGetLastError());
     PBRT_HIP(hipMemcpyAsync(&n_interior, prefix.as<uint32_t>() + (ns - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipMemcpyAsync(node_base.data(), d_base.p, nm * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     k_number<<<blocks_for(ns, 256), 256, 0, st>>>(used.as<uint32_t>(), ns, slots.as<BuildSlot>(), prefix.as<uint32_t>(),
                                                   pre.as<uint32_t>());
     PBRT_HIP(hipStreamSynchronize(st));
+  } else {
+    PBRT_HIP(hipMemsetAsync(d_base.p, 0, nm * sizeof(uint32_t), st));
   }
   DevNode* nodes = nullptr;
   DevTri* tris = nullptr;
@@ -988,14 +1147,14 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
   try {
     if (ns > 0)
       k_nodes<<<blocks_for(ns, 256), 256, 0, st>>>(used.as<uint32_t>(), ns, slots.as<BuildSlot>(), pre.as<uint32_t>(),
-                                                   nodes);
-    k_tris<<<blocks_for(nt, 256), 256, 0, st>>>(order.as<uint32_t>(), nt, xyz, nv, idx, tris);
+                                                   M, nm, d_base.as<uint32_t>(), nodes);
+    k_tris<<<blocks_for(nt, 256), 256, 0, st>>>(order.as<uint32_t>(), nt, xyz, idx, M, nm, tris);
     PBRT_HIP(hipGetLastError());
     int host_err = 0;
-    int32_t rref = 0;
+    std::vector<int32_t> rref(nm);
     unsigned long long st5[5];
     PBRT_HIP(hipMemcpyAsync(&host_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
-    PBRT_HIP(hipMemcpyAsync(&rref, root_ref, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipMemcpyAsync(rref.data(), root_ref, nm * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PBRT_HIP(hipMemcpyAsync(st5, stats, sizeof(st5), hipMemcpyDeviceToHost, st));
     PBRT_HIP(hipStreamSynchronize(st));
     if (host_err & kErrIndex) throw Error(PBRT_ERR_INVALID_ARG, "vertex index out of range");
@@ -1004,20 +1163,29 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
     out->nodes = nodes;
     out->tris = tris;
     out->n_nodes = n_interior;
-    // the root, an interior node, is first in preorder
-    out->root.ref = rref >= 0 ? 0 : rref;
-    uint32_t blo[3], bhi[3];
-    std::memcpy(blo, root_nb.blo, sizeof(blo));
-    std::memcpy(bhi, root_nb.bhi, sizeof(bhi));
-    for (int a = 0; a < 3; ++a) {
-      auto kf = [](uint32_t k) {
-        const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
-        float f;
-        std::memcpy(&f, &b, 4);
-        return f;
-      };
-      out->root.lo[a] = kf(blo[a]);
-      out->root.hi[a] = kf(bhi[a]);
+    out->meshes.assign(nm, GpuMeshOut{});
+    auto kf = [](uint32_t k) {
+      const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+      float f;
+      std::memcpy(&f, &b, 4);
+      return f;
+    };
+    for (uint32_t m = 0; m < nm; ++m) {
+      GpuMeshOut& o = out->meshes[m];
+      o.node_base = node_base[m];
+      o.n_nodes = (m + 1 < nm ? node_base[m + 1] : n_interior) - node_base[m];
+      o.tri_base = md[m].tri_start;
+      // a mesh's root, when interior, is first in its preorder; a leaf root is local too
+      if (rref[m] >= 0) {
+        o.root.ref = 0;
+      } else {
+        const uint32_t v = ~(uint32_t)rref[m];
+        o.root.ref = leaf_ref((v >> kLeafCountBits) - md[m].tri_start, (v & (kMaxLeafTris - 1)) + 1);
+      }
+      for (int a = 0; a < 3; ++a) {
+        o.root.lo[a] = kf(root_nb[m].blo[a]);
+        o.root.hi[a] = kf(root_nb[m].bhi[a]);
+      }
     }
     out->stats.node_count = top_interior + st5[0];
     out->stats.leaf_count = st5[1];
@@ -1029,6 +1197,18 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
     (void)hipFree(tris);
     throw;
   }
+}
+
+void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                        const pbrt_bvh_params& p, GpuBuilt* out) {
+  if (nt == 0) throw Error(PBRT_ERR_EMPTY_MESH, "a TriangleMesh needs at least one triangle");
+  GpuBuiltMany many;
+  gpu_build_meshes(st, xyz, idx, {GpuMeshSpan{0, nv, nt}}, p, &many);
+  out->nodes = many.nodes;
+  out->tris = many.tris;
+  out->n_nodes = many.n_nodes;
+  out->root = many.meshes[0].root;
+  out->stats = many.stats;
 }
 
 }  // namespace pbrt

@@ -15,6 +15,7 @@
 #include <thread>
 #include <vector>
 
+#include "bvh_gpu.hpp"
 #include "camera.hpp"
 #include "common.hpp"
 #include "context.hpp"
@@ -65,6 +66,8 @@ using namespace pbrt;
 struct pbrt_scene {
   pbrt_ctx* ctx = nullptr;
   pbrt_bvh_params mesh_params{12, 4, 1.0};
+  bool device_build = false;  // mesh BVHs built on the device at commit (one batched build)
+  double build_seconds = 0.0;  // wall time of the mesh BVH builds at commit
   std::vector<std::vector<float>> mesh_xyz;
   std::vector<std::vector<uint32_t>> mesh_idx;
   std::vector<HostObject> objects;
@@ -477,6 +480,23 @@ int pbrt_scene_add_area_light(pbrt_scene* s, uint32_t object, const double rgb[3
   });
 }
 
+int pbrt_scene_set_mesh_build(pbrt_scene* s, int where) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (where != PBRT_BUILD_HOST && where != PBRT_BUILD_DEVICE)
+      throw Error(PBRT_ERR_INVALID_ARG, "mesh build must be PBRT_BUILD_HOST or PBRT_BUILD_DEVICE");
+    s->device_build = where == PBRT_BUILD_DEVICE;
+  });
+}
+
+int pbrt_scene_build_seconds(pbrt_scene* s, double* seconds) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (!seconds) throw Error(PBRT_ERR_INVALID_ARG, "null output");
+    *seconds = s->build_seconds;
+  });
+}
+
 // Scene::commit (scene.rs:101-115): per-mesh BVHs, the object BVH, upload.
 int pbrt_scene_commit(pbrt_scene* s) {
   return guarded([&] {
@@ -493,65 +513,98 @@ int pbrt_scene_commit(pbrt_scene* s) {
       Box64 bounds;
     };
     std::vector<MeshInfo> info;
-    // the meshes' trees are independent: build them on the host's cores, then flatten in order
+    // vertices and indices of every mesh, concatenated; TriangleMesh::get_bounding_box
+    // (triangle_mesh.rs:150-174): every vertex
     const size_t nm = s->mesh_xyz.size();
-    std::vector<BuiltBVH> built(nm);
-    {
-      const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-      std::atomic<size_t> next{0};
-      std::vector<std::string> errors(hw);
-      auto work = [&](unsigned w) {
-        try {
-          for (size_t k = next++; k < nm; k = next++) {
-            const auto& x = s->mesh_xyz[k];
-            const auto& ix = s->mesh_idx[k];
-            built[k] = build_mesh_bvh(x.data(), (uint32_t)(x.size() / 3), ix.data(), (uint32_t)(ix.size() / 3),
-                                      s->mesh_params);
-          }
-        } catch (const std::exception& e) {
-          errors[w] = e.what();
-        }
-      };
-      std::vector<std::thread> pool;
-      for (unsigned w = 1; w < std::min<size_t>(hw, nm); ++w) pool.emplace_back(work, w);
-      work(0);
-      for (auto& t : pool) t.join();
-      for (const auto& e : errors)
-        if (!e.empty()) throw Error(PBRT_ERR_INTERNAL, "mesh BVH build: " + e);
-    }
+    info.resize(nm);
     for (size_t k = 0; k < nm; ++k) {
       const auto& x = s->mesh_xyz[k];
       const auto& ix = s->mesh_idx[k];
       const uint32_t nv = (uint32_t)(x.size() / 3);
-      BuiltBVH b = std::move(built[k]);
-      s->blas_depth = std::max<uint32_t>(s->blas_depth, (uint32_t)b.stats.max_depth);
-      std::vector<DevNode> n;
-      std::vector<DevTri> t;
-      RootDesc r;
-      flatten_mesh_bvh(b, x.data(), ix.data(), &n, &t, &r);
-      MeshInfo mi{};
-      mi.node_base = (int32_t)blas.size();
-      mi.tri_base = (int32_t)tris.size();
-      mi.root_ref = r.ref;
-      for (int a = 0; a < 3; ++a) {
-        mi.lo[a] = r.lo[a];
-        mi.hi[a] = r.hi[a];
-      }
+      MeshInfo& mi = info[k];
       mi.vert_base = (uint32_t)(verts.size() / 3);
       mi.idx_base = (uint32_t)(idx.size() / 3);
-      // TriangleMesh::get_bounding_box (triangle_mesh.rs:150-174): every vertex
       mi.bounds = Box64::empty();
       for (uint32_t v = 0; v < nv; ++v)
         for (int a = 0; a < 3; ++a) {
           mi.bounds.lo[a] = std::fmin((double)x[3 * v + a], mi.bounds.lo[a]);
           mi.bounds.hi[a] = std::fmax((double)x[3 * v + a], mi.bounds.hi[a]);
         }
-      blas.insert(blas.end(), n.begin(), n.end());
-      tris.insert(tris.end(), t.begin(), t.end());
       verts.insert(verts.end(), x.begin(), x.end());
       idx.insert(idx.end(), ix.begin(), ix.end());
-      info.push_back(mi);
     }
+    const auto build_t0 = std::chrono::steady_clock::now();
+    if (s->device_build && nm > 0) {
+      // one batched device build (bvh_gpu.hip): every mesh's nodes and triangles in place
+      dev_upload(&s->verts, verts);
+      dev_upload(&s->idx, idx);
+      std::vector<GpuMeshSpan> spans(nm);
+      for (size_t k = 0; k < nm; ++k)
+        spans[k] = GpuMeshSpan{info[k].vert_base, (uint32_t)(s->mesh_xyz[k].size() / 3),
+                               (uint32_t)(s->mesh_idx[k].size() / 3)};
+      GpuBuiltMany g;
+      gpu_build_meshes(s->ctx->stream, s->verts, s->idx, spans, s->mesh_params, &g);
+      s->blas = g.nodes;
+      s->tris = g.tris;
+      s->blas_depth = (uint32_t)g.stats.max_depth;
+      for (size_t k = 0; k < nm; ++k) {
+        MeshInfo& mi = info[k];
+        mi.node_base = (int32_t)g.meshes[k].node_base;
+        mi.tri_base = (int32_t)g.meshes[k].tri_base;
+        mi.root_ref = g.meshes[k].root.ref;
+        for (int a = 0; a < 3; ++a) {
+          mi.lo[a] = g.meshes[k].root.lo[a];
+          mi.hi[a] = g.meshes[k].root.hi[a];
+        }
+      }
+    } else {
+      // the meshes' trees are independent: build them on the host's cores, then flatten in order
+      std::vector<BuiltBVH> built(nm);
+      {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<size_t> next{0};
+        std::vector<std::string> errors(hw);
+        auto work = [&](unsigned w) {
+          try {
+            for (size_t k = next++; k < nm; k = next++) {
+              const auto& x = s->mesh_xyz[k];
+              const auto& ix = s->mesh_idx[k];
+              built[k] = build_mesh_bvh(x.data(), (uint32_t)(x.size() / 3), ix.data(), (uint32_t)(ix.size() / 3),
+                                        s->mesh_params);
+            }
+          } catch (const std::exception& e) {
+            errors[w] = e.what();
+          }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned w = 1; w < std::min<size_t>(hw, nm); ++w) pool.emplace_back(work, w);
+        work(0);
+        for (auto& t : pool) t.join();
+        for (const auto& e : errors)
+          if (!e.empty()) throw Error(PBRT_ERR_INTERNAL, "mesh BVH build: " + e);
+      }
+      for (size_t k = 0; k < nm; ++k) {
+        const auto& x = s->mesh_xyz[k];
+        const auto& ix = s->mesh_idx[k];
+        BuiltBVH b = std::move(built[k]);
+        s->blas_depth = std::max<uint32_t>(s->blas_depth, (uint32_t)b.stats.max_depth);
+        std::vector<DevNode> n;
+        std::vector<DevTri> t;
+        RootDesc r;
+        flatten_mesh_bvh(b, x.data(), ix.data(), &n, &t, &r);
+        MeshInfo& mi = info[k];
+        mi.node_base = (int32_t)blas.size();
+        mi.tri_base = (int32_t)tris.size();
+        mi.root_ref = r.ref;
+        for (int a = 0; a < 3; ++a) {
+          mi.lo[a] = r.lo[a];
+          mi.hi[a] = r.hi[a];
+        }
+        blas.insert(blas.end(), n.begin(), n.end());
+        tris.insert(tris.end(), t.begin(), t.end());
+      }
+    }
+    s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - build_t0).count();
     // object boxes (local bounds through the transform) and the median-split object BVH
     const uint32_t no = (uint32_t)s->objects.size();
     std::vector<Box64> boxes(no);
@@ -640,11 +693,13 @@ int pbrt_scene_commit(pbrt_scene* s) {
     }
     dev_upload(&s->dcsg, dc);
     dev_upload(&s->tlas, tlas);
-    dev_upload(&s->blas, blas);
-    dev_upload(&s->tris, tris);
+    if (!s->blas) {  // host-built meshes (the device build leaves its arrays in place)
+      dev_upload(&s->blas, blas);
+      dev_upload(&s->tris, tris);
+      dev_upload(&s->verts, verts);
+      dev_upload(&s->idx, idx);
+    }
     dev_upload(&s->objs, dobjs);
-    dev_upload(&s->verts, verts);
-    dev_upload(&s->idx, idx);
     dev_upload(&s->mats, s->materials);
     dev_upload(&s->dlights, dl);
     PBRT_HIP(hipMalloc(&s->status, sizeof(int)));
