@@ -300,7 +300,8 @@ def render_profile(pa, ctx, spp=16, frames_per_pass=16):
     cache = pa.RadianceCache(ctx, seed=4)
     _, st, pr = scene.render_ex(W, H, spp, D, cam["eye"], cam["look"], seed=1, cache=cache,
                                 samples_per_pass=frames_per_pass, **view)
-    return {"plain": plain, "cache": dict(ms=float(pr["ms"][-1]), rays=st["rays"], frames=spp)}
+    return {"plain": plain, "cache": dict(ms=float(pr["ms"][-1]), rays=st["rays"], frames=spp,
+                                          queries=st["cache_queries"])}
 
 
 def main():
