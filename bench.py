@@ -33,12 +33,13 @@ SEED_RAYS = 7
 # Memory-side bytes of one k_closest launch on each batch of this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2, WRITE x1), measured at 32
 # waves per CU; profiles/k_closest_icosphere1p3m_counters.md. Keyed by (batch, block, lds_stack,
-# refill): reported only for the kernel variants that were measured.
+# refill, tree width): reported only for the kernel variants that were measured.
 _COUNTERS = "profiles/k_closest_icosphere1p3m_counters.md"
 MEASURED_TRAFFIC = {
-    ("coherent", 128, 8, 0): {"bytes": 253.77e6, "source": _COUNTERS},
-    ("incoherent", 256, 8, 1): {"bytes": 3141.65e6, "source": _COUNTERS},
+    ("coherent", 128, 8, 0, 2): {"bytes": 253.77e6, "source": _COUNTERS},
+    ("incoherent", 256, 8, 1, 2): {"bytes": 3141.65e6, "source": _COUNTERS},
 }
+LAUNCH_KEYS = ("block", "waves_per_cu", "lds_stack", "refill", "width")
 
 
 def rows(r):
@@ -315,7 +316,7 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--batch-launches", default="",
-                    help="fixed launch per batch, 'block,waves,lds_stack,refill' for the coherent then the "
+                    help="fixed launch per batch, 'block,waves,lds_stack,refill[,width]' for the coherent then the "
                          "incoherent batch, separated by ';' (no autotuning)")
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
@@ -385,13 +386,13 @@ def main():
     launches = {k: mesh.get_launch() for k in dev}
     if args.batch_launches:
         for k, spec in zip(dev, args.batch_launches.split(";")):
-            b, w, d, r = (int(x) for x in spec.split(","))
-            launches[k] = dict(block=b, waves_per_cu=w, lds_stack=d, refill=r)
+            vals = [int(x) for x in spec.split(",")]
+            launches[k] = dict(zip(LAUNCH_KEYS, vals + [2] * (5 - len(vals))))
     elif not (args.block or args.waves or args.lds_stack or args.refill or args.no_autotune):
         tuned = {}
         for k, (dr, dh, n) in dev.items():
             tuned[k] = mesh.autotune(dr, n, dh, any_hit=False, reps=2)
-            launches[k] = {x: tuned[k][x] for x in ("block", "waves_per_cu", "lds_stack", "refill")}
+            launches[k] = {x: tuned[k][x] for x in LAUNCH_KEYS}
         print(f"[bench] autotuned launches: {tuned}", file=sys.stderr, flush=True)
 
     # exact per-ray work on the same rays (instrumented kernel, untimed)
@@ -442,7 +443,7 @@ def main():
     # roofline over both launches of the dominant kernel: algorithmic bytes / mean kernel time
     alg_bytes = sum(work[k]["bytes_per_launch"] for k in dev)
     achieved = alg_bytes / (step_kernel_ms * 1e-3) / 1e9
-    measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "lds_stack", "refill")))
+    measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "lds_stack", "refill", "width")))
                 for k in dev]
     measured = None if (args.side != 2048 or None in measured) else measured
 

@@ -70,6 +70,9 @@ typedef struct pbrt_launch {
   uint32_t refill;       /* 0: a lane takes its next ray as soon as its ray is done; N in 1..64: done lanes
                             wait and take their next rays together, once N lanes of the wave wait or no
                             lane of it is still tracing */
+  uint32_t width;        /* tree walked by closest- and any-hit: 0 or 2 the binary tree, 4 a 4-wide tree
+                            collapsed from it (128-byte nodes; built on first use). Same hits either way;
+                            pbrt_mesh_trace_closest_stats always counts on the binary tree */
 } pbrt_launch;
 
 /* ---- context, errors, device plumbing ---- */
@@ -135,7 +138,8 @@ int pbrt_mesh_trace_any(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, u
 int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
 int pbrt_mesh_get_launch(const pbrt_mesh* mesh, pbrt_launch* out);
 /* Autotuner (no counterpart in the reference, which has one CPU code path): times every supported
- * launch shape — block size x LDS stack depth x resident waves per CU x refill policy (0, 1, 4) — of the closest-hit
+ * launch shape — block size x LDS stack depth x resident waves per CU x refill policy (0, 1, 4) x tree
+ * width (2, 4) — of the closest-hit
  * (any_hit = 0) or any-hit kernel on the caller's rays, `reps` timed runs each, and keeps the
  * fastest for this mesh. out_dev receives hits (pbrt_hit) or occlusion bytes and is overwritten. */
 int pbrt_mesh_autotune(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, void* out_dev, int any_hit,

@@ -32,11 +32,14 @@ struct pbrt_mesh {
   DevNode* nodes = nullptr;
   DevTri* tris = nullptr;
   uint64_t n_nodes = 0;
+  DevNode4* nodes4 = nullptr;  // the 4-wide tree, collapsed on first use
+  uint64_t n_nodes4 = 0;
+  int32_t root4 = 0;
   RootDesc root{};
   uint2* spill = nullptr;
   uint64_t spill_entries = 0;
   int* status = nullptr;
-  pbrt_launch launch{128, 16, 16, 0};
+  pbrt_launch launch{128, 16, 16, 0, 2};
   std::mutex mu;
 };
 
@@ -61,15 +64,41 @@ LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
   a.lds_stack = m->launch.lds_stack;
   a.waves_per_cu = m->launch.waves_per_cu;
   a.refill = m->launch.refill;
+  a.width = m->launch.width == 4 ? 4 : 2;
+  a.nodes4 = m->nodes4;
+  a.root4 = m->root4;
   a.cu_count = (uint32_t)m->ctx->cu_count;
   return a;
+}
+
+// The 4-wide tree, collapsed on the host from the device's binary nodes (either builder's).
+void ensure_bvh4(pbrt_mesh* m) {
+  if (m->nodes4 || !m->ctx) return;
+  std::vector<DevNode> n2(m->n_nodes);
+  if (m->n_nodes) PBRT_HIP(hipMemcpy(n2.data(), m->nodes, m->n_nodes * sizeof(DevNode), hipMemcpyDeviceToHost));
+  std::vector<DevNode4> n4;
+  int32_t root4 = 0;
+  collapse_bvh4(n2, m->root.ref, &n4, &root4);
+  DevNode4* d = nullptr;
+  PBRT_HIP(hipMalloc(&d, std::max<size_t>(1, n4.size()) * sizeof(DevNode4)));
+  if (!n4.empty()) {
+    const hipError_t e = hipMemcpy(d, n4.data(), n4.size() * sizeof(DevNode4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      PBRT_HIP(e);
+    }
+  }
+  m->nodes4 = d;
+  m->n_nodes4 = n4.size();
+  m->root4 = root4;
 }
 
 // Sizes the HBM spill area for the launch shape and checks the kernels' error word.
 void prepare_spill(pbrt_mesh* m, LaunchArgs* a) {
   const uint32_t threads = traversal_grid_threads(*a);
   if (threads == 0) throw Error(PBRT_ERR_INVALID_ARG, "unsupported block size");
-  const uint64_t depth = m->stats.max_depth;  // the stack never holds more than depth - 1 entries
+  // pending entries: at most depth - 1 on the binary tree, 3 per level on the 4-wide one
+  const uint64_t depth = m->stats.max_depth * (a->width == 4 ? 3 : 1);
   const uint32_t per_thread = depth + 1 > a->lds_stack ? (uint32_t)(depth + 1 - a->lds_stack) : 1;
   const uint64_t need = (uint64_t)per_thread * threads;
   if (need > m->spill_entries) {
@@ -315,6 +344,7 @@ int pbrt_mesh_destroy(pbrt_mesh* m) {
     if (!m) return;
     if (m->ctx) (void)hipSetDevice(m->ctx->device);
     (void)hipFree(m->nodes);
+    (void)hipFree(m->nodes4);
     (void)hipFree(m->tris);
     (void)hipFree(m->spill);
     (void)hipFree(m->status);
@@ -378,6 +408,13 @@ int pbrt_mesh_set_launch(pbrt_mesh* m, const pbrt_launch* l) {
       throw Error(PBRT_ERR_INVALID_ARG, "lds_stack must be 8, 16, 24 or 32");
     if (v.waves_per_cu > 32) throw Error(PBRT_ERR_INVALID_ARG, "waves_per_cu above 32");
     if (v.refill > 64) throw Error(PBRT_ERR_INVALID_ARG, "refill above 64 lanes");
+    if (!v.width) v.width = 2;
+    if (v.width != 2 && v.width != 4) throw Error(PBRT_ERR_INVALID_ARG, "width must be 2 or 4");
+    if (v.width == 4) {
+      if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
+      check_ctx(m->ctx);
+      ensure_bvh4(m);
+    }
     m->launch = v;
   });
 }
@@ -432,6 +469,8 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
     const uint32_t depths[] = {8, 16, 24, 32};
     const uint32_t waves[] = {8, 12, 16, 24, 32};
     const uint32_t refills[] = {0, 1, 4};
+    const uint32_t widths[] = {2, 4};
+    ensure_bvh4(m);
     pbrt_launch best{};
     float best_t = 1e30f;
     const int kind = any_hit ? 2 : 0;
@@ -439,29 +478,30 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
       for (uint32_t b : blocks)
         for (uint32_t d : depths)
           for (uint32_t w : waves)
-            for (uint32_t f : refills) {
-              if (w < b / 64) continue;
-              m->launch = pbrt_launch{b, w, d, f};
-              LaunchArgs a = base_args(m, n, nullptr);
-              a.rays = rays;
-              a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
-              a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
-              run(m, a, kind);  // warm-up
-              float cand = 1e30f;
-              for (uint32_t r = 0; r < reps; ++r) {
-                PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
-                run(m, a, kind);
-                PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
-                PBRT_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
-                cand = std::min(cand, ms);
+            for (uint32_t f : refills)
+              for (uint32_t wd : widths) {
+                if (w < b / 64) continue;
+                m->launch = pbrt_launch{b, w, d, f, wd};
+                LaunchArgs a = base_args(m, n, nullptr);
+                a.rays = rays;
+                a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
+                a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
+                run(m, a, kind);  // warm-up
+                float cand = 1e30f;
+                for (uint32_t r = 0; r < reps; ++r) {
+                  PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
+                  run(m, a, kind);
+                  PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
+                  PBRT_HIP(hipEventSynchronize(e1));
+                  float ms = 0.f;
+                  PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                  cand = std::min(cand, ms);
+                }
+                if (cand < best_t) {
+                  best_t = cand;
+                  best = m->launch;
+                }
               }
-              if (cand < best_t) {
-                best_t = cand;
-                best = m->launch;
-              }
-            }
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

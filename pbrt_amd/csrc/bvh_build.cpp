@@ -224,6 +224,69 @@ BuiltBVH build_mesh_bvh(const float* xyz, uint32_t nv, const uint32_t* idx, uint
   return out;
 }
 
+namespace {
+
+struct Child4 {
+  int32_t ref;
+  float lo[3], hi[3];
+};
+
+void children_of(const DevNode& d, Child4 c[2]) {
+  c[0] = Child4{d.ref[0], {d.xy0[0], d.xy0[2], d.z01[0]}, {d.xy0[1], d.xy0[3], d.z01[1]}};
+  c[1] = Child4{d.ref[1], {d.xy1[0], d.xy1[2], d.z01[2]}, {d.xy1[1], d.xy1[3], d.z01[3]}};
+}
+
+double child_area(const Child4& c) {
+  const double dx = (double)c.hi[0] - c.lo[0], dy = (double)c.hi[1] - c.lo[1], dz = (double)c.hi[2] - c.lo[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+int32_t collapse_node(const std::vector<DevNode>& n2, int32_t ref2, std::vector<DevNode4>* n4) {
+  if (ref2 < 0) return ref2;
+  std::vector<Child4> ch(2);
+  children_of(n2[ref2], ch.data());
+  while (ch.size() < 4) {
+    int best = -1;
+    double best_area = -1.0;
+    for (int k = 0; k < (int)ch.size(); ++k)
+      if (ch[k].ref >= 0 && child_area(ch[k]) > best_area) {
+        best = k;
+        best_area = child_area(ch[k]);
+      }
+    if (best < 0) break;
+    Child4 sub[2];
+    children_of(n2[ch[best].ref], sub);
+    ch[best] = sub[0];
+    ch.insert(ch.begin() + best + 1, sub[1]);
+  }
+  const int32_t me = (int32_t)n4->size();
+  n4->push_back(DevNode4{});
+  int32_t refs[4] = {0, 0, 0, 0};
+  for (size_t k = 0; k < ch.size(); ++k) refs[k] = collapse_node(n2, ch[k].ref, n4);
+  DevNode4& d = (*n4)[me];
+  for (int k = 0; k < 4; ++k) {
+    const bool on = k < (int)ch.size();
+    d.lox[k] = on ? ch[k].lo[0] : 0.f;
+    d.loy[k] = on ? ch[k].lo[1] : 0.f;
+    d.loz[k] = on ? ch[k].lo[2] : 0.f;
+    d.hix[k] = on ? ch[k].hi[0] : 0.f;
+    d.hiy[k] = on ? ch[k].hi[1] : 0.f;
+    d.hiz[k] = on ? ch[k].hi[2] : 0.f;
+    d.ref[k] = refs[k];
+    d.count[k] = 0;
+  }
+  d.count[0] = (int32_t)ch.size();
+  return me;
+}
+
+}  // namespace
+
+void collapse_bvh4(const std::vector<DevNode>& n2, int32_t root_ref, std::vector<DevNode4>* n4, int32_t* root4) {
+  n4->clear();
+  n4->reserve(n2.size() / 2 + 1);
+  *root4 = collapse_node(n2, root_ref, n4);
+}
+
 void flatten_mesh_bvh(const BuiltBVH& b, const float* xyz, const uint32_t* idx, std::vector<DevNode>* nodes,
                       std::vector<DevTri>* tris, RootDesc* root) {
   const uint32_t nt = (uint32_t)b.order.size();

@@ -59,6 +59,22 @@ struct alignas(16) DevNode {
 };
 static_assert(sizeof(DevNode) == 64, "node is 64 bytes");
 
+// A 4-wide node collapsed from the binary tree (128 bytes, one cache line): the boxes of up to four
+// children as six float4 (min x, max x, min y, max y, min z, max z of children 0..3), their references
+// (as DevNode's), and the number of children in count[0] (2 to 4; unused slots come last).
+struct alignas(16) DevNode4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t ref[4];
+  int32_t count[4];
+};
+static_assert(sizeof(DevNode4) == 128, "4-wide node is 128 bytes");
+
+// Collapses a flattened binary tree into 4-wide nodes (depth-first, parents before children): each
+// node takes its binary node's two children and repeatedly opens the interior child of largest
+// surface area until it has four. Leaves keep their references. root4 receives the new root
+// reference (a leaf root stays a leaf).
+void collapse_bvh4(const std::vector<DevNode>& n2, int32_t root_ref, std::vector<DevNode4>* n4, int32_t* root4);
+
 // A triangle in leaf order, pre-gathered: p0 (w = original index bits), e1 = p1 - p0, e2 = p2 - p0.
 struct alignas(16) DevTri {
   float p0[4];

@@ -247,6 +247,229 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
   }
 }
 
+// The 4-wide node step shared by the closest- and any-hit kernels: the children's boxes tested over
+// [near, tmax], hit children ordered by entry distance (a box the ray misses sorts last with an
+// infinite key: a hit's entry is finite, at most tmax). Returns the number of children hit; the
+// nearest is in r0 / k0, the others in r1..r3 / k1..k3 in increasing distance.
+__device__ __forceinline__ int node4_step(const RayState& r, const DevNode4* __restrict__ nodes, int32_t cur, float tmax,
+                                          int32_t& r0, int32_t& r1, int32_t& r2, int32_t& r3, float& k0, float& k1,
+                                          float& k2, float& k3) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + cur);
+  const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+  const int4 rf = reinterpret_cast<const int4*>(p)[6];
+  const int cnt = reinterpret_cast<const int4*>(p)[7].x;
+  float d0, d1, d2, d3;
+  const bool h0 = box_test(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmax, &d0);
+  const bool h1 = box_test(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmax, &d1);
+  const bool h2 = cnt > 2 && box_test(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmax, &d2);
+  const bool h3 = cnt > 3 && box_test(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmax, &d3);
+  k0 = h0 ? d0 : INFINITY;
+  k1 = h1 ? d1 : INFINITY;
+  k2 = h2 ? d2 : INFINITY;
+  k3 = h3 ? d3 : INFINITY;
+  r0 = rf.x;
+  r1 = rf.y;
+  r2 = rf.z;
+  r3 = rf.w;
+  auto cswap = [](float& ka, int32_t& ra, float& kb, int32_t& rb) {
+    const bool sw = kb < ka;
+    const float tk = sw ? kb : ka;
+    kb = sw ? ka : kb;
+    ka = tk;
+    const int32_t tr = sw ? rb : ra;
+    rb = sw ? ra : rb;
+    ra = tr;
+  };
+  cswap(k0, r0, k1, r1);
+  cswap(k2, r2, k3, r3);
+  cswap(k0, r0, k2, r2);
+  cswap(k1, r1, k3, r3);
+  cswap(k1, r1, k2, r2);
+  return (int)h0 + (int)h1 + (int)h2 + (int)h3;
+}
+
+// Closest hit over the 4-wide tree: the flat loop of k_closest with one 4-wide node per iteration
+// (nearest child next, the others pushed farthest first). Hits are those of k_closest (the tie rule
+// makes them independent of the visiting order); work counters are taken on the binary tree only.
+template <int BLOCK, int DEPTH, bool WAVE_REFILL>
+__global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__ nodes, const DevTri* __restrict__ tris,
+                                                    RootDesc root, const float4* __restrict__ rays,
+                                                    float4* __restrict__ hits, uint32_t n, uint2* __restrict__ spill,
+                                                    uint32_t spill_cap, int* __restrict__ status, uint32_t refill) {
+  __shared__ uint2 lds[DEPTH][BLOCK];
+  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
+  const uint32_t stride = gridDim.x * BLOCK;
+  uint32_t i = st.gtid;
+  if (i >= n) return;
+  RayState r;
+  Closest s;
+  int32_t cur = 0;
+  uint32_t tk = 0, tend = 0;
+  bool live = false, idle = false;
+  auto start = [&](uint32_t k) {
+    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    s.min_t = FLT_MAX;
+    s.t = s.u = s.v = 0.f;
+    s.cur_far = r.far;
+    s.best = 0xffffffffu;
+    st.sp = 0;
+    tk = tend = 0;
+    float entry;
+    live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
+    cur = root.ref;
+  };
+  start(i);
+  for (;;) {
+    bool pop_now = false;
+    if (live && cur >= 0) {
+      int32_t r0, r1, r2, r3;
+      float k0, k1, k2, k3;
+      const int nh = node4_step(r, nodes, cur, s.cur_far, r0, r1, r2, r3, k0, k1, k2, k3);
+      if (nh == 0) {
+        pop_now = true;
+      } else {
+        bool ok = true;
+        if (nh > 3) ok = st.push(r3, k3);
+        if (nh > 2) ok = ok && st.push(r2, k2);
+        if (nh > 1) ok = ok && st.push(r1, k1);
+        if (!ok) {
+          atomicOr(status, 1);
+          live = false;
+        }
+        cur = r0;
+      }
+    }
+    if (live && !pop_now && cur < 0) {
+      if (tk == tend) {
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
+      }
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id)) {
+        if (t >= r.near && t <= r.far && (t < s.min_t || (t == s.min_t && id < s.best))) {
+          s.min_t = t; s.t = t; s.u = u; s.v = v; s.best = id;
+          s.cur_far = fminf(r.far, s.min_t);
+        }
+      }
+      ++tk;
+      pop_now = tk == tend;
+    }
+    if (live && pop_now) {
+      live = false;
+      while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (__uint_as_float(e.y) > s.min_t) continue;
+        cur = (int32_t)e.x;
+        live = true;
+        break;
+      }
+    }
+    if (!live && !idle) {
+      const bool found = s.best != 0xffffffffu;
+      hits[i] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
+                            found ? s.v : 0.f);
+      i += stride;
+      if (i >= n) break;
+      if (WAVE_REFILL)
+        idle = true;
+      else
+        start(i);
+    }
+    if (WAVE_REFILL) {
+      const uint64_t waiting = __ballot(idle), active = __ballot(1);
+      if (idle && ((uint32_t)__popcll(waiting) >= refill || waiting == active)) {
+        start(i);
+        idle = false;
+      }
+    }
+  }
+}
+
+// Any hit over the 4-wide tree (the result does not depend on the visiting order).
+template <int BLOCK, int DEPTH, bool WAVE_REFILL>
+__global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nodes, const DevTri* __restrict__ tris,
+                                                RootDesc root, const float4* __restrict__ rays,
+                                                uint8_t* __restrict__ occluded, uint32_t n, uint2* __restrict__ spill,
+                                                uint32_t spill_cap, int* __restrict__ status, uint32_t refill) {
+  __shared__ uint2 lds[DEPTH][BLOCK];
+  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
+  const uint32_t stride = gridDim.x * BLOCK;
+  uint32_t i = st.gtid;
+  if (i >= n) return;
+  RayState r;
+  int32_t cur = 0;
+  uint32_t tk = 0, tend = 0;
+  bool live = false, hit = false, idle = false;
+  auto start = [&](uint32_t k) {
+    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    st.sp = 0;
+    hit = false;
+    tk = tend = 0;
+    float entry;
+    live = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);
+    cur = root.ref;
+  };
+  start(i);
+  for (;;) {
+    bool pop_now = false;
+    if (live && cur >= 0) {
+      int32_t r0, r1, r2, r3;
+      float k0, k1, k2, k3;
+      const int nh = node4_step(r, nodes, cur, r.far, r0, r1, r2, r3, k0, k1, k2, k3);
+      if (nh == 0) {
+        pop_now = true;
+      } else {
+        bool ok = true;
+        if (nh > 3) ok = st.push(r3, 0.f);
+        if (nh > 2) ok = ok && st.push(r2, 0.f);
+        if (nh > 1) ok = ok && st.push(r1, 0.f);
+        if (!ok) {
+          atomicOr(status, 1);
+          live = false;
+        }
+        cur = r0;
+      }
+    }
+    if (live && !pop_now && cur < 0) {
+      if (tk == tend) {
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
+      }
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+        hit = true;
+        live = false;
+      }
+      ++tk;
+      pop_now = tk == tend;
+    }
+    if (live && pop_now) {
+      live = st.sp > 0;
+      if (live) cur = (int32_t)st.pop().x;
+    }
+    if (!live && !idle) {
+      occluded[i] = hit ? 1 : 0;
+      i += stride;
+      if (i >= n) break;
+      if (WAVE_REFILL)
+        idle = true;
+      else
+        start(i);
+    }
+    if (WAVE_REFILL) {
+      const uint64_t waiting = __ballot(idle), active = __ballot(1);
+      if (idle && ((uint32_t)__popcll(waiting) >= refill || waiting == active)) {
+        start(i);
+        idle = false;
+      }
+    }
+  }
+}
+
 template <int BLOCK, int DEPTH>
 void launch_shape(const LaunchArgs& a, uint32_t* grid) {
   const uint32_t per_block_waves = BLOCK / 64;
@@ -265,6 +488,23 @@ hipError_t launch_t(const LaunchArgs& a, int kind) {
   if ((uint64_t)spill_cap * spill_stride > a.spill_capacity_entries) return hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)a.stream;
   const bool wave = a.refill > 0;
+  if (a.width == 4 && kind != 1) {
+    RootDesc root4 = a.root;
+    root4.ref = a.root4;
+    if (kind == 0 && !wave)
+      hipLaunchKernelGGL((k_closest4<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
+                         (const float4*)a.rays, (float4*)a.hits, a.n, a.spill, spill_cap, a.status, 0u);
+    else if (kind == 0)
+      hipLaunchKernelGGL((k_closest4<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
+                         (const float4*)a.rays, (float4*)a.hits, a.n, a.spill, spill_cap, a.status, a.refill);
+    else if (!wave)
+      hipLaunchKernelGGL((k_any4<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
+                         (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, 0u);
+    else
+      hipLaunchKernelGGL((k_any4<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
+                         (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, a.refill);
+    return hipGetLastError();
+  }
   if (kind == 0 && !wave)
     hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, false, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
                        a.root, (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status, 0u);

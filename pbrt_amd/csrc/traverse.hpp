@@ -20,6 +20,9 @@ struct LaunchArgs {
   void* stream;
   uint32_t block, lds_stack, waves_per_cu, cu_count;
   uint32_t refill;  // 0: eager per-lane refill; N > 0: wave refill once N lanes wait
+  uint32_t width;   // 2: the binary tree; 4: the 4-wide tree (closest and any hit; counters always binary)
+  const DevNode4* nodes4;
+  int32_t root4;    // root reference in the 4-wide tree (the root box is root's)
 };
 
 // kind: 0 closest hit, 1 closest hit with per-ray work counters, 2 any hit.

@@ -3,7 +3,8 @@
 Bar: triangle indices bit-exact, t/u/v bit-exact (same FP32 operation sequence), per-ray work
 counters equal to the oracle's on the same tree, any-hit equal; hit t within 1e-5 relative of an
 f64 re-evaluation of the hit triangle. Published hit counts (docs/mesures_bvh.md:706-713) are
-reproduced on the device.
+reproduced on the device. The 4-wide tree (launch width 4) must give the same hits and occlusion
+as the binary one (the counters are taken on the binary tree only).
 """
 import numpy as np
 import pytest
@@ -81,15 +82,16 @@ def test_reference_grid_fixture_and_brute_force(gpu_ctx, precision_params):
     assert (hits["prim"] >= 0).sum() > 20
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("refill", [0, 1, 64])
-def test_any_hit_and_far_sweep(gpu_ctx, refill):
+def test_any_hit_and_far_sweep(gpu_ctx, refill, width):
     """triangle_mesh/bvh.rs:937-975 on the device: any-hit equals closest-hit existence, also with
     `far` cut at factors of the nearest distance; and equals the oracle's any-hit. For each refill
     policy (64: lanes wait until the whole wave is done)."""
     v, f = load_mesh("dragon_vrip_res4")
     rays = po.orbit_rays(v).astype(np.float32)[::3]
     mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
-    mesh.set_launch(refill=refill)
+    mesh.set_launch(refill=refill, width=width)
     orc = po.Mesh(v, f, precision=32, **pbrt_amd.DEVICE_BVH)
     hits = mesh.trace_host(rays)
     occ = mesh.trace_host(rays, mode="any")
@@ -116,6 +118,49 @@ def test_launch_shapes_agree(gpu_ctx, block, lds_stack, refill):
                            as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 20000, seed=9))])
     check_closest(gpu_ctx, v, f, rays, pbrt_amd.REFERENCE_BVH, stats=True,
                   launch=dict(block=block, lds_stack=lds_stack, waves_per_cu=8, refill=refill))
+
+
+@pytest.mark.parametrize("name", list(PUBLISHED))
+@pytest.mark.parametrize("which", ["reference", "device"])
+def test_wide_tree_published_meshes(gpu_ctx, name, which):
+    v, f = load_mesh(name)
+    params = pbrt_amd.REFERENCE_BVH if which == "reference" else pbrt_amd.DEVICE_BVH
+    rays = po.orbit_rays(v).astype(np.float32)
+    _, hits, _ = check_closest(gpu_ctx, v, f, rays, params, stats=False, launch=dict(width=4))
+    assert int((hits["prim"] >= 0).sum()) == PUBLISHED[name][0]
+
+
+@pytest.mark.parametrize("block", [64, 256])
+@pytest.mark.parametrize("lds_stack", [8, 32])
+@pytest.mark.parametrize("refill", [0, 4])
+def test_wide_tree_launch_shapes(gpu_ctx, block, lds_stack, refill):
+    """The 4-wide tree over launch shapes, with the 8-entry LDS stack spilling on the deep
+    reference-mode tree, and on incoherent rays."""
+    v, f = load_mesh("dragon_vrip_res3")
+    rays = np.concatenate([as_rows(pbrt_amd.orbit_rays(v, 1, 96)),
+                           as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 20000, seed=9))])
+    launch = dict(block=block, lds_stack=lds_stack, waves_per_cu=8, refill=refill, width=4)
+    check_closest(gpu_ctx, v, f, rays, pbrt_amd.REFERENCE_BVH, stats=False, launch=launch)
+    v, f = pbrt_amd.icosphere(6, noise="fbm", amp=0.08, octaves=3, seed=2)
+    rays = as_rows(pbrt_amd.random_rays((0, 0, 0), 1.1, 50000, seed=4))
+    check_closest(gpu_ctx, v, f, rays, pbrt_amd.DEVICE_BVH, stats=False, launch=launch)
+
+
+def test_wide_tree_edge_cases_and_device_build(gpu_ctx):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)  # the root is a leaf
+    mesh.set_launch(width=4)
+    rays = np.array([[0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, -1.0, 1000.0],
+                     [0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, 1.0, 1000.0]], np.float32)
+    assert mesh.trace_host(rays)["prim"].tolist() == [0, -1]
+    assert mesh.trace_host(rays, mode="any").tolist() == [1, 0]
+    v, f = load_mesh("bunny")  # a device-built tree collapses the same way
+    rays = po.orbit_rays(v).astype(np.float32)
+    h = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
+    d = pbrt_amd.TriangleMesh(gpu_ctx, v, f, device_build=True)
+    d.set_launch(width=4)
+    assert d.trace_host(rays).tobytes() == h.trace_host(rays).tobytes()
 
 
 def test_incoherent_rays_on_displaced_icosphere(gpu_ctx):
