@@ -1124,8 +1124,10 @@ __device__ __forceinline__ void count_rays(PathBuffers& b, uint32_t n) {
     atomicAdd(reinterpret_cast<unsigned long long*>(counter(b.counts, kCntRays)), (unsigned long long)n);
 }
 
+// Five waves per SIMD (96 registers): the scene walk is bound by the latency of dependent node
+// loads; the compiler's own choice (101 registers) allows four.
 template <int DEPTH>
-__global__ __launch_bounds__(kSceneBlock) void k_extend(SceneView S, PathBuffers b, int q, uint2* __restrict__ spill,
+__global__ __launch_bounds__(kSceneBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_extend(SceneView S, PathBuffers b, int q, uint2* __restrict__ spill,
                                                         uint32_t spill_cap, int* __restrict__ status) {
   __shared__ uint2 lds[DEPTH][kSceneBlock];
   Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
@@ -1155,11 +1157,12 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
   return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-// What one shaded vertex appends to the queues.
+// What one shaded vertex appends to the queues. A vertex queues a shadow ray or a cache query,
+// never both, so the two share w0..w2 (fewer registers live across the queue appends).
 struct ShadeOut {
   bool shadow = false, next = false, query = false;
-  float4 sh_o, sh_d, sh_c;     // shadow ray and its contribution (path index bits in w)
-  float4 q0, q1, q2, qb;       // cache query: raw inputs, throughput (path index bits in w)
+  float4 w0, w1, w2;  // shadow: origin, direction, contribution (path index bits in w); query: raw inputs
+  float4 qb;          // query: throughput (path index bits in w)
 };
 
 // One path vertex (path.rs:55-112): emission after a specular bounce, the bounce cut, next-event
@@ -1258,9 +1261,9 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
         b.rec[(kRecRaw + 2) * fs + base] = raw2;
       } else {
         out->query = true;  // the cache stands for everything this path would gather from here on
-        out->q0 = raw0;
-        out->q1 = raw1;
-        out->q2 = raw2;
+        out->w0 = raw0;
+        out->w1 = raw1;
+        out->w2 = raw2;
         out->qb = make_float4(beta4.x, beta4.y, beta4.z, __uint_as_float(j));
         return;
       }
@@ -1285,9 +1288,9 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
         contrib = {contrib.x * w, contrib.y * w, contrib.z * w};
       }
       out->shadow = true;
-      out->sh_o = make_float4(ls.so.x, ls.so.y, ls.so.z, 1e-5f);
-      out->sh_d = make_float4(ls.sd.x, ls.sd.y, ls.sd.z, ls.dist);
-      out->sh_c = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(j));
+      out->w0 = make_float4(ls.so.x, ls.so.y, ls.so.z, 1e-5f);
+      out->w1 = make_float4(ls.sd.x, ls.sd.y, ls.sd.z, ls.dist);
+      out->w2 = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(j));
     }
   }
   ScatterOut sc;
@@ -1339,9 +1342,11 @@ __device__ __forceinline__ void block_append(uint32_t* const* ctr, const bool* w
 }
 
 // The vertices of queue q: the grid strides over the queue a workgroup at a time, so every thread
-// of a workgroup reaches the queue appends together.
+// of a workgroup reaches the queue appends together. Four waves per SIMD (128 registers, no
+// spill): shading is bound by the latency of its scattered loads, and the compiler's own choice
+// (about 150 registers) allows three.
 template <bool NRC>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(SceneView S, RenderParams p, PathBuffers b, int q) {
   const uint32_t n = *counter(b.counts, q);
   for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
@@ -1357,15 +1362,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderParams
     uint32_t slot[3] = {0, 0, 0};
     block_append<K>(ctr, want, slot);
     if (o.shadow) {
-      b.sh_o[slot[0]] = o.sh_o;
-      b.sh_d[slot[0]] = o.sh_d;
-      b.sh_c[slot[0]] = o.sh_c;
+      b.sh_o[slot[0]] = o.w0;
+      b.sh_d[slot[0]] = o.w1;
+      b.sh_c[slot[0]] = o.w2;
     }
     if (o.next) b.queue[q ^ 1][slot[1]] = j;
     if (NRC && o.query) {
-      b.q_raw[3 * (size_t)slot[2] + 0] = o.q0;
-      b.q_raw[3 * (size_t)slot[2] + 1] = o.q1;
-      b.q_raw[3 * (size_t)slot[2] + 2] = o.q2;
+      b.q_raw[3 * (size_t)slot[2] + 0] = o.w0;
+      b.q_raw[3 * (size_t)slot[2] + 1] = o.w1;
+      b.q_raw[3 * (size_t)slot[2] + 2] = o.w2;
       b.q_beta[slot[2]] = o.qb;
     }
   }
