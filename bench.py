@@ -36,12 +36,13 @@ SEED_RAYS = 7
 # refill, tree width): reported only for the kernel variants that were measured.
 _COUNTERS = "profiles/k_closest_icosphere1p3m_counters.md"
 MEASURED_TRAFFIC = {
-    ("coherent", 256, 8, 0, 2): {"bytes": 255.14e6, "source": _COUNTERS},
-    ("incoherent", 128, 8, 1, 4): {"bytes": 2369.43e6, "source": _COUNTERS},
-    ("coherent", 128, 8, 0, 2): {"bytes": 253.77e6, "source": _COUNTERS},
-    ("incoherent", 256, 8, 1, 2): {"bytes": 3141.65e6, "source": _COUNTERS},
+    ("coherent", 256, 8, 0, 2, 0): {"bytes": 255.14e6, "source": _COUNTERS},
+    ("incoherent", 128, 8, 1, 4, 0): {"bytes": 2369.43e6, "source": _COUNTERS},
+    ("coherent", 128, 8, 0, 2, 0): {"bytes": 253.77e6, "source": _COUNTERS},
+    ("incoherent", 256, 8, 1, 2, 0): {"bytes": 3141.65e6, "source": _COUNTERS},
 }
-LAUNCH_KEYS = ("block", "waves_per_cu", "lds_stack", "refill", "width")
+LAUNCH_KEYS = ("block", "waves_per_cu", "lds_stack", "refill", "width", "packet")
+LAUNCH_DEFAULTS = dict(block=128, waves_per_cu=16, lds_stack=16, refill=0, width=2, packet=0)
 # The autotuner's picks on this workload (pbrt_mesh_autotune over block x stack x waves x refill x
 # tree width, each batch on its own rays), fixed so that runs are reproducible and match the
 # measured counters; --autotune searches again.
@@ -322,7 +323,7 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--batch-launches", default="",
-                    help="fixed launch per batch, 'block,waves,lds_stack,refill[,width]' for the coherent then the "
+                    help="fixed launch per batch, 'block,waves,lds_stack,refill[,width[,packet]]' for the coherent then the "
                          "incoherent batch, separated by ';' (no autotuning)")
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
@@ -396,7 +397,7 @@ def main():
     if fixed:
         for k, spec in zip(dev, fixed.split(";")):
             vals = [int(x) for x in spec.split(",")]
-            launches[k] = dict(zip(LAUNCH_KEYS, vals + [2] * (5 - len(vals))))
+            launches[k] = dict(LAUNCH_DEFAULTS, **dict(zip(LAUNCH_KEYS, vals)))
     elif not (args.block or args.waves or args.lds_stack or args.refill or args.no_autotune):
         tuned = {}
         for k, (dr, dh, n) in dev.items():
@@ -452,7 +453,8 @@ def main():
     # roofline over both launches of the dominant kernel: algorithmic bytes / mean kernel time
     alg_bytes = sum(work[k]["bytes_per_launch"] for k in dev)
     achieved = alg_bytes / (step_kernel_ms * 1e-3) / 1e9
-    measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "lds_stack", "refill", "width")))
+    measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "lds_stack", "refill", "width",
+                                                                          "packet")))
                 for k in dev]
     measured = None if (args.side != 2048 or None in measured) else measured
 
@@ -535,7 +537,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": sum(m["bytes"] for m in measured) / len(measured) if measured else None,
-            "kernel": ", ".join(f"{'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k})" for k in dev),
+            "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k})" for k in dev),
             "algorithmic_bytes_per_launch": alg_bytes / len(dev),
             "traffic_source": measured[0]["source"] if measured else None,
         },

@@ -73,6 +73,9 @@ typedef struct pbrt_launch {
   uint32_t width;        /* tree walked by closest- and any-hit: 0 or 2 the binary tree, 4 a 4-wide tree
                             collapsed from it (128-byte nodes; built on first use). Same hits either way;
                             pbrt_mesh_trace_closest_stats always counts on the binary tree */
+  uint32_t packet;       /* 1: closest hit by wave packets (64 consecutive rays walk the binary tree on one
+                            wave-wide stack; for coherent rays; block, lds_stack, refill unused); needs
+                            width 2 and a tree at most 33 levels deep. Same hits */
 } pbrt_launch;
 
 /* ---- context, errors, device plumbing ---- */
@@ -139,7 +142,7 @@ int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
 int pbrt_mesh_get_launch(const pbrt_mesh* mesh, pbrt_launch* out);
 /* Autotuner (no counterpart in the reference, which has one CPU code path): times every supported
  * launch shape — block size x LDS stack depth x resident waves per CU x refill policy (0, 1, 4) x tree
- * width (2, 4) — of the closest-hit
+ * width (2, 4), and for closest hit the wave-packet walk at each waves per CU — of the closest-hit
  * (any_hit = 0) or any-hit kernel on the caller's rays, `reps` timed runs each, and keeps the
  * fastest for this mesh. out_dev receives hits (pbrt_hit) or occlusion bytes and is overwritten. */
 int pbrt_mesh_autotune(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, void* out_dev, int any_hit,

@@ -199,13 +199,15 @@ class TriangleMesh:
         call("pbrt_mesh_copy_layout", self.h, _ptr(nodes), _ptr(tris), _ptr(box), ctypes.byref(ref))
         return nodes, tris, box, ref.value
 
-    def set_launch(self, block=0, waves_per_cu=0, lds_stack=0, refill=0, width=0):
-        call("pbrt_mesh_set_launch", self.h, ctypes.byref(Launch(block, waves_per_cu, lds_stack, refill, width)))
+    def set_launch(self, block=0, waves_per_cu=0, lds_stack=0, refill=0, width=0, packet=0):
+        call("pbrt_mesh_set_launch", self.h,
+             ctypes.byref(Launch(block, waves_per_cu, lds_stack, refill, width, packet)))
 
     def get_launch(self):
         l = Launch()
         call("pbrt_mesh_get_launch", self.h, ctypes.byref(l))
-        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, refill=l.refill, width=l.width)
+        return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, refill=l.refill, width=l.width,
+                    packet=l.packet)
 
     def autotune(self, rays, n, out, any_hit=False, reps=2):
         """Times every launch shape on these device rays and keeps the fastest (pbrt_mesh_autotune)."""
@@ -213,7 +215,7 @@ class TriangleMesh:
         ms = ctypes.c_float()
         call("pbrt_mesh_autotune", self.h, rays.ptr, n, out.ptr, int(any_hit), reps, ctypes.byref(l), ctypes.byref(ms))
         return dict(block=l.block, waves_per_cu=l.waves_per_cu, lds_stack=l.lds_stack, refill=l.refill,
-                    width=l.width, ms=ms.value)
+                    width=l.width, packet=l.packet, ms=ms.value)
 
     def export_tree(self):
         s = self.build_stats()

@@ -28,7 +28,7 @@ class BuildStats(ctypes.Structure):
 
 
 class Launch(ctypes.Structure):
-    _fields_ = [("block", U32), ("waves_per_cu", U32), ("lds_stack", U32), ("refill", U32), ("width", U32)]
+    _fields_ = [("block", U32), ("waves_per_cu", U32), ("lds_stack", U32), ("refill", U32), ("width", U32), ("packet", U32)]
 
 
 class RenderConfig(ctypes.Structure):

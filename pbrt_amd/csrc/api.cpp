@@ -39,7 +39,7 @@ struct pbrt_mesh {
   uint2* spill = nullptr;
   uint64_t spill_entries = 0;
   int* status = nullptr;
-  pbrt_launch launch{128, 16, 16, 0, 2};
+  pbrt_launch launch{128, 16, 16, 0, 2, 0};
   std::mutex mu;
 };
 
@@ -67,6 +67,7 @@ LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
   a.width = m->launch.width == 4 ? 4 : 2;
   a.nodes4 = m->nodes4;
   a.root4 = m->root4;
+  a.packet = m->launch.packet;
   a.cu_count = (uint32_t)m->ctx->cu_count;
   return a;
 }
@@ -410,6 +411,10 @@ int pbrt_mesh_set_launch(pbrt_mesh* m, const pbrt_launch* l) {
     if (v.refill > 64) throw Error(PBRT_ERR_INVALID_ARG, "refill above 64 lanes");
     if (!v.width) v.width = 2;
     if (v.width != 2 && v.width != 4) throw Error(PBRT_ERR_INVALID_ARG, "width must be 2 or 4");
+    if (v.packet > 1) throw Error(PBRT_ERR_INVALID_ARG, "packet must be 0 or 1");
+    if (v.packet && v.width != 2) throw Error(PBRT_ERR_INVALID_ARG, "packet traversal walks the binary tree (width 2)");
+    if (v.packet && m->stats.max_depth > kPacketStackDepth + 1)
+      throw Error(PBRT_ERR_INVALID_ARG, "tree too deep for packet traversal");
     if (v.width == 4) {
       if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
       check_ctx(m->ctx);
@@ -474,34 +479,37 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
     pbrt_launch best{};
     float best_t = 1e30f;
     const int kind = any_hit ? 2 : 0;
+    auto time_launch = [&](const pbrt_launch& L) {
+      m->launch = L;
+      LaunchArgs a = base_args(m, n, nullptr);
+      a.rays = rays;
+      a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
+      a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
+      run(m, a, kind);  // warm-up
+      float cand = 1e30f;
+      for (uint32_t r = 0; r < reps; ++r) {
+        PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
+        run(m, a, kind);
+        PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
+        PBRT_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        cand = std::min(cand, ms);
+      }
+      if (cand < best_t) {
+        best_t = cand;
+        best = m->launch;
+      }
+    };
     try {
       for (uint32_t b : blocks)
         for (uint32_t d : depths)
           for (uint32_t w : waves)
             for (uint32_t f : refills)
-              for (uint32_t wd : widths) {
-                if (w < b / 64) continue;
-                m->launch = pbrt_launch{b, w, d, f, wd};
-                LaunchArgs a = base_args(m, n, nullptr);
-                a.rays = rays;
-                a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
-                a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
-                run(m, a, kind);  // warm-up
-                float cand = 1e30f;
-                for (uint32_t r = 0; r < reps; ++r) {
-                  PBRT_HIP(hipEventRecord(e0, a.stream ? (hipStream_t)a.stream : m->ctx->stream));
-                  run(m, a, kind);
-                  PBRT_HIP(hipEventRecord(e1, (hipStream_t)a.stream));
-                  PBRT_HIP(hipEventSynchronize(e1));
-                  float ms = 0.f;
-                  PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
-                  cand = std::min(cand, ms);
-                }
-                if (cand < best_t) {
-                  best_t = cand;
-                  best = m->launch;
-                }
-              }
+              for (uint32_t wd : widths)
+                if (w >= b / 64) time_launch(pbrt_launch{b, w, d, f, wd, 0});
+      if (!any_hit && m->stats.max_depth <= kPacketStackDepth + 1)
+        for (uint32_t w : waves) time_launch(pbrt_launch{64, w, 8, 0, 2, 1});
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -23,7 +23,11 @@ struct LaunchArgs {
   uint32_t width;   // 2: the binary tree; 4: the 4-wide tree (closest and any hit; counters always binary)
   const DevNode4* nodes4;
   int32_t root4;    // root reference in the 4-wide tree (the root box is root's)
+  uint32_t packet;  // 1: closest hit by wave packets on the binary tree (k_closest_packet)
 };
+// Pending subtrees the packet kernel's wave-wide stack holds: trees deeper than this need the
+// per-lane kernels.
+constexpr uint32_t kPacketStackDepth = 32;
 
 // kind: 0 closest hit, 1 closest hit with per-ray work counters, 2 any hit.
 hipError_t launch_traversal(const LaunchArgs& a, int kind);

@@ -163,6 +163,33 @@ def test_wide_tree_edge_cases_and_device_build(gpu_ctx):
     assert d.trace_host(rays).tobytes() == h.trace_host(rays).tobytes()
 
 
+@pytest.mark.parametrize("name", list(PUBLISHED))
+def test_packet_traversal_published_meshes(gpu_ctx, name):
+    """Wave packets (64 consecutive rays on one wave-wide stack) give the per-ray hits, on the
+    orbit rays (coherent) and on random rays (incoherent), with a batch that is not a multiple of
+    64 rays."""
+    v, f = load_mesh(name)
+    rays = po.orbit_rays(v).astype(np.float32)[:-5]
+    _, hits, _ = check_closest(gpu_ctx, v, f, rays, pbrt_amd.DEVICE_BVH, stats=False,
+                               launch=dict(packet=1, waves_per_cu=8))
+    assert int((hits["prim"] >= 0).sum()) >= PUBLISHED[name][0] - 5
+    rays = as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 30001, seed=3))
+    check_closest(gpu_ctx, v, f, rays, pbrt_amd.DEVICE_BVH, stats=False, launch=dict(packet=1))
+
+
+def test_packet_traversal_edge_cases(gpu_ctx):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)  # the root is a leaf
+    mesh.set_launch(packet=1)
+    rays = np.array([[0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, -1.0, 1000.0],
+                     [0.25, 0.25, 1.0, 1e-4, 0.0, 0.0, 1.0, 1000.0],
+                     [0.25, 0.25, 1.0, 2.0, 0.0, 0.0, -1.0, 1000.0]], np.float32)
+    assert mesh.trace_host(rays)["prim"].tolist() == [0, -1, -1]
+    with pytest.raises(pbrt_amd.PbrtError):
+        mesh.set_launch(packet=1, width=4)
+
+
 def test_incoherent_rays_on_displaced_icosphere(gpu_ctx):
     v, f = pbrt_amd.icosphere(6, noise="fbm", amp=0.08, octaves=3, seed=2)
     rays = as_rows(pbrt_amd.random_rays((0, 0, 0), 1.1, 100000, seed=4))
