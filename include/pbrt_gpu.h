@@ -222,6 +222,9 @@ int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double r
 #define PBRT_BUILD_HOST 0
 #define PBRT_BUILD_DEVICE 1
 int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
+/* Tree the scene walks inside its meshes: 2 (default) the binary trees, 4 trees collapsed to 4-wide
+ * nodes at commit (as pbrt_launch.width 4 for a mesh). Same hits and films either way. */
+int pbrt_scene_set_mesh_width(pbrt_scene* scene, uint32_t width);
 int pbrt_scene_build_seconds(pbrt_scene* scene, double* seconds);
 int pbrt_scene_commit(pbrt_scene* scene);
 int pbrt_scene_info(pbrt_scene* scene, uint32_t* objects, uint32_t* tlas_nodes, uint32_t* tlas_depth,

@@ -99,6 +99,7 @@ SIGNATURES = {
     "pbrt_scene_add_area_light": [P, U32, P],
     "pbrt_scene_commit": [P],
     "pbrt_scene_set_mesh_build": [P, I32],
+    "pbrt_scene_set_mesh_width": [P, U32],
     "pbrt_scene_build_seconds": [P, ctypes.POINTER(F64)],
     "pbrt_scene_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)],
     "pbrt_scene_trace_closest": [P, P, U32, P, P],

@@ -132,6 +132,47 @@ __device__ __forceinline__ bool tri_test(const RayState& r, const DevTri* __rest
   return true;
 }
 
+// The 4-wide node step shared by the mesh and scene kernels: the children's boxes tested over
+// [near, tmax], hit children ordered by entry distance (a box the ray misses sorts last with an
+// infinite key: a hit's entry is finite, at most tmax). Returns the number of children hit; the
+// nearest is in r0 / k0, the others in r1..r3 / k1..k3 in increasing distance.
+__device__ __forceinline__ int node4_step(const RayState& r, const DevNode4* __restrict__ nodes, int32_t cur, float tmax,
+                                          int32_t& r0, int32_t& r1, int32_t& r2, int32_t& r3, float& k0, float& k1,
+                                          float& k2, float& k3) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + cur);
+  const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+  const int4 rf = reinterpret_cast<const int4*>(p)[6];
+  const int cnt = reinterpret_cast<const int4*>(p)[7].x;
+  float d0, d1, d2, d3;
+  const bool h0 = box_test(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmax, &d0);
+  const bool h1 = box_test(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmax, &d1);
+  const bool h2 = cnt > 2 && box_test(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmax, &d2);
+  const bool h3 = cnt > 3 && box_test(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmax, &d3);
+  k0 = h0 ? d0 : INFINITY;
+  k1 = h1 ? d1 : INFINITY;
+  k2 = h2 ? d2 : INFINITY;
+  k3 = h3 ? d3 : INFINITY;
+  r0 = rf.x;
+  r1 = rf.y;
+  r2 = rf.z;
+  r3 = rf.w;
+  auto cswap = [](float& ka, int32_t& ra, float& kb, int32_t& rb) {
+    const bool sw = kb < ka;
+    const float tk = sw ? kb : ka;
+    kb = sw ? ka : kb;
+    ka = tk;
+    const int32_t tr = sw ? rb : ra;
+    rb = sw ? ra : rb;
+    ra = tr;
+  };
+  cswap(k0, r0, k1, r1);
+  cswap(k2, r2, k3, r3);
+  cswap(k0, r0, k2, r2);
+  cswap(k1, r1, k3, r3);
+  cswap(k1, r1, k2, r2);
+  return (int)h0 + (int)h1 + (int)h2 + (int)h3;
+}
+
 template <int BLOCK, int DEPTH>
 struct Stack {
   uint2 (*lds)[BLOCK];

@@ -413,6 +413,123 @@ __device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState
   return hit;
 }
 
+// The same walks over the mesh's 4-wide tree (scenes committed with width 4): one 4-wide node per
+// step, the nearest child next and the others pushed farthest first; same hits.
+template <int BLOCK, int DEPTH>
+__device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const RayState& r, float far,
+                              Stack<BLOCK, DEPTH>& st, int* status, float* t_out, float* u_out, float* v_out,
+                              uint32_t* prim_out) {
+  float entry;
+  if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], far,
+                &entry))
+    return false;
+  const int base = st.sp;
+  const DevNode4* nodes = S.blas4 + ob.node_base4;
+  const DevTri* tris = S.tris + ob.tri_base;
+  float min_t = FLT_MAX, cur_far = far;
+  uint32_t best = 0xffffffffu;
+  int32_t cur = ob.root_ref4;
+  bool live = true;
+  while (live) {
+    bool pop_now = false;
+    if (cur >= 0) {
+      int32_t r0, r1, r2, r3;
+      float k0, k1, k2, k3;
+      const int nh = node4_step(r, nodes, cur, cur_far, r0, r1, r2, r3, k0, k1, k2, k3);
+      if (nh == 0) {
+        pop_now = true;
+      } else {
+        if (nh > 3 && !push_or_flag(st, r3, k3, status)) break;
+        if (nh > 2 && !push_or_flag(st, r2, k2, status)) break;
+        if (nh > 1 && !push_or_flag(st, r1, k1, status)) break;
+        cur = r0;
+      }
+    }
+    if (!pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        float t, u, v;
+        uint32_t id;
+        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= far &&
+            (t < min_t || (t == min_t && id < best))) {
+          min_t = t;
+          best = id;
+          *t_out = t;
+          *u_out = u;
+          *v_out = v;
+          cur_far = fminf(far, min_t);
+        }
+      }
+      pop_now = true;
+    }
+    if (pop_now) {
+      live = false;
+      while (st.sp > base) {
+        const uint2 e = st.pop();
+        if (__uint_as_float(e.y) > min_t) continue;
+        cur = (int32_t)e.x;
+        live = true;
+        break;
+      }
+    }
+  }
+  st.sp = base;
+  *prim_out = best;
+  return best != 0xffffffffu;
+}
+
+template <int BLOCK, int DEPTH>
+__device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayState& r, Stack<BLOCK, DEPTH>& st,
+                          int* status) {
+  float entry;
+  if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], r.far,
+                &entry))
+    return false;
+  const int base = st.sp;
+  const DevNode4* nodes = S.blas4 + ob.node_base4;
+  const DevTri* tris = S.tris + ob.tri_base;
+  int32_t cur = ob.root_ref4;
+  bool hit = false, live = true;
+  while (live && !hit) {
+    bool pop_now = false;
+    if (cur >= 0) {
+      int32_t r0, r1, r2, r3;
+      float k0, k1, k2, k3;
+      const int nh = node4_step(r, nodes, cur, r.far, r0, r1, r2, r3, k0, k1, k2, k3);
+      if (nh == 0) {
+        pop_now = true;
+      } else {
+        if (nh > 3 && !push_or_flag(st, r3, 0.f, status)) break;
+        if (nh > 2 && !push_or_flag(st, r2, 0.f, status)) break;
+        if (nh > 1 && !push_or_flag(st, r1, 0.f, status)) break;
+        cur = r0;
+      }
+    }
+    if (!pop_now && cur < 0) {
+      const uint32_t code = ~(uint32_t)cur;
+      const uint32_t first = code >> kLeafCountBits;
+      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        float t, u, v;
+        uint32_t id;
+        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+          hit = true;
+          break;
+        }
+      }
+      pop_now = true;
+    }
+    if (pop_now) {
+      live = st.sp > base;
+      if (live) cur = (int32_t)st.pop().x;
+    }
+  }
+  st.sp = base;
+  return hit;
+}
+
 // Object::intersect for leaf-order object k on the world ray; far = the nearest hit so far.
 template <int BLOCK, int DEPTH>
 __device__ bool object_closest(const SceneView& S, uint32_t k, V wo, V wd, float near, float far,
@@ -435,7 +552,9 @@ __device__ bool object_closest(const SceneView& S, uint32_t k, V wo, V wd, float
     default: {
       const RayState r = ray_from(o, d, near, far);
       uint32_t id;
-      if (!mesh_closest(S, ob, r, far, st, status, t, u, v, &id)) return false;
+      if (S.blas4 ? !mesh_closest4(S, ob, r, far, st, status, t, u, v, &id)
+                  : !mesh_closest(S, ob, r, far, st, status, t, u, v, &id))
+        return false;
       *prim = (int32_t)id;
       return true;
     }
@@ -463,7 +582,9 @@ __device__ bool object_any(const SceneView& S, uint32_t k, V wo, V wd, float nea
       return box_t(o, d, ob.param, near, far, &t, &face);
     }
     case kShapeRect: return rect_t(o, d, ob.param[0], ob.param[1], near, far, &t);
-    default: return mesh_any(S, ob, ray_from(o, d, near, far), st, status);
+    default:
+      return S.blas4 ? mesh_any4(S, ob, ray_from(o, d, near, far), st, status)
+                     : mesh_any(S, ob, ray_from(o, d, near, far), st, status);
   }
 }
 

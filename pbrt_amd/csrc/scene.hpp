@@ -30,7 +30,8 @@ struct alignas(16) DevObject {
   float m[12], minv[12];
   int32_t node_base, tri_base, root_ref, mesh;
   float root_lo[4], root_hi[4];               // mesh BVH root box
-  uint32_t vert_base, idx_base, pad0, pad1;   // mesh vertices / indices for shading
+  uint32_t vert_base, idx_base;               // mesh vertices / indices for shading
+  int32_t node_base4, root_ref4;              // the mesh's 4-wide tree (scenes committed with width 4)
 };
 
 struct alignas(16) DevMaterial {
@@ -53,6 +54,7 @@ struct SceneView {
   RootDesc root;
   const DevObject* objs;
   const DevNode* blas;
+  const DevNode4* blas4;  // the meshes' 4-wide trees, or null (the binary trees are walked)
   const DevTri* tris;
   const float* verts;     // xyz per vertex, all meshes
   const uint32_t* idx;    // 3 per triangle, relative to the mesh's vert_base

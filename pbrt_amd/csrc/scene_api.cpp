@@ -67,6 +67,7 @@ struct pbrt_scene {
   pbrt_ctx* ctx = nullptr;
   pbrt_bvh_params mesh_params{12, 4, 1.0};
   bool device_build = false;  // mesh BVHs built on the device at commit (one batched build)
+  uint32_t mesh_width = 2;     // 4: the meshes' trees are collapsed to 4-wide nodes for the walks
   double build_seconds = 0.0;  // wall time of the mesh BVH builds at commit
   std::vector<std::vector<float>> mesh_xyz;
   std::vector<std::vector<uint32_t>> mesh_idx;
@@ -80,6 +81,7 @@ struct pbrt_scene {
   std::vector<HL> lights;
   bool committed = false;
   DevNode *tlas = nullptr, *blas = nullptr;
+  DevNode4* blas4 = nullptr;
   DevTri* tris = nullptr;
   DevObject* objs = nullptr;
   float* verts = nullptr;
@@ -113,6 +115,7 @@ SceneView view_of(const pbrt_scene* s) {
   v.root = s->root;
   v.objs = s->objs;
   v.blas = s->blas;
+  v.blas4 = s->blas4;
   v.tris = s->tris;
   v.verts = s->verts;
   v.idx = s->idx;
@@ -139,7 +142,8 @@ uint32_t persistent_grid(const pbrt_scene* s, uint32_t n) {
 
 // The HBM spill area for a grid: a lane's stack holds at most (top-level + mesh depth) entries.
 uint32_t ensure_spill(pbrt_scene* s, uint32_t grid) {
-  const uint32_t depth = s->tlas_depth + s->blas_depth + 2;
+  // pending entries of the 4-wide walk: at most three per level
+  const uint32_t depth = s->tlas_depth + (s->blas4 ? 3 : 1) * s->blas_depth + 2;
   const uint32_t per = depth > (uint32_t)kSceneStack ? depth - (uint32_t)kSceneStack : 1u;
   const uint64_t need = (uint64_t)per * grid * kSceneBlock;
   if (need > s->spill_entries) {
@@ -336,7 +340,7 @@ int pbrt_scene_destroy(pbrt_scene* s) {
   return guarded([&] {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
-    for (void* p : {(void*)s->tlas, (void*)s->blas, (void*)s->tris, (void*)s->objs, (void*)s->verts, (void*)s->idx,
+    for (void* p : {(void*)s->tlas, (void*)s->blas, (void*)s->blas4, (void*)s->tris, (void*)s->objs, (void*)s->verts, (void*)s->idx,
                     (void*)s->mats, (void*)s->dlights, (void*)s->dcsg, (void*)s->status, (void*)s->spill})
       if (p) (void)hipFree(p);
     free_paths(s->pb);
@@ -489,6 +493,14 @@ int pbrt_scene_set_mesh_build(pbrt_scene* s, int where) {
   });
 }
 
+int pbrt_scene_set_mesh_width(pbrt_scene* s, uint32_t width) {
+  return guarded([&] {
+    check_scene(s, false);
+    if (width != 2 && width != 4) throw Error(PBRT_ERR_INVALID_ARG, "mesh tree width must be 2 or 4");
+    s->mesh_width = width;
+  });
+}
+
 int pbrt_scene_build_seconds(pbrt_scene* s, double* seconds) {
   return guarded([&] {
     check_scene(s, true);
@@ -508,6 +520,8 @@ int pbrt_scene_commit(pbrt_scene* s) {
     std::vector<uint32_t> idx;
     struct MeshInfo {
       int32_t node_base, tri_base, root_ref;
+      int32_t node_base4 = 0, root_ref4 = 0;
+      uint64_t n_nodes = 0;
       float lo[3], hi[3];
       uint32_t vert_base, idx_base;
       Box64 bounds;
@@ -550,6 +564,7 @@ int pbrt_scene_commit(pbrt_scene* s) {
       for (size_t k = 0; k < nm; ++k) {
         MeshInfo& mi = info[k];
         mi.node_base = (int32_t)g.meshes[k].node_base;
+        mi.n_nodes = g.meshes[k].n_nodes;
         mi.tri_base = (int32_t)g.meshes[k].tri_base;
         mi.root_ref = g.meshes[k].root.ref;
         for (int a = 0; a < 3; ++a) {
@@ -594,6 +609,7 @@ int pbrt_scene_commit(pbrt_scene* s) {
         flatten_mesh_bvh(b, x.data(), ix.data(), &n, &t, &r);
         MeshInfo& mi = info[k];
         mi.node_base = (int32_t)blas.size();
+        mi.n_nodes = n.size();
         mi.tri_base = (int32_t)tris.size();
         mi.root_ref = r.ref;
         for (int a = 0; a < 3; ++a) {
@@ -603,6 +619,27 @@ int pbrt_scene_commit(pbrt_scene* s) {
         blas.insert(blas.end(), n.begin(), n.end());
         tris.insert(tris.end(), t.begin(), t.end());
       }
+    }
+    if (s->mesh_width == 4 && nm > 0) {
+      // the 4-wide trees, collapsed on the host from each mesh's binary nodes
+      std::vector<DevNode> all;
+      if (s->blas) {
+        uint64_t total = 0;
+        for (const MeshInfo& mi : info) total = std::max<uint64_t>(total, (uint64_t)mi.node_base + mi.n_nodes);
+        all.resize(total);
+        if (total) PBRT_HIP(hipMemcpy(all.data(), s->blas, total * sizeof(DevNode), hipMemcpyDeviceToHost));
+      }
+      const std::vector<DevNode>& src = s->blas ? all : blas;
+      std::vector<DevNode4> blas4;
+      for (size_t k = 0; k < nm; ++k) {
+        MeshInfo& mi = info[k];
+        const std::vector<DevNode> n2(src.begin() + mi.node_base, src.begin() + mi.node_base + mi.n_nodes);
+        std::vector<DevNode4> n4;
+        collapse_bvh4(n2, mi.root_ref, &n4, &mi.root_ref4);
+        mi.node_base4 = (int32_t)blas4.size();
+        blas4.insert(blas4.end(), n4.begin(), n4.end());
+      }
+      dev_upload(&s->blas4, blas4);
     }
     s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - build_t0).count();
     // object boxes (local bounds through the transform) and the median-split object BVH
@@ -666,6 +703,8 @@ int pbrt_scene_commit(pbrt_scene* s) {
         }
         d.vert_base = mi.vert_base;
         d.idx_base = mi.idx_base;
+        d.node_base4 = mi.node_base4;
+        d.root_ref4 = mi.root_ref4;
       }
     }
     std::vector<DevLight> dl;

@@ -311,3 +311,33 @@ def test_device_checkerboard_and_background_closed_forms(gpu_ctx):
     for mis in (False, True):
         film, _ = scene.render(16, 16, 1024, 2, tm.EYE, tm.LOOK, fov=tm.FOV, seed=3, mis=mis)
         np.testing.assert_allclose(film.reshape(-1, 3).mean(0), albedo, rtol=0.02)
+
+
+@pytest.mark.parametrize("name", ["bunny_mesh", "many_meshes", "room"])
+@pytest.mark.parametrize("device_build", [False, True])
+def test_wide_mesh_walk_same_hits_and_films(gpu_ctx, name, device_build):
+    """The scene walking 4-wide trees inside its meshes (pbrt_scene_set_mesh_width 4) gives the
+    binary walk's hits, occlusion and films bit for bit."""
+    if name == "room":
+        desc, cam = scenes.room()
+        eye, look, fov = cam["eye"], cam["look"], cam["fov"]
+    else:
+        sc, desc = stage(name)
+        eye, look, fov = sc["camera"]["pos"], sc["camera"]["look"], math.pi / 2
+    a = pbrt_amd.Scene(gpu_ctx, desc)
+    b = pbrt_amd.Scene(gpu_ctx, desc, mesh_width=4, device_build=device_build)
+    rays = po.pinhole_rays(160, 120, fov, eye, look).astype(np.float32)
+    rng = np.random.default_rng(4)
+    extra = rays.copy()
+    extra[:, 4:7] = rng.normal(size=(len(rays), 3)).astype(np.float32)  # same origins, random directions
+    for rr in (rays, extra):
+        ha, hb = a.trace_host(rr), b.trace_host(rr)
+        np.testing.assert_array_equal(ha["object"], hb["object"])
+        hit = ha["object"] >= 0
+        for field in ("prim", "t", "u", "v"):
+            np.testing.assert_array_equal(ha[field][hit], hb[field][hit], err_msg=field)
+        np.testing.assert_array_equal(a.trace_host(rr, any_hit=True), b.trace_host(rr, any_hit=True))
+    assert b.status() == 0
+    fa, _ = a.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
+    fb, _ = b.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
+    np.testing.assert_array_equal(fa, fb)
