@@ -225,7 +225,13 @@ def jobs_measure(pa, ctx):
         seq_ms += ms
         seq_rays += sj["rays"]
     paths = st["paths"]
+    # the batch again with the scene walk's settings tuned on renders of the first job's view
     j0 = jobs[0]
+    scene_launch = scene.autotune_render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"],
+                                         fov=j0["fov"], reps=2)
+    (_, st_t), tuned_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=True))
+    print(f"[bench] jobs scene launch tuned on job 0: {scene_launch}; batch {tuned_ms:.1f} ms", file=sys.stderr,
+          flush=True)
     f_host, _ = scene.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
     f_dev, _ = scene_dev.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
     builds = {"scene_s": build_s, "mesh_bvh_s": scene.build_seconds(), "device_scene_s": dev_build_s,
@@ -240,6 +246,7 @@ def jobs_measure(pa, ctx):
                     f"{sum(1 for o in desc.objects if o[0] == 4)} CSG objects; 8 jobs "
                     + ", ".join(f"{j['width']}x{j['height']}x{j['spp']}" for j in jobs) + ", max depth 8",
         "scene_gen_s": gen_s, "scene_build_s": build_s, "scene_builds": builds,
+        "tuned_scene_launch": scene_launch, "batch_tuned_launch_ms": tuned_ms,
         "batch_ms": batch_ms, "batch_mrays_s": st["rays"] / batch_ms / 1e3, "batch_samples_s": paths / batch_ms * 1e3,
         "batch_no_graphs_ms": eager_ms, "sequential_ms": seq_ms, "sequential_mrays_s": seq_rays / seq_ms / 1e3,
         "speedup_vs_sequential": seq_ms / batch_ms, "passes": st["passes"], "rays": st["rays"], "paths": paths,

@@ -225,6 +225,23 @@ int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
 /* Tree the scene walks inside its meshes: 2 (default) the binary trees, 4 trees collapsed to 4-wide
  * nodes at commit (as pbrt_launch.width 4 for a mesh). Same hits and films either way. */
 int pbrt_scene_set_mesh_width(pbrt_scene* scene, uint32_t width);
+/* Launch settings of the scene walks (pbrt_scene_trace_*, the renderer's extension and shadow rays):
+ * LDS stack entries per lane (8, 16, 24; deeper entries spill to HBM), workgroups of 128 threads
+ * per CU in the persistent grid (1-8; 0 = as many as fit), mesh tree width (2, 4; 4 collapses the
+ * trees on first use), min_waves (0: the renderer's extension walk at the compiler's register
+ * count; 5: its variant compiled for five waves per SIMD). Same hits and films for every setting.
+ * Defaults 16, 0, 2, 5. */
+typedef struct pbrt_scene_launch {
+  uint32_t lds_stack, blocks_per_cu, width, min_waves;
+} pbrt_scene_launch;
+int pbrt_scene_set_launch(pbrt_scene* scene, const pbrt_scene_launch* launch);
+int pbrt_scene_get_launch(pbrt_scene* scene, pbrt_scene_launch* out);
+/* Scene autotuner (no counterpart in the reference): times the committed scene's closest-hit walk
+ * on the caller's rays for every LDS stack depth x workgroups per CU (2, 4, 6, 8) x mesh tree
+ * width, `reps` timed runs each, and keeps the fastest (min_waves is left as set: the walk timed
+ * here has one variant). hits_dev (n entries) is overwritten. */
+int pbrt_scene_autotune(pbrt_scene* scene, const pbrt_ray* rays_dev, uint32_t n, pbrt_scene_hit* hits_dev,
+                        uint32_t reps, pbrt_scene_launch* chosen, float* best_ms);
 int pbrt_scene_build_seconds(pbrt_scene* scene, double* seconds);
 int pbrt_scene_commit(pbrt_scene* scene);
 int pbrt_scene_info(pbrt_scene* scene, uint32_t* objects, uint32_t* tlas_nodes, uint32_t* tlas_depth,
@@ -265,6 +282,11 @@ typedef struct pbrt_render_stats {
 /* film_out (host, may be NULL): width*height*3 doubles, the per-pixel mean radiance, accumulated
  * in FP64 in a fixed sample order. */
 int pbrt_render(pbrt_scene* scene, const pbrt_render_config* config, double* film_out, pbrt_render_stats* stats);
+/* The scene autotuner on renders: the candidates of pbrt_scene_autotune x min_waves (0, 5), each timed by a
+ * pbrt_render of `config` (no film; one warm-up and `reps` timed renders, host wall time), and the
+ * fastest kept. A short configuration (a few milliseconds) is enough. */
+int pbrt_scene_autotune_render(pbrt_scene* scene, const pbrt_render_config* config, uint32_t reps,
+                               pbrt_scene_launch* chosen, float* best_ms);
 
 /* Film output (host). pbrt_film_to_rgba8: width*height RGBA bytes of a film as the reference
  * writes its image — Spectrum::gamma_correct then to_rgb (spectrum.rs:13-30, renderers/mt.rs:83-90):

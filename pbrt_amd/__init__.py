@@ -383,6 +383,34 @@ class Scene:
         except Exception:
             pass
 
+    def set_launch(self, lds_stack=0, blocks_per_cu=0, width=0, min_waves=5):
+        """Launch settings of the scene walks (pbrt_scene_set_launch); same hits for every setting."""
+        call("pbrt_scene_set_launch", self.h,
+             ctypes.byref(_native.SceneLaunch(lds_stack, blocks_per_cu, width, min_waves)))
+
+    def get_launch(self):
+        l = _native.SceneLaunch()
+        call("pbrt_scene_get_launch", self.h, ctypes.byref(l))
+        return dict(lds_stack=l.lds_stack, blocks_per_cu=l.blocks_per_cu, width=l.width, min_waves=l.min_waves)
+
+    def autotune(self, rays, n, hits, reps=2):
+        """Times the scene walk's launch settings on these device rays and keeps the fastest."""
+        l = _native.SceneLaunch()
+        ms = ctypes.c_float()
+        call("pbrt_scene_autotune", self.h, rays.ptr, n, hits.ptr, reps, ctypes.byref(l), ctypes.byref(ms))
+        return dict(lds_stack=l.lds_stack, blocks_per_cu=l.blocks_per_cu, width=l.width, min_waves=l.min_waves,
+                    ms=ms.value)
+
+    def autotune_render(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
+                        near=1e-4, far=1000.0, reps=2, mis=False):
+        """The scene autotuner timed on short renders of this view (pbrt_scene_autotune_render)."""
+        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, 0, mis)
+        l = _native.SceneLaunch()
+        ms = ctypes.c_float()
+        call("pbrt_scene_autotune_render", self.h, ctypes.byref(c), reps, ctypes.byref(l), ctypes.byref(ms))
+        return dict(lds_stack=l.lds_stack, blocks_per_cu=l.blocks_per_cu, width=l.width, min_waves=l.min_waves,
+                    ms=ms.value)
+
     def build_seconds(self):
         """Wall time of the meshes' BVH builds in the commit."""
         t = ctypes.c_double()

@@ -31,6 +31,10 @@ class Launch(ctypes.Structure):
     _fields_ = [("block", U32), ("waves_per_cu", U32), ("lds_stack", U32), ("refill", U32), ("width", U32), ("packet", U32)]
 
 
+class SceneLaunch(ctypes.Structure):
+    _fields_ = [("lds_stack", U32), ("blocks_per_cu", U32), ("width", U32), ("min_waves", U32)]
+
+
 class RenderConfig(ctypes.Structure):
     _fields_ = [("width", U32), ("height", U32), ("spp", U32), ("max_depth", U32), ("seed", U64), ("fov", F64),
                 ("eye", F64 * 3), ("look", F64 * 3), ("up", F64 * 3), ("near", F64), ("far", F64),
@@ -100,6 +104,10 @@ SIGNATURES = {
     "pbrt_scene_commit": [P],
     "pbrt_scene_set_mesh_build": [P, I32],
     "pbrt_scene_set_mesh_width": [P, U32],
+    "pbrt_scene_set_launch": [P, ctypes.POINTER(SceneLaunch)],
+    "pbrt_scene_get_launch": [P, ctypes.POINTER(SceneLaunch)],
+    "pbrt_scene_autotune": [P, P, U32, P, U32, ctypes.POINTER(SceneLaunch), ctypes.POINTER(ctypes.c_float)],
+    "pbrt_scene_autotune_render": [P, P, U32, ctypes.POINTER(SceneLaunch), ctypes.POINTER(ctypes.c_float)],
     "pbrt_scene_build_seconds": [P, ctypes.POINTER(F64)],
     "pbrt_scene_info": [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)],
     "pbrt_scene_trace_closest": [P, P, U32, P, P],

@@ -1245,10 +1245,12 @@ __device__ __forceinline__ void count_rays(PathBuffers& b, uint32_t n) {
     atomicAdd(reinterpret_cast<unsigned long long*>(counter(b.counts, kCntRays)), (unsigned long long)n);
 }
 
-// Five waves per SIMD (96 registers): the scene walk is bound by the latency of dependent node
-// loads; the compiler's own choice (101 registers) allows four.
-template <int DEPTH>
-__global__ __launch_bounds__(kSceneBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_extend(SceneView S, PathBuffers b, int q, uint2* __restrict__ spill,
+// OCC5: compiled for five waves per SIMD (96 registers, a few spilled) instead of the compiler's
+// own choice (101 registers, four waves): the walk is bound by the latency of dependent node loads,
+// and the extra wave pays on mesh-heavy scenes, less so where CSG objects need the registers. The
+// scene's launch settings pick the variant.
+template <int DEPTH, bool OCC5>
+__global__ __launch_bounds__(kSceneBlock) __attribute__((amdgpu_waves_per_eu(OCC5 ? 5 : 1))) void k_extend(SceneView S, PathBuffers b, int q, uint2* __restrict__ spill,
                                                         uint32_t spill_cap, int* __restrict__ status) {
   __shared__ uint2 lds[DEPTH][kSceneBlock];
   Stack<kSceneBlock, DEPTH> st{lds, spill, gridDim.x * kSceneBlock, spill_cap, threadIdx.x,
@@ -1608,10 +1610,29 @@ uint32_t blocks_for(uint32_t n, uint32_t block) {
 
 }  // namespace
 
+// The scene walks come in three LDS stack depths; the scene's launch settings pick one.
+#define PBRT_SCENE_STACK(stack, CALL)            \
+  switch (stack) {                               \
+    case 8: {                                    \
+      constexpr int D = 8;                       \
+      CALL;                                      \
+    } break;                                     \
+    case 24: {                                   \
+      constexpr int D = 24;                      \
+      CALL;                                      \
+    } break;                                     \
+    default: {                                   \
+      constexpr int D = 16;                      \
+      CALL;                                      \
+    } break;                                     \
+  }
+
 hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit) {
   if (a.n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_scene_trace<kSceneStack>), dim3(a.grid), dim3(kSceneBlock), 0, (hipStream_t)a.stream, a.view,
-                     (const float4*)a.rays, a.hits, a.occluded, a.n, a.spill, a.spill_per_thread, a.status, any_hit);
+  PBRT_SCENE_STACK(a.view.stack,
+                   hipLaunchKernelGGL((k_scene_trace<D>), dim3(a.grid), dim3(kSceneBlock), 0, (hipStream_t)a.stream,
+                                      a.view, (const float4*)a.rays, a.hits, a.occluded, a.n, a.spill,
+                                      a.spill_per_thread, a.status, any_hit));
   return hipGetLastError();
 }
 
@@ -1643,21 +1664,29 @@ hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream) {
   return hipGetLastError();
 }
 
-uint32_t scene_blocks_per_cu() {
+uint32_t scene_blocks_per_cu(uint32_t stack) {
   // the fewest workgroups of the traversal kernels that fit on one CU at once: a persistent grid
   // larger than that would leave workgroups waiting for a slot and finishing last
   int a = 0, b = 0, c = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<kSceneStack>, kSceneBlock, 0) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<kSceneStack, false>, kSceneBlock, 0) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<kSceneStack, true>, kSceneBlock, 0) != hipSuccess)
-    return 0;
+  hipError_t e = hipSuccess;
+  PBRT_SCENE_STACK(stack, {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<D, true>, kSceneBlock, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<D, false>, kSceneBlock, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<D, true>, kSceneBlock, 0);
+  });
+  if (e != hipSuccess) return 0;
   return (uint32_t)(a < b ? (a < c ? a : c) : (b < c ? b : c));
 }
 
 hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream) {
-  hipLaunchKernelGGL((k_extend<kSceneStack>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, b, q, spill,
-                     spill_per_thread, status);
+  if (v.occ5) {
+    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_extend<D, true>), dim3(grid), dim3(kSceneBlock), 0,
+                                                 (hipStream_t)stream, v, b, q, spill, spill_per_thread, status));
+  } else {
+    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_extend<D, false>), dim3(grid), dim3(kSceneBlock), 0,
+                                                 (hipStream_t)stream, v, b, q, spill, spill_per_thread, status));
+  }
   return hipGetLastError();
 }
 
@@ -1676,15 +1705,15 @@ hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const Pat
 hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream) {
   RenderParams none{};
-  hipLaunchKernelGGL((k_shadow<kSceneStack, false>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, none,
-                     b, spill, spill_per_thread, status);
+  PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, false>), dim3(grid), dim3(kSceneBlock), 0,
+                                               (hipStream_t)stream, v, none, b, spill, spill_per_thread, status));
   return hipGetLastError();
 }
 
 hipError_t launch_shadow_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, uint2* spill,
                              uint32_t spill_per_thread, uint32_t grid, int* status, void* stream) {
-  hipLaunchKernelGGL((k_shadow<kSceneStack, true>), dim3(grid), dim3(kSceneBlock), 0, (hipStream_t)stream, v, p, b,
-                     spill, spill_per_thread, status);
+  PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, true>), dim3(grid), dim3(kSceneBlock), 0,
+                                               (hipStream_t)stream, v, p, b, spill, spill_per_thread, status));
   return hipGetLastError();
 }
 

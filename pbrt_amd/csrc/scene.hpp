@@ -64,6 +64,8 @@ struct SceneView {
   uint32_t n_objects, n_lights;
   int has_tlas;
   uint32_t n_background;  // background lights among the lights
+  uint32_t stack;         // LDS stack entries per lane of the scene walks: 8, 16 or 24 (kernel variant)
+  uint32_t occ5;          // 1: the extension walk's variant compiled for five waves per SIMD
 };
 
 // Closest hit at scene level: object (caller's index), primitive (mesh triangle or -1), distance
@@ -169,8 +171,9 @@ __host__ __device__ inline bool is_training_path(const RenderParams& p, uint32_t
 }
 
 hipError_t launch_generate(const SceneView& v, const RenderParams& p, const PathBuffers& b, void* stream);
-// Workgroups of the scene traversal kernels resident on one CU at once (0 if the runtime cannot tell).
-uint32_t scene_blocks_per_cu();
+// Workgroups of the scene traversal kernels resident on one CU at once, for an LDS stack of `stack`
+// entries (0 if the runtime cannot tell).
+uint32_t scene_blocks_per_cu(uint32_t stack);
 hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream);
 hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream);

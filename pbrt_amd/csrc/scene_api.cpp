@@ -67,7 +67,15 @@ struct pbrt_scene {
   pbrt_ctx* ctx = nullptr;
   pbrt_bvh_params mesh_params{12, 4, 1.0};
   bool device_build = false;  // mesh BVHs built on the device at commit (one batched build)
-  uint32_t mesh_width = 2;     // 4: the meshes' trees are collapsed to 4-wide nodes for the walks
+  pbrt_scene_launch launch{16, 0, 2, 5};  // scene walks: LDS stack, workgroups per CU, tree width, min waves
+  uint32_t blocks_fit[3] = {0, 0, 0};  // workgroups per CU that fit, per stack depth (8, 16, 24)
+  struct MeshRange {
+    uint32_t node_base;
+    uint64_t n_nodes;
+    int32_t root_ref;
+  };
+  std::vector<MeshRange> mesh_ranges;  // each mesh's binary nodes in blas (for a later collapse)
+  std::vector<DevObject> dobjs_host;   // the device objects as uploaded
   double build_seconds = 0.0;  // wall time of the mesh BVH builds at commit
   std::vector<std::vector<float>> mesh_xyz;
   std::vector<std::vector<uint32_t>> mesh_idx;
@@ -115,7 +123,9 @@ SceneView view_of(const pbrt_scene* s) {
   v.root = s->root;
   v.objs = s->objs;
   v.blas = s->blas;
-  v.blas4 = s->blas4;
+  v.blas4 = s->launch.width == 4 ? s->blas4 : nullptr;
+  v.stack = s->launch.lds_stack;
+  v.occ5 = s->launch.min_waves == 5 ? 1u : 0u;
   v.tris = s->tris;
   v.verts = s->verts;
   v.idx = s->idx;
@@ -129,12 +139,15 @@ SceneView view_of(const pbrt_scene* s) {
   return v;
 }
 
-uint32_t persistent_grid(const pbrt_scene* s, uint32_t n) {
-  // as many workgroups of 2 waves as fit on every CU at once (8 at most)
-  static const uint32_t per_cu = [] {
-    const uint32_t b = scene_blocks_per_cu();
-    return b ? std::min(b, 8u) : 8u;
-  }();
+uint32_t persistent_grid(pbrt_scene* s, uint32_t n) {
+  // as many workgroups of 2 waves as fit on every CU at once (8 at most), or the launch's count
+  const uint32_t k = s->launch.lds_stack == 8 ? 0 : s->launch.lds_stack == 24 ? 2 : 1;
+  if (!s->blocks_fit[k]) {
+    const uint32_t b = scene_blocks_per_cu(s->launch.lds_stack);
+    s->blocks_fit[k] = b ? std::min(b, 8u) : 8u;
+  }
+  const uint32_t per_cu = s->launch.blocks_per_cu ? std::min(s->launch.blocks_per_cu, s->blocks_fit[k])
+                                                  : s->blocks_fit[k];
   const uint32_t cap = (uint32_t)std::max(1, s->ctx->cu_count) * per_cu;
   const uint32_t need = (n + kSceneBlock - 1) / kSceneBlock;
   return std::max(1u, std::min(cap, need));
@@ -143,8 +156,9 @@ uint32_t persistent_grid(const pbrt_scene* s, uint32_t n) {
 // The HBM spill area for a grid: a lane's stack holds at most (top-level + mesh depth) entries.
 uint32_t ensure_spill(pbrt_scene* s, uint32_t grid) {
   // pending entries of the 4-wide walk: at most three per level
-  const uint32_t depth = s->tlas_depth + (s->blas4 ? 3 : 1) * s->blas_depth + 2;
-  const uint32_t per = depth > (uint32_t)kSceneStack ? depth - (uint32_t)kSceneStack : 1u;
+  const uint32_t depth = s->tlas_depth + (s->launch.width == 4 ? 3 : 1) * s->blas_depth + 2;
+  const uint32_t stack = s->launch.lds_stack;
+  const uint32_t per = depth > stack ? depth - stack : 1u;
   const uint64_t need = (uint64_t)per * grid * kSceneBlock;
   if (need > s->spill_entries) {
     if (s->spill) PBRT_HIP(hipFree(s->spill));
@@ -484,6 +498,83 @@ int pbrt_scene_add_area_light(pbrt_scene* s, uint32_t object, const double rgb[3
   });
 }
 
+// The 4-wide trees of every mesh, collapsed on the host from its binary nodes (host_blas, or the
+// device array when null), uploaded as one array; base4 / root4 receive each mesh's first node
+// and root reference in it.
+static void collapse_scene_meshes(pbrt_scene* s, const std::vector<DevNode>* host_blas, std::vector<int32_t>* base4,
+                                  std::vector<int32_t>* root4) {
+  const size_t nm = s->mesh_ranges.size();
+  std::vector<DevNode> all;
+  if (!host_blas) {
+    uint64_t total = 0;
+    for (const auto& r : s->mesh_ranges) total = std::max<uint64_t>(total, (uint64_t)r.node_base + r.n_nodes);
+    all.resize(total);
+    if (total) PBRT_HIP(hipMemcpy(all.data(), s->blas, total * sizeof(DevNode), hipMemcpyDeviceToHost));
+  }
+  const std::vector<DevNode>& src = host_blas ? *host_blas : all;
+  std::vector<DevNode4> blas4;
+  base4->assign(nm, 0);
+  root4->assign(nm, 0);
+  for (size_t k = 0; k < nm; ++k) {
+    const auto& r = s->mesh_ranges[k];
+    const std::vector<DevNode> n2(src.begin() + r.node_base, src.begin() + r.node_base + r.n_nodes);
+    std::vector<DevNode4> n4;
+    collapse_bvh4(n2, r.root_ref, &n4, &(*root4)[k]);
+    (*base4)[k] = (int32_t)blas4.size();
+    blas4.insert(blas4.end(), n4.begin(), n4.end());
+  }
+  DevNode4* d = nullptr;
+  PBRT_HIP(hipMalloc(&d, std::max<size_t>(1, blas4.size()) * sizeof(DevNode4)));
+  if (!blas4.empty()) {
+    const hipError_t e = hipMemcpy(d, blas4.data(), blas4.size() * sizeof(DevNode4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      PBRT_HIP(e);
+    }
+  }
+  s->blas4 = d;
+}
+
+// After commit: the 4-wide trees on first use, and the device objects' references to them.
+static void ensure_scene_blas4(pbrt_scene* s) {
+  if (s->blas4 || s->mesh_ranges.empty()) return;
+  std::vector<int32_t> base4, root4;
+  collapse_scene_meshes(s, nullptr, &base4, &root4);
+  for (DevObject& d : s->dobjs_host)
+    if (d.shape == kShapeMesh) {
+      d.node_base4 = base4[d.mesh];
+      d.root_ref4 = root4[d.mesh];
+    }
+  if (!s->dobjs_host.empty())
+    PBRT_HIP(hipMemcpy(s->objs, s->dobjs_host.data(), s->dobjs_host.size() * sizeof(DevObject),
+                       hipMemcpyHostToDevice));
+}
+
+int pbrt_scene_set_launch(pbrt_scene* s, const pbrt_scene_launch* l) {
+  return guarded([&] {
+    if (!s || !l) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    PBRT_HIP(hipSetDevice(s->ctx->device));
+    pbrt_scene_launch v = *l;
+    if (!v.lds_stack) v.lds_stack = 16;
+    if (!v.width) v.width = 2;
+    if (v.lds_stack != 8 && v.lds_stack != 16 && v.lds_stack != 24)
+      throw Error(PBRT_ERR_INVALID_ARG, "lds_stack must be 8, 16 or 24");
+    if (v.blocks_per_cu > 8) throw Error(PBRT_ERR_INVALID_ARG, "blocks_per_cu above 8");
+    if (v.width != 2 && v.width != 4) throw Error(PBRT_ERR_INVALID_ARG, "width must be 2 or 4");
+    if (v.min_waves != 0 && v.min_waves != 5) throw Error(PBRT_ERR_INVALID_ARG, "min_waves must be 0 or 5");
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->committed && v.width == 4) ensure_scene_blas4(s);
+    s->launch = v;
+  });
+}
+
+int pbrt_scene_get_launch(pbrt_scene* s, pbrt_scene_launch* out) {
+  return guarded([&] {
+    if (!s || !out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    *out = s->launch;
+  });
+}
+
 int pbrt_scene_set_mesh_build(pbrt_scene* s, int where) {
   return guarded([&] {
     check_scene(s, false);
@@ -497,7 +588,7 @@ int pbrt_scene_set_mesh_width(pbrt_scene* s, uint32_t width) {
   return guarded([&] {
     check_scene(s, false);
     if (width != 2 && width != 4) throw Error(PBRT_ERR_INVALID_ARG, "mesh tree width must be 2 or 4");
-    s->mesh_width = width;
+    s->launch.width = width;
   });
 }
 
@@ -620,26 +711,15 @@ int pbrt_scene_commit(pbrt_scene* s) {
         tris.insert(tris.end(), t.begin(), t.end());
       }
     }
-    if (s->mesh_width == 4 && nm > 0) {
-      // the 4-wide trees, collapsed on the host from each mesh's binary nodes
-      std::vector<DevNode> all;
-      if (s->blas) {
-        uint64_t total = 0;
-        for (const MeshInfo& mi : info) total = std::max<uint64_t>(total, (uint64_t)mi.node_base + mi.n_nodes);
-        all.resize(total);
-        if (total) PBRT_HIP(hipMemcpy(all.data(), s->blas, total * sizeof(DevNode), hipMemcpyDeviceToHost));
-      }
-      const std::vector<DevNode>& src = s->blas ? all : blas;
-      std::vector<DevNode4> blas4;
+    s->mesh_ranges.resize(nm);
+    for (size_t k = 0; k < nm; ++k) s->mesh_ranges[k] = {(uint32_t)info[k].node_base, info[k].n_nodes, info[k].root_ref};
+    if (s->launch.width == 4 && nm > 0) {
+      std::vector<int32_t> base4, root4;
+      collapse_scene_meshes(s, s->blas ? nullptr : &blas, &base4, &root4);
       for (size_t k = 0; k < nm; ++k) {
-        MeshInfo& mi = info[k];
-        const std::vector<DevNode> n2(src.begin() + mi.node_base, src.begin() + mi.node_base + mi.n_nodes);
-        std::vector<DevNode4> n4;
-        collapse_bvh4(n2, mi.root_ref, &n4, &mi.root_ref4);
-        mi.node_base4 = (int32_t)blas4.size();
-        blas4.insert(blas4.end(), n4.begin(), n4.end());
+        info[k].node_base4 = base4[k];
+        info[k].root_ref4 = root4[k];
       }
-      dev_upload(&s->blas4, blas4);
     }
     s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - build_t0).count();
     // object boxes (local bounds through the transform) and the median-split object BVH
@@ -739,6 +819,7 @@ int pbrt_scene_commit(pbrt_scene* s) {
       dev_upload(&s->idx, idx);
     }
     dev_upload(&s->objs, dobjs);
+    s->dobjs_host = dobjs;
     dev_upload(&s->mats, s->materials);
     dev_upload(&s->dlights, dl);
     PBRT_HIP(hipMalloc(&s->status, sizeof(int)));
@@ -787,6 +868,69 @@ int pbrt_scene_trace_any(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, uint8_
   return scene_trace(s, rays, n, nullptr, occ, stream, 1);
 }
 
+int pbrt_scene_autotune(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, pbrt_scene_hit* hits, uint32_t reps,
+                        pbrt_scene_launch* chosen, float* best_ms) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (!rays || !hits || !n) throw Error(PBRT_ERR_INVALID_ARG, "null argument or empty batch");
+    if (reps == 0) reps = 2;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      ensure_scene_blas4(s);
+    }
+    hipEvent_t e0, e1;
+    PBRT_HIP(hipEventCreate(&e0));
+    PBRT_HIP(hipEventCreate(&e1));
+    const pbrt_scene_launch prior = s->launch;
+    pbrt_scene_launch best = prior;
+    float best_t = 1e30f;
+    try {
+      hipStream_t st = s->ctx->stream;
+      for (uint32_t stack : {8u, 16u, 24u})
+        for (uint32_t blocks : {2u, 4u, 6u, 8u})
+          for (uint32_t width : {2u, 4u}) {
+            std::lock_guard<std::mutex> lock(s->mu);
+            s->launch = pbrt_scene_launch{stack, blocks, width, prior.min_waves};
+            SceneLaunch a{};
+            a.view = view_of(s);
+            a.rays = rays;
+            a.hits = reinterpret_cast<SceneHitRec*>(hits);
+            a.n = n;
+            a.grid = persistent_grid(s, n);
+            a.spill_per_thread = ensure_spill(s, a.grid);
+            a.spill = s->spill;
+            a.status = s->status;
+            a.stream = st;
+            PBRT_HIP(launch_scene_trace(a, 0));  // warm-up
+            float cand = 1e30f;
+            for (uint32_t r = 0; r < reps; ++r) {
+              PBRT_HIP(hipEventRecord(e0, st));
+              PBRT_HIP(launch_scene_trace(a, 0));
+              PBRT_HIP(hipEventRecord(e1, st));
+              PBRT_HIP(hipEventSynchronize(e1));
+              float ms = 0.f;
+              PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+              cand = std::min(cand, ms);
+            }
+            if (cand < best_t) {
+              best_t = cand;
+              best = s->launch;
+            }
+          }
+    } catch (...) {
+      s->launch = prior;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->launch = best;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+  });
+}
+
 int pbrt_scene_status(pbrt_scene* s, int* out) {
   return guarded([&] {
     check_scene(s, true);
@@ -800,6 +944,48 @@ int pbrt_scene_status(pbrt_scene* s, int* out) {
 // (f64 RGB, row-major), before the reference's gamma and 8-bit quantisation (spectrum.rs:13-30).
 int pbrt_render(pbrt_scene* s, const pbrt_render_config* c, double* film_out, pbrt_render_stats* stats) {
   return pbrt_render_ex(s, c, nullptr, nullptr, nullptr, film_out, stats);
+}
+
+int pbrt_scene_autotune_render(pbrt_scene* s, const pbrt_render_config* c, uint32_t reps, pbrt_scene_launch* chosen,
+                               float* best_ms) {
+  return guarded([&] {
+    check_scene(s, true);
+    if (!c) throw Error(PBRT_ERR_INVALID_ARG, "null config");
+    if (reps == 0) reps = 2;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      ensure_scene_blas4(s);
+    }
+    const pbrt_scene_launch prior = s->launch;
+    pbrt_scene_launch best = prior;
+    double best_s = 1e30;
+    for (uint32_t stack : {8u, 16u, 24u})
+      for (uint32_t blocks : {2u, 4u, 6u, 8u})
+        for (uint32_t width : {2u, 4u})
+          for (uint32_t waves : {0u, 5u}) {
+            {
+              std::lock_guard<std::mutex> lock(s->mu);
+              s->launch = pbrt_scene_launch{stack, blocks, width, waves};
+            }
+            double cand = 1e30;
+            for (uint32_t r = 0; r <= reps; ++r) {  // the first run warms up
+              pbrt_render_stats st{};
+              const int rc = pbrt_render(s, c, nullptr, &st);
+              if (rc != PBRT_OK) {
+                s->launch = prior;
+                throw Error(rc, pbrt_last_error());
+              }
+              if (r) cand = std::min(cand, st.seconds);
+            }
+            if (cand < best_s) {
+              best_s = cand;
+              best = s->launch;
+            }
+          }
+    s->launch = best;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = (float)(best_s * 1e3);
+  });
 }
 
 int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, const pbrt_nrc_render* nc,

@@ -341,3 +341,65 @@ def test_wide_mesh_walk_same_hits_and_films(gpu_ctx, name, device_build):
     fa, _ = a.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
     fb, _ = b.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
     np.testing.assert_array_equal(fa, fb)
+
+
+def _mixed_rays(eye, look, fov, w=160, h=120, seed=4):
+    """Camera rays and the same origins with seeded random directions."""
+    rays = po.pinhole_rays(w, h, fov, eye, look).astype(np.float32)
+    extra = rays.copy()
+    d = np.random.default_rng(seed).normal(size=(len(rays), 3))
+    extra[:, 4:7] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return np.concatenate([rays, extra])
+
+
+@pytest.mark.parametrize("launch", [dict(lds_stack=8, blocks_per_cu=2, width=2, min_waves=0),
+                                    dict(lds_stack=24, blocks_per_cu=0, width=4, min_waves=5),
+                                    dict(lds_stack=8, blocks_per_cu=8, width=4, min_waves=0)],
+                         ids=lambda l: f"s{l['lds_stack']}b{l['blocks_per_cu']}w{l['width']}m{l['min_waves']}")
+def test_scene_launch_settings_same_hits_and_films(gpu_ctx, launch):
+    """Every scene launch setting (LDS stack depth, workgroups per CU, mesh tree width, set after
+    commit) gives the default's hits, occlusion and films bit for bit."""
+    desc = scenes.batch_scene(n_meshes=24, n_csg=6)
+    eye, look, fov = (0.0, 8.0, 30.0), (0.0, 0.0, 0.0), math.pi / 3
+    a = pbrt_amd.Scene(gpu_ctx, desc)
+    b = pbrt_amd.Scene(gpu_ctx, desc)
+    b.set_launch(**launch)
+    assert b.get_launch() == launch
+    rays = _mixed_rays(eye, look, fov)
+    ha, hb = a.trace_host(rays), b.trace_host(rays)
+    np.testing.assert_array_equal(ha["object"], hb["object"])
+    hit = ha["object"] >= 0
+    assert hit.mean() > 0.05
+    for field in ("prim", "t", "u", "v"):
+        np.testing.assert_array_equal(ha[field][hit], hb[field][hit], err_msg=field)
+    np.testing.assert_array_equal(a.trace_host(rays, any_hit=True), b.trace_host(rays, any_hit=True))
+    fa, _ = a.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
+    fb, _ = b.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
+    np.testing.assert_array_equal(fa, fb)
+    assert b.status() == 0
+
+
+def test_scene_autotune_keeps_results(gpu_ctx):
+    desc, cam = scenes.room()
+    a = pbrt_amd.Scene(gpu_ctx, desc)
+    b = pbrt_amd.Scene(gpu_ctx, desc)
+    rays = _mixed_rays(cam["eye"], cam["look"], cam["fov"], 256, 256)
+    dr = gpu_ctx.upload(rays)
+    dh = gpu_ctx.alloc(32 * len(rays))
+    tuned = b.autotune(dr, len(rays), dh, reps=1)
+    assert tuned["lds_stack"] in (8, 16, 24) and tuned["width"] in (2, 4) and tuned["ms"] > 0
+    assert b.get_launch() == {k: tuned[k] for k in ("lds_stack", "blocks_per_cu", "width", "min_waves")}
+    fa, _ = a.render(64, 64, 4, 8, cam["eye"], cam["look"], fov=cam["fov"], seed=2)
+    fb, _ = b.render(64, 64, 4, 8, cam["eye"], cam["look"], fov=cam["fov"], seed=2)
+    np.testing.assert_array_equal(fa, fb)
+
+
+def test_scene_autotune_on_renders_keeps_results(gpu_ctx):
+    desc, cam = scenes.small()
+    a = pbrt_amd.Scene(gpu_ctx, desc)
+    b = pbrt_amd.Scene(gpu_ctx, desc)
+    tuned = b.autotune_render(64, 64, 2, 8, cam["eye"], cam["look"], fov=cam["fov"], reps=1)
+    assert b.get_launch() == {k: tuned[k] for k in ("lds_stack", "blocks_per_cu", "width", "min_waves")}
+    fa, _ = a.render(64, 64, 4, 8, cam["eye"], cam["look"], fov=cam["fov"], seed=3)
+    fb, _ = b.render(64, 64, 4, 8, cam["eye"], cam["look"], fov=cam["fov"], seed=3)
+    np.testing.assert_array_equal(fa, fb)
