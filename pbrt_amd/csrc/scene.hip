@@ -531,7 +531,7 @@ __device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayStat
 }
 
 // Object::intersect for leaf-order object k on the world ray; far = the nearest hit so far.
-template <int BLOCK, int DEPTH>
+template <int BLOCK, int DEPTH, bool WIDE>
 __device__ bool object_closest(const SceneView& S, uint32_t k, V wo, V wd, float near, float far,
                                Stack<BLOCK, DEPTH>& st, int* status, float* t, float* u, float* v, int32_t* prim,
                                int32_t* face) {
@@ -552,16 +552,18 @@ __device__ bool object_closest(const SceneView& S, uint32_t k, V wo, V wd, float
     default: {
       const RayState r = ray_from(o, d, near, far);
       uint32_t id;
-      if (S.blas4 ? !mesh_closest4(S, ob, r, far, st, status, t, u, v, &id)
-                  : !mesh_closest(S, ob, r, far, st, status, t, u, v, &id))
-        return false;
+      if constexpr (WIDE) {
+        if (!mesh_closest4(S, ob, r, far, st, status, t, u, v, &id)) return false;
+      } else {
+        if (!mesh_closest(S, ob, r, far, st, status, t, u, v, &id)) return false;
+      }
       *prim = (int32_t)id;
       return true;
     }
   }
 }
 
-template <int BLOCK, int DEPTH>
+template <int BLOCK, int DEPTH, bool WIDE>
 __device__ bool object_any(const SceneView& S, uint32_t k, V wo, V wd, float near, float far, Stack<BLOCK, DEPTH>& st,
                            int* status) {
   const DevObject& ob = S.objs[k];
@@ -583,13 +585,16 @@ __device__ bool object_any(const SceneView& S, uint32_t k, V wo, V wd, float nea
     }
     case kShapeRect: return rect_t(o, d, ob.param[0], ob.param[1], near, far, &t);
     default:
-      return S.blas4 ? mesh_any4(S, ob, ray_from(o, d, near, far), st, status)
-                     : mesh_any(S, ob, ray_from(o, d, near, far), st, status);
+      if constexpr (WIDE)
+        return mesh_any4(S, ob, ray_from(o, d, near, far), st, status);
+      else
+        return mesh_any(S, ob, ray_from(o, d, near, far), st, status);
   }
 }
 
 // Closest hit over the scene (bvh.rs:153-230 + scene.rs:186-220).
-template <int BLOCK, int DEPTH>
+// WIDE: the meshes' 4-wide trees are walked (the scene has them); the binary trees otherwise.
+template <int BLOCK, int DEPTH, bool WIDE>
 __device__ bool scene_closest(const SceneView& S, V o, V d, float near, float far_in, Stack<BLOCK, DEPTH>& st,
                               int* status, SceneHitRec* out) {
   out->obj = -1;
@@ -637,7 +642,8 @@ __device__ bool scene_closest(const SceneView& S, V o, V d, float near, float fa
       for (uint32_t k = first; k < first + cnt; ++k) {
         float t, u, v;
         int32_t prim, face = 0;
-        if (!object_closest(S, k, o, d, near, min_t, st, status, &t, &u, &v, &prim, &face)) continue;
+        if (!object_closest<BLOCK, DEPTH, WIDE>(S, k, o, d, near, min_t, st, status, &t, &u, &v, &prim, &face))
+          continue;
         const int32_t orig = S.objs[k].orig;
         const bool take = t < min_t || (t == min_t && (orig < best_orig || (orig == best_orig && prim < best_prim)));
         if (!take) continue;
@@ -670,7 +676,7 @@ __device__ bool scene_closest(const SceneView& S, V o, V d, float near, float fa
 }
 
 // Any hit over the scene (bvh.rs:239-275 + scene.rs:157-168).
-template <int BLOCK, int DEPTH>
+template <int BLOCK, int DEPTH, bool WIDE>
 __device__ bool scene_any(const SceneView& S, V o, V d, float near, float far_in, Stack<BLOCK, DEPTH>& st,
                           int* status) {
   if (!S.has_tlas) return false;
@@ -708,7 +714,7 @@ __device__ bool scene_any(const SceneView& S, V o, V d, float near, float far_in
       const uint32_t first = code >> kLeafCountBits;
       const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
       for (uint32_t k = first; k < first + cnt; ++k)
-        if (object_any(S, k, o, d, near, far, st, status)) return true;
+        if (object_any<BLOCK, DEPTH, WIDE>(S, k, o, d, near, far, st, status)) return true;
       pop_now = true;
     }
     if (pop_now) {
@@ -718,7 +724,7 @@ __device__ bool scene_any(const SceneView& S, V o, V d, float near, float far_in
   }
 }
 
-template <int DEPTH>
+template <int DEPTH, bool WIDE>
 __global__ __launch_bounds__(kSceneBlock) void k_scene_trace(SceneView S, const float4* __restrict__ rays,
                                                              SceneHitRec* __restrict__ hits,
                                                              uint8_t* __restrict__ occluded, uint32_t n,
@@ -731,10 +737,10 @@ __global__ __launch_bounds__(kSceneBlock) void k_scene_trace(SceneView S, const 
     const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
     const V o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
     if (any_hit) {
-      occluded[i] = scene_any(S, o, d, a.w, b.w, st, status) ? 1 : 0;
+      occluded[i] = scene_any<kSceneBlock, DEPTH, WIDE>(S, o, d, a.w, b.w, st, status) ? 1 : 0;
     } else {
       SceneHitRec h;
-      scene_closest(S, o, d, a.w, b.w, st, status, &h);
+      scene_closest<kSceneBlock, DEPTH, WIDE>(S, o, d, a.w, b.w, st, status, &h);
       hits[i] = h;
     }
   }
@@ -1249,7 +1255,7 @@ __device__ __forceinline__ void count_rays(PathBuffers& b, uint32_t n) {
 // own choice (101 registers, four waves): the walk is bound by the latency of dependent node loads,
 // and the extra wave pays on mesh-heavy scenes, less so where CSG objects need the registers. The
 // scene's launch settings pick the variant.
-template <int DEPTH, bool OCC5>
+template <int DEPTH, bool OCC5, bool WIDE>
 __global__ __launch_bounds__(kSceneBlock) __attribute__((amdgpu_waves_per_eu(OCC5 ? 5 : 1))) void k_extend(SceneView S, PathBuffers b, int q, uint2* __restrict__ spill,
                                                         uint32_t spill_cap, int* __restrict__ status) {
   __shared__ uint2 lds[DEPTH][kSceneBlock];
@@ -1261,7 +1267,7 @@ __global__ __launch_bounds__(kSceneBlock) __attribute__((amdgpu_waves_per_eu(OCC
     const uint32_t j = b.queue[q][i];
     const float4 o = b.ray_o[j], d = b.ray_d[j];
     SceneHitRec h;
-    scene_closest(S, V{o.x, o.y, o.z}, V{d.x, d.y, d.z}, o.w, d.w, st, status, &h);
+    scene_closest<kSceneBlock, DEPTH, WIDE>(S, V{o.x, o.y, o.z}, V{d.x, d.y, d.z}, o.w, d.w, st, status, &h);
     b.hit[j] = h;
   }
 }
@@ -1500,7 +1506,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
 }
 
 // VisibilityTester::unoccluded (lights.rs:67-69): add the contribution when nothing blocks it.
-template <int DEPTH, bool NRC>
+template <int DEPTH, bool NRC, bool WIDE>
 __global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, RenderParams p, PathBuffers b,
                                                         uint2* __restrict__ spill, uint32_t spill_cap,
                                                         int* __restrict__ status) {
@@ -1511,7 +1517,7 @@ __global__ __launch_bounds__(kSceneBlock) void k_shadow(SceneView S, RenderParam
   count_rays(b, n);
   for (uint32_t i = st.gtid; i < n; i += gridDim.x * kSceneBlock) {
     const float4 o = b.sh_o[i], d = b.sh_d[i], c = b.sh_c[i];
-    if (scene_any(S, V{o.x, o.y, o.z}, V{d.x, d.y, d.z}, o.w, d.w, st, status)) continue;
+    if (scene_any<kSceneBlock, DEPTH, WIDE>(S, V{o.x, o.y, o.z}, V{d.x, d.y, d.z}, o.w, d.w, st, status)) continue;
     const uint32_t j = __float_as_uint(c.w);
     float4 L = b.L[j];
     L.x = L.x + c.x;
@@ -1629,10 +1635,17 @@ uint32_t blocks_for(uint32_t n, uint32_t block) {
 
 hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit) {
   if (a.n == 0) return hipSuccess;
-  PBRT_SCENE_STACK(a.view.stack,
-                   hipLaunchKernelGGL((k_scene_trace<D>), dim3(a.grid), dim3(kSceneBlock), 0, (hipStream_t)a.stream,
-                                      a.view, (const float4*)a.rays, a.hits, a.occluded, a.n, a.spill,
-                                      a.spill_per_thread, a.status, any_hit));
+  if (a.view.blas4) {
+    PBRT_SCENE_STACK(a.view.stack,
+                     hipLaunchKernelGGL((k_scene_trace<D, true>), dim3(a.grid), dim3(kSceneBlock), 0,
+                                        (hipStream_t)a.stream, a.view, (const float4*)a.rays, a.hits, a.occluded, a.n,
+                                        a.spill, a.spill_per_thread, a.status, any_hit));
+  } else {
+    PBRT_SCENE_STACK(a.view.stack,
+                     hipLaunchKernelGGL((k_scene_trace<D, false>), dim3(a.grid), dim3(kSceneBlock), 0,
+                                        (hipStream_t)a.stream, a.view, (const float4*)a.rays, a.hits, a.occluded, a.n,
+                                        a.spill, a.spill_per_thread, a.status, any_hit));
+  }
   return hipGetLastError();
 }
 
@@ -1670,9 +1683,10 @@ uint32_t scene_blocks_per_cu(uint32_t stack) {
   int a = 0, b = 0, c = 0;
   hipError_t e = hipSuccess;
   PBRT_SCENE_STACK(stack, {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<D, true>, kSceneBlock, 0);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<D, false>, kSceneBlock, 0);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<D, true>, kSceneBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<D, true, false>, kSceneBlock, 0);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<D, false, false>, kSceneBlock, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shadow<D, true, false>, kSceneBlock, 0);
   });
   if (e != hipSuccess) return 0;
   return (uint32_t)(a < b ? (a < c ? a : c) : (b < c ? b : c));
@@ -1680,13 +1694,19 @@ uint32_t scene_blocks_per_cu(uint32_t stack) {
 
 hipError_t launch_extend(const SceneView& v, const PathBuffers& b, int q, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream) {
-  if (v.occ5) {
-    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_extend<D, true>), dim3(grid), dim3(kSceneBlock), 0,
-                                                 (hipStream_t)stream, v, b, q, spill, spill_per_thread, status));
+#define PBRT_EXTEND(OCC, WIDE)                                                                             \
+  PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_extend<D, OCC, WIDE>), dim3(grid), dim3(kSceneBlock), 0, \
+                                               (hipStream_t)stream, v, b, q, spill, spill_per_thread, status))
+  if (v.occ5 && v.blas4) {
+    PBRT_EXTEND(true, true);
+  } else if (v.occ5) {
+    PBRT_EXTEND(true, false);
+  } else if (v.blas4) {
+    PBRT_EXTEND(false, true);
   } else {
-    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_extend<D, false>), dim3(grid), dim3(kSceneBlock), 0,
-                                                 (hipStream_t)stream, v, b, q, spill, spill_per_thread, status));
+    PBRT_EXTEND(false, false);
   }
+#undef PBRT_EXTEND
   return hipGetLastError();
 }
 
@@ -1705,15 +1725,25 @@ hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const Pat
 hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill, uint32_t spill_per_thread,
                          uint32_t grid, int* status, void* stream) {
   RenderParams none{};
-  PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, false>), dim3(grid), dim3(kSceneBlock), 0,
-                                               (hipStream_t)stream, v, none, b, spill, spill_per_thread, status));
+  if (v.blas4) {
+    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, false, true>), dim3(grid), dim3(kSceneBlock), 0,
+                                                 (hipStream_t)stream, v, none, b, spill, spill_per_thread, status));
+  } else {
+    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, false, false>), dim3(grid), dim3(kSceneBlock), 0,
+                                                 (hipStream_t)stream, v, none, b, spill, spill_per_thread, status));
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_shadow_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, uint2* spill,
                              uint32_t spill_per_thread, uint32_t grid, int* status, void* stream) {
-  PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, true>), dim3(grid), dim3(kSceneBlock), 0,
-                                               (hipStream_t)stream, v, p, b, spill, spill_per_thread, status));
+  if (v.blas4) {
+    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, true, true>), dim3(grid), dim3(kSceneBlock), 0,
+                                                 (hipStream_t)stream, v, p, b, spill, spill_per_thread, status));
+  } else {
+    PBRT_SCENE_STACK(v.stack, hipLaunchKernelGGL((k_shadow<D, true, false>), dim3(grid), dim3(kSceneBlock), 0,
+                                                 (hipStream_t)stream, v, p, b, spill, spill_per_thread, status));
+  }
   return hipGetLastError();
 }
 
