@@ -257,7 +257,8 @@ def render_measure(pa, ctx):
     """Path-tracing throughput of the wavefront renderer on the two image configurations: the small
     scene (5,120-triangle icosphere, floor, 64 spheres, point light; 256^2, 16 spp, depth 8) and the
     room (walls, emissive ceiling panel as an area light, two boxes, an 81,920-triangle icosphere;
-    512^2, 64 spp, depth 8), each with the reference's estimator and with MIS. GPU time from the
+    512^2, 64 spp, depth 8), each with the reference's estimator, with MIS and with Russian
+    roulette. GPU time from the
     render's own progress events (first launch to the last pass)."""
     from pbrt_amd import scenes
     out = {}
@@ -266,17 +267,18 @@ def render_measure(pa, ctx):
         scene = pa.Scene(ctx, desc)
         scene.render(64, 64, 1, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"])  # warm-up
         row = {}
-        for mis in (False, True):
+        for key, opt in (("reference_estimator", {}), ("mis", dict(mis=True)), ("roulette", dict(roulette=True))):
             _, st, pr = scene.render_ex(w, h, spp, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=1,
-                                        mis=mis)
+                                        **opt)
             ms = float(pr["ms"][-1])
-            row["mis" if mis else "reference_estimator"] = {
+            row[key] = {
                 "ms": ms, "mrays_s": st["rays"] / ms / 1e3, "samples_s": w * h * spp / ms * 1e3,
                 "rays_per_sample": st["rays"] / (w * h * spp), "passes": st["passes"]}
         out[name] = row
         print(f"[bench] render {name}: {row['reference_estimator']['mrays_s']:.0f} Mrays/s, "
               f"{row['reference_estimator']['samples_s'] / 1e6:.1f} M samples/s (MIS "
-              f"{row['mis']['mrays_s']:.0f} Mrays/s)", file=sys.stderr, flush=True)
+              f"{row['mis']['mrays_s']:.0f} Mrays/s; roulette {row['roulette']['samples_s'] / 1e6:.1f} M samples/s)",
+              file=sys.stderr, flush=True)
     return out
 
 

@@ -269,6 +269,11 @@ typedef struct pbrt_render_config {
                                 estimation, emission only after a specular bounce); 1: multiple
                                 importance sampling of next-event estimation and the BSDF sample,
                                 power heuristic (the reference's plan, ideas/area_light.md:158-165) */
+  uint32_t roulette;         /* 1: Russian roulette after each scattering, as integrators/path.rs:93-100
+                                has it commented out (continue with probability q = the throughput's
+                                largest component, capped at 1 here, the throughput divided by q);
+                                not with the radiance cache. 0: paths end only at max_depth */
+  uint32_t reserved;
 } pbrt_render_config;
 
 typedef struct pbrt_render_stats {

@@ -410,6 +410,13 @@ int orc_scene_set_mis(void* hv, int mis) {
   return 0;
 }
 
+int orc_scene_set_roulette(void* hv, int on) {
+  auto* h = (SceneHandle*)hv;
+  if (h->a) h->a->roulette = on != 0;
+  if (h->b) h->b->roulette = on != 0;
+  return 0;
+}
+
 // Rows [y0, y1) of a width x height render: film receives the per-pixel mean radiance (f64 RGB).
 int orc_scene_render(void* hv, uint32_t w, uint32_t hgt, double fov, const double* eye, const double* look, uint32_t spp,
                      uint32_t max_depth, uint64_t seed, double near, double far, uint32_t y0, uint32_t y1, double* film,

@@ -147,6 +147,8 @@ def _scene_sigs(L):
         fn.argtypes = [P, P, U64, ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int]
     L.orc_scene_set_mis.restype = ctypes.c_int
     L.orc_scene_set_mis.argtypes = [P, ctypes.c_int]
+    L.orc_scene_set_roulette.restype = ctypes.c_int
+    L.orc_scene_set_roulette.argtypes = [P, ctypes.c_int]
     L.orc_scene_render.restype = ctypes.c_int
     L.orc_scene_render.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, ctypes.c_uint32,
                                    ctypes.c_uint32, U64, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
@@ -218,8 +220,9 @@ class Scene:
                     primary=st[:3] / out[0], shadow=st[3:] / max(int(out[1]), 1))
 
     def render(self, w, h, fov, eye, look, spp, max_depth, seed, near=1e-4, far=1000.0, rows=None, threads=8,
-               mis=False):
+               mis=False, roulette=False):
         lib().orc_scene_set_mis(self.h, int(bool(mis)))
+        lib().orc_scene_set_roulette(self.h, int(bool(roulette)))
         y0, y1 = rows if rows is not None else (0, h)
         film = np.zeros(((y1 - y0) * w, 3), np.float64)
         rays = ctypes.c_uint64()

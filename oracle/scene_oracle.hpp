@@ -63,7 +63,8 @@ inline float uni<float>(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t 
 }
 enum Slot : uint32_t {
   kJitterX = 0, kJitterY = 1, kLightPick = 2, kLightU = 3, kLightV = 4, kScatter0 = 5, kScatter1 = 6, kCoin = 7,
-  kMetal0 = 8  // Metal's point in the unit ball: attempt k uses slots kMetal0 + 3k .. + 2
+  kMetal0 = 8,  // Metal's point in the unit ball: attempt k uses slots kMetal0 + 3k .. + 2
+  kRoulette = 8 + 3 * 64  // Russian roulette's draw, past the 64 Metal attempts
 };
 constexpr uint32_t kMetalAttempts = 64;
 
@@ -176,6 +177,7 @@ class SceneOracle {
   std::vector<LightDesc> lights;
   bool canonical;
   bool mis = false;  // multiple importance sampling (power heuristic) instead of the specular guard
+  bool roulette = false;  // Russian roulette after each scattering (path.rs:93-100, commented out there)
   // scene BVH in the reference's layout (bvh.rs:15-31)
   struct SNode {
     AABB<double> box;
@@ -1041,6 +1043,13 @@ class SceneOracle {
       // a non-finite throughput (grazing specular event, 1/|cos| = inf) ends the path, as on the
       // device; the reference would carry a NaN
       if (!(std::isfinite(beta.c[0]) && std::isfinite(beta.c[1]) && std::isfinite(beta.c[2]))) break;
+      if (roulette) {
+        // path.rs:93-100: continue with probability q, the throughput's largest component (capped
+        // at 1 here, so that the estimator stays unbiased), and divide the throughput by q
+        const R q = std::fmin(R(1), std::fmax(std::fmax(beta.c[0], beta.c[1]), beta.c[2]));
+        if (uni<R>(seed, path, bounce, kRoulette) > q) break;
+        beta = sdiv(beta, q);
+      }
       ray = s.ray;
       specular = s.specular;
       prev_pdf = s.pdf;

@@ -448,19 +448,23 @@ class Scene:
         return dh.read(SCENE_HIT_DTYPE, n)
 
     @staticmethod
-    def _config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis=False):
+    def _config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis=False,
+                roulette=False):
         c = _native.RenderConfig()
         c.width, c.height, c.spp, c.max_depth, c.seed, c.fov = width, height, spp, max_depth, seed, fov
         c.eye[:], c.look[:], c.up[:] = list(eye), list(look), list(up)
         c.near, c.far, c.samples_per_pass = near, far, samples_per_pass
         c.mis = int(bool(mis))
+        c.roulette = int(bool(roulette))
         return c
 
     def render(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
-               near=1e-4, far=1000.0, samples_per_pass=0, film=True, mis=False):
+               near=1e-4, far=1000.0, samples_per_pass=0, film=True, mis=False, roulette=False):
         """Per-pixel mean radiance (height, width, 3) float64 and the render statistics; `mis`:
-        multiple importance sampling instead of the reference's estimator."""
-        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis)
+        multiple importance sampling instead of the reference's estimator; `roulette`: Russian
+        roulette after each scattering."""
+        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis,
+                         roulette)
         out = np.zeros((height, width, 3), np.float64) if film else None
         st = _native.RenderStats()
         call("pbrt_render", self.h, ctypes.byref(c), _ptr(out) if film else None, ctypes.byref(st))
@@ -476,7 +480,8 @@ class Scene:
         for k, j in enumerate(jobs):
             cfgs[k] = self._config(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"], j["look"],
                                    j.get("up", (0.0, 1.0, 0.0)), j.get("fov", np.pi / 2), j.get("seed", 1),
-                                   j.get("near", 1e-4), j.get("far", 1000.0), 0, j.get("mis", False))
+                                   j.get("near", 1e-4), j.get("far", 1000.0), 0, j.get("mis", False),
+                                   j.get("roulette", False))
             films.append(np.zeros((j["height"], j["width"], 3), np.float64))
         ptrs = (ctypes.c_void_p * n)(*[f.ctypes.data for f in films])
         st = _native.RenderStats()
@@ -485,12 +490,14 @@ class Scene:
 
     def render_ex(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
                   near=1e-4, far=1000.0, samples_per_pass=0, cache=None, query_vertex=3, train_stride=33,
-                  train_batch=65536, train_steps=1, reference=None, record=True, debug_slots=0, mis=False):
+                  train_batch=65536, train_steps=1, reference=None, record=True, debug_slots=0, mis=False,
+                  roulette=False):
         """Render with the radiance cache `cache` (a RadianceCache, or None for plain path tracing)
         and, with `record`, the per-pass time (ms) and RMSE against `reference` (height, width, 3).
         Returns (film, stats, progress); progress has `ms`, `rmse` (or None) and `spp` per pass, and
         `debug` (raw inputs, targets) of the last pass's training slots when debug_slots > 0."""
-        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis)
+        c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis,
+                         roulette)
         out = np.zeros((height, width, 3), np.float64)
         st = _native.RenderStats()
         nc = None

@@ -891,7 +891,8 @@ __device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bo
 }
 enum : uint32_t {
   kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7,
-  kSlotMetal0 = 8  // Metal's point in the unit ball: attempt k uses slots kSlotMetal0 + 3k .. + 2
+  kSlotMetal0 = 8,  // Metal's point in the unit ball: attempt k uses slots kSlotMetal0 + 3k .. + 2
+  kSlotRoulette = 8 + 3 * 64  // Russian roulette's draw, past the 64 Metal attempts
 };
 constexpr uint32_t kMetalAttempts = 64;
 
@@ -1431,6 +1432,15 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   // a grazing specular event can give 1/|cos| = inf against cos = 0 in FP32: such a path carries
   // nothing further (the reference, in f64, would carry a NaN); end it
   if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) return;
+  if (!NRC && p.roulette) {
+    // path.rs:93-100 (commented out in the reference): continue with probability q, the largest
+    // throughput component capped at 1, the throughput divided by q
+    const float q = fminf(1.0f, fmaxf(fmaxf(beta4.x, beta4.y), beta4.z));
+    if (urand(seed, path, bounce, kSlotRoulette) > q) return;
+    beta4.x = beta4.x / q;
+    beta4.y = beta4.y / q;
+    beta4.z = beta4.z / q;
+  }
   beta4.w = sc.specular ? 1.0f : (p.mis ? -sc.pdf : 0.0f);
   b.beta[j] = beta4;
   b.ray_o[j] = make_float4(sc.o.x, sc.o.y, sc.o.z, near);

@@ -123,6 +123,7 @@ struct RenderParams {
   const JobDesc* jobs;
   uint32_t n_jobs, pass_paths, batch_pixels;
   uint32_t mis;  // multiple importance sampling (power heuristic) instead of the specular guard
+  uint32_t roulette;  // Russian roulette after each scattering
 };
 
 // Per-vertex record of a training path, field-major: rec[(f * max_depth + m - 1) * train_cap + rank]

@@ -1058,6 +1058,9 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     p.far = (float)c->far;
     if (c->mis > 1) throw Error(PBRT_ERR_INVALID_ARG, "mis must be 0 or 1");
     p.mis = c->mis;
+    if (c->roulette > 1) throw Error(PBRT_ERR_INVALID_ARG, "roulette must be 0 or 1");
+    if (c->roulette && nrc) throw Error(PBRT_ERR_INVALID_ARG, "Russian roulette is not supported with the radiance cache");
+    p.roulette = c->roulette;
 
     // radiance cache buffers
     NrcTrainBuffers tb{};
@@ -1415,6 +1418,10 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
     for (uint32_t k = 0; k < n; ++k)
       if (cfg[k].mis != cfg[0].mis || cfg[k].mis > 1) throw Error(PBRT_ERR_INVALID_ARG, "mis must be 0 or 1, the same for every job");
     p.mis = cfg[0].mis;
+    for (uint32_t k = 0; k < n; ++k)
+      if (cfg[k].roulette != cfg[0].roulette || cfg[k].roulette > 1)
+        throw Error(PBRT_ERR_INVALID_ARG, "roulette must be 0 or 1, the same for every job");
+    p.roulette = cfg[0].roulette;
     p.batch_pixels = (uint32_t)pix;
     const SceneView v = view_of(s);
     const uint32_t grid = persistent_grid(s, (uint32_t)capacity);

@@ -257,6 +257,36 @@ def test_mis_render_matches_oracle(gpu_ctx, which):
     assert not np.array_equal(plain, film)
 
 
+@pytest.mark.parametrize("which", ["room", "mirror"])
+def test_roulette_render_matches_oracle(gpu_ctx, which):
+    """Russian roulette on the device against the FP32 oracle's roulette at equal spp under the
+    same sampler (the draw that ends a path is the same counter-based sample on both sides), with
+    the bars of test_render_matches_oracle; the ray counts agree within 0.5 % and roulette traces
+    fewer rays than the plain render. Jobs that disagree on the option are refused."""
+    from test_oracle_roulette import EYE, FOV, LOOK, mirror_scene
+    if which == "room":
+        desc, cam = scenes.room(subdiv=4, ggx=True)
+        eye, look, fov, depth = cam["eye"], cam["look"], cam["fov"], cam["max_depth"]
+    else:
+        desc, eye, look, fov, depth = mirror_scene(), EYE, LOOK, FOV, 3
+    w = h = 64
+    spp = 8
+    scene = pbrt_amd.Scene(gpu_ctx, desc)
+    film, st = scene.render(w, h, spp, depth, eye, look, fov=fov, seed=3, roulette=True)
+    orc = po.Scene(desc, precision=32, **pbrt_amd.DEVICE_BVH)
+    f32, orays = orc.render(w, h, fov, eye, look, spp, depth, 3, roulette=True)
+    scale = float(f32.mean())
+    assert scale > 0 and np.isfinite(film).all()
+    assert rmse(film, f32) / scale < 0.02, rmse(film, f32) / scale
+    assert abs(film.mean() - f32.mean()) / scale < 0.01
+    assert abs(st["rays"] - orays) < 0.005 * orays, (st["rays"], orays)
+    _, st0 = scene.render(w, h, spp, depth, eye, look, fov=fov, seed=3, film=False)
+    assert st["rays"] < st0["rays"]
+    job = dict(width=8, height=8, spp=1, max_depth=depth, eye=eye, look=look, fov=fov)
+    with pytest.raises(pbrt_amd.PbrtError, match="roulette"):
+        scene.render_jobs([job, dict(job, roulette=True)])
+
+
 def spheres_on_checkerboard():
     """default.stage's content built by hand: a fuzzy metal, a glass and a Lambertian sphere on a
     checkerboard floor, an emissive panel above, the loader's point light and background light."""
