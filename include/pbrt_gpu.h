@@ -314,7 +314,12 @@ int pbrt_ply_read(const char* path, int reverse, float* xyz_out, uint32_t* nv, u
  * (loader/visitors/scene_builder.rs): objects in reverse file order, transform steps applied in
  * listed order, mesh files read relative to base_dir (NULL: the working directory, as the
  * reference), then a point light of intensity 15 at (0, 2, 1) and BackgroundInfiniteLight(white,
- * (0.5, 0.7, 1.0)). Returns a committed scene; view_out (may be NULL) receives the pin-hole camera
+ * (0.5, 0.7, 1.0)). A file may instead declare its lights after `scene`, beside its objects (the
+ * `light` production the reference plans, IDEAS.md:25-27, 180-199): `light point color r g b
+ * [transform { ... }]` (the transform applied to the origin), `light uniform_infinite color r g b`,
+ * `light background_infinite color r g b color r g b` (y = -1, then y = +1). Such a file gets only
+ * its declared lights, in file order, after area lights for its emissive (diffuse_light, colour
+ * texture) rectangles, registered as the objects are built. Returns a committed scene; view_out (may be NULL) receives the pin-hole camera
  * (eye, look, up) and the renderer's defaults fov 90 degrees, near 1e-4, far 1000, other fields
  * untouched. A compound object becomes its children; cylinders, planes and the thin-lens camera
  * (no device counterpart) are refused with PBRT_ERR_INVALID_ARG, as is any syntax error (the

@@ -195,9 +195,18 @@ struct ObjN {
   Xf xf;
   std::vector<ObjN> children;  // transformed: one; compound: in file order
 };
+// A geometry-less light of the `light` production (planned in the reference, IDEAS.md:25-27 and
+// 180-199: a scene member beside the objects, positioned by a transform block; area lights stay a
+// diffuse_light material on an object).
+struct LightN {
+  enum Kind { kPoint, kUniform, kBackground } kind = kPoint;
+  double rgb[3] = {0, 0, 0}, rgb2[3] = {0, 0, 0};
+  Xf xf;
+};
 struct StageN {
   double pos[3], look[3], up[3];
   std::vector<ObjN> objects;
+  std::vector<LightN> lights;  // file order
 };
 
 // ---- parser (parser.rs) --------------------------------------------------------------------------
@@ -220,7 +229,11 @@ class Parser {
     for (;;) {
       const Tok t = lx_.next();
       if (t.kind == Tok::kEnd) break;
-      if (t.kind != Tok::kWord || t.text != "object") Lexer::fail(t.line, "object expected");
+      if (t.kind == Tok::kWord && t.text == "light") {
+        st.lights.push_back(light());
+        continue;
+      }
+      if (t.kind != Tok::kWord || t.text != "object") Lexer::fail(t.line, "object or light expected");
       st.objects.push_back(object());
     }
     return st;
@@ -269,6 +282,36 @@ class Parser {
       Lexer::fail(t.line, "simple, transformed or compound expected");
     }
     return o;
+  }
+  // light point color r g b [transform {...}] | light uniform_infinite color r g b |
+  // light background_infinite color r g b color r g b (the colours at y = -1 and y = +1)
+  LightN light() {
+    const Tok t = lx_.next();
+    LightN l;
+    if (t.kind != Tok::kWord) Lexer::fail(t.line, "light kind expected");
+    if (t.text == "point") {
+      l.kind = LightN::kPoint;
+      word("color");
+      vec(l.rgb);
+      const Tok x = lx_.next();
+      lx_.take_back(x);
+      if (x.kind == Tok::kWord && x.text == "transform") l.xf = transform();
+    } else if (t.text == "uniform_infinite") {
+      l.kind = LightN::kUniform;
+      word("color");
+      vec(l.rgb);
+    } else if (t.text == "background_infinite") {
+      l.kind = LightN::kBackground;
+      word("color");
+      vec(l.rgb);
+      word("color");
+      vec(l.rgb2);
+    } else if (t.text == "area" || t.text == "diffuse_light") {
+      Lexer::fail(t.line, "an area light is a diffuse_light material on a rectangle object, not a light");
+    } else {
+      Lexer::fail(t.line, "point, uniform_infinite or background_infinite expected");
+    }
+    return l;
   }
   ShapeN shape() {
     const Tok t = lx_.next();
@@ -580,14 +623,32 @@ int pbrt_scene_load_stage(pbrt_ctx* ctx, const char* path, const char* base_dir,
       } else {
         const int kind = sh.kind == "sphere" ? PBRT_SHAPE_SPHERE : sh.kind == "aabox" ? PBRT_SHAPE_BOX : PBRT_SHAPE_RECT;
         check(pbrt_scene_add_object(s, kind, -1, sh.param, mat, M, MI, &id));
+        // a file that declares its lights also gets its emissive rectangles as area lights
+        // (ideas/area_light.md section 4: the builder registers them as it assembles the object)
+        if (!st.lights.empty() && kind == PBRT_SHAPE_RECT && o.mat.kind == PBRT_MAT_DIFFUSE_LIGHT && !o.mat.tex)
+          check(pbrt_scene_add_area_light(s, id, o.mat.rgb));
       }
     }
-    // Loader::load_scene: a point light of intensity 15 at (0, 2, 1) and the white-to-sky
-    // background light
-    const double lp[3] = {0.0, 2.0, 1.0}, li[3] = {15.0, 15.0, 15.0};
-    const double down[3] = {1.0, 1.0, 1.0}, up[3] = {0.5, 0.7, 1.0};
-    check(pbrt_scene_add_point_light(s, lp, li));
-    check(pbrt_scene_add_background_light(s, down, up));
+    if (st.lights.empty()) {
+      // Loader::load_scene: a point light of intensity 15 at (0, 2, 1) and the white-to-sky
+      // background light
+      const double lp[3] = {0.0, 2.0, 1.0}, li[3] = {15.0, 15.0, 15.0};
+      const double down[3] = {1.0, 1.0, 1.0}, up[3] = {0.5, 0.7, 1.0};
+      check(pbrt_scene_add_point_light(s, lp, li));
+      check(pbrt_scene_add_background_light(s, down, up));
+    }
+    for (const LightN& l : st.lights) {
+      if (l.kind == LightN::kPoint) {
+        // PointLight::sample_li: the transform applied to the origin (lights/point_light.rs:24)
+        const double p[3] = {l.xf.m.a[0][3], l.xf.m.a[1][3], l.xf.m.a[2][3]};
+        check(pbrt_scene_add_point_light(s, p, l.rgb));
+      } else if (l.kind == LightN::kUniform) {
+        // UniformInfiniteLight: the background light with one colour (the same sphere sampling)
+        check(pbrt_scene_add_background_light(s, l.rgb, l.rgb));
+      } else {
+        check(pbrt_scene_add_background_light(s, l.rgb, l.rgb2));
+      }
+    }
     check(pbrt_scene_commit(s));
     if (view_out) {
       for (int c = 0; c < 3; ++c) {

@@ -114,3 +114,71 @@ def test_default_stage_render_matches_oracle(gpu_ctx):
     scale = float(f32.mean())
     rmse = float(np.sqrt(np.mean((film - f32) ** 2)))
     assert scale > 0 and rmse / scale < 0.02, rmse / scale
+
+
+LIT = """camera pin_hole pos 0 1 3.5 look 0 0.8 0 up 0 1 0
+scene
+  object simple rectangle 6 6 lambertian color 0.7 0.7 0.7
+  object transformed object simple rectangle 1 1 diffuse_light color 8 8 8
+    transform { rotate_x -3.14159 translate 0 2 0 }
+  object transformed object simple sphere 0.5 metal 0.2 color 0.9 0.8 0.6 transform { translate -0.6 0.5 0 }
+  object transformed object simple aabox 0.6 0.6 0.6 lambertian color 0.2 0.4 0.8
+    transform { rotate_y 0.4 translate 0.7 0.3 0.2 }
+  light point color 4 4 4 transform { rotate_z 0.3 translate 1 1.5 1.5 }
+  light background_infinite color 0.1 0.1 0.1 color 0.2 0.3 0.5
+  light uniform_infinite color 0.05 0.05 0.05
+"""
+
+
+def lit_described():
+    """LIT built through SceneDesc in the loader's order: objects reversed, the emissive rectangle
+    registered as an area light as it is added, then the declared lights in file order."""
+    xf = Transform.from_steps
+    d = SceneDesc()
+    m = d.add_material("lambertian", (0.2, 0.4, 0.8))
+    d.add_object("aabox", m, param=(0.6, 0.6, 0.6), transform=xf([["rotate_y", 0.4], ["translate", 0.7, 0.3, 0.2]]))
+    m = d.add_material("metal", (0.9, 0.8, 0.6), fuzz=0.2)
+    d.add_object("sphere", m, param=(0.5,), transform=xf([["translate", -0.6, 0.5, 0.0]]))
+    m = d.add_material("diffuse_light", (8.0, 8.0, 8.0))
+    panel = d.add_object("rectangle", m, param=(1.0, 1.0),
+                         transform=xf([["rotate_x", -3.14159], ["translate", 0.0, 2.0, 0.0]]))
+    d.add_area_light(panel, (8.0, 8.0, 8.0))
+    m = d.add_material("lambertian", (0.7, 0.7, 0.7))
+    d.add_object("rectangle", m, param=(6.0, 6.0))
+    p, _ = xf([["rotate_z", 0.3], ["translate", 1.0, 1.5, 1.5]]).flat()
+    d.add_point_light((p[3], p[7], p[11]), (4.0, 4.0, 4.0))
+    d.add_background_light((0.1, 0.1, 0.1), (0.2, 0.3, 0.5))
+    d.add_background_light((0.05, 0.05, 0.05), (0.05, 0.05, 0.05))
+    return d
+
+
+def test_declared_lights_replace_the_loaders_lights(gpu_ctx, tmp_path):
+    """A `.stage` file with `light` productions: the loaded scene equals lit_described() (scene
+    info, renders bit for bit with and without MIS) and its render matches the FP32 oracle; the
+    same file without its light lines gets the loader's point and background lights and no area
+    light, so its film differs."""
+    print("stage with declared lights", flush=True)
+    path = tmp_path / "lit.stage"
+    path.write_text(LIT)
+    loaded, view = pbrt_amd.Scene.load_stage(gpu_ctx, str(path), str(tmp_path))
+    desc = lit_described()
+    built = pbrt_amd.Scene(gpu_ctx, desc)
+    assert loaded.info() == built.info()
+    kw = dict(fov=view["fov"], up=view["up"], seed=5)
+    for mis in (False, True):
+        fa, _ = loaded.render(48, 36, 8, 6, view["eye"], view["look"], mis=mis, **kw)
+        fb, _ = built.render(48, 36, 8, 6, view["eye"], view["look"], mis=mis, **kw)
+        assert np.isfinite(fa).all() and fa.max() > 0
+        np.testing.assert_array_equal(fa, fb)
+    w = h = 64
+    film, _ = loaded.render(w, h, 8, 6, view["eye"], view["look"], fov=view["fov"], seed=3)
+    f32, _ = po.Scene(desc, precision=32, **pbrt_amd.DEVICE_BVH).render(w, h, view["fov"], view["eye"], view["look"],
+                                                                      8, 6, 3)
+    scale = float(f32.mean())
+    assert scale > 0 and float(np.sqrt(np.mean((film - f32) ** 2))) / scale < 0.02
+    unlit = tmp_path / "unlit.stage"
+    unlit.write_text("".join(l + "\n" for l in LIT.splitlines() if not l.lstrip().startswith("light")))
+    plain, _ = pbrt_amd.Scene.load_stage(gpu_ctx, str(unlit), str(tmp_path))
+    fp, _ = plain.render(48, 36, 8, 6, view["eye"], view["look"], **kw)
+    fl, _ = loaded.render(48, 36, 8, 6, view["eye"], view["look"], **kw)
+    assert not np.array_equal(fp, fl)

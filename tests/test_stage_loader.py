@@ -112,6 +112,15 @@ scene
     (GOOD + "  object simple csg union { elem rectangle 1 1 transform { } } lambertian color 1 1 1\n",
      "line 6: CSG elements are spheres and boxes"),
     (GOOD + "  object compound { object simple sphere 1 lambertian color 1 1 1\n", "'}' or object expected"),
+    (GOOD + "  light point color 15 15 15 transform { translate 0 2 1 }\n"
+     "  light background_infinite color 1 1 1 color 0.5 0.7 1\n  light uniform_infinite color 0.5 0.5 0.5\n",
+     "null context"),
+    (GOOD + "  light spot color 1 1 1\n", "line 6: point, uniform_infinite or background_infinite expected"),
+    (GOOD + "  light area color 1 1 1\n", "line 6: an area light is a diffuse_light material"),
+    (GOOD + "  light background_infinite color 1 1 1\n", "line 7: 'color' expected"),
+    (GOOD + "  light point 1 1 1\n", "line 6: 'color' expected"),
+    (GOOD + "  light point color 1 1 1 transform { scale 2 }\n", "line 6: translate or rotate_x/y/z expected"),
+    (GOOD + "  camera pin_hole\n", "line 6: object or light expected"),
 ])
 def test_stage_syntax(tmp_path, text, what):
     p = tmp_path / "s.stage"
