@@ -33,13 +33,15 @@ SEED_RAYS = 7
 # Memory-side bytes of one k_closest launch on each batch of this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2, WRITE x1), measured at 32
 # waves per CU; profiles/k_closest_icosphere1p3m_counters.md. Keyed by (batch, block, lds_stack,
-# refill, tree width): reported only for the kernel variants that were measured.
+# refill, tree width, packet, image width of the 16 x 4 tiles or 0 for batch order): reported only
+# for the kernel variants that were measured.
 _COUNTERS = "profiles/k_closest_icosphere1p3m_counters.md"
 MEASURED_TRAFFIC = {
-    ("coherent", 256, 8, 0, 2, 0): {"bytes": 255.14e6, "source": _COUNTERS},
-    ("incoherent", 128, 8, 1, 4, 0): {"bytes": 2369.43e6, "source": _COUNTERS},
-    ("coherent", 128, 8, 0, 2, 0): {"bytes": 253.77e6, "source": _COUNTERS},
-    ("incoherent", 256, 8, 1, 2, 0): {"bytes": 3141.65e6, "source": _COUNTERS},
+    ("coherent", 256, 8, 0, 2, 0, 2048): {"bytes": 257.30e6, "source": _COUNTERS},
+    ("coherent", 256, 8, 0, 2, 0, 0): {"bytes": 255.14e6, "source": _COUNTERS},
+    ("incoherent", 128, 8, 1, 4, 0, 0): {"bytes": 2369.43e6, "source": _COUNTERS},
+    ("coherent", 128, 8, 0, 2, 0, 0): {"bytes": 253.77e6, "source": _COUNTERS},
+    ("incoherent", 256, 8, 1, 2, 0, 0): {"bytes": 3141.65e6, "source": _COUNTERS},
 }
 LAUNCH_KEYS = ("block", "waves_per_cu", "lds_stack", "refill", "width", "packet")
 LAUNCH_DEFAULTS = dict(block=128, waves_per_cu=16, lds_stack=16, refill=0, width=2, packet=0)
@@ -337,6 +339,8 @@ def main():
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--no-tiles", action="store_true",
+                    help="take the coherent (image) batch in row order instead of 16 x 4 tiles")
     ap.add_argument("--autotune", action="store_true", help="autotune each batch's launch instead of the fixed picks")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
@@ -398,6 +402,10 @@ def main():
         dev[k] = (all_rays.view(32 * off, 32 * n), all_hits.view(16 * off, 16 * n), n)
         off += n
 
+    # the coherent batch is a side x side image: its kernels take it in 16 x 4 tiles
+    # (pbrt_mesh_set_ray_tiles); the incoherent batch in batch order
+    tiles = {"coherent": 0 if args.no_tiles or args.side % 16 else args.side, "incoherent": 0}
+
     # launch shape per batch: the autotuner's choice on that batch's own rays, or the fixed one
     tuned = None
     launches = {k: mesh.get_launch() for k in dev}
@@ -410,6 +418,7 @@ def main():
     elif not (args.block or args.waves or args.lds_stack or args.refill or args.no_autotune):
         tuned = {}
         for k, (dr, dh, n) in dev.items():
+            mesh.set_ray_tiles(tiles[k])
             tuned[k] = mesh.autotune(dr, n, dh, any_hit=False, reps=2)
             launches[k] = {x: tuned[k][x] for x in LAUNCH_KEYS}
         print(f"[bench] autotuned launches: {tuned}", file=sys.stderr, flush=True)
@@ -437,6 +446,7 @@ def main():
     for _ in range(args.warmup):
         for k, (dr, dh, n) in dev.items():
             mesh.set_launch(**launches[k])
+            mesh.set_ray_tiles(tiles[k])
             mesh.query(dr, n, dh)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -444,6 +454,7 @@ def main():
         for j, (k, (dr, dh, n)) in enumerate(dev.items()):
             e0, e1 = evs[s][j]
             mesh.set_launch(**launches[k])
+            mesh.set_ray_tiles(tiles[k])
             e0.record(stream)
             mesh.query(dr, n, dh)
             e1.record(stream)
@@ -463,7 +474,7 @@ def main():
     alg_bytes = sum(work[k]["bytes_per_launch"] for k in dev)
     achieved = alg_bytes / (step_kernel_ms * 1e-3) / 1e9
     measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "lds_stack", "refill", "width",
-                                                                          "packet")))
+                                                                          "packet")) + (tiles[k],))
                 for k in dev]
     measured = None if (args.side != 2048 or None in measured) else measured
 
@@ -472,6 +483,7 @@ def main():
     for k, (dr, dh, n) in dev.items():
         occ = ctx.alloc(n)
         mesh.set_launch(**launches[k])
+        mesh.set_ray_tiles(tiles[k])
         mesh.intersect_p(dr, n, occ)
         e0, e1 = evs[0][0]
         e0.record(stream)
@@ -546,7 +558,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": sum(m["bytes"] for m in measured) / len(measured) if measured else None,
-            "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k})" for k in dev),
+            "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k}{', 16x4 tiles' if tiles[k] else ''})" for k in dev),
             "algorithmic_bytes_per_launch": alg_bytes / len(dev),
             "traffic_source": measured[0]["source"] if measured else None,
         },

@@ -199,6 +199,11 @@ class TriangleMesh:
         call("pbrt_mesh_copy_layout", self.h, _ptr(nodes), _ptr(tris), _ptr(box), ctypes.byref(ref))
         return nodes, tris, box, ref.value
 
+    def set_ray_tiles(self, image_width=0):
+        """pbrt_mesh_set_ray_tiles: later batches are images `image_width` rays wide, traversed in
+        16 x 4 tiles (0: batch order); results are unchanged."""
+        call("pbrt_mesh_set_ray_tiles", self.h, int(image_width))
+
     def set_launch(self, block=0, waves_per_cu=0, lds_stack=0, refill=0, width=0, packet=0):
         call("pbrt_mesh_set_launch", self.h,
              ctypes.byref(Launch(block, waves_per_cu, lds_stack, refill, width, packet)))

@@ -90,6 +90,7 @@ SIGNATURES = {
     "pbrt_mesh_trace_closest_stats": [P, P, U32, P, P, P],
     "pbrt_mesh_trace_any": [P, P, U32, P, P],
     "pbrt_mesh_set_launch": [P, ctypes.POINTER(Launch)],
+    "pbrt_mesh_set_ray_tiles": [P, U32],
     "pbrt_mesh_get_launch": [P, ctypes.POINTER(Launch)],
     "pbrt_mesh_autotune": [P, P, U32, P, I32, U32, ctypes.POINTER(Launch), ctypes.POINTER(F32)],
     "pbrt_mesh_status": [P, ctypes.POINTER(I32)],

@@ -40,6 +40,7 @@ struct pbrt_mesh {
   uint64_t spill_entries = 0;
   int* status = nullptr;
   pbrt_launch launch{128, 16, 16, 0, 2, 0};
+  uint32_t tile_w = 0;  // pbrt_mesh_set_ray_tiles
   std::mutex mu;
 };
 
@@ -68,6 +69,9 @@ LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
   a.nodes4 = m->nodes4;
   a.root4 = m->root4;
   a.packet = m->launch.packet;
+  a.tile_w = m->tile_w;
+  if (m->tile_w && n % (4 * m->tile_w))
+    throw Error(PBRT_ERR_INVALID_ARG, "ray count not a multiple of four image rows (pbrt_mesh_set_ray_tiles)");
   a.cu_count = (uint32_t)m->ctx->cu_count;
   return a;
 }
@@ -394,6 +398,15 @@ int pbrt_mesh_export_tree(const pbrt_mesh* m, double* boxes, uint8_t* is_leaf, u
       count[i] = n.leaf ? n.count : 0;
     }
     for (size_t i = 0; i < b.order.size(); ++i) leaf_tris[i] = b.order[i];
+  });
+}
+
+int pbrt_mesh_set_ray_tiles(pbrt_mesh* m, uint32_t image_width) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    if (image_width % 16) throw Error(PBRT_ERR_INVALID_ARG, "image width must be a multiple of 16");
+    if (image_width > (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "image width too large");
+    m->tile_w = image_width;
   });
 }
 

@@ -42,24 +42,37 @@ struct Closest {
 // to different schedules; which is faster depends on the rays' coherence, and the autotuner picks.
 // The order of node visits and triangle tests within a ray is the reference's, so results and work
 // counters are the same either way.
+// The batch index of the k-th ray a kernel takes. tile_w = 0: batch order. tile_w > 0: the batch is
+// an image of tile_w rays per row (tile_w a multiple of 16, the ray count a multiple of 4 tile_w)
+// taken in tiles of 16 x 4 rays, so that the 64 lanes of a wave start on one compact patch of the
+// image rather than on a 64-ray strip of one row. Results are stored at the ray's own index.
+__device__ __forceinline__ uint32_t tiled_ray(uint32_t k, uint32_t tile_w) {
+  if (tile_w == 0) return k;
+  const uint32_t band = k / (4 * tile_w), in_band = k - band * 4 * tile_w;
+  const uint32_t tile = in_band >> 6, lane = in_band & 63;
+  return (band * 4 + (lane >> 4)) * tile_w + (tile << 4) + (lane & 15);
+}
+
 template <int BLOCK, int DEPTH, bool STATS, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                    RootDesc root, const float4* __restrict__ rays,
                                                    float4* __restrict__ hits, uint32_t n, uint32_t* __restrict__ stats,
                                                    uint2* __restrict__ spill, uint32_t spill_cap,
-                                                   int* __restrict__ status, uint32_t refill) {
+                                                   int* __restrict__ status, uint32_t refill, uint32_t tile_w) {
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
   RayState r;
+  uint32_t ri = 0;  // the batch index of the ray in progress
   Closest s;
   int32_t cur = 0;
   uint32_t tk = 0, tend = 0;  // triangles of the current leaf still to test: [tk, tend)
   bool live = false, idle = false;
   auto start = [&](uint32_t k) {
-    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    ri = tiled_ray(k, tile_w);
+    r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
     s.min_t = FLT_MAX;
     s.t = s.u = s.v = 0.f;
     s.cur_far = r.far;  // min(far, min_t): the box tests' narrowed interval
@@ -134,12 +147,12 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
     }
     if (!live && !idle) {
       const bool found = s.best != 0xffffffffu;
-      hits[i] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
+      hits[ri] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
                             found ? s.v : 0.f);
       if (STATS) {
-        stats[3 * (size_t)i] = s.n_vis;
-        stats[3 * (size_t)i + 1] = s.n_box;
-        stats[3 * (size_t)i + 2] = s.n_tri;
+        stats[3 * (size_t)ri] = s.n_vis;
+        stats[3 * (size_t)ri + 1] = s.n_box;
+        stats[3 * (size_t)ri + 2] = s.n_tri;
       }
       i += stride;
       if (i >= n) break;
@@ -164,18 +177,20 @@ template <int BLOCK, int DEPTH, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                RootDesc root, const float4* __restrict__ rays,
                                                uint8_t* __restrict__ occluded, uint32_t n, uint2* __restrict__ spill,
-                                               uint32_t spill_cap, int* __restrict__ status, uint32_t refill) {
+                                               uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w) {
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
   RayState r;
+  uint32_t ri = 0;  // the batch index of the ray in progress
   int32_t cur = 0;
   uint32_t tk = 0, tend = 0;
   bool live = false, hit = false, idle = false;
   auto start = [&](uint32_t k) {
-    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    ri = tiled_ray(k, tile_w);
+    r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
     st.sp = 0;
     hit = false;
     tk = tend = 0;
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
       if (live) cur = (int32_t)st.pop().x;
     }
     if (!live && !idle) {
-      occluded[i] = hit ? 1 : 0;
+      occluded[ri] = hit ? 1 : 0;
       i += stride;
       if (i >= n) break;
       if (WAVE_REFILL)
@@ -254,19 +269,21 @@ template <int BLOCK, int DEPTH, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                     RootDesc root, const float4* __restrict__ rays,
                                                     float4* __restrict__ hits, uint32_t n, uint2* __restrict__ spill,
-                                                    uint32_t spill_cap, int* __restrict__ status, uint32_t refill) {
+                                                    uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w) {
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
   RayState r;
+  uint32_t ri = 0;  // the batch index of the ray in progress
   Closest s;
   int32_t cur = 0;
   uint32_t tk = 0, tend = 0;
   bool live = false, idle = false;
   auto start = [&](uint32_t k) {
-    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    ri = tiled_ray(k, tile_w);
+    r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
     s.min_t = FLT_MAX;
     s.t = s.u = s.v = 0.f;
     s.cur_far = r.far;
@@ -327,7 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__
     }
     if (!live && !idle) {
       const bool found = s.best != 0xffffffffu;
-      hits[i] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
+      hits[ri] = make_float4(__int_as_float(found ? (int32_t)s.best : -1), found ? s.t : 0.f, found ? s.u : 0.f,
                             found ? s.v : 0.f);
       i += stride;
       if (i >= n) break;
@@ -351,18 +368,20 @@ template <int BLOCK, int DEPTH, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                 RootDesc root, const float4* __restrict__ rays,
                                                 uint8_t* __restrict__ occluded, uint32_t n, uint2* __restrict__ spill,
-                                                uint32_t spill_cap, int* __restrict__ status, uint32_t refill) {
+                                                uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w) {
   __shared__ uint2 lds[DEPTH][BLOCK];
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
   RayState r;
+  uint32_t ri = 0;  // the batch index of the ray in progress
   int32_t cur = 0;
   uint32_t tk = 0, tend = 0;
   bool live = false, hit = false, idle = false;
   auto start = [&](uint32_t k) {
-    r = make_ray(rays[2 * (size_t)k], rays[2 * (size_t)k + 1]);
+    ri = tiled_ray(k, tile_w);
+    r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
     st.sp = 0;
     hit = false;
     tk = tend = 0;
@@ -411,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nod
       if (live) cur = (int32_t)st.pop().x;
     }
     if (!live && !idle) {
-      occluded[i] = hit ? 1 : 0;
+      occluded[ri] = hit ? 1 : 0;
       i += stride;
       if (i >= n) break;
       if (WAVE_REFILL)
@@ -558,38 +577,38 @@ hipError_t launch_t(const LaunchArgs& a, int kind) {
     root4.ref = a.root4;
     if (kind == 0 && !wave)
       hipLaunchKernelGGL((k_closest4<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
-                         (const float4*)a.rays, (float4*)a.hits, a.n, a.spill, spill_cap, a.status, 0u);
+                         (const float4*)a.rays, (float4*)a.hits, a.n, a.spill, spill_cap, a.status, 0u, a.tile_w);
     else if (kind == 0)
       hipLaunchKernelGGL((k_closest4<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
-                         (const float4*)a.rays, (float4*)a.hits, a.n, a.spill, spill_cap, a.status, a.refill);
+                         (const float4*)a.rays, (float4*)a.hits, a.n, a.spill, spill_cap, a.status, a.refill, a.tile_w);
     else if (!wave)
       hipLaunchKernelGGL((k_any4<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
-                         (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, 0u);
+                         (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, 0u, a.tile_w);
     else
       hipLaunchKernelGGL((k_any4<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes4, a.tris, root4,
-                         (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, a.refill);
+                         (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, a.refill, a.tile_w);
     return hipGetLastError();
   }
   if (kind == 0 && !wave)
     hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, false, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
-                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status, 0u);
+                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status, 0u, a.tile_w);
   else if (kind == 0)
     hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, false, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
                        a.root, (const float4*)a.rays, (float4*)a.hits, a.n, nullptr, a.spill, spill_cap, a.status,
-                       a.refill);
+                       a.refill, a.tile_w);
   else if (kind == 1 && !wave)
     hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, true, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
-                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, a.stats, a.spill, spill_cap, a.status, 0u);
+                       a.root, (const float4*)a.rays, (float4*)a.hits, a.n, a.stats, a.spill, spill_cap, a.status, 0u, a.tile_w);
   else if (kind == 1)
     hipLaunchKernelGGL((k_closest<BLOCK, DEPTH, true, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris,
                        a.root, (const float4*)a.rays, (float4*)a.hits, a.n, a.stats, a.spill, spill_cap, a.status,
-                       a.refill);
+                       a.refill, a.tile_w);
   else if (!wave)
     hipLaunchKernelGGL((k_any<BLOCK, DEPTH, false>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
-                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, 0u);
+                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, 0u, a.tile_w);
   else
     hipLaunchKernelGGL((k_any<BLOCK, DEPTH, true>), dim3(grid), dim3(BLOCK), 0, s, a.nodes, a.tris, a.root,
-                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, a.refill);
+                       (const float4*)a.rays, a.occluded, a.n, a.spill, spill_cap, a.status, a.refill, a.tile_w);
   return hipGetLastError();
 }
 

@@ -228,3 +228,46 @@ def test_repeated_runs_are_bitwise_identical(gpu_ctx):
     a = mesh.trace_host(rays)
     b = mesh.trace_host(rays)
     assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("w,h", [(64, 32), (48, 8)])
+def test_tiled_ray_order_gives_batch_order_results(gpu_ctx, w, h):
+    """pbrt_mesh_set_ray_tiles: an image batch taken in 16 x 4 tiles gives every ray the hit,
+    counters and occlusion of batch order, for each kernel family (binary with per-lane and wave
+    refill, 4-wide, instrumented, any hit); every result slot is written (outputs start filled
+    with 0xff). Widths not a multiple of 16, and batches not a multiple of four rows, fail."""
+    v, f = pbrt_amd.icosphere(5, noise="fbm", amp=0.08, octaves=3, seed=3)
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f, **pbrt_amd.DEVICE_BVH)
+    rays = pbrt_amd.pinhole_rays(w, h, 0.9, (0.3, 0.4, 2.6), (0.0, 0.0, 0.0))
+    n = len(rays)
+    dr = gpu_ctx.upload(rays)
+    hits, occ, st = gpu_ctx.alloc(16 * n), gpu_ctx.alloc(n), gpu_ctx.alloc(12 * n)
+
+    def run(tiles, launch):
+        mesh.set_ray_tiles(tiles)
+        mesh.set_launch(**launch)
+        for d in (hits, occ, st):
+            d.fill(0xFF)
+        mesh.query(dr, n, hits)
+        a = hits.read(pbrt_amd.HIT_DTYPE, n)
+        mesh.intersect_p(dr, n, occ)
+        b = occ.read(np.uint8, n)
+        mesh.query_instrumented(dr, n, hits, st)
+        c = st.read(np.uint32, 3 * n)
+        return a, b, c
+
+    for launch in (dict(block=256, lds_stack=8, refill=0), dict(block=128, lds_stack=16, refill=4),
+                   dict(block=128, lds_stack=8, refill=1, width=4)):
+        base = run(0, launch)
+        tiled = run(w, launch)
+        assert (base[0]["prim"] >= 0).any() and (base[0]["prim"] < 0).any()
+        for x, y in zip(base, tiled):
+            np.testing.assert_array_equal(x.view(np.uint8), y.view(np.uint8), err_msg=str(launch))
+    mesh.set_ray_tiles(0)
+    with pytest.raises(pbrt_amd.PbrtError, match="multiple of 16"):
+        mesh.set_ray_tiles(20)
+    mesh.set_ray_tiles(16)
+    with pytest.raises(pbrt_amd.PbrtError, match="four image rows"):
+        mesh.query(dr, n - 16, hits)
+    mesh.set_ray_tiles(0)
+    assert mesh.status() == 0
