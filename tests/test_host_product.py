@@ -149,3 +149,15 @@ def test_random_rays_are_unit_and_inside_the_ball():
     o = r[:, :3].astype(np.float64) - [1, 2, 3]
     assert np.sqrt((o * o).sum(1)).max() <= 0.5 + 1e-6
     assert abs(d.mean(0)).max() < 0.05
+
+
+def test_ray_tiles_setting_is_validated_on_the_host():
+    """pbrt_mesh_set_ray_tiles accepts 0 and multiples of 16 and refuses other widths (no device
+    call: the setting is checked when it is made)."""
+    v, f = grid_mesh(2)
+    m = pbrt_amd.TriangleMesh(None, v, f)
+    m.set_ray_tiles(0)
+    m.set_ray_tiles(2048)
+    for bad in (8, 20, 1000):
+        with pytest.raises(pbrt_amd.PbrtError, match="multiple of 16"):
+            m.set_ray_tiles(bad)
