@@ -42,17 +42,6 @@ struct Closest {
 // to different schedules; which is faster depends on the rays' coherence, and the autotuner picks.
 // The order of node visits and triangle tests within a ray is the reference's, so results and work
 // counters are the same either way.
-// The batch index of the k-th ray a kernel takes. tile_w = 0: batch order. tile_w > 0: the batch is
-// an image of tile_w rays per row (tile_w a multiple of 16, the ray count a multiple of 4 tile_w)
-// taken in tiles of 16 x 4 rays, so that the 64 lanes of a wave start on one compact patch of the
-// image rather than on a 64-ray strip of one row. Results are stored at the ray's own index.
-__device__ __forceinline__ uint32_t tiled_ray(uint32_t k, uint32_t tile_w) {
-  if (tile_w == 0) return k;
-  const uint32_t band = k / (4 * tile_w), in_band = k - band * 4 * tile_w;
-  const uint32_t tile = in_band >> 6, lane = in_band & 63;
-  return (band * 4 + (lane >> 4)) * tile_w + (tile << 4) + (lane & 15);
-}
-
 template <int BLOCK, int DEPTH, bool STATS, bool WAVE_REFILL>
 __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ nodes, const DevTri* __restrict__ tris,
                                                    RootDesc root, const float4* __restrict__ rays,
