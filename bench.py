@@ -33,12 +33,11 @@ SEED_RAYS = 7
 # Memory-side bytes of one k_closest launch on each batch of this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2, WRITE x1), measured at 32
 # waves per CU; profiles/k_closest_icosphere1p3m_counters.md. Keyed by (batch, block, lds_stack,
-# refill, tree width, packet, image width of the 16 x 4 tiles or 0 for batch order): reported only
+# refill, tree width, packet, image width of the 8 x 16 tiles or 0 for batch order): reported only
 # for the kernel variants that were measured.
 _COUNTERS = "profiles/k_closest_icosphere1p3m_counters.md"
 MEASURED_TRAFFIC = {
-    ("coherent", 64, 16, 0, 4, 0, 2048): {"bytes": 266.65e6, "source": _COUNTERS},
-    ("coherent", 256, 8, 0, 2, 0, 2048): {"bytes": 257.30e6, "source": _COUNTERS},
+    ("coherent", 64, 16, 0, 4, 0, 2048): {"bytes": 317.61e6, "source": _COUNTERS},
     ("coherent", 256, 8, 0, 2, 0, 0): {"bytes": 255.14e6, "source": _COUNTERS},
     ("incoherent", 128, 8, 1, 4, 0, 0): {"bytes": 2369.43e6, "source": _COUNTERS},
     ("coherent", 128, 8, 0, 2, 0, 0): {"bytes": 253.77e6, "source": _COUNTERS},
@@ -341,7 +340,7 @@ def main():
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
     ap.add_argument("--no-tiles", action="store_true",
-                    help="take the coherent (image) batch in row order instead of 16 x 4 tiles")
+                    help="take the coherent (image) batch in row order instead of 8 x 16 tiles")
     ap.add_argument("--autotune", action="store_true", help="autotune each batch's launch instead of the fixed picks")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
@@ -403,7 +402,7 @@ def main():
         dev[k] = (all_rays.view(32 * off, 32 * n), all_hits.view(16 * off, 16 * n), n)
         off += n
 
-    # the coherent batch is a side x side image: its kernels take it in 16 x 4 tiles
+    # the coherent batch is a side x side image: its kernels take it in 8 x 16 tiles
     # (pbrt_mesh_set_ray_tiles); the incoherent batch in batch order
     tiles = {"coherent": 0 if args.no_tiles or args.side % 16 else args.side, "incoherent": 0}
 
@@ -559,7 +558,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": sum(m["bytes"] for m in measured) / len(measured) if measured else None,
-            "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k}{', 16x4 tiles' if tiles[k] else ''})" for k in dev),
+            "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k}{', 8x16 tiles' if tiles[k] else ''})" for k in dev),
             "algorithmic_bytes_per_launch": alg_bytes / len(dev),
             "traffic_source": measured[0]["source"] if measured else None,
         },

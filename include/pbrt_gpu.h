@@ -142,9 +142,9 @@ int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
 /* Ray order of the mesh's traversal kernels (no counterpart in the reference, which queries one ray
  * at a time): image_width 0 (the default) takes each batch's rays in batch order; image_width W > 0
  * declares every later batch an image W rays wide stored row after row (W a multiple of 16, the
- * batch's ray count a multiple of 4 W or the call fails with PBRT_ERR_INVALID_ARG), which the
- * closest-hit, instrumented and any-hit kernels take in tiles of 16 x 4 rays so that a wave starts
- * on a compact patch of the image. Every result is stored at its ray's own index and is the same as
+ * batch's ray count a multiple of 16 W or the call fails with PBRT_ERR_INVALID_ARG), which the
+ * closest-hit, instrumented and any-hit kernels take in tiles of 8 x 16 rays so that a wave starts
+ * on an 8 x 8 patch of the image. Every result is stored at its ray's own index and is the same as
  * in batch order. Packet traversal ignores the setting. */
 int pbrt_mesh_set_ray_tiles(pbrt_mesh* mesh, uint32_t image_width);
 int pbrt_mesh_get_launch(const pbrt_mesh* mesh, pbrt_launch* out);

@@ -201,7 +201,7 @@ class TriangleMesh:
 
     def set_ray_tiles(self, image_width=0):
         """pbrt_mesh_set_ray_tiles: later batches are images `image_width` rays wide, traversed in
-        16 x 4 tiles (0: batch order); results are unchanged."""
+        8 x 16 tiles (0: batch order); results are unchanged."""
         call("pbrt_mesh_set_ray_tiles", self.h, int(image_width))
 
     def set_launch(self, block=0, waves_per_cu=0, lds_stack=0, refill=0, width=0, packet=0):

@@ -70,8 +70,8 @@ LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
   a.root4 = m->root4;
   a.packet = m->launch.packet;
   a.tile_w = m->tile_w;
-  if (m->tile_w && n % (4 * m->tile_w))
-    throw Error(PBRT_ERR_INVALID_ARG, "ray count not a multiple of four image rows (pbrt_mesh_set_ray_tiles)");
+  if (m->tile_w && n % (16 * m->tile_w))
+    throw Error(PBRT_ERR_INVALID_ARG, "ray count not a multiple of sixteen image rows (pbrt_mesh_set_ray_tiles)");
   a.cu_count = (uint32_t)m->ctx->cu_count;
   return a;
 }

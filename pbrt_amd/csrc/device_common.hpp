@@ -30,14 +30,15 @@ struct RayState {
 };
 
 // The batch index of the k-th ray a kernel takes. tile_w = 0: batch order. tile_w > 0: the batch is
-// an image of tile_w rays per row (tile_w a multiple of 16, the ray count a multiple of 4 tile_w)
-// taken in tiles of 16 x 4 rays, so that the 64 lanes of a wave start on one compact patch of the
-// image rather than on a 64-ray strip of one row. Results are stored at the ray's own index.
+// an image of tile_w rays per row (tile_w a multiple of 8, the ray count a multiple of 16 tile_w)
+// taken in tiles of 8 x 16 rays, band of 16 rows after band, so that the 64 lanes of a wave start
+// on an 8 x 8 patch of the image (the next wave on the patch below) rather than on a 64-ray strip
+// of one row. Results are stored at the ray's own index.
 __device__ __forceinline__ uint32_t tiled_ray(uint32_t k, uint32_t tile_w) {
   if (tile_w == 0) return k;
-  const uint32_t band = k / (4 * tile_w), in_band = k - band * 4 * tile_w;
-  const uint32_t tile = in_band >> 6, lane = in_band & 63;
-  return (band * 4 + (lane >> 4)) * tile_w + (tile << 4) + (lane & 15);
+  const uint32_t band = k / (16 * tile_w), in_band = k - band * 16 * tile_w;
+  const uint32_t tile = in_band >> 7, slot = in_band & 127;
+  return (band * 16 + (slot >> 3)) * tile_w + (tile << 3) + (slot & 7);
 }
 
 __device__ __forceinline__ RayState make_ray(float4 a, float4 b) {

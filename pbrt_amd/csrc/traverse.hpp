@@ -24,7 +24,7 @@ struct LaunchArgs {
   const DevNode4* nodes4;
   int32_t root4;    // root reference in the 4-wide tree (the root box is root's)
   uint32_t packet;  // 1: closest hit by wave packets on the binary tree (k_closest_packet)
-  uint32_t tile_w;  // 0: rays in batch order; W: an image W rays wide, taken in 16 x 4 tiles (not packets)
+  uint32_t tile_w;  // 0: rays in batch order; W: an image W rays wide, taken in 8 x 16 tiles (not packets)
 };
 // Pending subtrees the packet kernel's wave-wide stack holds: trees deeper than this need the
 // per-lane kernels.

@@ -230,12 +230,12 @@ def test_repeated_runs_are_bitwise_identical(gpu_ctx):
     assert a.tobytes() == b.tobytes()
 
 
-@pytest.mark.parametrize("w,h", [(64, 32), (48, 8)])
+@pytest.mark.parametrize("w,h", [(64, 32), (48, 16)])
 def test_tiled_ray_order_gives_batch_order_results(gpu_ctx, w, h):
-    """pbrt_mesh_set_ray_tiles: an image batch taken in 16 x 4 tiles gives every ray the hit,
+    """pbrt_mesh_set_ray_tiles: an image batch taken in 8 x 16 tiles gives every ray the hit,
     counters and occlusion of batch order, for each kernel family (binary with per-lane and wave
     refill, 4-wide, instrumented, any hit); every result slot is written (outputs start filled
-    with 0xff). Widths not a multiple of 16, and batches not a multiple of four rows, fail."""
+    with 0xff). Widths not a multiple of 16, and batches not a multiple of sixteen rows, fail."""
     v, f = pbrt_amd.icosphere(5, noise="fbm", amp=0.08, octaves=3, seed=3)
     mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f, **pbrt_amd.DEVICE_BVH)
     rays = pbrt_amd.pinhole_rays(w, h, 0.9, (0.3, 0.4, 2.6), (0.0, 0.0, 0.0))
@@ -267,7 +267,7 @@ def test_tiled_ray_order_gives_batch_order_results(gpu_ctx, w, h):
     with pytest.raises(pbrt_amd.PbrtError, match="multiple of 16"):
         mesh.set_ray_tiles(20)
     mesh.set_ray_tiles(16)
-    with pytest.raises(pbrt_amd.PbrtError, match="four image rows"):
+    with pytest.raises(pbrt_amd.PbrtError, match="sixteen image rows"):
         mesh.query(dr, n - 16, hits)
     mesh.set_ray_tiles(0)
     assert mesh.status() == 0
