@@ -9,11 +9,12 @@ Workload (1,310,720-triangle pure-traversal configuration): an icosphere of 8 su
 tmin 1e-4, tmax 1000. One step = closest hit over both batches (8,388,608 rays).
 
 Prints one JSON line. `roofline` describes the dominant kernel (closest hit): algorithmic bytes per
-launch = sum over rays of 32 (ray) + 16 (hit) + 64 x interior nodes fetched + 48 x triangles
-fetched, with the node/triangle counts measured exactly by the instrumented kernel on the same
-rays, divided by the kernel's mean duration (HIP events on the launch stream). `cpu_baseline` times
-the f64 CPU restatement of the reference algorithm (the reference's own BVH parameters: 8 bins,
-t_trav = 0) on a bounded sample of the same rays.
+launch = sum over rays of 32 (ray) + 16 (hit) + node records fetched x their size (128 bytes on the
+4-wide tree the timed kernels walk) + 48 x triangle records fetched, with the counts measured
+exactly by the fetch-counting walk of the same tree on the same rays, divided by the kernel's mean
+duration (HIP events on the launch stream). `cpu_baseline` times the f64 CPU restatement of the
+reference algorithm (the reference's own BVH parameters: 8 bins, t_trav = 0) over the same rays;
+`parity` checks the timed launches' own hits against the FP32 and f64 restatements.
 """
 import argparse
 import json
@@ -65,31 +66,90 @@ def make_workload(pa, side=2048, n_incoherent=1 << 22):
     return v, f, coherent, incoherent
 
 
-def cpu_baseline(v, f, coherent, incoherent, side, threads):
-    """f64 restatement of the reference CPU traversal over the whole step (both 4M-ray batches, all
-    host threads up to 16), plus a one-thread pass over every 16th image row of batch (a) and the
-    first 131,072 rays of batch (b) for comparison with the reference's single-thread figures."""
+def host_info():
+    """What the host offers this process: CPUs in its affinity mask, the machine's CPU count, the
+    cgroup CPU quota (cpu.max: quota and period in microseconds, if readable) and the model name."""
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q = fh.read().split()
+        info["cgroup_cpu_max"] = " ".join(q)
+        if q and q[0] != "max":
+            info["cgroup_cpus"] = round(int(q[0]) / int(q[1]), 2)
+    except (OSError, ValueError, IndexError):
+        info["cgroup_cpu_max"] = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def _fastest(fn, passes=3):
+    fn()  # warm-up
+    best = None
+    for _ in range(passes):
+        t0 = time.perf_counter()
+        out = fn()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best, out
+
+
+def cpu_baseline(pa, v, f, coherent, incoherent, side, threads, device_hits, device_occ):
+    """The CPU path timed on this host, and the checks of the device's results against it.
+
+    Timing: Oracle A (f64 restatement of the reference traversal over the reference's BVH, 8 bins,
+    t_trav = 0) over the whole step (both 4M-ray batches) on `threads` threads, one warm-up pass then
+    the fastest of 3 (the bvh_stats protocol, bvh_stats.rs:162-185); Oracle B (FP32, the device's
+    tree) the same way; Oracle A on one thread over every 16th image row of batch (a) and the first
+    131,072 rays of batch (b), for comparison with the reference's single-thread figures.
+    Parity (the timed kernels' own outputs, read back after the timed loop): closest hits against
+    Oracle B field by field, any hit against Oracle B's, every triangle that differs from Oracle A
+    classified, hit t against the f64 re-evaluation of the hit triangle."""
+    from oracle import parity
     from oracle import pyoracle as po
     t0 = time.time()
-    m = po.Mesh(v, f, precision=64, bins=8, t_trav=0.0)
+    ma = parity.oracle_a(v, f)
+    mb = parity.oracle_b(v, f, pa.DEVICE_BVH)
     build_s = time.time() - t0
-    sample = np.concatenate([rows(coherent), rows(incoherent)])
-    m.trace(sample[:65536], mode=po.MODE_REFERENCE, threads=threads)  # warm-up
-    best = None
-    for _ in range(2):
-        t0 = time.time()
-        m.trace(sample, mode=po.MODE_REFERENCE, threads=threads)
-        dt = time.time() - t0
-        best = dt if best is None else min(best, dt)
+    batches = {"coherent": rows(coherent), "incoherent": rows(incoherent)}
+    sample = np.concatenate(list(batches.values()))
+    ta, ra = _fastest(lambda: ma.trace(sample, mode=po.MODE_REFERENCE, threads=threads))
+    tb, rb = _fastest(lambda: mb.trace(sample, mode=po.MODE_CANONICAL, threads=threads))
     small = np.concatenate([rows(coherent).reshape(side, side, 8)[::16].reshape(-1, 8), rows(incoherent)[:131072]])
-    t0 = time.time()
-    m.trace(small, mode=po.MODE_REFERENCE, threads=1)
-    one = len(small) / (time.time() - t0) / 1e6
-    return dict(value=len(sample) / best / 1e6, unit="Mrays/s", cores=threads, kind="port",
-                sample=f"the full step ({len(sample)} rays, both batches) on {threads} threads; f64 C++ restatement "
-                       f"of the reference traversal over the reference BVH (8 bins, t_trav=0), fastest of 2 passes "
-                       f"after a warm-up; BVH build {build_s:.1f}s not timed",
-                seconds=best, one_thread_mrays_s=round(one, 3), one_thread_sample_rays=len(small))
+    t1, _ = _fastest(lambda: ma.trace(small, mode=po.MODE_REFERENCE, threads=1), passes=1)
+    parity_out = {}
+    off = 0
+    for k, r in batches.items():
+        n = len(r)
+        sl = slice(off, off + n)
+        off += n
+        a_k = {x: ra[x][sl] for x in ("prim", "t", "u", "v")}
+        b_k = {x: rb[x][sl] for x in ("prim", "t", "u", "v")}
+        h = device_hits[k]
+        any_b = mb.trace(r, mode=po.MODE_ANY, threads=threads)["prim"] >= 0
+        cls = parity.classify_vs_reference(a_k, h["prim"], h["t"], h["u"], h["v"], v, f, r)
+        cls.pop("unexplained_rays")
+        parity_out[k] = {
+            "mismatch_vs_oracle_b": parity.compare_hits(h, b_k),
+            "anyhit_mismatch_vs_oracle_b": int((device_occ[k].astype(bool) != any_b).sum()),
+            "vs_oracle_a": cls,
+            "t_vs_fp64": parity.t_error_by_incidence(v, f, r, h["prim"].astype(np.int64), h["t"]),
+        }
+    host = host_info()
+    return dict(value=len(sample) / ta / 1e6, unit="Mrays/s", cores=threads, kind="port",
+                sample=f"the full step ({len(sample)} rays, both batches) on {threads} threads (the CPUs this "
+                       f"process may use, at most 16, the host share of one GPU); f64 C++ restatement of the "
+                       f"reference traversal over the reference BVH (8 bins, t_trav=0); one warm-up pass, then "
+                       f"the fastest of 3; BVH builds {build_s:.1f}s not timed",
+                seconds=ta, oracle_b_mrays_s=len(sample) / tb / 1e6,
+                one_thread_mrays_s=round(len(small) / t1 / 1e6, 3), one_thread_sample_rays=len(small),
+                host=host), parity_out
 
 
 def first_below(prog, bar):
@@ -423,19 +483,35 @@ def main():
             launches[k] = {x: tuned[k][x] for x in LAUNCH_KEYS}
         print(f"[bench] autotuned launches: {tuned}", file=sys.stderr, flush=True)
 
-    # exact per-ray work on the same rays (instrumented kernel, untimed)
-    work = {}
+    # exact per-ray work on the same rays, untimed: the records the timed walk fetches (its tree
+    # width, pbrt_mesh_trace_work) give the algorithmic bytes; the reference's instrumented counters
+    # on the binary tree are reported beside them, and its hits are kept to check the timed launches
+    work, binary_hits = {}, {}
     for k, (dr, dh, n) in dev.items():
-        ds = ctx.alloc(12 * n)
+        ds, dw = ctx.alloc(12 * n), ctx.alloc(8 * n)
+        mesh.set_launch(block=128, waves_per_cu=16, lds_stack=16, width=2)
+        mesh.set_ray_tiles(0)
         mesh.query_instrumented(dr, n, dh, ds)
         st = ds.read(np.uint32, 3 * n).reshape(n, 3).astype(np.int64)
-        hits = dh.read(pa.HIT_DTYPE, n)
+        binary_hits[k] = dh.read(pa.HIT_DTYPE, n)
         interior = (st[:, 1] - 1) // 2
-        bytes_alg = int(n * (32 + 16) + 64 * interior.sum() + 48 * st[:, 2].sum())
-        work[k] = dict(n=n, hit_rate=float((hits["prim"] >= 0).mean()), nodes_per_ray=float(st[:, 0].mean()),
-                       box_tests_per_ray=float(st[:, 1].mean()), tri_tests_per_ray=float(st[:, 2].mean()),
-                       bytes_per_launch=bytes_alg, bytes_per_ray=bytes_alg / n)
+        bytes_binary = int(n * (32 + 16) + 64 * interior.sum() + 48 * st[:, 2].sum())
+        mesh.set_launch(**launches[k])
+        mesh.set_ray_tiles(tiles[k])
+        mesh.query_work(dr, n, dh, dw)
+        wk = dw.read(np.uint32, 2 * n).reshape(n, 2).astype(np.int64)
+        node_bytes_rec = 128 if launches[k]["width"] == 4 else 64
+        bytes_alg = int(n * (32 + 16) + node_bytes_rec * wk[:, 0].sum() + 48 * wk[:, 1].sum())
+        work[k] = dict(n=n, hit_rate=float((binary_hits[k]["prim"] >= 0).mean()),
+                       node_records_per_ray=float(wk[:, 0].mean()), node_record_bytes=node_bytes_rec,
+                       tri_records_per_ray=float(wk[:, 1].mean()),
+                       bytes_per_launch=bytes_alg, bytes_per_ray=bytes_alg / n,
+                       binary_tree={"nodes_per_ray": float(st[:, 0].mean()),
+                                    "box_tests_per_ray": float(st[:, 1].mean()),
+                                    "tri_tests_per_ray": float(st[:, 2].mean()),
+                                    "bytes_per_launch": bytes_binary})
         ds.free()
+        dw.free()
     if mesh.status() != 0:
         raise RuntimeError("traversal kernel reported a stack overflow")
 
@@ -462,6 +538,14 @@ def main():
     wall = time.perf_counter() - t0
     per_batch_ms = {k: [evs[s][j][0].elapsed_ms(evs[s][j][1]) for s in range(args.steps)]
                     for j, k in enumerate(dev)}
+    # the timed launches' own results: no stack overflow, and the hits of the reference's
+    # instrumented walk on the binary tree, byte for byte
+    status_after = mesh.status()
+    if status_after != 0:
+        raise RuntimeError("traversal kernel reported a stack overflow in the timed loop")
+    timed_hits = {k: dh.read(pa.HIT_DTYPE, n) for k, (dr, dh, n) in dev.items()}
+    timed_vs_binary = {k: int((timed_hits[k].view(np.uint8).reshape(-1, 16)
+                               != binary_hits[k].view(np.uint8).reshape(-1, 16)).any(1).sum()) for k in dev}
 
     total_rays = sum(n for (_, _, n) in dev.values())
     kern_ms = {k: float(np.mean(x)) for k, x in per_batch_ms.items()}
@@ -479,7 +563,7 @@ def main():
     measured = None if (args.side != 2048 or None in measured) else measured
 
     # any-hit throughput on the same batches (informational)
-    anyhit = {}
+    anyhit, occlusion = {}, {}
     for k, (dr, dh, n) in dev.items():
         occ = ctx.alloc(n)
         mesh.set_launch(**launches[k])
@@ -491,7 +575,10 @@ def main():
             mesh.intersect_p(dr, n, occ)
         e1.record(stream)
         anyhit[k] = n * 5 / (e0.elapsed_ms(e1) * 1e-3) / 1e6
+        occlusion[k] = occ.read(np.uint8, n)
         occ.free()
+    if mesh.status() != 0:
+        raise RuntimeError("any-hit kernel reported a stack overflow")
 
     copy_gbs = None
     if args.calibrate:
@@ -521,11 +608,22 @@ def main():
     if not args.no_build:
         build = build_measure(pa, ctx, v, f, mesh, build_s)
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(v, f, coherent, incoherent, args.side, threads)
-        print(f"[bench] cpu baseline: {cpu['value']:.2f} Mrays/s on {threads} threads", file=sys.stderr, flush=True)
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cpu, parity = cpu_baseline(pa, v, f, coherent, incoherent, args.side, threads, timed_hits, occlusion)
+        print(f"[bench] cpu baseline: {cpu['value']:.2f} Mrays/s (Oracle A), {cpu['oracle_b_mrays_s']:.2f} "
+              f"(Oracle B) on {threads} threads; host {cpu['host']}", file=sys.stderr, flush=True)
+        print(f"[bench] parity: {parity}", file=sys.stderr, flush=True)
+    parity_line = {"timed_hits_vs_binary_instrumented_walk": timed_vs_binary, "status_after_timed_loop": status_after}
+    if parity:
+        parity_line.update({k: {"mismatch_vs_oracle_b": p["mismatch_vs_oracle_b"]["total"],
+                                "anyhit_mismatch_vs_oracle_b": p["anyhit_mismatch_vs_oracle_b"],
+                                "mismatch_rate_vs_oracle_a": p["vs_oracle_a"]["rate"],
+                                "vs_oracle_a": p["vs_oracle_a"],
+                                "max_rel_t_err_fp64": p["t_vs_fp64"]["max_rel"],
+                                "max_rel_t_err_fp64_non_grazing": p["t_vs_fp64"]["max_rel_non_grazing"],
+                                "t_err_above_1e-5": p["t_vs_fp64"]["above_1e-5"]} for k, p in parity.items()})
 
     out = {
         "metric": "closest-hit Mrays/s (1.3M tris); time-to-RMSE 1e-2, radiance cache vs plain PT",
@@ -560,8 +658,14 @@ def main():
             "traffic": sum(m["bytes"] for m in measured) / len(measured) if measured else None,
             "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k}{', 8x16 tiles' if tiles[k] else ''})" for k in dev),
             "algorithmic_bytes_per_launch": alg_bytes / len(dev),
+            "algorithmic_bytes_rule": "per ray 32 (ray) + 16 (hit) + node records fetched x record size (128 B "
+                                      "4-wide, 64 B binary) + triangle records fetched x 48 B, counted by "
+                                      "pbrt_mesh_trace_work on the same rays and tree",
+            "frac_binary_tree_bytes": round(sum(work[k]["binary_tree"]["bytes_per_launch"] for k in dev)
+                                            / (step_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic_source": measured[0]["source"] if measured else None,
         },
+        "parity": parity_line,
         "per_batch": per_batch,
         "anyhit_mrays_s": anyhit,
         "copy_gbs": copy_gbs,

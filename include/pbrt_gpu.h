@@ -93,6 +93,10 @@ int pbrt_memset_dev(pbrt_ctx* ctx, void* dst_dev, int value, size_t bytes);
 /* Device-to-device copy by a plain streaming kernel (bytes a multiple of 16); also the known-size
  * reference for calibrating hardware byte counters. */
 int pbrt_dev_copy(pbrt_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes, void* stream);
+/* A non-blocking stream of default priority on the context's device (NULL stream arguments elsewhere
+ * mean the context's own stream). */
+int pbrt_stream_create(pbrt_ctx* ctx, void** stream_out);
+int pbrt_stream_destroy(pbrt_ctx* ctx, void* stream);
 int pbrt_stream_sync(pbrt_ctx* ctx, void* stream);
 int pbrt_event_create(pbrt_ctx* ctx, void** ev_out);
 int pbrt_event_record(void* ev, void* stream);
@@ -137,6 +141,15 @@ int pbrt_mesh_trace_closest_stats(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uin
                                   uint32_t* stats_dev, void* stream);
 /* Batched any hit: BVHTree::intersect_p (triangle_mesh/bvh.rs:381-420); occluded_dev[i] = 0 or 1. */
 int pbrt_mesh_trace_any(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, uint8_t* occluded_dev, void* stream);
+/* The records a traversal fetches, per ray, on the tree the mesh's launch walks (width 2 or 4), as
+ * BVHTree::query_instrumented counts work (triangle_mesh/bvh.rs:18-47,354-361) but for the device
+ * layout: work_dev receives n x {node records (64 bytes each on the binary tree, 128 bytes on the
+ * 4-wide tree), triangle records (48 bytes)} (uint32). out_dev receives what pbrt_mesh_trace_closest
+ * (any_hit = 0, pbrt_hit) or pbrt_mesh_trace_any (any_hit != 0, bytes) gives. The ray order of
+ * pbrt_mesh_set_ray_tiles is followed; the counts do not depend on the launch shape. Not available
+ * with a packet launch (PBRT_ERR_INVALID_ARG). */
+int pbrt_mesh_trace_work(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, void* out_dev, int any_hit,
+                         uint32_t* work_dev, void* stream);
 /* Launch shape used by the mesh's traversal kernels (all fields 0 = defaults). */
 int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
 /* Ray order of the mesh's traversal kernels (no counterpart in the reference, which queries one ray
@@ -157,7 +170,10 @@ int pbrt_mesh_autotune(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, vo
                        uint32_t reps, pbrt_launch* chosen, float* best_ms);
 
 /* Sticky error word of the mesh's kernels (waits for the device): 0 = fine, bit 0 = a traversal
- * stack exceeded its bound (cannot happen for a tree of the reported depth). */
+ * stack exceeded its bound (cannot happen for a tree of the reported depth).
+ * Traces of one mesh share its HBM stack spill area: a trace on another stream than the mesh's
+ * previous trace waits on the device for that trace to end, so traces on different streams are
+ * ordered, never concurrent. */
 int pbrt_mesh_status(pbrt_mesh* mesh, int* status_out);
 
 /* ---- scene: Scene::{new, add_object, add_light, commit} (src/scene.rs:80-115) ----------------

@@ -1045,9 +1045,10 @@ class SceneOracle {
       if (!(std::isfinite(beta.c[0]) && std::isfinite(beta.c[1]) && std::isfinite(beta.c[2]))) break;
       if (roulette) {
         // path.rs:93-100: continue with probability q, the throughput's largest component (capped
-        // at 1 here, so that the estimator stays unbiased), and divide the throughput by q
+        // at 1 here, so that the estimator stays unbiased), and divide the throughput by q; the path
+        // survives only when the draw is below q, so a zero throughput (q = 0) always ends it
         const R q = std::fmin(R(1), std::fmax(std::fmax(beta.c[0], beta.c[1]), beta.c[2]));
-        if (uni<R>(seed, path, bounce, kRoulette) > q) break;
+        if (!(uni<R>(seed, path, bounce, kRoulette) < q)) break;
         beta = sdiv(beta, q);
       }
       ray = s.ray;

@@ -74,6 +74,29 @@ class Context:
     def event(self):
         return Event(self)
 
+    def new_stream(self):
+        """A further non-blocking stream on the device (pbrt_stream_create)."""
+        return Stream(self)
+
+
+class Stream:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        call("pbrt_stream_create", ctx.h, ctypes.byref(h))
+        self.h = h
+
+    def synchronize(self):
+        call("pbrt_stream_sync", self.ctx.h, self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pbrt_stream_destroy(self.ctx.h, self.h)
+                self.h = None
+        except Exception:
+            pass
+
 
 class Event:
     def __init__(self, ctx):
@@ -245,6 +268,11 @@ class TriangleMesh:
     def intersect_p(self, rays, n, occluded, stream=None):
         """BVHTree::intersect_p (bvh.rs:381-420): occluded[i] in {0, 1}."""
         call("pbrt_mesh_trace_any", self.h, rays.ptr, n, occluded.ptr, stream)
+
+    def query_work(self, rays, n, out, work, any_hit=False, stream=None):
+        """pbrt_mesh_trace_work: the launch's walk (tree width 2 or 4) with n x {node records, triangle
+        records} fetched per ray into `work` (uint32); `out` as query / intersect_p."""
+        call("pbrt_mesh_trace_work", self.h, rays.ptr, n, out.ptr, int(any_hit), work.ptr, stream)
 
     def status(self):
         s = ctypes.c_int()

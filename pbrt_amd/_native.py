@@ -73,6 +73,8 @@ SIGNATURES = {
     "pbrt_memcpy_d2h": [P, P, P, ctypes.c_size_t],
     "pbrt_memset_dev": [P, P, I32, ctypes.c_size_t],
     "pbrt_dev_copy": [P, P, P, ctypes.c_size_t, P],
+    "pbrt_stream_create": [P, PP],
+    "pbrt_stream_destroy": [P, P],
     "pbrt_stream_sync": [P, P],
     "pbrt_event_create": [P, PP],
     "pbrt_event_record": [P, P],
@@ -88,6 +90,7 @@ SIGNATURES = {
     "pbrt_mesh_export_tree": [P, P, P, P, P, P, P, P],
     "pbrt_mesh_trace_closest": [P, P, U32, P, P],
     "pbrt_mesh_trace_closest_stats": [P, P, U32, P, P, P],
+    "pbrt_mesh_trace_work": [P, P, U32, P, I32, P, P],
     "pbrt_mesh_trace_any": [P, P, U32, P, P],
     "pbrt_mesh_set_launch": [P, ctypes.POINTER(Launch)],
     "pbrt_mesh_set_ray_tiles": [P, U32],

@@ -1,5 +1,9 @@
 """Builds the native library in-tree: pbrt_amd/libpbrt_gpu.so (HIP for gfx950 + host C++).
 
+Each source compiles to its own object (in parallel, under pbrt_amd/_build/), then one hipcc link
+makes the shared library. Device code is not relocatable (no -fgpu-rdc): every translation unit
+carries its own code object, so the units are independent.
+
 Flags that matter for parity with the CPU path:
   -ffp-contract=off      no fused multiply-add, host and device, so FP32 traversal arithmetic is the
                          same sequence of roundings as the FP32 CPU restatement;
@@ -10,18 +14,20 @@ Packed FP32 instructions are disabled (-packed-fp32-ops), by policy for this har
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpbrt_gpu.so")
+OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "scene_api.cpp", "nrc_api.cpp", "film.cpp", "stage_loader.cpp", "traverse.hip", "bvh_gpu.hip", "scene.hip",
-           "nrc.hip", "nrc_render.hip"]
+SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "scene_api.cpp", "nrc_api.cpp", "film.cpp", "stage_loader.cpp",
+           "traverse.hip", "traverse_work.hip", "bvh_gpu.hip", "scene.hip", "nrc.hip", "nrc_render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     "--offload-arch=gfx950",
     "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero",
@@ -30,18 +36,44 @@ FLAGS = [
 ]
 
 
+def _jobs():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n, len(SOURCES)))
+
+
+def _compile(src, verbose):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return obj, r
+
+
 def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
     srcs = [os.path.join(SRC, s) for s in SOURCES]
-    cmd = [HIPCC] + FLAGS + ["-x", "hip"] + srcs + ["-o", OUT + ".tmp"]
+    with ThreadPoolExecutor(_jobs()) as ex:
+        results = list(ex.map(lambda s: _compile(s, verbose), srcs))
+    failed = [r for _, r in results if r.returncode != 0]
+    for r in results:
+        if r[1].returncode != 0 or (verbose and (r[1].stdout or r[1].stderr)):
+            sys.stderr.write(r[1].stdout + r[1].stderr)
+    if failed:
+        raise RuntimeError("hipcc failed")
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + [o for o, _ in results] + ["-o", OUT + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc link failed")
     os.replace(OUT + ".tmp", OUT)
     return OUT
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose="-v" in sys.argv))

@@ -35,13 +35,20 @@ struct pbrt_mesh {
   DevNode4* nodes4 = nullptr;  // the 4-wide tree, collapsed on first use
   uint64_t n_nodes4 = 0;
   int32_t root4 = 0;
+  uint32_t depth4 = 0;         // interior levels of the 4-wide tree
   RootDesc root{};
   uint2* spill = nullptr;
   uint64_t spill_entries = 0;
   int* status = nullptr;
   pbrt_launch launch{128, 16, 16, 0, 2, 0};
   uint32_t tile_w = 0;  // pbrt_mesh_set_ray_tiles
-  std::mutex mu;
+  // Every launch uses the one spill area: a launch on another stream than the last one waits for
+  // the last one's end (event `done`), so that launches on different streams never share it at once.
+  hipEvent_t done = nullptr;
+  hipStream_t last_stream = nullptr;
+  // Guards every field above that a call reads or writes after creation (launch settings, the
+  // 4-wide tree, the spill area, the stream order).
+  mutable std::mutex mu;
 };
 
 namespace {
@@ -70,7 +77,7 @@ LaunchArgs base_args(pbrt_mesh* m, uint32_t n, void* stream) {
   a.root4 = m->root4;
   a.packet = m->launch.packet;
   a.tile_w = m->tile_w;
-  if (m->tile_w && n % (16 * m->tile_w))
+  if (m->tile_w && n % (16ull * m->tile_w))
     throw Error(PBRT_ERR_INVALID_ARG, "ray count not a multiple of sixteen image rows (pbrt_mesh_set_ray_tiles)");
   a.cu_count = (uint32_t)m->ctx->cu_count;
   return a;
@@ -96,14 +103,27 @@ void ensure_bvh4(pbrt_mesh* m) {
   m->nodes4 = d;
   m->n_nodes4 = n4.size();
   m->root4 = root4;
+  // interior levels of the 4-wide tree (iterative walk; a leaf root has none)
+  uint32_t depth = 0;
+  std::vector<std::pair<int32_t, uint32_t>> todo;
+  if (root4 >= 0) todo.push_back({root4, 1});
+  while (!todo.empty()) {
+    const auto [ref, level] = todo.back();
+    todo.pop_back();
+    depth = std::max(depth, level);
+    const DevNode4& nd = n4[ref];
+    for (int c = 0; c < nd.count[0]; ++c)
+      if (nd.ref[c] >= 0) todo.push_back({nd.ref[c], level + 1});
+  }
+  m->depth4 = depth;
 }
 
 // Sizes the HBM spill area for the launch shape and checks the kernels' error word.
 void prepare_spill(pbrt_mesh* m, LaunchArgs* a) {
   const uint32_t threads = traversal_grid_threads(*a);
   if (threads == 0) throw Error(PBRT_ERR_INVALID_ARG, "unsupported block size");
-  // pending entries: at most depth - 1 on the binary tree, 3 per level on the 4-wide one
-  const uint64_t depth = m->stats.max_depth * (a->width == 4 ? 3 : 1);
+  // pending entries: at most depth - 1 on the binary tree, 3 per interior level on the 4-wide one
+  const uint64_t depth = a->width == 4 ? 3ull * m->depth4 : m->stats.max_depth;
   const uint32_t per_thread = depth + 1 > a->lds_stack ? (uint32_t)(depth + 1 - a->lds_stack) : 1;
   const uint64_t need = (uint64_t)per_thread * threads;
   if (need > m->spill_entries) {
@@ -118,11 +138,23 @@ void prepare_spill(pbrt_mesh* m, LaunchArgs* a) {
   a->spill_capacity_entries = m->spill_entries;
 }
 
+// Stream order of the launches that share the mesh's spill area (the caller holds m->mu).
+void before_launch(pbrt_mesh* m, hipStream_t s) {
+  if (m->last_stream && m->last_stream != s) PBRT_HIP(hipStreamWaitEvent(s, m->done, 0));
+}
+void after_launch(pbrt_mesh* m, hipStream_t s) {
+  PBRT_HIP(hipEventRecord(m->done, s));
+  m->last_stream = s;
+}
+
+// One traversal launch (the caller holds m->mu).
 void run(pbrt_mesh* m, LaunchArgs& a, int kind) {
   if (a.n == 0) return;
-  std::lock_guard<std::mutex> lock(m->mu);
   prepare_spill(m, &a);
+  const hipStream_t s = (hipStream_t)a.stream;
+  before_launch(m, s);
   PBRT_HIP(launch_traversal(a, kind));
+  after_launch(m, s);
 }
 
 }  // namespace
@@ -222,6 +254,23 @@ int pbrt_dev_copy(pbrt_ctx* ctx, void* dst, const void* src, size_t bytes, void*
   });
 }
 
+int pbrt_stream_create(pbrt_ctx* ctx, void** s) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!s) throw Error(PBRT_ERR_INVALID_ARG, "null output");
+    hipStream_t st = nullptr;
+    PBRT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = st;
+  });
+}
+
+int pbrt_stream_destroy(pbrt_ctx* ctx, void* s) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (s) PBRT_HIP(hipStreamDestroy((hipStream_t)s));
+  });
+}
+
 int pbrt_stream_sync(pbrt_ctx* ctx, void* s) {
   return guarded([&] {
     check_ctx(ctx);
@@ -282,6 +331,7 @@ int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_
     if (!nodes.empty()) PBRT_HIP(hipMemcpy(m->nodes, nodes.data(), nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice));
     PBRT_HIP(hipMemcpy(m->tris, tris.data(), tris.size() * sizeof(DevTri), hipMemcpyHostToDevice));
     PBRT_HIP(hipMemset(m->status, 0, sizeof(int)));
+    PBRT_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
     *out = m.release();
   });
 }
@@ -322,6 +372,7 @@ int pbrt_mesh_create_device(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const 
     m->stats = g.stats;
     PBRT_HIP(hipMalloc(&m->status, sizeof(int)));
     PBRT_HIP(hipMemset(m->status, 0, sizeof(int)));
+    PBRT_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
     *out = m.release();
   });
 }
@@ -353,6 +404,7 @@ int pbrt_mesh_destroy(pbrt_mesh* m) {
     (void)hipFree(m->tris);
     (void)hipFree(m->spill);
     (void)hipFree(m->status);
+    if (m->done) (void)hipEventDestroy(m->done);
     delete m;
   });
 }
@@ -405,7 +457,9 @@ int pbrt_mesh_set_ray_tiles(pbrt_mesh* m, uint32_t image_width) {
   return guarded([&] {
     if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
     if (image_width % 16) throw Error(PBRT_ERR_INVALID_ARG, "image width must be a multiple of 16");
-    if (image_width > (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "image width too large");
+    // sixteen rows of the image must fit a batch of at most 2^32 - 1 rays
+    if (image_width >= (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "image width too large (at most 2^28 - 16)");
+    std::lock_guard<std::mutex> lock(m->mu);
     m->tile_w = image_width;
   });
 }
@@ -428,6 +482,7 @@ int pbrt_mesh_set_launch(pbrt_mesh* m, const pbrt_launch* l) {
     if (v.packet && v.width != 2) throw Error(PBRT_ERR_INVALID_ARG, "packet traversal walks the binary tree (width 2)");
     if (v.packet && m->stats.max_depth > kPacketStackDepth + 1)
       throw Error(PBRT_ERR_INVALID_ARG, "tree too deep for packet traversal");
+    std::lock_guard<std::mutex> lock(m->mu);
     if (v.width == 4) {
       if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
       check_ctx(m->ctx);
@@ -447,6 +502,7 @@ static int trace_common(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, pbrt_hit
     if (n && kind != 2 && !hits) throw Error(PBRT_ERR_INVALID_ARG, "null hits");
     if (n && kind == 2 && !occ) throw Error(PBRT_ERR_INVALID_ARG, "null occlusion output");
     if (n && kind == 1 && !stats) throw Error(PBRT_ERR_INVALID_ARG, "null stats output");
+    std::lock_guard<std::mutex> lock(m->mu);
     LaunchArgs a = base_args(m, n, stream);
     a.rays = rays;
     a.hits = hits;
@@ -469,6 +525,31 @@ int pbrt_mesh_trace_any(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, uint8_t*
   return trace_common(m, rays, n, nullptr, occ, nullptr, stream, 2);
 }
 
+int pbrt_mesh_trace_work(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out_dev, int any_hit, uint32_t* work,
+                         void* stream) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
+    check_ctx(m->ctx);
+    if (n && (!rays || !out_dev || !work)) throw Error(PBRT_ERR_INVALID_ARG, "null rays, output or work array");
+    std::lock_guard<std::mutex> lock(m->mu);
+    if (m->launch.packet) throw Error(PBRT_ERR_INVALID_ARG, "the packet walk has no per-ray fetch counters");
+    LaunchArgs a = base_args(m, n, stream);
+    if (n == 0) return;
+    a.rays = rays;
+    a.hits = any_hit ? nullptr : (pbrt_hit*)out_dev;
+    a.occluded = any_hit ? (uint8_t*)out_dev : nullptr;
+    a.block = kWorkBlock;
+    a.lds_stack = kWorkLdsStack;
+    a.refill = 0;
+    prepare_spill(m, &a);
+    const hipStream_t s = (hipStream_t)a.stream;
+    before_launch(m, s);
+    PBRT_HIP(launch_traversal_work(a, any_hit ? 1 : 0, work));
+    after_launch(m, s);
+  });
+}
+
 // Times every supported launch shape of one traversal kernel on the caller's rays and keeps the
 // fastest: block {64, 128, 256} x LDS stack {8, 16, 24, 32} x resident waves per CU
 // {8, 12, 16, 24, 32} x refill {0, 1, 4}; each candidate is run `reps` times after one warm-up
@@ -480,6 +561,7 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
     if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
     check_ctx(m->ctx);
     if (reps == 0) reps = 2;
+    std::lock_guard<std::mutex> lock(m->mu);
     hipEvent_t e0, e1;
     PBRT_HIP(hipEventCreate(&e0));
     PBRT_HIP(hipEventCreate(&e1));
@@ -539,6 +621,7 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
 int pbrt_mesh_get_launch(const pbrt_mesh* m, pbrt_launch* out) {
   return guarded([&] {
     if (!m || !out) throw Error(PBRT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(m->mu);
     *out = m->launch;
   });
 }

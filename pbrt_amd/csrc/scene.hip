@@ -1434,9 +1434,10 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   if (!(isfinite(beta4.x) && isfinite(beta4.y) && isfinite(beta4.z))) return;
   if (!NRC && p.roulette) {
     // path.rs:93-100 (commented out in the reference): continue with probability q, the largest
-    // throughput component capped at 1, the throughput divided by q
+    // throughput component capped at 1, the throughput divided by q. The path survives only when
+    // the draw is below q, so a throughput of zero (q = 0) always ends it (never 0 / 0)
     const float q = fminf(1.0f, fmaxf(fmaxf(beta4.x, beta4.y), beta4.z));
-    if (urand(seed, path, bounce, kSlotRoulette) > q) return;
+    if (!(urand(seed, path, bounce, kSlotRoulette) < q)) return;
     beta4.x = beta4.x / q;
     beta4.y = beta4.y / q;
     beta4.z = beta4.z / q;

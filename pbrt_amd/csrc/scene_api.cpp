@@ -104,6 +104,10 @@ struct pbrt_scene {
   int* status = nullptr;
   uint2* spill = nullptr;
   uint64_t spill_entries = 0;
+  // Every walk on the render stream or a caller's stream uses the one spill area: work on another
+  // stream than the last one waits for the last one's end (event spill_done).
+  hipEvent_t spill_done = nullptr;
+  hipStream_t spill_stream = nullptr;
   PathBuffers pb{};
   std::mutex mu;
 };
@@ -151,6 +155,17 @@ uint32_t persistent_grid(pbrt_scene* s, uint32_t n) {
   const uint32_t cap = (uint32_t)std::max(1, s->ctx->cu_count) * per_cu;
   const uint32_t need = (n + kSceneBlock - 1) / kSceneBlock;
   return std::max(1u, std::min(cap, need));
+}
+
+// Stream order of the work that shares the scene's spill area (the caller holds s->mu): before work
+// on stream st, wait for the last work if it ran on another stream; after it, mark its end.
+void spill_order(pbrt_scene* s, hipStream_t st) {
+  if (!s->spill_done) PBRT_HIP(hipEventCreateWithFlags(&s->spill_done, hipEventDisableTiming));
+  if (s->spill_stream && s->spill_stream != st) PBRT_HIP(hipStreamWaitEvent(st, s->spill_done, 0));
+}
+void spill_mark(pbrt_scene* s, hipStream_t st) {
+  PBRT_HIP(hipEventRecord(s->spill_done, st));
+  s->spill_stream = st;
 }
 
 // The HBM spill area for a grid: a lane's stack holds at most (top-level + mesh depth) entries.
@@ -358,6 +373,7 @@ int pbrt_scene_destroy(pbrt_scene* s) {
                     (void*)s->mats, (void*)s->dlights, (void*)s->dcsg, (void*)s->status, (void*)s->spill})
       if (p) (void)hipFree(p);
     free_paths(s->pb);
+    if (s->spill_done) (void)hipEventDestroy(s->spill_done);
     delete s;
   });
 }
@@ -856,7 +872,9 @@ static int scene_trace(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, pbrt_sce
     a.spill = s->spill;
     a.status = s->status;
     a.stream = stream ? stream : s->ctx->stream;
+    spill_order(s, (hipStream_t)a.stream);
     PBRT_HIP(launch_scene_trace(a, any));
+    spill_mark(s, (hipStream_t)a.stream);
   });
 }
 
@@ -901,12 +919,14 @@ int pbrt_scene_autotune(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, pbrt_sc
             a.spill = s->spill;
             a.status = s->status;
             a.stream = st;
+            spill_order(s, st);
             PBRT_HIP(launch_scene_trace(a, 0));  // warm-up
             float cand = 1e30f;
             for (uint32_t r = 0; r < reps; ++r) {
               PBRT_HIP(hipEventRecord(e0, st));
               PBRT_HIP(launch_scene_trace(a, 0));
               PBRT_HIP(hipEventRecord(e1, st));
+              spill_mark(s, st);
               PBRT_HIP(hipEventSynchronize(e1));
               float ms = 0.f;
               PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1011,6 +1031,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     if ((uint64_t)npix * per_pass > (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "pass larger than 2^28 paths");
     ensure_paths(s, npix * per_pass);
     hipStream_t st = s->ctx->stream;
+    spill_order(s, st);
     const uint32_t npass_paths = npix * per_pass;
     const uint32_t passes_total = (c->spp + per_pass - 1) / per_pass;
 
@@ -1268,6 +1289,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     if (aux) PBRT_HIP(hipStreamSynchronize(aux));
     uint64_t rays = 0, queries = 0;
     PBRT_HIP(hipStreamSynchronize(st));
+    spill_mark(s, st);
     for (int k = 0; k < (nrc ? 2 : 1); ++k) {
       uint64_t r = 0;
       PBRT_HIP(hipMemcpy(&r, counter(sets[k].counts, kCntRays), sizeof(r), hipMemcpyDeviceToHost));
@@ -1321,6 +1343,7 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
     std::lock_guard<std::mutex> lock(s->mu);
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = s->ctx->stream;
+    spill_order(s, st);
     // the pass schedule, fixed in advance: each pass, every unfinished job takes its share of the
     // path budget, at least one sample per pixel
     const uint64_t budget = 1u << 22;
@@ -1456,6 +1479,7 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
       PBRT_HIP(launch_accumulate_jobs(p, pb, st));
     }
     PBRT_HIP(hipStreamSynchronize(st));
+    spill_mark(s, st);
     uint64_t rays = 0;
     PBRT_HIP(hipMemcpy(&rays, counter(pb.counts, kCntRays), sizeof(rays), hipMemcpyDeviceToHost));
     if (films_out) {

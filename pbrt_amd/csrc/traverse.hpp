@@ -32,6 +32,11 @@ constexpr uint32_t kPacketStackDepth = 32;
 
 // kind: 0 closest hit, 1 closest hit with per-ray work counters, 2 any hit.
 hipError_t launch_traversal(const LaunchArgs& a, int kind);
+// The walk of a.width (closest hit, or any hit when any_hit != 0) counting, per ray, the node and
+// triangle records it fetches (work: 2 words per ray). Runs at one launch shape: a.block must be
+// kWorkBlock and a.lds_stack kWorkLdsStack (the spill area sized for them); refill is not used.
+constexpr uint32_t kWorkBlock = 128, kWorkLdsStack = 16;
+hipError_t launch_traversal_work(const LaunchArgs& a, int any_hit, uint32_t* work);
 // Threads of the persistent grid the launch would use (the spill area is sized from it).
 uint32_t traversal_grid_threads(const LaunchArgs& a);
 // Device-to-device copy of bytes (multiple of 16) by a plain streaming kernel.

@@ -271,3 +271,42 @@ def test_tiled_ray_order_gives_batch_order_results(gpu_ctx, w, h):
         mesh.query(dr, n - 16, hits)
     mesh.set_ray_tiles(0)
     assert mesh.status() == 0
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_traces_on_two_streams_share_the_spill_area_safely(gpu_ctx, width):
+    """Traces of one mesh on two streams, queued back to back without a host wait, at an 8-entry
+    LDS stack that spills to HBM on the deep reference-mode tree: each batch gets the hits it gets
+    alone on the context's stream (the second launch waits on the device for the first)."""
+    v, f = load_mesh("dragon_vrip_res3")
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f, **pbrt_amd.REFERENCE_BVH)
+    mesh.set_launch(block=128, lds_stack=8, waves_per_cu=8, width=width)
+    a = as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 200000, seed=21))
+    b = as_rows(pbrt_amd.random_rays(v.mean(0), 0.12, 200000, seed=22))
+    ref_a, ref_b = mesh.trace_host(a), mesh.trace_host(b)
+    s1, s2 = gpu_ctx.new_stream(), gpu_ctx.new_stream()
+    da, db = gpu_ctx.upload(a), gpu_ctx.upload(b)
+    ha, hb = gpu_ctx.alloc(16 * len(a)), gpu_ctx.alloc(16 * len(b))
+    for _ in range(3):
+        ha.fill(0xFF)
+        hb.fill(0xFF)
+        mesh.query(da, len(a), ha, stream=s1.h)
+        mesh.query(db, len(b), hb, stream=s2.h)
+        mesh.query(da, len(a), ha, stream=gpu_ctx.stream)
+        s1.synchronize()
+        s2.synchronize()
+        gpu_ctx.synchronize()
+        assert ha.read(pbrt_amd.HIT_DTYPE, len(a)).tobytes() == ref_a.tobytes()
+        assert hb.read(pbrt_amd.HIT_DTYPE, len(b)).tobytes() == ref_b.tobytes()
+    assert mesh.status() == 0
+
+
+def test_ray_tile_width_bound_at_trace(gpu_ctx):
+    """A batch must hold a whole number of sixteen-row bands of the declared image width."""
+    v, f = pbrt_amd.icosphere(2)
+    mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f)
+    d = gpu_ctx.alloc(64)
+    mesh.set_ray_tiles((1 << 28) - 16)
+    with pytest.raises(pbrt_amd.PbrtError, match="sixteen image rows"):
+        mesh.query(d, 16, d)
+    mesh.set_ray_tiles(0)

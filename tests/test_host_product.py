@@ -161,3 +161,17 @@ def test_ray_tiles_setting_is_validated_on_the_host():
     for bad in (8, 20, 1000):
         with pytest.raises(pbrt_amd.PbrtError, match="multiple of 16"):
             m.set_ray_tiles(bad)
+
+
+def test_ray_tile_width_bounds():
+    """Sixteen image rows must fit a 32-bit ray count: widths from 2^28 up are refused when set
+    (16 x 2^28 wraps to 0 in 32 bits), as are widths not a multiple of 16."""
+    v, f = pbrt_amd.icosphere(2)
+    mesh = pbrt_amd.TriangleMesh(None, v, f)
+    mesh.set_ray_tiles((1 << 28) - 16)
+    for w in (1 << 28, (1 << 32) - 16):
+        with pytest.raises(pbrt_amd.PbrtError, match="too large"):
+            mesh.set_ray_tiles(w)
+    with pytest.raises(pbrt_amd.PbrtError, match="multiple of 16"):
+        mesh.set_ray_tiles(24)
+    mesh.set_ray_tiles(0)
