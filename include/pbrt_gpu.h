@@ -141,6 +141,18 @@ int pbrt_mesh_trace_closest_stats(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uin
                                   uint32_t* stats_dev, void* stream);
 /* Batched any hit: BVHTree::intersect_p (triangle_mesh/bvh.rs:381-420); occluded_dev[i] = 0 or 1. */
 int pbrt_mesh_trace_any(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, uint8_t* occluded_dev, void* stream);
+/* Closest hit as pbrt_mesh_trace_closest, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid allocation); no counterpart
+ * in the reference. Calls no hipMalloc or hipFree: the traversal stack's spill past the LDS stack
+ * goes to a 64 MiB array the library holds on every device it runs on until the process ends, not
+ * to the mesh's own spill area. Same hits as pbrt_mesh_trace_closest with the mesh's launch, whose
+ * persistent grid is capped at the threads whose spill the array holds (fewer resident waves for
+ * deep trees with short LDS stacks). A width-4 launch must have been set by pbrt_mesh_set_launch
+ * (which builds the 4-wide tree). A tree whose spill for one workgroup exceeds the array is refused
+ * with PBRT_ERR_INVALID_ARG. Every no-alloc call on a device shares that device's array: calls that
+ * overlap on one device (two streams, two threads) race and must be ordered by the caller. */
+int pbrt_mesh_trace_closest_noalloc(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, pbrt_hit* hits_dev,
+                                    void* stream);
 /* The records a traversal fetches, per ray, on the tree the mesh's launch walks (width 2 or 4), as
  * BVHTree::query_instrumented counts work (triangle_mesh/bvh.rs:18-47,354-361) but for the device
  * layout: work_dev receives n x {node records (64 bytes each on the binary tree, 128 bytes on the

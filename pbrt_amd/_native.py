@@ -91,6 +91,7 @@ SIGNATURES = {
     "pbrt_mesh_trace_closest": [P, P, U32, P, P],
     "pbrt_mesh_trace_closest_stats": [P, P, U32, P, P, P],
     "pbrt_mesh_trace_work": [P, P, U32, P, I32, P, P],
+    "pbrt_mesh_trace_closest_noalloc": [P, P, U32, P, P],
     "pbrt_mesh_trace_any": [P, P, U32, P, P],
     "pbrt_mesh_set_launch": [P, ctypes.POINTER(Launch)],
     "pbrt_mesh_set_ray_tiles": [P, U32],

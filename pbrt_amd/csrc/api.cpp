@@ -22,6 +22,19 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace pbrt;
 
+// The spill area of pbrt_mesh_trace_closest_noalloc: traversal stack entries past the LDS stack,
+// 64 MiB per device, held by every device the library runs on until the process ends. Host code
+// never uses the array's name as an address: each device's copy is found once, at context creation
+// with the device current, by hipGetSymbolAddress, and that address is what the kernels receive.
+// Nothing in it needs to start at zero (every entry is written before it is read). All no-alloc
+// calls on one device share it: calls that overlap on one device race.
+constexpr uint64_t kNoallocSpillBytes = 64ull << 20;
+constexpr uint64_t kNoallocSpillEntries = kNoallocSpillBytes / sizeof(uint2);
+static __device__ __align__(256) uint2 scratch[kNoallocSpillEntries];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
 
 struct pbrt_mesh {
   pbrt_ctx* ctx = nullptr;
@@ -170,9 +183,20 @@ int pbrt_ctx_create(int device, pbrt_ctx** out) {
     int n = 0;
     PBRT_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) throw Error(PBRT_ERR_INVALID_ARG, "no such device");
+    if (device >= kMaxDevices) throw Error(PBRT_ERR_INVALID_ARG, "device index above 63");
     auto c = std::make_unique<pbrt_ctx>();
     c->device = device;
     PBRT_HIP(hipSetDevice(device));
+    {
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        PBRT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(PBRT_ERR_DEVICE, "no device address for the no-alloc spill array");
+        g_scratch_ptr[device] = p;
+      }
+      c->scratch = g_scratch_ptr[device];
+    }
     hipDeviceProp_t prop;
     PBRT_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
@@ -523,6 +547,37 @@ int pbrt_mesh_trace_closest_stats(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n
 
 int pbrt_mesh_trace_any(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, uint8_t* occ, void* stream) {
   return trace_common(m, rays, n, nullptr, occ, nullptr, stream, 2);
+}
+
+int pbrt_mesh_trace_closest_noalloc(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, pbrt_hit* hits, void* stream) {
+  return guarded([&] {
+    if (!m) throw Error(PBRT_ERR_INVALID_ARG, "null mesh");
+    if (!m->ctx) throw Error(PBRT_ERR_INVALID_ARG, "mesh was built without a device context");
+    check_ctx(m->ctx);
+    if (n && (!rays || !hits)) throw Error(PBRT_ERR_INVALID_ARG, "null rays or hits");
+    if (!m->ctx->scratch) throw Error(PBRT_ERR_INTERNAL, "the context has no spill array address");
+    std::lock_guard<std::mutex> lock(m->mu);
+    LaunchArgs a = base_args(m, n, stream);
+    if (n == 0) return;
+    if (a.width == 4 && !m->nodes4)
+      throw Error(PBRT_ERR_INVALID_ARG, "the 4-wide tree is built by pbrt_mesh_set_launch; set the launch first");
+    a.rays = rays;
+    a.hits = hits;
+    if (!a.packet) {
+      const uint64_t depth = a.width == 4 ? 3ull * m->depth4 : m->stats.max_depth;
+      const uint64_t per_thread = depth + 1 > a.lds_stack ? depth + 1 - a.lds_stack : 1;
+      if (per_thread * a.block > kNoallocSpillEntries)
+        throw Error(PBRT_ERR_INVALID_ARG,
+                    "tree too deep for the no-alloc spill array: " + std::to_string(per_thread * a.block * sizeof(uint2)) +
+                        " bytes per workgroup needed, " + std::to_string(kNoallocSpillBytes) + " available");
+      // the persistent grid shrinks to the threads whose spill the array holds
+      a.max_threads = (uint32_t)std::min<uint64_t>(kNoallocSpillEntries / per_thread, 1u << 30);
+      a.spill = (uint2*)m->ctx->scratch;
+      a.spill_entries_per_thread = (uint32_t)per_thread;
+      a.spill_capacity_entries = kNoallocSpillEntries;
+    }
+    PBRT_HIP(launch_traversal(a, 0));
+  });
 }
 
 int pbrt_mesh_trace_work(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out_dev, int any_hit, uint32_t* work,

@@ -9,4 +9,6 @@ struct pbrt_ctx {
   int cu_count = 0;
   std::string name;
   hipStream_t stream = nullptr;
+  // Device address of this device's copy of the no-alloc entry point's spill array (api.cpp).
+  void* scratch = nullptr;
 };

@@ -127,6 +127,7 @@ void launch_shape(const LaunchArgs& a, uint32_t* grid) {
   const uint32_t g = a.cu_count * (a.waves_per_cu / per_block_waves > 0 ? a.waves_per_cu / per_block_waves : 1);
   const uint32_t need = (a.n + BLOCK - 1) / BLOCK;
   *grid = g < need ? g : need;
+  if (a.max_threads && *grid > a.max_threads / BLOCK) *grid = a.max_threads / BLOCK;
   if (*grid == 0) *grid = 1;
 }
 

@@ -25,6 +25,7 @@ struct LaunchArgs {
   int32_t root4;    // root reference in the 4-wide tree (the root box is root's)
   uint32_t packet;  // 1: closest hit by wave packets on the binary tree (k_closest_packet)
   uint32_t tile_w;  // 0: rays in batch order; W: an image W rays wide, taken in 8 x 16 tiles (not packets)
+  uint32_t max_threads;  // 0: the grid of waves_per_cu; else the persistent grid holds at most this many threads
 };
 // Pending subtrees the packet kernel's wave-wide stack holds: trees deeper than this need the
 // per-lane kernels.
