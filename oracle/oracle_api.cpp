@@ -417,6 +417,124 @@ int orc_scene_set_roulette(void* hv, int on) {
   return 0;
 }
 
+// ---- shading primitives, for the closed-form checks of the oracle ------------------------------
+}  // extern "C"
+
+namespace {
+// A canonical shading frame: p at the origin, n = +z, dpdu = +x (so the local frame is the world's).
+template <class R>
+Isect<R> canonical_isect(const double* wo) {
+  Isect<R> is{};
+  is.n = {0, 0, 1};
+  is.dpdu = {1, 0, 0};
+  is.dpdv = {0, 1, 0};
+  is.wo = {(R)wo[0], (R)wo[1], (R)wo[2]};
+  return is;
+}
+template <class R>
+double fresnel_impl(const double* wo, const double* n, double n1, double n2) {
+  return (double)SceneOracle<R>::fresnel({(R)wo[0], (R)wo[1], (R)wo[2]}, {(R)n[0], (R)n[1], (R)n[2]}, (R)n1, (R)n2);
+}
+template <class R>
+int refract_impl(const double* wo, const double* n, double eta, double* t) {
+  V3<R> r{};
+  const bool ok = SceneOracle<R>::refract({(R)wo[0], (R)wo[1], (R)wo[2]}, {(R)n[0], (R)n[1], (R)n[2]}, (R)eta, &r);
+  t[0] = r.x;
+  t[1] = r.y;
+  t[2] = r.z;
+  return ok ? 1 : 0;
+}
+template <class R>
+void material_eval_impl(const MaterialDesc* m, const double* wo, const double* wi, double* f, double* pdf) {
+  SceneOracle<R> s;
+  Isect<R> is = canonical_isect<R>(wo);
+  V3<R> wiv{(R)wi[0], (R)wi[1], (R)wi[2]};
+  Spec<R> v = s.mat_f(*m, is, is.wo, wiv);
+  for (int c = 0; c < 3; ++c) f[c] = v.c[c];
+  *pdf = s.bsdf_pdf(*m, is, is.wo, wiv);
+}
+template <class R>
+int material_scatter_impl(const MaterialDesc* m, const double* wo, uint64_t seed, uint64_t path, uint32_t bounce,
+                          double* att, double* dir, double* pdf, int* specular) {
+  SceneOracle<R> s;
+  Isect<R> is = canonical_isect<R>(wo);
+  typename SceneOracle<R>::Scatter sc{};
+  if (!s.scatter(*m, is, seed, path, bounce, &sc)) return 0;
+  for (int c = 0; c < 3; ++c) att[c] = sc.att.c[c];
+  dir[0] = sc.ray.d.x;
+  dir[1] = sc.ray.d.y;
+  dir[2] = sc.ray.d.z;
+  *pdf = sc.pdf;
+  *specular = sc.specular ? 1 : 0;
+  return 1;
+}
+template <class R>
+int sample_light_impl(const SceneOracle<R>* s, uint32_t light, const double* p, const double* n, uint64_t seed,
+                      uint64_t path, uint32_t bounce, double* li, double* wi, double* pdf, double* dist) {
+  if (light >= s->lights.size()) return -1;
+  Isect<R> is{};
+  is.p = {(R)p[0], (R)p[1], (R)p[2]};
+  is.n = {(R)n[0], (R)n[1], (R)n[2]};
+  typename SceneOracle<R>::LiSample ls{};
+  if (!s->sample_li(s->lights[light], is, seed, path, bounce, &ls)) return 0;
+  for (int c = 0; c < 3; ++c) li[c] = ls.li.c[c];
+  wi[0] = ls.wi.x;
+  wi[1] = ls.wi.y;
+  wi[2] = ls.wi.z;
+  *pdf = ls.pdf;
+  *dist = ls.dist;
+  return 1;
+}
+}  // namespace
+
+extern "C" {
+
+
+// dielectric.rs:168-182 (wo and n in the local frame: only their z components enter).
+double orc_fresnel(int precision, const double* wo, const double* n, double n1, double n2) {
+  return precision == 64 ? fresnel_impl<double>(wo, n, n1, n2) : fresnel_impl<float>(wo, n, n1, n2);
+}
+
+// dielectric.rs:129-157: 1 and the refracted direction, or 0 (total internal reflection).
+int orc_refract(int precision, const double* wo, const double* n, double eta, double* t) {
+  return precision == 64 ? refract_impl<double>(wo, n, eta, t) : refract_impl<float>(wo, n, eta, t);
+}
+
+double orc_ggx_d(int precision, double cos_h, double a) {
+  return precision == 64 ? SceneOracle<double>::ggx_d(cos_h, a) : (double)SceneOracle<float>::ggx_d((float)cos_h, (float)a);
+}
+double orc_ggx_g1(int precision, double cos_v, double a) {
+  return precision == 64 ? SceneOracle<double>::ggx_g1(cos_v, a)
+                         : (double)SceneOracle<float>::ggx_g1((float)cos_v, (float)a);
+}
+
+// Material::f and the density of its own sampling, for outgoing wo and incoming wi in the
+// canonical frame (n = +z).
+void orc_material_eval(int precision, const void* material, const double* wo, const double* wi, double* f, double* pdf) {
+  const auto* m = (const MaterialDesc*)material;
+  if (precision == 64) material_eval_impl<double>(m, wo, wi, f, pdf);
+  else material_eval_impl<float>(m, wo, wi, f, pdf);
+}
+
+// Material::scatter in the canonical frame under the counter sampler's draws for (seed, path,
+// bounce): 1 and the attenuation, direction, density and specular flag, or 0 (no scattering).
+int orc_material_scatter(int precision, const void* material, const double* wo, uint64_t seed, uint64_t path,
+                         uint32_t bounce, double* att, double* dir, double* pdf, int* specular) {
+  const auto* m = (const MaterialDesc*)material;
+  return precision == 64 ? material_scatter_impl<double>(m, wo, seed, path, bounce, att, dir, pdf, specular)
+                         : material_scatter_impl<float>(m, wo, seed, path, bounce, att, dir, pdf, specular);
+}
+
+// Light::sample_li of the scene's light `light` from point p (normal n) under the counter sampler's
+// draws: 1 and the radiance, direction, density (solid angle) and shadow-ray length, 0 when the
+// sample is not usable (the back of a one-sided emitter), -1 for no such light.
+int orc_scene_sample_light(void* hv, uint32_t light, const double* p, const double* n, uint64_t seed, uint64_t path,
+                           uint32_t bounce, double* li, double* wi, double* pdf, double* dist) {
+  auto* h = (SceneHandle*)hv;
+  return h->a ? sample_light_impl(h->a, light, p, n, seed, path, bounce, li, wi, pdf, dist)
+              : sample_light_impl(h->b, light, p, n, seed, path, bounce, li, wi, pdf, dist);
+}
+
 // Rows [y0, y1) of a width x height render: film receives the per-pixel mean radiance (f64 RGB).
 int orc_scene_render(void* hv, uint32_t w, uint32_t hgt, double fov, const double* eye, const double* look, uint32_t spp,
                      uint32_t max_depth, uint64_t seed, double near, double far, uint32_t y0, uint32_t y1, double* film,

@@ -297,3 +297,100 @@ def nrc_adam(params, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
     for a in (params, m, v):
         assert a.dtype == np.float32 and a.flags.c_contiguous
     L.orc_nrc_adam(_p(params), _p(m), _p(v), _p(grad), len(params), step, lr, b1, b2, eps)
+
+
+# ---- shading primitives (closed-form checks) -----------------------------------------------------
+def _shade_sigs(L):
+    if getattr(L, "_shade_sigs", False):
+        return
+    D = ctypes.c_double
+    L.orc_fresnel.restype = D
+    L.orc_fresnel.argtypes = [ctypes.c_int, P, P, D, D]
+    L.orc_refract.restype = ctypes.c_int
+    L.orc_refract.argtypes = [ctypes.c_int, P, P, D, P]
+    L.orc_ggx_d.restype = D
+    L.orc_ggx_d.argtypes = [ctypes.c_int, D, D]
+    L.orc_ggx_g1.restype = D
+    L.orc_ggx_g1.argtypes = [ctypes.c_int, D, D]
+    L.orc_material_eval.restype = None
+    L.orc_material_eval.argtypes = [ctypes.c_int, P, P, P, P, P]
+    L.orc_material_scatter.restype = ctypes.c_int
+    L.orc_material_scatter.argtypes = [ctypes.c_int, P, P, U64, U64, ctypes.c_uint32, P, P, P, P]
+    L.orc_scene_sample_light.restype = ctypes.c_int
+    L.orc_scene_sample_light.argtypes = [P, ctypes.c_uint32, P, P, U64, U64, ctypes.c_uint32, P, P, P, P]
+    L._shade_sigs = True
+
+
+def _v3(x):
+    return np.ascontiguousarray(x, np.float64).reshape(3)
+
+
+def fresnel(wo, n, n1, n2, precision=64):
+    """dielectric.rs:168-182 in the oracle (local frame: only the z components enter)."""
+    L = lib()
+    _shade_sigs(L)
+    return L.orc_fresnel(precision, _p(_v3(wo)), _p(_v3(n)), n1, n2)
+
+
+def refract(wo, n, eta, precision=64):
+    """dielectric.rs:129-157: the refracted direction, or None (total internal reflection)."""
+    L = lib()
+    _shade_sigs(L)
+    t = np.zeros(3)
+    ok = L.orc_refract(precision, _p(_v3(wo)), _p(_v3(n)), eta, _p(t))
+    return t if ok else None
+
+
+def ggx_d(cos_h, a, precision=64):
+    L = lib()
+    _shade_sigs(L)
+    return L.orc_ggx_d(precision, cos_h, a)
+
+
+def ggx_g1(cos_v, a, precision=64):
+    L = lib()
+    _shade_sigs(L)
+    return L.orc_ggx_g1(precision, cos_v, a)
+
+
+def material(kind, rgb, ior=1.5, roughness=0.0):
+    """A one-record material descriptor (the layout of pbrt_amd.scene.MATERIAL_DTYPE)."""
+    from pbrt_amd.scene import MATERIAL, MATERIAL_DTYPE
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["kind"], m["rgb"], m["ior"], m["roughness"] = MATERIAL[kind], rgb, ior, roughness
+    return m
+
+
+def material_eval(m, wo, wi, precision=64):
+    """Material::f(wo, wi) and the density of the material's own sampling of wi, in the canonical
+    frame (normal +z, tangent +x)."""
+    L = lib()
+    _shade_sigs(L)
+    f = np.zeros(3)
+    pdf = np.zeros(1)
+    L.orc_material_eval(precision, _p(m), _p(_v3(wo)), _p(_v3(wi)), _p(f), _p(pdf))
+    return f, float(pdf[0])
+
+
+def material_scatter(m, wo, seed, path, bounce=0, precision=64):
+    """Material::scatter in the canonical frame under the counter sampler's draws: dict(att, dir,
+    pdf, specular), or None when the material does not scatter."""
+    L = lib()
+    _shade_sigs(L)
+    att, d, pdf, spec = np.zeros(3), np.zeros(3), np.zeros(1), ctypes.c_int()
+    ok = L.orc_material_scatter(precision, _p(m), _p(_v3(wo)), seed, path, bounce, _p(att), _p(d), _p(pdf),
+                                ctypes.byref(spec))
+    return dict(att=att, dir=d, pdf=float(pdf[0]), specular=bool(spec.value)) if ok else None
+
+
+def scene_sample_light(scene, light, p, n, seed, path, bounce=0):
+    """Light::sample_li of the oracle scene's light `light` from p (normal n): dict(li, wi, pdf,
+    dist), or None when the sample is unusable."""
+    L = lib()
+    _shade_sigs(L)
+    li, wi, pdf, dist = np.zeros(3), np.zeros(3), np.zeros(1), np.zeros(1)
+    ok = L.orc_scene_sample_light(scene.h, light, _p(_v3(p)), _p(_v3(n)), seed, path, bounce, _p(li), _p(wi), _p(pdf),
+                                  _p(dist))
+    if ok < 0:
+        raise IndexError("no such light")
+    return dict(li=li, wi=wi, pdf=float(pdf[0]), dist=float(dist[0])) if ok else None
