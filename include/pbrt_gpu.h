@@ -309,8 +309,17 @@ typedef struct pbrt_render_config {
                                 has it commented out (continue with probability q = the throughput's
                                 largest component, capped at 1 here, the throughput divided by q);
                                 not with the radiance cache. 0: paths end only at max_depth */
-  uint32_t reserved;
+  uint32_t integrator;       /* the reference's integrators::Type (src/integrators.rs:23-28, chosen in
+                                src/main.rs:16-20): PBRT_INTEGRATOR_PATH (0), PBRT_INTEGRATOR_NAIVE or
+                                PBRT_INTEGRATOR_NORMAL; any other value is PBRT_ERR_INVALID_ARG, and so
+                                are mis, roulette and the radiance cache with anything but PATH */
 } pbrt_render_config;
+#define PBRT_INTEGRATOR_PATH 0   /* PathIntegrator::li, integrators/path.rs:48-115 */
+#define PBRT_INTEGRATOR_NAIVE 1  /* NaiveIntegrator::li, integrators/naive.rs:30-54: emission at every hit,
+                                    the scattered ray followed to max_depth, no next-event estimation,
+                                    the sky gradient of naive.rs:21-26 on a miss */
+#define PBRT_INTEGRATOR_NORMAL 2 /* NormalIntegrator::li, integrators/normal.rs:17-30: (n + 1) / 2 of the
+                                    camera ray's hit, black on a miss */
 
 typedef struct pbrt_render_stats {
   uint64_t rays;   /* every ray traced: camera, extension and shadow */

@@ -410,6 +410,15 @@ int orc_scene_set_mis(void* hv, int mis) {
   return 0;
 }
 
+// The integrator of li (integrators.rs:23-28): 0 path, 1 naive, 2 normal.
+int orc_scene_set_integrator(void* hv, int kind) {
+  auto* h = (SceneHandle*)hv;
+  if (kind < 0 || kind > 2) return -1;
+  if (h->a) h->a->integrator = kind;
+  if (h->b) h->b->integrator = kind;
+  return 0;
+}
+
 int orc_scene_set_roulette(void* hv, int on) {
   auto* h = (SceneHandle*)hv;
   if (h->a) h->a->roulette = on != 0;

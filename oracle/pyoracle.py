@@ -149,6 +149,8 @@ def _scene_sigs(L):
     L.orc_scene_set_mis.argtypes = [P, ctypes.c_int]
     L.orc_scene_set_roulette.restype = ctypes.c_int
     L.orc_scene_set_roulette.argtypes = [P, ctypes.c_int]
+    L.orc_scene_set_integrator.restype = ctypes.c_int
+    L.orc_scene_set_integrator.argtypes = [P, ctypes.c_int]
     L.orc_scene_render.restype = ctypes.c_int
     L.orc_scene_render.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, ctypes.c_uint32,
                                    ctypes.c_uint32, U64, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
@@ -219,10 +221,13 @@ class Scene:
         return dict(rays=int(out[0]), primary_hits=int(out[1]), shadow_hits=int(out[2]),
                     primary=st[:3] / out[0], shadow=st[3:] / max(int(out[1]), 1))
 
+    INTEGRATORS = {"path": 0, "naive": 1, "normal": 2}
+
     def render(self, w, h, fov, eye, look, spp, max_depth, seed, near=1e-4, far=1000.0, rows=None, threads=8,
-               mis=False, roulette=False):
+               mis=False, roulette=False, integrator="path"):
         lib().orc_scene_set_mis(self.h, int(bool(mis)))
         lib().orc_scene_set_roulette(self.h, int(bool(roulette)))
+        lib().orc_scene_set_integrator(self.h, self.INTEGRATORS[integrator])
         y0, y1 = rows if rows is not None else (0, h)
         film = np.zeros(((y1 - y0) * w, 3), np.float64)
         rays = ctypes.c_uint64()

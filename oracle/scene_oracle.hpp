@@ -28,6 +28,8 @@
 #include <limits>
 #include <cstdint>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "camera_oracle.hpp"
 #include "mesh_oracle.hpp"
@@ -178,6 +180,7 @@ class SceneOracle {
   bool canonical;
   bool mis = false;  // multiple importance sampling (power heuristic) instead of the specular guard
   bool roulette = false;  // Russian roulette after each scattering (path.rs:93-100, commented out there)
+  int integrator = 0;     // integrators.rs:23-28: 0 PathIntegrator, 1 NaiveIntegrator, 2 NormalIntegrator
   // scene BVH in the reference's layout (bvh.rs:15-31)
   struct SNode {
     AABB<double> box;
@@ -556,9 +559,21 @@ class SceneOracle {
         }
       csg.push_back(e);
     }
-    for (uint32_t i = 0; i < n_mesh; ++i) {
-      uint64_t nv = (vert_off[i + 1] - vert_off[i]), nt = (idx_off[i + 1] - idx_off[i]);
-      meshes.push_back(new MeshBVH<R>(verts + 3 * vert_off[i], nv, idx + 3 * idx_off[i], nt, bp));
+    // the meshes' trees are independent: built by up to 16 threads, each taking the next mesh
+    meshes.assign(n_mesh, nullptr);
+    {
+      std::atomic<uint32_t> next{0};
+      auto work = [&] {
+        for (uint32_t i = next++; i < n_mesh; i = next++) {
+          uint64_t nv = (vert_off[i + 1] - vert_off[i]), nt = (idx_off[i + 1] - idx_off[i]);
+          meshes[i] = new MeshBVH<R>(verts + 3 * vert_off[i], nv, idx + 3 * idx_off[i], nt, bp);
+        }
+      };
+      const uint32_t nthr = std::min<uint32_t>(n_mesh, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+      std::vector<std::thread> pool;
+      for (uint32_t t = 1; t < nthr; ++t) pool.emplace_back(work);
+      work();
+      for (auto& th : pool) th.join();
     }
     mats.assign(m, m + n_mat);
     lights.assign(l, l + n_light);
@@ -990,7 +1005,52 @@ class SceneOracle {
   // the power heuristic against the BSDF's density of the light's direction, and the emission a
   // BSDF-sampled ray meets after a non-specular vertex added too, weighted the other way (an
   // emitter no light samples gets weight 1).
+  // NormalIntegrator::li (integrators/normal.rs:17-30): (n + 1) * 0.5 of the hit, black on a miss
+  // (the trait's background_radiance, integrators.rs:9-11).
+  Spec<R> li_normal(const Ray<R>& cam, R near, R far, uint64_t* rays_cast) const {
+    SceneHit<R> hit;
+    SStats st;
+    (*rays_cast)++;
+    if (!intersect(cam, near, far, &hit, &st)) return {{0, 0, 0}};
+    const V3<R> n = hit.is.n;
+    return {{(n.x + R(1)) * R(0.5), (n.y + R(1)) * R(0.5), (n.z + R(1)) * R(0.5)}};
+  }
+  // NaiveIntegrator::li (integrators/naive.rs:30-54), its recursion unrolled: at every hit the
+  // emission times the throughput, then the scattered ray with the throughput times
+  // attenuation * |cos| / pdf, until the depth is spent (li at depth 0 is black) or the material does
+  // not scatter; on a miss the integrator's own sky (naive.rs:21-26) times the throughput. The
+  // draws of vertex k are the path integrator's (bounce k).
+  Spec<R> li_naive(const Ray<R>& cam, R near, R far, uint32_t max_depth, uint64_t seed, uint64_t path,
+                   uint64_t* rays_cast) const {
+    Spec<R> beta{{1, 1, 1}}, L{{0, 0, 0}};
+    Ray<R> ray = cam;
+    SStats st;
+    for (uint32_t bounce = 1; bounce <= max_depth; ++bounce) {
+      SceneHit<R> hit;
+      (*rays_cast)++;
+      if (!intersect(ray, near, far, &hit, &st)) {
+        V3<R> u = normalized(ray.d);
+        R t = R(0.5) * (u.y + R(1));
+        Spec<R> sky{{R(1) * (R(1) - t) + R(0.5) * t, R(1) * (R(1) - t) + R(0.7) * t, R(1) * (R(1) - t) + R(1) * t}};
+        L = sadd(L, smul(sky, beta));
+        break;
+      }
+      const MaterialDesc m = shaded(mats[objs[hit.object].d.material], hit.is);
+      if (m.kind == kDiffuseLight) L = sadd(L, smul(albedo(m), beta));
+      if (bounce == max_depth) break;
+      Scatter s;
+      if (!scatter(m, hit.is, seed, path, bounce, &s)) break;
+      R c = std::fabs(dot(s.ray.d, hit.is.n));
+      beta = smul(beta, sdiv(sscale(s.att, c), s.pdf));
+      if (!(std::isfinite(beta.c[0]) && std::isfinite(beta.c[1]) && std::isfinite(beta.c[2]))) break;
+      ray = s.ray;
+    }
+    return L;
+  }
+
   Spec<R> li(const Ray<R>& cam, R near, R far, uint32_t max_depth, uint64_t seed, uint64_t path, uint64_t* rays_cast) const {
+    if (integrator == 2) return li_normal(cam, near, far, rays_cast);
+    if (integrator == 1) return li_naive(cam, near, far, max_depth, seed, path, rays_cast);
     Spec<R> beta{{1, 1, 1}}, L{{0, 0, 0}};
     Ray<R> ray = cam;
     bool specular = true;

@@ -1304,7 +1304,18 @@ template <bool NRC>
 __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderParams& p, const PathBuffers& b,
                                              uint32_t j, ShadeOut* out) {
   const SceneHitRec h = b.hit[j];
-  if (h.obj < 0 && S.n_background == 0) return;  // the path leaves the scene: it ends
+  const bool naive = p.integrator == PBRT_INTEGRATOR_NAIVE;
+  if (p.integrator == PBRT_INTEGRATOR_NORMAL) {
+    // normal.rs:17-30: the camera ray's hit normal mapped to (n + 1) / 2 (black on a miss); the path
+    // ends at its first vertex
+    if (h.obj >= 0) {
+      const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
+      const Isect is = interaction(S, h, V{o4.x, o4.y, o4.z}, V{d4.x, d4.y, d4.z}, false);
+      b.L[j] = make_float4((is.n.x + 1.0f) * 0.5f, (is.n.y + 1.0f) * 0.5f, (is.n.z + 1.0f) * 0.5f, 0.f);
+    }
+    return;
+  }
+  if (h.obj < 0 && S.n_background == 0 && !naive) return;  // the path leaves the scene: it ends
   uint4 info = b.info[j];
   // the path's job (a batch of jobs) or the render's own parameters
   uint32_t spp = p.spp, max_depth = p.max_depth;
@@ -1325,6 +1336,14 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   // beta.w: 1 after a specular bounce (and for the camera ray), else minus the BSDF density of the
   // direction that led here (0 without multiple importance sampling)
   const bool specular = beta4.w > 0.0f;
+  if (h.obj < 0 && naive) {
+    // naive.rs:21-26: the integrator's own sky, white to (0.5, 0.7, 1.0) with the direction's y
+    const V u = vnorm(rd);
+    const float t = 0.5f * (u.y + 1.0f);
+    const V sky{1.0f * (1.0f - t) + 0.5f * t, 1.0f * (1.0f - t) + 0.7f * t, 1.0f * (1.0f - t) + 1.0f * t};
+    b.L[j] = make_float4(L4.x + sky.x * beta4.x, L4.y + sky.y * beta4.y, L4.z + sky.z * beta4.z, L4.w);
+    return;
+  }
   if (h.obj < 0) {
     // path.rs:101-108: the background's radiance after a specular bounce; with MIS also after a
     // non-specular one, weighted against next-event estimation's density
@@ -1356,7 +1375,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
     m.rgb[2] = m0.rgb2[2];
   }
   V emitted{0.f, 0.f, 0.f};
-  if (specular && m.kind == kMatDiffuseLight) {
+  if ((specular || naive) && m.kind == kMatDiffuseLight) {  // naive.rs:36-39: at every hit
     emitted = {m.rgb[0] * beta4.x, m.rgb[1] * beta4.y, m.rgb[2] * beta4.z};
     L4.x = L4.x + emitted.x;
     L4.y = L4.y + emitted.y;
@@ -1400,7 +1419,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
     }
   }
   if (bounce == max_depth) return;
-  if (S.n_lights > 0) {
+  if (S.n_lights > 0 && !naive) {
     const float u = urand(seed, path, bounce, kSlotLight);
     const float fi = fminf(u * (float)S.n_lights, (float)S.n_lights - 1.0f);
     const DevLight Ld = S.lights[(uint32_t)fi];

@@ -124,6 +124,7 @@ struct RenderParams {
   uint32_t n_jobs, pass_paths, batch_pixels;
   uint32_t mis;  // multiple importance sampling (power heuristic) instead of the specular guard
   uint32_t roulette;  // Russian roulette after each scattering
+  uint32_t integrator;  // PBRT_INTEGRATOR_PATH, _NAIVE or _NORMAL
 };
 
 // Per-vertex record of a training path, field-major: rec[(f * max_depth + m - 1) * train_cap + rank]

@@ -168,6 +168,15 @@ void spill_mark(pbrt_scene* s, hipStream_t st) {
   s->spill_stream = st;
 }
 
+// The integrator of a render (integrators.rs:23-28): the path integrator takes every option; the
+// naive and normal integrators of the reference have none of them.
+void check_integrator(const pbrt_render_config& c, bool with_cache) {
+  if (c.integrator > PBRT_INTEGRATOR_NORMAL)
+    throw Error(PBRT_ERR_INVALID_ARG, "integrator must be PBRT_INTEGRATOR_PATH, _NAIVE or _NORMAL");
+  if (c.integrator != PBRT_INTEGRATOR_PATH && (c.mis || c.roulette || with_cache))
+    throw Error(PBRT_ERR_INVALID_ARG, "mis, roulette and the radiance cache are options of the path integrator");
+}
+
 // The HBM spill area for a grid: a lane's stack holds at most (top-level + mesh depth) entries.
 uint32_t ensure_spill(pbrt_scene* s, uint32_t grid) {
   // pending entries of the 4-wide walk: at most three per level
@@ -1082,6 +1091,8 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     if (c->roulette > 1) throw Error(PBRT_ERR_INVALID_ARG, "roulette must be 0 or 1");
     if (c->roulette && nrc) throw Error(PBRT_ERR_INVALID_ARG, "Russian roulette is not supported with the radiance cache");
     p.roulette = c->roulette;
+    check_integrator(*c, nrc != nullptr);
+    p.integrator = c->integrator;
 
     // radiance cache buffers
     NrcTrainBuffers tb{};
@@ -1445,6 +1456,12 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
       if (cfg[k].roulette != cfg[0].roulette || cfg[k].roulette > 1)
         throw Error(PBRT_ERR_INVALID_ARG, "roulette must be 0 or 1, the same for every job");
     p.roulette = cfg[0].roulette;
+    for (uint32_t k = 0; k < n; ++k) {
+      check_integrator(cfg[k], false);
+      if (cfg[k].integrator != cfg[0].integrator)
+        throw Error(PBRT_ERR_INVALID_ARG, "the integrator must be the same for every job");
+    }
+    p.integrator = cfg[0].integrator;
     p.batch_pixels = (uint32_t)pix;
     const SceneView v = view_of(s);
     const uint32_t grid = persistent_grid(s, (uint32_t)capacity);
