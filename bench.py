@@ -161,7 +161,8 @@ def first_below(prog, bar):
     return float(prog["ms"][k]), int(prog["spp"][k])
 
 
-def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096, plain_spp=1024):
+def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096, plain_spp=1024, mis=False,
+                sampler="independent"):
     """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
     cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
     max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device
@@ -182,19 +183,20 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
     print(f"[bench] nrc: reference {ref_spp} spp in {ref_pr['ms'][-1] / 1e3:.1f}s, mean radiance {ref.mean():.4f}",
           file=sys.stderr, flush=True)
     plain, plain_st, plain_pr = scene.render_ex(W, H, plain_spp, D, E, L, seed=8, reference=ref, samples_per_pass=4,
-                                                **view)
+                                                mis=mis, sampler=sampler, **view)
     t_plain, spp_plain = first_below(plain_pr, bar)
     cache = pa.RadianceCache(ctx, seed=4)
     film, nrc_st, nrc_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=cache, reference=ref,
                                            samples_per_pass=frames_per_pass, query_vertex=3, train_stride=33,
-                                           train_batch=65536, train_steps=1, **view)
+                                           train_batch=65536, train_steps=1, mis=mis, sampler=sampler, **view)
     t_nrc, f_nrc = first_below(nrc_pr, bar)
     k256 = int(np.searchsorted(nrc_pr["spp"], 256))
     # the same cache-terminated render without the learned part (paths end at vertex 3 with nothing)
     zero = pa.RadianceCache(ctx, seed=4)
     zero.params = np.zeros(pa.NRC_PARAMS, np.float32)
     cut, _, cut_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=zero, reference=ref,
-                                     samples_per_pass=frames_per_pass, train_stride=0, **view)
+                                     samples_per_pass=frames_per_pass, train_stride=0, mis=mis, sampler=sampler,
+                                     **view)
     print(f"[bench] nrc: plain reaches RMSE {bar} at {spp_plain} spp / {t_plain} ms; cache at {f_nrc} frames / "
           f"{t_nrc} ms; after {frames} frames cache RMSE {nrc_pr['rmse'][-1]:.4f} (cut {cut_pr['rmse'][-1]:.4f})",
           file=sys.stderr, flush=True)
@@ -227,6 +229,7 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
     train_tf = n_tr * 3 * flop_q / (train_ms * 1e-3) / 1e12
     return {
         "metric": f"time to RMSE {bar}, radiance cache vs plain path tracing",
+        "sampler": sampler, "mis": mis,
         "workload": "room with the 1,310,720-triangle displaced icosphere and GGX materials (roughness "
                     "U[0.05,0.5]), 1280x720, max depth 8; RMSE of linear radiance against a "
                     f"{ref_spp}-spp plain render (mean radiance {ref.mean():.4f})",
@@ -404,6 +407,12 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="autotune each batch's launch instead of the fixed picks")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
+    ap.add_argument("--nrc-sampler", default="independent", choices=["independent", "sobol"],
+                    help="sampler of both sides of the radiance-cache race (the reference render keeps the "
+                         "independent sampler)")
+    ap.add_argument("--nrc-mis", action="store_true",
+                    help="radiance-cache race with multiple importance sampling on both sides (the reference render "
+                         "keeps the reference estimator)")
     ap.add_argument("--no-jobs", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing throughput leg")
     ap.add_argument("--render-only", action="store_true", help="run only the path-tracing throughput leg")
@@ -421,7 +430,7 @@ def main():
     import pbrt_amd as pa
 
     if args.nrc_only:
-        print(json.dumps(nrc_measure(pa, pa.Context(0))))
+        print(json.dumps(nrc_measure(pa, pa.Context(0), mis=args.nrc_mis, sampler=args.nrc_sampler)))
         return
     if args.jobs_only:
         print(json.dumps(jobs_measure(pa, pa.Context(0))))

@@ -313,7 +313,17 @@ typedef struct pbrt_render_config {
                                 src/main.rs:16-20): PBRT_INTEGRATOR_PATH (0), PBRT_INTEGRATOR_NAIVE or
                                 PBRT_INTEGRATOR_NORMAL; any other value is PBRT_ERR_INVALID_ARG, and so
                                 are mis, roulette and the radiance cache with anything but PATH */
+  uint32_t sampler;          /* PBRT_SAMPLER_INDEPENDENT (0): every uniform from the counter keyed by
+                                (seed, pixel * spp + sample, bounce, slot); PBRT_SAMPLER_SOBOL (1): the
+                                pixel jitter and, at vertices 1 and 2, the light and BSDF samples' pairs
+                                from the pixel's Owen-scrambled Sobol (0,2)-sequence at its sample index
+                                (stratified across a pixel's samples; the other draws as for 0). The
+                                reference samples with an unseeded thread RNG (utils.rs:5-7); both are
+                                unbiased, the second converges faster at equal spp */
+  uint32_t reserved;
 } pbrt_render_config;
+#define PBRT_SAMPLER_INDEPENDENT 0
+#define PBRT_SAMPLER_SOBOL 1
 #define PBRT_INTEGRATOR_PATH 0   /* PathIntegrator::li, integrators/path.rs:48-115 */
 #define PBRT_INTEGRATOR_NAIVE 1  /* NaiveIntegrator::li, integrators/naive.rs:30-54: emission at every hit,
                                     the scattered ray followed to max_depth, no next-event estimation,

@@ -283,6 +283,7 @@ void render_impl(const SceneOracle<R>* s, uint32_t w, uint32_t h, double fov, co
                  uint32_t spp, uint32_t max_depth, uint64_t seed, double near, double far, uint32_t y0, uint32_t y1,
                  double* film, uint64_t* rays_cast, int threads) {
   PinHole cam(w, h, fov, near, far, m4_look_at({eye[0], eye[1], eye[2]}, {look[0], look[1], look[2]}, {0, 1, 0}));
+  s->draw_spp = spp;
   std::vector<uint64_t> per(std::max(threads, 1), 0);
   const uint64_t npix = (uint64_t)w * (y1 - y0);
   parallel_for(npix, threads, [&](uint64_t k, int tid) {
@@ -293,7 +294,7 @@ void render_impl(const SceneOracle<R>* s, uint32_t w, uint32_t h, double fov, co
     for (uint32_t sidx = 0; sidx < spp; ++sidx) {
       const uint64_t path = pixel * spp + sidx;
       // renderers/mt.rs:123-127: pixel + jitter, camera ray, li
-      double jx = rand_u(seed, path, 0, kJitterX), jy = rand_u(seed, path, 0, kJitterY);
+      double jx = s->jitter(seed, pixel, sidx, spp, kJitterX), jy = s->jitter(seed, pixel, sidx, spp, kJitterY);
       Ray<double> cr = cam.get_ray((double)px + jx, (double)py + jy);
       Ray<R> ray{{(R)cr.o.x, (R)cr.o.y, (R)cr.o.z}, {(R)cr.d.x, (R)cr.d.y, (R)cr.d.z}};
       Spec<R> L = s->li(ray, (R)near, (R)far, max_depth, seed, path, &per[tid]);
@@ -399,6 +400,22 @@ int orc_scene_bvh_stats(void* hv, uint32_t w, uint32_t hgt, double fov, const do
   out[2] = sh;
   stats[0] = ps.nodes_visited; stats[1] = ps.box_tests; stats[2] = ps.object_tests;
   stats[3] = ss.nodes_visited; stats[4] = ss.box_tests; stats[5] = ss.object_tests;
+  return 0;
+}
+
+// 32 bits of coordinate `axis` of the Sobol sampler's point `sample` of pixel `pixel` in stratified
+// pair `pair`, for n consecutive samples from `first`.
+void orc_sobol_bits(uint64_t seed, uint64_t pixel, uint32_t first, uint32_t n, uint32_t pair, uint32_t axis,
+                    uint32_t* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = sobol_bits(seed, pixel, first + i, pair, axis);
+}
+
+// The sampler of the renders (0: counter-based draws; 1: stratified pairs from scrambled Sobol points).
+int orc_scene_set_sampler(void* hv, int kind) {
+  auto* h = (SceneHandle*)hv;
+  if (kind < 0 || kind > 1) return -1;
+  if (h->a) h->a->sampler = kind;
+  if (h->b) h->b->sampler = kind;
   return 0;
 }
 

@@ -151,6 +151,8 @@ def _scene_sigs(L):
     L.orc_scene_set_roulette.argtypes = [P, ctypes.c_int]
     L.orc_scene_set_integrator.restype = ctypes.c_int
     L.orc_scene_set_integrator.argtypes = [P, ctypes.c_int]
+    L.orc_scene_set_sampler.restype = ctypes.c_int
+    L.orc_scene_set_sampler.argtypes = [P, ctypes.c_int]
     L.orc_scene_render.restype = ctypes.c_int
     L.orc_scene_render.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, P, P, ctypes.c_uint32,
                                    ctypes.c_uint32, U64, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
@@ -223,11 +225,14 @@ class Scene:
 
     INTEGRATORS = {"path": 0, "naive": 1, "normal": 2}
 
+    SAMPLERS = {"independent": 0, "sobol": 1}
+
     def render(self, w, h, fov, eye, look, spp, max_depth, seed, near=1e-4, far=1000.0, rows=None, threads=8,
-               mis=False, roulette=False, integrator="path"):
+               mis=False, roulette=False, integrator="path", sampler="independent"):
         lib().orc_scene_set_mis(self.h, int(bool(mis)))
         lib().orc_scene_set_roulette(self.h, int(bool(roulette)))
         lib().orc_scene_set_integrator(self.h, self.INTEGRATORS[integrator])
+        lib().orc_scene_set_sampler(self.h, self.SAMPLERS[sampler])
         y0, y1 = rows if rows is not None else (0, h)
         film = np.zeros(((y1 - y0) * w, 3), np.float64)
         rays = ctypes.c_uint64()
@@ -399,3 +404,15 @@ def scene_sample_light(scene, light, p, n, seed, path, bounce=0):
     if ok < 0:
         raise IndexError("no such light")
     return dict(li=li, wi=wi, pdf=float(pdf[0]), dist=float(dist[0])) if ok else None
+
+
+def sobol_bits(seed, pixel, pair, axis, n, first=0):
+    """The Sobol sampler's 32-bit coordinates of samples first .. first + n - 1 of a pixel."""
+    L = lib()
+    if not getattr(L, "_sobol_sigs", False):
+        L.orc_sobol_bits.restype = None
+        L.orc_sobol_bits.argtypes = [U64, U64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P]
+        L._sobol_sigs = True
+    out = np.zeros(n, np.uint32)
+    L.orc_sobol_bits(seed, pixel, first, n, pair, axis, _p(out))
+    return out

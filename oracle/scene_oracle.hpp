@@ -63,6 +63,55 @@ template <>
 inline float uni<float>(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
   return (float)(hash4(seed, path, bounce, slot) >> 40) * 0x1p-24f;
 }
+// ---- the Sobol sampler's stratified draws -------------------------------------------------------
+// Owen-scrambled Sobol (0,2)-sequences by hashing (Burley, "Practical Hash-based Owen Scrambling",
+// JCGT 9(4), 2020): per pixel and per stratified pair of draws, the sample index is shuffled by a
+// nested uniform scramble, the point is (radical inverse, second Sobol dimension) of the shuffled
+// index, and each coordinate is scrambled the same way.
+inline uint32_t bit_reverse(uint32_t x) {
+  uint32_t r = 0;
+  for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
+  return r;
+}
+inline uint32_t owen_hash(uint32_t x, uint32_t seed) {  // Burley's Laine-Karras style permutation
+  x += seed;
+  x ^= x * 0x6c50b47cu;
+  x ^= x * 0xb82f1e52u;
+  x ^= x * 0xc7afe638u;
+  x ^= x * 0x8d22f6e6u;
+  return x;
+}
+inline uint32_t owen_scramble(uint32_t x, uint32_t seed) { return bit_reverse(owen_hash(bit_reverse(x), seed)); }
+// Sobol's second dimension: the generator matrix is Pascal's triangle mod 2 (columns v_k with
+// v_0 = 2^31 and v_k = v_{k-1} xor (v_{k-1} >> 1)).
+inline uint32_t sobol_second(uint32_t i) {
+  uint32_t r = 0, v = 1u << 31;
+  for (int k = 0; k < 32; ++k) {
+    if ((i >> k) & 1u) r ^= v;
+    v ^= v >> 1;
+  }
+  return r;
+}
+// The stratified pairs: the jitter (bounce 0, slots 0, 1) is pair 0; at bounces 1 and 2 the light
+// sample (slots 3, 4) is pair 2b - 1 and the BSDF sample (slots 5, 6) pair 2b; -1 otherwise.
+inline int stratified_pair(uint32_t bounce, uint32_t slot, uint32_t* axis) {
+  if (bounce == 0 && (slot == 0 || slot == 1)) {
+    *axis = slot;
+    return 0;
+  }
+  if ((bounce == 1 || bounce == 2) && slot >= 3 && slot <= 6) {
+    *axis = (slot == 4 || slot == 6) ? 1u : 0u;
+    return (int)(2 * bounce - 1) + (slot >= 5 ? 1 : 0);
+  }
+  return -1;
+}
+inline uint32_t sobol_bits(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t pair, uint32_t axis) {
+  const uint64_t h = hash4(seed, pixel, 0x50B0Cu + pair, 0);
+  const uint32_t idx = owen_scramble(sample, (uint32_t)h);
+  const uint32_t pt = axis == 0 ? bit_reverse(idx) : sobol_second(idx);
+  return owen_scramble(pt, (uint32_t)(h >> 32) ^ (axis ? 0x9E3779B9u : 0u));
+}
+
 enum Slot : uint32_t {
   kJitterX = 0, kJitterY = 1, kLightPick = 2, kLightU = 3, kLightV = 4, kScatter0 = 5, kScatter1 = 6, kCoin = 7,
   kMetal0 = 8,  // Metal's point in the unit ball: attempt k uses slots kMetal0 + 3k .. + 2
@@ -181,6 +230,23 @@ class SceneOracle {
   bool mis = false;  // multiple importance sampling (power heuristic) instead of the specular guard
   bool roulette = false;  // Russian roulette after each scattering (path.rs:93-100, commented out there)
   int integrator = 0;     // integrators.rs:23-28: 0 PathIntegrator, 1 NaiveIntegrator, 2 NormalIntegrator
+  int sampler = 0;        // 0: every draw counter-based; 1: the stratified pairs from scrambled Sobol points
+  mutable uint32_t draw_spp = 1;  // samples per pixel of the render in progress (path = pixel * spp + sample)
+
+  // One uniform of path `path` at the working precision.
+  R draw(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) const {
+    uint32_t axis = 0;
+    const int pair = sampler ? stratified_pair(bounce, slot, &axis) : -1;
+    if (pair < 0) return uni<R>(seed, path, bounce, slot);
+    const uint32_t bits = sobol_bits(seed, path / draw_spp, (uint32_t)(path % draw_spp), (uint32_t)pair, axis);
+    if (sizeof(R) == 4) return (R)((float)(bits >> 8) * 0x1p-24f);
+    return (R)((double)bits * 0x1p-32);
+  }
+  // The f64 pixel jitter (renderers/mt.rs:123-127).
+  double jitter(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t spp, uint32_t slot) const {
+    if (sampler) return (double)sobol_bits(seed, pixel, sample, 0, slot) * 0x1p-32;
+    return rand_u(seed, pixel * spp + sample, 0, slot);
+  }
   // scene BVH in the reference's layout (bvh.rs:15-31)
   struct SNode {
     AABB<double> box;
@@ -782,8 +848,8 @@ class SceneOracle {
     if (m.kind == kLambertian) {
       // lambertian.rs:24-38, pdfs/cosine.rs:14-24
       R two_pi = R(2) * R(M_PI);
-      R phi = two_pi * uni<R>(seed, path, bounce, kScatter0);
-      R r2 = uni<R>(seed, path, bounce, kScatter1);
+      R phi = two_pi * draw(seed, path, bounce, kScatter0);
+      R r2 = draw(seed, path, bounce, kScatter1);
       R r2s = std::sqrt(r2);
       V3<R> lwi{std::cos(phi) * r2s, std::sin(phi) * r2s, std::sqrt(R(1) - r2)};
       V3<R> wi = to_world(is, lwi);
@@ -801,9 +867,9 @@ class SceneOracle {
       V3<R> lrefl{-lwo.x, -lwo.y, lwo.z};
       V3<R> q{0, 0, 0};
       for (uint32_t k = 0; k < kMetalAttempts; ++k) {
-        V3<R> c{uni<R>(seed, path, bounce, kMetal0 + 3 * k) * R(2) - R(1),
-                uni<R>(seed, path, bounce, kMetal0 + 3 * k + 1) * R(2) - R(1),
-                uni<R>(seed, path, bounce, kMetal0 + 3 * k + 2) * R(2) - R(1)};
+        V3<R> c{draw(seed, path, bounce, kMetal0 + 3 * k) * R(2) - R(1),
+                draw(seed, path, bounce, kMetal0 + 3 * k + 1) * R(2) - R(1),
+                draw(seed, path, bounce, kMetal0 + 3 * k + 2) * R(2) - R(1)};
         if (dot(c, c) < R(1)) {
           q = c;
           break;
@@ -821,7 +887,7 @@ class SceneOracle {
       V3<R> wo = to_local(is, is.wo);
       if (wo.z <= R(0)) return false;
       R a = (R)m.roughness;
-      R u1 = uni<R>(seed, path, bounce, kScatter0), u2 = uni<R>(seed, path, bounce, kScatter1);
+      R u1 = draw(seed, path, bounce, kScatter0), u2 = draw(seed, path, bounce, kScatter1);
       // half vector from D(h) cos(theta_h): tan^2 = a^2 u / (1 - u)
       R phi = R(2) * R(M_PI) * u1;
       R cos_t = std::sqrt((R(1) - u2) / (R(1) + (a * a - R(1)) * u2));
@@ -882,7 +948,7 @@ class SceneOracle {
     V3<R> ldir, org;
     if (refract(lwo, lnorm, ni / nt, &lrefr)) {
       R refl = fresnel(lwo, lnorm, ni, nt);
-      if (uni<R>(seed, path, bounce, kCoin) < refl) {
+      if (draw(seed, path, bounce, kCoin) < refl) {
         ldir = lrefl;
         org = is.p + wnorm * R(0.001);
       } else {
@@ -923,8 +989,8 @@ class SceneOracle {
     if (L.kind == kBackgroundLight) {
       // background_infinite_light.rs:31-44: a direction uniform on the sphere (pdfs/sphere.rs:
       // 14-23), VisibilityTester::towards_infinity (lights.rs:57-62)
-      R phi = R(2) * R(M_PI) * uni<R>(seed, path, bounce, kLightU);
-      R cos_t = R(1) - R(2) * uni<R>(seed, path, bounce, kLightV);
+      R phi = R(2) * R(M_PI) * draw(seed, path, bounce, kLightU);
+      R cos_t = R(1) - R(2) * draw(seed, path, bounce, kLightV);
       R sin_t = std::sqrt(R(1) - cos_t * cos_t);
       V3<R> wi{std::cos(phi) * sin_t, std::sin(phi) * sin_t, cos_t};
       R factor = R(0.5) * (normalized(wi).y + R(1));
@@ -938,7 +1004,7 @@ class SceneOracle {
     }
     const Obj& o = objs[L.object];
     R w = (R)o.d.param[0], h = (R)o.d.param[1];
-    R u1 = uni<R>(seed, path, bounce, kLightU), u2 = uni<R>(seed, path, bounce, kLightV);
+    R u1 = draw(seed, path, bounce, kLightU), u2 = draw(seed, path, bounce, kLightV);
     V3<R> pl_local{(u1 - R(0.5)) * w, R(0), (u2 - R(0.5)) * h};
     V3<R> pl = o.d.has_xform ? o.x.point(o.x.m, pl_local) : pl_local;
     V3<R> nl = o.d.has_xform ? normalized(o.x.normal(o.x.inv, V3<R>{0, 1, 0})) : V3<R>{0, 1, 0};
@@ -1083,7 +1149,7 @@ class SceneOracle {
       if (bounce == max_depth) break;
       if (!lights.empty()) {
         uint32_t n = (uint32_t)lights.size();
-        R u = uni<R>(seed, path, bounce, kLightPick);
+        R u = draw(seed, path, bounce, kLightPick);
         R fi = std::fmin(u * (R)n, (R)n - R(1));
         const LightDesc& Ld = lights[(uint32_t)fi];
         R pl = R(1) / (R)n;
@@ -1108,7 +1174,7 @@ class SceneOracle {
         // at 1 here, so that the estimator stays unbiased), and divide the throughput by q; the path
         // survives only when the draw is below q, so a zero throughput (q = 0) always ends it
         const R q = std::fmin(R(1), std::fmax(std::fmax(beta.c[0], beta.c[1]), beta.c[2]));
-        if (!(uni<R>(seed, path, bounce, kRoulette) < q)) break;
+        if (!(draw(seed, path, bounce, kRoulette) < q)) break;
         beta = sdiv(beta, q);
       }
       ray = s.ray;

@@ -24,6 +24,8 @@ RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4
 HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 # integrators::Type (src/integrators.rs:23-28) -> pbrt_render_config.integrator
 INTEGRATOR = {"path": 0, "naive": 1, "normal": 2}
+# pbrt_render_config.sampler
+SAMPLER = {"independent": 0, "sobol": 1}
 
 # The reference's mesh BVH parameters (triangle_mesh/bvh.rs:129, 733-735) and the device default.
 REFERENCE_BVH = dict(bins=8, max_leaf=0, t_trav=0.0)
@@ -484,7 +486,7 @@ class Scene:
 
     @staticmethod
     def _config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis=False,
-                roulette=False, integrator="path"):
+                roulette=False, integrator="path", sampler="independent"):
         c = _native.RenderConfig()
         c.width, c.height, c.spp, c.max_depth, c.seed, c.fov = width, height, spp, max_depth, seed, fov
         c.eye[:], c.look[:], c.up[:] = list(eye), list(look), list(up)
@@ -492,16 +494,19 @@ class Scene:
         c.mis = int(bool(mis))
         c.roulette = int(bool(roulette))
         c.integrator = INTEGRATOR[integrator] if isinstance(integrator, str) else int(integrator)
+        c.sampler = SAMPLER[sampler] if isinstance(sampler, str) else int(sampler)
         return c
 
     def render(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
-               near=1e-4, far=1000.0, samples_per_pass=0, film=True, mis=False, roulette=False, integrator="path"):
+               near=1e-4, far=1000.0, samples_per_pass=0, film=True, mis=False, roulette=False, integrator="path",
+               sampler="independent"):
         """Per-pixel mean radiance (height, width, 3) float64 and the render statistics; `mis`:
         multiple importance sampling instead of the reference's estimator; `roulette`: Russian
         roulette after each scattering; `integrator`: "path", "naive" or "normal" (the reference's
-        integrators::Type)."""
+        integrators::Type); `sampler`: "independent" (counter-based draws) or "sobol" (stratified
+        pixel jitter and first-vertex light / BSDF pairs)."""
         c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis,
-                         roulette, integrator)
+                         roulette, integrator, sampler)
         out = np.zeros((height, width, 3), np.float64) if film else None
         st = _native.RenderStats()
         call("pbrt_render", self.h, ctypes.byref(c), _ptr(out) if film else None, ctypes.byref(st))
@@ -518,7 +523,8 @@ class Scene:
             cfgs[k] = self._config(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"], j["look"],
                                    j.get("up", (0.0, 1.0, 0.0)), j.get("fov", np.pi / 2), j.get("seed", 1),
                                    j.get("near", 1e-4), j.get("far", 1000.0), 0, j.get("mis", False),
-                                   j.get("roulette", False), j.get("integrator", "path"))
+                                   j.get("roulette", False), j.get("integrator", "path"),
+                                   j.get("sampler", "independent"))
             films.append(np.zeros((j["height"], j["width"], 3), np.float64))
         ptrs = (ctypes.c_void_p * n)(*[f.ctypes.data for f in films])
         st = _native.RenderStats()
@@ -528,13 +534,13 @@ class Scene:
     def render_ex(self, width, height, spp, max_depth, eye, look, up=(0.0, 1.0, 0.0), fov=np.pi / 2, seed=1,
                   near=1e-4, far=1000.0, samples_per_pass=0, cache=None, query_vertex=3, train_stride=33,
                   train_batch=65536, train_steps=1, reference=None, record=True, debug_slots=0, mis=False,
-                  roulette=False, integrator="path"):
+                  roulette=False, integrator="path", sampler="independent"):
         """Render with the radiance cache `cache` (a RadianceCache, or None for plain path tracing)
         and, with `record`, the per-pass time (ms) and RMSE against `reference` (height, width, 3).
         Returns (film, stats, progress); progress has `ms`, `rmse` (or None) and `spp` per pass, and
         `debug` (raw inputs, targets) of the last pass's training slots when debug_slots > 0."""
         c = self._config(width, height, spp, max_depth, eye, look, up, fov, seed, near, far, samples_per_pass, mis,
-                         roulette, integrator)
+                         roulette, integrator, sampler)
         out = np.zeros((height, width, 3), np.float64)
         st = _native.RenderStats()
         nc = None

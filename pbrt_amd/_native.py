@@ -38,7 +38,8 @@ class SceneLaunch(ctypes.Structure):
 class RenderConfig(ctypes.Structure):
     _fields_ = [("width", U32), ("height", U32), ("spp", U32), ("max_depth", U32), ("seed", U64), ("fov", F64),
                 ("eye", F64 * 3), ("look", F64 * 3), ("up", F64 * 3), ("near", F64), ("far", F64),
-                ("samples_per_pass", U32), ("mis", U32), ("roulette", U32), ("integrator", U32)]
+                ("samples_per_pass", U32), ("mis", U32), ("roulette", U32), ("integrator", U32),
+                ("sampler", U32), ("reserved", U32)]
 
 
 class RenderStats(ctypes.Structure):

@@ -886,8 +886,22 @@ __device__ __forceinline__ V to_world(const Isect& i, V v) {
           ss.z * v.x + ts.z * v.y + ns.z * v.z};
 }
 
-__device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot) {
+// One uniform of path `path` (= pixel * spp + sample). sobol_spp = 0: the counter-based draw;
+// sobol_spp = spp (the Sobol sampler): the stratified pairs' draws come from the pixel's scrambled
+// Sobol points at its sample index (rng.hpp), the others stay counter-based.
+__device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bounce, uint32_t slot, uint32_t sobol_spp) {
+  uint32_t axis = 0;
+  const int pair = sobol_spp ? sobol_pair(bounce, slot, &axis) : -1;
+  if (pair >= 0)
+    return (float)(sobol_draw32(seed, path / sobol_spp, (uint32_t)(path % sobol_spp), (uint32_t)pair, axis) >> 8) *
+           0x1p-24f;
   return to_unit_f32(hash4(seed, path, bounce, slot));
+}
+// The f64 pixel jitter of the camera ray (renderers/mt.rs:123-127), under the same rule.
+__device__ __forceinline__ double jitter(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t slot,
+                                        uint32_t sobol) {
+  if (sobol) return (double)sobol_draw32(seed, pixel, sample, 0, slot) * 0x1p-32;
+  return to_unit(hash4(seed, (uint64_t)pixel * spp + sample, 0, slot));
 }
 enum : uint32_t {
   kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7,
@@ -969,14 +983,14 @@ struct ScatterOut {
 // branch computes into locals and the output is written once (stores to different fields from
 // different branches would otherwise keep the output in private memory).
 __device__ __forceinline__ bool scatter(const DevMaterial& m, const Isect& is, uint64_t seed, uint64_t path,
-                                        uint32_t bounce, ScatterOut* s) {
+                                        uint32_t bounce, uint32_t sobol, ScatterOut* s) {
   V att, org, dir;
   float pdf;
   bool spec;
   if (m.kind == kMatLambertian) {
     const float two_pi = 2.0f * kPi;
-    const float phi = two_pi * urand(seed, path, bounce, kSlotS0);
-    const float r2 = urand(seed, path, bounce, kSlotS1);
+    const float phi = two_pi * urand(seed, path, bounce, kSlotS0, sobol);
+    const float r2 = urand(seed, path, bounce, kSlotS1, sobol);
     const float r2s = sqrtf(r2);
     const V lwi{cosf(phi) * r2s, sinf(phi) * r2s, sqrtf(1.0f - r2)};
     const V wi = to_world(is, lwi);
@@ -1010,7 +1024,7 @@ __device__ __forceinline__ bool scatter(const DevMaterial& m, const Isect& is, u
       const float ro = (ni * fci - nt * fct) / (ni * fci + nt * fct);
       const float rp = (nt * fci - ni * fct) / (nt * fci + ni * fct);
       const float refl = (ro * ro + rp * rp) * 0.5f;
-      reflect = urand(seed, path, bounce, kSlotCoin) < refl;
+      reflect = urand(seed, path, bounce, kSlotCoin, sobol) < refl;
     }
     const V ldir = reflect ? lrefl : lrefr;
     org = vadd(is.p, vmul(wnorm, reflect ? 0.001f : -0.001f));
@@ -1027,9 +1041,9 @@ __device__ __forceinline__ bool scatter(const DevMaterial& m, const Isect& is, u
     V q{0.f, 0.f, 0.f};
 #pragma unroll 1
     for (uint32_t k = 0; k < kMetalAttempts; ++k) {
-      const V c{urand(seed, path, bounce, kSlotMetal0 + 3 * k) * 2.0f - 1.0f,
-                urand(seed, path, bounce, kSlotMetal0 + 3 * k + 1) * 2.0f - 1.0f,
-                urand(seed, path, bounce, kSlotMetal0 + 3 * k + 2) * 2.0f - 1.0f};
+      const V c{urand(seed, path, bounce, kSlotMetal0 + 3 * k, sobol) * 2.0f - 1.0f,
+                urand(seed, path, bounce, kSlotMetal0 + 3 * k + 1, sobol) * 2.0f - 1.0f,
+                urand(seed, path, bounce, kSlotMetal0 + 3 * k + 2, sobol) * 2.0f - 1.0f};
       if (vdot(c, c) < 1.0f) {
         q = c;
         break;
@@ -1047,7 +1061,7 @@ __device__ __forceinline__ bool scatter(const DevMaterial& m, const Isect& is, u
     const V wo = to_local(is, is.wo);
     if (wo.z <= 0.0f) return false;
     const float a = m.roughness;
-    const float u1 = urand(seed, path, bounce, kSlotS0), u2 = urand(seed, path, bounce, kSlotS1);
+    const float u1 = urand(seed, path, bounce, kSlotS0, sobol), u2 = urand(seed, path, bounce, kSlotS1, sobol);
     const float phi = 2.0f * kPi * u1;
     const float cos_t = sqrtf((1.0f - u2) / (1.0f + (a * a - 1.0f) * u2));
     const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
@@ -1079,7 +1093,7 @@ struct LiOut {
 // Light::sample_li: point_light.rs:23-35 + lights.rs:38-46; area light: uniform point on the
 // rectangle, one-sided emission, solid-angle pdf d^2 / (A cos) (ideas/area_light.md §3).
 __device__ __forceinline__ bool sample_li(const SceneView& S, const DevLight& L, const Isect& is, uint64_t seed, uint64_t path,
-                          uint32_t bounce, LiOut* s) {
+                          uint32_t bounce, uint32_t sobol, LiOut* s) {
   if (L.kind == kLightPoint) {
     const V pos{L.pos[0], L.pos[1], L.pos[2]};
     const V wi = vsub(pos, is.p);
@@ -1095,8 +1109,8 @@ __device__ __forceinline__ bool sample_li(const SceneView& S, const DevLight& L,
   if (L.kind == kLightBackground) {
     // background_infinite_light.rs:31-44: a direction uniform on the sphere (pdfs/sphere.rs:14-23),
     // VisibilityTester::towards_infinity (lights.rs:57-62)
-    const float phi = 2.0f * kPi * urand(seed, path, bounce, kSlotLu);
-    const float cos_t = 1.0f - 2.0f * urand(seed, path, bounce, kSlotLv);
+    const float phi = 2.0f * kPi * urand(seed, path, bounce, kSlotLu, sobol);
+    const float cos_t = 1.0f - 2.0f * urand(seed, path, bounce, kSlotLv, sobol);
     const float sin_t = sqrtf(1.0f - cos_t * cos_t);
     const V wi{cosf(phi) * sin_t, sinf(phi) * sin_t, cos_t};
     const float factor = 0.5f * (vnorm(wi).y + 1.0f);
@@ -1111,7 +1125,7 @@ __device__ __forceinline__ bool sample_li(const SceneView& S, const DevLight& L,
   }
   const DevObject& o = S.objs[L.object];
   const float w = o.param[0], h = o.param[1];
-  const float u1 = urand(seed, path, bounce, kSlotLu), u2 = urand(seed, path, bounce, kSlotLv);
+  const float u1 = urand(seed, path, bounce, kSlotLu, sobol), u2 = urand(seed, path, bounce, kSlotLv, sobol);
   const V pl_local{(u1 - 0.5f) * w, 0.0f, (u2 - 0.5f) * h};
   const V pl = o.has_xform ? xpoint(o.m, pl_local) : pl_local;
   const V nl = o.has_xform ? vnorm(xnormal(o.minv, V{0.f, 1.f, 0.f})) : V{0.f, 1.f, 0.f};
@@ -1214,8 +1228,8 @@ __global__ void k_generate(RenderParams p, PathBuffers b) {
           b.rec[((size_t)f * p.max_depth + m) * b.train_cap + rank] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const uint32_t pixel = j % npix, sample = p.pass_first_sample + j / npix;
-    const uint64_t path = (uint64_t)pixel * p.spp + sample;
-    const double jx = to_unit(hash4(p.seed, path, 0, kSlotJx)), jy = to_unit(hash4(p.seed, path, 0, kSlotJy));
+    const double jx = jitter(p.seed, pixel, sample, p.spp, kSlotJx, p.sampler),
+                 jy = jitter(p.seed, pixel, sample, p.spp, kSlotJy, p.sampler);
     double cv[3], o[3], d[3];
     cam_point(p.r2s, (double)(pixel % p.width) + jx, (double)(pixel / p.width) + jy, 0.0, cv);
     dnorm(cv);
@@ -1330,6 +1344,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
     far = J.far;
   }
   const uint64_t path = (uint64_t)info.x * spp + info.y;
+  const uint32_t sobol = p.sampler ? spp : 0u;
   float4 beta4 = b.beta[j], L4 = b.L[j];
   const float4 o4 = b.ray_o[j], d4 = b.ray_d[j];
   const V rd{d4.x, d4.y, d4.z};
@@ -1420,12 +1435,12 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
   }
   if (bounce == max_depth) return;
   if (S.n_lights > 0 && !naive) {
-    const float u = urand(seed, path, bounce, kSlotLight);
+    const float u = urand(seed, path, bounce, kSlotLight, sobol);
     const float fi = fminf(u * (float)S.n_lights, (float)S.n_lights - 1.0f);
     const DevLight Ld = S.lights[(uint32_t)fi];
     const float pl = 1.0f / (float)S.n_lights;
     LiOut ls;
-    if (sample_li(S, Ld, is, seed, path, bounce, &ls)) {
+    if (sample_li(S, Ld, is, seed, path, bounce, sobol, &ls)) {
       const V f0 = mat_f(m, is, vmul(rd, -1.0f), ls.wi);
       const float c = fabsf(vdot(ls.wi, is.n));
       const V f{f0.x * c, f0.y * c, f0.z * c};
@@ -1443,7 +1458,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
     }
   }
   ScatterOut sc;
-  if (!scatter(m, is, seed, path, bounce, &sc)) return;
+  if (!scatter(m, is, seed, path, bounce, sobol, &sc)) return;
   const float c = fabsf(vdot(sc.d, is.n));
   beta4.x = beta4.x * ((sc.att.x * c) / sc.pdf);
   beta4.y = beta4.y * ((sc.att.y * c) / sc.pdf);
@@ -1456,7 +1471,7 @@ __device__ __forceinline__ void shade_vertex(const SceneView& S, const RenderPar
     // throughput component capped at 1, the throughput divided by q. The path survives only when
     // the draw is below q, so a throughput of zero (q = 0) always ends it (never 0 / 0)
     const float q = fminf(1.0f, fmaxf(fmaxf(beta4.x, beta4.y), beta4.z));
-    if (!(urand(seed, path, bounce, kSlotRoulette) < q)) return;
+    if (!(urand(seed, path, bounce, kSlotRoulette, sobol) < q)) return;
     beta4.x = beta4.x / q;
     beta4.y = beta4.y / q;
     beta4.z = beta4.z / q;
@@ -1579,8 +1594,8 @@ __global__ void k_generate_jobs(RenderParams p, PathBuffers b) {
     const JobDesc& J = p.jobs[k];
     const uint32_t npix = J.width * J.height, local = j - J.path_base;
     const uint32_t pixel = local % npix, sample = J.pass_first + local / npix;
-    const uint64_t path = (uint64_t)pixel * J.spp + sample;
-    const double jx = to_unit(hash4(J.seed, path, 0, kSlotJx)), jy = to_unit(hash4(J.seed, path, 0, kSlotJy));
+    const double jx = jitter(J.seed, pixel, sample, J.spp, kSlotJx, p.sampler),
+                 jy = jitter(J.seed, pixel, sample, J.spp, kSlotJy, p.sampler);
     double cv[3], o[3], d[3];
     cam_point(J.r2s, (double)(pixel % J.width) + jx, (double)(pixel / J.width) + jy, 0.0, cv);
     dnorm(cv);

@@ -125,6 +125,7 @@ struct RenderParams {
   uint32_t mis;  // multiple importance sampling (power heuristic) instead of the specular guard
   uint32_t roulette;  // Russian roulette after each scattering
   uint32_t integrator;  // PBRT_INTEGRATOR_PATH, _NAIVE or _NORMAL
+  uint32_t sampler;     // PBRT_SAMPLER_INDEPENDENT or PBRT_SAMPLER_SOBOL
 };
 
 // Per-vertex record of a training path, field-major: rec[(f * max_depth + m - 1) * train_cap + rank]

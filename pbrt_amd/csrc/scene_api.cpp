@@ -1093,6 +1093,8 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     p.roulette = c->roulette;
     check_integrator(*c, nrc != nullptr);
     p.integrator = c->integrator;
+    if (c->sampler > PBRT_SAMPLER_SOBOL) throw Error(PBRT_ERR_INVALID_ARG, "sampler must be PBRT_SAMPLER_INDEPENDENT or _SOBOL");
+    p.sampler = c->sampler;
 
     // radiance cache buffers
     NrcTrainBuffers tb{};
@@ -1462,6 +1464,10 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
         throw Error(PBRT_ERR_INVALID_ARG, "the integrator must be the same for every job");
     }
     p.integrator = cfg[0].integrator;
+    for (uint32_t k = 0; k < n; ++k)
+      if (cfg[k].sampler != cfg[0].sampler || cfg[k].sampler > PBRT_SAMPLER_SOBOL)
+        throw Error(PBRT_ERR_INVALID_ARG, "sampler must be PBRT_SAMPLER_INDEPENDENT or _SOBOL, the same for every job");
+    p.sampler = cfg[0].sampler;
     p.batch_pixels = (uint32_t)pix;
     const SceneView v = view_of(s);
     const uint32_t grid = persistent_grid(s, (uint32_t)capacity);
