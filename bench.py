@@ -161,45 +161,81 @@ def first_below(prog, bar):
     return float(prog["ms"][k]), int(prog["spp"][k])
 
 
-def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096, plain_spp=1024, mis=False,
-                sampler="independent"):
-    """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
-    cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
-    max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device
-    after every pass; times are HIP events on the render stream from the first launch of the render.
-    Plain path tracing runs passes of 4 samples per pixel; the cache renders 1 sample per pixel per
-    frame, `frames_per_pass` frames per pass, one training step of 2^16 samples per frame from the
-    3 % training paths, and ends the other paths into the cache at their third vertex. The cache
-    render runs past the workload's 256 frames when RMSE `bar` needs more."""
-    from pbrt_amd import scenes
-    desc, cam = scenes.nrc_room()
-    t0 = time.time()
-    scene = pa.Scene(ctx, desc)
-    build_s = time.time() - t0
+NRC_VARIANTS = {
+    # the headline race: the reference's estimator (next-event estimation, emission after specular
+    # bounces) under the Sobol sampler, on both sides
+    "sobol": dict(sampler="sobol", mis=False),
+    "sobol_mis": dict(sampler="sobol", mis=True),
+    "independent": dict(sampler="independent", mis=False),
+    "independent_mis": dict(sampler="independent", mis=True),
+}
+
+
+def nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, sampler, mis):
+    """One race to RMSE `bar` against `ref`: plain path tracing against the radiance cache, both with
+    the given sampler and estimator, and the cache-terminated render without the learned part."""
     W, H, D = cam["width"], cam["height"], cam["max_depth"]
-    view = dict(fov=cam["fov"])
+    view = dict(fov=cam["fov"], mis=mis, sampler=sampler)
     E, L = cam["eye"], cam["look"]
-    ref, ref_st, ref_pr = scene.render_ex(W, H, ref_spp, D, E, L, seed=7, **view)
-    print(f"[bench] nrc: reference {ref_spp} spp in {ref_pr['ms'][-1] / 1e3:.1f}s, mean radiance {ref.mean():.4f}",
-          file=sys.stderr, flush=True)
     plain, plain_st, plain_pr = scene.render_ex(W, H, plain_spp, D, E, L, seed=8, reference=ref, samples_per_pass=4,
-                                                mis=mis, sampler=sampler, **view)
+                                                **view)
     t_plain, spp_plain = first_below(plain_pr, bar)
     cache = pa.RadianceCache(ctx, seed=4)
     film, nrc_st, nrc_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=cache, reference=ref,
                                            samples_per_pass=frames_per_pass, query_vertex=3, train_stride=33,
-                                           train_batch=65536, train_steps=1, mis=mis, sampler=sampler, **view)
+                                           train_batch=65536, train_steps=1, **view)
     t_nrc, f_nrc = first_below(nrc_pr, bar)
     k256 = int(np.searchsorted(nrc_pr["spp"], 256))
     # the same cache-terminated render without the learned part (paths end at vertex 3 with nothing)
     zero = pa.RadianceCache(ctx, seed=4)
     zero.params = np.zeros(pa.NRC_PARAMS, np.float32)
     cut, _, cut_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=zero, reference=ref,
-                                     samples_per_pass=frames_per_pass, train_stride=0, mis=mis, sampler=sampler,
-                                     **view)
-    print(f"[bench] nrc: plain reaches RMSE {bar} at {spp_plain} spp / {t_plain} ms; cache at {f_nrc} frames / "
-          f"{t_nrc} ms; after {frames} frames cache RMSE {nrc_pr['rmse'][-1]:.4f} (cut {cut_pr['rmse'][-1]:.4f})",
+                                     samples_per_pass=frames_per_pass, train_stride=0, **view)
+    print(f"[bench] nrc ({sampler}, mis {mis}): plain reaches RMSE {bar} at {spp_plain} spp / {t_plain} ms; cache at "
+          f"{f_nrc} frames / {t_nrc} ms; at 256 frames cache RMSE {nrc_pr['rmse'][k256]:.4f}; after {frames} frames "
+          f"{nrc_pr['rmse'][-1]:.4f} (cut {cut_pr['rmse'][-1]:.4f})", file=sys.stderr, flush=True)
+    return cache, {
+        "sampler": sampler, "mis": mis,
+        "plain_ms": t_plain, "plain_spp": spp_plain,
+        "cache_ms": t_nrc, "cache_frames": f_nrc,
+        "time_ratio": (t_nrc / t_plain) if (t_nrc and t_plain) else None,
+        "cache_rmse_256_frames": float(nrc_pr["rmse"][k256]),
+        "plain_rmse_256_spp": float(plain_pr["rmse"][int(np.searchsorted(plain_pr["spp"], 256))]),
+        "cache_rmse_final": float(nrc_pr["rmse"][-1]), "cut_rmse_final": float(cut_pr["rmse"][-1]),
+        "cache_ms_per_frame": float(nrc_pr["ms"][-1] / frames),
+        "plain_ms_per_spp": float(plain_pr["ms"][-1] / plain_pr["spp"][-1]),
+        "plain_mrays_s": plain_st["rays"] / (plain_pr["ms"][-1] * 1e-3) / 1e6,
+        "plain_samples_s": W * H * plain_pr["spp"][-1] / (plain_pr["ms"][-1] * 1e-3),
+        "cache_samples_s": W * H * frames / (nrc_pr["ms"][-1] * 1e-3),
+        "cache_queries_per_frame": nrc_st["cache_queries"] / frames,
+    }
+
+
+def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096, plain_spp=1024,
+                variants=("sobol", "sobol_mis")):
+    """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
+    cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
+    max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device
+    after every pass; times are HIP events on the render stream from the first launch of the render.
+    Plain path tracing runs passes of 4 samples per pixel; the cache renders 1 sample per pixel per
+    frame, `frames_per_pass` frames per pass, one training step of 2^16 samples per frame from the
+    3 % training paths, and ends the other paths into the cache at their third vertex. Each variant
+    (NRC_VARIANTS: sampler and estimator, the same on both sides) races against the one reference
+    render (independent sampler, the reference's estimator); the first variant is the headline."""
+    from pbrt_amd import scenes
+    desc, cam = scenes.nrc_room()
+    t0 = time.time()
+    scene = pa.Scene(ctx, desc)
+    build_s = time.time() - t0
+    W, H, D = cam["width"], cam["height"], cam["max_depth"]
+    ref, ref_st, ref_pr = scene.render_ex(W, H, ref_spp, D, cam["eye"], cam["look"], seed=7, fov=cam["fov"])
+    print(f"[bench] nrc: reference {ref_spp} spp in {ref_pr['ms'][-1] / 1e3:.1f}s, mean radiance {ref.mean():.4f}",
           file=sys.stderr, flush=True)
+    races = {}
+    cache = None
+    for v in variants:
+        c, races[v] = nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, **NRC_VARIANTS[v])
+        cache = cache or c
 
     # MLP throughput: inference over one frame's worth of queries, a training step of 2^16
     n_inf, n_tr = W * H, 65536
@@ -227,30 +263,21 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
     flop_q = 2 * (81 * 64 + 3 * 64 * 64 + 64 * 3)
     infer_tf = n_inf * flop_q / (infer_ms * 1e-3) / 1e12
     train_tf = n_tr * 3 * flop_q / (train_ms * 1e-3) / 1e12
-    return {
+    head = races[variants[0]]
+    return dict({
         "metric": f"time to RMSE {bar}, radiance cache vs plain path tracing",
-        "sampler": sampler, "mis": mis,
         "workload": "room with the 1,310,720-triangle displaced icosphere and GGX materials (roughness "
-                    "U[0.05,0.5]), 1280x720, max depth 8; RMSE of linear radiance against a "
+                    "U[0.05,0.5]), 1280x720, 1 spp per frame, max depth 8; RMSE of linear radiance against a "
                     f"{ref_spp}-spp plain render (mean radiance {ref.mean():.4f})",
-        "plain_ms": t_plain, "plain_spp": spp_plain,
-        "cache_ms": t_nrc, "cache_frames": f_nrc,
-        "time_ratio": (t_nrc / t_plain) if (t_nrc and t_plain) else None,
-        "cache_rmse_final": float(nrc_pr["rmse"][-1]), "cut_rmse_final": float(cut_pr["rmse"][-1]),
-        "cache_rmse_256_frames": float(nrc_pr["rmse"][k256]),
-        "plain_rmse_256_spp": float(plain_pr["rmse"][int(np.searchsorted(plain_pr["spp"], 256))]),
+        "headline_variant": variants[0],
         "frames": frames, "frames_per_pass": frames_per_pass,
-        "cache_ms_per_frame": float(nrc_pr["ms"][-1] / frames),
-        "plain_ms_per_spp": float(plain_pr["ms"][-1] / plain_pr["spp"][-1]),
-        "plain_mrays_s": plain_st["rays"] / (plain_pr["ms"][-1] * 1e-3) / 1e6,
-        "plain_samples_s": W * H * plain_pr["spp"][-1] / (plain_pr["ms"][-1] * 1e-3),
-        "cache_samples_s": W * H * frames / (nrc_pr["ms"][-1] * 1e-3),
         "reference_s": float(ref_pr["ms"][-1] / 1e3), "scene_build_s": build_s,
+        "variants": races,
         "mlp": {"infer_ms": infer_ms, "infer_queries": n_inf, "infer_tflops": infer_tf,
                 "train_step_ms": train_ms, "train_samples": n_tr, "train_tflops": train_tf,
                 "peak_bf16_tflops": 2500.0, "infer_frac": infer_tf / 2500.0, "train_frac": train_tf / 2500.0,
                 "flop_per_query": flop_q},
-    }
+    }, **head)
 
 
 def jobs_measure(pa, ctx):
@@ -407,12 +434,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="autotune each batch's launch instead of the fixed picks")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
-    ap.add_argument("--nrc-sampler", default="independent", choices=["independent", "sobol"],
-                    help="sampler of both sides of the radiance-cache race (the reference render keeps the "
-                         "independent sampler)")
-    ap.add_argument("--nrc-mis", action="store_true",
-                    help="radiance-cache race with multiple importance sampling on both sides (the reference render "
-                         "keeps the reference estimator)")
+    ap.add_argument("--nrc-variants", default="sobol,sobol_mis",
+                    help="radiance-cache races, comma-separated, the first the headline: " + ", ".join(NRC_VARIANTS))
     ap.add_argument("--no-jobs", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing throughput leg")
     ap.add_argument("--render-only", action="store_true", help="run only the path-tracing throughput leg")
@@ -430,7 +453,7 @@ def main():
     import pbrt_amd as pa
 
     if args.nrc_only:
-        print(json.dumps(nrc_measure(pa, pa.Context(0), mis=args.nrc_mis, sampler=args.nrc_sampler)))
+        print(json.dumps(nrc_measure(pa, pa.Context(0), variants=tuple(args.nrc_variants.split(",")))))
         return
     if args.jobs_only:
         print(json.dumps(jobs_measure(pa, pa.Context(0))))
@@ -605,7 +628,7 @@ def main():
 
     nrc = None
     if not args.no_nrc:
-        nrc = nrc_measure(pa, ctx)
+        nrc = nrc_measure(pa, ctx, variants=tuple(args.nrc_variants.split(",")))
     jobs = None
     if not args.no_jobs:
         jobs = jobs_measure(pa, ctx)

@@ -187,6 +187,7 @@ template <int BLOCK>
 hipError_t launch_b(const LaunchArgs& a, int kind) {
   switch (a.lds_stack) {
     case 8: return launch_t<BLOCK, 8>(a, kind);
+    case 12: return launch_t<BLOCK, 12>(a, kind);
     case 16: return launch_t<BLOCK, 16>(a, kind);
     case 24: return launch_t<BLOCK, 24>(a, kind);
     case 32: return launch_t<BLOCK, 32>(a, kind);

@@ -108,7 +108,7 @@ def test_any_hit_and_far_sweep(gpu_ctx, refill, width):
 
 
 @pytest.mark.parametrize("block", [64, 128, 256])
-@pytest.mark.parametrize("lds_stack", [8, 16, 32])
+@pytest.mark.parametrize("lds_stack", [8, 12, 16, 32])
 @pytest.mark.parametrize("refill", [0, 4])
 def test_launch_shapes_agree(gpu_ctx, block, lds_stack, refill):
     """Every launch shape and refill policy gives the oracle's answer, including an 8-entry LDS
@@ -131,7 +131,7 @@ def test_wide_tree_published_meshes(gpu_ctx, name, which):
 
 
 @pytest.mark.parametrize("block", [64, 256])
-@pytest.mark.parametrize("lds_stack", [8, 32])
+@pytest.mark.parametrize("lds_stack", [8, 12, 32])
 @pytest.mark.parametrize("refill", [0, 4])
 def test_wide_tree_launch_shapes(gpu_ctx, block, lds_stack, refill):
     """The 4-wide tree over launch shapes, with the 8-entry LDS stack spilling on the deep
