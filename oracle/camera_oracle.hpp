@@ -128,6 +128,35 @@ struct PinHole {
     V3<double> d = normalized(m4_dir(cam_to_world, cv));
     return {o, d};
   }
+  // The same ray in FP32 (the renderer's camera in Oracle B): both matrices rounded to FP32 once,
+  // then matrix4.rs:187-206's statements and the two normalisations in FP32.
+  Ray<float> get_ray_f32(float px, float py) const {
+    float r[4][4], c[4][4];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        r[i][j] = (float)raster_to_screen.m[i][j];
+        c[i][j] = (float)cam_to_world.m[i][j];
+      }
+    auto point = [](const float (*m)[4], float x, float y, float z) {
+      float a = m[0][0] * x + m[0][1] * y + m[0][2] * z + m[0][3];
+      float b = m[1][0] * x + m[1][1] * y + m[1][2] * z + m[1][3];
+      float e = m[2][0] * x + m[2][1] * y + m[2][2] * z + m[2][3];
+      float w = m[3][0] * x + m[3][1] * y + m[3][2] * z + m[3][3];
+      if (w != 1.0f) {
+        float iw = 1.0f / w;
+        a = a * iw;
+        b = b * iw;
+        e = e * iw;
+      }
+      return V3<float>{a, b, e};
+    };
+    V3<float> cv = normalized(point(r, px, py, 0.0f));
+    V3<float> o = point(c, 0.0f, 0.0f, 0.0f);
+    V3<float> d = normalized(V3<float>{c[0][0] * cv.x + c[0][1] * cv.y + c[0][2] * cv.z,
+                                       c[1][0] * cv.x + c[1][1] * cv.y + c[1][2] * cv.z,
+                                       c[2][0] * cv.x + c[2][1] * cv.y + c[2][2] * cv.z});
+    return {o, d};
+  }
 };
 
 // src/bin/bvh_stats.rs:95-116, 345-401: the six orbit cameras framing the mesh's bounding sphere.

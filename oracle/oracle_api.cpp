@@ -294,9 +294,17 @@ void render_impl(const SceneOracle<R>* s, uint32_t w, uint32_t h, double fov, co
     for (uint32_t sidx = 0; sidx < spp; ++sidx) {
       const uint64_t path = pixel * spp + sidx;
       // renderers/mt.rs:123-127: pixel + jitter, camera ray, li
-      double jx = s->jitter(seed, pixel, sidx, spp, kJitterX), jy = s->jitter(seed, pixel, sidx, spp, kJitterY);
-      Ray<double> cr = cam.get_ray((double)px + jx, (double)py + jy);
-      Ray<R> ray{{(R)cr.o.x, (R)cr.o.y, (R)cr.o.z}, {(R)cr.d.x, (R)cr.d.y, (R)cr.d.z}};
+      // the f64 camera (the reference's arithmetic) for Oracle A, the renderer's FP32 camera for B
+      Ray<R> ray;
+      if (sizeof(R) == 8) {
+        double jx = s->jitter(seed, pixel, sidx, spp, kJitterX), jy = s->jitter(seed, pixel, sidx, spp, kJitterY);
+        Ray<double> cr = cam.get_ray((double)px + jx, (double)py + jy);
+        ray = {{(R)cr.o.x, (R)cr.o.y, (R)cr.o.z}, {(R)cr.d.x, (R)cr.d.y, (R)cr.d.z}};
+      } else {
+        float jx = s->jitter_f32(seed, pixel, sidx, spp, kJitterX), jy = s->jitter_f32(seed, pixel, sidx, spp, kJitterY);
+        Ray<float> cr = cam.get_ray_f32((float)px + jx, (float)py + jy);
+        ray = {{(R)cr.o.x, (R)cr.o.y, (R)cr.o.z}, {(R)cr.d.x, (R)cr.d.y, (R)cr.d.z}};
+      }
       Spec<R> L = s->li(ray, (R)near, (R)far, max_depth, seed, path, &per[tid]);
       for (int c = 0; c < 3; ++c) acc[c] += (double)L.c[c];
       (void)sum;

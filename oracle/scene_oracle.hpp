@@ -242,10 +242,14 @@ class SceneOracle {
     if (sizeof(R) == 4) return (R)((float)(bits >> 8) * 0x1p-24f);
     return (R)((double)bits * 0x1p-32);
   }
-  // The f64 pixel jitter (renderers/mt.rs:123-127).
+  // The pixel jitter (renderers/mt.rs:123-127): f64 for the f64 camera, FP32 for the FP32 one.
   double jitter(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t spp, uint32_t slot) const {
     if (sampler) return (double)sobol_bits(seed, pixel, sample, 0, slot) * 0x1p-32;
     return rand_u(seed, pixel * spp + sample, 0, slot);
+  }
+  float jitter_f32(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t spp, uint32_t slot) const {
+    if (sampler) return (float)(sobol_bits(seed, pixel, sample, 0, slot) >> 8) * 0x1p-24f;
+    return uni<float>(seed, pixel * spp + sample, 0, slot);
   }
   // scene BVH in the reference's layout (bvh.rs:15-31)
   struct SNode {

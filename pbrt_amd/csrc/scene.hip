@@ -897,11 +897,11 @@ __device__ __forceinline__ float urand(uint64_t seed, uint64_t path, uint32_t bo
            0x1p-24f;
   return to_unit_f32(hash4(seed, path, bounce, slot));
 }
-// The f64 pixel jitter of the camera ray (renderers/mt.rs:123-127), under the same rule.
-__device__ __forceinline__ double jitter(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t slot,
-                                        uint32_t sobol) {
-  if (sobol) return (double)sobol_draw32(seed, pixel, sample, 0, slot) * 0x1p-32;
-  return to_unit(hash4(seed, (uint64_t)pixel * spp + sample, 0, slot));
+// The pixel jitter of the camera ray (renderers/mt.rs:123-127), under the same rule, in FP32.
+__device__ __forceinline__ float jitter(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t spp, uint32_t slot,
+                                       uint32_t sobol) {
+  if (sobol) return (float)(sobol_draw32(seed, pixel, sample, 0, slot) >> 8) * 0x1p-24f;
+  return to_unit_f32(hash4(seed, (uint64_t)pixel * spp + sample, 0, slot));
 }
 enum : uint32_t {
   kSlotJx = 0, kSlotJy = 1, kSlotLight = 2, kSlotLu = 3, kSlotLv = 4, kSlotS0 = 5, kSlotS1 = 6, kSlotCoin = 7,
@@ -1191,13 +1191,16 @@ __device__ __forceinline__ float mis_power(float a, float b) {
 }
 
 // ---- wavefront kernels -------------------------------------------------------------------------------
-__device__ __forceinline__ void cam_point(const double* M, double x, double y, double z, double* r) {
-  double rx = M[0] * x + M[1] * y + M[2] * z + M[3];
-  double ry = M[4] * x + M[5] * y + M[6] * z + M[7];
-  double rz = M[8] * x + M[9] * y + M[10] * z + M[11];
-  const double w = M[12] * x + M[13] * y + M[14] * z + M[15];
-  if (w != 1.0) {
-    const double iw = 1.0 / w;
+// PinHoleCamera::get_ray (pin_hole.rs:42-50) with Matrix4::transform_point / transform_direction
+// (matrix4.rs:187-206: the w-divide only when w != 1) and the normalisations of get_ray and
+// Ray::new, in FP32: the matrices are built in f64 on the host and rounded once.
+__device__ __forceinline__ void cam_point(const float* M, float x, float y, float z, float* r) {
+  float rx = M[0] * x + M[1] * y + M[2] * z + M[3];
+  float ry = M[4] * x + M[5] * y + M[6] * z + M[7];
+  float rz = M[8] * x + M[9] * y + M[10] * z + M[11];
+  const float w = M[12] * x + M[13] * y + M[14] * z + M[15];
+  if (w != 1.0f) {
+    const float iw = 1.0f / w;
     rx = rx * iw;
     ry = ry * iw;
     rz = rz * iw;
@@ -1206,16 +1209,27 @@ __device__ __forceinline__ void cam_point(const double* M, double x, double y, d
   r[1] = ry;
   r[2] = rz;
 }
-__device__ __forceinline__ void dnorm(double* v) {
-  const double n = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-  const double inv = 1.0 / sqrt(n);
+__device__ __forceinline__ void fnorm3(float* v) {
+  const float n = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const float inv = 1.0f / sqrtf(n);
   v[0] = v[0] * inv;
   v[1] = v[1] * inv;
   v[2] = v[2] * inv;
 }
+// The camera ray through pixel coordinate (x, y) (the pixel's corner plus its jitter).
+__device__ __forceinline__ void camera_ray(const float* r2s, const float* c2w, float x, float y, float* o, float* d) {
+  float cv[3];
+  cam_point(r2s, x, y, 0.0f, cv);
+  fnorm3(cv);
+  cam_point(c2w, 0.0f, 0.0f, 0.0f, o);
+  d[0] = c2w[0] * cv[0] + c2w[1] * cv[1] + c2w[2] * cv[2];
+  d[1] = c2w[4] * cv[0] + c2w[5] * cv[1] + c2w[6] * cv[2];
+  d[2] = c2w[8] * cv[0] + c2w[9] * cv[1] + c2w[10] * cv[2];
+  fnorm3(d);
+}
 
 // One camera path per (pixel, sample of this pass): PinHoleCamera::get_ray (pin_hole.rs:42-50) in
-// f64, rounded to FP32, through the pixel jittered by two uniforms (renderers/mt.rs:123-127).
+// FP32 through the pixel jittered by two uniforms (renderers/mt.rs:123-127).
 template <bool NRC>
 __global__ void k_generate(RenderParams p, PathBuffers b) {
   const uint32_t npix = p.width * p.height;
@@ -1228,18 +1242,12 @@ __global__ void k_generate(RenderParams p, PathBuffers b) {
           b.rec[((size_t)f * p.max_depth + m) * b.train_cap + rank] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const uint32_t pixel = j % npix, sample = p.pass_first_sample + j / npix;
-    const double jx = jitter(p.seed, pixel, sample, p.spp, kSlotJx, p.sampler),
-                 jy = jitter(p.seed, pixel, sample, p.spp, kSlotJy, p.sampler);
-    double cv[3], o[3], d[3];
-    cam_point(p.r2s, (double)(pixel % p.width) + jx, (double)(pixel / p.width) + jy, 0.0, cv);
-    dnorm(cv);
-    cam_point(p.c2w, 0.0, 0.0, 0.0, o);
-    d[0] = p.c2w[0] * cv[0] + p.c2w[1] * cv[1] + p.c2w[2] * cv[2];
-    d[1] = p.c2w[4] * cv[0] + p.c2w[5] * cv[1] + p.c2w[6] * cv[2];
-    d[2] = p.c2w[8] * cv[0] + p.c2w[9] * cv[1] + p.c2w[10] * cv[2];
-    dnorm(d);
-    b.ray_o[j] = make_float4((float)o[0], (float)o[1], (float)o[2], p.near);
-    b.ray_d[j] = make_float4((float)d[0], (float)d[1], (float)d[2], p.far);
+    const float jx = jitter(p.seed, pixel, sample, p.spp, kSlotJx, p.sampler),
+                jy = jitter(p.seed, pixel, sample, p.spp, kSlotJy, p.sampler);
+    float o[3], d[3];
+    camera_ray(p.r2s, p.c2w, (float)(pixel % p.width) + jx, (float)(pixel / p.width) + jy, o, d);
+    b.ray_o[j] = make_float4(o[0], o[1], o[2], p.near);
+    b.ray_d[j] = make_float4(d[0], d[1], d[2], p.far);
     b.beta[j] = make_float4(1.f, 1.f, 1.f, 1.f);
     b.L[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     b.info[j] = make_uint4(pixel, sample, 0u, 1u);
@@ -1594,18 +1602,12 @@ __global__ void k_generate_jobs(RenderParams p, PathBuffers b) {
     const JobDesc& J = p.jobs[k];
     const uint32_t npix = J.width * J.height, local = j - J.path_base;
     const uint32_t pixel = local % npix, sample = J.pass_first + local / npix;
-    const double jx = jitter(J.seed, pixel, sample, J.spp, kSlotJx, p.sampler),
-                 jy = jitter(J.seed, pixel, sample, J.spp, kSlotJy, p.sampler);
-    double cv[3], o[3], d[3];
-    cam_point(J.r2s, (double)(pixel % J.width) + jx, (double)(pixel / J.width) + jy, 0.0, cv);
-    dnorm(cv);
-    cam_point(J.c2w, 0.0, 0.0, 0.0, o);
-    d[0] = J.c2w[0] * cv[0] + J.c2w[1] * cv[1] + J.c2w[2] * cv[2];
-    d[1] = J.c2w[4] * cv[0] + J.c2w[5] * cv[1] + J.c2w[6] * cv[2];
-    d[2] = J.c2w[8] * cv[0] + J.c2w[9] * cv[1] + J.c2w[10] * cv[2];
-    dnorm(d);
-    b.ray_o[j] = make_float4((float)o[0], (float)o[1], (float)o[2], J.near);
-    b.ray_d[j] = make_float4((float)d[0], (float)d[1], (float)d[2], J.far);
+    const float jx = jitter(J.seed, pixel, sample, J.spp, kSlotJx, p.sampler),
+                jy = jitter(J.seed, pixel, sample, J.spp, kSlotJy, p.sampler);
+    float o[3], d[3];
+    camera_ray(J.r2s, J.c2w, (float)(pixel % J.width) + jx, (float)(pixel / J.width) + jy, o, d);
+    b.ray_o[j] = make_float4(o[0], o[1], o[2], J.near);
+    b.ray_d[j] = make_float4(d[0], d[1], d[2], J.far);
     b.beta[j] = make_float4(1.f, 1.f, 1.f, 1.f);
     b.L[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     b.info[j] = make_uint4(pixel, sample, 0u, k);

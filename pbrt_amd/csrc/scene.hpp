@@ -99,7 +99,7 @@ hipError_t launch_scene_trace(const SceneLaunch& a, int any_hit);
 // One render job of a batch rendered together (pbrt_render_jobs): its camera, sampler seed and
 // sizes, the samples of the current pass, and where its paths, pixels and film start.
 struct JobDesc {
-  double r2s[16], c2w[16];
+  float r2s[16], c2w[16];    // the job's pinhole matrices, rounded to FP32
   uint64_t seed, film_base;  // film_base: first double of the job's film
   uint32_t width, height, spp, max_depth;
   uint32_t pass_first, pass_samples;  // samples [pass_first, pass_first + pass_samples) this pass
@@ -112,7 +112,7 @@ struct RenderParams {
   uint32_t width, height, spp, max_depth;
   uint32_t pass_first_sample, pass_samples;  // samples [first, first + count) of every pixel
   uint64_t seed;
-  double r2s[16], c2w[16];  // pinhole raster->camera and camera->world (f64, row-major)
+  float r2s[16], c2w[16];  // pinhole raster->camera and camera->world, built in f64, rounded to FP32 (row-major)
   float near, far;
   // radiance cache (nrc != 0): a path that is not a training path ends into the cache at its first
   // non-specular, non-emissive vertex at or past `query_vertex`; path j of the pass is a training

@@ -1081,8 +1081,8 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     p.max_depth = c->max_depth;
     p.seed = c->seed;
     for (int i = 0; i < 16; ++i) {
-      p.r2s[i] = cam.r2s.m[i / 4][i % 4];
-      p.c2w[i] = cam.c2w.m[i / 4][i % 4];
+      p.r2s[i] = (float)cam.r2s.m[i / 4][i % 4];
+      p.c2w[i] = (float)cam.c2w.m[i / 4][i % 4];
     }
     p.near = (float)c->near;
     p.far = (float)c->far;
@@ -1370,8 +1370,8 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
       JobDesc& J = base[k];
       J = JobDesc{};
       for (int i = 0; i < 16; ++i) {
-        J.r2s[i] = cam.r2s.m[i / 4][i % 4];
-        J.c2w[i] = cam.c2w.m[i / 4][i % 4];
+        J.r2s[i] = (float)cam.r2s.m[i / 4][i % 4];
+        J.c2w[i] = (float)cam.c2w.m[i / 4][i % 4];
       }
       J.seed = c.seed;
       J.film_base = film_words;
