@@ -18,7 +18,8 @@ U64 = ctypes.c_uint64
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        variant = os.environ.get("ORACLE_LIB_VARIANT", "")  # "asan": the host-sanitised build
+        path = os.path.join(_HERE, f"liboracle_{variant}.so" if variant else "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
         L = ctypes.CDLL(path)

@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpbrt_gpu.so")
+# PBRT_AMD_LIB_VARIANT=asan loads the host-sanitised build of the same sources (tests/sanitize_host.py)
+_VARIANT = os.environ.get("PBRT_AMD_LIB_VARIANT", "")
+LIB_PATH = os.path.join(HERE, f"libpbrt_gpu_{_VARIANT}.so" if _VARIANT else "libpbrt_gpu.so")
 
 P = ctypes.c_void_p
 U32 = ctypes.c_uint32

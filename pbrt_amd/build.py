@@ -44,36 +44,47 @@ def _jobs():
     return max(1, min(16, n, len(SOURCES)))
 
 
-def _compile(src, verbose):
-    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj]
+def _compile(src, verbose, obj_dir, extra):
+    obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + FLAGS + extra + ["-x", "hip", "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     return obj, r
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+# Host-side AddressSanitizer and UndefinedBehaviorSanitizer (device code unchanged): each flag
+# applies to the host compilation only.
+SANITIZE_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-Xarch_host",
+                 "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+
+
+def build(verbose=False, sanitize=False):
+    """Compiles and links the library; sanitize=True builds libpbrt_gpu_asan.so with the host code
+    under ASan + UBSan instead (loaded with PBRT_AMD_LIB_VARIANT=asan)."""
+    obj_dir = OBJ + ("_asan" if sanitize else "")
+    out = OUT.replace("libpbrt_gpu.so", "libpbrt_gpu_asan.so") if sanitize else OUT
+    os.makedirs(obj_dir, exist_ok=True)
     srcs = [os.path.join(SRC, s) for s in SOURCES]
+    extra = SANITIZE_HOST if sanitize else []
     with ThreadPoolExecutor(_jobs()) as ex:
-        results = list(ex.map(lambda s: _compile(s, verbose), srcs))
+        results = list(ex.map(lambda s: _compile(s, verbose, obj_dir, extra), srcs))
     failed = [r for _, r in results if r.returncode != 0]
     for r in results:
         if r[1].returncode != 0 or (verbose and (r[1].stdout or r[1].stderr)):
             sys.stderr.write(r[1].stdout + r[1].stderr)
     if failed:
         raise RuntimeError("hipcc failed")
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + [o for o, _ in results] + ["-o", OUT + ".tmp"]
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + extra + [o for o, _ in results] + ["-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc link failed")
-    os.replace(OUT + ".tmp", OUT)
-    return OUT
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, sanitize="--sanitize" in sys.argv))
