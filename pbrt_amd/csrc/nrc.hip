@@ -94,9 +94,9 @@ __device__ __forceinline__ float col_sum(float s) {
 constexpr int IN = kNrcIn, INP = kNrcInPad, H = kNrcHidden, OUTP = 16, OUTK = 32;
 
 // One layer of the forward pass for the 16 rows of a wave: out[r][c] = act(sum_k in[r][k] W[c][k]
-// + b[c]), `in` in LDS (row stride in_stride), W the forward weights in HBM.
+// + b[c]), `in` in LDS (row stride in_stride), W the forward weights (row stride w_stride).
 template <int K, int NOUT, bool RELU>
-__device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, const uint16_t* __restrict__ W,
+__device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, const uint16_t* W, int w_stride,
                                           const float* __restrict__ b, uint16_t* out, int out_stride, int lane,
                                           f32x4* keep) {
   const int r = lane & 15, g = lane >> 4;
@@ -106,7 +106,7 @@ __device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, con
 #pragma unroll
     for (int kk = 0; kk < K / 32; ++kk) {
       const bf16x8 a = load8(in + r * in_stride + kk * 32 + g * 8);
-      const bf16x8 bb = load8(W + (nb * 16 + r) * K + kk * 32 + g * 8);
+      const bf16x8 bb = load8(W + (nb * 16 + r) * w_stride + kk * 32 + g * 8);
       acc = mfma(a, bb, acc);
     }
     const int col = nb * 16 + r;
@@ -395,14 +395,29 @@ __global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __
 // Training: forward, relative-L2 loss gradient and backward over 64-row tiles (4 waves x 16 rows),
 // weight gradients accumulated in MFMA registers across every tile the block processes, then one
 // partial (padded layout, FP32) per block. Deterministic: fixed tile order, fixed reduction order.
+// Rows of activations, gradients and weights are padded by 16 bytes in LDS, so that the 16 rows a
+// product reads fall in distinct banks; the weights (forward [out][in], backward [in][out]) are
+// staged once per workgroup.
+constexpr int kSX = INP + 8, kSO = OUTK + 8;  // LDS row strides (BF16) of the inputs and dout
 struct TrainLds {
-  uint16_t x[64 * INP];
-  uint16_t h[4][64 * H];   // post-ReLU activations of layers 0..3
-  uint16_t dz[2][64 * H];  // BF16 gradients at a layer's pre-activation (ping-pong)
-  uint16_t dout[64 * OUTK];
+  uint16_t x[64 * kSX];
+  uint16_t h[4][64 * kSH];   // post-ReLU activations of layers 0..3
+  uint16_t dz[2][64 * kSH];  // BF16 gradients at a layer's pre-activation (ping-pong)
+  uint16_t dout[64 * kSO];
+  uint16_t f0[64 * kS0], f1[64 * kSH], f2[64 * kSH], f3[64 * kSH], f4[16 * kSH];  // forward weights
+  uint16_t t1[64 * kSH], t2[64 * kSH], t3[64 * kSH], t4[64 * kSO];                // backward weights
   float db[4][5][H];       // per-wave bias gradient partials
   float loss[4];
 };
+static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS above 160 KiB");
+
+// Copies a rows x cols BF16 matrix (cols a multiple of 8) into LDS rows of `stride` elements.
+__device__ __forceinline__ void stage_rows(uint16_t* dst, int stride, const uint16_t* __restrict__ src, int rows, int cols) {
+  for (int c = threadIdx.x; c < rows * cols / 8; c += blockDim.x) {
+    const int row = c / (cols / 8), k8 = c % (cols / 8);
+    *reinterpret_cast<uint4*>(dst + row * stride + k8 * 8) = *reinterpret_cast<const uint4*>(src + row * cols + k8 * 8);
+  }
+}
 
 // dW tile (rows m0.. of the output dim, cols n0.. of the input dim) += dz^T act over the tile's 64 rows
 __device__ __forceinline__ f32x4 dw_tile(const uint16_t* dz, int dz_stride, const uint16_t* act, int act_stride, int m0,
@@ -421,7 +436,7 @@ __device__ __forceinline__ f32x4 dw_tile(const uint16_t* dz, int dz_stride, cons
 // ReLU'(a) from the stored activation; writes BF16 dz of the previous layer and sums its FP32
 // values into the bias partial.
 template <int KOUT>
-__device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, const uint16_t* __restrict__ Wt,
+__device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, const uint16_t* Wt, int wt_stride,
                                           const uint16_t* act, uint16_t* dz_out, float* dbp, int row0, int lane) {
   const int r = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -430,7 +445,7 @@ __device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, 
 #pragma unroll
     for (int kk = 0; kk < KOUT / 32; ++kk) {
       const bf16x8 a = load8(dz_in + (row0 + r) * dz_stride + kk * 32 + g * 8);
-      const bf16x8 b = load8(Wt + (nb * 16 + r) * KOUT + kk * 32 + g * 8);
+      const bf16x8 b = load8(Wt + (nb * 16 + r) * wt_stride + kk * 32 + g * 8);
       acc = mfma(a, b, acc);
     }
     const int col = nb * 16 + r;
@@ -438,9 +453,9 @@ __device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, 
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = row0 + g * 4 + i;
-      const float v = bf16_float(act[row * H + col]) > 0.f ? acc[i] : 0.f;
+      const float v = bf16_float(act[row * kSH + col]) > 0.f ? acc[i] : 0.f;
       s += v;
-      dz_out[row * H + col] = bf16_bits(v);
+      dz_out[row * kSH + col] = bf16_bits(v);
     }
     s = col_sum(s);
     if (g == 0) dbp[col] += s;
@@ -456,7 +471,16 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
   const float inv_count = n ? 1.0f / (3.0f * (float)n) : 0.0f;  // mean over samples and channels
   const int r = lane & 15, g = lane >> 4;
   for (int i = threadIdx.x; i < 4 * 5 * H; i += 256) (&L.db[0][0][0])[i] = 0.f;
-  for (int i = threadIdx.x; i < 64 * OUTK; i += 256) L.dout[i] = 0;
+  for (int i = threadIdx.x; i < 64 * kSO; i += 256) L.dout[i] = 0;
+  stage_rows(L.f0, kS0, P.wf0, 64, INP);
+  stage_rows(L.f1, kSH, P.wf1, 64, H);
+  stage_rows(L.f2, kSH, P.wf2, 64, H);
+  stage_rows(L.f3, kSH, P.wf3, 64, H);
+  stage_rows(L.f4, kSH, P.wf4, 16, H);
+  stage_rows(L.t1, kSH, P.wt1, 64, H);
+  stage_rows(L.t2, kSH, P.wt2, 64, H);
+  stage_rows(L.t3, kSH, P.wt3, 64, H);
+  stage_rows(L.t4, kSO, P.wt4, 64, OUTK);
   if (threadIdx.x < 4) L.loss[threadIdx.x] = 0.f;
   // weight-gradient accumulators, tile t of a layer owned by wave t % 4
   f32x4 gw0[6], gw1[4], gw2[4], gw3[4], gw4;
@@ -475,21 +499,21 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
       const int row = c / (INP / 8), k8 = c % (INP / 8);
       uint4 v = make_uint4(0, 0, 0, 0);
       if (base + row < n) v = *reinterpret_cast<const uint4*>(x + (size_t)(base + row) * INP + k8 * 8);
-      *reinterpret_cast<uint4*>(L.x + row * INP + k8 * 8) = v;
+      *reinterpret_cast<uint4*>(L.x + row * kSX + k8 * 8) = v;
     }
     __syncthreads();
     const int row0 = wave * 16;
     // forward (the wave's own rows)
-    layer_fwd<INP, H, true>(L.x + row0 * INP, INP, P.wf0, P.b0, L.h[0] + row0 * H, H, lane, nullptr);
+    layer_fwd<INP, H, true>(L.x + row0 * kSX, kSX, L.f0, kS0, P.b0, L.h[0] + row0 * kSH, kSH, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(L.h[0] + row0 * H, H, P.wf1, P.b1, L.h[1] + row0 * H, H, lane, nullptr);
+    layer_fwd<H, H, true>(L.h[0] + row0 * kSH, kSH, L.f1, kSH, P.b1, L.h[1] + row0 * kSH, kSH, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(L.h[1] + row0 * H, H, P.wf2, P.b2, L.h[2] + row0 * H, H, lane, nullptr);
+    layer_fwd<H, H, true>(L.h[1] + row0 * kSH, kSH, L.f2, kSH, P.b2, L.h[2] + row0 * kSH, kSH, lane, nullptr);
     wave_sync();
-    layer_fwd<H, H, true>(L.h[2] + row0 * H, H, P.wf3, P.b3, L.h[3] + row0 * H, H, lane, nullptr);
+    layer_fwd<H, H, true>(L.h[2] + row0 * kSH, kSH, L.f3, kSH, P.b3, L.h[3] + row0 * kSH, kSH, lane, nullptr);
     wave_sync();
     f32x4 y;
-    layer_fwd<H, OUTP, false>(L.h[3] + row0 * H, H, P.wf4, P.b4, nullptr, 0, lane, &y);
+    layer_fwd<H, OUTP, false>(L.h[3] + row0 * kSH, kSH, L.f4, kSH, P.b4, nullptr, 0, lane, &y);
     // relative L2 loss and its gradient at the outputs
     {
       float s = 0.f;
@@ -507,7 +531,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
             dz = 2.0f * diff / den * inv_count;
           }
           s += dz;
-          L.dout[row * OUTK + r] = bf16_bits(dz);
+          L.dout[row * kSO + r] = bf16_bits(dz);
         }
       }
       s = col_sum(s);
@@ -515,38 +539,38 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
     }
     __syncthreads();
     // layer 4: dW4 (one 16-row tile x four 16-col tiles: wave w owns col tile w), then dz3 = dout W4 * relu'
-    gw4 = dw_tile(L.dout, OUTK, L.h[3], H, 0, wave * 16, lane, gw4);
-    back_rows<OUTK>(L.dout, OUTK, P.wt4, L.h[3], L.dz[0], L.db[wave][3], row0, lane);
+    gw4 = dw_tile(L.dout, kSO, L.h[3], kSH, 0, wave * 16, lane, gw4);
+    back_rows<OUTK>(L.dout, kSO, L.t4, kSO, L.h[3], L.dz[0], L.db[wave][3], row0, lane);
     __syncthreads();
     // layer 3
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int tile = wave + 4 * i;
-      gw3[i] = dw_tile(L.dz[0], H, L.h[2], H, (tile / 4) * 16, (tile % 4) * 16, lane, gw3[i]);
+      gw3[i] = dw_tile(L.dz[0], kSH, L.h[2], kSH, (tile / 4) * 16, (tile % 4) * 16, lane, gw3[i]);
     }
-    back_rows<H>(L.dz[0], H, P.wt3, L.h[2], L.dz[1], L.db[wave][2], row0, lane);
+    back_rows<H>(L.dz[0], kSH, L.t3, kSH, L.h[2], L.dz[1], L.db[wave][2], row0, lane);
     __syncthreads();
     // layer 2
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int tile = wave + 4 * i;
-      gw2[i] = dw_tile(L.dz[1], H, L.h[1], H, (tile / 4) * 16, (tile % 4) * 16, lane, gw2[i]);
+      gw2[i] = dw_tile(L.dz[1], kSH, L.h[1], kSH, (tile / 4) * 16, (tile % 4) * 16, lane, gw2[i]);
     }
-    back_rows<H>(L.dz[1], H, P.wt2, L.h[1], L.dz[0], L.db[wave][1], row0, lane);
+    back_rows<H>(L.dz[1], kSH, L.t2, kSH, L.h[1], L.dz[0], L.db[wave][1], row0, lane);
     __syncthreads();
     // layer 1
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int tile = wave + 4 * i;
-      gw1[i] = dw_tile(L.dz[0], H, L.h[0], H, (tile / 4) * 16, (tile % 4) * 16, lane, gw1[i]);
+      gw1[i] = dw_tile(L.dz[0], kSH, L.h[0], kSH, (tile / 4) * 16, (tile % 4) * 16, lane, gw1[i]);
     }
-    back_rows<H>(L.dz[0], H, P.wt1, L.h[0], L.dz[1], L.db[wave][0], row0, lane);
+    back_rows<H>(L.dz[0], kSH, L.t1, kSH, L.h[0], L.dz[1], L.db[wave][0], row0, lane);
     __syncthreads();
     // layer 0: dW0 = dz0^T x (4 row tiles x 6 col tiles)
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       const int tile = wave + 4 * i;
-      gw0[i] = dw_tile(L.dz[1], H, L.x, INP, (tile / 6) * 16, (tile % 6) * 16, lane, gw0[i]);
+      gw0[i] = dw_tile(L.dz[1], kSH, L.x, kSX, (tile / 6) * 16, (tile % 6) * 16, lane, gw0[i]);
     }
     __syncthreads();
   }
