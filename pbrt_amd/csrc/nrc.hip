@@ -6,9 +6,10 @@
 //
 // Data layout (padded so every matrix product is a whole number of 16 x 16 x 32 MFMA tiles):
 //   inputs   N x 96 BF16 (81 features, zero padded), row-major
-//   forward  Wf0 [64][96], Wf1..3 [64][64], Wf4 [16][64] (rows 3.. zero), [out][in] row-major
-//   inference the same, with the input columns of Wf1..4 permuted by nrc_chain_column
-//   backward Wt1..3 [64][64], Wt4 [64][32] ([in][out], out padded with zeros)
+//   forward  W0 [64][96], W1..3 [64][64], W4 [16][64] (rows 3.. zero), [out][in] row-major, the
+//            input columns of W1..4 permuted by nrc_chain_column (training and inference)
+//   backward Wt1..3 [64][64] ([in][out], the out columns permuted likewise), Wt4 [64][8] ([in][out],
+//            outputs 3.. zero)
 //   biases   b0..b3 [64], b4 [16] (FP32)
 // MFMA operand layout (16x16x32): lane l feeds A[l % 16][8 (l / 16) + j] and
 // B[8 (l / 16) + j][l % 16] for j = 0..7, and receives D[4 (l / 16) + i][l % 16] for i = 0..3.
@@ -84,43 +85,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// sum over the four lanes l, l ^ 16, l ^ 32, l ^ 48 (same MFMA output column), fixed order
-__device__ __forceinline__ float col_sum(float s) {
-  s += __shfl_xor(s, 16);
-  s += __shfl_xor(s, 32);
-  return s;
-}
-
 constexpr int IN = kNrcIn, INP = kNrcInPad, H = kNrcHidden, OUTP = 16, OUTK = 32;
-
-// One layer of the forward pass for the 16 rows of a wave: out[r][c] = act(sum_k in[r][k] W[c][k]
-// + b[c]), `in` in LDS (row stride in_stride), W the forward weights (row stride w_stride).
-template <int K, int NOUT, bool RELU>
-__device__ __forceinline__ void layer_fwd(const uint16_t* in, int in_stride, const uint16_t* W, int w_stride,
-                                          const float* __restrict__ b, uint16_t* out, int out_stride, int lane,
-                                          f32x4* keep) {
-  const int r = lane & 15, g = lane >> 4;
-#pragma unroll
-  for (int nb = 0; nb < NOUT / 16; ++nb) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kk = 0; kk < K / 32; ++kk) {
-      const bf16x8 a = load8(in + r * in_stride + kk * 32 + g * 8);
-      const bf16x8 bb = load8(W + (nb * 16 + r) * w_stride + kk * 32 + g * 8);
-      acc = mfma(a, bb, acc);
-    }
-    const int col = nb * 16 + r;
-    const float bias = b[col];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float v = acc[i] + bias;
-      if (RELU) v = v > 0.f ? v : 0.f;
-      acc[i] = v;
-      if (out) out[(g * 4 + i) * out_stride + col] = bf16_bits(v);
-    }
-    if (keep) keep[nb] = acc;
-  }
-}
 
 // Inference with the activations in registers. Each product takes the weights as its A operand and
 // 16 batch rows as its B operand, so lane (r, g) receives features 16 nb + 4 g + i of row r; with
@@ -392,73 +357,123 @@ __global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __
   }
 }
 
-// Training: forward, relative-L2 loss gradient and backward over 64-row tiles (4 waves x 16 rows),
-// weight gradients accumulated in MFMA registers across every tile the block processes, then one
-// partial (padded layout, FP32) per block. Deterministic: fixed tile order, fixed reduction order.
-// Rows of activations, gradients and weights are padded by 16 bytes in LDS, so that the 16 rows a
-// product reads fall in distinct banks; the weights (forward [out][in], backward [in][out]) are
-// staged once per workgroup.
-constexpr int kSX = INP + 8, kSO = OUTK + 8;  // LDS row strides (BF16) of the inputs and dout
+// Training: forward, relative-L2 loss gradient and backward over 64-row tiles (4 waves x 16 rows) in
+// the feature-major form of the inference chain. Every product of the forward and backward passes
+// takes a weight matrix as its A operand and the wave's 16 batch rows as its B operand, so lane
+// (r, g) holds features 16 nb + 4 g + i of row r from the first layer to the output and back: the
+// backward products take W^T of layers 1..3 with its output columns in the chained order, and the
+// ReLU masks come from the activations the lane kept. The weight gradients dW = dz^T a contract over
+// the batch rows, so the waves also store each a and dz transposed ([feature][row], rows padded by
+// 16 bytes) in LDS; after one barrier every wave accumulates its share of the dW tiles from 16-byte
+// row reads, in MFMA registers across all the tiles the workgroup processes, and writes one
+// partial per workgroup (padded layout, FP32). The bias gradients sum the lane's FP32 dz in
+// registers. Deterministic: fixed tile order, fixed reduction order.
+constexpr int kST = 64 + 8;  // row stride (BF16) of the transposed tiles: 64 batch rows + 16 bytes
+constexpr int kBw4 = 8;      // columns of the layer-4 backward weights the kernel reads
 struct TrainLds {
-  uint16_t x[64 * kSX];
-  uint16_t h[4][64 * kSH];   // post-ReLU activations of layers 0..3
-  uint16_t dz[2][64 * kSH];  // BF16 gradients at a layer's pre-activation (ping-pong)
-  uint16_t dout[64 * kSO];
-  uint16_t f0[64 * kS0], f1[64 * kSH], f2[64 * kSH], f3[64 * kSH], f4[16 * kSH];  // forward weights
-  uint16_t t1[64 * kSH], t2[64 * kSH], t3[64 * kSH], t4[64 * kSO];                // backward weights
-  float db[4][5][H];       // per-wave bias gradient partials
+  ChainLds fw;                // forward weights, chained
+  uint16_t bw[3][64 * kSH];   // backward weights of layers 1..3: [in][chained out]
+  uint16_t bw4[64 * kBw4];    // backward weights of layer 4: [in][out], outputs 0..7
+  uint16_t xT[INP * kST];     // inputs [feature][row]
+  uint16_t hT[4][64 * kST];   // activations of layers 0..3 [feature][row]
+  uint16_t dzT[4][64 * kST];  // gradients at the pre-activations of layers 0..3
+  uint16_t doT[4 * kST];      // gradients at the outputs (features 0..2; 3 is zero)
   float loss[4];
 };
 static_assert(sizeof(TrainLds) <= 160 * 1024, "training LDS above 160 KiB");
+static_assert(4 * 5 * 64 * 4 <= sizeof(TrainLds::hT), "bias partials overlay the activations");
 
-// Copies a rows x cols BF16 matrix (cols a multiple of 8) into LDS rows of `stride` elements.
-__device__ __forceinline__ void stage_rows(uint16_t* dst, int stride, const uint16_t* __restrict__ src, int rows, int cols) {
-  for (int c = threadIdx.x; c < rows * cols / 8; c += blockDim.x) {
-    const int row = c / (cols / 8), k8 = c % (cols / 8);
-    *reinterpret_cast<uint4*>(dst + row * stride + k8 * 8) = *reinterpret_cast<const uint4*>(src + row * cols + k8 * 8);
-  }
+__device__ __forceinline__ uint32_t dpp_swap_pairs(uint32_t v) {  // lanes 2k <-> 2k + 1
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float row16_sum(float v) {  // the sum over the 16 lanes of a DPP row
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x128, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x124, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x122, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x121, 0xF, 0xF, false));
+  return v;
 }
 
-// dW tile (rows m0.. of the output dim, cols n0.. of the input dim) += dz^T act over the tile's 64 rows
-__device__ __forceinline__ f32x4 dw_tile(const uint16_t* dz, int dz_stride, const uint16_t* act, int act_stride, int m0,
-                                         int n0, int lane, f32x4 acc) {
-  const int r = lane & 15, g = lane >> 4;
+// Where a lane stores its words transposed. Lanes of rows 2k and 2k + 1 swap words; the even lane
+// then writes feature f of both rows and the odd lane feature f + 1, each one 32-bit word of
+// T[feature][row]: byte selector `sel` of v_perm over (swapped word, own word), and element offset
+// `off` within a feature-row pair.
+struct PairT {
+  uint32_t sel;
+  int off;
+  __device__ explicit PairT(int row)
+      : sel((row & 1) ? 0x03020706u : 0x05040100u), off((row & 1) * kST + (row & ~1)) {}
+};
+// Stores word w (features f and f + 1 of the lane's row, low half first) transposed. Every lane of
+// a swapped pair takes part.
+__device__ __forceinline__ void store_pair_t(uint16_t* T, int f, const PairT& pt, uint32_t w) {
+  const uint32_t o = dpp_swap_pairs(w);
+  *reinterpret_cast<uint32_t*>(T + f * kST + pt.off) = __builtin_amdgcn_perm(o, w, pt.sel);
+}
+// The two B operands of a hidden layer (chained features of the lane's row) into T
+__device__ __forceinline__ void store_chain_t(uint16_t* T, const bf16x8 (&h)[2], const PairT& pt, int g) {
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk) {
-    const bf16x8 a = gather8(dz + (kk * 32 + g * 8) * dz_stride + m0 + r, dz_stride);
-    const bf16x8 b = gather8(act + (kk * 32 + g * 8) * act_stride + n0 + r, act_stride);
-    acc = mfma(a, b, acc);
+    Pack8 p;
+    p.v = h[kk];
+    store_pair_t(T, 32 * kk + 4 * g, pt, p.q.x);
+    store_pair_t(T, 32 * kk + 4 * g + 2, pt, p.q.y);
+    store_pair_t(T, 32 * kk + 16 + 4 * g, pt, p.q.z);
+    store_pair_t(T, 32 * kk + 16 + 4 * g + 2, pt, p.q.w);
   }
-  return acc;
 }
 
-// Backward through W_l for the wave's 16 rows: da = dz W (K = the layer's output dim), times
-// ReLU'(a) from the stored activation; writes BF16 dz of the previous layer and sums its FP32
-// values into the bias partial.
-template <int KOUT>
-__device__ __forceinline__ void back_rows(const uint16_t* dz_in, int dz_stride, const uint16_t* Wt, int wt_stride,
-                                          const uint16_t* act, uint16_t* dz_out, float* dbp, int row0, int lane) {
-  const int r = lane & 15, g = lane >> 4;
+// Weights into LDS with every global load issued before the first LDS store: chunk c = i + 256 k
+// (16 bytes) of a [rows][cols] BF16 matrix goes to row c / (cols / 8) of `stride` elements.
+template <int COLS>
+__device__ __forceinline__ void stage_put(uint16_t* dst, int stride, int c, uint4 v) {
+  *reinterpret_cast<uint4*>(dst + (c / (COLS / 8)) * stride + (c % (COLS / 8)) * 8) = v;
+}
+__device__ __forceinline__ uint4 stage_get(const uint16_t* __restrict__ src, int c) {
+  return *reinterpret_cast<const uint4*>(src + c * 8);
+}
+
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
+
+// dz of a hidden layer from da (accumulator layout) and the layer's kept activations: ReLU'(a) is
+// a > 0 of the BF16 activation. The FP32 values go into the bias sums, their BF16 roundings into
+// the chained B operands of the next backward product.
+__device__ __forceinline__ void relu_grad(const f32x4 (&da)[4], const bf16x8 (&h)[2], float (&db)[16],
+                                          bf16x8 (&dz)[2]) {
 #pragma unroll
-  for (int nb = 0; nb < H / 16; ++nb) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int kk = 0; kk < 2; ++kk) {
+    Pack8 a, d;
+    a.v = h[kk];
 #pragma unroll
-    for (int kk = 0; kk < KOUT / 32; ++kk) {
-      const bf16x8 a = load8(dz_in + (row0 + r) * dz_stride + kk * 32 + g * 8);
-      const bf16x8 b = load8(Wt + (nb * 16 + r) * wt_stride + kk * 32 + g * 8);
-      acc = mfma(a, b, acc);
+    for (int j = 0; j < 8; ++j) {
+      const int nb = 2 * kk + j / 4, i = j % 4;
+      const float v = (int16_t)a.u[j] > 0 ? da[nb][i] : 0.f;
+      db[nb * 4 + i] += v;
+      d.u[j] = bf16_bits(v);
     }
-    const int col = nb * 16 + r;
-    float s = 0.f;
+    dz[kk] = d.v;
+  }
+}
+
+// da of the layer below: W^T (LDS, [in][chained out], row stride kSH) times the chained dz operands
+__device__ __forceinline__ void back_chain(const uint16_t* Wt, const bf16x8 (&dz)[2], int r, int g, f32x4 (&da)[4]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = row0 + g * 4 + i;
-      const float v = bf16_float(act[row * kSH + col]) > 0.f ? acc[i] : 0.f;
-      s += v;
-      dz_out[row * kSH + col] = bf16_bits(v);
-    }
-    s = col_sum(s);
-    if (g == 0) dbp[col] += s;
+  for (int nb = 0; nb < 4; ++nb) {
+    da[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) da[nb] = mfma(load8(Wt + (nb * 16 + r) * kSH + kk * 32 + g * 8), dz[kk], da[nb]);
+  }
+}
+
+// acc[i] += dz^T a over the 64 rows for the dW tiles (rows 16 m of dzT, rows 16 i of aT), i < NT
+template <int NT>
+__device__ __forceinline__ void dw_tiles(const uint16_t* dzT, const uint16_t* aT, int m, int r, int g,
+                                         f32x4 (&acc)[NT]) {
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const bf16x8 a = load8(dzT + (16 * m + r) * kST + kk * 32 + g * 8);
+#pragma unroll
+    for (int i = 0; i < NT; ++i) acc[i] = mfma(a, load8(aT + (16 * i + r) * kST + kk * 32 + g * 8), acc[i]);
   }
 }
 
@@ -466,179 +481,238 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
                                                    const float* __restrict__ target, uint32_t n,
                                                    const uint32_t* __restrict__ n_dev, float* __restrict__ partial) {
   __shared__ __attribute__((aligned(16))) TrainLds L;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   if (n_dev) n = min(n, *n_dev);
   const float inv_count = n ? 1.0f / (3.0f * (float)n) : 0.0f;  // mean over samples and channels
-  const int r = lane & 15, g = lane >> 4;
-  for (int i = threadIdx.x; i < 4 * 5 * H; i += 256) (&L.db[0][0][0])[i] = 0.f;
-  for (int i = threadIdx.x; i < 64 * kSO; i += 256) L.dout[i] = 0;
-  stage_rows(L.f0, kS0, P.wf0, 64, INP);
-  stage_rows(L.f1, kSH, P.wf1, 64, H);
-  stage_rows(L.f2, kSH, P.wf2, 64, H);
-  stage_rows(L.f3, kSH, P.wf3, 64, H);
-  stage_rows(L.f4, kSH, P.wf4, 16, H);
-  stage_rows(L.t1, kSH, P.wt1, 64, H);
-  stage_rows(L.t2, kSH, P.wt2, 64, H);
-  stage_rows(L.t3, kSH, P.wt3, 64, H);
-  stage_rows(L.t4, kSO, P.wt4, 64, OUTK);
-  if (threadIdx.x < 4) L.loss[threadIdx.x] = 0.f;
-  // weight-gradient accumulators, tile t of a layer owned by wave t % 4
-  f32x4 gw0[6], gw1[4], gw2[4], gw3[4], gw4;
+  const NrcForward& F = P.chain;
+  {
+    const int i = threadIdx.x;
+    static_assert(64 * INP / 8 == 3 * 256 && 64 * H / 8 == 2 * 256 && 16 * H / 8 == 128 && 64 * kBw4 / 8 == 64, "");
+    uint4 v0[3], v[6][2], v4 = make_uint4(0u, 0u, 0u, 0u), u4 = make_uint4(0u, 0u, 0u, 0u);
+    const uint16_t* hid[6] = {F.w1, F.w2, F.w3, P.wt1, P.wt2, P.wt3};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v0[k] = stage_get(F.w0, i + 256 * k);
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) v[m][k] = stage_get(hid[m], i + 256 * k);
+    if (i < 128) v4 = stage_get(F.w4, i);
+    if (i < 64) u4 = stage_get(P.wt4, i);
+    uint16_t* dst[6] = {L.fw.w1, L.fw.w2, L.fw.w3, L.bw[0], L.bw[1], L.bw[2]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) stage_put<INP>(L.fw.w0, kS0, i + 256 * k, v0[k]);
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) stage_put<H>(dst[m], kSH, i + 256 * k, v[m][k]);
+    if (i < 128) stage_put<H>(L.fw.w4, kSH, i, v4);
+    if (i < 64) stage_put<kBw4>(L.bw4, kBw4, i, u4);
+  }
+  // weight-gradient tiles: layer l's tiles (16 m + .., 16 i + ..) with m = wave
+  f32x4 gw0[6], gw1[4], gw2[4], gw3[4], gw4[1];
 #pragma unroll
   for (int i = 0; i < 6; ++i) gw0[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int i = 0; i < 4; ++i) gw1[i] = gw2[i] = gw3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-  gw4 = f32x4{0.f, 0.f, 0.f, 0.f};
+  gw4[0] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db0[16], db1[16], db2[16], db3[16], db4[3] = {0.f, 0.f, 0.f};  // features 16 nb + 4 g + i
+#pragma unroll
+  for (int i = 0; i < 16; ++i) db0[i] = db1[i] = db2[i] = db3[i] = 0.f;
   float loss_acc = 0.f;
-  __syncthreads();
+  const bf16x8 zero = zero8();
+  const int tr = wave * 16 + r;  // the lane's row within the tile
+  const PairT pt(tr);
   const uint32_t tiles = (n + 63) / 64;
+  // the lane's inputs (features 32 kk + 8 g .. + 7) and, in lane group 0, targets of tile t's row
+  bf16x8 xb[INP / 32];
+  float tg[3];
+  auto fetch = [&](uint32_t t) {
+    const uint32_t row = t * 64 + tr;
+    const bool live = row < n;
+#pragma unroll
+    for (int kk = 0; kk < INP / 32; ++kk) xb[kk] = live ? load8(x + (size_t)row * INP + kk * 32 + g * 8) : zero;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tg[i] = live && g == 0 ? target[(size_t)row * 3 + i] : 0.f;
+  };
+  __syncthreads();  // the weights are in LDS
+  fetch(blockIdx.x);
   for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const uint32_t base = t * 64;
-    // stage the 64 input rows (16-byte chunks), zeros past the end
-    for (int c = threadIdx.x; c < 64 * INP / 8; c += 256) {
-      const int row = c / (INP / 8), k8 = c % (INP / 8);
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (base + row < n) v = *reinterpret_cast<const uint4*>(x + (size_t)(base + row) * INP + k8 * 8);
-      *reinterpret_cast<uint4*>(L.x + row * kSX + k8 * 8) = v;
-    }
-    __syncthreads();
-    const int row0 = wave * 16;
-    // forward (the wave's own rows)
-    layer_fwd<INP, H, true>(L.x + row0 * kSX, kSX, L.f0, kS0, P.b0, L.h[0] + row0 * kSH, kSH, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(L.h[0] + row0 * kSH, kSH, L.f1, kSH, P.b1, L.h[1] + row0 * kSH, kSH, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(L.h[1] + row0 * kSH, kSH, L.f2, kSH, P.b2, L.h[2] + row0 * kSH, kSH, lane, nullptr);
-    wave_sync();
-    layer_fwd<H, H, true>(L.h[2] + row0 * kSH, kSH, L.f3, kSH, P.b3, L.h[3] + row0 * kSH, kSH, lane, nullptr);
-    wave_sync();
-    f32x4 y;
-    layer_fwd<H, OUTP, false>(L.h[3] + row0 * kSH, kSH, L.f4, kSH, P.b4, nullptr, 0, lane, &y);
-    // relative L2 loss and its gradient at the outputs
-    {
-      float s = 0.f;
-      if (r < 3) {
+    const uint32_t row = t * 64 + tr;
+    const bool live = row < n;
+    // forward, activations kept in registers and stored transposed
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = row0 + g * 4 + i;
-          float dz = 0.f;
-          if (base + row < n) {
-            const float tv = target[(size_t)(base + row) * 3 + r];
-            const float yv = y[i];
-            const float den = yv * yv + 0.01f;
-            const float diff = yv - tv;
-            loss_acc += diff * diff / den * inv_count;
-            dz = 2.0f * diff / den * inv_count;
-          }
-          s += dz;
-          L.dout[row * kSO + r] = bf16_bits(dz);
-        }
+    for (int kk = 0; kk < INP / 32; ++kk) {
+      Pack8 p;
+      p.v = xb[kk];
+      store_pair_t(L.xT, 32 * kk + 8 * g, pt, p.q.x);
+      store_pair_t(L.xT, 32 * kk + 8 * g + 2, pt, p.q.y);
+      store_pair_t(L.xT, 32 * kk + 8 * g + 4, pt, p.q.z);
+      store_pair_t(L.xT, 32 * kk + 8 * g + 6, pt, p.q.w);
+    }
+    f32x4 acc[1][4];
+    bf16x8 h0[1][2], h1[1][2], h2[1][2], h3[1][2];
+    chain_bias<1>(F.b0, g, acc);
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+      for (int kk = 0; kk < INP / 32; ++kk)
+        acc[0][nb] = mfma(load8(L.fw.w0 + (nb * 16 + r) * kS0 + kk * 32 + g * 8), xb[kk], acc[0][nb]);
+    chain_act<1>(acc, h0);
+    store_chain_t(L.hT[0], h0[0], pt, g);
+    chain_hidden<1>(h0, L.fw.w1, F.b1, r, g, acc);
+    chain_act<1>(acc, h1);
+    store_chain_t(L.hT[1], h1[0], pt, g);
+    chain_hidden<1>(h1, L.fw.w2, F.b2, r, g, acc);
+    chain_act<1>(acc, h2);
+    store_chain_t(L.hT[2], h2[0], pt, g);
+    chain_hidden<1>(h2, L.fw.w3, F.b3, r, g, acc);
+    chain_act<1>(acc, h3);
+    store_chain_t(L.hT[3], h3[0], pt, g);
+    const float4 b4 = *reinterpret_cast<const float4*>(F.b4 + 4 * g);
+    f32x4 y = f32x4{b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) y = mfma(load8(L.fw.w4 + r * kSH + kk * 32 + g * 8), h3[0][kk], y);
+    // relative L2 loss and its gradient at the outputs (lanes g = 0 hold outputs 0..3 of row r)
+    Pack8 dout;
+    dout.q = make_uint4(0, 0, 0, 0);
+    if (g == 0 && live) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float tv = tg[i];
+        const float yv = y[i];
+        const float den = yv * yv + 0.01f;
+        const float diff = yv - tv;
+        loss_acc += diff * diff / den * inv_count;
+        const float dz = 2.0f * diff / den * inv_count;
+        db4[i] += dz;
+        dout.u[i] = bf16_bits(dz);
       }
-      s = col_sum(s);
-      if (g == 0 && r < 3) L.db[wave][4][r] += s;
     }
-    __syncthreads();
-    // layer 4: dW4 (one 16-row tile x four 16-col tiles: wave w owns col tile w), then dz3 = dout W4 * relu'
-    gw4 = dw_tile(L.dout, kSO, L.h[3], kSH, 0, wave * 16, lane, gw4);
-    back_rows<OUTK>(L.dout, kSO, L.t4, kSO, L.h[3], L.dz[0], L.db[wave][3], row0, lane);
-    __syncthreads();
-    // layer 3
+    if (g == 0) {  // both lanes of every swapped pair are in lane group 0
+      store_pair_t(L.doT, 0, pt, dout.q.x);
+      store_pair_t(L.doT, 2, pt, dout.q.y);
+    }
+    // backward through layer 4: da3 = W4^T dout (K = outputs 0..7 in lane group 0)
+    f32x4 da[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int tile = wave + 4 * i;
-      gw3[i] = dw_tile(L.dz[0], kSH, L.h[2], kSH, (tile / 4) * 16, (tile % 4) * 16, lane, gw3[i]);
+    for (int nb = 0; nb < 4; ++nb) {
+      const bf16x8 a = g == 0 ? load8(L.bw4 + (nb * 16 + r) * kBw4) : zero;
+      da[nb] = mfma(a, dout.v, f32x4{0.f, 0.f, 0.f, 0.f});
     }
-    back_rows<H>(L.dz[0], kSH, L.t3, kSH, L.h[2], L.dz[1], L.db[wave][2], row0, lane);
+    bf16x8 dz[2];
+    relu_grad(da, h3[0], db3, dz);
+    store_chain_t(L.dzT[3], dz, pt, g);
+    back_chain(L.bw[2], dz, r, g, da);
+    relu_grad(da, h2[0], db2, dz);
+    store_chain_t(L.dzT[2], dz, pt, g);
+    back_chain(L.bw[1], dz, r, g, da);
+    relu_grad(da, h1[0], db1, dz);
+    store_chain_t(L.dzT[1], dz, pt, g);
+    back_chain(L.bw[0], dz, r, g, da);
+    relu_grad(da, h0[0], db0, dz);
+    store_chain_t(L.dzT[0], dz, pt, g);
     __syncthreads();
-    // layer 2
+    if (t + gridDim.x < tiles) fetch(t + gridDim.x);  // in flight during the products below
+    // weight gradients over the tile's 64 rows: wave w owns output rows 16 w .. 16 w + 15 of
+    // layers 0..3 and input columns 16 w .. of layer 4
+    {
+      f32x4 a4 = gw4[0];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int tile = wave + 4 * i;
-      gw2[i] = dw_tile(L.dz[1], kSH, L.h[1], kSH, (tile / 4) * 16, (tile % 4) * 16, lane, gw2[i]);
+      for (int kk = 0; kk < 2; ++kk) {
+        const bf16x8 a = r < 4 ? load8(L.doT + r * kST + kk * 32 + g * 8) : zero;
+        a4 = mfma(a, load8(L.hT[3] + (16 * wave + r) * kST + kk * 32 + g * 8), a4);
+      }
+      gw4[0] = a4;
     }
-    back_rows<H>(L.dz[1], kSH, L.t2, kSH, L.h[1], L.dz[0], L.db[wave][1], row0, lane);
-    __syncthreads();
-    // layer 1
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int tile = wave + 4 * i;
-      gw1[i] = dw_tile(L.dz[0], kSH, L.h[0], kSH, (tile / 4) * 16, (tile % 4) * 16, lane, gw1[i]);
-    }
-    back_rows<H>(L.dz[0], kSH, L.t1, kSH, L.h[0], L.dz[1], L.db[wave][0], row0, lane);
-    __syncthreads();
-    // layer 0: dW0 = dz0^T x (4 row tiles x 6 col tiles)
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int tile = wave + 4 * i;
-      gw0[i] = dw_tile(L.dz[1], kSH, L.x, kSX, (tile / 6) * 16, (tile % 6) * 16, lane, gw0[i]);
-    }
+    dw_tiles<4>(L.dzT[3], L.hT[2], wave, r, g, gw3);
+    dw_tiles<4>(L.dzT[2], L.hT[1], wave, r, g, gw2);
+    dw_tiles<4>(L.dzT[1], L.hT[0], wave, r, g, gw1);
+    dw_tiles<6>(L.dzT[0], L.xT, wave, r, g, gw0);
     __syncthreads();
   }
-  // write this block's partial in the padded layout
+  // this workgroup's partial in the padded layout: dW element (16 m + 4 g + e, 16 i + r) in lane
+  // (r, g), element e of tile i
   float* out = partial + (size_t)blockIdx.x * kNrcPartialStride;
   auto put = [&](int off, int ncols, int m0, int n0, f32x4 v) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[off + (m0 + g * 4 + i) * ncols + n0 + r] = v[i];
+    for (int e = 0; e < 4; ++e) out[off + (m0 + g * 4 + e) * ncols + n0 + r] = v[e];
   };
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const int tile = wave + 4 * i;
-    put(kNrcOffW0, INP, (tile / 6) * 16, (tile % 6) * 16, gw0[i]);
-  }
+  for (int i = 0; i < 6; ++i) put(kNrcOffW0, INP, wave * 16, i * 16, gw0[i]);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int tile = wave + 4 * i;
-    put(kNrcOffW1, H, (tile / 4) * 16, (tile % 4) * 16, gw1[i]);
-    put(kNrcOffW2, H, (tile / 4) * 16, (tile % 4) * 16, gw2[i]);
-    put(kNrcOffW3, H, (tile / 4) * 16, (tile % 4) * 16, gw3[i]);
+    put(kNrcOffW1, H, wave * 16, i * 16, gw1[i]);
+    put(kNrcOffW2, H, wave * 16, i * 16, gw2[i]);
+    put(kNrcOffW3, H, wave * 16, i * 16, gw3[i]);
   }
-  put(kNrcOffW4, H, 0, wave * 16, gw4);
-  // loss: reduce over the wave's lanes in a fixed order
+  put(kNrcOffW4, H, 0, wave * 16, gw4[0]);
+  // biases: the 16 lanes of a lane group (rows) summed in DPP rows, then the four waves in order
+  float* dbw = reinterpret_cast<float*>(L.hT[0]);  // [wave][layer][feature], the loop has ended
+  auto put_db = [&](int l, const float (&db)[16]) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float s = row16_sum(db[j]);
+      if (r == 0) dbw[(wave * 5 + l) * 64 + (j / 4) * 16 + 4 * g + j % 4] = s;
+    }
+  };
+  put_db(0, db0);
+  put_db(1, db1);
+  put_db(2, db2);
+  put_db(3, db3);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float s = row16_sum(db4[i]);
+    if (lane == 0) dbw[(wave * 5 + 4) * 64 + i] = s;
+  }
+  if (lane < 64 - 3) dbw[(wave * 5 + 4) * 64 + 3 + lane] = 0.f;
   for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
   if (lane == 0) L.loss[wave] = loss_acc;
   __syncthreads();
   for (int i = threadIdx.x; i < 5 * H; i += 256) {
-    const int l = i / H, c = i % H;  // bias of layer l (0..3 hidden, 4 output), summed over the waves
-    const float s = ((L.db[0][l][c] + L.db[1][l][c]) + L.db[2][l][c]) + L.db[3][l][c];
-    out[kNrcOffB + l * H + c] = s;
+    const float s = ((dbw[0 * 5 * 64 + i] + dbw[1 * 5 * 64 + i]) + dbw[2 * 5 * 64 + i]) + dbw[3 * 5 * 64 + i];
+    out[kNrcOffB + i] = s;
   }
   if (threadIdx.x == 0) out[kNrcOffLoss] = ((L.loss[0] + L.loss[1]) + L.loss[2]) + L.loss[3];
 }
 
-// grads (padded layout) = sum over the workgroups' partials, loss likewise: eight running sums
-// over blocks b = 8k + r (independent loads in flight), added in a fixed order at the end.
-__global__ __launch_bounds__(64) void k_nrc_reduce(const float* __restrict__ partial, uint32_t blocks,
-                                                   float* __restrict__ grads) {
-  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-  if (i > (uint32_t)kNrcOffLoss) return;
+// Sum over the workgroups' partials of padded entry `pad`: wave w of the kReduceWaves takes blocks
+// w, w + kReduceWaves, ... in eight running sums and s[w] their sum; returns the sum of the waves
+// in a fixed order (valid in every lane after the barrier inside). Every thread calls it.
+constexpr int kReduceWaves = 16;
+__device__ __forceinline__ float reduce_entry(const float* __restrict__ partial, uint32_t blocks, int pad, bool valid,
+                                              float (&s)[kReduceWaves][64]) {
+  constexpr uint32_t W = kReduceWaves;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  uint32_t b = 0;
-  for (; b + 8 <= blocks; b += 8) {
+  if (valid) {
+    uint32_t b = wave;
+    for (; b + 7 * W < blocks; b += 8 * W) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) acc[r] += partial[(size_t)(b + r) * kNrcPartialStride + i];
+      for (int q = 0; q < 8; ++q) acc[q] += partial[(size_t)(b + W * q) * kNrcPartialStride + pad];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (b + W * q < blocks) acc[q] += partial[(size_t)(b + W * q) * kNrcPartialStride + pad];
   }
-  for (int r = 0; b < blocks; ++b, ++r) acc[r] += partial[(size_t)b * kNrcPartialStride + i];
-  grads[i] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  s[wave][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  __syncthreads();
+  float v = 0.f;
+#pragma unroll
+  for (int w = 0; w < kReduceWaves; ++w) v += s[w][lane];
+  return v;
 }
 
-// Adam (Kingma & Ba 2015) on the flat FP32 parameters, then the BF16 forward / backward copies.
-__global__ void k_nrc_adam(NrcDevice P, const float* __restrict__ grads, float lr, float b1, float b2, float eps,
-                           float c1, float c2) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNrcParams; i += gridDim.x * blockDim.x) {
-    const float gr = grads[nrc_padded_index(i)];
-    const float m = b1 * P.m[i] + (1.0f - b1) * gr;
-    const float v = b2 * P.v[i] + (1.0f - b2) * gr * gr;
-    P.m[i] = m;
-    P.v[i] = v;
-    const float mh = m / c1, vh = v / c2;
-    const float w = P.master[i] - lr * mh / (sqrtf(vh) + eps);
-    P.master[i] = w;
-    if (P.ema_decay > 0.f) P.ema[i] = P.ema_decay * P.ema[i] + (1.0f - P.ema_decay) * w;
-  }
+// grads (padded layout) = the sum of the workgroups' partials, loss likewise
+__global__ __launch_bounds__(64 * kReduceWaves) void k_nrc_reduce(const float* __restrict__ partial, uint32_t blocks,
+                                                                  float* __restrict__ grads) {
+  __shared__ float s[kReduceWaves][64];
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const float v = reduce_entry(partial, blocks, i, i <= kNrcOffLoss, s);
+  if (threadIdx.x < 64 && i <= kNrcOffLoss) grads[i] = v;
 }
 
 // Parameter (layer, o, k) into the forward set F; `chained`: hidden input columns in the order
-// nrc_chain_column gives (the inference sets).
+// nrc_chain_column gives.
 __device__ __forceinline__ void pack_forward(const NrcForward& F, int layer, int o, int k, bool is_bias, float w,
                                              bool chained) {
   if (is_bias) {
@@ -657,24 +731,55 @@ __device__ __forceinline__ void pack_forward(const NrcForward& F, int layer, int
   }
 }
 
-__global__ void k_nrc_pack(NrcDevice P) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNrcParams; i += gridDim.x * blockDim.x) {
-    const float w = P.master[i];
-    int layer, o, k;
-    const bool is_bias = nrc_locate(i, &layer, &o, &k);
-    pack_forward(P.train(), layer, o, k, is_bias, w, false);
-    pack_forward(P.chain, layer, o, k, is_bias, w, true);  // w0 and biases: the same words again
-    if (P.ema_decay > 0.f) pack_forward(P.avg, layer, o, k, is_bias, P.ema[i], true);
-    if (is_bias) continue;
-    const uint16_t q = bf16_bits(w);
-    switch (layer) {
-      case 0: break;
-      case 1: P.wt1[k * H + o] = q; break;
-      case 2: P.wt2[k * H + o] = q; break;
-      case 3: P.wt3[k * H + o] = q; break;
-      default: P.wt4[k * OUTK + o] = q; break;
-    }
+// Parameter i (FP32 value w, average a) into the BF16 sets: the chained forward weights (training
+// and inference), the average's (with ema_decay > 0) and the backward weights.
+__device__ __forceinline__ void pack_param(const NrcDevice& P, int i, float w, float a) {
+  int layer, o, k;
+  const bool is_bias = nrc_locate(i, &layer, &o, &k);
+  pack_forward(P.chain, layer, o, k, is_bias, w, true);
+  if (P.ema_decay > 0.f) pack_forward(P.avg, layer, o, k, is_bias, a, true);
+  if (is_bias) return;
+  const uint16_t q = bf16_bits(w);
+  switch (layer) {
+    case 0: break;
+    case 1: P.wt1[k * H + nrc_chain_column(o)] = q; break;
+    case 2: P.wt2[k * H + nrc_chain_column(o)] = q; break;
+    case 3: P.wt3[k * H + nrc_chain_column(o)] = q; break;
+    default: P.wt4[k * kBw4 + o] = q; break;
   }
+}
+
+// One optimiser step: the gradient of each parameter (the sum of the partials, also stored in
+// grads, as k_nrc_reduce would), Adam (Kingma & Ba 2015) on the FP32 parameters, the average and
+// the BF16 copies. Threads 64 b .. 64 b + 63 of the flat order, plus the loss after the last one.
+__global__ __launch_bounds__(64 * kReduceWaves) void k_nrc_step(NrcDevice P, const float* __restrict__ partial,
+                                                                uint32_t blocks, float* __restrict__ grads, float lr,
+                                                                float b1, float b2, float eps, float c1, float c2) {
+  __shared__ float s[kReduceWaves][64];
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int pad = i < kNrcParams ? nrc_padded_index(i) : kNrcOffLoss;
+  const float gr = reduce_entry(partial, blocks, pad, i <= kNrcParams, s);
+  if (threadIdx.x >= 64 || i > kNrcParams) return;
+  grads[pad] = gr;
+  if (i == kNrcParams) return;
+  const float m = b1 * P.m[i] + (1.0f - b1) * gr;
+  const float v = b2 * P.v[i] + (1.0f - b2) * gr * gr;
+  P.m[i] = m;
+  P.v[i] = v;
+  const float mh = m / c1, vh = v / c2;
+  const float w = P.master[i] - lr * mh / (sqrtf(vh) + eps);
+  P.master[i] = w;
+  float a = 0.f;
+  if (P.ema_decay > 0.f) {
+    a = P.ema_decay * P.ema[i] + (1.0f - P.ema_decay) * w;
+    P.ema[i] = a;
+  }
+  pack_param(P, i, w, a);
+}
+
+__global__ void k_nrc_pack(NrcDevice P) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNrcParams; i += gridDim.x * blockDim.x)
+    pack_param(P, i, P.master[i], P.ema[i]);
 }
 
 // 12 raw floats per sample (position, outgoing direction, normal, albedo) -> 96 BF16 features
@@ -759,16 +864,20 @@ hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* 
   hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, target, n, n_dev, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcOffLoss + 1 + 63) / 64), dim3(64), 0, (hipStream_t)s, partial, blocks,
-                     grads);
+  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcOffLoss + 1 + 63) / 64), dim3(64 * kReduceWaves), 0, (hipStream_t)s,
+                     partial, blocks, grads);
   return hipGetLastError();
 }
 
-hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,
-                           void* s) {
+hipError_t nrc_launch_train_step(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
+                                 const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, float lr,
+                                 float b1, float b2, float eps, int step, void* s) {
+  hipLaunchKernelGGL(k_nrc_train, dim3(blocks), dim3(256), 0, (hipStream_t)s, P, x, target, n, n_dev, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, (hipStream_t)s, P, grads, lr, b1, b2,
-                     eps, c1, c2);
+  hipLaunchKernelGGL(k_nrc_step, dim3((kNrcParams + 1 + 63) / 64), dim3(64 * kReduceWaves), 0, (hipStream_t)s, P,
+                     partial, blocks, grads, lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
 }
 

@@ -85,16 +85,14 @@ struct NrcForward {
 
 // Device views of the network (one allocation). BF16 values are stored as their 16-bit patterns.
 struct NrcDevice {
-  uint16_t *wf0, *wf1, *wf2, *wf3, *wf4;  // forward, [out][in]: [64][96], [64][64] x 3, [16][64]
-  uint16_t *wt1, *wt2, *wt3, *wt4;        // backward, [in][out]: [64][64] x 3, [64][32]
-  float *b0, *b1, *b2, *b3, *b4;          // [64] x 4, [16]
-  float *master, *m, *v;                  // FP32 parameters and Adam moments, flat order
-  float* ema;                             // exponential moving average of the parameters, flat order
-  NrcForward chain;                       // inference weights: layers 1..4 in the chained column order
-                                          // (w0 and the biases are those of the training set)
-  NrcForward avg;                         // inference weights of the average, chained likewise
-  float ema_decay;                        // 0: inference reads the parameters themselves
-  __host__ __device__ NrcForward train() const { return {wf0, wf1, wf2, wf3, wf4, b0, b1, b2, b3, b4}; }
+  NrcForward chain;                // the parameters in BF16: layer 0 [64][96], the hidden layers' input
+                                   // columns in the chained order (training and inference)
+  uint16_t *wt1, *wt2, *wt3, *wt4;  // backward, [in][out]: [64][64] x 3 with the output columns in the
+                                   // chained order, [64][8] (outputs 0..2 used)
+  float *master, *m, *v;           // FP32 parameters and Adam moments, flat order
+  float* ema;                      // exponential moving average of the parameters, flat order
+  NrcForward avg;                  // inference weights of the average, chained likewise
+  float ema_decay;                 // 0: inference reads the parameters themselves
   __host__ __device__ NrcForward infer() const { return ema_decay > 0.f ? avg : chain; }
 };
 
@@ -115,8 +113,11 @@ hipError_t nrc_launch_query_apply(const NrcForward& F, const float* raw, const f
                                   const uint32_t* n_dev, uint32_t cu, void* stream);
 hipError_t nrc_launch_train(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
                             const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, void* stream);
-hipError_t nrc_launch_adam(const NrcDevice& P, const float* grads, float lr, float b1, float b2, float eps, int step,
-                           void* stream);
+// The training kernel, then the reduction of its partials fused with Adam (step 1-based) and the
+// BF16 copies of the new parameters; grads receives the gradient as nrc_launch_train leaves it.
+hipError_t nrc_launch_train_step(const NrcDevice& P, const uint16_t* x, const float* target, uint32_t n,
+                                 const uint32_t* n_dev, uint32_t blocks, float* partial, float* grads, float lr,
+                                 float b1, float b2, float eps, int step, void* stream);
 // BF16 copies of the parameters (training) and, with ema_decay > 0, of their average (inference).
 hipError_t nrc_launch_pack(const NrcDevice& P, void* stream);
 hipError_t nrc_launch_encode(const float* raw, uint32_t n, const uint32_t* n_dev, const float lo[3], const float hi[3],

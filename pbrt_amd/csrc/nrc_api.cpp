@@ -20,15 +20,14 @@ constexpr size_t kAlign = 256;
 size_t up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 }  // namespace
 
+// Two forward sets (the parameters and their average), each BF16 weights [64][96], [64][64] x 3,
+// [16][64] and FP32 biases [64] x 4, [16]; the backward weights; master, m, v and ema.
 size_t nrc_device_bytes() {
   size_t b = 0;
-  b += up(64 * kNrcInPad * 2) + 3 * up(64 * 64 * 2) + up(16 * 64 * 2);  // forward weights
-  b += 3 * up(64 * 64 * 2) + up(64 * 32 * 2);                          // backward weights
-  b += 4 * up(64 * 4) + up(16 * 4);                                     // biases
-  b += 4 * up((size_t)kNrcParams * 4);                                  // master, m, v, ema
-  b += 3 * up(64 * 64 * 2) + up(16 * 64 * 2);                           // chained inference weights
-  b += up(64 * kNrcInPad * 2) + 3 * up(64 * 64 * 2) + up(16 * 64 * 2);  // forward weights of the average
-  b += 4 * up(64 * 4) + up(16 * 4);                                     // and its biases
+  b += 2 * (up(64 * kNrcInPad * 2) + 3 * up(64 * 64 * 2) + up(16 * 64 * 2));  // forward weights
+  b += 2 * (4 * up(64 * 4) + up(16 * 4));                                     // and biases
+  b += 3 * up(64 * 64 * 2) + up(64 * 8 * 2);                                  // backward weights
+  b += 4 * up((size_t)kNrcParams * 4);                                        // master, m, v, ema
   return b;
 }
 
@@ -39,40 +38,29 @@ NrcDevice nrc_carve(void* base) {
     p += up(bytes);
     return r;
   };
+  auto forward_set = [&](NrcForward& f) {
+    f.w0 = (uint16_t*)take(64 * kNrcInPad * 2);
+    f.w1 = (uint16_t*)take(64 * 64 * 2);
+    f.w2 = (uint16_t*)take(64 * 64 * 2);
+    f.w3 = (uint16_t*)take(64 * 64 * 2);
+    f.w4 = (uint16_t*)take(16 * 64 * 2);
+    f.b0 = (float*)take(64 * 4);
+    f.b1 = (float*)take(64 * 4);
+    f.b2 = (float*)take(64 * 4);
+    f.b3 = (float*)take(64 * 4);
+    f.b4 = (float*)take(16 * 4);
+  };
   NrcDevice d{};
-  d.wf0 = (uint16_t*)take(64 * kNrcInPad * 2);
-  d.wf1 = (uint16_t*)take(64 * 64 * 2);
-  d.wf2 = (uint16_t*)take(64 * 64 * 2);
-  d.wf3 = (uint16_t*)take(64 * 64 * 2);
-  d.wf4 = (uint16_t*)take(16 * 64 * 2);
+  forward_set(d.chain);
+  forward_set(d.avg);
   d.wt1 = (uint16_t*)take(64 * 64 * 2);
   d.wt2 = (uint16_t*)take(64 * 64 * 2);
   d.wt3 = (uint16_t*)take(64 * 64 * 2);
-  d.wt4 = (uint16_t*)take(64 * 32 * 2);
-  d.b0 = (float*)take(64 * 4);
-  d.b1 = (float*)take(64 * 4);
-  d.b2 = (float*)take(64 * 4);
-  d.b3 = (float*)take(64 * 4);
-  d.b4 = (float*)take(16 * 4);
+  d.wt4 = (uint16_t*)take(64 * 8 * 2);
   d.master = (float*)take((size_t)kNrcParams * 4);
   d.m = (float*)take((size_t)kNrcParams * 4);
   d.v = (float*)take((size_t)kNrcParams * 4);
   d.ema = (float*)take((size_t)kNrcParams * 4);
-  d.chain = d.train();
-  d.chain.w1 = (uint16_t*)take(64 * 64 * 2);
-  d.chain.w2 = (uint16_t*)take(64 * 64 * 2);
-  d.chain.w3 = (uint16_t*)take(64 * 64 * 2);
-  d.chain.w4 = (uint16_t*)take(16 * 64 * 2);
-  d.avg.w0 = (uint16_t*)take(64 * kNrcInPad * 2);
-  d.avg.w1 = (uint16_t*)take(64 * 64 * 2);
-  d.avg.w2 = (uint16_t*)take(64 * 64 * 2);
-  d.avg.w3 = (uint16_t*)take(64 * 64 * 2);
-  d.avg.w4 = (uint16_t*)take(16 * 64 * 2);
-  d.avg.b0 = (float*)take(64 * 4);
-  d.avg.b1 = (float*)take(64 * 4);
-  d.avg.b2 = (float*)take(64 * 4);
-  d.avg.b3 = (float*)take(64 * 4);
-  d.avg.b4 = (float*)take(16 * 4);
   d.ema_decay = 0.f;
   return d;
 }
@@ -81,12 +69,9 @@ hipError_t nrc_train_device(pbrt_nrc* n, const uint16_t* x, const float* target,
                             void* stream) {
   const uint32_t tiles = (cap + 63) / 64;
   const uint32_t blocks = tiles < n->max_blocks ? tiles : n->max_blocks;
-  hipError_t e = nrc_launch_train(n->dev, x, target, cap, n_dev, blocks ? blocks : 1, n->partial, n->grads, stream);
-  if (e != hipSuccess) return e;
   const pbrt_nrc_optim& o = n->optim;
-  e = nrc_launch_adam(n->dev, n->grads, o.lr, o.beta1, o.beta2, o.eps, (int)(n->steps + 1), stream);
-  if (e != hipSuccess) return e;
-  e = nrc_launch_pack(n->dev, stream);
+  const hipError_t e = nrc_launch_train_step(n->dev, x, target, cap, n_dev, blocks ? blocks : 1, n->partial, n->grads,
+                                             o.lr, o.beta1, o.beta2, o.eps, (int)(n->steps + 1), stream);
   if (e == hipSuccess) ++n->steps;
   return e;
 }
