@@ -122,8 +122,9 @@ inline double loss_grad(const float* params, const float* x, const float* target
   const double inv = 1.0 / ((double)n * kOut);
   for (int s = 0; s < n; ++s) {
     forward(params, x + (size_t)s * kIn, bf16, &f);
-    // dz: the gradient at a layer's pre-activation; the bias gradient takes it unrounded, the two
-    // products (weights, previous activations) take it rounded to BF16, as the device's MFMAs do.
+    // dz: the gradient at a layer's pre-activation; with `bf16` the bias gradient and the two
+    // products (weights, previous activations) take it rounded to BF16, as the device's MFMAs do
+    // (the bias gradient is dz^T 1 on the matrix cores).
     std::vector<double> dz(kOut), dzr(kOut);
     for (int c = 0; c < kOut; ++c) {
       const double y = f.act[kLayers][c], t = targets[(size_t)s * kOut + c];
@@ -137,7 +138,7 @@ inline double loss_grad(const float* params, const float* x, const float* target
       const int ni = kInDim[l], no = kOutDim[l];
       const float* W = params + w_off(l);
       for (int o = 0; o < no; ++o) {
-        (*grad)[b_off(l) + o] += dz[o];
+        (*grad)[b_off(l) + o] += dzr[o];
         for (int i = 0; i < ni; ++i) (*grad)[w_off(l) + o * ni + i] += dzr[o] * f.act[l][i];
       }
       if (l == 0) break;

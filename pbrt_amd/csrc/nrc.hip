@@ -99,6 +99,7 @@ struct ChainLds {
   uint16_t w0[64 * kS0];
   uint16_t w1[64 * kSH], w2[64 * kSH], w3[64 * kSH];
   uint16_t w4[16 * kSH];
+  float b[4][H], b4[16];  // biases of layers 0..3 and of the output layer
 };
 
 __device__ __forceinline__ void chain_stage(const NrcForward& P, ChainLds& L) {
@@ -113,6 +114,14 @@ __device__ __forceinline__ void chain_stage(const NrcForward& P, ChainLds& L) {
     *reinterpret_cast<uint4*>(L.w3 + row * kSH + k8 * 8) = *reinterpret_cast<const uint4*>(P.w3 + row * H + k8 * 8);
     if (row < 16)
       *reinterpret_cast<uint4*>(L.w4 + row * kSH + k8 * 8) = *reinterpret_cast<const uint4*>(P.w4 + row * H + k8 * 8);
+  }
+  if (threadIdx.x < 4 * H / 4) {
+    const int l = threadIdx.x / (H / 4), c = threadIdx.x % (H / 4);
+    const float* b = l == 0 ? P.b0 : l == 1 ? P.b1 : l == 2 ? P.b2 : P.b3;
+    *reinterpret_cast<float4*>(&L.b[l][4 * c]) = *reinterpret_cast<const float4*>(b + 4 * c);
+  } else if (threadIdx.x < 4 * H / 4 + 4) {
+    const int c = threadIdx.x - 4 * H / 4;
+    *reinterpret_cast<float4*>(&L.b4[4 * c]) = *reinterpret_cast<const float4*>(P.b4 + 4 * c);
   }
   __syncthreads();
 }
@@ -180,7 +189,7 @@ __device__ __forceinline__ void chain_forward(const NrcForward& P, const ChainLd
   asm volatile("v_mov_b32 %0, 0" : "=v"(z));
   f32x4 acc[RT][4];
   bf16x8 h[RT][2];
-  chain_bias<RT>(P.b0, g, acc);
+  chain_bias<RT>(L.b[0], g, acc);
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
@@ -190,13 +199,13 @@ __device__ __forceinline__ void chain_forward(const NrcForward& P, const ChainLd
       for (int t = 0; t < RT; ++t) acc[t][nb] = mfma(a, x0[t][kk], acc[t][nb]);
     }
   chain_act<RT>(acc, h);
-  chain_hidden<RT>(h, L.w1 + z, P.b1, r, g, acc);
+  chain_hidden<RT>(h, L.w1 + z, L.b[1], r, g, acc);
   chain_act<RT>(acc, h);
-  chain_hidden<RT>(h, L.w2 + z, P.b2, r, g, acc);
+  chain_hidden<RT>(h, L.w2 + z, L.b[2], r, g, acc);
   chain_act<RT>(acc, h);
-  chain_hidden<RT>(h, L.w3 + z, P.b3, r, g, acc);
+  chain_hidden<RT>(h, L.w3 + z, L.b[3], r, g, acc);
   chain_act<RT>(acc, h);
-  const float4 b4 = *reinterpret_cast<const float4*>(P.b4 + 4 * g);
+  const float4 b4 = *reinterpret_cast<const float4*>(L.b4 + 4 * g);
 #pragma unroll
   for (int t = 0; t < RT; ++t) y[t] = f32x4{b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
@@ -366,8 +375,8 @@ __global__ __launch_bounds__(256) void k_nrc_query(NrcForward P, const float* __
 // the batch rows, so the waves also store each a and dz transposed ([feature][row], rows padded by
 // 16 bytes) in LDS; after one barrier every wave accumulates its share of the dW tiles from 16-byte
 // row reads, in MFMA registers across all the tiles the workgroup processes, and writes one
-// partial per workgroup (padded layout, FP32). The bias gradients sum the lane's FP32 dz in
-// registers. Deterministic: fixed tile order, fixed reduction order.
+// partial per workgroup (padded layout, FP32). The bias gradients are dz^T 1 on the same operands
+// (BF16 dz, as for the weights). Deterministic: fixed tile order, fixed reduction order.
 constexpr int kST = 64 + 8;  // row stride (BF16) of the transposed tiles: 64 batch rows + 16 bytes
 constexpr int kBw4 = 8;      // columns of the layer-4 backward weights the kernel reads
 struct TrainLds {
@@ -435,11 +444,10 @@ __device__ __forceinline__ uint4 stage_get(const uint16_t* __restrict__ src, int
 
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
 
-// dz of a hidden layer from da (accumulator layout) and the layer's kept activations: ReLU'(a) is
-// a > 0 of the BF16 activation. The FP32 values go into the bias sums, their BF16 roundings into
-// the chained B operands of the next backward product.
-__device__ __forceinline__ void relu_grad(const f32x4 (&da)[4], const bf16x8 (&h)[2], float (&db)[16],
-                                          bf16x8 (&dz)[2]) {
+// dz of a hidden layer, rounded to BF16, from da (accumulator layout) and the layer's kept
+// activations: ReLU'(a) is a > 0 of the BF16 activation. The result is the chained B operand of the
+// next backward product.
+__device__ __forceinline__ void relu_grad(const f32x4 (&da)[4], const bf16x8 (&h)[2], bf16x8 (&dz)[2]) {
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk) {
     Pack8 a, d;
@@ -447,9 +455,7 @@ __device__ __forceinline__ void relu_grad(const f32x4 (&da)[4], const bf16x8 (&h
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int nb = 2 * kk + j / 4, i = j % 4;
-      const float v = (int16_t)a.u[j] > 0 ? da[nb][i] : 0.f;
-      db[nb * 4 + i] += v;
-      d.u[j] = bf16_bits(v);
+      d.u[j] = bf16_bits((int16_t)a.u[j] > 0 ? da[nb][i] : 0.f);
     }
     dz[kk] = d.v;
   }
@@ -465,15 +471,28 @@ __device__ __forceinline__ void back_chain(const uint16_t* Wt, const bf16x8 (&dz
   }
 }
 
-// acc[i] += dz^T a over the 64 rows for the dW tiles (rows 16 m of dzT, rows 16 i of aT), i < NT
+__device__ __forceinline__ bf16x8 ones8() {  // eight BF16 1.0
+  return __builtin_bit_cast(bf16x8, make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u));
+}
+
+// acc[i] += dz^T a over the 64 rows for the dW tiles (rows 16 m of dzT, rows 16 i of aT), i < NT,
+// and db += dz^T 1 (every column the bias gradient of features 16 m + 4 g + e). Operands are read
+// before the products.
 template <int NT>
 __device__ __forceinline__ void dw_tiles(const uint16_t* dzT, const uint16_t* aT, int m, int r, int g,
-                                         f32x4 (&acc)[NT]) {
+                                         f32x4 (&acc)[NT], f32x4& db) {
+  bf16x8 a[2], b[2][NT];
 #pragma unroll
   for (int kk = 0; kk < 2; ++kk) {
-    const bf16x8 a = load8(dzT + (16 * m + r) * kST + kk * 32 + g * 8);
+    a[kk] = load8(dzT + (16 * m + r) * kST + kk * 32 + g * 8);
 #pragma unroll
-    for (int i = 0; i < NT; ++i) acc[i] = mfma(a, load8(aT + (16 * i + r) * kST + kk * 32 + g * 8), acc[i]);
+    for (int i = 0; i < NT; ++i) b[kk][i] = load8(aT + (16 * i + r) * kST + kk * 32 + g * 8);
+  }
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+    for (int i = 0; i < NT; ++i) acc[i] = mfma(a[kk], b[kk][i], acc[i]);
+    db = mfma(a[kk], ones8(), db);
   }
 }
 
@@ -484,30 +503,6 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
   if (n_dev) n = min(n, *n_dev);
   const float inv_count = n ? 1.0f / (3.0f * (float)n) : 0.0f;  // mean over samples and channels
-  const NrcForward& F = P.chain;
-  {
-    const int i = threadIdx.x;
-    static_assert(64 * INP / 8 == 3 * 256 && 64 * H / 8 == 2 * 256 && 16 * H / 8 == 128 && 64 * kBw4 / 8 == 64, "");
-    uint4 v0[3], v[6][2], v4 = make_uint4(0u, 0u, 0u, 0u), u4 = make_uint4(0u, 0u, 0u, 0u);
-    const uint16_t* hid[6] = {F.w1, F.w2, F.w3, P.wt1, P.wt2, P.wt3};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) v0[k] = stage_get(F.w0, i + 256 * k);
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-      for (int k = 0; k < 2; ++k) v[m][k] = stage_get(hid[m], i + 256 * k);
-    if (i < 128) v4 = stage_get(F.w4, i);
-    if (i < 64) u4 = stage_get(P.wt4, i);
-    uint16_t* dst[6] = {L.fw.w1, L.fw.w2, L.fw.w3, L.bw[0], L.bw[1], L.bw[2]};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) stage_put<INP>(L.fw.w0, kS0, i + 256 * k, v0[k]);
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-#pragma unroll
-      for (int k = 0; k < 2; ++k) stage_put<H>(dst[m], kSH, i + 256 * k, v[m][k]);
-    if (i < 128) stage_put<H>(L.fw.w4, kSH, i, v4);
-    if (i < 64) stage_put<kBw4>(L.bw4, kBw4, i, u4);
-  }
   // weight-gradient tiles: layer l's tiles (16 m + .., 16 i + ..) with m = wave
   f32x4 gw0[6], gw1[4], gw2[4], gw3[4], gw4[1];
 #pragma unroll
@@ -515,11 +510,12 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
 #pragma unroll
   for (int i = 0; i < 4; ++i) gw1[i] = gw2[i] = gw3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
   gw4[0] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float db0[16], db1[16], db2[16], db3[16], db4[3] = {0.f, 0.f, 0.f};  // features 16 nb + 4 g + i
+  f32x4 gb[5];  // bias gradients of layers 0..3 (features 16 wave + 4 g + e) and 4 (wave 0, 4 g + e)
 #pragma unroll
-  for (int i = 0; i < 16; ++i) db0[i] = db1[i] = db2[i] = db3[i] = 0.f;
+  for (int l = 0; l < 5; ++l) gb[l] = f32x4{0.f, 0.f, 0.f, 0.f};
   float loss_acc = 0.f;
   const bf16x8 zero = zero8();
+  const NrcForward& F = P.chain;
   const int tr = wave * 16 + r;  // the lane's row within the tile
   const PairT pt(tr);
   const uint32_t tiles = (n + 63) / 64;
@@ -534,8 +530,38 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
 #pragma unroll
     for (int i = 0; i < 3; ++i) tg[i] = live && g == 0 ? target[(size_t)row * 3 + i] : 0.f;
   };
+  {
+    const int i = threadIdx.x;
+    static_assert(64 * INP / 8 == 3 * 256 && 64 * H / 8 == 2 * 256 && 16 * H / 8 == 128 && 64 * kBw4 / 8 == 64, "");
+    uint4 v0[3], v[6][2], v4 = make_uint4(0u, 0u, 0u, 0u), u4 = make_uint4(0u, 0u, 0u, 0u);
+    const uint16_t* hid[6] = {F.w1, F.w2, F.w3, P.wt1, P.wt2, P.wt3};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v0[k] = stage_get(F.w0, i + 256 * k);
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) v[m][k] = stage_get(hid[m], i + 256 * k);
+    if (i < 128) v4 = stage_get(F.w4, i);
+    if (i < 64) u4 = stage_get(P.wt4, i);
+    fetch(blockIdx.x);  // the first tile's rows, in flight with the weights
+    uint16_t* dst[6] = {L.fw.w1, L.fw.w2, L.fw.w3, L.bw[0], L.bw[1], L.bw[2]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) stage_put<INP>(L.fw.w0, kS0, i + 256 * k, v0[k]);
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) stage_put<H>(dst[m], kSH, i + 256 * k, v[m][k]);
+    if (i < 128) stage_put<H>(L.fw.w4, kSH, i, v4);
+    if (i < 64) stage_put<kBw4>(L.bw4, kBw4, i, u4);
+    if (i >= 128 && i < 128 + H) {  // biases: 64 float4 of layers 0..3, 4 of the output layer
+      const int c = i - 128, l = c / (H / 4), k = c % (H / 4);
+      const float* b = l == 0 ? F.b0 : l == 1 ? F.b1 : l == 2 ? F.b2 : F.b3;
+      *reinterpret_cast<float4*>(&L.fw.b[l][4 * k]) = *reinterpret_cast<const float4*>(b + 4 * k);
+    } else if (i >= 192 && i < 196) {
+      *reinterpret_cast<float4*>(&L.fw.b4[4 * (i - 192)]) = *reinterpret_cast<const float4*>(F.b4 + 4 * (i - 192));
+    }
+  }
   __syncthreads();  // the weights are in LDS
-  fetch(blockIdx.x);
   for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
     const uint32_t row = t * 64 + tr;
     const bool live = row < n;
@@ -551,7 +577,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
     }
     f32x4 acc[1][4];
     bf16x8 h0[1][2], h1[1][2], h2[1][2], h3[1][2];
-    chain_bias<1>(F.b0, g, acc);
+    chain_bias<1>(L.fw.b[0], g, acc);
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
@@ -559,16 +585,16 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
         acc[0][nb] = mfma(load8(L.fw.w0 + (nb * 16 + r) * kS0 + kk * 32 + g * 8), xb[kk], acc[0][nb]);
     chain_act<1>(acc, h0);
     store_chain_t(L.hT[0], h0[0], pt, g);
-    chain_hidden<1>(h0, L.fw.w1, F.b1, r, g, acc);
+    chain_hidden<1>(h0, L.fw.w1, L.fw.b[1], r, g, acc);
     chain_act<1>(acc, h1);
     store_chain_t(L.hT[1], h1[0], pt, g);
-    chain_hidden<1>(h1, L.fw.w2, F.b2, r, g, acc);
+    chain_hidden<1>(h1, L.fw.w2, L.fw.b[2], r, g, acc);
     chain_act<1>(acc, h2);
     store_chain_t(L.hT[2], h2[0], pt, g);
-    chain_hidden<1>(h2, L.fw.w3, F.b3, r, g, acc);
+    chain_hidden<1>(h2, L.fw.w3, L.fw.b[3], r, g, acc);
     chain_act<1>(acc, h3);
     store_chain_t(L.hT[3], h3[0], pt, g);
-    const float4 b4 = *reinterpret_cast<const float4*>(F.b4 + 4 * g);
+    const float4 b4 = *reinterpret_cast<const float4*>(L.fw.b4 + 4 * g);
     f32x4 y = f32x4{b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) y = mfma(load8(L.fw.w4 + r * kSH + kk * 32 + g * 8), h3[0][kk], y);
@@ -583,9 +609,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
         const float den = yv * yv + 0.01f;
         const float diff = yv - tv;
         loss_acc += diff * diff / den * inv_count;
-        const float dz = 2.0f * diff / den * inv_count;
-        db4[i] += dz;
-        dout.u[i] = bf16_bits(dz);
+        dout.u[i] = bf16_bits(2.0f * diff / den * inv_count);
       }
     }
     if (g == 0) {  // both lanes of every swapped pair are in lane group 0
@@ -600,34 +624,38 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
       da[nb] = mfma(a, dout.v, f32x4{0.f, 0.f, 0.f, 0.f});
     }
     bf16x8 dz[2];
-    relu_grad(da, h3[0], db3, dz);
+    relu_grad(da, h3[0], dz);
     store_chain_t(L.dzT[3], dz, pt, g);
     back_chain(L.bw[2], dz, r, g, da);
-    relu_grad(da, h2[0], db2, dz);
+    relu_grad(da, h2[0], dz);
     store_chain_t(L.dzT[2], dz, pt, g);
     back_chain(L.bw[1], dz, r, g, da);
-    relu_grad(da, h1[0], db1, dz);
+    relu_grad(da, h1[0], dz);
     store_chain_t(L.dzT[1], dz, pt, g);
     back_chain(L.bw[0], dz, r, g, da);
-    relu_grad(da, h0[0], db0, dz);
+    relu_grad(da, h0[0], dz);
     store_chain_t(L.dzT[0], dz, pt, g);
     __syncthreads();
     if (t + gridDim.x < tiles) fetch(t + gridDim.x);  // in flight during the products below
     // weight gradients over the tile's 64 rows: wave w owns output rows 16 w .. 16 w + 15 of
     // layers 0..3 and input columns 16 w .. of layer 4
     {
-      f32x4 a4 = gw4[0];
+      bf16x8 a[2], b[2];
 #pragma unroll
       for (int kk = 0; kk < 2; ++kk) {
-        const bf16x8 a = r < 4 ? load8(L.doT + r * kST + kk * 32 + g * 8) : zero;
-        a4 = mfma(a, load8(L.hT[3] + (16 * wave + r) * kST + kk * 32 + g * 8), a4);
+        a[kk] = r < 4 ? load8(L.doT + r * kST + kk * 32 + g * 8) : zero;
+        b[kk] = load8(L.hT[3] + (16 * wave + r) * kST + kk * 32 + g * 8);
       }
-      gw4[0] = a4;
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        gw4[0] = mfma(a[kk], b[kk], gw4[0]);
+        if (wave == 0) gb[4] = mfma(a[kk], ones8(), gb[4]);
+      }
     }
-    dw_tiles<4>(L.dzT[3], L.hT[2], wave, r, g, gw3);
-    dw_tiles<4>(L.dzT[2], L.hT[1], wave, r, g, gw2);
-    dw_tiles<4>(L.dzT[1], L.hT[0], wave, r, g, gw1);
-    dw_tiles<6>(L.dzT[0], L.xT, wave, r, g, gw0);
+    dw_tiles<4>(L.dzT[3], L.hT[2], wave, r, g, gw3, gb[3]);
+    dw_tiles<4>(L.dzT[2], L.hT[1], wave, r, g, gw2, gb[2]);
+    dw_tiles<4>(L.dzT[1], L.hT[0], wave, r, g, gw1, gb[1]);
+    dw_tiles<6>(L.dzT[0], L.xT, wave, r, g, gw0, gb[0]);
     __syncthreads();
   }
   // this workgroup's partial in the padded layout: dW element (16 m + 4 g + e, 16 i + r) in lane
@@ -646,38 +674,28 @@ __global__ __launch_bounds__(256) void k_nrc_train(NrcDevice P, const uint16_t* 
     put(kNrcOffW3, H, wave * 16, i * 16, gw3[i]);
   }
   put(kNrcOffW4, H, 0, wave * 16, gw4[0]);
-  // biases: the 16 lanes of a lane group (rows) summed in DPP rows, then the four waves in order
-  float* dbw = reinterpret_cast<float*>(L.hT[0]);  // [wave][layer][feature], the loop has ended
-  auto put_db = [&](int l, const float (&db)[16]) {
+  // biases: features 16 wave + 4 g + e of layers 0..3 from lanes r = 0; layer 4's 64 entries from
+  // wave 0 (features 0..15, then zeros)
+  if (r == 0) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const float s = row16_sum(db[j]);
-      if (r == 0) dbw[(wave * 5 + l) * 64 + (j / 4) * 16 + 4 * g + j % 4] = s;
-    }
-  };
-  put_db(0, db0);
-  put_db(1, db1);
-  put_db(2, db2);
-  put_db(3, db3);
+    for (int l = 0; l < 4; ++l)
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float s = row16_sum(db4[i]);
-    if (lane == 0) dbw[(wave * 5 + 4) * 64 + i] = s;
+      for (int e = 0; e < 4; ++e) out[kNrcOffB + l * H + 16 * wave + 4 * g + e] = gb[l][e];
+    if (wave == 0)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[kNrcOffB + 4 * H + 4 * g + e] = gb[4][e];
   }
-  if (lane < 64 - 3) dbw[(wave * 5 + 4) * 64 + 3 + lane] = 0.f;
+  if (wave == 1 && lane >= 16) out[kNrcOffB + 4 * H + lane] = 0.f;
   for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
   if (lane == 0) L.loss[wave] = loss_acc;
   __syncthreads();
-  for (int i = threadIdx.x; i < 5 * H; i += 256) {
-    const float s = ((dbw[0 * 5 * 64 + i] + dbw[1 * 5 * 64 + i]) + dbw[2 * 5 * 64 + i]) + dbw[3 * 5 * 64 + i];
-    out[kNrcOffB + i] = s;
-  }
   if (threadIdx.x == 0) out[kNrcOffLoss] = ((L.loss[0] + L.loss[1]) + L.loss[2]) + L.loss[3];
 }
 
 // Sum over the workgroups' partials of padded entry `pad`: wave w of the kReduceWaves takes blocks
-// w, w + kReduceWaves, ... in eight running sums and s[w] their sum; returns the sum of the waves
-// in a fixed order (valid in every lane after the barrier inside). Every thread calls it.
+// w, w + kReduceWaves, ... (sixteen loads in flight) in eight running sums and s[w] their sum;
+// returns the sum of the waves in a fixed order (valid in every lane after the barrier inside).
+// Every thread calls it.
 constexpr int kReduceWaves = 16;
 __device__ __forceinline__ float reduce_entry(const float* __restrict__ partial, uint32_t blocks, int pad, bool valid,
                                               float (&s)[kReduceWaves][64]) {
@@ -686,13 +704,16 @@ __device__ __forceinline__ float reduce_entry(const float* __restrict__ partial,
   float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (valid) {
     uint32_t b = wave;
-    for (; b + 7 * W < blocks; b += 8 * W) {
+    for (; b + 15 * W < blocks; b += 16 * W) {
+      float v[16];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) acc[q] += partial[(size_t)(b + W * q) * kNrcPartialStride + pad];
+      for (int q = 0; q < 16; ++q) v[q] = partial[(size_t)(b + W * q) * kNrcPartialStride + pad];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q % 8] += v[q];
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (b + W * q < blocks) acc[q] += partial[(size_t)(b + W * q) * kNrcPartialStride + pad];
+    for (int q = 0; q < 16; ++q)
+      if (b + W * q < blocks) acc[q % 8] += partial[(size_t)(b + W * q) * kNrcPartialStride + pad];
   }
   s[wave][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   __syncthreads();
