@@ -237,17 +237,34 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
         c, races[v] = nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, **NRC_VARIANTS[v])
         cache = cache or c
 
-    # MLP throughput: inference over one frame's worth of queries, a training step of 2^16
-    n_inf, n_tr = W * H, 65536
+    mlp = mlp_measure(pa, ctx, cache, W * H)
+    head = races[variants[0]]
+    return dict({
+        "metric": f"time to RMSE {bar}, radiance cache vs plain path tracing",
+        "workload": "room with the 1,310,720-triangle displaced icosphere and GGX materials (roughness "
+                    "U[0.05,0.5]), 1280x720, 1 spp per frame, max depth 8; RMSE of linear radiance against a "
+                    f"{ref_spp}-spp plain render (mean radiance {ref.mean():.4f})",
+        "headline_variant": variants[0],
+        "frames": frames, "frames_per_pass": frames_per_pass,
+        "reference_s": float(ref_pr["ms"][-1] / 1e3), "scene_build_s": build_s,
+        "variants": races,
+        "mlp": mlp,
+    }, **head)
+
+
+def mlp_measure(pa, ctx, cache, n_inf=1280 * 720, n_tr=65536, reps=20):
+    """The radiance cache's MLP alone: inference over `n_inf` encoded rows (one 1280x720 frame's
+    worth of queries) and one training step (forward, backward, partial reduction, Adam, BF16
+    copies) of `n_tr` rows, as HIP events over `reps` launches on the library's stream; TFLOP/s of
+    the algorithmic products (2 x 17,664 per row forward, 3x that per training row)."""
     rng = np.random.default_rng(1)
-    raw = rng.uniform(0.0, 1.0, (n_inf, 12)).astype(np.float32)
+    raw = rng.uniform(0.0, 1.0, (max(n_inf, n_tr), 12)).astype(np.float32)
     d_raw = ctx.upload(raw)
-    d_x = ctx.alloc(n_inf * pa.NRC_FEATURES * 2)
+    d_x = ctx.alloc(len(raw) * pa.NRC_FEATURES * 2)
     d_y = ctx.alloc(n_inf * 12)
-    d_t = ctx.upload(rng.uniform(0.0, 1.0, (n_inf, 3)).astype(np.float32))
-    cache.encode(d_raw, n_inf, d_x)
+    d_t = ctx.upload(rng.uniform(0.0, 1.0, (len(raw), 3)).astype(np.float32))
+    cache.encode(d_raw, len(raw), d_x)
     e0, e1 = ctx.event(), ctx.event()
-    reps = 20
     cache.infer(d_x, n_inf, d_y)
     e0.record(ctx.stream)
     for _ in range(reps):
@@ -263,21 +280,10 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096,
     flop_q = 2 * (81 * 64 + 3 * 64 * 64 + 64 * 3)
     infer_tf = n_inf * flop_q / (infer_ms * 1e-3) / 1e12
     train_tf = n_tr * 3 * flop_q / (train_ms * 1e-3) / 1e12
-    head = races[variants[0]]
-    return dict({
-        "metric": f"time to RMSE {bar}, radiance cache vs plain path tracing",
-        "workload": "room with the 1,310,720-triangle displaced icosphere and GGX materials (roughness "
-                    "U[0.05,0.5]), 1280x720, 1 spp per frame, max depth 8; RMSE of linear radiance against a "
-                    f"{ref_spp}-spp plain render (mean radiance {ref.mean():.4f})",
-        "headline_variant": variants[0],
-        "frames": frames, "frames_per_pass": frames_per_pass,
-        "reference_s": float(ref_pr["ms"][-1] / 1e3), "scene_build_s": build_s,
-        "variants": races,
-        "mlp": {"infer_ms": infer_ms, "infer_queries": n_inf, "infer_tflops": infer_tf,
-                "train_step_ms": train_ms, "train_samples": n_tr, "train_tflops": train_tf,
-                "peak_bf16_tflops": 2500.0, "infer_frac": infer_tf / 2500.0, "train_frac": train_tf / 2500.0,
-                "flop_per_query": flop_q},
-    }, **head)
+    return {"infer_ms": infer_ms, "infer_queries": n_inf, "infer_tflops": infer_tf,
+            "train_step_ms": train_ms, "train_samples": n_tr, "train_tflops": train_tf,
+            "peak_bf16_tflops": 2500.0, "infer_frac": infer_tf / 2500.0, "train_frac": train_tf / 2500.0,
+            "flop_per_query": flop_q}
 
 
 def jobs_measure(pa, ctx):
@@ -434,6 +440,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="autotune each batch's launch instead of the fixed picks")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache time-to-RMSE leg")
     ap.add_argument("--nrc-only", action="store_true", help="run only the radiance-cache leg (prints its JSON)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="run only the radiance cache's MLP throughput (inference and training step; prints its JSON)")
     ap.add_argument("--nrc-variants", default="sobol,sobol_mis",
                     help="radiance-cache races, comma-separated, the first the headline: " + ", ".join(NRC_VARIANTS))
     ap.add_argument("--no-jobs", action="store_true", help="skip the batched-jobs leg")
@@ -454,6 +462,10 @@ def main():
 
     if args.nrc_only:
         print(json.dumps(nrc_measure(pa, pa.Context(0), variants=tuple(args.nrc_variants.split(",")))))
+        return
+    if args.mlp_only:
+        ctx = pa.Context(0)
+        print(json.dumps(mlp_measure(pa, ctx, pa.RadianceCache(ctx, seed=4))))
         return
     if args.jobs_only:
         print(json.dumps(jobs_measure(pa, pa.Context(0))))
