@@ -26,7 +26,9 @@ struct RayState {
   float dx, dy, dz;
   float near, far;
   bool nx, ny, nz;    // reciprocal < 0: the slab's near plane is its max
-  bool parallel;      // a direction component is exactly zero
+  bool parallel;      // a direction component is exactly zero, or its reciprocal overflows
+  uint32_t sx, sy, sz;  // byte offset (0 or 16) of the near-plane word of each axis in a 4-wide node
+                        // (0 for a parallel ray, whose literal test takes the planes in order)
 };
 
 // The batch index of the k-th ray a kernel takes. tile_w = 0: batch order. tile_w > 0: the batch is
@@ -41,6 +43,18 @@ __device__ __forceinline__ uint32_t tiled_ray(uint32_t k, uint32_t tile_w) {
   return (band * 16 + (slot >> 3)) * tile_w + (tile << 3) + (slot & 7);
 }
 
+// The logical index of this workgroup in a grid of gridDim.x: the hardware hands workgroups to the
+// eight XCDs in turn (workgroup b to XCD b % 8), so the logical index numbers each XCD's workgroups
+// consecutively (XCD x takes logical workgroups offset_x .. offset_x + count_x - 1). With rays
+// assigned by logical index, the rays in flight on one XCD are neighbours in the batch, and its L2
+// holds the part of the tree they share. A bijection of [0, gridDim.x).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t b = blockIdx.x, g = gridDim.x;
+  const uint32_t q = g / kXcds, rem = g % kXcds, x = b % kXcds, j = b / kXcds;
+  return x * q + min(x, rem) + j;
+}
+
 __device__ __forceinline__ RayState make_ray(float4 a, float4 b) {
   RayState r;
   r.ox = a.x; r.oy = a.y; r.oz = a.z; r.near = a.w;
@@ -52,14 +66,51 @@ __device__ __forceinline__ RayState make_ray(float4 a, float4 b) {
   r.nx = r.ix < 0.0f;
   r.ny = r.iy < 0.0f;
   r.nz = r.iz < 0.0f;
-  r.parallel = (b.x == 0.0f) | (b.y == 0.0f) | (b.z == 0.0f);
+  r.parallel = (b.x == 0.0f) | (b.y == 0.0f) | (b.z == 0.0f) | (fabsf(r.ix) == INFINITY) | (fabsf(r.iy) == INFINITY) |
+               (fabsf(r.iz) == INFINITY);
+  r.sx = r.parallel ? 0u : (r.nx ? 16u : 0u);
+  r.sy = r.parallel ? 0u : (r.ny ? 16u : 0u);
+  r.sz = r.parallel ? 0u : (r.nz ? 16u : 0u);
   return r;
 }
 
-// Slab test for a ray with no zero direction component. Computing both slab distances through the
-// plane selected by the reciprocal's sign equals the reference's compute-then-swap; taking the
-// max/min over all three axes at the end equals its per-axis early exit (a rejection at an earlier
-// axis implies one at the end). fmaxf/fminf ignore a NaN operand, as Rust's f64::max/min do.
+// Slab test of a non-parallel ray against planes already ordered by the reciprocal's sign (px0 the
+// near plane of x, px1 the far one; likewise y and z), for the 4-wide step. Computing both slab
+// distances through the plane selected by the sign equals the reference's compute-then-swap
+// (aabound.rs:335-340), and taking the max / min over the three axes at the end equals its
+// per-axis early exit (a rejection at an earlier axis implies one at the end). The reference widens
+// every plane distance,
+// t0 - |t0| W and t1 + |t1| W (aabound.rs:343-344); both maps are non-decreasing in FP32 on
+// [-inf, inf), so widening the largest entry and the smallest exit once gives the same interval,
+// except where an entry distance is +inf or an exit distance -inf (an overflowed product: the
+// per-plane map gives NaN there, which max / min ignore), which takes the per-plane form.
+// fmaxf / fminf ignore a NaN operand, as Rust's f64::max / min do.
+__device__ __forceinline__ bool box_sel(const RayState& r, float px0, float px1, float py0, float py1, float pz0,
+                                        float pz1, float tmax_in, float* entry) {
+  const float W = slab_widening();
+  float t0x = (px0 - r.ox) * r.ix, t1x = (px1 - r.ox) * r.ix;
+  float t0y = (py0 - r.oy) * r.iy, t1y = (py1 - r.oy) * r.iy;
+  float t0z = (pz0 - r.oz) * r.iz, t1z = (pz1 - r.oz) * r.iz;
+  const float a = fmaxf(fmaxf(t0x, t0y), t0z), b = fminf(fminf(t1x, t1y), t1z);
+  float tmin, tmax;
+  if (__builtin_expect(a == INFINITY || b == -INFINITY, 0)) {
+    t0x = t0x - fabsf(t0x) * W; t1x = t1x + fabsf(t1x) * W;
+    t0y = t0y - fabsf(t0y) * W; t1y = t1y + fabsf(t1y) * W;
+    t0z = t0z - fabsf(t0z) * W; t1z = t1z + fabsf(t1z) * W;
+    tmin = fmaxf(fmaxf(fmaxf(t0x, r.near), t0y), t0z);
+    tmax = fminf(fminf(fminf(t1x, tmax_in), t1y), t1z);
+  } else {
+    tmin = fmaxf(a - fabsf(a) * W, r.near);
+    tmax = fminf(b + fabsf(b) * W, tmax_in);
+  }
+  *entry = tmin;
+  return !(tmax < tmin);
+}
+
+// Slab test for a non-parallel ray with the planes in box order (lo, hi per axis), in the
+// reference's per-plane form: both distances through the plane the reciprocal's sign selects (its
+// compute-then-swap), each widened, then the max / min over the axes (its per-axis early exit).
+// The binary walks use this form (on the renderer's walks it measured faster than box_sel's).
 __device__ __forceinline__ bool box_fast(const RayState& r, float lx, float hx, float ly, float hy, float lz, float hz,
                                          float tmax_in, float* entry) {
   const float W = slab_widening();
@@ -151,15 +202,25 @@ __device__ __forceinline__ bool tri_test(const RayState& r, const DevTri* __rest
 __device__ __forceinline__ int node4_step(const RayState& r, const DevNode4* __restrict__ nodes, int32_t cur, float tmax,
                                           int32_t& r0, int32_t& r1, int32_t& r2, int32_t& r3, float& k0, float& k1,
                                           float& k2, float& k3) {
-  const float4* p = reinterpret_cast<const float4*>(nodes + cur);
-  const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-  const int4 rf = reinterpret_cast<const int4*>(p)[6];
-  const int cnt = reinterpret_cast<const int4*>(p)[7].x;
+  // the near and far plane words of each axis, chosen by the ray's sign through the load offset
+  // (in box order for a parallel ray)
+  const char* nb = reinterpret_cast<const char*>(nodes + cur);
+  const float4 x0 = *reinterpret_cast<const float4*>(nb + r.sx), x1 = *reinterpret_cast<const float4*>(nb + (r.sx ^ 16u));
+  const float4 y0 = *reinterpret_cast<const float4*>(nb + 32 + r.sy);
+  const float4 y1 = *reinterpret_cast<const float4*>(nb + 32 + (r.sy ^ 16u));
+  const float4 z0 = *reinterpret_cast<const float4*>(nb + 64 + r.sz);
+  const float4 z1 = *reinterpret_cast<const float4*>(nb + 64 + (r.sz ^ 16u));
+  const int4 rf = *reinterpret_cast<const int4*>(nb + 96);
+  const int cnt = reinterpret_cast<const int4*>(nb + 112)->x;
+  auto test = [&](float a0, float a1, float b0, float b1, float c0, float c1, float* d) {
+    if (__builtin_expect(r.parallel, 0)) return box_literal(r, a0, a1, b0, b1, c0, c1, tmax, d);
+    return box_sel(r, a0, a1, b0, b1, c0, c1, tmax, d);
+  };
   float d0, d1, d2, d3;
-  const bool h0 = box_test(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmax, &d0);
-  const bool h1 = box_test(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmax, &d1);
-  const bool h2 = cnt > 2 && box_test(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmax, &d2);
-  const bool h3 = cnt > 3 && box_test(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmax, &d3);
+  const bool h0 = test(x0.x, x1.x, y0.x, y1.x, z0.x, z1.x, &d0);
+  const bool h1 = test(x0.y, x1.y, y0.y, y1.y, z0.y, z1.y, &d1);
+  const bool h2 = cnt > 2 && test(x0.z, x1.z, y0.z, y1.z, z0.z, z1.z, &d2);
+  const bool h3 = cnt > 3 && test(x0.w, x1.w, y0.w, y1.w, z0.w, z1.w, &d3);
   k0 = h0 ? d0 : INFINITY;
   k1 = h1 ? d1 : INFINITY;
   k2 = h2 ? d2 : INFINITY;

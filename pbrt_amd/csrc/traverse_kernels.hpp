@@ -63,7 +63,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
                                                    uint2* __restrict__ spill, uint32_t spill_cap,
                                                    int* __restrict__ status, uint32_t refill, uint32_t tile_w) {
   __shared__ uint2 lds[DEPTH][BLOCK];
-  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
+  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
                                                uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w,
                                                uint32_t* __restrict__ work) {
   __shared__ uint2 lds[DEPTH][BLOCK];
-  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
+  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__
                                                     uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w,
                                                     uint32_t* __restrict__ work) {
   __shared__ uint2 lds[DEPTH][BLOCK];
-  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
+  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nod
                                                 uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w,
                                                 uint32_t* __restrict__ work) {
   __shared__ uint2 lds[DEPTH][BLOCK];
-  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, blockIdx.x * BLOCK + threadIdx.x, 0};
+  Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
   if (i >= n) return;
