@@ -32,17 +32,14 @@ SEED_MESH = 2024
 SEED_RAYS = 7
 
 # Memory-side bytes of one k_closest launch on each batch of this workload, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2, WRITE x1), measured at 32
-# waves per CU; profiles/k_closest_icosphere1p3m_counters.md. Keyed by (batch, block, lds_stack,
+# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy (FETCH x2, WRITE x1), measured on the
+# bench's fixed launches; profiles/k_closest_icosphere1p3m_counters.md. Keyed by (batch, block, lds_stack,
 # refill, tree width, packet, image width of the 8 x 16 tiles or 0 for batch order): reported only
 # for the kernel variants that were measured.
 _COUNTERS = "profiles/k_closest_icosphere1p3m_counters.md"
 MEASURED_TRAFFIC = {
-    ("coherent", 64, 16, 0, 4, 0, 2048): {"bytes": 317.61e6, "source": _COUNTERS},
-    ("coherent", 256, 8, 0, 2, 0, 0): {"bytes": 255.14e6, "source": _COUNTERS},
-    ("incoherent", 128, 8, 1, 4, 0, 0): {"bytes": 2369.43e6, "source": _COUNTERS},
-    ("coherent", 128, 8, 0, 2, 0, 0): {"bytes": 253.77e6, "source": _COUNTERS},
-    ("incoherent", 256, 8, 1, 2, 0, 0): {"bytes": 3141.65e6, "source": _COUNTERS},
+    ("coherent", 64, 16, 0, 4, 0, 2048): {"bytes": 259.52e6, "source": _COUNTERS},
+    ("incoherent", 128, 8, 1, 4, 0, 0): {"bytes": 2364.90e6, "source": _COUNTERS},
 }
 LAUNCH_KEYS = ("block", "waves_per_cu", "lds_stack", "refill", "width", "packet")
 LAUNCH_DEFAULTS = dict(block=128, waves_per_cu=16, lds_stack=16, refill=0, width=2, packet=0)
