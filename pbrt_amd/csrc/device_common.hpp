@@ -27,6 +27,7 @@ struct RayState {
   float near, far;
   bool nx, ny, nz;    // reciprocal < 0: the slab's near plane is its max
   bool parallel;      // a direction component is exactly zero, or its reciprocal overflows
+  bool once;          // box tests may widen each interval once (set by guard_once; see box_once)
   uint32_t sx, sy, sz;  // byte offset (0 or 16) of the near-plane word of each axis in a 4-wide node
                         // (0 for a parallel ray, whose literal test takes the planes in order)
 };
@@ -71,38 +72,58 @@ __device__ __forceinline__ RayState make_ray(float4 a, float4 b) {
   r.sx = r.parallel ? 0u : (r.nx ? 16u : 0u);
   r.sy = r.parallel ? 0u : (r.ny ? 16u : 0u);
   r.sz = r.parallel ? 0u : (r.nz ? 16u : 0u);
+  r.once = false;
   return r;
 }
 
-// Slab test of a non-parallel ray against planes already ordered by the reciprocal's sign (px0 the
-// near plane of x, px1 the far one; likewise y and z), for the 4-wide step. Computing both slab
+// Marks a ray whose slab distances through every plane of the box [lo, hi] (a tree's root box, which
+// holds every child box of the tree) are finite: with o the origin and i the reciprocal, each
+// |(p - o) * i| is at most max(|lo - o|, |hi - o|) * |i| in FP32 (rounding is monotonic), so when
+// that bound is finite no plane product overflows and the box tests may take box_once.
+__device__ __forceinline__ void guard_once(RayState& r, const float lo[3], const float hi[3]) {
+  const float bx = fmaxf(fabsf(lo[0] - r.ox), fabsf(hi[0] - r.ox)) * fabsf(r.ix);
+  const float by = fmaxf(fabsf(lo[1] - r.oy), fabsf(hi[1] - r.oy)) * fabsf(r.iy);
+  const float bz = fmaxf(fabsf(lo[2] - r.oz), fabsf(hi[2] - r.oz)) * fabsf(r.iz);
+  r.once = !r.parallel && bx <= FLT_MAX && by <= FLT_MAX && bz <= FLT_MAX;
+}
+
+// Slab tests of a non-parallel ray against planes already ordered by the reciprocal's sign (px0 the
+// near plane of x, px1 the far one; likewise y and z). Computing both slab
 // distances through the plane selected by the sign equals the reference's compute-then-swap
 // (aabound.rs:335-340), and taking the max / min over the three axes at the end equals its
-// per-axis early exit (a rejection at an earlier axis implies one at the end). The reference widens
-// every plane distance,
-// t0 - |t0| W and t1 + |t1| W (aabound.rs:343-344); both maps are non-decreasing in FP32 on
-// [-inf, inf), so widening the largest entry and the smallest exit once gives the same interval,
-// except where an entry distance is +inf or an exit distance -inf (an overflowed product: the
-// per-plane map gives NaN there, which max / min ignore), which takes the per-plane form.
+// per-axis early exit (a rejection at an earlier axis implies one at the end).
 // fmaxf / fminf ignore a NaN operand, as Rust's f64::max / min do.
-__device__ __forceinline__ bool box_sel(const RayState& r, float px0, float px1, float py0, float py1, float pz0,
-                                        float pz1, float tmax_in, float* entry) {
+//
+// box_planes: the reference's widening of every plane distance, t0 - |t0| W and t1 + |t1| W
+// (aabound.rs:343-344).
+__device__ __forceinline__ bool box_planes(const RayState& r, float px0, float px1, float py0, float py1, float pz0,
+                                           float pz1, float tmax_in, float* entry) {
   const float W = slab_widening();
   float t0x = (px0 - r.ox) * r.ix, t1x = (px1 - r.ox) * r.ix;
   float t0y = (py0 - r.oy) * r.iy, t1y = (py1 - r.oy) * r.iy;
   float t0z = (pz0 - r.oz) * r.iz, t1z = (pz1 - r.oz) * r.iz;
+  t0x = t0x - fabsf(t0x) * W; t1x = t1x + fabsf(t1x) * W;
+  t0y = t0y - fabsf(t0y) * W; t1y = t1y + fabsf(t1y) * W;
+  t0z = t0z - fabsf(t0z) * W; t1z = t1z + fabsf(t1z) * W;
+  const float tmin = fmaxf(fmaxf(fmaxf(t0x, r.near), t0y), t0z);
+  const float tmax = fminf(fminf(fminf(t1x, tmax_in), t1y), t1z);
+  *entry = tmin;
+  return !(tmax < tmin);
+}
+
+// box_once: the same interval with the largest entry and the smallest exit widened once. Both maps
+// are non-decreasing in FP32 on [-inf, inf), so this equals box_planes whenever no plane distance is
+// infinite (an overflowed product, where the per-plane map gives NaN, which max / min ignore): the
+// rays guard_once marks.
+__device__ __forceinline__ bool box_once(const RayState& r, float px0, float px1, float py0, float py1, float pz0,
+                                         float pz1, float tmax_in, float* entry) {
+  const float W = slab_widening();
+  const float t0x = (px0 - r.ox) * r.ix, t1x = (px1 - r.ox) * r.ix;
+  const float t0y = (py0 - r.oy) * r.iy, t1y = (py1 - r.oy) * r.iy;
+  const float t0z = (pz0 - r.oz) * r.iz, t1z = (pz1 - r.oz) * r.iz;
   const float a = fmaxf(fmaxf(t0x, t0y), t0z), b = fminf(fminf(t1x, t1y), t1z);
-  float tmin, tmax;
-  if (__builtin_expect(a == INFINITY || b == -INFINITY, 0)) {
-    t0x = t0x - fabsf(t0x) * W; t1x = t1x + fabsf(t1x) * W;
-    t0y = t0y - fabsf(t0y) * W; t1y = t1y + fabsf(t1y) * W;
-    t0z = t0z - fabsf(t0z) * W; t1z = t1z + fabsf(t1z) * W;
-    tmin = fmaxf(fmaxf(fmaxf(t0x, r.near), t0y), t0z);
-    tmax = fminf(fminf(fminf(t1x, tmax_in), t1y), t1z);
-  } else {
-    tmin = fmaxf(a - fabsf(a) * W, r.near);
-    tmax = fminf(b + fabsf(b) * W, tmax_in);
-  }
+  const float tmin = fmaxf(a - fabsf(a) * W, r.near);
+  const float tmax = fminf(b + fabsf(b) * W, tmax_in);
   *entry = tmin;
   return !(tmax < tmin);
 }
@@ -110,7 +131,7 @@ __device__ __forceinline__ bool box_sel(const RayState& r, float px0, float px1,
 // Slab test for a non-parallel ray with the planes in box order (lo, hi per axis), in the
 // reference's per-plane form: both distances through the plane the reciprocal's sign selects (its
 // compute-then-swap), each widened, then the max / min over the axes (its per-axis early exit).
-// The binary walks use this form (on the renderer's walks it measured faster than box_sel's).
+// The binary walks take it for the rays guard_once does not mark.
 __device__ __forceinline__ bool box_fast(const RayState& r, float lx, float hx, float ly, float hy, float lz, float hz,
                                          float tmax_in, float* entry) {
   const float W = slab_widening();
@@ -159,8 +180,13 @@ __device__ __forceinline__ bool box_literal(const RayState& r, float lx, float h
   return true;
 }
 
+// The binary walks' box test: the widen-once form for the rays guard_once marks (planes chosen by
+// the reciprocal's sign), else the per-plane form, or the literal one for a parallel ray.
 __device__ __forceinline__ bool box_test(const RayState& r, float lx, float hx, float ly, float hy, float lz, float hz,
                                          float tmax, float* entry) {
+  if (__builtin_expect(r.once, 1))
+    return box_once(r, r.nx ? hx : lx, r.nx ? lx : hx, r.ny ? hy : ly, r.ny ? ly : hy, r.nz ? hz : lz,
+                    r.nz ? lz : hz, tmax, entry);
   if (__builtin_expect(r.parallel, 0)) return box_literal(r, lx, hx, ly, hy, lz, hz, tmax, entry);
   return box_fast(r, lx, hx, ly, hy, lz, hz, tmax, entry);
 }
@@ -212,15 +238,25 @@ __device__ __forceinline__ int node4_step(const RayState& r, const DevNode4* __r
   const float4 z1 = *reinterpret_cast<const float4*>(nb + 64 + (r.sz ^ 16u));
   const int4 rf = *reinterpret_cast<const int4*>(nb + 96);
   const int cnt = reinterpret_cast<const int4*>(nb + 112)->x;
-  auto test = [&](float a0, float a1, float b0, float b1, float c0, float c1, float* d) {
-    if (__builtin_expect(r.parallel, 0)) return box_literal(r, a0, a1, b0, b1, c0, c1, tmax, d);
-    return box_sel(r, a0, a1, b0, b1, c0, c1, tmax, d);
-  };
   float d0, d1, d2, d3;
-  const bool h0 = test(x0.x, x1.x, y0.x, y1.x, z0.x, z1.x, &d0);
-  const bool h1 = test(x0.y, x1.y, y0.y, y1.y, z0.y, z1.y, &d1);
-  const bool h2 = cnt > 2 && test(x0.z, x1.z, y0.z, y1.z, z0.z, z1.z, &d2);
-  const bool h3 = cnt > 3 && test(x0.w, x1.w, y0.w, y1.w, z0.w, z1.w, &d3);
+  bool h0, h1, h2, h3;
+  if (__builtin_expect(r.once, 1)) {
+    // all four slots tested without branches (an unused slot holds a zero box; its result is
+    // dropped by the child count)
+    h0 = box_once(r, x0.x, x1.x, y0.x, y1.x, z0.x, z1.x, tmax, &d0);
+    h1 = box_once(r, x0.y, x1.y, y0.y, y1.y, z0.y, z1.y, tmax, &d1);
+    h2 = box_once(r, x0.z, x1.z, y0.z, y1.z, z0.z, z1.z, tmax, &d2) & (cnt > 2);
+    h3 = box_once(r, x0.w, x1.w, y0.w, y1.w, z0.w, z1.w, tmax, &d3) & (cnt > 3);
+  } else {
+    auto test = [&](float a0, float a1, float b0, float b1, float c0, float c1, float* d) {
+      if (r.parallel) return box_literal(r, a0, a1, b0, b1, c0, c1, tmax, d);
+      return box_planes(r, a0, a1, b0, b1, c0, c1, tmax, d);
+    };
+    h0 = test(x0.x, x1.x, y0.x, y1.x, z0.x, z1.x, &d0);
+    h1 = test(x0.y, x1.y, y0.y, y1.y, z0.y, z1.y, &d1);
+    h2 = cnt > 2 && test(x0.z, x1.z, y0.z, y1.z, z0.z, z1.z, &d2);
+    h3 = cnt > 3 && test(x0.w, x1.w, y0.w, y1.w, z0.w, z1.w, &d3);
+  }
   k0 = h0 ? d0 : INFINITY;
   k1 = h1 ? d1 : INFINITY;
   k2 = h2 ? d2 : INFINITY;

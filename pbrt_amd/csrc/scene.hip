@@ -284,9 +284,11 @@ __device__ __forceinline__ bool push_or_flag(Stack<BLOCK, DEPTH>& st, int32_t re
 // BVHTree::query on an object's mesh (nested on the lane's stack above `base`): closest (t, prim)
 // over [near, far] with the order-independent tie rule.
 template <int BLOCK, int DEPTH>
-__device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayState& r, float far,
+__device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayState& ray, float far,
                              Stack<BLOCK, DEPTH>& st, int* status, float* t_out, float* u_out, float* v_out,
                              uint32_t* prim_out) {
+  RayState r = ray;
+  guard_once(r, ob.root_lo, ob.root_hi);
   float entry;
   if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], far,
                 &entry))
@@ -357,8 +359,10 @@ __device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayS
 }
 
 template <int BLOCK, int DEPTH>
-__device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState& r, Stack<BLOCK, DEPTH>& st,
+__device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState& ray, Stack<BLOCK, DEPTH>& st,
                          int* status) {
+  RayState r = ray;
+  guard_once(r, ob.root_lo, ob.root_hi);
   float entry;
   if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], r.far,
                 &entry))
@@ -416,9 +420,11 @@ __device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState
 // The same walks over the mesh's 4-wide tree (scenes committed with width 4): one 4-wide node per
 // step, the nearest child next and the others pushed farthest first; same hits.
 template <int BLOCK, int DEPTH>
-__device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const RayState& r, float far,
+__device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const RayState& ray, float far,
                               Stack<BLOCK, DEPTH>& st, int* status, float* t_out, float* u_out, float* v_out,
                               uint32_t* prim_out) {
+  RayState r = ray;
+  guard_once(r, ob.root_lo, ob.root_hi);
   float entry;
   if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], far,
                 &entry))
@@ -481,8 +487,10 @@ __device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const Ray
 }
 
 template <int BLOCK, int DEPTH>
-__device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayState& r, Stack<BLOCK, DEPTH>& st,
+__device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayState& ray, Stack<BLOCK, DEPTH>& st,
                           int* status) {
+  RayState r = ray;
+  guard_once(r, ob.root_lo, ob.root_hi);
   float entry;
   if (!box_test(r, ob.root_lo[0], ob.root_hi[0], ob.root_lo[1], ob.root_hi[1], ob.root_lo[2], ob.root_hi[2], r.far,
                 &entry))
@@ -602,7 +610,8 @@ __device__ bool scene_closest(const SceneView& S, V o, V d, float near, float fa
   out->orig = -1;
   if (!S.has_tlas) return false;
   const float far = fminf(far_in, FLT_MAX);
-  const RayState r = ray_from(o, d, near, far);
+  RayState r = ray_from(o, d, near, far);
+  guard_once(r, S.root.lo, S.root.hi);
   float min_t = far;
   int32_t best_orig = INT_MAX, best_prim = INT_MAX;
   bool found = false;
@@ -681,7 +690,8 @@ __device__ bool scene_any(const SceneView& S, V o, V d, float near, float far_in
                           int* status) {
   if (!S.has_tlas) return false;
   const float far = fminf(far_in, FLT_MAX);
-  const RayState r = ray_from(o, d, near, far);
+  RayState r = ray_from(o, d, near, far);
+  guard_once(r, S.root.lo, S.root.hi);
   float entry;
   st.sp = 0;
   if (!box_test(r, S.root.lo[0], S.root.hi[0], S.root.lo[1], S.root.hi[1], S.root.lo[2], S.root.hi[2], far, &entry))

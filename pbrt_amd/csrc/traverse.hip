@@ -44,6 +44,7 @@ __global__ __launch_bounds__(64) void k_closest_packet(const DevNode* __restrict
     bool act = false;
     if (valid) {
       r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
+      guard_once(r, root.lo, root.hi);
       s.cur_far = r.far;
       float entry;
       act = box_test(r, root.lo[0], root.hi[0], root.lo[1], root.hi[1], root.lo[2], root.hi[2], r.far, &entry);

@@ -76,6 +76,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
   auto start = [&](uint32_t k) {
     ri = tiled_ray(k, tile_w);
     r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
+    guard_once(r, root.lo, root.hi);
     s.min_t = FLT_MAX;
     s.t = s.u = s.v = 0.f;
     s.cur_far = r.far;  // min(far, min_t): the box tests' narrowed interval
@@ -198,6 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
   auto start = [&](uint32_t k) {
     ri = tiled_ray(k, tile_w);
     r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
+    guard_once(r, root.lo, root.hi);
     st.sp = 0;
     hit = false;
     n_node = n_tri = 0;
@@ -296,6 +298,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__
   auto start = [&](uint32_t k) {
     ri = tiled_ray(k, tile_w);
     r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
+    guard_once(r, root.lo, root.hi);
     s.min_t = FLT_MAX;
     s.t = s.u = s.v = 0.f;
     s.cur_far = r.far;
@@ -400,6 +403,7 @@ __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nod
   auto start = [&](uint32_t k) {
     ri = tiled_ray(k, tile_w);
     r = make_ray(rays[2 * (size_t)ri], rays[2 * (size_t)ri + 1]);
+    guard_once(r, root.lo, root.hi);
     st.sp = 0;
     hit = false;
     n_node = n_tri = 0;

@@ -163,6 +163,42 @@ def test_wide_tree_edge_cases_and_device_build(gpu_ctx):
     assert d.trace_host(rays).tobytes() == h.trace_host(rays).tobytes()
 
 
+@pytest.mark.parametrize("refill", [0, 1])
+def test_wide_tree_overflowing_slab_products(gpu_ctx, refill):
+    """Rays that run almost parallel to a coordinate plane across a mesh wider than 3.4 units: a
+    direction component of 1e-38 makes the reciprocal 1e38, so the slab distance through a plane
+    more than 3.4 units away overflows to infinity. Such rays take the 4-wide step's per-plane form
+    (the widen-once form is exact only without infinite plane distances), exactly-zero components
+    the literal form, the rest the widen-once form; all must equal Oracle B (and so the binary
+    walk) bit for bit, in a mix inside one wave."""
+    v, f = pbrt_amd.icosphere(5, radius=10.0, noise="fbm", amp=0.1, octaves=3, seed=5)
+    rng = np.random.default_rng(11)
+    n = 40000
+    o = rng.uniform(-12.0, 12.0, (n, 3)).astype(np.float32)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    tiny = rng.choice(np.array([1e-38, -1e-38, 3e-37, -2e-36, 1e-30, -1e-33, 0.0, 5e-39, 0.3], np.float32), n)
+    axis = rng.integers(0, 3, n)
+    d = np.empty((n, 3), np.float32)
+    for k in range(3):
+        a, b = (k + 1) % 3, (k + 2) % 3
+        m = axis == k
+        d[m, k] = tiny[m]
+        d[m, a] = np.cos(phi[m])
+        d[m, b] = np.sin(phi[m])
+    rays = np.concatenate([o, np.full((n, 1), 1e-4, np.float32), d, np.full((n, 1), 1000.0, np.float32)], 1)
+    _, hits, ref = check_closest(gpu_ctx, v, f, rays, pbrt_amd.DEVICE_BVH, stats=False,
+                                 launch=dict(width=4, refill=refill))
+    # the mix really holds every class: hits among the overflowing rays, the parallel ones, the rest
+    with np.errstate(divide="ignore", over="ignore"):
+        inv = np.float32(1.0) / np.abs(tiny).astype(np.float32)
+    overflow = (tiny != 0) & (np.abs(tiny) < 1e-37) & np.isfinite(inv)
+    hit = ref["prim"] >= 0
+    assert hit[overflow].sum() > 100 and hit[tiny == 0].sum() > 100 and hit[np.abs(tiny) > 1e-35].sum() > 100
+    occ = pbrt_amd.TriangleMesh(gpu_ctx, v, f, **pbrt_amd.DEVICE_BVH)
+    occ.set_launch(width=4, refill=refill)
+    np.testing.assert_array_equal(occ.trace_host(rays, mode="any").astype(bool), hit)
+
+
 @pytest.mark.parametrize("name", list(PUBLISHED))
 def test_packet_traversal_published_meshes(gpu_ctx, name):
     """Wave packets (64 consecutive rays on one wave-wide stack) give the per-ray hits, on the
