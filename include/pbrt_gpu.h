@@ -258,15 +258,15 @@ int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double r
 #define PBRT_BUILD_HOST 0
 #define PBRT_BUILD_DEVICE 1
 int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
-/* Tree the scene walks inside its meshes: 2 (default) the binary trees, 4 trees collapsed to 4-wide
- * nodes at commit (as pbrt_launch.width 4 for a mesh). Same hits and films either way. */
+/* Tree the scene walks inside its meshes: 4 (default) trees collapsed to 4-wide nodes at commit (as
+ * pbrt_launch.width 4 for a mesh), 2 the binary trees. Same hits and films either way. */
 int pbrt_scene_set_mesh_width(pbrt_scene* scene, uint32_t width);
 /* Launch settings of the scene walks (pbrt_scene_trace_*, the renderer's extension and shadow rays):
  * LDS stack entries per lane (8, 16, 24; deeper entries spill to HBM), workgroups of 128 threads
  * per CU in the persistent grid (1-8; 0 = as many as fit), mesh tree width (2, 4; 4 collapses the
  * trees on first use), min_waves (0: the renderer's extension walk at the compiler's register
  * count; 5: its variant compiled for five waves per SIMD). Same hits and films for every setting.
- * Defaults 16, 0, 2, 5. */
+ * Defaults 16, 0, 4, 5 (width 0 also means 4). */
 typedef struct pbrt_scene_launch {
   uint32_t lds_stack, blocks_per_cu, width, min_waves;
 } pbrt_scene_launch;

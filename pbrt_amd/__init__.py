@@ -349,9 +349,10 @@ class Scene:
     `pbrt_amd.scene.SceneDesc`; `intersect` / `intersect_p` are batched (scene.rs:118-132) and
     `render` is renderers::mt::render (src/renderers/mt.rs:23) with the path integrator."""
 
-    def __init__(self, ctx, desc, bins=12, max_leaf=4, t_trav=1.0, device_build=False, mesh_width=2):
+    def __init__(self, ctx, desc, bins=12, max_leaf=4, t_trav=1.0, device_build=False, mesh_width=None):
         """device_build: the meshes' BVHs built on the device at commit (pbrt_scene_set_mesh_build);
-        mesh_width 4: the scene walks 4-wide trees inside its meshes (pbrt_scene_set_mesh_width)."""
+        mesh_width: the tree the scene walks inside its meshes, 4 (the library's default: 4-wide trees
+        collapsed at commit) or 2 (the binary trees); pbrt_scene_set_mesh_width."""
         self.ctx = ctx
         self.desc = desc
         h = ctypes.c_void_p()
@@ -359,7 +360,7 @@ class Scene:
         self.h = h
         if device_build:
             call("pbrt_scene_set_mesh_build", h, 1)
-        if mesh_width != 2:
+        if mesh_width is not None:
             call("pbrt_scene_set_mesh_width", h, mesh_width)
         out = ctypes.c_uint32()
         for v, f in desc.meshes:

@@ -243,12 +243,13 @@ double child_area(const Child4& c) {
 
 int32_t collapse_node(const std::vector<DevNode>& n2, int32_t ref2, std::vector<DevNode4>* n4) {
   if (ref2 < 0) return ref2;
-  std::vector<Child4> ch(2);
-  children_of(n2[ref2], ch.data());
-  while (ch.size() < 4) {
+  Child4 ch[4];
+  int nc = 2;
+  children_of(n2[ref2], ch);
+  while (nc < 4) {
     int best = -1;
     double best_area = -1.0;
-    for (int k = 0; k < (int)ch.size(); ++k)
+    for (int k = 0; k < nc; ++k)
       if (ch[k].ref >= 0 && child_area(ch[k]) > best_area) {
         best = k;
         best_area = child_area(ch[k]);
@@ -256,16 +257,18 @@ int32_t collapse_node(const std::vector<DevNode>& n2, int32_t ref2, std::vector<
     if (best < 0) break;
     Child4 sub[2];
     children_of(n2[ch[best].ref], sub);
+    for (int k = nc; k > best + 1; --k) ch[k] = ch[k - 1];  // the opened child's two take its place
     ch[best] = sub[0];
-    ch.insert(ch.begin() + best + 1, sub[1]);
+    ch[best + 1] = sub[1];
+    ++nc;
   }
   const int32_t me = (int32_t)n4->size();
   n4->push_back(DevNode4{});
   int32_t refs[4] = {0, 0, 0, 0};
-  for (size_t k = 0; k < ch.size(); ++k) refs[k] = collapse_node(n2, ch[k].ref, n4);
+  for (int k = 0; k < nc; ++k) refs[k] = collapse_node(n2, ch[k].ref, n4);
   DevNode4& d = (*n4)[me];
   for (int k = 0; k < 4; ++k) {
-    const bool on = k < (int)ch.size();
+    const bool on = k < nc;
     d.lox[k] = on ? ch[k].lo[0] : 0.f;
     d.loy[k] = on ? ch[k].lo[1] : 0.f;
     d.loz[k] = on ? ch[k].lo[2] : 0.f;
@@ -275,7 +278,7 @@ int32_t collapse_node(const std::vector<DevNode>& n2, int32_t ref2, std::vector<
     d.ref[k] = refs[k];
     d.count[k] = 0;
   }
-  d.count[0] = (int32_t)ch.size();
+  d.count[0] = nc;
   return me;
 }
 

@@ -354,7 +354,7 @@ def test_wide_mesh_walk_same_hits_and_films(gpu_ctx, name, device_build):
     else:
         sc, desc = stage(name)
         eye, look, fov = sc["camera"]["pos"], sc["camera"]["look"], math.pi / 2
-    a = pbrt_amd.Scene(gpu_ctx, desc)
+    a = pbrt_amd.Scene(gpu_ctx, desc, mesh_width=2)
     b = pbrt_amd.Scene(gpu_ctx, desc, mesh_width=4, device_build=device_build)
     rays = po.pinhole_rays(160, 120, fov, eye, look).astype(np.float32)
     rng = np.random.default_rng(4)
