@@ -38,8 +38,8 @@ SEED_RAYS = 7
 # for the kernel variants that were measured.
 _COUNTERS = "profiles/k_closest_icosphere1p3m_counters.md"
 MEASURED_TRAFFIC = {
-    ("coherent", 64, 16, 0, 4, 0, 2048): {"bytes": 259.52e6, "source": _COUNTERS},
-    ("incoherent", 128, 8, 1, 4, 0, 0): {"bytes": 2364.90e6, "source": _COUNTERS},
+    ("coherent", 64, 16, 0, 4, 0, 2048): {"bytes": 259.54e6, "source": _COUNTERS},
+    ("incoherent", 128, 8, 1, 4, 0, 0): {"bytes": 2360.16e6, "source": _COUNTERS},
 }
 LAUNCH_KEYS = ("block", "waves_per_cu", "lds_stack", "refill", "width", "packet")
 LAUNCH_DEFAULTS = dict(block=128, waves_per_cu=16, lds_stack=16, refill=0, width=2, packet=0)
@@ -287,7 +287,9 @@ def jobs_measure(pa, ctx):
     """The batched-jobs workload: 256 mesh instances (1k-200k triangles each, 9.1M in all) and 32
     CSG objects; eight render jobs (256^2 to 1024^2, 16 or 64 spp, depth 8) rendered as one batch
     sharing the wavefront's queues with per-bounce graphs, against the same jobs rendered one after
-    the other. GPU time from HIP events on the render stream around each call."""
+    the other, both with the scene walk's launch settings the autotuner picks on short renders (4
+    spp) of the view of the job with the most samples (the library's default settings timed beside
+    them). GPU time from HIP events on the render stream around each call."""
     from pbrt_amd import scenes
     t0 = time.time()
     desc = scenes.batch_scene()
@@ -311,6 +313,12 @@ def jobs_measure(pa, ctx):
         return out, e0.elapsed_ms(e1)
 
     scene.render_jobs(jobs[:1], use_graphs=True)  # warm-up
+    (_, st_d), default_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=True))
+    jt = max(jobs, key=lambda j: j["width"] * j["height"] * j["spp"])
+    scene_launch = scene.autotune_render(jt["width"], jt["height"], 4, jt["max_depth"], jt["eye"], jt["look"],
+                                         fov=jt["fov"], reps=2)
+    print(f"[bench] jobs scene launch tuned on the largest job's view: {scene_launch}; batch at the default launch "
+          f"{default_ms:.1f} ms", file=sys.stderr, flush=True)
     (films, st), batch_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=True))
     (_, st_eager), eager_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=False))
     seq_ms, seq_rays = 0.0, 0
@@ -320,13 +328,7 @@ def jobs_measure(pa, ctx):
         seq_ms += ms
         seq_rays += sj["rays"]
     paths = st["paths"]
-    # the batch again with the scene walk's settings tuned on renders of the first job's view
     j0 = jobs[0]
-    scene_launch = scene.autotune_render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"],
-                                         fov=j0["fov"], reps=2)
-    (_, st_t), tuned_ms = timed(lambda: scene.render_jobs(jobs, use_graphs=True))
-    print(f"[bench] jobs scene launch tuned on job 0: {scene_launch}; batch {tuned_ms:.1f} ms", file=sys.stderr,
-          flush=True)
     f_host, _ = scene.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
     f_dev, _ = scene_dev.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
     builds = {"scene_s": build_s, "mesh_bvh_s": scene.build_seconds(), "device_scene_s": dev_build_s,
@@ -341,7 +343,7 @@ def jobs_measure(pa, ctx):
                     f"{sum(1 for o in desc.objects if o[0] == 4)} CSG objects; 8 jobs "
                     + ", ".join(f"{j['width']}x{j['height']}x{j['spp']}" for j in jobs) + ", max depth 8",
         "scene_gen_s": gen_s, "scene_build_s": build_s, "scene_builds": builds,
-        "tuned_scene_launch": scene_launch, "batch_tuned_launch_ms": tuned_ms,
+        "tuned_scene_launch": scene_launch, "batch_default_launch_ms": default_ms,
         "batch_ms": batch_ms, "batch_mrays_s": st["rays"] / batch_ms / 1e3, "batch_samples_s": paths / batch_ms * 1e3,
         "batch_no_graphs_ms": eager_ms, "sequential_ms": seq_ms, "sequential_mrays_s": seq_rays / seq_ms / 1e3,
         "speedup_vs_sequential": seq_ms / batch_ms, "passes": st["passes"], "rays": st["rays"], "paths": paths,

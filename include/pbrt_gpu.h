@@ -254,19 +254,22 @@ int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double r
 /* Where pbrt_scene_commit builds the meshes' BVHs: PBRT_BUILD_HOST (default; the host's cores, one
  * mesh per thread) or PBRT_BUILD_DEVICE (every mesh in one batched device build, the arrays of
  * pbrt_mesh_create_device: the same trees, so the same hits). pbrt_scene_build_seconds: wall time
- * of that step in the last commit. */
+ * of that step in the last commit, with the 4-wide collapse (on the device) when the scene walks
+ * 4-wide trees, without the upload of host-built arrays. */
 #define PBRT_BUILD_HOST 0
 #define PBRT_BUILD_DEVICE 1
 int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
-/* Tree the scene walks inside its meshes: 4 (default) trees collapsed to 4-wide nodes at commit (as
- * pbrt_launch.width 4 for a mesh), 2 the binary trees. Same hits and films either way. */
+/* Tree the scene walks inside its meshes: 4 (default) trees collapsed to 4-wide nodes on the device
+ * at commit (the rule of pbrt_launch.width 4 for a mesh), 2 the binary trees. Renders walk the
+ * binary trees for camera rays and, with the radiance cache, throughout (faster there). Same hits and
+ * films either way. */
 int pbrt_scene_set_mesh_width(pbrt_scene* scene, uint32_t width);
 /* Launch settings of the scene walks (pbrt_scene_trace_*, the renderer's extension and shadow rays):
  * LDS stack entries per lane (8, 16, 24; deeper entries spill to HBM), workgroups of 128 threads
  * per CU in the persistent grid (1-8; 0 = as many as fit), mesh tree width (2, 4; 4 collapses the
  * trees on first use), min_waves (0: the renderer's extension walk at the compiler's register
  * count; 5: its variant compiled for five waves per SIMD). Same hits and films for every setting.
- * Defaults 16, 0, 4, 5 (width 0 also means 4). */
+ * Defaults 16, 0, 4, 0 (width 0 also means 4). */
 typedef struct pbrt_scene_launch {
   uint32_t lds_stack, blocks_per_cu, width, min_waves;
 } pbrt_scene_launch;

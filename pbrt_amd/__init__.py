@@ -421,7 +421,7 @@ class Scene:
         except Exception:
             pass
 
-    def set_launch(self, lds_stack=0, blocks_per_cu=0, width=0, min_waves=5):
+    def set_launch(self, lds_stack=0, blocks_per_cu=0, width=0, min_waves=0):
         """Launch settings of the scene walks (pbrt_scene_set_launch); same hits for every setting."""
         call("pbrt_scene_set_launch", self.h,
              ctypes.byref(_native.SceneLaunch(lds_stack, blocks_per_cu, width, min_waves)))

@@ -938,6 +938,94 @@ __global__ void k_tris(const uint32_t* order, uint32_t nt, const float* xyz, con
   tris[i] = d;
 }
 
+
+// ---- 4-wide collapse (the host's collapse_bvh4 rule, level by level over every mesh) ------------------
+// A frontier entry: the binary node (global index) a 4-wide node at the level's slot base + i is made from.
+struct C4Entry {
+  uint32_t bin, mesh;
+};
+
+struct C4Child {
+  int32_t ref;  // binary reference, local to the mesh (or a leaf code)
+  float lo[3], hi[3];
+};
+
+__device__ inline void c4_children(const DevNode& d, C4Child* c) {
+  c[0] = C4Child{d.ref[0], {d.xy0[0], d.xy0[2], d.z01[0]}, {d.xy0[1], d.xy0[3], d.z01[1]}};
+  c[1] = C4Child{d.ref[1], {d.xy1[0], d.xy1[2], d.z01[2]}, {d.xy1[1], d.xy1[3], d.z01[3]}};
+}
+
+__device__ inline double c4_area(const C4Child& c) {
+  const double dx = (double)c.hi[0] - c.lo[0], dy = (double)c.hi[1] - c.lo[1], dz = (double)c.hi[2] - c.lo[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// One 4-wide node per frontier entry: its binary node's two children, the interior child of largest
+// surface area opened (the first of equal areas) until there are four; boxes, child count and the
+// children's binary references written to out[base + i]; cnt[i] = its interior children.
+__global__ void k_c4_expand(const DevNode* __restrict__ blas, const uint64_t* __restrict__ node_base,
+                            const C4Entry* __restrict__ F, uint32_t n, uint32_t base, DevNode4* __restrict__ out,
+                            uint32_t* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const C4Entry e = F[i];
+  const DevNode* nodes = blas + node_base[e.mesh];
+  C4Child ch[4];
+  int nc = 2;
+  c4_children(blas[e.bin], ch);
+  while (nc < 4) {
+    int best = -1;
+    double best_area = -1.0;
+    for (int k = 0; k < nc; ++k)
+      if (ch[k].ref >= 0 && c4_area(ch[k]) > best_area) {
+        best = k;
+        best_area = c4_area(ch[k]);
+      }
+    if (best < 0) break;
+    C4Child sub[2];
+    c4_children(nodes[ch[best].ref], sub);
+    for (int k = nc; k > best + 1; --k) ch[k] = ch[k - 1];
+    ch[best] = sub[0];
+    ch[best + 1] = sub[1];
+    ++nc;
+  }
+  DevNode4 d;
+  uint32_t interior = 0;
+  for (int k = 0; k < 4; ++k) {
+    const bool on = k < nc;
+    d.lox[k] = on ? ch[k].lo[0] : 0.f;
+    d.loy[k] = on ? ch[k].lo[1] : 0.f;
+    d.loz[k] = on ? ch[k].lo[2] : 0.f;
+    d.hix[k] = on ? ch[k].hi[0] : 0.f;
+    d.hiy[k] = on ? ch[k].hi[1] : 0.f;
+    d.hiz[k] = on ? ch[k].hi[2] : 0.f;
+    d.ref[k] = on ? ch[k].ref : 0;
+    d.count[k] = 0;
+    interior += (on && ch[k].ref >= 0) ? 1u : 0u;
+  }
+  d.count[0] = nc;
+  out[base + i] = d;
+  cnt[i] = interior;
+}
+
+// The interior children of entry i take the next level's slots next + (its exclusive prefix) + j,
+// in child order; their references are rewritten to those slots and they form the next frontier.
+__global__ void k_c4_link(const uint64_t* __restrict__ node_base, const C4Entry* __restrict__ F, uint32_t n,
+                          uint32_t base, uint32_t next, const uint32_t* __restrict__ cnt,
+                          const uint32_t* __restrict__ prefix, DevNode4* __restrict__ out, C4Entry* __restrict__ F2) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const C4Entry e = F[i];
+  DevNode4& d = out[base + i];
+  uint32_t j = prefix[i] - cnt[i];
+  for (int k = 0; k < 4; ++k) {
+    if (k >= d.count[0] || d.ref[k] < 0) continue;
+    F2[j] = C4Entry{(uint32_t)(node_base[e.mesh] + (uint64_t)d.ref[k]), e.mesh};
+    d.ref[k] = (int32_t)(next + j);
+    ++j;
+  }
+}
+
 // ---- host orchestration -------------------------------------------------------------------------
 struct DevBuf {
   void* p = nullptr;
@@ -1209,6 +1297,71 @@ void gpu_build_mesh_bvh(hipStream_t st, const float* xyz, uint32_t nv, const uin
   out->n_nodes = many.n_nodes;
   out->root = many.meshes[0].root;
   out->stats = many.stats;
+}
+
+void gpu_collapse4(hipStream_t st, const DevNode* blas, const std::vector<uint64_t>& node_base,
+                   const std::vector<int32_t>& root_ref, uint64_t n_binary, DevNode4** out, uint64_t* n_out,
+                   std::vector<int32_t>* root4) {
+  const size_t nm = node_base.size();
+  if (root_ref.size() != nm) throw Error(PBRT_ERR_INVALID_ARG, "one root per mesh");
+  if (n_binary >= (1ull << 31)) throw Error(PBRT_ERR_LAYOUT, "more than 2^31 nodes for the 4-wide layout");
+  root4->assign(nm, 0);
+  // the first level: every mesh with an interior root, in mesh order (a leaf root stays a leaf)
+  std::vector<C4Entry> first;
+  for (size_t m = 0; m < nm; ++m) {
+    if (root_ref[m] < 0) {
+      (*root4)[m] = root_ref[m];
+      continue;
+    }
+    (*root4)[m] = (int32_t)first.size();
+    first.push_back(C4Entry{(uint32_t)(node_base[m] + (uint64_t)root_ref[m]), (uint32_t)m});
+  }
+  *out = nullptr;
+  *n_out = 0;
+  // each 4-wide node consumes at least one binary node, so n_binary bounds the node and frontier counts
+  const size_t cap = std::max<size_t>(1, (size_t)n_binary);
+  DevBuf d_base(nm * sizeof(uint64_t)), fa(cap * sizeof(C4Entry)), fb(cap * sizeof(C4Entry));
+  DevBuf cnt(cap * sizeof(uint32_t)), prefix(cap * sizeof(uint32_t)), tile(blocks_for(cap, kScanTile) * sizeof(uint32_t));
+  DevBuf wide(cap * sizeof(DevNode4));
+  PBRT_HIP(hipMemcpyAsync(d_base.p, node_base.data(), nm * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  if (!first.empty())
+    PBRT_HIP(hipMemcpyAsync(fa.p, first.data(), first.size() * sizeof(C4Entry), hipMemcpyHostToDevice, st));
+  C4Entry* F = fa.as<C4Entry>();
+  C4Entry* F2 = fb.as<C4Entry>();
+  uint32_t n = (uint32_t)first.size(), base = 0;
+  while (n > 0) {
+    if ((uint64_t)base + n > cap) throw Error(PBRT_ERR_INTERNAL, "4-wide collapse exceeds its node bound");
+    hipLaunchKernelGGL(k_c4_expand, dim3(blocks_for(n, 256)), dim3(256), 0, st, blas, d_base.as<uint64_t>(), F, n,
+                       base, wide.as<DevNode4>(), cnt.as<uint32_t>());
+    const uint32_t tiles = blocks_for(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanBlock), 0, st, cnt.as<uint32_t>(), n, tile.as<uint32_t>());
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, tile.as<uint32_t>(), tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanBlock), 0, st, cnt.as<uint32_t>(), n, tile.as<uint32_t>(),
+                       prefix.as<uint32_t>());
+    const uint32_t next = base + n;
+    hipLaunchKernelGGL(k_c4_link, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_base.as<uint64_t>(), F, n, base, next,
+                       cnt.as<uint32_t>(), prefix.as<uint32_t>(), wide.as<DevNode4>(), F2);
+    PBRT_HIP(hipGetLastError());
+    uint32_t n2 = 0;
+    PBRT_HIP(hipMemcpyAsync(&n2, prefix.as<uint32_t>() + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipStreamSynchronize(st));
+    base = next;
+    n = n2;
+    std::swap(F, F2);
+  }
+  // the exact-size array
+  DevNode4* d = nullptr;
+  PBRT_HIP(hipMalloc(&d, std::max<size_t>(1, base) * sizeof(DevNode4)));
+  if (base) {
+    const hipError_t e = hipMemcpyAsync(d, wide.p, (size_t)base * sizeof(DevNode4), hipMemcpyDeviceToDevice, st);
+    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(st) : e;
+    if (e2 != hipSuccess) {
+      (void)hipFree(d);
+      PBRT_HIP(e2);
+    }
+  }
+  *out = d;
+  *n_out = base;
 }
 
 }  // namespace pbrt

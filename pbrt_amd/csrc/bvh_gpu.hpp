@@ -43,4 +43,16 @@ void gpu_build_meshes(hipStream_t stream, const float* xyz_dev, const uint32_t* 
 void gpu_build_mesh_bvh(hipStream_t stream, const float* xyz_dev, uint32_t nv, const uint32_t* idx_dev, uint32_t nt,
                         const pbrt_bvh_params& p, GpuBuilt* out);
 
+// Collapses every mesh's binary tree into 4-wide nodes on the device, by the host collapse's rule
+// (collapse_bvh4: each node takes its binary node's two children and opens the interior child of
+// largest surface area until it has four). blas: the meshes' binary nodes (mesh m's from
+// node_base[m], references local to the mesh, root root_ref[m]), n_binary of them in all. The 4-wide
+// nodes are laid out level by level over all meshes, each node's interior children consecutive;
+// interior references index the returned array (node base 0 for every mesh), leaf references are
+// unchanged. root4[m] receives mesh m's root reference (a leaf root stays a leaf). *out is
+// hipMalloc'd (n_out nodes); the caller frees it. Returns when the array is complete.
+void gpu_collapse4(hipStream_t stream, const DevNode* blas, const std::vector<uint64_t>& node_base,
+                   const std::vector<int32_t>& root_ref, uint64_t n_binary, DevNode4** out, uint64_t* n_out,
+                   std::vector<int32_t>* root4);
+
 }  // namespace pbrt

@@ -67,7 +67,7 @@ struct pbrt_scene {
   pbrt_ctx* ctx = nullptr;
   pbrt_bvh_params mesh_params{12, 4, 1.0};
   bool device_build = false;  // mesh BVHs built on the device at commit (one batched build)
-  pbrt_scene_launch launch{16, 0, 4, 5};  // scene walks: LDS stack, workgroups per CU, tree width, min waves
+  pbrt_scene_launch launch{16, 0, 4, 0};  // scene walks: LDS stack, workgroups per CU, tree width, min waves
   uint32_t blocks_fit[3] = {0, 0, 0};  // workgroups per CU that fit, per stack depth (8, 16, 24)
   struct MeshRange {
     uint32_t node_base;
@@ -76,7 +76,7 @@ struct pbrt_scene {
   };
   std::vector<MeshRange> mesh_ranges;  // each mesh's binary nodes in blas (for a later collapse)
   std::vector<DevObject> dobjs_host;   // the device objects as uploaded
-  double build_seconds = 0.0;  // wall time of the mesh BVH builds at commit
+  double build_seconds = 0.0;  // wall time of the mesh BVH builds and the 4-wide collapse at commit
   std::vector<std::vector<float>> mesh_xyz;
   std::vector<std::vector<uint32_t>> mesh_idx;
   std::vector<HostObject> objects;
@@ -523,64 +523,24 @@ int pbrt_scene_add_area_light(pbrt_scene* s, uint32_t object, const double rgb[3
   });
 }
 
-// The 4-wide trees of every mesh, collapsed on the host from its binary nodes (host_blas, or the
-// device array when null), uploaded as one array; base4 / root4 receive each mesh's first node
-// and root reference in it.
-static void collapse_scene_meshes(pbrt_scene* s, const std::vector<DevNode>* host_blas, std::vector<int32_t>* base4,
-                                  std::vector<int32_t>* root4) {
+// The 4-wide trees of every mesh, collapsed on the device from the binary nodes in s->blas
+// (gpu_collapse4: the host collapse's rule, laid out level by level over all meshes, references
+// global, so every mesh's node base is 0); root4 receives each mesh's root reference.
+static void collapse_scene_meshes(pbrt_scene* s, std::vector<int32_t>* base4, std::vector<int32_t>* root4) {
   const size_t nm = s->mesh_ranges.size();
-  std::vector<DevNode> all;
-  if (!host_blas) {
-    uint64_t total = 0;
-    for (const auto& r : s->mesh_ranges) total = std::max<uint64_t>(total, (uint64_t)r.node_base + r.n_nodes);
-    all.resize(total);
-    if (total) PBRT_HIP(hipMemcpy(all.data(), s->blas, total * sizeof(DevNode), hipMemcpyDeviceToHost));
-  }
-  const std::vector<DevNode>& src = host_blas ? *host_blas : all;
-  base4->assign(nm, 0);
-  root4->assign(nm, 0);
-  // the meshes collapse independently, on the host's cores; then concatenated in mesh order
-  std::vector<std::vector<DevNode4>> per(nm);
-  {
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::atomic<size_t> next{0};
-    std::vector<std::string> errors(hw);
-    auto work = [&](unsigned w) {
-      try {
-        for (size_t k = next++; k < nm; k = next++) {
-          const auto& r = s->mesh_ranges[k];
-          const std::vector<DevNode> n2(src.begin() + r.node_base, src.begin() + r.node_base + r.n_nodes);
-          collapse_bvh4(n2, r.root_ref, &per[k], &(*root4)[k]);
-        }
-      } catch (const std::exception& e) {
-        errors[w] = e.what();
-      }
-    };
-    std::vector<std::thread> pool;
-    for (unsigned w = 1; w < std::min<size_t>(hw, nm); ++w) pool.emplace_back(work, w);
-    work(0);
-    for (auto& t : pool) t.join();
-    for (const auto& e : errors)
-      if (!e.empty()) throw Error(PBRT_ERR_INTERNAL, "4-wide collapse: " + e);
-  }
-  size_t total4 = 0;
-  for (const auto& n4 : per) total4 += n4.size();
-  std::vector<DevNode4> blas4;
-  blas4.reserve(total4);
+  std::vector<uint64_t> node_base(nm);
+  std::vector<int32_t> root_ref(nm);
+  uint64_t n_binary = 0;
   for (size_t k = 0; k < nm; ++k) {
-    (*base4)[k] = (int32_t)blas4.size();
-    blas4.insert(blas4.end(), per[k].begin(), per[k].end());
-    std::vector<DevNode4>().swap(per[k]);
+    const auto& r = s->mesh_ranges[k];
+    node_base[k] = r.node_base;
+    root_ref[k] = r.root_ref;
+    n_binary = std::max<uint64_t>(n_binary, (uint64_t)r.node_base + r.n_nodes);
   }
   DevNode4* d = nullptr;
-  PBRT_HIP(hipMalloc(&d, std::max<size_t>(1, blas4.size()) * sizeof(DevNode4)));
-  if (!blas4.empty()) {
-    const hipError_t e = hipMemcpy(d, blas4.data(), blas4.size() * sizeof(DevNode4), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(d);
-      PBRT_HIP(e);
-    }
-  }
+  uint64_t n4 = 0;
+  gpu_collapse4(s->ctx->stream, s->blas, node_base, root_ref, n_binary, &d, &n4, root4);
+  base4->assign(nm, 0);
   s->blas4 = d;
 }
 
@@ -588,7 +548,7 @@ static void collapse_scene_meshes(pbrt_scene* s, const std::vector<DevNode>* hos
 static void ensure_scene_blas4(pbrt_scene* s) {
   if (s->blas4 || s->mesh_ranges.empty()) return;
   std::vector<int32_t> base4, root4;
-  collapse_scene_meshes(s, nullptr, &base4, &root4);
+  collapse_scene_meshes(s, &base4, &root4);
   for (DevObject& d : s->dobjs_host)
     if (d.shape == kShapeMesh) {
       d.node_base4 = base4[d.mesh];
@@ -688,6 +648,7 @@ int pbrt_scene_commit(pbrt_scene* s) {
       idx.insert(idx.end(), ix.begin(), ix.end());
     }
     const auto build_t0 = std::chrono::steady_clock::now();
+    double upload_s = 0.0;
     if (s->device_build && nm > 0) {
       // one batched device build (bvh_gpu.hip): every mesh's nodes and triangles in place
       dev_upload(&s->verts, verts);
@@ -759,18 +720,26 @@ int pbrt_scene_commit(pbrt_scene* s) {
         blas.insert(blas.end(), n.begin(), n.end());
         tris.insert(tris.end(), t.begin(), t.end());
       }
+      // the arrays go to the device now (the 4-wide collapse runs there); the upload is not counted
+      // as build time
+      const auto up_t0 = std::chrono::steady_clock::now();
+      dev_upload(&s->blas, blas);
+      dev_upload(&s->tris, tris);
+      dev_upload(&s->verts, verts);
+      dev_upload(&s->idx, idx);
+      upload_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - up_t0).count();
     }
     s->mesh_ranges.resize(nm);
     for (size_t k = 0; k < nm; ++k) s->mesh_ranges[k] = {(uint32_t)info[k].node_base, info[k].n_nodes, info[k].root_ref};
     if (s->launch.width == 4 && nm > 0) {
       std::vector<int32_t> base4, root4;
-      collapse_scene_meshes(s, s->blas ? nullptr : &blas, &base4, &root4);
+      collapse_scene_meshes(s, &base4, &root4);
       for (size_t k = 0; k < nm; ++k) {
         info[k].node_base4 = base4[k];
         info[k].root_ref4 = root4[k];
       }
     }
-    s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - build_t0).count();
+    s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - build_t0).count() - upload_s;
     // object boxes (local bounds through the transform) and the median-split object BVH
     const uint32_t no = (uint32_t)s->objects.size();
     std::vector<Box64> boxes(no);
@@ -861,12 +830,6 @@ int pbrt_scene_commit(pbrt_scene* s) {
     }
     dev_upload(&s->dcsg, dc);
     dev_upload(&s->tlas, tlas);
-    if (!s->blas) {  // host-built meshes (the device build leaves its arrays in place)
-      dev_upload(&s->blas, blas);
-      dev_upload(&s->tris, tris);
-      dev_upload(&s->verts, verts);
-      dev_upload(&s->idx, idx);
-    }
     dev_upload(&s->objs, dobjs);
     s->dobjs_host = dobjs;
     dev_upload(&s->mats, s->materials);
@@ -1181,11 +1144,15 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     const SceneView v = view_of(s);
     // camera rays walk the meshes' binary trees: on rays in pixel order the binary walk, which
     // culls more boxes, is the faster one; later bounces take the scene's setting. A render with
-    // the radiance cache walks the binary trees throughout: its paths end after two bounces, and
-    // with its two streams overlapping it measured faster that way (1.18 against 1.20 ms per
-    // 1280x720 frame on the room with the 1.3M-triangle mesh); same films either way.
+    // the radiance cache walks the binary trees throughout, with the extension walk's five-wave
+    // variant: its paths end after two bounces and the kernels of its two streams share the CUs,
+    // where the smaller register footprint lets more waves in. Measured per 1280x720 frame on the
+    // room with the 1.3M-triangle mesh: 1.18 ms that way, 1.20 with the 4-wide walk, 1.22 with the
+    // compiler's register count. Same films either way.
     SceneView v_binary = v;
     v_binary.blas4 = nullptr;
+    SceneView v_cache = v_binary;
+    v_cache.occ5 = 1u;
     const uint32_t grid = persistent_grid(s, npass_paths);
     const uint32_t spill_per = ensure_spill(s, grid);
     uint64_t passes = 0;
@@ -1279,30 +1246,26 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       int q = 0;
       const uint32_t front = std::min(p.query_vertex, c->max_depth);
       for (uint32_t b = 0; b < front; ++b) {
-        PBRT_HIP(launch_extend(v_binary, B, q, s->spill, spill_per, grid, s->status, st));
+        PBRT_HIP(launch_extend(v_cache, B, q, s->spill, spill_per, grid, s->status, st));
         PBRT_HIP(launch_reset_queue(B, q, st));
         PBRT_HIP(launch_shade_nrc(v, p, B, q, st));
-        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v_binary, p, B, s->spill, spill_per, grid, s->status, st));
+        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v_cache, p, B, s->spill, spill_per, grid, s->status, st));
         q ^= 1;
       }
       PBRT_HIP(hipEventRecord(ev_front[S], st));
       // back (second stream, overlapping the next pass's front): the last front shadow rays, the
-      // remaining bounces, the cache queries of the whole pass answered by the cache trained on
-      // every earlier pass (this stream trained it, in pass order), the film, the training
+      // remaining bounces, the training on this pass's training paths (this stream trains the cache
+      // in pass order), then the cache queries of the whole pass, answered by the cache trained on
+      // this pass and every earlier one, and the film
       PBRT_HIP(hipStreamWaitEvent(aux, ev_front[S], 0));
-      PBRT_HIP(launch_shadow_nrc(v_binary, p, B, spill_aux, spill_per, grid, s->status, aux));
+      PBRT_HIP(launch_shadow_nrc(v_cache, p, B, spill_aux, spill_per, grid, s->status, aux));
       for (uint32_t b = front; b < c->max_depth; ++b) {
-        PBRT_HIP(launch_extend(v_binary, B, q, spill_aux, spill_per, grid, s->status, aux));
+        PBRT_HIP(launch_extend(v_cache, B, q, spill_aux, spill_per, grid, s->status, aux));
         PBRT_HIP(launch_reset_queue(B, q, aux));
         PBRT_HIP(launch_shade_nrc(v, p, B, q, aux));
-        PBRT_HIP(launch_shadow_nrc(v_binary, p, B, spill_aux, spill_per, grid, s->status, aux));
+        PBRT_HIP(launch_shadow_nrc(v_cache, p, B, spill_aux, spill_per, grid, s->status, aux));
         q ^= 1;
       }
-      PBRT_HIP(nrc_launch_query_apply(nrc->dev.infer(), (const float*)B.q_raw, B.q_beta, B.L,
-                                      reinterpret_cast<unsigned long long*>(counter(B.counts, kCntQueries)),
-                                      nrc->lo, nrc->hi, npass, counter(B.counts, kCntQuery),
-                                      (uint32_t)s->ctx->cu_count, aux));
-      PBRT_HIP(launch_accumulate(p, B, aux));
       if (p.train_stride) {
         const uint32_t ranks = (npass - 1 + p.train_offset) / p.train_stride + 1;
         PBRT_HIP(launch_nrc_targets(p, B, tb, ranks, aux));
@@ -1321,6 +1284,11 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
                                   hipMemcpyDeviceToHost, aux));
         }
       }
+      PBRT_HIP(nrc_launch_query_apply(nrc->dev.infer(), (const float*)B.q_raw, B.q_beta, B.L,
+                                      reinterpret_cast<unsigned long long*>(counter(B.counts, kCntQueries)),
+                                      nrc->lo, nrc->hi, npass, counter(B.counts, kCntQuery),
+                                      (uint32_t)s->ctx->cu_count, aux));
+      PBRT_HIP(launch_accumulate(p, B, aux));
       if (passes < n_record) {
         if (ref)
           PBRT_HIP(launch_rmse(film, 1.0 / (double)(first + p.pass_samples), ref, npix * 3, rmse_partial, kRmsePartials,

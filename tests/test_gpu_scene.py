@@ -373,6 +373,46 @@ def test_wide_mesh_walk_same_hits_and_films(gpu_ctx, name, device_build):
     np.testing.assert_array_equal(fa, fb)
 
 
+@pytest.mark.parametrize("device_build", [False, True])
+def test_wide_mesh_walk_leaf_roots_and_mixed_meshes(gpu_ctx, device_build):
+    """Meshes whose tree is a single leaf (one triangle, a two-triangle quad) beside deep ones
+    (bunny, a displaced icosphere) under transforms, in one scene: the 4-wide trees, collapsed on
+    the device level by level over all meshes, give the binary walk's hits, occlusion and films,
+    and Oracle B's hits."""
+    d = SceneDesc()
+    white = d.add_material("lambertian", (0.7, 0.7, 0.7))
+    tri = d.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    quad = d.add_mesh(np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32),
+                      np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
+    bv, bf = load_mesh("bunny")
+    bunny = d.add_mesh(bv * 10.0, bf)
+    iv, iff = pbrt_amd.icosphere(4, noise="fbm", amp=0.1, octaves=3, seed=3)
+    ico = d.add_mesh(iv, iff)
+    d.add_object("mesh", white, mesh=quad, transform=Transform.scale(6.0, 1.0, 6.0) * Transform.translation(0, -1, 0))
+    d.add_object("mesh", white, mesh=tri, transform=Transform.translation(-1.5, 0.0, 0.5))
+    d.add_object("mesh", white, mesh=bunny, transform=Transform.translation(0.5, -1.3, 0.0))
+    d.add_object("mesh", white, mesh=ico, transform=Transform.translation(2.0, 0.2, -0.5))
+    d.add_object("mesh", white, mesh=tri, transform=Transform.rotation("y", 0.7) * Transform.translation(1.0, 1.0, 1.0))
+    d.add_point_light((0.0, 4.0, 3.0), (20.0, 20.0, 20.0))
+    eye, look, fov = (0.0, 1.5, 6.0), (0.0, 0.0, 0.0), math.pi / 3
+    a = pbrt_amd.Scene(gpu_ctx, d, mesh_width=2)
+    b = pbrt_amd.Scene(gpu_ctx, d, mesh_width=4, device_build=device_build)
+    rays = _mixed_rays(eye, look, fov)
+    ha, hb = a.trace_host(rays), b.trace_host(rays)
+    np.testing.assert_array_equal(ha["object"], hb["object"])
+    hit = ha["object"] >= 0
+    assert hit.mean() > 0.2
+    assert len(np.unique(ha["object"][hit])) == 5  # every object is hit somewhere
+    for field in ("prim", "t", "u", "v"):
+        np.testing.assert_array_equal(ha[field][hit], hb[field][hit], err_msg=field)
+    np.testing.assert_array_equal(a.trace_host(rays, any_hit=True), b.trace_host(rays, any_hit=True))
+    compare_hits(b.trace_host(rays), po.Scene(d, precision=32, **pbrt_amd.DEVICE_BVH).trace(rays))
+    assert b.status() == 0
+    fa, _ = a.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
+    fb, _ = b.render(48, 36, 4, 6, eye, look, fov=fov, seed=9)
+    np.testing.assert_array_equal(fa, fb)
+
+
 def _mixed_rays(eye, look, fov, w=160, h=120, seed=4):
     """Camera rays and the same origins with seeded random directions."""
     rays = po.pinhole_rays(w, h, fov, eye, look).astype(np.float32)
