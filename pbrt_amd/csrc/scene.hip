@@ -282,7 +282,9 @@ __device__ __forceinline__ bool push_or_flag(Stack<BLOCK, DEPTH>& st, int32_t re
 }
 
 // BVHTree::query on an object's mesh (nested on the lane's stack above `base`): closest (t, prim)
-// over [near, far] with the order-independent tie rule.
+// over [near, far] with the order-independent tie rule. Each iteration advances the lane by one
+// interior node or by one triangle of its leaf, so a wave never runs a whole leaf loop for the
+// few lanes in leaves (the mesh kernels' flat loop, inside the scene walk).
 template <int BLOCK, int DEPTH>
 __device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayState& ray, float far,
                              Stack<BLOCK, DEPTH>& st, int* status, float* t_out, float* u_out, float* v_out,
@@ -299,6 +301,7 @@ __device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayS
   float min_t = FLT_MAX, cur_far = far;
   uint32_t best = 0xffffffffu;
   int32_t cur = ob.root_ref;
+  uint32_t tk = 0, tend = 0;  // triangles of the current leaf still to test: [tk, tend)
   bool live = true;
   while (live) {
     bool pop_now = false;
@@ -324,23 +327,24 @@ __device__ bool mesh_closest(const SceneView& S, const DevObject& ob, const RayS
       }
     }
     if (!pop_now && cur < 0) {
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code >> kLeafCountBits;
-      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        float t, u, v;
-        uint32_t id;
-        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= far &&
-            (t < min_t || (t == min_t && id < best))) {
-          min_t = t;
-          best = id;
-          *t_out = t;
-          *u_out = u;
-          *v_out = v;
-          cur_far = fminf(far, min_t);
-        }
+      if (tk == tend) {  // a leaf reached: ~cur = (first << 5) | (count - 1)
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
       }
-      pop_now = true;
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= far &&
+          (t < min_t || (t == min_t && id < best))) {
+        min_t = t;
+        best = id;
+        *t_out = t;
+        *u_out = u;
+        *v_out = v;
+        cur_far = fminf(far, min_t);
+      }
+      ++tk;
+      pop_now = tk == tend;
     }
     if (pop_now) {
       live = false;
@@ -371,6 +375,7 @@ __device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState
   const DevNode* nodes = S.blas + ob.node_base;
   const DevTri* tris = S.tris + ob.tri_base;
   int32_t cur = ob.root_ref;
+  uint32_t tk = 0, tend = 0;  // triangles of the current leaf still to test: [tk, tend)
   bool hit = false, live = true;
   while (live && !hit) {
     bool pop_now = false;
@@ -395,18 +400,16 @@ __device__ bool mesh_any(const SceneView& S, const DevObject& ob, const RayState
       }
     }
     if (!pop_now && cur < 0) {
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code >> kLeafCountBits;
-      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        float t, u, v;
-        uint32_t id;
-        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
-          hit = true;
-          break;
-        }
+      if (tk == tend) {
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
       }
-      pop_now = true;
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) hit = true;
+      ++tk;
+      pop_now = !hit && tk == tend;
     }
     if (pop_now) {
       live = st.sp > base;
@@ -435,6 +438,7 @@ __device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const Ray
   float min_t = FLT_MAX, cur_far = far;
   uint32_t best = 0xffffffffu;
   int32_t cur = ob.root_ref4;
+  uint32_t tk = 0, tend = 0;  // triangles of the current leaf still to test: [tk, tend)
   bool live = true;
   while (live) {
     bool pop_now = false;
@@ -452,23 +456,24 @@ __device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const Ray
       }
     }
     if (!pop_now && cur < 0) {
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code >> kLeafCountBits;
-      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        float t, u, v;
-        uint32_t id;
-        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= far &&
-            (t < min_t || (t == min_t && id < best))) {
-          min_t = t;
-          best = id;
-          *t_out = t;
-          *u_out = u;
-          *v_out = v;
-          cur_far = fminf(far, min_t);
-        }
+      if (tk == tend) {  // a leaf reached: ~cur = (first << 5) | (count - 1)
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
       }
-      pop_now = true;
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= far &&
+          (t < min_t || (t == min_t && id < best))) {
+        min_t = t;
+        best = id;
+        *t_out = t;
+        *u_out = u;
+        *v_out = v;
+        cur_far = fminf(far, min_t);
+      }
+      ++tk;
+      pop_now = tk == tend;
     }
     if (pop_now) {
       live = false;
@@ -499,6 +504,7 @@ __device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayStat
   const DevNode4* nodes = S.blas4 + ob.node_base4;
   const DevTri* tris = S.tris + ob.tri_base;
   int32_t cur = ob.root_ref4;
+  uint32_t tk = 0, tend = 0;  // triangles of the current leaf still to test: [tk, tend)
   bool hit = false, live = true;
   while (live && !hit) {
     bool pop_now = false;
@@ -516,18 +522,16 @@ __device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayStat
       }
     }
     if (!pop_now && cur < 0) {
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code >> kLeafCountBits;
-      const uint32_t cnt = (code & (kMaxLeafTris - 1)) + 1;
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        float t, u, v;
-        uint32_t id;
-        if (tri_test(r, tris, k, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
-          hit = true;
-          break;
-        }
+      if (tk == tend) {
+        const uint32_t code = ~(uint32_t)cur;
+        tk = code >> kLeafCountBits;
+        tend = tk + (code & (kMaxLeafTris - 1)) + 1;
       }
-      pop_now = true;
+      float t, u, v;
+      uint32_t id;
+      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) hit = true;
+      ++tk;
+      pop_now = !hit && tk == tend;
     }
     if (pop_now) {
       live = st.sp > base;
