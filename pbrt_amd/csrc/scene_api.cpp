@@ -1142,16 +1142,14 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     }
 
     const SceneView v = view_of(s);
-    // camera rays walk the meshes' binary trees: on rays in pixel order the binary walk, which
-    // culls more boxes, is the faster one; later bounces take the scene's setting. A render with
-    // the radiance cache walks the binary trees throughout, with the extension walk's five-wave
-    // variant: its paths end after two bounces and the kernels of its two streams share the CUs,
-    // where the smaller register footprint lets more waves in. Measured per 1280x720 frame on the
-    // room with the 1.3M-triangle mesh: 1.18 ms that way, 1.20 with the 4-wide walk, 1.22 with the
-    // compiler's register count. Same films either way.
-    SceneView v_binary = v;
-    v_binary.blas4 = nullptr;
-    SceneView v_cache = v_binary;
+    // A render with the radiance cache walks the meshes' binary trees with the extension walk's
+    // five-wave variant: its paths end after two bounces and the kernels of its two streams share
+    // the CUs, where the smaller register footprint lets more waves in. Measured per 1280x720
+    // frame on the room with the 1.3M-triangle mesh: 1.18 ms that way, 1.20 with the 4-wide walk,
+    // 1.22 with the compiler's register count (the scene's settings, which plain renders use).
+    // Same films either way.
+    SceneView v_cache = v;
+    v_cache.blas4 = nullptr;
     v_cache.occ5 = 1u;
     const uint32_t grid = persistent_grid(s, npass_paths);
     const uint32_t spill_per = ensure_spill(s, grid);
@@ -1218,7 +1216,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
         PBRT_HIP(launch_generate(v, p, pb, st));
         int q = 0;
         for (uint32_t b = 0; b < c->max_depth; ++b) {
-          PBRT_HIP(launch_extend(b == 0 ? v_binary : v, pb, q, s->spill, spill_per, grid, s->status, st));
+          PBRT_HIP(launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st));
           PBRT_HIP(launch_reset_queue(pb, q, st));
           PBRT_HIP(launch_shade(v, p, pb, q, st));
           PBRT_HIP(launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st));
