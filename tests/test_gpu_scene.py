@@ -81,7 +81,14 @@ def rmse(a, b):
 
 @pytest.mark.parametrize("which", ["small", "room"])
 def test_render_matches_oracle(gpu_ctx, which):
-    desc, cam = scenes.small() if which == "small" else scenes.room(subdiv=4)
+    """The small scene (5,120-triangle icosphere, floor, 64 spheres, point light) and the room with
+    its 81,920-triangle dielectric icosphere, at 64^2 x 8 spp, against the FP32 oracle's film under
+    the same sampler: RMSE below 0.2 % of the mean radiance, 99.5 % of the pixels within 1e-3
+    relative, and most pixels bit-identical on the small scene (measured 0.033 %, 99.98 %, 94 %; the
+    room 0.001 %, 100 %, 27 %: its dielectric and area-light paths pass through more transcendental
+    functions, where the device and host libm differ in the last bit); against the f64 oracle within
+    5 % (measured 1.9 % and 0.09 %)."""
+    desc, cam = scenes.small() if which == "small" else scenes.room()
     w = h = 64
     spp = 8
     scene = pbrt_amd.Scene(gpu_ctx, desc)
@@ -92,7 +99,12 @@ def test_render_matches_oracle(gpu_ctx, which):
     scale = float(f64.mean())
     assert scale > 0
     # stated bars, relative to the mean radiance of the image
-    assert rmse(film, f32) / scale < 0.02, rmse(film, f32) / scale
+    close = np.abs(film - f32) <= 1e-3 * np.maximum(np.abs(f32), 1e-3 * scale)
+    frac, same = float(close.all(-1).mean()), float((film == f32).all(-1).mean())
+    print(f"[{which}] vs FP32 oracle: rmse/mean {rmse(film, f32) / scale:.5f}, within 1e-3 {frac:.4f}, "
+          f"bit-identical {same:.4f}; vs f64 rmse/mean {rmse(film, f64) / scale:.5f}", flush=True)
+    assert rmse(film, f32) / scale < 0.002, rmse(film, f32) / scale
+    assert frac > 0.995 and same > (0.8 if which == "small" else 0.2), (frac, same)
     assert rmse(film, f64) / scale < 0.05, rmse(film, f64) / scale
     assert abs(film.mean() - f64.mean()) / scale < 0.01
     assert stats["paths"] == w * h * spp and stats["rays"] > stats["paths"]
