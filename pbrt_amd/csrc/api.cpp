@@ -104,6 +104,9 @@ void ensure_bvh4(pbrt_mesh* m) {
   std::vector<DevNode4> n4;
   int32_t root4 = 0;
   collapse_bvh4(n2, m->root.ref, &n4, &root4);
+  // the kernels address 4-wide nodes by 32-bit byte offsets
+  if ((uint64_t)n4.size() * sizeof(DevNode4) > 0xffffffffull)
+    throw Error(PBRT_ERR_LAYOUT, "4-wide tree above 4 GiB (use the binary tree, launch width 2)");
   DevNode4* d = nullptr;
   PBRT_HIP(hipMalloc(&d, std::max<size_t>(1, n4.size()) * sizeof(DevNode4)));
   if (!n4.empty()) {

@@ -192,12 +192,9 @@ __device__ __forceinline__ bool box_test(const RayState& r, float lx, float hx, 
 }
 
 // Möller–Trumbore with the edges pre-computed in FP32 on the host (the same subtraction the
-// reference performs first).
-__device__ __forceinline__ bool tri_test(const RayState& r, const DevTri* __restrict__ tris, uint32_t k, float* t_out,
-                                         float* u_out, float* v_out, uint32_t* id_out) {
-  const float4 a = *reinterpret_cast<const float4*>(tris[k].p0);
-  const float4 e1 = *reinterpret_cast<const float4*>(tris[k].e1);
-  const float4 e2 = *reinterpret_cast<const float4*>(tris[k].e2);
+// reference performs first): a = (p0, original index bits), e1 = p1 - p0, e2 = p2 - p0.
+__device__ __forceinline__ bool tri_test_words(const RayState& r, const float4 a, const float4 e1, const float4 e2,
+                                               float* t_out, float* u_out, float* v_out, uint32_t* id_out) {
   // pv = cross(d, e2)
   const float pvx = r.dy * e2.z - r.dz * e2.y;
   const float pvy = r.dz * e2.x - r.dx * e2.z;
@@ -221,23 +218,90 @@ __device__ __forceinline__ bool tri_test(const RayState& r, const DevTri* __rest
   return true;
 }
 
-// The 4-wide node step shared by the mesh and scene kernels: the children's boxes tested over
-// [near, tmax], hit children ordered by entry distance (a box the ray misses sorts last with an
-// infinite key: a hit's entry is finite, at most tmax). Returns the number of children hit; the
-// nearest is in r0 / k0, the others in r1..r3 / k1..k3 in increasing distance.
-__device__ __forceinline__ int node4_step(const RayState& r, const DevNode4* __restrict__ nodes, int32_t cur, float tmax,
-                                          int32_t& r0, int32_t& r1, int32_t& r2, int32_t& r3, float& k0, float& k1,
-                                          float& k2, float& k3) {
-  // the near and far plane words of each axis, chosen by the ray's sign through the load offset
-  // (in box order for a parallel ray)
+__device__ __forceinline__ bool tri_test(const RayState& r, const DevTri* __restrict__ tris, uint32_t k, float* t_out,
+                                         float* u_out, float* v_out, uint32_t* id_out) {
+  return tri_test_words(r, *reinterpret_cast<const float4*>(tris[k].p0), *reinterpret_cast<const float4*>(tris[k].e1),
+                        *reinterpret_cast<const float4*>(tris[k].e2), t_out, u_out, v_out, id_out);
+}
+
+// A buffer resource over a 4-wide node array (base in scalar registers; raw loads, no range
+// limit): the node loads then take one 32-bit byte offset each instead of a 64-bit address. A
+// 4-wide array stays below 4 GiB (the host refuses larger ones).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t node4_rsrc(const DevNode4* nodes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<DevNode4*>(nodes), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ float4 load_f4(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+}
+
+// The same over a mesh's triangle array (one mesh's triangles stay below 4 GiB: the leaf code
+// bounds a mesh to 2^26 triangles, 3.2 GB): triangle k at byte offset 48 k.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tri_rsrc(const DevTri* tris) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<DevTri*>(tris), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ bool tri_test(const RayState& r, __amdgpu_buffer_rsrc_t tris, uint32_t k, float* t_out,
+                                         float* u_out, float* v_out, uint32_t* id_out) {
+  const uint32_t o = k * 48u;
+  return tri_test_words(r, load_f4(tris, o), load_f4(tris, o + 16u), load_f4(tris, o + 32u), t_out, u_out, v_out,
+                        id_out);
+}
+
+// The words of a 4-wide node a ray's step reads: the near and far plane words of each axis, chosen
+// by the ray's sign through the load offset (in box order for a parallel ray), the references and
+// the child count.
+struct Node4Words {
+  float4 x0, x1, y0, y1, z0, z1;
+  int4 rf;
+  int cnt;
+};
+
+// From the mesh kernels' buffer resource (node4_rsrc; node cur at byte offset 128 cur).
+__device__ __forceinline__ Node4Words load_node4(const RayState& r, __amdgpu_buffer_rsrc_t nodes, uint32_t cur) {
+  const uint32_t o = cur << 7;
+  Node4Words w;
+  w.x0 = load_f4(nodes, o | r.sx);
+  w.x1 = load_f4(nodes, o | (r.sx ^ 16u));
+  w.y0 = load_f4(nodes, o | (32u | r.sy));
+  w.y1 = load_f4(nodes, o | (32u | (r.sy ^ 16u)));
+  w.z0 = load_f4(nodes, o | (64u | r.sz));
+  w.z1 = load_f4(nodes, o | (64u | (r.sz ^ 16u)));
+  const float4 rc = load_f4(nodes, o | 96u);
+  w.rf = make_int4(__float_as_int(rc.x), __float_as_int(rc.y), __float_as_int(rc.z), __float_as_int(rc.w));
+  w.cnt = (int)__builtin_amdgcn_raw_buffer_load_b32(nodes, o | 112u, 0, 0);
+  return w;
+}
+
+// From a node array by address (the scene walks, whose array base differs per lane's object: a
+// buffer resource needs a wave-uniform base).
+__device__ __forceinline__ Node4Words load_node4(const RayState& r, const DevNode4* __restrict__ nodes, uint32_t cur) {
   const char* nb = reinterpret_cast<const char*>(nodes + cur);
-  const float4 x0 = *reinterpret_cast<const float4*>(nb + r.sx), x1 = *reinterpret_cast<const float4*>(nb + (r.sx ^ 16u));
-  const float4 y0 = *reinterpret_cast<const float4*>(nb + 32 + r.sy);
-  const float4 y1 = *reinterpret_cast<const float4*>(nb + 32 + (r.sy ^ 16u));
-  const float4 z0 = *reinterpret_cast<const float4*>(nb + 64 + r.sz);
-  const float4 z1 = *reinterpret_cast<const float4*>(nb + 64 + (r.sz ^ 16u));
-  const int4 rf = *reinterpret_cast<const int4*>(nb + 96);
-  const int cnt = reinterpret_cast<const int4*>(nb + 112)->x;
+  Node4Words w;
+  w.x0 = *reinterpret_cast<const float4*>(nb + r.sx);
+  w.x1 = *reinterpret_cast<const float4*>(nb + (r.sx ^ 16u));
+  w.y0 = *reinterpret_cast<const float4*>(nb + 32 + r.sy);
+  w.y1 = *reinterpret_cast<const float4*>(nb + 32 + (r.sy ^ 16u));
+  w.z0 = *reinterpret_cast<const float4*>(nb + 64 + r.sz);
+  w.z1 = *reinterpret_cast<const float4*>(nb + 64 + (r.sz ^ 16u));
+  w.rf = *reinterpret_cast<const int4*>(nb + 96);
+  w.cnt = reinterpret_cast<const int4*>(nb + 112)->x;
+  return w;
+}
+
+// The 4-wide node step shared by the mesh and scene kernels: node `cur` (from a buffer resource or
+// an array), its children's boxes tested over [near, tmax], hit children ordered by entry distance
+// (a box the ray misses sorts last with an infinite key: a hit's entry is finite, at most tmax).
+// Returns the number of children hit; the nearest is in r0 / k0, the others in r1..r3 / k1..k3 in
+// increasing distance.
+template <class Nodes>
+__device__ __forceinline__ int node4_step(const RayState& r, Nodes nodes, uint32_t cur, float tmax, int32_t& r0,
+                                          int32_t& r1, int32_t& r2, int32_t& r3, float& k0, float& k1, float& k2,
+                                          float& k3) {
+  const Node4Words w = load_node4(r, nodes, cur);
+  const float4 &x0 = w.x0, &x1 = w.x1, &y0 = w.y0, &y1 = w.y1, &z0 = w.z0, &z1 = w.z1;
+  const int4& rf = w.rf;
+  const int cnt = w.cnt;
   float d0, d1, d2, d3;
   bool h0, h1, h2, h3;
   if (__builtin_expect(r.once, 1)) {

@@ -445,7 +445,7 @@ __device__ bool mesh_closest4(const SceneView& S, const DevObject& ob, const Ray
     if (cur >= 0) {
       int32_t r0, r1, r2, r3;
       float k0, k1, k2, k3;
-      const int nh = node4_step(r, nodes, cur, cur_far, r0, r1, r2, r3, k0, k1, k2, k3);
+      const int nh = node4_step(r, nodes, (uint32_t)cur, cur_far, r0, r1, r2, r3, k0, k1, k2, k3);
       if (nh == 0) {
         pop_now = true;
       } else {
@@ -511,7 +511,7 @@ __device__ bool mesh_any4(const SceneView& S, const DevObject& ob, const RayStat
     if (cur >= 0) {
       int32_t r0, r1, r2, r3;
       float k0, k1, k2, k3;
-      const int nh = node4_step(r, nodes, cur, r.far, r0, r1, r2, r3, k0, k1, k2, k3);
+      const int nh = node4_step(r, nodes, (uint32_t)cur, r.far, r0, r1, r2, r3, k0, k1, k2, k3);
       if (nh == 0) {
         pop_now = true;
       } else {

@@ -540,6 +540,11 @@ static void collapse_scene_meshes(pbrt_scene* s, std::vector<int32_t>* base4, st
   DevNode4* d = nullptr;
   uint64_t n4 = 0;
   gpu_collapse4(s->ctx->stream, s->blas, node_base, root_ref, n_binary, &d, &n4, root4);
+  // the kernels address 4-wide nodes by 32-bit byte offsets
+  if (n4 * sizeof(DevNode4) > 0xffffffffull) {
+    (void)hipFree(d);
+    throw Error(PBRT_ERR_LAYOUT, "4-wide trees above 4 GiB (use the binary trees, mesh width 2)");
+  }
   base4->assign(nm, 0);
   s->blas4 = d;
 }

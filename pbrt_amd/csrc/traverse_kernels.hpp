@@ -63,6 +63,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
                                                    uint2* __restrict__ spill, uint32_t spill_cap,
                                                    int* __restrict__ status, uint32_t refill, uint32_t tile_w) {
   __shared__ uint2 lds[DEPTH][BLOCK];
+  const __amdgpu_buffer_rsrc_t trs = tri_rsrc(tris);
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest(const DevNode* __restrict__ n
       if (COUNT) ++s.n_tri;
       float t, u, v;
       uint32_t id;
-      if (tri_test(r, tris, tk, &t, &u, &v, &id)) {
+      if (tri_test(r, trs, tk, &t, &u, &v, &id)) {
         if (t >= r.near && t <= r.far && (t < s.min_t || (t == s.min_t && id < s.best))) {
           s.min_t = t; s.t = t; s.u = u; s.v = v; s.best = id;
           s.cur_far = fminf(r.far, s.min_t);
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
                                                uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w,
                                                uint32_t* __restrict__ work) {
   __shared__ uint2 lds[DEPTH][BLOCK];
+  const __amdgpu_buffer_rsrc_t trs = tri_rsrc(tris);
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_any(const DevNode* __restrict__ nodes
       if (COUNT == 2) ++n_tri;
       float t, u, v;
       uint32_t id;
-      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+      if (tri_test(r, trs, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
         hit = true;
         live = false;
       }
@@ -285,6 +287,8 @@ __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__
                                                     uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w,
                                                     uint32_t* __restrict__ work) {
   __shared__ uint2 lds[DEPTH][BLOCK];
+  const __amdgpu_buffer_rsrc_t trs = tri_rsrc(tris);
+  const __amdgpu_buffer_rsrc_t nrs = node4_rsrc(nodes);
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__
       if (COUNT == 2) ++s.n_vis;
       int32_t r0, r1, r2, r3;
       float k0, k1, k2, k3;
-      const int nh = node4_step(r, nodes, cur, s.cur_far, r0, r1, r2, r3, k0, k1, k2, k3);
+      const int nh = node4_step(r, nrs, (uint32_t)cur, s.cur_far, r0, r1, r2, r3, k0, k1, k2, k3);
       if (nh == 0) {
         pop_now = true;
       } else {
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void k_closest4(const DevNode4* __restrict__
       if (COUNT == 2) ++s.n_tri;
       float t, u, v;
       uint32_t id;
-      if (tri_test(r, tris, tk, &t, &u, &v, &id)) {
+      if (tri_test(r, trs, tk, &t, &u, &v, &id)) {
         if (t >= r.near && t <= r.far && (t < s.min_t || (t == s.min_t && id < s.best))) {
           s.min_t = t; s.t = t; s.u = u; s.v = v; s.best = id;
           s.cur_far = fminf(r.far, s.min_t);
@@ -390,6 +394,8 @@ __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nod
                                                 uint32_t spill_cap, int* __restrict__ status, uint32_t refill, uint32_t tile_w,
                                                 uint32_t* __restrict__ work) {
   __shared__ uint2 lds[DEPTH][BLOCK];
+  const __amdgpu_buffer_rsrc_t trs = tri_rsrc(tris);
+  const __amdgpu_buffer_rsrc_t nrs = node4_rsrc(nodes);
   Stack<BLOCK, DEPTH> st{lds, spill, gridDim.x * BLOCK, spill_cap, threadIdx.x, xcd_block() * BLOCK + threadIdx.x, 0};
   const uint32_t stride = gridDim.x * BLOCK;
   uint32_t i = st.gtid;
@@ -419,7 +425,7 @@ __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nod
       if (COUNT == 2) ++n_node;
       int32_t r0, r1, r2, r3;
       float k0, k1, k2, k3;
-      const int nh = node4_step(r, nodes, cur, r.far, r0, r1, r2, r3, k0, k1, k2, k3);
+      const int nh = node4_step(r, nrs, (uint32_t)cur, r.far, r0, r1, r2, r3, k0, k1, k2, k3);
       if (nh == 0) {
         pop_now = true;
       } else {
@@ -443,7 +449,7 @@ __global__ __launch_bounds__(BLOCK) void k_any4(const DevNode4* __restrict__ nod
       if (COUNT == 2) ++n_tri;
       float t, u, v;
       uint32_t id;
-      if (tri_test(r, tris, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
+      if (tri_test(r, trs, tk, &t, &u, &v, &id) && t >= r.near && t <= r.far) {
         hit = true;
         live = false;
       }
