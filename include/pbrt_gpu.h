@@ -260,8 +260,8 @@ int pbrt_scene_add_area_light(pbrt_scene* scene, uint32_t object, const double r
 #define PBRT_BUILD_DEVICE 1
 int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
 /* Tree the scene walks inside its meshes: 4 (default) trees collapsed to 4-wide nodes on the device
- * at commit (the rule of pbrt_launch.width 4 for a mesh), 2 the binary trees. Renders with the
- * radiance cache walk the binary trees (faster there). Same hits and films either way. */
+ * at commit (the rule of pbrt_launch.width 4 for a mesh), 2 the binary trees. Same hits and films
+ * either way. */
 int pbrt_scene_set_mesh_width(pbrt_scene* scene, uint32_t width);
 /* Launch settings of the scene walks (pbrt_scene_trace_*, the renderer's extension and shadow rays):
  * LDS stack entries per lane (8, 16, 24; deeper entries spill to HBM), workgroups of 128 threads

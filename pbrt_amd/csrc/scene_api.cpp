@@ -1147,15 +1147,6 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     }
 
     const SceneView v = view_of(s);
-    // A render with the radiance cache walks the meshes' binary trees with the extension walk's
-    // five-wave variant: its paths end after two bounces and the kernels of its two streams share
-    // the CUs, where the smaller register footprint lets more waves in. Measured per 1280x720
-    // frame on the room with the 1.3M-triangle mesh: 1.18 ms that way, 1.20 with the 4-wide walk,
-    // 1.22 with the compiler's register count (the scene's settings, which plain renders use).
-    // Same films either way.
-    SceneView v_cache = v;
-    v_cache.blas4 = nullptr;
-    v_cache.occ5 = 1u;
     const uint32_t grid = persistent_grid(s, npass_paths);
     const uint32_t spill_per = ensure_spill(s, grid);
     uint64_t passes = 0;
@@ -1249,10 +1240,10 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       int q = 0;
       const uint32_t front = std::min(p.query_vertex, c->max_depth);
       for (uint32_t b = 0; b < front; ++b) {
-        PBRT_HIP(launch_extend(v_cache, B, q, s->spill, spill_per, grid, s->status, st));
+        PBRT_HIP(launch_extend(v, B, q, s->spill, spill_per, grid, s->status, st));
         PBRT_HIP(launch_reset_queue(B, q, st));
         PBRT_HIP(launch_shade_nrc(v, p, B, q, st));
-        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v_cache, p, B, s->spill, spill_per, grid, s->status, st));
+        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v, p, B, s->spill, spill_per, grid, s->status, st));
         q ^= 1;
       }
       PBRT_HIP(hipEventRecord(ev_front[S], st));
@@ -1261,12 +1252,12 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       // in pass order), then the cache queries of the whole pass, answered by the cache trained on
       // this pass and every earlier one, and the film
       PBRT_HIP(hipStreamWaitEvent(aux, ev_front[S], 0));
-      PBRT_HIP(launch_shadow_nrc(v_cache, p, B, spill_aux, spill_per, grid, s->status, aux));
+      PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
       for (uint32_t b = front; b < c->max_depth; ++b) {
-        PBRT_HIP(launch_extend(v_cache, B, q, spill_aux, spill_per, grid, s->status, aux));
+        PBRT_HIP(launch_extend(v, B, q, spill_aux, spill_per, grid, s->status, aux));
         PBRT_HIP(launch_reset_queue(B, q, aux));
         PBRT_HIP(launch_shade_nrc(v, p, B, q, aux));
-        PBRT_HIP(launch_shadow_nrc(v_cache, p, B, spill_aux, spill_per, grid, s->status, aux));
+        PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
         q ^= 1;
       }
       if (p.train_stride) {
