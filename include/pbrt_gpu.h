@@ -65,7 +65,8 @@ typedef struct pbrt_build_stats {
 /* Launch shape of the traversal kernels. Zero fields take the tuned defaults. */
 typedef struct pbrt_launch {
   uint32_t block;        /* threads per workgroup (64, 128, 256) */
-  uint32_t waves_per_cu; /* persistent grid: resident waves per CU */
+  uint32_t waves_per_cu; /* waves per CU in the grid (up to 128): the persistent grid, or more than fit
+                            at once, in which case workgroups queue and start as others finish */
   uint32_t lds_stack;    /* LDS short-stack entries per lane (8, 12, 16, 24, 32); deeper entries spill to HBM */
   uint32_t refill;       /* 0: a lane takes its next ray as soon as its ray is done; N in 1..64: done lanes
                             wait and take their next rays together, once N lanes of the wave wait or no
@@ -174,7 +175,7 @@ int pbrt_mesh_set_launch(pbrt_mesh* mesh, const pbrt_launch* launch);
 int pbrt_mesh_set_ray_tiles(pbrt_mesh* mesh, uint32_t image_width);
 int pbrt_mesh_get_launch(const pbrt_mesh* mesh, pbrt_launch* out);
 /* Autotuner (no counterpart in the reference, which has one CPU code path): times every supported
- * launch shape — block size x LDS stack depth (8 to 32) x resident waves per CU x refill policy (0, 1, 4) x tree
+ * launch shape — block size x LDS stack depth (8 to 32) x waves per CU (8 to 128) x refill policy (0, 1, 4) x tree
  * width (2, 4), and for closest hit the wave-packet walk at each waves per CU — of the closest-hit
  * (any_hit = 0) or any-hit kernel on the caller's rays, `reps` timed runs each, and keeps the
  * fastest for this mesh. out_dev receives hits (pbrt_hit) or occlusion bytes and is overwritten. */

@@ -501,7 +501,7 @@ int pbrt_mesh_set_launch(pbrt_mesh* m, const pbrt_launch* l) {
     if (v.block != 64 && v.block != 128 && v.block != 256) throw Error(PBRT_ERR_INVALID_ARG, "block must be 64, 128 or 256");
     if (v.lds_stack != 8 && v.lds_stack != 12 && v.lds_stack != 16 && v.lds_stack != 24 && v.lds_stack != 32)
       throw Error(PBRT_ERR_INVALID_ARG, "lds_stack must be 8, 12, 16, 24 or 32");
-    if (v.waves_per_cu > 32) throw Error(PBRT_ERR_INVALID_ARG, "waves_per_cu above 32");
+    if (v.waves_per_cu > 128) throw Error(PBRT_ERR_INVALID_ARG, "waves_per_cu above 128");
     if (v.refill > 64) throw Error(PBRT_ERR_INVALID_ARG, "refill above 64 lanes");
     if (!v.width) v.width = 2;
     if (v.width != 2 && v.width != 4) throw Error(PBRT_ERR_INVALID_ARG, "width must be 2 or 4");
@@ -609,9 +609,10 @@ int pbrt_mesh_trace_work(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* o
 }
 
 // Times every supported launch shape of one traversal kernel on the caller's rays and keeps the
-// fastest: block {64, 128, 256} x LDS stack {8, 12, 16, 24, 32} x resident waves per CU
-// {8, 12, 16, 24, 32} x refill {0, 1, 4}; each candidate is run `reps` times after one warm-up
-// launch, scored by its fastest run. The hits/occlusion output buffer is overwritten.
+// fastest: block {64, 128, 256} x LDS stack {8, 12, 16, 24, 32} x waves per CU in the grid
+// {8, 12, 16, 24, 32, 64, 128} x refill {0, 1, 4} x tree width {2, 4}; each candidate is run
+// `reps` times after one warm-up launch, scored by its fastest run. The hits/occlusion output
+// buffer is overwritten.
 int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out_dev, int any_hit, uint32_t reps,
                        pbrt_launch* chosen, float* best_ms) {
   return guarded([&] {
@@ -625,7 +626,7 @@ int pbrt_mesh_autotune(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* out
     PBRT_HIP(hipEventCreate(&e1));
     const uint32_t blocks[] = {64, 128, 256};
     const uint32_t depths[] = {8, 12, 16, 24, 32};
-    const uint32_t waves[] = {8, 12, 16, 24, 32};
+    const uint32_t waves[] = {8, 12, 16, 24, 32, 64, 128};
     const uint32_t refills[] = {0, 1, 4};
     const uint32_t widths[] = {2, 4};
     ensure_bvh4(m);

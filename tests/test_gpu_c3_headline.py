@@ -3,9 +3,10 @@ launches: the 1,310,720-triangle displaced icosphere (bench.make_workload), 4,19
 rays of a 2048^2 pinhole taken in 8 x 16 tiles, and 4,194,304 incoherent rays.
 
 Bar, for each batch:
-  * closest hit of the timed kernels (4-wide tree; coherent: block 64, 32 waves/CU, LDS stack 16,
-    per-lane refill, tiles; incoherent: block 128, 24 waves/CU, LDS stack 8 spilling to HBM, wave
-    refill) bit-exact against Oracle B: triangle index, t, u, v;
+  * closest hit of the timed kernels (4-wide tree; coherent: block 128, 64 waves per CU (twice what
+    fits at once: workgroups queue), LDS stack 12, per-lane refill, tiles; incoherent: block 128,
+    24 waves/CU, LDS stack 8 spilling to HBM, wave refill) bit-exact against Oracle B: triangle
+    index, t, u, v;
   * the reference's instrumented walk on the binary tree gives byte-identical hits;
   * any hit of the timed launches equals Oracle B's any hit and closest-hit existence;
   * the fetch-counting walk (pbrt_mesh_trace_work) gives the same hits, and on the binary tree its
