@@ -208,13 +208,14 @@ def nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, 
     }
 
 
-def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=16, ref_spp=4096, plain_spp=1024,
+def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=8, ref_spp=4096, plain_spp=1024,
                 variants=("sobol", "sobol_mis")):
     """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
     cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
     max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device
     after every pass; times are HIP events on the render stream from the first launch of the render.
-    Plain path tracing runs passes of 4 samples per pixel; the cache renders 1 sample per pixel per
+    Plain path tracing runs passes of 4 samples per pixel, consecutive passes side by side on two
+    streams; the cache renders 1 sample per pixel per
     frame, `frames_per_pass` frames per pass, one training step of 2^16 samples per frame from the
     3 % training paths, and ends the other paths into the cache at their third vertex. Each variant
     (NRC_VARIANTS: sampler and estimator, the same on both sides) races against the one reference
@@ -289,7 +290,9 @@ def jobs_measure(pa, ctx):
     sharing the wavefront's queues with per-bounce graphs, against the same jobs rendered one after
     the other, both with the scene walk's launch settings the autotuner picks on short renders (4
     spp) of the view of the job with the most samples (the library's default settings timed beside
-    them). GPU time from HIP events on the render stream around each call."""
+    them). GPU time from HIP events on the render stream around each call. Beside them, sixteen
+    small jobs (64^2 to 256x144, 1-4 spp), none of which fills the GPU alone, batched and one after
+    the other."""
     from pbrt_amd import scenes
     t0 = time.time()
     desc = scenes.batch_scene()
@@ -328,6 +331,23 @@ def jobs_measure(pa, ctx):
         seq_ms += ms
         seq_rays += sj["rays"]
     paths = st["paths"]
+    # sixteen small jobs, none of which fills the GPU alone: where sharing the queues pays
+    small = scenes.small_jobs()
+    scene.render_jobs(small, use_graphs=True)  # warm-up
+    (_, st_small), small_batch_ms = timed(lambda: scene.render_jobs(small, use_graphs=True))
+    (_, _), small_eager_ms = timed(lambda: scene.render_jobs(small, use_graphs=False))
+    small_seq_ms = 0.0
+    for j in small:
+        (_, _), ms = timed(lambda j=j: scene.render(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"],
+                                                     j["look"], fov=j["fov"], seed=j["seed"], film=False))
+        small_seq_ms += ms
+    small_out = {
+        "workload": "16 jobs " + ", ".join(f"{j['width']}x{j['height']}x{j['spp']}" for j in small) + ", max depth 8",
+        "batch_ms": small_batch_ms, "batch_no_graphs_ms": small_eager_ms, "sequential_ms": small_seq_ms,
+        "speedup_vs_sequential": small_seq_ms / small_batch_ms, "rays": st_small["rays"], "paths": st_small["paths"],
+        "batch_mrays_s": st_small["rays"] / small_batch_ms / 1e3}
+    print(f"[bench] small jobs: batch {small_batch_ms:.2f} ms, eager batch {small_eager_ms:.2f} ms, one after "
+          f"another {small_seq_ms:.2f} ms", file=sys.stderr, flush=True)
     j0 = jobs[0]
     f_host, _ = scene.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
     f_dev, _ = scene_dev.render(j0["width"], j0["height"], 4, j0["max_depth"], j0["eye"], j0["look"], fov=j0["fov"])
@@ -347,6 +367,7 @@ def jobs_measure(pa, ctx):
         "batch_ms": batch_ms, "batch_mrays_s": st["rays"] / batch_ms / 1e3, "batch_samples_s": paths / batch_ms * 1e3,
         "batch_no_graphs_ms": eager_ms, "sequential_ms": seq_ms, "sequential_mrays_s": seq_rays / seq_ms / 1e3,
         "speedup_vs_sequential": seq_ms / batch_ms, "passes": st["passes"], "rays": st["rays"], "paths": paths,
+        "small_jobs": small_out,
     }
 
 
@@ -401,7 +422,7 @@ def build_measure(pa, ctx, v, f, host_mesh, host_s, runs=5):
     return out
 
 
-def render_profile(pa, ctx, spp=16, frames_per_pass=16):
+def render_profile(pa, ctx, spp=16, frames_per_pass=8):
     """A short run for kernel traces of the renderer on the radiance-cache workload: `spp` samples
     per pixel of plain path tracing (4 per pass) and `spp` frames of the cache render."""
     from pbrt_amd import scenes

@@ -391,8 +391,10 @@ int pbrt_scene_load_stage(pbrt_ctx* ctx, const char* path, const char* base_dir,
  * device-resident queues and its persistent traversal kernels. Each job's film is bit for bit what
  * pbrt_render returns for that job alone. films_out: n host pointers (NULL entries skipped), each
  * width*height*3 doubles. use_graphs != 0: each bounce's kernels (closest hit, queue reset, shade,
- * shadow rays) are captured once per queue parity as a hipGraph and replayed for every bounce of
- * every pass. stats: totals over the batch. */
+ * shadow rays) are captured once per queue parity and path set as a hipGraph and replayed for every
+ * bounce of every pass. Consecutive passes run side by side on the render stream and a second
+ * stream the call creates (two path sets; the film takes the passes in order), and the call
+ * returns once both are done. stats: totals over the batch. */
 int pbrt_render_jobs(pbrt_scene* scene, const pbrt_render_config* configs, uint32_t n, double* const* films_out,
                      int use_graphs, pbrt_render_stats* stats);
 
@@ -484,7 +486,12 @@ typedef struct pbrt_progress {
  * (progress != NULL). With the cache, every sample of every pixel is a frame; a pass holds
  * config->samples_per_pass frames (0: 1) traced together, then trains the cache once per frame
  * (frames later in a pass query a cache up to samples_per_pass - 1 steps older). The cache's scene
- * box is set to the scene's bounds. With nrc == NULL this is pbrt_render. */
+ * box is set to the scene's bounds. With nrc == NULL this is pbrt_render. Passes run side by side
+ * on the render stream and streams the call creates (plain: two, consecutive passes on either; with
+ * the cache: a second stream for every other pass's first bounces and a third for the passes' last
+ * bounces, queries, training and film), with as many path sets; the film, the convergence record
+ * and the cache's training take the passes in order, so results are those of running the passes
+ * one after the other, and the call returns once every stream is done. */
 int pbrt_render_ex(pbrt_scene* scene, const pbrt_render_config* config, pbrt_nrc* nrc, const pbrt_nrc_render* nrc_cfg,
                    pbrt_progress* progress, double* film_out, pbrt_render_stats* stats);
 

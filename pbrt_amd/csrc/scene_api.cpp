@@ -1150,31 +1150,42 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     const uint32_t grid = persistent_grid(s, npass_paths);
     const uint32_t spill_per = ensure_spill(s, grid);
     uint64_t passes = 0;
-    // with the cache: two path sets and a second stream. Pass k (set k % 2) traces bounces
-    // 1..query_vertex on the render stream; the remaining bounces (training paths, paths past a
-    // specular vertex), the cache queries, the film accumulation and the cache training run on the
-    // second stream, under the next pass's first bounces. That stream runs the passes' back halves
-    // in order, so pass k's queries see the training of every earlier pass, and a set is reused only
-    // once the pass before last has finished with it: the results are those of running every step
-    // in order on one stream.
-    PathBuffers sets[2] = {pb, pb};
-    hipStream_t aux = nullptr;
-    uint2* spill_aux = nullptr;
-    hipEvent_t ev_front[2] = {nullptr, nullptr}, ev_back[2] = {nullptr, nullptr};
+    // without the cache, several passes: two path sets and a second stream, pass k on set k % 2, so
+    // one pass's bounces run beside the other's and the last, slowest walks of each traversal launch
+    // overlap the other pass's work instead of leaving CUs idle; the film accumulations (and the
+    // convergence records) wait for each other's events and run in pass order, so the film is that
+    // of running the passes one after the other.
+    const bool two = !nrc && passes_total > 1;
+    // with the cache: three path sets and three streams. Pass k (set k % 3) traces bounces 1..
+    // query_vertex on one of two front streams (the render stream for even k, a second one for odd
+    // k), so two passes' fronts run side by side and fill each other's traversal tails; the
+    // remaining bounces (training paths, paths past a specular vertex), the cache queries, the film
+    // accumulation and the cache training run on a third, back stream, under later fronts. The back
+    // stream runs the passes' back halves in order, so pass k's queries see the training of every
+    // earlier pass, and a set is reused only once its last pass's back half has finished: the
+    // results are those of running every step in order on one stream. The fronts do not read the
+    // cache.
+    const int nsets = nrc ? (passes_total > 1 ? 3 : 2) : (two ? 2 : 1);
+    PathBuffers sets[3] = {pb, pb, pb};
+    hipStream_t aux = nullptr, side = nullptr;
+    uint2 *spill_aux = nullptr, *spill_side = nullptr;
+    hipEvent_t ev_front[3] = {nullptr, nullptr, nullptr}, ev_back[3] = {nullptr, nullptr, nullptr};
     struct ReleaseAux {
       hipStream_t& s;
+      hipStream_t& s2;
       hipEvent_t* a;
       hipEvent_t* b;
       ~ReleaseAux() {
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 3; ++i) {
           if (a[i]) (void)hipEventDestroy(a[i]);
           if (b[i]) (void)hipEventDestroy(b[i]);
         }
         if (s) (void)hipStreamDestroy(s);
+        if (s2) (void)hipStreamDestroy(s2);
       }
-    } release_aux{aux, ev_front, ev_back};
-    if (nrc) {
-      PathBuffers& o = sets[1];
+    } release_aux{aux, side, ev_front, ev_back};
+    for (int k = 1; k < nsets; ++k) {
+      PathBuffers& o = sets[k];
       const size_t n = npass_paths;
       o.ray_o = (float4*)alloc(n * sizeof(float4));
       o.ray_d = (float4*)alloc(n * sizeof(float4));
@@ -1189,17 +1200,26 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       o.sh_d = (float4*)alloc(n * sizeof(float4));
       o.sh_c = (float4*)alloc(n * sizeof(float4));
       o.capacity = (uint32_t)n;
-      if (ncfg.train_stride) o.rec = (float4*)alloc((size_t)kRecFields * c->max_depth * ranks_cap * sizeof(float4));
+      if (nrc && ncfg.train_stride)
+        o.rec = (float4*)alloc((size_t)kRecFields * c->max_depth * ranks_cap * sizeof(float4));
       // each set queues its own cache queries: the back half of one pass answers them while the next
       // pass's front (the other set) queues new ones
-      o.q_raw = (float4*)alloc(n * 3 * sizeof(float4));
-      o.q_beta = (float4*)alloc(n * sizeof(float4));
+      if (nrc) {
+        o.q_raw = (float4*)alloc(n * 3 * sizeof(float4));
+        o.q_beta = (float4*)alloc(n * sizeof(float4));
+      }
       PBRT_HIP(hipMemsetAsync(o.counts, 0, kCounterWords * sizeof(uint32_t), st));
+    }
+    if (nsets > 1) {
       spill_aux = (uint2*)alloc((size_t)spill_per * grid * kSceneBlock * sizeof(uint2));
       PBRT_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < 3; ++i) {
         PBRT_HIP(hipEventCreateWithFlags(&ev_front[i], hipEventDisableTiming));
         PBRT_HIP(hipEventCreateWithFlags(&ev_back[i], hipEventDisableTiming));
+      }
+      if (nsets == 3) {
+        spill_side = (uint2*)alloc((size_t)spill_per * grid * kSceneBlock * sizeof(uint2));
+        PBRT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
       }
       PBRT_HIP(hipStreamSynchronize(st));
     }
@@ -1209,44 +1229,54 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       p.pass_samples = std::min(per_pass, c->spp - first);
       const uint32_t npass = npix * p.pass_samples;
       if (!nrc) {
-        PBRT_HIP(launch_generate(v, p, pb, st));
+        const int S = two ? (int)(passes & 1) : 0;
+        const PathBuffers& B = sets[S];
+        hipStream_t on = S ? aux : st;
+        uint2* sp = S ? spill_aux : s->spill;
+        PBRT_HIP(launch_generate(v, p, B, on));
         int q = 0;
         for (uint32_t b = 0; b < c->max_depth; ++b) {
-          PBRT_HIP(launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st));
-          PBRT_HIP(launch_reset_queue(pb, q, st));
-          PBRT_HIP(launch_shade(v, p, pb, q, st));
-          PBRT_HIP(launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st));
+          PBRT_HIP(launch_extend(v, B, q, sp, spill_per, grid, s->status, on));
+          PBRT_HIP(launch_reset_queue(B, q, on));
+          PBRT_HIP(launch_shade(v, p, B, q, on));
+          PBRT_HIP(launch_shadow(v, B, sp, spill_per, grid, s->status, on));
           q ^= 1;
         }
-        PBRT_HIP(launch_accumulate(p, pb, st));
+        // the film takes the passes in order
+        if (two && passes > 0) PBRT_HIP(hipStreamWaitEvent(on, ev_back[S ^ 1], 0));
+        PBRT_HIP(launch_accumulate(p, B, on));
         if (passes < n_record) {
           if (ref)
             PBRT_HIP(launch_rmse(film, 1.0 / (double)(first + p.pass_samples), ref, npix * 3, rmse_partial,
-                                 kRmsePartials, rmse_dev + passes, st));
-          PBRT_HIP(hipEventRecord(events[passes + 1], st));
+                                 kRmsePartials, rmse_dev + passes, on));
+          PBRT_HIP(hipEventRecord(events[passes + 1], on));
         }
+        if (two) PBRT_HIP(hipEventRecord(ev_back[S], on));
         passes++;
         continue;
       }
-      const int S = (int)(passes & 1);
+      const int S = (int)(passes % (uint64_t)nsets);
       PathBuffers& B = sets[S];
+      const bool odd = nsets == 3 && (passes & 1);
+      hipStream_t F = odd ? side : st;
+      uint2* fsp = odd ? spill_side : s->spill;
       if (p.train_stride) p.train_offset = (uint32_t)(hash4(c->seed, passes, 0x747261696eull, 0) % p.train_stride);
-      // front: the set is free once the pass before last has finished with it
-      if (passes >= 2) PBRT_HIP(hipStreamWaitEvent(st, ev_back[S], 0));
-      // front (render stream): the camera rays and every bounce up to the query vertex, where all
+      // front: the set is free once the last pass that used it has finished its back half
+      if (passes >= (uint64_t)nsets) PBRT_HIP(hipStreamWaitEvent(F, ev_back[S], 0));
+      // front (a front stream): the camera rays and every bounce up to the query vertex, where all
       // but the training paths end; the last of those bounces' shadow rays (training paths and
       // paths past a specular vertex only) are left to the back
-      PBRT_HIP(launch_generate_nrc(p, B, st));
+      PBRT_HIP(launch_generate_nrc(p, B, F));
       int q = 0;
       const uint32_t front = std::min(p.query_vertex, c->max_depth);
       for (uint32_t b = 0; b < front; ++b) {
-        PBRT_HIP(launch_extend(v, B, q, s->spill, spill_per, grid, s->status, st));
-        PBRT_HIP(launch_reset_queue(B, q, st));
-        PBRT_HIP(launch_shade_nrc(v, p, B, q, st));
-        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v, p, B, s->spill, spill_per, grid, s->status, st));
+        PBRT_HIP(launch_extend(v, B, q, fsp, spill_per, grid, s->status, F));
+        PBRT_HIP(launch_reset_queue(B, q, F));
+        PBRT_HIP(launch_shade_nrc(v, p, B, q, F));
+        if (b + 1 < front) PBRT_HIP(launch_shadow_nrc(v, p, B, fsp, spill_per, grid, s->status, F));
         q ^= 1;
       }
-      PBRT_HIP(hipEventRecord(ev_front[S], st));
+      PBRT_HIP(hipEventRecord(ev_front[S], F));
       // back (second stream, overlapping the next pass's front): the last front shadow rays, the
       // remaining bounces, the training on this pass's training paths (this stream trains the cache
       // in pass order), then the cache queries of the whole pass, answered by the cache trained on
@@ -1292,11 +1322,12 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       PBRT_HIP(hipEventRecord(ev_back[S], aux));
       passes++;
     }
+    if (side) PBRT_HIP(hipStreamSynchronize(side));
     if (aux) PBRT_HIP(hipStreamSynchronize(aux));
     uint64_t rays = 0, queries = 0;
     PBRT_HIP(hipStreamSynchronize(st));
     spill_mark(s, st);
-    for (int k = 0; k < (nrc ? 2 : 1); ++k) {
+    for (int k = 0; k < nsets; ++k) {
       uint64_t r = 0;
       PBRT_HIP(hipMemcpy(&r, counter(sets[k].counts, kCntRays), sizeof(r), hipMemcpyDeviceToHost));
       rays += r;
@@ -1419,18 +1450,34 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
       owned.push_back(q);
       return q;
     };
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};  // [set * 2 + queue]
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     struct ReleaseGraphs {
       hipGraphExec_t* e;
       hipGraph_t* g;
       ~ReleaseGraphs() {
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 4; ++i) {
           if (e[i]) (void)hipGraphExecDestroy(e[i]);
           if (g[i]) (void)hipGraphDestroy(g[i]);
         }
       }
     } release_graphs{exec, graph};
+    // two path sets on two streams: pass k runs on set k % 2, so one pass's bounces run beside the
+    // other's and the last, slowest walks of each traversal launch overlap the other pass's work
+    // instead of leaving CUs idle. Each set has its own queues, counters, job table and spill area;
+    // the film accumulations wait for each other's events and run in pass order, so the films are
+    // those of running the passes one after the other.
+    hipStream_t side = nullptr;
+    hipEvent_t acc_ev[2] = {nullptr, nullptr};
+    struct ReleaseSide {
+      hipStream_t& s;
+      hipEvent_t* e;
+      ~ReleaseSide() {
+        for (int i = 0; i < 2; ++i)
+          if (e[i]) (void)hipEventDestroy(e[i]);
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } release_side{side, acc_ev};
 
     double* film = (double*)alloc(film_words * sizeof(double));
     JobDesc* all = (JobDesc*)alloc(sched.size() * sizeof(JobDesc));
@@ -1441,6 +1488,11 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
     PBRT_HIP(hipStreamSynchronize(st));
     PathBuffers pb = s->pb;
     pb.film = film;
+    const int nsets = passes > 1 ? 2 : 1;
+    PathBuffers sets[2] = {pb, pb};
+    JobDesc* curs[2] = {cur, cur};
+    hipStream_t streams[2] = {st, st};
+    uint2* spills[2] = {nullptr, nullptr};
     RenderParams p{};
     p.jobs = cur;
     p.n_jobs = n;
@@ -1465,39 +1517,80 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
     const SceneView v = view_of(s);
     const uint32_t grid = persistent_grid(s, (uint32_t)capacity);
     const uint32_t spill_per = ensure_spill(s, grid);
-    auto bounce = [&](int q) -> hipError_t {
-      hipError_t e = launch_extend(v, pb, q, s->spill, spill_per, grid, s->status, st);
-      if (e == hipSuccess) e = launch_reset_queue(pb, q, st);
-      if (e == hipSuccess) e = launch_shade(v, p, pb, q, st);
-      if (e == hipSuccess) e = launch_shadow(v, pb, s->spill, spill_per, grid, s->status, st);
+    spills[0] = s->spill;
+    if (nsets == 2) {
+      PathBuffers& o = sets[1];
+      const size_t m = capacity;
+      o.ray_o = (float4*)alloc(m * sizeof(float4));
+      o.ray_d = (float4*)alloc(m * sizeof(float4));
+      o.beta = (float4*)alloc(m * sizeof(float4));
+      o.L = (float4*)alloc(m * sizeof(float4));
+      o.info = (uint4*)alloc(m * sizeof(uint4));
+      o.queue[0] = (uint32_t*)alloc(m * sizeof(uint32_t));
+      o.queue[1] = (uint32_t*)alloc(m * sizeof(uint32_t));
+      o.counts = (uint32_t*)alloc(kCounterWords * sizeof(uint32_t));
+      o.hit = (SceneHitRec*)alloc(m * sizeof(SceneHitRec));
+      o.sh_o = (float4*)alloc(m * sizeof(float4));
+      o.sh_d = (float4*)alloc(m * sizeof(float4));
+      o.sh_c = (float4*)alloc(m * sizeof(float4));
+      o.capacity = (uint32_t)m;
+      curs[1] = (JobDesc*)alloc(n * sizeof(JobDesc));
+      spills[1] = (uint2*)alloc((size_t)spill_per * grid * kSceneBlock * sizeof(uint2));
+      PBRT_HIP(hipMemsetAsync(o.counts, 0, kCounterWords * sizeof(uint32_t), st));
+      PBRT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+      streams[1] = side;
+      for (int i = 0; i < 2; ++i) PBRT_HIP(hipEventCreateWithFlags(&acc_ev[i], hipEventDisableTiming));
+      PBRT_HIP(hipStreamSynchronize(st));
+    }
+    auto bounce = [&](int set, int q, hipStream_t on) -> hipError_t {
+      RenderParams ps = p;
+      ps.jobs = curs[set];
+      const PathBuffers& B = sets[set];
+      hipError_t e = launch_extend(v, B, q, spills[set], spill_per, grid, s->status, on);
+      if (e == hipSuccess) e = launch_reset_queue(B, q, on);
+      if (e == hipSuccess) e = launch_shade(v, ps, B, q, on);
+      if (e == hipSuccess) e = launch_shadow(v, B, spills[set], spill_per, grid, s->status, on);
       return e;
     };
     if (use_graphs) {
-      for (int q = 0; q < 2; ++q) {
-        PBRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const hipError_t e = bounce(q);
-        const hipError_t e2 = hipStreamEndCapture(st, &graph[q]);
-        PBRT_HIP(e);
-        PBRT_HIP(e2);
-        PBRT_HIP(hipGraphInstantiate(&exec[q], graph[q], nullptr, nullptr, 0));
-      }
+      for (int set = 0; set < nsets; ++set)
+        for (int q = 0; q < 2; ++q) {
+          PBRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          const hipError_t e = bounce(set, q, st);
+          const hipError_t e2 = hipStreamEndCapture(st, &graph[set * 2 + q]);
+          PBRT_HIP(e);
+          PBRT_HIP(e2);
+          PBRT_HIP(hipGraphInstantiate(&exec[set * 2 + q], graph[set * 2 + q], nullptr, nullptr, 0));
+        }
     }
     for (uint32_t k = 0; k < passes; ++k) {
-      PBRT_HIP(hipMemcpyAsync(cur, all + (size_t)k * n, n * sizeof(JobDesc), hipMemcpyDeviceToDevice, st));
-      p.pass_paths = pass_paths[k];
-      PBRT_HIP(launch_generate_jobs(p, pb, st));
+      const int S = (int)(k % (uint32_t)nsets);
+      hipStream_t on = streams[S];
+      PBRT_HIP(hipMemcpyAsync(curs[S], all + (size_t)k * n, n * sizeof(JobDesc), hipMemcpyDeviceToDevice, on));
+      RenderParams ps = p;
+      ps.jobs = curs[S];
+      ps.pass_paths = pass_paths[k];
+      PBRT_HIP(launch_generate_jobs(ps, sets[S], on));
       int q = 0;
       for (uint32_t b = 0; b < max_depth; ++b) {
-        if (use_graphs) PBRT_HIP(hipGraphLaunch(exec[q], st));
-        else PBRT_HIP(bounce(q));
+        if (use_graphs) PBRT_HIP(hipGraphLaunch(exec[S * 2 + q], on));
+        else PBRT_HIP(bounce(S, q, on));
         q ^= 1;
       }
-      PBRT_HIP(launch_accumulate_jobs(p, pb, st));
+      // the film takes the passes in order
+      if (nsets == 2 && k > 0) PBRT_HIP(hipStreamWaitEvent(on, acc_ev[S ^ 1], 0));
+      PBRT_HIP(launch_accumulate_jobs(ps, sets[S], on));
+      if (nsets == 2) PBRT_HIP(hipEventRecord(acc_ev[S], on));
     }
+    if (nsets == 2) PBRT_HIP(hipStreamSynchronize(side));
     PBRT_HIP(hipStreamSynchronize(st));
     spill_mark(s, st);
     uint64_t rays = 0;
-    PBRT_HIP(hipMemcpy(&rays, counter(pb.counts, kCntRays), sizeof(rays), hipMemcpyDeviceToHost));
+    for (int set = 0; set < nsets; ++set) {
+      uint64_t r = 0;
+      PBRT_HIP(hipMemcpy(&r, counter(sets[set].counts, kCntRays), sizeof(r), hipMemcpyDeviceToHost));
+      rays += r;
+    }
     if (films_out) {
       std::vector<double> sum(film_words);
       PBRT_HIP(hipMemcpy(sum.data(), film, film_words * sizeof(double), hipMemcpyDeviceToHost));

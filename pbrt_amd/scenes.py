@@ -178,3 +178,19 @@ def batch_jobs(seed=6, n=8):
         jobs.append(dict(width=w, height=h, spp=int(rng.choice([16, 64])), max_depth=8, eye=eye,
                          look=(0.0, 0.0, 0.0), fov=math.radians(50.0), seed=100 + k))
     return jobs
+
+
+def small_jobs(seed=7, n=16):
+    """Sixteen small render jobs on the batched-jobs scene (previews, tiles, thumbnails): resolutions
+    drawn from 64^2, 96x64, 128^2 and 256x144, 1, 2 or 4 samples per pixel, max depth 8, cameras
+    around the scene. None of them fills the GPU on its own."""
+    rng = np.random.default_rng(seed)
+    sizes = [(64, 64), (96, 64), (128, 128), (256, 144)]
+    jobs = []
+    for k in range(n):
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        a = rng.uniform(0, 2 * math.pi)
+        eye = (40.0 * math.cos(a), float(rng.uniform(8.0, 20.0)), 40.0 * math.sin(a))
+        jobs.append(dict(width=w, height=h, spp=int(rng.choice([1, 2, 4])), max_depth=8, eye=eye,
+                         look=(0.0, 0.0, 0.0), fov=math.radians(50.0), seed=200 + k))
+    return jobs

@@ -91,6 +91,20 @@ def test_c4_eight_jobs_batch_equals_single_renders(c4, use_graphs):
         assert f.tobytes() == alone.tobytes()
 
 
+def test_c4_sixteen_small_jobs_batch_equals_single_renders(c4):
+    """The API's largest batch (16 jobs), each too small to fill the GPU alone: every film equals the
+    job rendered alone, bit for bit."""
+    _, scene, _ = c4
+    jobs = scenes.small_jobs()
+    assert len(jobs) == 16
+    films, stats = scene.render_jobs(jobs, use_graphs=True)
+    assert stats["paths"] == sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    for j, f in zip(jobs, films):
+        alone, _ = scene.render(j["width"], j["height"], j["spp"], j["max_depth"], j["eye"], j["look"], fov=j["fov"],
+                                seed=j["seed"])
+        assert f.tobytes() == alone.tobytes()
+
+
 def test_c4_render_matches_oracle(c4):
     _, scene, orc = c4
     j = scenes.batch_jobs()[0]
