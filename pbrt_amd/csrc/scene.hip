@@ -1777,9 +1777,11 @@ hipError_t launch_shade(const SceneView& v, const RenderParams& p, const PathBuf
   return hipGetLastError();
 }
 
-hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream) {
-  hipLaunchKernelGGL(k_shade<true>, dim3(blocks_for(b.capacity, kShadeBlock)), dim3(kShadeBlock), 0,
-                     (hipStream_t)stream, v, p, b, q);
+hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream,
+                            uint32_t grid_cap) {
+  uint32_t grid = blocks_for(b.capacity, kShadeBlock);
+  if (grid_cap && grid > grid_cap) grid = grid_cap;
+  hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, (hipStream_t)stream, v, p, b, q);
   return hipGetLastError();
 }
 

@@ -184,7 +184,11 @@ hipError_t launch_shadow(const SceneView& v, const PathBuffers& b, uint2* spill,
                          uint32_t grid, int* status, void* stream);
 hipError_t launch_accumulate(const RenderParams& p, const PathBuffers& b, void* stream);
 hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream);
-hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream);
+// grid_cap > 0: at most that many workgroups, striding over the queue (for queues known to be short,
+// such as the training paths' bounces past the query vertex, whose full-size grids would take CU
+// slots from the traversal launches running beside them); 0: one lane per path of the set.
+hipError_t launch_shade_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, int q, void* stream,
+                            uint32_t grid_cap = 0);
 hipError_t launch_shadow_nrc(const SceneView& v, const RenderParams& p, const PathBuffers& b, uint2* spill,
                              uint32_t spill_per_thread, uint32_t grid, int* status, void* stream);
 hipError_t launch_generate_nrc(const RenderParams& p, const PathBuffers& b, void* stream);

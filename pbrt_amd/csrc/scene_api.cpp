@@ -1286,7 +1286,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
       for (uint32_t b = front; b < c->max_depth; ++b) {
         PBRT_HIP(launch_extend(v, B, q, spill_aux, spill_per, grid, s->status, aux));
         PBRT_HIP(launch_reset_queue(B, q, aux));
-        PBRT_HIP(launch_shade_nrc(v, p, B, q, aux));
+        PBRT_HIP(launch_shade_nrc(v, p, B, q, aux, 4u * (uint32_t)s->ctx->cu_count));
         PBRT_HIP(launch_shadow_nrc(v, p, B, spill_aux, spill_per, grid, s->status, aux));
         q ^= 1;
       }
