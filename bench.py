@@ -168,34 +168,56 @@ NRC_VARIANTS = {
 }
 
 
-def nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, sampler, mis):
+# Pass sizes each side of a race is run at; each side is timed at the size that reaches the bar
+# soonest. Larger passes trace faster per sample, but the bar is only seen at the end of a pass:
+# on the cache workload, plain path tracing with the Sobol sampler took 504 ms with passes of 4
+# samples per pixel, 489 with 16 and 477 with 64 (the bar, first crossed at 242 spp, seen at 256),
+# while with MIS, whose bar is crossed at 160 spp, passes of 64 see it only at 192.
+PLAIN_PASS_SIZES = (4, 16, 64)
+CACHE_PASS_SIZES = (4, 8, 16)
+
+
+def nrc_race(pa, ctx, scene, cam, ref, bar, frames, plain_spp, sampler, mis):
     """One race to RMSE `bar` against `ref`: plain path tracing against the radiance cache, both with
-    the given sampler and estimator, and the cache-terminated render without the learned part."""
+    the given sampler and estimator, each at its fastest pass size, and the cache-terminated render
+    without the learned part."""
     W, H, D = cam["width"], cam["height"], cam["max_depth"]
     view = dict(fov=cam["fov"], mis=mis, sampler=sampler)
     E, L = cam["eye"], cam["look"]
-    plain, plain_st, plain_pr = scene.render_ex(W, H, plain_spp, D, E, L, seed=8, reference=ref, samples_per_pass=4,
-                                                **view)
-    t_plain, spp_plain = first_below(plain_pr, bar)
-    cache = pa.RadianceCache(ctx, seed=4)
-    film, nrc_st, nrc_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=cache, reference=ref,
-                                           samples_per_pass=frames_per_pass, query_vertex=3, train_stride=33,
-                                           train_batch=65536, train_steps=1, **view)
-    t_nrc, f_nrc = first_below(nrc_pr, bar)
+    plain_by, best_p = {}, None
+    for sp in PLAIN_PASS_SIZES:
+        _, st_, pr_ = scene.render_ex(W, H, plain_spp, D, E, L, seed=8, reference=ref, samples_per_pass=sp, **view)
+        t, n = first_below(pr_, bar)
+        plain_by[sp] = {"ms": t, "spp": n, "ms_per_spp": float(pr_["ms"][-1] / pr_["spp"][-1])}
+        if t is not None and (best_p is None or t < best_p[0]):
+            best_p = (t, n, sp, st_, pr_)
+    t_plain, spp_plain, plain_pass, plain_st, plain_pr = best_p
+    cache_by, best_c = {}, None
+    for fp in CACHE_PASS_SIZES:
+        c_ = pa.RadianceCache(ctx, seed=4)
+        _, st_, pr_ = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=c_, reference=ref, samples_per_pass=fp,
+                                      query_vertex=3, train_stride=33, train_batch=65536, train_steps=1, **view)
+        t, n = first_below(pr_, bar)
+        cache_by[fp] = {"ms": t, "frames": n, "ms_per_frame": float(pr_["ms"][-1] / frames)}
+        if t is not None and (best_c is None or t < best_c[0]):
+            best_c = (t, n, fp, st_, pr_, c_)
+    t_nrc, f_nrc, cache_pass, nrc_st, nrc_pr, cache = best_c
     k256 = int(np.searchsorted(nrc_pr["spp"], 256))
     # the same cache-terminated render without the learned part (paths end at vertex 3 with nothing)
     zero = pa.RadianceCache(ctx, seed=4)
     zero.params = np.zeros(pa.NRC_PARAMS, np.float32)
     cut, _, cut_pr = scene.render_ex(W, H, frames, D, E, L, seed=8, cache=zero, reference=ref,
-                                     samples_per_pass=frames_per_pass, train_stride=0, **view)
-    print(f"[bench] nrc ({sampler}, mis {mis}): plain reaches RMSE {bar} at {spp_plain} spp / {t_plain} ms; cache at "
-          f"{f_nrc} frames / {t_nrc} ms; at 256 frames cache RMSE {nrc_pr['rmse'][k256]:.4f}; after {frames} frames "
-          f"{nrc_pr['rmse'][-1]:.4f} (cut {cut_pr['rmse'][-1]:.4f})", file=sys.stderr, flush=True)
+                                     samples_per_pass=cache_pass, train_stride=0, **view)
+    print(f"[bench] nrc ({sampler}, mis {mis}): plain reaches RMSE {bar} at {spp_plain} spp / {t_plain} ms "
+          f"({plain_pass} spp per pass); cache at {f_nrc} frames / {t_nrc} ms ({cache_pass} frames per pass); at 256 "
+          f"frames cache RMSE {nrc_pr['rmse'][k256]:.4f}; after {frames} frames {nrc_pr['rmse'][-1]:.4f} "
+          f"(cut {cut_pr['rmse'][-1]:.4f})", file=sys.stderr, flush=True)
     return cache, {
         "sampler": sampler, "mis": mis,
-        "plain_ms": t_plain, "plain_spp": spp_plain,
-        "cache_ms": t_nrc, "cache_frames": f_nrc,
+        "plain_ms": t_plain, "plain_spp": spp_plain, "plain_spp_per_pass": plain_pass,
+        "cache_ms": t_nrc, "cache_frames": f_nrc, "cache_frames_per_pass": cache_pass,
         "time_ratio": (t_nrc / t_plain) if (t_nrc and t_plain) else None,
+        "plain_by_pass_size": plain_by, "cache_by_pass_size": cache_by,
         "cache_rmse_256_frames": float(nrc_pr["rmse"][k256]),
         "plain_rmse_256_spp": float(plain_pr["rmse"][int(np.searchsorted(plain_pr["spp"], 256))]),
         "cache_rmse_final": float(nrc_pr["rmse"][-1]), "cut_rmse_final": float(cut_pr["rmse"][-1]),
@@ -208,16 +230,17 @@ def nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, 
     }
 
 
-def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=8, ref_spp=4096, plain_spp=1024,
+def nrc_measure(pa, ctx, bar=1e-2, frames=512, ref_spp=4096, plain_spp=1024,
                 variants=("sobol", "sobol_mis")):
     """Time to RMSE `bar` against a `ref_spp` plain render, plain path tracing versus the radiance
     cache, on the cache workload (the room with the 1.3M-triangle mesh and GGX materials, 1280x720,
     max depth 8). RMSE is over pixels and channels of linear radiance, reduced in FP64 on the device
     after every pass; times are HIP events on the render stream from the first launch of the render.
-    Plain path tracing runs passes of 4 samples per pixel, consecutive passes side by side on two
-    streams; the cache renders 1 sample per pixel per
-    frame, `frames_per_pass` frames per pass, one training step of 2^16 samples per frame from the
-    3 % training paths, and ends the other paths into the cache at their third vertex. Each variant
+    Plain path tracing runs passes of PLAIN_PASS_SIZES samples per pixel, consecutive passes side by
+    side on two streams; the cache renders 1 sample per pixel per frame, CACHE_PASS_SIZES frames per
+    pass, one training step of 2^16 samples per frame from the 3 % training paths, and ends the
+    other paths into the cache at their third vertex; each side is timed at the pass size that
+    reaches the bar soonest. Each variant
     (NRC_VARIANTS: sampler and estimator, the same on both sides) races against the one reference
     render (independent sampler, the reference's estimator); the first variant is the headline."""
     from pbrt_amd import scenes
@@ -232,7 +255,7 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=8, ref_spp=4096, 
     races = {}
     cache = None
     for v in variants:
-        c, races[v] = nrc_race(pa, ctx, scene, cam, ref, bar, frames, frames_per_pass, plain_spp, **NRC_VARIANTS[v])
+        c, races[v] = nrc_race(pa, ctx, scene, cam, ref, bar, frames, plain_spp, **NRC_VARIANTS[v])
         cache = cache or c
 
     mlp = mlp_measure(pa, ctx, cache, W * H)
@@ -243,7 +266,7 @@ def nrc_measure(pa, ctx, bar=1e-2, frames=512, frames_per_pass=8, ref_spp=4096, 
                     "U[0.05,0.5]), 1280x720, 1 spp per frame, max depth 8; RMSE of linear radiance against a "
                     f"{ref_spp}-spp plain render (mean radiance {ref.mean():.4f})",
         "headline_variant": variants[0],
-        "frames": frames, "frames_per_pass": frames_per_pass,
+        "frames": frames, "plain_pass_sizes": list(PLAIN_PASS_SIZES), "cache_pass_sizes": list(CACHE_PASS_SIZES),
         "reference_s": float(ref_pr["ms"][-1] / 1e3), "scene_build_s": build_s,
         "variants": races,
         "mlp": mlp,
