@@ -102,9 +102,9 @@ def test_queries_past_a_specular_vertex_are_answered(gpu_ctx):
 
 
 def test_cache_render_is_deterministic(gpu_ctx):
-    """One frame per pass, so the two path sets alternate and each pass's queries are answered on
-    the second stream while the next pass's front queues its own: the film is the same, bit for
-    bit, on every run (each set owns its query buffers)."""
+    """One frame per pass, so the three path sets rotate: consecutive passes' fronts run on two
+    streams and each pass's queries are answered on a third while later fronts queue their own: the
+    film is the same, bit for bit, on every run (each set owns its query buffers)."""
     scene = pbrt_amd.Scene(gpu_ctx, glass_box_scene())
     cache = pbrt_amd.RadianceCache(gpu_ctx, seed=2)
     params = np.zeros(pbrt_amd.NRC_PARAMS, np.float32)
@@ -113,6 +113,33 @@ def test_cache_render_is_deterministic(gpu_ctx):
     films = [scene.render_ex(48, 48, 6, 8, (0.0, 0.0, 1.8), (0.0, 0.0, -0.5), fov=1.2, seed=3, cache=cache,
                              query_vertex=1, train_stride=0, record=False, samples_per_pass=1)[0] for _ in range(3)]
     assert films[0].tobytes() == films[1].tobytes() == films[2].tobytes()
+
+
+def test_fixed_cache_film_does_not_depend_on_the_pass_size(gpu_ctx):
+    """Without training the cache is a fixed function, so the film cannot depend on how the frames
+    are grouped into passes: one pass of six frames (one path set), six passes of one (three sets
+    rotating over three streams), three of two and two of three give the same film, bit for bit."""
+    scene = pbrt_amd.Scene(gpu_ctx, glass_box_scene())
+    cache = pbrt_amd.RadianceCache(gpu_ctx, seed=2)
+    params = np.zeros(pbrt_amd.NRC_PARAMS, np.float32)
+    params[-3:] = [0.25, 0.5, 0.75]
+    cache.params = params
+    films = [scene.render_ex(48, 48, 6, 8, (0.0, 0.0, 1.8), (0.0, 0.0, -0.5), fov=1.2, seed=3, cache=cache,
+                             query_vertex=2, train_stride=0, record=False, samples_per_pass=k)[0] for k in (6, 1, 2, 3)]
+    assert films[0].mean() > 0
+    for f in films[1:]:
+        assert f.tobytes() == films[0].tobytes()
+
+
+def test_plain_film_does_not_depend_on_the_pass_size(gpu_ctx):
+    """Plain renders of several passes run consecutive passes on two streams with two path sets; the
+    film takes the passes in order, so it equals the one-pass film bit for bit."""
+    scene, cam = small_scene(gpu_ctx)
+    args = (40, 24, 12, cam["max_depth"], cam["eye"], cam["look"])
+    films = [scene.render(*args, fov=cam["fov"], seed=5, samples_per_pass=k)[0] for k in (12, 1, 5, 4)]
+    assert films[0].mean() > 0
+    for f in films[1:]:
+        assert f.tobytes() == films[0].tobytes()
 
 
 def test_progress_record(gpu_ctx):
