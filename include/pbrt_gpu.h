@@ -266,17 +266,18 @@ int pbrt_scene_set_mesh_build(pbrt_scene* scene, int where);
 int pbrt_scene_set_mesh_width(pbrt_scene* scene, uint32_t width);
 /* Launch settings of the scene walks (pbrt_scene_trace_*, the renderer's extension and shadow rays):
  * LDS stack entries per lane (8, 16, 24; deeper entries spill to HBM), workgroups of 128 threads
- * per CU in the persistent grid (1-8; 0 = as many as fit), mesh tree width (2, 4; 4 collapses the
- * trees on first use), min_waves (0: the renderer's extension walk at the compiler's register
- * count; 5: its variant compiled for five waves per SIMD). Same hits and films for every setting.
- * Defaults 16, 0, 4, 0 (width 0 also means 4). */
+ * per CU in the grid (1-32; beyond what fits on a CU at once, workgroups queue and start as others
+ * finish; 0 = twice as many as fit), mesh tree width (2, 4; 4 collapses the trees on first use),
+ * min_waves (0: the renderer's extension walk at the compiler's register count; 5: its variant
+ * compiled for five waves per SIMD). Same hits and films for every setting. Defaults 16, 0, 4, 0
+ * (width 0 also means 4). */
 typedef struct pbrt_scene_launch {
   uint32_t lds_stack, blocks_per_cu, width, min_waves;
 } pbrt_scene_launch;
 int pbrt_scene_set_launch(pbrt_scene* scene, const pbrt_scene_launch* launch);
 int pbrt_scene_get_launch(pbrt_scene* scene, pbrt_scene_launch* out);
 /* Scene autotuner (no counterpart in the reference): times the committed scene's closest-hit walk
- * on the caller's rays for every LDS stack depth x workgroups per CU (2, 4, 6, 8) x mesh tree
+ * on the caller's rays for every LDS stack depth x workgroups per CU (2, 4, 6, 8, 16, 32) x mesh tree
  * width, `reps` timed runs each, and keeps the fastest (min_waves is left as set: the walk timed
  * here has one variant). hits_dev (n entries) is overwritten. */
 int pbrt_scene_autotune(pbrt_scene* scene, const pbrt_ray* rays_dev, uint32_t n, pbrt_scene_hit* hits_dev,

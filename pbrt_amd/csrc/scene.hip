@@ -1739,8 +1739,8 @@ hipError_t launch_reset_queue(const PathBuffers& b, int q, void* stream) {
 }
 
 uint32_t scene_blocks_per_cu(uint32_t stack) {
-  // the fewest workgroups of the traversal kernels that fit on one CU at once: a persistent grid
-  // larger than that would leave workgroups waiting for a slot and finishing last
+  // the fewest workgroups of the traversal kernels that fit on one CU at once (the scene's grids are
+  // sized from it)
   int a = 0, b = 0, c = 0;
   hipError_t e = hipSuccess;
   PBRT_SCENE_STACK(stack, {

@@ -144,14 +144,17 @@ SceneView view_of(const pbrt_scene* s) {
 }
 
 uint32_t persistent_grid(pbrt_scene* s, uint32_t n) {
-  // as many workgroups of 2 waves as fit on every CU at once (8 at most), or the launch's count
+  // the launch's workgroups (of 2 waves) per CU, or by default twice as many as fit on a CU at once
+  // (8 at most fit): the second half queues and starts as workgroups of the first finish, so CUs
+  // whose first workgroups end early take on more rays instead of idling while the slowest walks of
+  // a grid that fits exactly drain (small scene 899 -> 988 Mrays/s, room 2,932 -> 3,021, jobs
+  // 702 -> 714; the cache render unchanged)
   const uint32_t k = s->launch.lds_stack == 8 ? 0 : s->launch.lds_stack == 24 ? 2 : 1;
   if (!s->blocks_fit[k]) {
     const uint32_t b = scene_blocks_per_cu(s->launch.lds_stack);
     s->blocks_fit[k] = b ? std::min(b, 8u) : 8u;
   }
-  const uint32_t per_cu = s->launch.blocks_per_cu ? std::min(s->launch.blocks_per_cu, s->blocks_fit[k])
-                                                  : s->blocks_fit[k];
+  const uint32_t per_cu = s->launch.blocks_per_cu ? s->launch.blocks_per_cu : 2u * s->blocks_fit[k];
   const uint32_t cap = (uint32_t)std::max(1, s->ctx->cu_count) * per_cu;
   const uint32_t need = (n + kSceneBlock - 1) / kSceneBlock;
   return std::max(1u, std::min(cap, need));
@@ -573,7 +576,7 @@ int pbrt_scene_set_launch(pbrt_scene* s, const pbrt_scene_launch* l) {
     if (!v.width) v.width = 4;
     if (v.lds_stack != 8 && v.lds_stack != 16 && v.lds_stack != 24)
       throw Error(PBRT_ERR_INVALID_ARG, "lds_stack must be 8, 16 or 24");
-    if (v.blocks_per_cu > 8) throw Error(PBRT_ERR_INVALID_ARG, "blocks_per_cu above 8");
+    if (v.blocks_per_cu > 32) throw Error(PBRT_ERR_INVALID_ARG, "blocks_per_cu above 32");
     if (v.width != 2 && v.width != 4) throw Error(PBRT_ERR_INVALID_ARG, "width must be 2 or 4");
     if (v.min_waves != 0 && v.min_waves != 5) throw Error(PBRT_ERR_INVALID_ARG, "min_waves must be 0 or 5");
     std::lock_guard<std::mutex> lock(s->mu);
@@ -906,7 +909,7 @@ int pbrt_scene_autotune(pbrt_scene* s, const pbrt_ray* rays, uint32_t n, pbrt_sc
     try {
       hipStream_t st = s->ctx->stream;
       for (uint32_t stack : {8u, 16u, 24u})
-        for (uint32_t blocks : {2u, 4u, 6u, 8u})
+        for (uint32_t blocks : {2u, 4u, 6u, 8u, 16u, 32u})
           for (uint32_t width : {2u, 4u}) {
             std::lock_guard<std::mutex> lock(s->mu);
             s->launch = pbrt_scene_launch{stack, blocks, width, prior.min_waves};
@@ -981,7 +984,7 @@ int pbrt_scene_autotune_render(pbrt_scene* s, const pbrt_render_config* c, uint3
     pbrt_scene_launch best = prior;
     double best_s = 1e30;
     for (uint32_t stack : {8u, 16u, 24u})
-      for (uint32_t blocks : {2u, 4u, 6u, 8u})
+      for (uint32_t blocks : {2u, 4u, 6u, 8u, 16u, 32u})
         for (uint32_t width : {2u, 4u})
           for (uint32_t waves : {0u, 5u}) {
             {

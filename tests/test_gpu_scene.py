@@ -436,11 +436,13 @@ def _mixed_rays(eye, look, fov, w=160, h=120, seed=4):
 
 @pytest.mark.parametrize("launch", [dict(lds_stack=8, blocks_per_cu=2, width=2, min_waves=0),
                                     dict(lds_stack=24, blocks_per_cu=0, width=4, min_waves=5),
-                                    dict(lds_stack=8, blocks_per_cu=8, width=4, min_waves=0)],
+                                    dict(lds_stack=8, blocks_per_cu=8, width=4, min_waves=0),
+                                    dict(lds_stack=16, blocks_per_cu=32, width=4, min_waves=0)],
                          ids=lambda l: f"s{l['lds_stack']}b{l['blocks_per_cu']}w{l['width']}m{l['min_waves']}")
 def test_scene_launch_settings_same_hits_and_films(gpu_ctx, launch):
-    """Every scene launch setting (LDS stack depth, workgroups per CU, mesh tree width, set after
-    commit) gives the default's hits, occlusion and films bit for bit."""
+    """Every scene launch setting (LDS stack depth, workgroups per CU up to a grid four times what
+    fits at once, mesh tree width, set after commit) gives the default's hits, occlusion and films
+    bit for bit."""
     desc = scenes.batch_scene(n_meshes=24, n_csg=6)
     eye, look, fov = (0.0, 8.0, 30.0), (0.0, 0.0, 0.0), math.pi / 3
     a = pbrt_amd.Scene(gpu_ctx, desc)
