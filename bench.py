@@ -399,8 +399,8 @@ def render_measure(pa, ctx):
     scene (5,120-triangle icosphere, floor, 64 spheres, point light; 256^2, 16 spp, depth 8) and the
     room (walls, emissive ceiling panel as an area light, two boxes, an 81,920-triangle icosphere;
     512^2, 64 spp, depth 8), each with the reference's estimator, with MIS and with Russian
-    roulette. GPU time from the
-    render's own progress events (first launch to the last pass)."""
+    roulette. GPU time from the render's own progress events (first launch to the last pass), the
+    fastest of three renders."""
     from pbrt_amd import scenes
     out = {}
     for name, (desc, cam), (w, h, spp) in [("small_256x256_16spp", scenes.small(), (256, 256, 16)),
@@ -409,9 +409,11 @@ def render_measure(pa, ctx):
         scene.render(64, 64, 1, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"])  # warm-up
         row = {}
         for key, opt in (("reference_estimator", {}), ("mis", dict(mis=True)), ("roulette", dict(roulette=True))):
-            _, st, pr = scene.render_ex(w, h, spp, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=1,
-                                        **opt)
-            ms = float(pr["ms"][-1])
+            ms = None
+            for _ in range(3):  # the fastest of three renders
+                _, st, pr = scene.render_ex(w, h, spp, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"],
+                                            seed=1, **opt)
+                ms = float(pr["ms"][-1]) if ms is None else min(ms, float(pr["ms"][-1]))
             row[key] = {
                 "ms": ms, "mrays_s": st["rays"] / ms / 1e3, "samples_s": w * h * spp / ms * 1e3,
                 "rays_per_sample": st["rays"] / (w * h * spp), "passes": st["passes"]}

@@ -304,7 +304,8 @@ typedef struct pbrt_render_config {
   double fov;
   double eye[3], look[3], up[3];
   double near, far;
-  uint32_t samples_per_pass; /* samples of every pixel in flight at once (0 = about 4M paths) */
+  uint32_t samples_per_pass; /* samples of every pixel in flight at once (0 = half the samples, at
+                              * most about 8M paths) */
   uint32_t mis;              /* 0: the reference's estimator (integrators/path.rs:48-115: next-event
                                 estimation, emission only after a specular bounce); 1: multiple
                                 importance sampling of next-event estimation and the BSDF sample,

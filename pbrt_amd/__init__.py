@@ -552,7 +552,8 @@ class Scene:
                 dbg = np.zeros(debug_slots * 16, np.float32)
                 nc.debug = dbg.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         prog = None
-        per = samples_per_pass or (1 if cache is not None else max(1, (1 << 22) // (width * height)))
+        per = samples_per_pass or (1 if cache is not None else
+                                   max(1, min(max(1, (1 << 23) // (width * height)), (spp + 1) // 2)))
         per = min(per, spp)
         passes = (spp + per - 1) // per
         if record:

@@ -143,6 +143,15 @@ SceneView view_of(const pbrt_scene* s) {
   return v;
 }
 
+// Samples per pixel in flight per pass of a plain render when the caller leaves it open: at most
+// about 8M paths, and half the samples, so that a render of two passes or more runs them side by
+// side (small scene at 256^2 x 16 spp: 2.17 ms in one pass, 2.04 in two; room at 512^2 x 64: 51.6
+// ms in four passes of 16, 51.0 in two of 32).
+uint32_t default_samples_per_pass(uint32_t npix, uint32_t spp) {
+  const uint32_t budget = std::max(1u, (1u << 23) / std::max(1u, npix));
+  return std::max(1u, std::min(budget, (spp + 1) / 2));
+}
+
 uint32_t persistent_grid(pbrt_scene* s, uint32_t n) {
   // the launch's workgroups (of 2 waves) per CU, or by default twice as many as fit on a CU at once
   // (8 at most fit): the second half queues and starts as workgroups of the first finish, so CUs
@@ -1030,7 +1039,7 @@ int pbrt_render_ex(pbrt_scene* s, const pbrt_render_config* c, pbrt_nrc* nrc, co
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t npix = c->width * c->height;
     uint32_t per_pass = c->samples_per_pass ? c->samples_per_pass
-                                            : (nrc ? 1u : std::max(1u, (1u << 22) / npix));
+                                            : (nrc ? 1u : default_samples_per_pass(npix, c->spp));
     per_pass = std::min(per_pass, c->spp);
     if ((uint64_t)npix * per_pass > (1u << 28)) throw Error(PBRT_ERR_INVALID_ARG, "pass larger than 2^28 paths");
     ensure_paths(s, npix * per_pass);
