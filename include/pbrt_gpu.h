@@ -389,7 +389,7 @@ int pbrt_scene_load_stage(pbrt_ctx* ctx, const char* path, const char* base_dir,
 
 /* A batch of n render jobs (1 to 16) on one scene traced together (no counterpart in the
  * reference, which renders one image per run): every pass gives each unfinished job a share of
- * about 4M paths (at least one sample per pixel), and the jobs' paths share the wavefront's
+ * about 8M paths (at least one sample per pixel), and the jobs' paths share the wavefront's
  * device-resident queues and its persistent traversal kernels. Each job's film is bit for bit what
  * pbrt_render returns for that job alone. films_out: n host pointers (NULL entries skipped), each
  * width*height*3 doubles. use_graphs != 0: each bounce's kernels (closest hit, queue reset, shade,

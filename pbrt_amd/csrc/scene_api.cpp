@@ -1395,7 +1395,7 @@ int pbrt_render_jobs(pbrt_scene* s, const pbrt_render_config* cfg, uint32_t n, d
     spill_order(s, st);
     // the pass schedule, fixed in advance: each pass, every unfinished job takes its share of the
     // path budget, at least one sample per pixel
-    const uint64_t budget = 1u << 22;
+    const uint64_t budget = 1u << 23;  // about 8M paths (4M: 833 against 814 ms on the jobs workload)
     std::vector<JobDesc> base(n);
     uint64_t film_words = 0, pix = 0;
     for (uint32_t k = 0; k < n; ++k) {
