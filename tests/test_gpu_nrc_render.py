@@ -154,6 +154,20 @@ def test_progress_record(gpu_ctx):
     assert prog["rmse"][-1] < prog["rmse"][0]
 
 
+@pytest.mark.parametrize("w,h,spp", [(32, 32, 16), (40, 24, 1), (48, 48, 7)])
+def test_default_pass_size_record(gpu_ctx, w, h, spp):
+    """A plain render left to choose its pass size (half its samples per pass, at most about 8M
+    paths): the convergence record has one entry per pass the library ran, the last at the full
+    sample count, and its RMSE is that of the returned film."""
+    scene, cam = small_scene(gpu_ctx)
+    ref, _ = scene.render(w, h, 32, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=1)
+    film, st, prog = scene.render_ex(w, h, spp, cam["max_depth"], cam["eye"], cam["look"], fov=cam["fov"], seed=3,
+                                     reference=ref)
+    assert len(prog["ms"]) == st["passes"] == (1 if spp == 1 else 2)
+    assert prog["spp"][-1] == spp
+    assert prog["rmse"][-1] == pytest.approx(float(np.sqrt(np.mean((film - ref) ** 2))), rel=1e-9)
+
+
 def test_trained_cache_beats_termination_without_it(gpu_ctx):
     desc, cam = scenes.room(subdiv=4)
     scene = pbrt_amd.Scene(gpu_ctx, desc)
