@@ -6,13 +6,15 @@ Workload (1,310,720-triangle pure-traversal configuration): an icosphere of 8 su
   (a) coherent primary rays of a 2048x2048 pinhole through pixel centres, framed as the reference's
       bvh_stats does (FOV 40 degrees, elevation 20 degrees, tangent to the bounding sphere);
   (b) incoherent rays, origins uniform in the bounding sphere, directions uniform on the sphere.
-tmin 1e-4, tmax 1000. One step = closest hit over both batches (8,388,608 rays).
+tmin 1e-4, tmax 1000. One step = closest hit over both batches (8,388,608 rays), the two batches
+traced at once on two streams (`--sequential`: one after the other on one stream).
 
 Prints one JSON line. `roofline` describes the dominant kernel (closest hit): algorithmic bytes per
 launch = sum over rays of 32 (ray) + 16 (hit) + node records fetched x their size (128 bytes on the
 4-wide tree the timed kernels walk) + 48 x triangle records fetched, with the counts measured
-exactly by the fetch-counting walk of the same tree on the same rays, divided by the kernel's mean
-duration (HIP events on the launch stream). `cpu_baseline` times the f64 CPU restatement of the
+exactly by the fetch-counting walk of the same tree on the same rays, divided by the time the
+step's two launches occupy the GPU (HIP events: fork to join of the step on the context's stream;
+`frac_alone` divides by the two launches' times each alone). `cpu_baseline` times the f64 CPU restatement of the
 reference algorithm (the reference's own BVH parameters: 8 bins, t_trav = 0) over the same rays;
 `parity` checks the timed launches' own hits against the FP32 and f64 restatements.
 """
@@ -480,6 +482,8 @@ def main():
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--calibrate", action="store_true", help="also run a 1 GiB plain copy (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--sequential", action="store_true",
+                    help="trace the step's two batches one after the other on one stream instead of at once")
     ap.add_argument("--no-tiles", action="store_true",
                     help="take the coherent (image) batch in row order instead of 8 x 16 tiles")
     ap.add_argument("--autotune", action="store_true", help="autotune each batch's launch instead of the fixed picks")
@@ -605,28 +609,59 @@ def main():
         raise RuntimeError("traversal kernel reported a stack overflow")
 
     stream = ctx.stream
-    # one event pair per launch, so the timed loop never waits on the host
+    # The two batches are independent: a step traces them at once, the coherent batch on the
+    # context's stream and the incoherent one on a second stream (the mesh gives each stream its own
+    # spill area), forked from and joined back into the context's stream. --sequential traces them
+    # one after the other on one stream. One event pair per launch and per step, so the timed loop
+    # never waits on the host.
+    side = ctx.new_stream()
+    concurrent = not args.sequential
+    on = {k: (side.h if concurrent and j == 1 else stream) for j, k in enumerate(dev)}
     evs = [[(ctx.event(), ctx.event()) for _ in dev] for _ in range(args.steps)]
+    spans = [(ctx.event(), ctx.event(), ctx.event()) for _ in range(args.steps)]
 
-    for _ in range(args.warmup):
-        for k, (dr, dh, n) in dev.items():
+    def step(s=None):
+        fork, joined, end = spans[s] if s is not None else spans[0]
+        fork.record(stream)
+        if concurrent:
+            fork.wait(ctx, side.h)
+        for j, (k, (dr, dh, n)) in enumerate(dev.items()):
             mesh.set_launch(**launches[k])
             mesh.set_ray_tiles(tiles[k])
-            mesh.query(dr, n, dh)
+            if s is not None:
+                evs[s][j][0].record(on[k])
+            mesh.query(dr, n, dh, stream=on[k])
+            if s is not None:
+                evs[s][j][1].record(on[k])
+        if concurrent:
+            joined.record(side.h)
+            joined.wait(ctx, stream)
+        end.record(stream)
+
+    for _ in range(args.warmup):
+        step()
     ctx.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        for j, (k, (dr, dh, n)) in enumerate(dev.items()):
-            e0, e1 = evs[s][j]
-            mesh.set_launch(**launches[k])
-            mesh.set_ray_tiles(tiles[k])
-            e0.record(stream)
-            mesh.query(dr, n, dh)
-            e1.record(stream)
+        step(s)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     per_batch_ms = {k: [evs[s][j][0].elapsed_ms(evs[s][j][1]) for s in range(args.steps)]
                     for j, k in enumerate(dev)}
+    span_ms = float(np.mean([spans[s][0].elapsed_ms(spans[s][2]) for s in range(args.steps)]))
+    # each batch alone on the context's stream, after the timed loop: its own kernel time
+    alone_ms = {}
+    for k, (dr, dh, n) in dev.items():
+        mesh.set_launch(**launches[k])
+        mesh.set_ray_tiles(tiles[k])
+        e0, e1 = evs[0][0]
+        ts = []
+        for _ in range(5):
+            e0.record(stream)
+            mesh.query(dr, n, dh, stream=stream)
+            e1.record(stream)
+            ts.append(e0.elapsed_ms(e1))
+        alone_ms[k] = float(np.mean(ts))
     # the timed launches' own results: no stack overflow, and the hits of the reference's
     # instrumented walk on the binary tree, byte for byte
     status_after = mesh.status()
@@ -638,12 +673,15 @@ def main():
 
     total_rays = sum(n for (_, _, n) in dev.values())
     kern_ms = {k: float(np.mean(x)) for k, x in per_batch_ms.items()}
-    step_kernel_ms = sum(kern_ms.values())
+    # the time the step's traversal launches occupy the GPU: fork to join when they run at once,
+    # the sum of the two launches when they run one after the other
+    step_kernel_ms = span_ms if concurrent else sum(kern_ms.values())
     ms_per_step = wall * 1e3 / args.steps
     value = total_rays / (ms_per_step * 1e-3) / 1e6
-    per_batch = {k: dict(mrays_s=work[k]["n"] / (kern_ms[k] * 1e-3) / 1e6, kernel_ms=kern_ms[k], **work[k])
+    per_batch = {k: dict(mrays_s=work[k]["n"] / (alone_ms[k] * 1e-3) / 1e6, kernel_ms=alone_ms[k],
+                         kernel_ms_in_step=kern_ms[k], stream="second" if on[k] != stream else "context", **work[k])
                  for k in dev}
-    # roofline over both launches of the dominant kernel: algorithmic bytes / mean kernel time
+    # roofline over both launches of the dominant kernel: algorithmic bytes / the time they occupy
     alg_bytes = sum(work[k]["bytes_per_launch"] for k in dev)
     achieved = alg_bytes / (step_kernel_ms * 1e-3) / 1e9
     measured = [MEASURED_TRAFFIC.get((k,) + tuple(launches[k][x] for x in ("block", "lds_stack", "refill", "width",
@@ -734,7 +772,8 @@ def main():
             "triangles": len(f),
             "rays_per_step": total_rays,
             "bvh": dict(bs, node_bytes=node_bytes, tri_bytes=tri_bytes),
-            "parallelism": "single GPU",
+            "parallelism": "single GPU; the step's two batches on two streams at once" if not args.sequential
+                           else "single GPU; the step's two batches one after the other",
             "launch": launches,
             "autotuned": tuned is not None,
         },
@@ -746,6 +785,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": sum(m["bytes"] for m in measured) / len(measured) if measured else None,
             "kernel": ", ".join(f"{'k_closest_packet' if launches[k]['packet'] else 'k_closest4' if launches[k]['width'] == 4 else 'k_closest'} ({k}{', 8x16 tiles' if tiles[k] else ''})" for k in dev),
+            "kernel_time": ("fork-to-join span of the two launches running at once, HIP events on the context's "
+                            "stream" if concurrent else "sum of the two launches' HIP event times"),
+            "kernel_time_ms": round(step_kernel_ms, 4),
+            "frac_alone": round(alg_bytes / (sum(alone_ms.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "mrays_s_kernels_one_after_another": round(total_rays / (sum(alone_ms.values()) * 1e-3) / 1e6, 1),
             "algorithmic_bytes_per_launch": alg_bytes / len(dev),
             "algorithmic_bytes_rule": "per ray 32 (ray) + 16 (hit) + node records fetched x record size (128 B "
                                       "4-wide, 64 B binary) + triangle records fetched x 48 B, counted by "

@@ -101,6 +101,9 @@ int pbrt_stream_destroy(pbrt_ctx* ctx, void* stream);
 int pbrt_stream_sync(pbrt_ctx* ctx, void* stream);
 int pbrt_event_create(pbrt_ctx* ctx, void** ev_out);
 int pbrt_event_record(void* ev, void* stream);
+/* Work queued on `stream` (NULL: the context's) after this call waits, on the device, for the work
+ * before the last record of `ev`. */
+int pbrt_stream_wait_event(pbrt_ctx* ctx, void* stream, void* ev);
 int pbrt_event_elapsed_ms(void* ev_start, void* ev_end, float* ms_out);
 int pbrt_event_destroy(void* ev);
 
@@ -184,9 +187,10 @@ int pbrt_mesh_autotune(pbrt_mesh* mesh, const pbrt_ray* rays_dev, uint32_t n, vo
 
 /* Sticky error word of the mesh's kernels (waits for the device): 0 = fine, bit 0 = a traversal
  * stack exceeded its bound (cannot happen for a tree of the reported depth).
- * Traces of one mesh share its HBM stack spill area: a trace on another stream than the mesh's
- * previous trace waits on the device for that trace to end, so traces on different streams are
- * ordered, never concurrent. */
+ * A mesh keeps an HBM stack spill area for each stream it traces on (up to four): traces on
+ * different streams run at once. Past four streams an area changes hands, and a trace on its new
+ * stream waits on the device for the old stream's last trace on it. Traces on one stream run in
+ * stream order. */
 int pbrt_mesh_status(pbrt_mesh* mesh, int* status_out);
 
 /* ---- scene: Scene::{new, add_object, add_light, commit} (src/scene.rs:80-115) ----------------

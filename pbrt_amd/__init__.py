@@ -111,6 +111,10 @@ class Event:
     def record(self, stream):
         call("pbrt_event_record", self.h, stream)
 
+    def wait(self, ctx, stream):
+        """Work queued on `stream` from now on waits on the device for this event's last record."""
+        call("pbrt_stream_wait_event", ctx.h, stream, self.h)
+
     def elapsed_ms(self, end):
         ms = ctypes.c_float()
         call("pbrt_event_elapsed_ms", self.h, end.h, ctypes.byref(ms))

@@ -81,6 +81,7 @@ SIGNATURES = {
     "pbrt_stream_sync": [P, P],
     "pbrt_event_create": [P, PP],
     "pbrt_event_record": [P, P],
+    "pbrt_stream_wait_event": [P, P, P],
     "pbrt_event_elapsed_ms": [P, P, ctypes.POINTER(F32)],
     "pbrt_event_destroy": [P],
     "pbrt_mesh_create": [P, P, U32, P, U32, ctypes.POINTER(BvhParams), PP],

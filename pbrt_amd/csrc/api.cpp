@@ -50,17 +50,24 @@ struct pbrt_mesh {
   int32_t root4 = 0;
   uint32_t depth4 = 0;         // interior levels of the 4-wide tree
   RootDesc root{};
-  uint2* spill = nullptr;
-  uint64_t spill_entries = 0;
+  // HBM overflow areas of the traversal stacks, one per stream the mesh has traced on (up to
+  // kSpillAreas): launches on different streams run at once, each on its own area. Past that many
+  // streams an area changes hands, and its new stream waits for the end of the old one's last
+  // launch on it (event `done`); launches on one stream are ordered by the stream.
+  struct SpillArea {
+    hipStream_t stream = nullptr;
+    uint2* p = nullptr;
+    uint64_t entries = 0;
+    hipEvent_t done = nullptr;
+  };
+  static constexpr int kSpillAreas = 4;
+  SpillArea spill[kSpillAreas];
+  int spill_next = 0;  // the area to hand over next once all are taken
   int* status = nullptr;
   pbrt_launch launch{128, 16, 16, 0, 2, 0};
   uint32_t tile_w = 0;  // pbrt_mesh_set_ray_tiles
-  // Every launch uses the one spill area: a launch on another stream than the last one waits for
-  // the last one's end (event `done`), so that launches on different streams never share it at once.
-  hipEvent_t done = nullptr;
-  hipStream_t last_stream = nullptr;
   // Guards every field above that a call reads or writes after creation (launch settings, the
-  // 4-wide tree, the spill area, the stream order).
+  // 4-wide tree, the spill areas and their streams).
   mutable std::mutex mu;
 };
 
@@ -134,43 +141,49 @@ void ensure_bvh4(pbrt_mesh* m) {
   m->depth4 = depth;
 }
 
-// Sizes the HBM spill area for the launch shape and checks the kernels' error word.
-void prepare_spill(pbrt_mesh* m, LaunchArgs* a) {
+// The spill area of stream s, sized for the launch shape (the caller holds m->mu): the stream's own,
+// or a free one, or the next one handed over (s then waits for its old stream's last launch on it).
+pbrt_mesh::SpillArea& prepare_spill(pbrt_mesh* m, LaunchArgs* a) {
   const uint32_t threads = traversal_grid_threads(*a);
   if (threads == 0) throw Error(PBRT_ERR_INVALID_ARG, "unsupported block size");
   // pending entries: at most depth - 1 on the binary tree, 3 per interior level on the 4-wide one
   const uint64_t depth = a->width == 4 ? 3ull * m->depth4 : m->stats.max_depth;
   const uint32_t per_thread = depth + 1 > a->lds_stack ? (uint32_t)(depth + 1 - a->lds_stack) : 1;
   const uint64_t need = (uint64_t)per_thread * threads;
-  if (need > m->spill_entries) {
-    if (m->spill) PBRT_HIP(hipFree(m->spill));
-    m->spill = nullptr;
-    m->spill_entries = 0;
-    PBRT_HIP(hipMalloc(&m->spill, need * sizeof(uint2)));
-    m->spill_entries = need;
+  const hipStream_t s = (hipStream_t)a->stream;
+  int k = -1;
+  for (int i = 0; i < pbrt_mesh::kSpillAreas && k < 0; ++i)
+    if (m->spill[i].done && m->spill[i].stream == s) k = i;
+  for (int i = 0; i < pbrt_mesh::kSpillAreas && k < 0; ++i)
+    if (!m->spill[i].done) k = i;
+  if (k < 0) {
+    k = m->spill_next;
+    m->spill_next = (m->spill_next + 1) % pbrt_mesh::kSpillAreas;
+    PBRT_HIP(hipStreamWaitEvent(s, m->spill[k].done, 0));
   }
-  a->spill = m->spill;
+  pbrt_mesh::SpillArea& area = m->spill[k];
+  if (!area.done) PBRT_HIP(hipEventCreateWithFlags(&area.done, hipEventDisableTiming));
+  area.stream = s;
+  if (need > area.entries) {
+    // hipFree waits for the device, so no launch still reads the old area
+    if (area.p) PBRT_HIP(hipFree(area.p));
+    area.p = nullptr;
+    area.entries = 0;
+    PBRT_HIP(hipMalloc(&area.p, need * sizeof(uint2)));
+    area.entries = need;
+  }
+  a->spill = area.p;
   a->spill_entries_per_thread = per_thread;
-  a->spill_capacity_entries = m->spill_entries;
-}
-
-// Stream order of the launches that share the mesh's spill area (the caller holds m->mu).
-void before_launch(pbrt_mesh* m, hipStream_t s) {
-  if (m->last_stream && m->last_stream != s) PBRT_HIP(hipStreamWaitEvent(s, m->done, 0));
-}
-void after_launch(pbrt_mesh* m, hipStream_t s) {
-  PBRT_HIP(hipEventRecord(m->done, s));
-  m->last_stream = s;
+  a->spill_capacity_entries = area.entries;
+  return area;
 }
 
 // One traversal launch (the caller holds m->mu).
 void run(pbrt_mesh* m, LaunchArgs& a, int kind) {
   if (a.n == 0) return;
-  prepare_spill(m, &a);
-  const hipStream_t s = (hipStream_t)a.stream;
-  before_launch(m, s);
+  pbrt_mesh::SpillArea& area = prepare_spill(m, &a);
   PBRT_HIP(launch_traversal(a, kind));
-  after_launch(m, s);
+  PBRT_HIP(hipEventRecord(area.done, (hipStream_t)a.stream));
 }
 
 }  // namespace
@@ -318,6 +331,14 @@ int pbrt_event_record(void* ev, void* s) {
   return guarded([&] { PBRT_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)s)); });
 }
 
+int pbrt_stream_wait_event(pbrt_ctx* ctx, void* s, void* ev) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!ev) throw Error(PBRT_ERR_INVALID_ARG, "null event");
+    PBRT_HIP(hipStreamWaitEvent(stream_of(ctx, s), (hipEvent_t)ev, 0));
+  });
+}
+
 int pbrt_event_elapsed_ms(void* a, void* b, float* ms) {
   return guarded([&] {
     PBRT_HIP(hipEventSynchronize((hipEvent_t)b));
@@ -358,7 +379,6 @@ int pbrt_mesh_create(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const uint32_
     if (!nodes.empty()) PBRT_HIP(hipMemcpy(m->nodes, nodes.data(), nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice));
     PBRT_HIP(hipMemcpy(m->tris, tris.data(), tris.size() * sizeof(DevTri), hipMemcpyHostToDevice));
     PBRT_HIP(hipMemset(m->status, 0, sizeof(int)));
-    PBRT_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
     *out = m.release();
   });
 }
@@ -399,7 +419,6 @@ int pbrt_mesh_create_device(pbrt_ctx* ctx, const float* xyz, uint32_t nv, const 
     m->stats = g.stats;
     PBRT_HIP(hipMalloc(&m->status, sizeof(int)));
     PBRT_HIP(hipMemset(m->status, 0, sizeof(int)));
-    PBRT_HIP(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
     *out = m.release();
   });
 }
@@ -429,9 +448,11 @@ int pbrt_mesh_destroy(pbrt_mesh* m) {
     (void)hipFree(m->nodes);
     (void)hipFree(m->nodes4);
     (void)hipFree(m->tris);
-    (void)hipFree(m->spill);
+    for (auto& area : m->spill) {
+      (void)hipFree(area.p);
+      if (area.done) (void)hipEventDestroy(area.done);
+    }
     (void)hipFree(m->status);
-    if (m->done) (void)hipEventDestroy(m->done);
     delete m;
   });
 }
@@ -600,11 +621,9 @@ int pbrt_mesh_trace_work(pbrt_mesh* m, const pbrt_ray* rays, uint32_t n, void* o
     a.block = kWorkBlock;
     a.lds_stack = kWorkLdsStack;
     a.refill = 0;
-    prepare_spill(m, &a);
-    const hipStream_t s = (hipStream_t)a.stream;
-    before_launch(m, s);
+    pbrt_mesh::SpillArea& area = prepare_spill(m, &a);
     PBRT_HIP(launch_traversal_work(a, any_hit ? 1 : 0, work));
-    after_launch(m, s);
+    PBRT_HIP(hipEventRecord(area.done, (hipStream_t)a.stream));
   });
 }
 

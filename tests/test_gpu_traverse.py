@@ -310,30 +310,28 @@ def test_tiled_ray_order_gives_batch_order_results(gpu_ctx, w, h):
 
 
 @pytest.mark.parametrize("width", [2, 4])
-def test_traces_on_two_streams_share_the_spill_area_safely(gpu_ctx, width):
-    """Traces of one mesh on two streams, queued back to back without a host wait, at an 8-entry
-    LDS stack that spills to HBM on the deep reference-mode tree: each batch gets the hits it gets
-    alone on the context's stream (the second launch waits on the device for the first)."""
+def test_traces_on_several_streams_at_once(gpu_ctx, width):
+    """Traces of one mesh on six streams, queued back to back without a host wait, at an 8-entry LDS
+    stack that spills to HBM on the deep reference-mode tree: each stream traces on a spill area of
+    its own, and past the mesh's four areas one changes hands (its new stream waits on the device
+    for the old one's launch). Every batch gets the hits it gets alone, on every repetition."""
     v, f = load_mesh("dragon_vrip_res3")
     mesh = pbrt_amd.TriangleMesh(gpu_ctx, v, f, **pbrt_amd.REFERENCE_BVH)
     mesh.set_launch(block=128, lds_stack=8, waves_per_cu=8, width=width)
-    a = as_rows(pbrt_amd.random_rays(v.mean(0), 0.1, 200000, seed=21))
-    b = as_rows(pbrt_amd.random_rays(v.mean(0), 0.12, 200000, seed=22))
-    ref_a, ref_b = mesh.trace_host(a), mesh.trace_host(b)
-    s1, s2 = gpu_ctx.new_stream(), gpu_ctx.new_stream()
-    da, db = gpu_ctx.upload(a), gpu_ctx.upload(b)
-    ha, hb = gpu_ctx.alloc(16 * len(a)), gpu_ctx.alloc(16 * len(b))
+    batches = [as_rows(pbrt_amd.random_rays(v.mean(0), 0.1 + 0.01 * i, 100000, seed=21 + i)) for i in range(6)]
+    refs = [mesh.trace_host(b) for b in batches]
+    streams = [gpu_ctx.new_stream() for _ in batches]
+    dev = [(gpu_ctx.upload(b), gpu_ctx.alloc(16 * len(b))) for b in batches]
     for _ in range(3):
-        ha.fill(0xFF)
-        hb.fill(0xFF)
-        mesh.query(da, len(a), ha, stream=s1.h)
-        mesh.query(db, len(b), hb, stream=s2.h)
-        mesh.query(da, len(a), ha, stream=gpu_ctx.stream)
-        s1.synchronize()
-        s2.synchronize()
-        gpu_ctx.synchronize()
-        assert ha.read(pbrt_amd.HIT_DTYPE, len(a)).tobytes() == ref_a.tobytes()
-        assert hb.read(pbrt_amd.HIT_DTYPE, len(b)).tobytes() == ref_b.tobytes()
+        for _, h in dev:
+            h.fill(0xFF)
+        gpu_ctx.synchronize()  # the fills (on the context's stream) before the other streams' traces
+        for s, (d, h), b in zip(streams, dev, batches):
+            mesh.query(d, len(b), h, stream=s.h)
+        for s in streams:
+            s.synchronize()
+        for (_, h), b, ref in zip(dev, batches, refs):
+            assert h.read(pbrt_amd.HIT_DTYPE, len(b)).tobytes() == ref.tobytes()
     assert mesh.status() == 0
 
 
