@@ -3,10 +3,12 @@ launches: the 1,310,720-triangle displaced icosphere (bench.make_workload), 4,19
 rays of a 2048^2 pinhole taken in 8 x 16 tiles, and 4,194,304 incoherent rays.
 
 Bar, for each batch:
-  * closest hit of the timed kernels (4-wide tree; coherent: block 128, 64 waves per CU (twice what
-    fits at once: workgroups queue), LDS stack 12, per-lane refill, tiles; incoherent: block 128,
-    24 waves/CU, LDS stack 8 spilling to HBM, wave refill) bit-exact against Oracle B: triangle
-    index, t, u, v;
+  * closest hit of the timed kernels (4-wide tree; coherent: block 128, a grid of 128 waves per CU
+    (four times what fits at once: workgroups queue), LDS stack 12, per-lane refill, tiles;
+    incoherent: block 128, a grid of 32 waves per CU, LDS stack 8 spilling to HBM, wave refill)
+    bit-exact against Oracle B: triangle index, t, u, v;
+  * the two batches traced at once on two streams, as the benchmark's step does, give the hits
+    each gives alone;
   * the reference's instrumented walk on the binary tree gives byte-identical hits;
   * any hit of the timed launches equals Oracle B's any hit and closest-hit existence;
   * the fetch-counting walk (pbrt_mesh_trace_work) gives the same hits, and on the binary tree its
@@ -130,3 +132,34 @@ def test_headline_against_reference_arithmetic(gpu_ctx, c3, name):
     assert cls["rate"] < 1e-4, cls
     assert err["max_rel_non_grazing"] <= 1e-5, err
     assert err["max_rel"] <= 1e-4, err
+
+
+def test_headline_batches_at_once_on_two_streams(gpu_ctx, c3):
+    """The benchmark's step: the coherent batch on the context's stream and the incoherent one on a
+    second stream, forked and joined with events, three steps back to back; every step's hits equal
+    each batch's hits alone, byte for byte, and no traversal stack overflowed."""
+    mesh = c3["mesh"]
+    alone = {name: _device(gpu_ctx, c3, name)["hits"] for name in ("coherent", "incoherent")}
+    side = gpu_ctx.new_stream()
+    dev = {}
+    for name in ("coherent", "incoherent"):
+        b = c3["batches"][name]
+        dev[name] = (gpu_ctx.upload(b["rays"]), gpu_ctx.alloc(16 * len(b["rays"])), len(b["rays"]))
+    fork, joined = gpu_ctx.event(), gpu_ctx.event()
+    on = {"coherent": gpu_ctx.stream, "incoherent": side.h}
+    for _ in range(3):
+        for _, dh, _ in dev.values():
+            dh.fill(0xFF)
+        fork.record(gpu_ctx.stream)
+        fork.wait(gpu_ctx, side.h)
+        for name, (dr, dh, n) in dev.items():
+            b = c3["batches"][name]
+            mesh.set_launch(**b["launch"])
+            mesh.set_ray_tiles(b["tiles"])
+            mesh.query(dr, n, dh, stream=on[name])
+        joined.record(side.h)
+        joined.wait(gpu_ctx, gpu_ctx.stream)
+        gpu_ctx.synchronize()
+        for name, (_, dh, n) in dev.items():
+            assert dh.read(pbrt_amd.HIT_DTYPE, n).tobytes() == alone[name].tobytes(), name
+    assert mesh.status() == 0
